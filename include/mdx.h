@@ -1,0 +1,187 @@
+/*
+ * mdx.h -- C-ABI of the MI355X (gfx950) molecular-dynamics engine.
+ *
+ * The reference (vakuras/moleculardynamics) has no FFI: its cpumd step loop
+ * calls free C++ functions linked into the executable.  Each entry point
+ * below replaces one of them, with the state kept resident in HBM between
+ * calls instead of host arrays:
+ *
+ *   reference function (file:line)                        entry point here
+ *   -----------------------------------------------------  --------------------------
+ *   hostMain allocation          cpu/cpumd/Host.cpp:43-48   mdx_create / mdx_destroy
+ *   readInput / writeOutput       common/Loader.cpp:88-155   mdx_set_state / mdx_get_state
+ *   readyList                     cpu/cpumd/Host.cpp:245-271 mdx_ready_list
+ *   buildList/buildNeighborList   cpu/cpumd/Host.cpp:276-291,
+ *                                 common/NeighborList.cpp:14-156
+ *                                                            mdx_build_list
+ *   (the padded int list itself)  common/NeighborList.cpp:88-145
+ *                                                            mdx_copy_neighbor_list
+ *   predict                       cpu/common/Device.cpp:17-39 mdx_predict
+ *   lennardJonesForces            cpu/common/LennardJones.h:35-93
+ *                                                            mdx_lj_forces
+ *   manyBodyForces                cpu/common/BondOrderFuncs.h:283-397
+ *                                                            mdx_bond_forces
+ *   correct                       cpu/common/Device.cpp:44-120 mdx_correct
+ *   calculateAccelerations        cpu/common/Device.cpp:125-134 mdx_accelerations
+ *   the step loop                 cpu/cpumd/Host.cpp:141-195 mdx_run
+ *   PerformCalculations           common/DebugCalculations.cpp:46-124
+ *                                                            mdx_kinetics
+ *   calculatePotentional          common/DebugCalculations.cpp:129-270
+ *                                                            mdx_potential_allpairs
+ *
+ * Arrays crossing the boundary use the reference's host layout: positions as
+ * real4 (x, y, z, species code in w) and v/a/b/c/f as real3, all double
+ * (the shipped cpumd is built with real = double), in the caller's atom
+ * order.  Every entry point returns MDX_OK (0) or a negative MDX_ERR_* code
+ * in place of the reference's print-and-exit (common/Global.h:116-117);
+ * mdx_last_error() gives the message.  A system is single-threaded: calls on
+ * one system must not overlap.  Different systems are independent.
+ */
+#ifndef MDX_H
+#define MDX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDX_OK 0
+#define MDX_ERR_ARG (-1)       /* invalid argument or configuration */
+#define MDX_ERR_HIP (-2)       /* HIP runtime error (no device, launch failure) */
+#define MDX_ERR_ALLOC (-3)     /* device or host allocation failed */
+#define MDX_ERR_GRID (-4)      /* cell grid exceeds its limit */
+#define MDX_ERR_CAPACITY (-5)  /* a per-atom list exceeded its capacity */
+#define MDX_ERR_STATE (-6)     /* call out of order (e.g. forces before a list build) */
+#define MDX_ERR_LIST_STALE (-7) /* an atom moved farther than the list margin allows */
+
+/* box boundary */
+#define MDX_BOX_WALLED 0    /* reference walls at +-wall (cpu/common/Device.cpp:86-118) */
+#define MDX_BOX_PERIODIC 1  /* cubic periodic box of edge box_length */
+
+/* arithmetic of the pair and bond-order terms; accumulation and integration
+ * are FP64 in both */
+#define MDX_PREC_MIXED 0    /* FP32 pair / bond-order arithmetic */
+#define MDX_PREC_FP64 1     /* FP64 everywhere (the GPU oracle run) */
+
+/* Lennard-Jones pair table */
+#define MDX_LJ_REFERENCE 0  /* N/O/Ne rule of cpu/common/LennardJones.h:63-88, reference units */
+#define MDX_LJ_UNIFORM 1    /* one species: lj_sigma, lj_eps, mass */
+
+/* bond-order functional form */
+#define MDX_BOND_REFERENCE 0  /* the reference's N/O potential, cpu/common/BondOrderFuncs.h */
+#define MDX_BOND_TERSOFF 1    /* Tersoff silicon, PRB 38, 9902 (1988), R = 2.7 A, D = 0.3 A */
+
+/* list selector for mdx_build_list / mdx_copy_neighbor_list */
+#define MDX_LIST_LJ 0
+#define MDX_LIST_BOND 1
+
+typedef struct mdx_system mdx_system;
+
+/*
+ * Configuration.  The reference's 27 keys (common/Loader.h:25-51) map as:
+ * n_atoms = Lennard Jones Particles + Many Body Particles, n_bond = Many Body
+ * Particles, use_lj/use_bond = UseLennardJones/UseManyBody, dt = DT,
+ * lj_rs/lj_rcut/mb_rs/mb_rcut = LennardJonesRS/RCUT, ManyBodyRS/RCUT,
+ * energy_loss = sqrt(EnergyLoss) (cpu/cpumd/Host.cpp:73), wall = the box size
+ * N^(1/3) * 52.8 (Host.cpp:31,166).  The remaining fields select the periodic
+ * benchmark systems that the reference does not have.
+ */
+typedef struct mdx_config {
+    int32_t n_atoms;
+    int32_t n_bond;        /* the first n_bond atoms carry the bond-order potential */
+    int32_t box;           /* MDX_BOX_* */
+    int32_t precision;     /* MDX_PREC_* */
+    int32_t use_lj;
+    int32_t use_bond;
+    int32_t lj_table;      /* MDX_LJ_* */
+    int32_t bond_form;     /* MDX_BOND_* */
+    float dt;
+    float lj_rs, lj_rcut;  /* list radius (cell edge) and cutoff */
+    float mb_rs, mb_rcut;
+    float energy_loss;     /* +x wall restitution factor, already square-rooted */
+    double wall;           /* walled box: walls at +-wall */
+    double box_length;     /* periodic box edge */
+    double lj_sigma, lj_eps, mass;  /* MDX_LJ_UNIFORM species */
+    double accel_scale;    /* a = F / m * accel_scale (1 in reference and reduced units) */
+    int32_t max_neighbors; /* capacity of compact per-atom lists (Tersoff); 0 = default 32 */
+    int32_t reserved[7];
+} mdx_config;
+
+/* lifetime (cpu/cpumd/Host.cpp:43-48, 218-237) */
+int mdx_create(const mdx_config* cfg, mdx_system** out);
+int mdx_destroy(mdx_system* sys);
+const char* mdx_last_error(const mdx_system* sys);  /* sys may be NULL */
+
+/* state transfer (common/Loader.cpp:88-155 layout).  pos4: n*4 doubles (x, y,
+ * z, species), vel3/acc3/b3/c3: n*3 doubles; acc3/b3/c3 may be NULL (zero). */
+int mdx_set_state(mdx_system* sys, const double* pos4, const double* vel3,
+                  const double* acc3, const double* b3, const double* c3);
+/* any output pointer may be NULL; force3 is the last computed force */
+int mdx_get_state(mdx_system* sys, double* pos4, double* vel3, double* acc3,
+                  double* b3, double* c3, double* force3);
+
+/* cpu/cpumd/Host.cpp:245-271: vmax = max |v|, box_size_list = 2 max |coordinate| */
+int mdx_ready_list(mdx_system* sys, double* vmax, double* box_size_list);
+
+/* common/NeighborList.cpp:14-156 on the device: bins atoms into cells of edge
+ * rs and counting-sorts them into cell order.  Walled box: the reference's
+ * grid from box_size_list; periodic box: floor(L/rs) cells per axis
+ * (box_size_list ignored).  Reports the reference's nlistsize (min(27 * max
+ * cell occupancy, n - 1)) and nlargestsize (longest row). */
+int mdx_build_list(mdx_system* sys, int32_t which, double box_size_list,
+                   int32_t* nlistsize, int32_t* nlargestsize);
+
+/* The last built list of `which` in the reference's padded format: rows of
+ * nlistsize ints in atom order, 27 cells in (dx,dy,dz) lexicographic order,
+ * ascending index within a cell, self excluded, -1 padding.  capacity is in
+ * ints and must be at least n * nlistsize (n = n_atoms or n_bond). */
+int mdx_copy_neighbor_list(mdx_system* sys, int32_t which, int32_t* out, int64_t capacity);
+
+/* cpu/common/Device.cpp:17-39 */
+int mdx_predict(mdx_system* sys);
+/* cpu/common/LennardJones.h:35-93: assigns the force */
+int mdx_lj_forces(mdx_system* sys);
+/* cpu/common/BondOrderFuncs.h:283-397: adds to the force when use_lj, else assigns */
+int mdx_bond_forces(mdx_system* sys);
+/* cpu/common/Device.cpp:44-120; *wall_hit = 1 when any wall acted (may be NULL) */
+int mdx_correct(mdx_system* sys, int32_t* wall_hit);
+/* cpu/common/Device.cpp:125-134 */
+int mdx_accelerations(mdx_system* sys);
+
+/* The step loop of cpu/cpumd/Host.cpp:141-195 for nsteps steps, device
+ * resident.  Walled box: the reference's order and list schedule
+ * (next build = t + (int)(((rs - rc)/2)/vmax/|dt|), forces recomputed after a
+ * wall hit).  Periodic box: corrector and next predictor fused, lists rebuilt
+ * when the largest displacement since the last build reaches (rs - rc)/2.
+ * The first call computes the initial forces and a = F/m (Host.cpp:132-138). */
+int mdx_run(mdx_system* sys, int32_t nsteps);
+/* number of list builds done by mdx_run so far (LJ/periodic grid) */
+int mdx_build_count(mdx_system* sys, int64_t* builds);
+
+/* common/DebugCalculations.cpp:46-124 as FP64 device reductions.
+ * out8 = {EK, T, COMx, COMy, COMz, Px, Py, Pz}; T uses kB = kboltz. */
+int mdx_kinetics(mdx_system* sys, double kboltz, double* out8);
+/* Potential energy over the current lists with the force cutoffs (LJ shifted
+ * to zero at rc in the periodic box; unshifted in the walled box).
+ * out2 = {LJ part, bond-order part}. */
+int mdx_potential(mdx_system* sys, double* out2);
+/* common/DebugCalculations.cpp:129-270: LJ over all pairs without cutoff and
+ * the reference bond-order energy with xi over all atoms (O(N^2) / O(N^3)). */
+int mdx_potential_allpairs(mdx_system* sys, double* out2);
+
+/* Per-kernel timing with HIP events on the system's stream.  While enabled,
+ * every launch is bracketed by events; mdx_kernel_times() syncs and returns
+ * per kernel name the launch count and total milliseconds. */
+int mdx_set_timing(mdx_system* sys, int32_t enabled);
+/* writes up to cap entries; names is cap * 48 chars; returns count in *n */
+int mdx_kernel_times(mdx_system* sys, char* names, int64_t* counts, double* total_ms,
+                     int32_t cap, int32_t* n);
+/* blocks until all work on the system's stream is done */
+int mdx_synchronize(mdx_system* sys);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

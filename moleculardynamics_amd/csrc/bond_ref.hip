@@ -1,0 +1,381 @@
+// bond_ref.hip -- the reference's N/O bond-order force
+// (cpu/common/BondOrderFuncs.h:12-397 with the pair terms of
+// common/arvdwfc.cpp:16-59 and the constants of common/Constants.h:36-79).
+//
+// Two gather kernels, one thread per bond-order atom, no atomics:
+//  pass 1 (BondOrderFuncs.h:296-337): pair + bond-order force on i from every
+//    candidate j within the cutoff; xi_ij / xi_ji are stored per (i, j) in a
+//    compact per-atom table instead of the reference's N x N scratch.
+//  pass 2 (BondOrderFuncs.h:339-393): three-body force on k from candidate
+//    pairs (i, j) of k's row, reading the xi values of pass 1.
+// Candidates are the 27-cell neighbourhood of the atom's cell in the
+// bond-order grid, visited in the reference's row order.  The arithmetic
+// type T is float (mixed precision) or double (FP64); differences of FP64
+// positions are formed in FP64 before the conversion.
+#include "mdx_internal.h"
+
+namespace mdx {
+
+namespace {
+
+constexpr int kBlock = 128;
+constexpr int kNe = 2;   // species code of Ne (common/Constants.h:15)
+
+template <class T> struct K;
+template <> struct K<double> {
+    static constexpr double R0 = (double)1.3f, ALFA = 2.0, LAMDA3 = (double)1.33f;
+    static constexpr double C = 25.0, D = (double)4.0417f, H = 0.0, GM = (double)0.3f, ALFAV = 3.0;
+    static constexpr double ALFAVSQ = 9.0, E7 = (double)7.0e-8f, VDW = (double)1.0e-4f;
+    static constexpr double COD = (double)(25.f / 4.0417f);
+};
+template <> struct K<float> {
+    static constexpr float R0 = 1.3f, ALFA = 2.0f, LAMDA3 = 1.33f;
+    static constexpr float C = 25.0f, D = 4.0417f, H = 0.0f, GM = 0.3f, ALFAV = 3.0f;
+    static constexpr float ALFAVSQ = 9.0f, E7 = 7.0e-8f, VDW = 1.0e-4f;
+    static constexpr float COD = 25.f / 4.0417f;
+};
+// vdW numerator constant 1 + 50 GM + GM^2 and 10 (GM - 1), float-evaluated as in arvdwfc.cpp:45
+__device__ constexpr float kVdwP0 = 1.0f + 50.0f * 0.3f + 0.09f;
+__device__ constexpr float kVdwK10 = 10.0f * (0.3f - 1.0f);
+
+__device__ inline double d_exp(double x) { return exp(x); }
+__device__ inline float d_exp(float x) { return expf(x); }
+__device__ inline double d_tanh(double x) { return tanh(x); }
+__device__ inline float d_tanh(float x) { return tanhf(x); }
+__device__ inline double d_sqrt(double x) { return sqrt(x); }
+__device__ inline float d_sqrt(float x) { return sqrtf(x); }
+
+template <class T> __device__ inline void cutoff(T r, T& fc, T& dfc) {
+    T t = d_tanh(K<T>::ALFA * (r - K<T>::R0));
+    dfc = (T)-0.5 * K<T>::ALFA * ((T)1 - t * t);
+    fc = (T)0.5 * ((T)1 - t);
+}
+// V_A with dva = -dV_A/dr; S = 2 so S De/(S-1) = 2 De and beta sqrt(2/S) = beta
+template <class T> __device__ inline void attractive(T r, T de, T beta, T re, T& va, T& dva) {
+    va = (T)2 * de * d_exp(-beta * (r - re));
+    dva = beta * va;
+}
+// V_R = De exp(-2 beta (r - re)) + 7e-8 exp(-10 beta (r - re)), dvr = 2 beta V_R
+template <class T> __device__ inline void repulsive(T r, T de, T beta, T re, T& vr, T& dvr) {
+    vr = de * d_exp((T)-2 * beta * (r - re)) + K<T>::E7 * d_exp((T)-10 * beta * (r - re));
+    dvr = (T)2 * beta * vr;
+}
+template <class T> __device__ inline void vanderwaals(T r, T r0v, T& vdw, T& dvdw) {
+    T x = K<T>::ALFAV * (r - r0v);
+    T ep = d_exp(x), em = K<T>::GM * d_exp(-x);
+    T a = ep + em, ap = K<T>::ALFAV * (ep - em), app = K<T>::ALFAVSQ * a;
+    T num = (T)kVdwP0 + (T)kVdwK10 * ap;
+    dvdw = -K<T>::VDW * (((T)kVdwK10 * app * a - (T)2 * ap * num) / (a * a * a));
+    vdw = K<T>::VDW * num / (a * a);
+}
+template <class T> __device__ inline void ang(T cs, T& g, T& dg) {
+    T u = K<T>::H - cs;
+    T den = K<T>::D + u * u;
+    g = (T)1 + K<T>::COD - K<T>::C / den;
+    dg = (T)-2 * K<T>::C * u / (den * den);
+}
+
+// pair kind: 0 = N-N, 1 = O-O, 2 = unlike (common/Getters.cpp:24-78)
+__device__ inline int pair_kind(int ti, int tj) { return ti == tj ? (ti == 0 ? 0 : 1) : 2; }
+
+template <class T> struct V3 { T x, y, z; };
+template <class T> __device__ inline V3<T> mk(T x, T y, T z) { return V3<T>{x, y, z}; }
+template <class T> __device__ inline V3<T> operator+(V3<T> a, V3<T> b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <class T> __device__ inline V3<T> operator-(V3<T> a, V3<T> b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <class T> __device__ inline V3<T> operator*(V3<T> a, T s) { return mk(a.x * s, a.y * s, a.z * s); }
+template <class T> __device__ inline T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+struct Ctx {
+    DevState st;
+    const int32_t* slot_of_id;
+    const int32_t* cell_of;   // bond-order grid cell per id
+    const int32_t* ids;       // grid members (ids), cell-sorted
+    GridView g;
+};
+
+// FP64 difference r_a - r_b of two atoms given by id, converted to T;
+// *r2 (optional) receives the FP64 squared distance for cutoff tests
+template <class T> __device__ inline V3<T> diff(const Ctx& c, int32_t a, int32_t b,
+                                                 double* r2 = nullptr) {
+    int32_t sa = c.slot_of_id[a], sb = c.slot_of_id[b];
+    double dx = c.st.r[0][sa] - c.st.r[0][sb], dy = c.st.r[1][sa] - c.st.r[1][sb];
+    double dz = c.st.r[2][sa] - c.st.r[2][sb];
+    if (r2) *r2 = dx * dx + dy * dy + dz * dz;
+    return mk((T)dx, (T)dy, (T)dz);
+}
+
+// visit the candidate row of atom `self` in the reference's order
+template <class F> __device__ inline void for_row(const Ctx& c, int32_t self, F&& fn) {
+    int32_t own = c.cell_of[self];
+    int32_t cz = own % c.g.g[2], cy = (own / c.g.g[2]) % c.g.g[1], cx = own / (c.g.g[2] * c.g.g[1]);
+    for (int dx = -1; dx <= 1; dx++) {
+        int32_t x = cx + dx;
+        if (x < 0 || x >= c.g.cells) continue;
+        for (int dy = -1; dy <= 1; dy++) {
+            int32_t y = cy + dy;
+            if (y < 0 || y >= c.g.cells) continue;
+            for (int dz = -1; dz <= 1; dz++) {
+                int32_t z = cz + dz;
+                if (z < 0 || z >= c.g.cells) continue;
+                int32_t cell = (x * c.g.g[1] + y) * c.g.g[2] + z;
+                for (int32_t p = c.g.start[cell]; p < c.g.start[cell + 1]; p++) {
+                    int32_t m = c.ids[p];
+                    if (m != self) fn(m);
+                }
+            }
+        }
+    }
+}
+
+__device__ inline int32_t type_of(const Ctx& c, int32_t id) { return c.st.type[c.slot_of_id[id]]; }
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_bond_pass1(Ctx c, BondRefParams p, int32_t n_bond, int32_t cap, int32_t assign,
+             int32_t* __restrict__ xi_j, double* __restrict__ xi_v, double* __restrict__ e_atom,
+             const int32_t* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_bond) return;
+    const int32_t ti = type_of(c, i);
+    double Fx = 0, Fy = 0, Fz = 0, e = 0;
+    int32_t nx = 0;
+    for_row(c, i, [&](int32_t j) {
+        double r2d;
+        V3<T> ij = diff<T>(c, i, j, &r2d);
+        if (r2d >= p.rcut2 || r2d > p.pair_cut2) return;
+        const int32_t tj = type_of(c, j);
+        if (tj == kNe) return;
+        const T rij = d_sqrt((T)r2d);
+        const int kd = pair_kind(ti, tj);
+        T va = 0, dva = 0, vr = 0, dvr = 0, vdw, dvdw, bo = 0;
+        V3<T> dbo = mk<T>(0, 0, 0);
+        T fcut, dfcut;
+        cutoff<T>(rij / (T)10, fcut, dfcut);
+        if (fcut > 0) {
+            const T re = (T)p.re[kd], de = (T)p.de[kd], be = (T)p.beta[kd];
+            attractive<T>(rij, de, be, re, va, dva);
+            repulsive<T>(rij, de, be, re, vr, dvr);
+            // bondOrder1 (BondOrderFuncs.h:12-143)
+            const V3<T> eij = ij * ((T)1 / rij);
+            const T ex = d_exp(K<T>::LAMDA3 * (rij - re));
+            T xi = 0, xj = 0;
+            V3<T> dxi = mk<T>(0, 0, 0), dxj = mk<T>(0, 0, 0);
+            for_row(c, i, [&](int32_t m) {
+                if (m == j || type_of(c, m) == kNe) return;
+                double rim2;
+                V3<T> im = diff<T>(c, i, m, &rim2);
+                if (rim2 >= p.rcut2) return;
+                T rim = d_sqrt((T)rim2);
+                V3<T> jm = diff<T>(c, j, m);
+                T rjm = d_sqrt(dot(jm, jm));
+                T fci, dfci, fcj, dfcj;
+                cutoff<T>(rim, fci, dfci);
+                cutoff<T>(rjm, fcj, dfcj);
+                V3<T> eim = im * ((T)1 / rim);
+                if (fci > 0) {
+                    T cs = (rij * rij + rim * rim - rjm * rjm) / ((T)2 * rij * rim);
+                    T g, dg;
+                    ang<T>(cs, g, dg);
+                    xi += fci * g * ex;
+                    V3<T> dcs = (eim - eij * cs) * ((T)1 / rij) + (eij - eim * cs) * ((T)1 / rim);
+                    dxi = dxi + eim * (dfci * g * ex) + dcs * (fci * dg * ex) + eij * (fci * g * K<T>::LAMDA3 * ex);
+                }
+                if (fcj > 0) {
+                    T cs = (rij * rij + rjm * rjm - rim * rim) / ((T)2 * rij * rjm);
+                    T g, dg;
+                    ang<T>(cs, g, dg);
+                    xj += fcj * g * ex;
+                    V3<T> dcs = (jm * ((T)-1 / rjm) - eij * cs) * ((T)1 / rij);
+                    dxj = dxj + dcs * (fcj * dg * ex) + eij * (fcj * g * K<T>::LAMDA3 * ex);
+                }
+            });
+            T qi = (T)1 / (((T)1 + xi) * ((T)1 + xi)), qj = (T)1 / (((T)1 + xj) * ((T)1 + xj));
+            bo = (T)0.5 * ((T)1 / ((T)1 + xi) + (T)1 / ((T)1 + xj));
+            dbo = (dxi * qi + dxj * qj) * (T)0.5;
+            if (nx < cap) {
+                int64_t q = (int64_t)i * cap + nx;
+                xi_j[q] = j;
+                xi_v[2 * q] = (double)xi;
+                xi_v[2 * q + 1] = (double)xj;
+            }
+            nx++;
+        }
+        vanderwaals<T>(rij, (T)p.r0v[kd], vdw, dvdw);
+        T fij = dvr - bo * dva + dvdw;
+        T s = fij / rij;
+        Fx += (double)(ij.x * s - dbo.x * va);
+        Fy += (double)(ij.y * s - dbo.y * va);
+        Fz += (double)(ij.z * s - dbo.z * va);
+        e += 0.5 * (double)((vr - bo * va) + vdw);
+    });
+    if (nx < cap) xi_j[(int64_t)i * cap + nx] = -1;
+    const int32_t si = c.slot_of_id[i];
+    if (assign) {
+        c.st.f[0][si] = Fx; c.st.f[1][si] = Fy; c.st.f[2][si] = Fz;
+    } else {
+        c.st.f[0][si] += Fx; c.st.f[1][si] += Fy; c.st.f[2][si] += Fz;
+    }
+    if (e_atom) e_atom[i] = e;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_bond_pass2(Ctx c, BondRefParams p, int32_t n_bond, int32_t cap, const int32_t* __restrict__ xi_j,
+             const double* __restrict__ xi_v, const int32_t* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_bond) return;
+    double Fx = 0, Fy = 0, Fz = 0;
+    for_row(c, k, [&](int32_t i) {
+        double rik2d;
+        const V3<T> ik = diff<T>(c, k, i, &rik2d);
+        if (rik2d >= p.rcut2) return;
+        const T rik = d_sqrt((T)rik2d);
+        const int32_t ti = type_of(c, i);
+        for_row(c, k, [&](int32_t j) {
+            if (j == i) return;
+            double r2d;
+            diff<T>(c, i, j, &r2d);
+            if (r2d >= p.rcut2 || r2d > p.pair_cut2) return;
+            const T rij = d_sqrt((T)r2d);
+            const int kd = pair_kind(ti, type_of(c, j));
+            const T re = (T)p.re[kd], de = (T)p.de[kd], be = (T)p.beta[kd];
+            T va, dva;
+            attractive<T>(rij, de, be, re, va, dva);
+            T xi = 0, xj = 0;
+            for (int32_t q = 0; q < cap; q++) {
+                int32_t jj = xi_j[(int64_t)i * cap + q];
+                if (jj < 0) break;
+                if (jj == j) {
+                    xi = (T)xi_v[2 * ((int64_t)i * cap + q)];
+                    xj = (T)xi_v[2 * ((int64_t)i * cap + q) + 1];
+                    break;
+                }
+            }
+            // bondOrder2 (BondOrderFuncs.h:148-237): -db_ij/dx_k, halves included
+            V3<T> kj = diff<T>(c, k, j);
+            T rjk = d_sqrt(dot(kj, kj));
+            V3<T> ejk = kj * ((T)1 / rjk);
+            V3<T> eik = ik * ((T)1 / rik);
+            V3<T> uij = diff<T>(c, j, i) * ((T)1 / rij);
+            T ex = d_exp(K<T>::LAMDA3 * (rij - re));
+            T fci, dfci, fcj, dfcj;
+            cutoff<T>(rik, fci, dfci);
+            cutoff<T>(rjk, fcj, dfcj);
+            V3<T> d = mk<T>(0, 0, 0);
+            if (fci > 0) {
+                T cs = (rij * rij + rik * rik - rjk * rjk) / ((T)2 * rij * rik);
+                T g, dg;
+                ang<T>(cs, g, dg);
+                V3<T> dcs = (uij - eik * cs) * ((T)1 / rik);
+                V3<T> dx_ = eik * (dfci * g * ex) + dcs * (fci * dg * ex);
+                d = d + dx_ * ((T)0.5 / (((T)1 + xi) * ((T)1 + xi)));
+            }
+            if (fcj > 0) {
+                T cs = (rij * rij + rjk * rjk - rik * rik) / ((T)2 * rij * rjk);
+                T g, dg;
+                ang<T>(cs, g, dg);
+                V3<T> dcs = (uij * (T)-1 - ejk * cs) * ((T)1 / rjk);
+                V3<T> dx_ = ejk * (dfcj * g * ex) + dcs * (fcj * dg * ex);
+                d = d + dx_ * ((T)0.5 / (((T)1 + xj) * ((T)1 + xj)));
+            }
+            Fx -= (double)((T)0.5 * d.x * va);
+            Fy -= (double)((T)0.5 * d.y * va);
+            Fz -= (double)((T)0.5 * d.z * va);
+        });
+    });
+    const int32_t sk = c.slot_of_id[k];
+    c.st.f[0][sk] += Fx;
+    c.st.f[1][sk] += Fy;
+    c.st.f[2][sk] += Fz;
+}
+
+// common/DebugCalculations.cpp:129-270: per-atom share of the brute-force
+// potential.  Ordered pairs with a Ne atom add 2 eps [(s/r)^12 - (s/r)^6]
+// (no cutoff); other pairs with r <= CUTOFF_RIJ add (V_R - b V_A + vdW)/2
+// with xi over every non-Ne atom.  out[s] = LJ part, out[n + s] = bond part.
+__global__ void __launch_bounds__(kBlock)
+k_allpairs(DevState st, int32_t n, LjTable lj, BondRefParams p, double* __restrict__ out) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t ti = st.type[i];
+    double elj = 0, emb = 0;
+    for (int32_t j = 0; j < n; j++) {
+        if (j == i) continue;
+        const int32_t tj = st.type[j];
+        double dx = st.r[0][i] - st.r[0][j], dy = st.r[1][i] - st.r[1][j], dz = st.r[2][i] - st.r[2][j];
+        double r2 = dx * dx + dy * dy + dz * dz;
+        if (ti == kNe || tj == kNe) {
+            int k = (ti & (kTypes - 1)) * kTypes + (tj & (kTypes - 1));
+            double s2 = lj.sig2[k] / r2, s6 = s2 * s2 * s2;
+            elj += 0.5 * lj.e4[k] * (s6 * s6 - s6);
+            continue;
+        }
+        double rij = sqrt(r2);
+        if (rij > p.pair_cut) continue;
+        const int kd = pair_kind(ti, tj);
+        double va = 0, dva, vr = 0, dvr, vdw, dvdw, bo = 0, fcut, dfcut;
+        cutoff<double>(rij / 10, fcut, dfcut);
+        if (fcut > 0) {
+            const double re = p.re[kd], de = p.de[kd], be = p.beta[kd];
+            attractive<double>(rij, de, be, re, va, dva);
+            repulsive<double>(rij, de, be, re, vr, dvr);
+            double ex = exp(K<double>::LAMDA3 * (rij - re)), xi = 0, xj = 0;
+            for (int32_t m = 0; m < n; m++) {
+                if (st.type[m] == kNe || m == i || m == j) continue;
+                double ax = st.r[0][i] - st.r[0][m], ay = st.r[1][i] - st.r[1][m], az = st.r[2][i] - st.r[2][m];
+                double bx = st.r[0][j] - st.r[0][m], by = st.r[1][j] - st.r[1][m], bz = st.r[2][j] - st.r[2][m];
+                double rim = sqrt(ax * ax + ay * ay + az * az), rjm = sqrt(bx * bx + by * by + bz * bz);
+                double fci, dfci, fcj, dfcj, g, dg;
+                cutoff<double>(rim, fci, dfci);
+                cutoff<double>(rjm, fcj, dfcj);
+                if (fci > 0) {
+                    ang<double>((rij * rij + rim * rim - rjm * rjm) / (2.0 * rij * rim), g, dg);
+                    xi += fci * g * ex;
+                }
+                if (fcj > 0) {
+                    ang<double>((rij * rij + rjm * rjm - rim * rim) / (2.0 * rij * rjm), g, dg);
+                    xj += fcj * g * ex;
+                }
+            }
+            bo = 0.5 * (1.0 / (1.0 + xi) + 1.0 / (1.0 + xj));
+        }
+        vanderwaals<double>(rij, p.r0v[kd], vdw, dvdw);
+        emb += 0.5 * ((vr - bo * va) + vdw);
+    }
+    out[i] = elj;
+    out[n + i] = emb;
+}
+
+}  // namespace
+
+hipError_t launch_allpairs(hipStream_t s, const DevState& st, int32_t n, int32_t n_bond,
+                           const LjTable& lj, const BondRefParams& p, double* per_atom2) {
+    (void)n_bond;
+    if (n <= 0) return hipSuccess;
+    k_allpairs<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(st, n, lj, p, per_atom2);
+    return hipGetLastError();
+}
+
+hipError_t launch_bond_ref(hipStream_t s, const DevState& st, const int32_t* slot_of_id,
+                           const int32_t* mb_cell_of, const int32_t* mb_ids, const GridView& g,
+                           const BondRefParams& p, int32_t n_bond, int32_t cap,
+                           int32_t precision, int32_t assign, int32_t* xi_j, double* xi_v,
+                           double* energy_per_atom, const int32_t* gate) {
+    if (n_bond <= 0) return hipSuccess;
+    Ctx c{st, slot_of_id, mb_cell_of, mb_ids, g};
+    dim3 grid((n_bond + kBlock - 1) / kBlock);
+    if (precision == MDX_PREC_FP64) {
+        k_bond_pass1<double><<<grid, kBlock, 0, s>>>(c, p, n_bond, cap, assign, xi_j, xi_v,
+                                                    energy_per_atom, gate);
+        k_bond_pass2<double><<<grid, kBlock, 0, s>>>(c, p, n_bond, cap, xi_j, xi_v, gate);
+    } else {
+        k_bond_pass1<float><<<grid, kBlock, 0, s>>>(c, p, n_bond, cap, assign, xi_j, xi_v,
+                                                   energy_per_atom, gate);
+        k_bond_pass2<float><<<grid, kBlock, 0, s>>>(c, p, n_bond, cap, xi_j, xi_v, gate);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace mdx

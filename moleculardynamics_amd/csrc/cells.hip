@@ -1,0 +1,384 @@
+// cells.hip -- cell-list build on gfx950: atom binning, integer counting sort
+// into cell order, and the state permutation that makes neighbour cells
+// contiguous slot ranges.  Restates the set semantics of
+// common/NeighborList.cpp:14-156 (cells of edge rs, 27-cell stencil,
+// ascending index within a cell) with dense-grid integer work instead of the
+// reference's hashed buckets.
+#include "mdx_internal.h"
+
+namespace mdx {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kScanItems = 8;                      // items per thread in the scan
+constexpr int kScanTile = kBlock * kScanItems;     // items per block
+
+inline int grid_for(int64_t n, int block = kBlock) {
+    int64_t g = (n + block - 1) / block;
+    return (int)(g < 1 ? 1 : g);
+}
+
+__device__ inline int32_t cell_coord_walled(double x, double npos, double h) {
+    return (int32_t)((x + npos) / h);
+}
+
+// Periodic wrap into [0, L); identical in binning and permutation so an
+// atom's stored position always lies in the cell it was binned into.
+__device__ inline double wrap_periodic(double x, double L) {
+    double y = x - L * floor(x / L);
+    return y >= L ? y - L : y;
+}
+
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+// One thread per item; item q lives in slot item_slot[q] (or slot q).
+__global__ void k_bin(DevState st, const int32_t* __restrict__ item_slot, int32_t items, GridView g,
+                      int32_t* __restrict__ cell_out, int32_t* __restrict__ counts,
+                      int32_t* __restrict__ err) {
+    int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= items) return;
+    int32_t s = item_slot ? item_slot[q] : q;
+    int32_t c[3];
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double x = st.r[d][s];
+        if (g.periodic) {
+            int32_t k = (int32_t)(wrap_periodic(x, g.L) / g.h);
+            c[d] = k >= g.g[d] ? g.g[d] - 1 : k;
+        } else {
+            c[d] = cell_coord_walled(x, g.npos, g.h);
+            ok = ok && (c[d] >= 0 && c[d] < g.g[d]);
+        }
+    }
+    if (!ok) {
+        atomicOr(err, 1);
+        cell_out[q] = 0;
+        atomicAdd(&counts[0], 1);
+        return;
+    }
+    int32_t cell = (c[0] * g.g[1] + c[1]) * g.g[2] + c[2];
+    cell_out[q] = cell;
+    atomicAdd(&counts[cell], 1);
+}
+
+__device__ inline int32_t wave_incl_scan(int32_t x) {
+    int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        int32_t y = __shfl_up(x, o, kWave);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+// exclusive scan over a 256-thread block; *total receives the block sum
+__device__ inline int32_t block_excl_scan(int32_t x, int32_t* total) {
+    __shared__ int32_t wsum[kBlock / kWave];
+    int32_t incl = wave_incl_scan(x);
+    int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    if (lane == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    int32_t off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kBlock / kWave; i++) {
+        tot += wsum[i];
+        if (i < w) off += wsum[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + incl - x;
+}
+
+// pass 1: block-local exclusive prefix of kScanTile items, block totals out
+__global__ void k_scan_local(const int32_t* __restrict__ in, int32_t* __restrict__ out, int64_t n,
+                             int32_t* __restrict__ block_sums) {
+    int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int32_t v[kScanItems];
+    int32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        sum += v[k];
+    }
+    int32_t total;
+    int32_t run = block_excl_scan(sum, &total);
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+// pass 2: one block scans the block totals in place; out[n] = grand total
+__global__ void k_scan_sums(int32_t* __restrict__ sums, int64_t nb, int32_t* __restrict__ out,
+                            int64_t n) {
+    int32_t carry = 0;
+    for (int64_t b0 = 0; b0 < nb; b0 += kBlock) {
+        int64_t i = b0 + threadIdx.x;
+        int32_t x = i < nb ? sums[i] : 0;
+        int32_t total;
+        int32_t ex = block_excl_scan(x, &total);
+        if (i < nb) sums[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
+// pass 3: add block offsets
+__global__ void k_scan_add(int32_t* __restrict__ out, int64_t n, const int32_t* __restrict__ sums) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += sums[i / kScanTile];
+}
+
+__global__ void k_scatter(const int32_t* __restrict__ cell_of, const int32_t* __restrict__ start,
+                          int32_t* __restrict__ fill, int32_t* __restrict__ sorted_items,
+                          int32_t items) {
+    int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= items) return;
+    int32_t c = cell_of[q];
+    int32_t pos = start[c] + atomicAdd(&fill[c], 1);
+    sorted_items[pos] = q;
+}
+
+// Restore ascending key order inside each cell (the atomic scatter is
+// unordered): insertion sort of one short segment per thread.
+__global__ void k_sort_cells(const int32_t* __restrict__ start, int64_t ncell,
+                             int32_t* __restrict__ items, const int32_t* __restrict__ key_of) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int32_t b = start[c], e = start[c + 1];
+    for (int32_t i = b + 1; i < e; i++) {
+        int32_t it = items[i];
+        int32_t key = key_of ? key_of[it] : it;
+        int32_t j = i - 1;
+        while (j >= b) {
+            int32_t o = items[j];
+            int32_t ko = key_of ? key_of[o] : o;
+            if (ko <= key) break;
+            items[j + 1] = o;
+            j--;
+        }
+        items[j + 1] = it;
+    }
+}
+
+__global__ void k_permute(DevState src, DevState dst, const int32_t* __restrict__ sorted_src,
+                          const int32_t* __restrict__ cell_of_src, int32_t* __restrict__ cell_of_dst,
+                          int32_t* __restrict__ slot_of_id, double* rr0, double* rr1, double* rr2,
+                          int32_t n, int32_t wrap, double L) {
+    int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int32_t s = sorted_src[k];
+    double* rref[3] = {rr0, rr1, rr2};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double x = src.r[d][s];
+        if (wrap) x = wrap_periodic(x, L);
+        dst.r[d][k] = x;
+        if (rref[d]) rref[d][k] = x;
+        dst.v[d][k] = src.v[d][s];
+        dst.a[d][k] = src.a[d][s];
+        dst.b[d][k] = src.b[d][s];
+        dst.c[d][k] = src.c[d][s];
+        dst.f[d][k] = src.f[d][s];
+    }
+    dst.type[k] = src.type[s];
+    int32_t id = src.id[s];
+    dst.id[k] = id;
+    cell_of_dst[k] = cell_of_src[s];
+    slot_of_id[id] = k;
+}
+
+__global__ void k_max_count(const int32_t* __restrict__ counts, int64_t ncell, int32_t* out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int32_t m = 0;
+    for (; i < ncell; i += stride) m = max(m, counts[i]);
+    for (int o = kWave / 2; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(out, m);
+}
+
+__device__ inline void decode_cell(int32_t c, const GridView& g, int32_t& x, int32_t& y, int32_t& z) {
+    z = c % g.g[2];
+    y = (c / g.g[2]) % g.g[1];
+    x = c / (g.g[2] * g.g[1]);
+}
+
+// neighbour cell (x+dx, y+dy, z+dz): returns -1 when skipped (walled bound)
+__device__ inline int32_t neighbour_cell(const GridView& g, int32_t x, int32_t y, int32_t z) {
+    if (g.periodic) {
+        x = x < 0 ? x + g.g[0] : (x >= g.g[0] ? x - g.g[0] : x);
+        y = y < 0 ? y + g.g[1] : (y >= g.g[1] ? y - g.g[1] : y);
+        z = z < 0 ? z + g.g[2] : (z >= g.g[2] ? z - g.g[2] : z);
+    } else if (x < 0 || y < 0 || z < 0 || x >= g.cells || y >= g.cells || z >= g.cells) {
+        return -1;
+    }
+    return (x * g.g[1] + y) * g.g[2] + z;
+}
+
+__global__ void k_row_lengths(GridView g, const int32_t* __restrict__ cell_by_item, int32_t items,
+                              int32_t* __restrict__ max_len) {
+    int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t len = 0;
+    if (q < items) {
+        int32_t cx, cy, cz;
+        int32_t own = cell_by_item[q];
+        decode_cell(own, g, cx, cy, cz);
+        for (int dx = -1; dx <= 1; dx++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dz = -1; dz <= 1; dz++) {
+                    int32_t c = neighbour_cell(g, cx + dx, cy + dy, cz + dz);
+                    if (c < 0) continue;
+                    len += g.start[c + 1] - g.start[c];
+                    if (c == own) len -= 1;
+                }
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) len = max(len, __shfl_xor(len, o, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(max_len, len);
+}
+
+__global__ void k_emit_list(GridView g, const int32_t* __restrict__ cell_by_item,
+                            const int32_t* __restrict__ member_ids, int32_t items, int32_t nls,
+                            int32_t* __restrict__ out) {
+    int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= items) return;
+    int32_t* row = out + (int64_t)q * nls;
+    int32_t k = 0, cx, cy, cz;
+    decode_cell(cell_by_item[q], g, cx, cy, cz);
+    for (int dx = -1; dx <= 1; dx++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dz = -1; dz <= 1; dz++) {
+                int32_t c = neighbour_cell(g, cx + dx, cy + dy, cz + dz);
+                if (c < 0) continue;
+                for (int32_t p = g.start[c]; p < g.start[c + 1]; p++) {
+                    int32_t m = member_ids[p];
+                    if (m != q && k < nls) row[k++] = m;
+                }
+            }
+    for (; k < nls; k++) row[k] = -1;
+}
+
+// move the whole state between caller order (index = id) and slot order
+// (to_slots: dst[idx[i]] = src[i]; otherwise dst[i] = src[idx[i]])
+__global__ void k_reorder(DevState src, DevState dst, const int32_t* __restrict__ idx, int32_t n,
+                          int32_t to_slots) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t si = to_slots ? i : idx[i];
+    int32_t di = to_slots ? idx[i] : i;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        dst.r[d][di] = src.r[d][si];
+        dst.v[d][di] = src.v[d][si];
+        dst.a[d][di] = src.a[d][si];
+        dst.b[d][di] = src.b[d][si];
+        dst.c[d][di] = src.c[d][si];
+        dst.f[d][di] = src.f[d][si];
+    }
+    dst.type[di] = src.type[si];
+    dst.id[di] = src.id[si];
+}
+
+__global__ void k_gather_i32(const int32_t* __restrict__ src, const int32_t* __restrict__ idx,
+                             int32_t* __restrict__ dst, int32_t n) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
+}  // namespace
+
+hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v) {
+    if (n <= 0) return hipSuccess;
+    int g = grid_for(n);
+    if (g > 65536) g = 65536;
+    k_fill_i32<<<g, kBlock, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin(hipStream_t s, const DevState& st, const int32_t* item_slot, int32_t items,
+                      const GridView& g, int32_t* cell_out, int32_t* counts, int32_t* err) {
+    if (items <= 0) return hipSuccess;
+    k_bin<<<grid_for(items), kBlock, 0, s>>>(st, item_slot, items, g, cell_out, counts, err);
+    return hipGetLastError();
+}
+
+int64_t scan_block_sums_needed(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+hipError_t launch_exclusive_scan(hipStream_t s, const int32_t* in, int32_t* out, int64_t n,
+                                 int32_t* block_sums, int64_t cap) {
+    int64_t nb = (n + kScanTile - 1) / kScanTile;
+    if (nb < 1) nb = 1;
+    if (nb > cap) return hipErrorInvalidValue;
+    k_scan_local<<<(unsigned)nb, kBlock, 0, s>>>(in, out, n, block_sums);
+    k_scan_sums<<<1, kBlock, 0, s>>>(block_sums, nb, out, n);
+    k_scan_add<<<grid_for(n), kBlock, 0, s>>>(out, n, block_sums);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(hipStream_t s, const int32_t* cell_of, const int32_t* start,
+                          int32_t* fill, int32_t* sorted_items, int32_t items) {
+    if (items <= 0) return hipSuccess;
+    k_scatter<<<grid_for(items), kBlock, 0, s>>>(cell_of, start, fill, sorted_items, items);
+    return hipGetLastError();
+}
+
+hipError_t launch_sort_cells(hipStream_t s, const int32_t* start, int64_t ncell,
+                             int32_t* sorted_items, const int32_t* key_of_item) {
+    k_sort_cells<<<grid_for(ncell), kBlock, 0, s>>>(start, ncell, sorted_items, key_of_item);
+    return hipGetLastError();
+}
+
+hipError_t launch_permute(hipStream_t s, const DevState& src, const DevState& dst,
+                          const int32_t* sorted_src, const int32_t* cell_of_src,
+                          int32_t* cell_of_dst, int32_t* slot_of_id, double* const rref[3],
+                          int32_t n, int32_t wrap, double L) {
+    if (n <= 0) return hipSuccess;
+    k_permute<<<grid_for(n), kBlock, 0, s>>>(src, dst, sorted_src, cell_of_src, cell_of_dst,
+                                             slot_of_id, rref[0], rref[1], rref[2], n, wrap, L);
+    return hipGetLastError();
+}
+
+hipError_t launch_max_count(hipStream_t s, const int32_t* counts, int64_t ncell, int32_t* out) {
+    int g = grid_for(ncell);
+    if (g > 4096) g = 4096;
+    k_max_count<<<g, kBlock, 0, s>>>(counts, ncell, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_row_lengths(hipStream_t s, const GridView& g, const int32_t* cell_by_item,
+                              int32_t items, int32_t* max_len) {
+    if (items <= 0) return hipSuccess;
+    k_row_lengths<<<grid_for(items), kBlock, 0, s>>>(g, cell_by_item, items, max_len);
+    return hipGetLastError();
+}
+
+hipError_t launch_emit_list(hipStream_t s, const GridView& g, const int32_t* cell_by_item,
+                            const int32_t* member_ids, int32_t items, int32_t nlistsize,
+                            int32_t* out) {
+    if (items <= 0) return hipSuccess;
+    k_emit_list<<<grid_for(items), kBlock, 0, s>>>(g, cell_by_item, member_ids, items, nlistsize, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_reorder(hipStream_t s, const DevState& src, const DevState& dst,
+                          const int32_t* idx, int32_t n, int32_t to_slots) {
+    if (n <= 0) return hipSuccess;
+    k_reorder<<<grid_for(n), kBlock, 0, s>>>(src, dst, idx, n, to_slots);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_i32(hipStream_t s, const int32_t* src, const int32_t* idx, int32_t* dst,
+                             int32_t n) {
+    if (n <= 0) return hipSuccess;
+    k_gather_i32<<<grid_for(n), kBlock, 0, s>>>(src, idx, dst, n);
+    return hipGetLastError();
+}
+
+}  // namespace mdx

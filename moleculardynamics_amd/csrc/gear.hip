@@ -1,0 +1,165 @@
+// gear.hip -- Gear 5th-order predictor-corrector in FP64 on SoA streams
+// (cpu/common/Device.cpp:17-134, coefficients common/Constants.h:64-67).
+//
+// Each kernel is one pass over unit-stride FP64 arrays; the arithmetic of
+// every component is written in the reference's operation order so the
+// device result equals the FP64 oracle bit for bit (the library is compiled
+// with -ffp-contract=off).  The periodic-box path fuses the corrector of
+// step t with the predictor of step t+1 and tracks the largest displacement
+// since the last list build for the rebuild decision.
+#include "mdx_internal.h"
+
+namespace mdx {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ void __launch_bounds__(kBlock)
+k_predict(DevState st, int32_t n, GearCoeffs gc) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double r = st.r[d][i], v = st.v[d][i], a = st.a[d][i], b = st.b[d][i], c = st.c[d][i];
+        st.r[d][i] = r + gc.k[0] * v + gc.k[1] * a + gc.k[2] * b + gc.k[3] * c;
+        st.v[d][i] = v + gc.k[0] * a + gc.k[1] * b + gc.k[2] * c;
+        st.a[d][i] = a + gc.k[0] * b + gc.k[1] * c;
+        st.b[d][i] = b + gc.k[0] * c;
+    }
+}
+
+// corrector of one component; returns the corrected position
+__device__ inline void correct1(double f, double mass, double scale, const GearCoeffs& gc,
+                                double& r, double& v, double& a, double& b, double& c) {
+    double anew = f / mass * scale;
+    double corr = anew - a;
+    r = r + gc.kc[0] * corr;
+    v = v + gc.kc[1] * corr;
+    a = anew;
+    b = b + gc.kc[2] * corr;
+    c = c + gc.kc[3] * corr;
+}
+
+// Device.cpp:44-120: corrector, then the walls at +-npos.
+__global__ void __launch_bounds__(kBlock)
+k_correct_walled(DevState st, int32_t n, GearCoeffs gc, MassTable m, double npos, float eloss,
+                 int32_t* __restrict__ hit) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool h = false;
+    if (i < n) {
+        const double mass = m.mass[st.type[i] & (kTypes - 1)];
+        double R[3], V[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            double r = st.r[d][i], v = st.v[d][i], a = st.a[d][i], b = st.b[d][i], c = st.c[d][i];
+            correct1(st.f[d][i], mass, m.scale, gc, r, v, a, b, c);
+            R[d] = r; V[d] = v;
+            st.a[d][i] = a; st.b[d][i] = b; st.c[d][i] = c;
+        }
+        if (R[0] > npos && R[1] > 0) { h = true; V[0] = -V[0] * (double)eloss; R[0] = npos; }
+        if (R[0] > npos && R[1] < 0) { h = true; R[0] = -npos; }
+        if (R[0] < -npos) { h = true; V[0] = -V[0]; R[0] = -npos; }
+#pragma unroll
+        for (int d = 1; d < 3; d++)
+            if (fabs(R[d]) > npos) { h = true; V[d] = -V[d]; R[d] = npos * (R[d] / fabs(R[d])); }
+#pragma unroll
+        for (int d = 0; d < 3; d++) { st.r[d][i] = R[d]; st.v[d][i] = V[d]; }
+    }
+    if (__any(h) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(hit, 1);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_accelerations(DevState st, int32_t n, MassTable m, const int32_t* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double mass = m.mass[st.type[i] & (kTypes - 1)];
+#pragma unroll
+    for (int d = 0; d < 3; d++) st.a[d][i] = st.f[d][i] / mass * m.scale;
+}
+
+// Corrector of step t then predictor of step t+1 on each component, and the
+// squared displacement of the predicted position from the list-build
+// position, max-reduced over the wave into *maxdisp2_bits (a non-negative
+// double's bits order like the value).
+__global__ void __launch_bounds__(kBlock)
+k_correct_predict(DevState st, int32_t n, GearCoeffs gc, MassTable m, const double* __restrict__ r0x,
+                  const double* __restrict__ r0y, const double* __restrict__ r0z,
+                  unsigned long long* __restrict__ maxdisp2_bits) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double d2 = 0.0;
+    if (i < n) {
+        const double mass = m.mass[st.type[i] & (kTypes - 1)];
+        const double* r0[3] = {r0x, r0y, r0z};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            double r = st.r[d][i], v = st.v[d][i], a = st.a[d][i], b = st.b[d][i], c = st.c[d][i];
+            correct1(st.f[d][i], mass, m.scale, gc, r, v, a, b, c);
+            double rp = r + gc.k[0] * v + gc.k[1] * a + gc.k[2] * b + gc.k[3] * c;
+            double vp = v + gc.k[0] * a + gc.k[1] * b + gc.k[2] * c;
+            double ap = a + gc.k[0] * b + gc.k[1] * c;
+            double bp = b + gc.k[0] * c;
+            st.r[d][i] = rp; st.v[d][i] = vp; st.a[d][i] = ap; st.b[d][i] = bp; st.c[d][i] = c;
+            double dd = rp - r0[d][i];
+            d2 += dd * dd;
+        }
+    }
+    unsigned long long bits = __double_as_longlong(d2);
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        unsigned long long other = __shfl_xor(bits, o, kWave);
+        bits = other > bits ? other : bits;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(maxdisp2_bits, bits);
+}
+
+}  // namespace
+
+// Device.cpp:19-22 and :48-56: c1 = dt, c2 = c1 dt/2, c3 = c2 dt/3,
+// c4 = c3 dt/4 in real (double) from the float DT; corrector factors from the
+// float Gear constants.
+GearCoeffs gear_coeffs(float dtf) {
+    GearCoeffs g;
+    double dt = (double)dtf;
+    g.k[0] = dt;
+    g.k[1] = g.k[0] * dt / 2.0f;
+    g.k[2] = g.k[1] * dt / 3.0f;
+    g.k[3] = g.k[2] * dt / 4.0f;
+    g.kc[0] = 0.15833333333333333333333333333333f * g.k[1];
+    g.kc[1] = 0.75f * g.k[1] / g.k[0];
+    g.kc[2] = 0.5f * g.k[1] / g.k[2];
+    g.kc[3] = 0.08333333333333333333333333333333f * g.k[1] / g.k[3];
+    return g;
+}
+
+static inline dim3 grid_of(int32_t n) { return dim3((n + kBlock - 1) / kBlock); }
+
+hipError_t launch_predict(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc) {
+    if (n <= 0) return hipSuccess;
+    k_predict<<<grid_of(n), kBlock, 0, s>>>(st, n, gc);
+    return hipGetLastError();
+}
+
+hipError_t launch_correct_walled(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
+                                 const MassTable& m, double npos, float eloss, int32_t* hit) {
+    if (n <= 0) return hipSuccess;
+    k_correct_walled<<<grid_of(n), kBlock, 0, s>>>(st, n, gc, m, npos, eloss, hit);
+    return hipGetLastError();
+}
+
+hipError_t launch_accelerations(hipStream_t s, const DevState& st, int32_t n, const MassTable& m,
+                                const int32_t* gate) {
+    if (n <= 0) return hipSuccess;
+    k_accelerations<<<grid_of(n), kBlock, 0, s>>>(st, n, m, gate);
+    return hipGetLastError();
+}
+
+hipError_t launch_correct_predict(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
+                                  const MassTable& m, double* const rref[3], uint32_t* maxdisp2_bits) {
+    if (n <= 0) return hipSuccess;
+    k_correct_predict<<<grid_of(n), kBlock, 0, s>>>(st, n, gc, m, rref[0], rref[1], rref[2],
+                                                    reinterpret_cast<unsigned long long*>(maxdisp2_bits));
+    return hipGetLastError();
+}
+
+}  // namespace mdx

@@ -1,0 +1,120 @@
+// lj.hip -- Lennard-Jones pair force over the 27-cell neighbourhood of each
+// atom's cell (cpu/common/LennardJones.h:12-93).
+//
+// Gather formulation: one thread per atom (slot order), no atomics, every
+// atom's force summed in the reference's candidate order (neighbour cells in
+// (dx,dy,dz) lexicographic order, ascending index within a cell), so the
+// FP64 variant reproduces the oracle's sums operation for operation.
+// The displacement and the cutoff test are FP64 in both variants; the mixed
+// variant evaluates the 12-6 force law in FP32 from the FP64 distance and
+// accumulates the per-atom force in FP64.
+#include "mdx_internal.h"
+
+namespace mdx {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+template <bool kFp64, bool kEnergy>
+__global__ void __launch_bounds__(kBlock)
+k_lj(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable t, int32_t n,
+     double* __restrict__ e_atom, const int32_t* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const double xi = st.r[0][s], yi = st.r[1][s], zi = st.r[2][s];
+    const int32_t ti = st.type[s] & (kTypes - 1);
+    const int32_t own = cell_of[s];
+    const int32_t cz = own % g.g[2], cy = (own / g.g[2]) % g.g[1], cx = own / (g.g[2] * g.g[1]);
+    double fx = 0, fy = 0, fz = 0, e = 0;
+    for (int dx = -1; dx <= 1; dx++) {
+        int32_t x = cx + dx;
+        double sx = 0.0;
+        if (g.periodic) {
+            if (x < 0) { x += g.g[0]; sx = -g.L; }
+            else if (x >= g.g[0]) { x -= g.g[0]; sx = g.L; }
+        } else if (x < 0 || x >= g.cells) continue;
+        for (int dy = -1; dy <= 1; dy++) {
+            int32_t y = cy + dy;
+            double sy = 0.0;
+            if (g.periodic) {
+                if (y < 0) { y += g.g[1]; sy = -g.L; }
+                else if (y >= g.g[1]) { y -= g.g[1]; sy = g.L; }
+            } else if (y < 0 || y >= g.cells) continue;
+            for (int dz = -1; dz <= 1; dz++) {
+                int32_t z = cz + dz;
+                double sz = 0.0;
+                if (g.periodic) {
+                    if (z < 0) { z += g.g[2]; sz = -g.L; }
+                    else if (z >= g.g[2]) { z -= g.g[2]; sz = g.L; }
+                } else if (z < 0 || z >= g.cells) continue;
+                const int32_t c = (x * g.g[1] + y) * g.g[2] + z;
+                const int32_t pe = g.start[c + 1];
+                for (int32_t p = g.start[c]; p < pe; p++) {
+                    if (p == s) continue;
+                    const double ddx = (xi - st.r[0][p]) - sx;
+                    const double ddy = (yi - st.r[1][p]) - sy;
+                    const double ddz = (zi - st.r[2][p]) - sz;
+                    const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                    if (r2 >= t.rc2) continue;
+                    const int32_t k = ti * kTypes + (st.type[p] & (kTypes - 1));
+                    if (!((t.active >> k) & 1u)) continue;
+                    if constexpr (kFp64) {
+                        const double q = r2 / t.sig2[k];
+                        const double iq4 = 1.0 / (q * q * q * q);
+                        const double iq7 = iq4 / (q * q * q);
+                        const double fr = (2 * iq7 - iq4) * t.coeff[k];
+                        fx += ddx * fr;
+                        fy += ddy * fr;
+                        fz += ddz * fr;
+                        if constexpr (kEnergy) {
+                            const double iq6 = 1.0 / (q * q * q);
+                            e += 0.5 * (t.e4[k] * iq6 * (iq6 - 1) - t.shift[k]);
+                        }
+                    } else {
+                        const float q = (float)r2 * t.inv_sig2f[k];
+                        const float iq = __builtin_amdgcn_rcpf(q);
+                        const float iq2 = iq * iq;
+                        const float iq4 = iq2 * iq2;
+                        const float iq6 = iq4 * iq2;
+                        const float iq8 = iq4 * iq4;
+                        const float fr = __builtin_fmaf(2.0f * iq8, iq6, -iq8) * t.coeff_f[k];
+                        const double frd = (double)fr;
+                        fx = __builtin_fma(frd, ddx, fx);
+                        fy = __builtin_fma(frd, ddy, fy);
+                        fz = __builtin_fma(frd, ddz, fz);
+                        if constexpr (kEnergy) {
+                            e += 0.5 * ((double)(t.e4_f[k] * iq6 * (iq6 - 1.0f)) - t.shift[k]);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    st.f[0][s] = fx;
+    st.f[1][s] = fy;
+    st.f[2][s] = fz;
+    if constexpr (kEnergy) e_atom[s] = e;
+}
+
+}  // namespace
+
+hipError_t launch_lj(hipStream_t s, const DevState& st, const int32_t* cell_of, const GridView& g,
+                     const LjTable& t, int32_t n, int32_t precision, double* energy_per_atom,
+                     const int32_t* gate) {
+    if (n <= 0) return hipSuccess;
+    dim3 grid((n + kBlock - 1) / kBlock);
+    const bool fp64 = precision == MDX_PREC_FP64;
+    if (fp64 && energy_per_atom)
+        k_lj<true, true><<<grid, kBlock, 0, s>>>(st, cell_of, g, t, n, energy_per_atom, gate);
+    else if (fp64)
+        k_lj<true, false><<<grid, kBlock, 0, s>>>(st, cell_of, g, t, n, nullptr, gate);
+    else if (energy_per_atom)
+        k_lj<false, true><<<grid, kBlock, 0, s>>>(st, cell_of, g, t, n, energy_per_atom, gate);
+    else
+        k_lj<false, false><<<grid, kBlock, 0, s>>>(st, cell_of, g, t, n, nullptr, gate);
+    return hipGetLastError();
+}
+
+}  // namespace mdx

@@ -1,0 +1,148 @@
+// mdx_internal.h -- device-side data layout and kernel entry points of the
+// gfx950 engine.  Not part of the C-ABI (include/mdx.h is).
+//
+// Layout in HBM: every per-atom quantity is a separate FP64 array (SoA) in
+// "slot" order.  Slots are the cell-sorted order produced by the last build
+// of the primary cell grid: the atoms of cell c occupy slots
+// [start[c], start[c+1]) in ascending original index.  id[slot] is the
+// caller's atom index and slot_of_id[] its inverse.  Kernels stream the SoA
+// arrays with unit stride; neighbour cells are contiguous slot ranges.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mdx.h"
+
+namespace mdx {
+
+constexpr int kWave = 64;
+
+// Per-atom SoA state (slot order).
+struct DevState {
+    double* r[3];
+    double* v[3];
+    double* a[3];
+    double* b[3];
+    double* c[3];
+    double* f[3];
+    int32_t* type;
+    int32_t* id;
+};
+
+// Cell grid as kernels see it.
+struct GridView {
+    int32_t g[3];      // storage extent per axis
+    int32_t cells;     // neighbour cells with index >= cells are skipped (walled)
+    int32_t periodic;  // 1: wrap with image shift
+    double L;          // periodic edge
+    double h;          // cell edge (walled: rs as double)
+    double npos;       // walled: half of box_size_list
+    const int32_t* start;  // ncell + 1 prefix offsets
+};
+
+// Pair table for Lennard-Jones, indexed by ti*kTypes + tj.
+constexpr int kTypes = 4;
+struct LjTable {
+    float sig2f[kTypes * kTypes];
+    float inv_sig2f[kTypes * kTypes];
+    float coeff_f[kTypes * kTypes];   // 24 eps / sig^2
+    float e4_f[kTypes * kTypes];      // 4 eps
+    double sig2[kTypes * kTypes];
+    double coeff[kTypes * kTypes];
+    double e4[kTypes * kTypes];
+    double shift[kTypes * kTypes];    // energy at rc (0 for the walled box)
+    uint32_t active;                  // bit ti*kTypes+tj
+    double rc2;
+    float rc2f;
+};
+
+struct MassTable {
+    double mass[kTypes];
+    double scale;   // a = F / mass * scale
+};
+
+// Reference bond-order parameter table (common/Constants.h, common/Getters.cpp),
+// indexed by [same-species N, same-species O, unlike] via pair_kind().
+struct BondRefParams {
+    double re[3], de[3], beta[3], r0v[3];
+    double rcut2;        // list cutoff squared (mb rcut^2)
+    double pair_cut2;    // CUTOFF_RIJSQ
+    double pair_cut;     // CUTOFF_RIJ
+};
+
+struct TersoffParams {
+    double A, B, lambda1, lambda2, lambda3, beta, n, c2, d2, h, R, D;
+    int32_t m;
+};
+
+// ---- host-side launch wrappers (one translation unit each) ----
+// cells.hip
+hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v);
+hipError_t launch_bin(hipStream_t s, const DevState& st, const int32_t* item_slot, int32_t items,
+                      const GridView& g, int32_t* cell_out, int32_t* counts, int32_t* err);
+hipError_t launch_exclusive_scan(hipStream_t s, const int32_t* in, int32_t* out, int64_t n,
+                                 int32_t* block_sums, int64_t block_sums_cap);
+int64_t scan_block_sums_needed(int64_t n);
+hipError_t launch_scatter(hipStream_t s, const int32_t* cell_of, const int32_t* start,
+                          int32_t* fill, int32_t* sorted_items, int32_t items);
+hipError_t launch_sort_cells(hipStream_t s, const int32_t* start, int64_t ncell,
+                             int32_t* sorted_items, const int32_t* key_of_item);
+hipError_t launch_permute(hipStream_t s, const DevState& src, const DevState& dst,
+                          const int32_t* sorted_src, const int32_t* cell_of_src,
+                          int32_t* cell_of_dst, int32_t* slot_of_id, double* const rref[3],
+                          int32_t n, int32_t wrap, double L);
+hipError_t launch_max_count(hipStream_t s, const int32_t* counts, int64_t ncell, int32_t* out);
+hipError_t launch_row_lengths(hipStream_t s, const GridView& g, const int32_t* cell_by_item,
+                              int32_t items, int32_t* max_len);
+hipError_t launch_emit_list(hipStream_t s, const GridView& g, const int32_t* cell_by_item,
+                            const int32_t* member_ids, int32_t items, int32_t nlistsize,
+                            int32_t* out);
+hipError_t launch_gather_i32(hipStream_t s, const int32_t* src, const int32_t* idx, int32_t* dst,
+                             int32_t n);
+// to_slots: dst[idx[i]] = src[i] (caller order -> slots); else dst[i] = src[idx[i]]
+hipError_t launch_reorder(hipStream_t s, const DevState& src, const DevState& dst,
+                          const int32_t* idx, int32_t n, int32_t to_slots);
+
+// lj.hip
+// gate: when non-null the kernel does nothing unless *gate != 0 (device-side
+// predicate for the force recompute after a wall hit)
+hipError_t launch_lj(hipStream_t s, const DevState& st, const int32_t* cell_of, const GridView& g,
+                     const LjTable& t, int32_t n, int32_t precision, double* energy_per_atom,
+                     const int32_t* gate);
+
+// gear.hip
+struct GearCoeffs {
+    double k[4];    // dt, dt^2/2, dt^3/6, dt^4/24 (as the reference forms them)
+    double kc[4];   // corrector: GEAR1 k1, GEAR2 k1/k0, GEAR3 k1/k2, GEAR4 k1/k3
+};
+GearCoeffs gear_coeffs(float dt);
+hipError_t launch_predict(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc);
+hipError_t launch_correct_walled(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
+                                 const MassTable& m, double npos, float eloss, int32_t* hit);
+hipError_t launch_accelerations(hipStream_t s, const DevState& st, int32_t n, const MassTable& m,
+                                const int32_t* gate);
+hipError_t launch_correct_predict(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
+                                  const MassTable& m, double* const rref[3], uint32_t* maxdisp2_bits);
+
+// bond_ref.hip
+hipError_t launch_bond_ref(hipStream_t s, const DevState& st, const int32_t* slot_of_id,
+                           const int32_t* mb_cell_of, const int32_t* mb_ids, const GridView& g,
+                           const BondRefParams& p, int32_t n_bond, int32_t nlistsize,
+                           int32_t precision, int32_t assign, int32_t* xi_j, double* xi_v,
+                           double* energy_per_atom, const int32_t* gate);
+
+// reduce.hip
+hipError_t launch_kinetics(hipStream_t s, const DevState& st, int32_t n, const MassTable& m,
+                           double* partials, int32_t nblocks, double* out8, double kboltz);
+int32_t kinetics_blocks(int32_t n);
+hipError_t launch_sum(hipStream_t s, const double* x, int32_t n, double* partials, int32_t nblocks,
+                      double* out);
+hipError_t launch_ready_list(hipStream_t s, const DevState& st, int32_t n, uint64_t* out2_bits);
+hipError_t launch_allpairs(hipStream_t s, const DevState& st, int32_t n, int32_t n_bond,
+                           const LjTable& lj, const BondRefParams& p, double* per_atom2);
+
+}  // namespace mdx
