@@ -1,0 +1,138 @@
+"""Headline benchmark: atom-steps/s of the device-resident MD step loop.
+
+Workload (one MI355X): periodic reduced-unit Lennard-Jones, 1,048,576 atoms
+(64^3 FCC cells x 4) at density 0.8442, T* = 0.728 seeded Maxwell-Boltzmann
+velocities, cutoff 2.5 sigma with 0.3 sigma skin (cells of edge >= 2.8),
+FP32 pair arithmetic with FP64 accumulation and FP64 Gear 5th-order
+predictor-corrector, dt = 0.005.  One step = predictor, [cell-list rebuild
+when due], LJ forces, corrector (fused with the next predictor).
+
+Prints one JSON line (see the fields below).  --gpus > 1 is not supported:
+this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+# bytes per atom per step of the whole step, FP64 state: Gear state r,v,a,b,c
+# read+written by the integrator (240), force written by the force kernel and
+# read by the integrator (48), positions read by the force kernel (24)
+STEP_BYTES_PER_ATOM = 312
+# dominant kernel (LJ forces): positions read (3 x 8 B) + force written (3 x 8 B)
+LJ_BYTES_PER_ATOM = 48
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--ncell", type=int, default=64, help="FCC cells per edge (4 ncell^3 atoms)")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--cpu-ncell", type=int, default=32)
+    p.add_argument("--cpu-steps", type=int, default=20)
+    return p.parse_args()
+
+
+def cpu_baseline(ncell: int, steps: int) -> dict:
+    """The FP64 oracle's single-threaded periodic LJ step loop (the reference
+    algorithm: 27-cell candidate lists, Gear PC) on a sub-box sample."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import _oracle as O
+    from moleculardynamics_amd import initial
+
+    pos, vel, L = initial.lj_fluid(ncell, 0.8442, 0.728, seed=12345)
+    _, _, _, builds, elapsed = O.pbc_lj_run(pos, vel, L, 2.5, 2.8, 0.005, steps)
+    return {
+        "value": len(pos) * steps / elapsed,
+        "unit": "atom-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{len(pos)} atoms ({ncell}^3 FCC x4, same density/T/cutoffs), {steps} steps, "
+                  f"{builds} list builds, {elapsed:.1f} s on {platform.processor() or platform.machine()}",
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    from moleculardynamics_amd import BOX_PERIODIC, LJ_UNIFORM, PREC_MIXED, System, initial
+
+    pos, vel, L = initial.lj_fluid(a.ncell, 0.8442, 0.728, seed=2024)
+    n = len(pos)
+    print(f"bench: LJ {n} atoms, L={L:.4f}; warmup {a.warmup}, steps {a.steps}", file=sys.stderr, flush=True)
+    s = System(n_atoms=n, box=BOX_PERIODIC, precision=PREC_MIXED, use_lj=1, lj_table=LJ_UNIFORM,
+               dt=0.005, lj_rs=2.8, lj_rcut=2.5, box_length=L, lj_sigma=1.0, lj_eps=1.0, mass=1.0)
+    s.set_state(np.c_[pos, np.zeros(n)], vel)
+    s.run(a.warmup)
+    s.synchronize()
+    b0 = s.build_count()
+    s.set_timing(True)
+    t0 = time.perf_counter()
+    s.run(a.steps)
+    s.synchronize()
+    t1 = time.perf_counter()
+    times = s.kernel_times()
+    s.set_timing(False)
+    builds = s.build_count() - b0
+    elapsed = t1 - t0
+    value = n * a.steps / elapsed
+    print(f"bench: {elapsed:.3f} s, {builds} list builds; kernels {times}", file=sys.stderr, flush=True)
+
+    cnt, ms = times.get("lj_forces", (0, 0.0))
+    lj_ms = ms / max(cnt, 1)
+    achieved = n * LJ_BYTES_PER_ATOM / (lj_ms * 1e-3) / 1e9 if cnt else None
+    step_gbs = value * STEP_BYTES_PER_ATOM / 1e9
+    out = {
+        "metric": "atom-steps/s per GPU (LJ 1M, mixed precision)",
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * elapsed / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-pair/fp64-accum",
+        "data": "synthetic (seeded FCC lattice + Maxwell-Boltzmann velocities)",
+        "config": {"workload": f"periodic LJ {n} atoms (FCC {a.ncell}^3 x4), rho 0.8442, T* 0.728, "
+                               f"rc 2.5 skin 0.3, dt 0.005", "atoms": n, "list_builds": builds},
+        "roofline": {
+            "kernel": "lj_forces",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": None,
+            "avg_launch_ms": lj_ms,
+            "bytes_per_atom": LJ_BYTES_PER_ATOM,
+            "step_gbs_at_312B_per_atom": step_gbs,
+            "step_frac": step_gbs / HBM_PEAK_GBS,
+        },
+        "kernel_ms": {k: v[1] / max(v[0], 1) for k, v in times.items()},
+    }
+    s.close()
+    if not a.no_cpu:
+        print("bench: cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(a.cpu_ncell, a.cpu_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

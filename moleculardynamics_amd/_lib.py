@@ -1,0 +1,222 @@
+"""ctypes binding of the C-ABI in include/mdx.h (libmdx.so, built in-tree).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+LIB_PATH = PKG / "libmdx.so"
+HEADER = ROOT / "include" / "mdx.h"
+
+MDX_OK = 0
+ERRORS = {
+    -1: "MDX_ERR_ARG", -2: "MDX_ERR_HIP", -3: "MDX_ERR_ALLOC", -4: "MDX_ERR_GRID",
+    -5: "MDX_ERR_CAPACITY", -6: "MDX_ERR_STATE", -7: "MDX_ERR_LIST_STALE",
+}
+BOX_WALLED, BOX_PERIODIC = 0, 1
+PREC_MIXED, PREC_FP64 = 0, 1
+LJ_REFERENCE, LJ_UNIFORM = 0, 1
+BOND_REFERENCE, BOND_TERSOFF = 0, 1
+LIST_LJ, LIST_BOND = 0, 1
+
+
+class MdxConfig(C.Structure):
+    _fields_ = [
+        ("n_atoms", C.c_int32), ("n_bond", C.c_int32), ("box", C.c_int32), ("precision", C.c_int32),
+        ("use_lj", C.c_int32), ("use_bond", C.c_int32), ("lj_table", C.c_int32), ("bond_form", C.c_int32),
+        ("dt", C.c_float), ("lj_rs", C.c_float), ("lj_rcut", C.c_float), ("mb_rs", C.c_float),
+        ("mb_rcut", C.c_float), ("energy_loss", C.c_float),
+        ("wall", C.c_double), ("box_length", C.c_double),
+        ("lj_sigma", C.c_double), ("lj_eps", C.c_double), ("mass", C.c_double),
+        ("accel_scale", C.c_double), ("max_neighbors", C.c_int32), ("reserved", C.c_int32 * 7),
+    ]
+
+
+class MdxError(RuntimeError):
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} -> {ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def header_symbols() -> list[str]:
+    """Entry points declared in include/mdx.h."""
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(mdx_\w+)\s*\(", text, re.M)))
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+    lib = C.CDLL(str(LIB_PATH))
+    P, I32, I64, D = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+    sig = {
+        "mdx_create": [C.POINTER(MdxConfig), C.POINTER(P)],
+        "mdx_destroy": [P],
+        "mdx_set_state": [P, P, P, P, P, P],
+        "mdx_get_state": [P, P, P, P, P, P, P],
+        "mdx_ready_list": [P, C.POINTER(D), C.POINTER(D)],
+        "mdx_build_list": [P, I32, D, C.POINTER(I32), C.POINTER(I32)],
+        "mdx_copy_neighbor_list": [P, I32, P, I64],
+        "mdx_predict": [P], "mdx_lj_forces": [P], "mdx_bond_forces": [P],
+        "mdx_correct": [P, C.POINTER(I32)], "mdx_accelerations": [P],
+        "mdx_run": [P, I32], "mdx_build_count": [P, C.POINTER(I64)],
+        "mdx_kinetics": [P, D, P], "mdx_potential": [P, P], "mdx_potential_allpairs": [P, P],
+        "mdx_set_timing": [P, I32], "mdx_kernel_times": [P, P, P, P, I32, C.POINTER(I32)],
+        "mdx_synchronize": [P],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    lib.mdx_last_error.argtypes = [P]
+    lib.mdx_last_error.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class System:
+    """Device-resident system; a thin owner of an mdx_system handle."""
+
+    def __init__(self, **cfg):
+        self.lib = load()
+        c = MdxConfig()
+        c.accel_scale = 1.0
+        c.lj_sigma = c.lj_eps = c.mass = 1.0
+        c.energy_loss = 1.0
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        self.cfg = c
+        self.n = c.n_atoms
+        h = C.c_void_p()
+        rc = self.lib.mdx_create(C.byref(c), C.byref(h))
+        if rc != MDX_OK:
+            raise MdxError("mdx_create", rc, self.lib.mdx_last_error(None).decode())
+        self.h = h
+
+    def _chk(self, fn: str, rc: int):
+        if rc != MDX_OK:
+            raise MdxError(fn, rc, self.lib.mdx_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mdx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- state --
+    def set_state(self, pos4, vel3, acc3=None, b3=None, c3=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (pos4, vel3, acc3, b3, c3)]
+        assert arrs[0].shape == (self.n, 4) and arrs[1].shape == (self.n, 3)
+        self._chk("mdx_set_state", self.lib.mdx_set_state(self.h, *[_ptr(a) for a in arrs]))
+
+    def get_state(self):
+        out = {"pos": np.zeros((self.n, 4)), "vel": np.zeros((self.n, 3)), "acc": np.zeros((self.n, 3)),
+               "b": np.zeros((self.n, 3)), "c": np.zeros((self.n, 3)), "force": np.zeros((self.n, 3))}
+        self._chk("mdx_get_state", self.lib.mdx_get_state(
+            self.h, *[_ptr(out[k]) for k in ("pos", "vel", "acc", "b", "c", "force")]))
+        return out
+
+    # -- reference entry points --
+    def ready_list(self):
+        vmax, box = C.c_double(), C.c_double()
+        self._chk("mdx_ready_list", self.lib.mdx_ready_list(self.h, C.byref(vmax), C.byref(box)))
+        return vmax.value, box.value
+
+    def build_list(self, which=LIST_LJ, box_size_list=0.0):
+        nls, nlg = C.c_int32(), C.c_int32()
+        self._chk("mdx_build_list", self.lib.mdx_build_list(self.h, which, box_size_list, C.byref(nls), C.byref(nlg)))
+        return nls.value, nlg.value
+
+    def neighbor_list(self, which=LIST_LJ, nlistsize=None, items=None):
+        items = self.n if items is None else items
+        out = np.zeros(max(items * nlistsize, 1), dtype=np.int32)
+        self._chk("mdx_copy_neighbor_list", self.lib.mdx_copy_neighbor_list(self.h, which, _ptr(out), out.size))
+        return out[: items * nlistsize].reshape(items, nlistsize)
+
+    def predict(self):
+        self._chk("mdx_predict", self.lib.mdx_predict(self.h))
+
+    def lj_forces(self):
+        self._chk("mdx_lj_forces", self.lib.mdx_lj_forces(self.h))
+
+    def bond_forces(self):
+        self._chk("mdx_bond_forces", self.lib.mdx_bond_forces(self.h))
+
+    def correct(self):
+        hit = C.c_int32()
+        self._chk("mdx_correct", self.lib.mdx_correct(self.h, C.byref(hit)))
+        return hit.value
+
+    def accelerations(self):
+        self._chk("mdx_accelerations", self.lib.mdx_accelerations(self.h))
+
+    def run(self, nsteps):
+        self._chk("mdx_run", self.lib.mdx_run(self.h, int(nsteps)))
+
+    def build_count(self):
+        b = C.c_int64()
+        self._chk("mdx_build_count", self.lib.mdx_build_count(self.h, C.byref(b)))
+        return b.value
+
+    def kinetics(self, kboltz):
+        out = np.zeros(8)
+        self._chk("mdx_kinetics", self.lib.mdx_kinetics(self.h, kboltz, _ptr(out)))
+        return out
+
+    def potential(self):
+        out = np.zeros(2)
+        self._chk("mdx_potential", self.lib.mdx_potential(self.h, _ptr(out)))
+        return out
+
+    def potential_allpairs(self):
+        out = np.zeros(2)
+        self._chk("mdx_potential_allpairs", self.lib.mdx_potential_allpairs(self.h, _ptr(out)))
+        return out
+
+    def synchronize(self):
+        self._chk("mdx_synchronize", self.lib.mdx_synchronize(self.h))
+
+    def set_timing(self, on: bool):
+        self._chk("mdx_set_timing", self.lib.mdx_set_timing(self.h, 1 if on else 0))
+
+    def kernel_times(self):
+        cap = 64
+        names = C.create_string_buffer(48 * cap)
+        counts = np.zeros(cap, dtype=np.int64)
+        tot = np.zeros(cap)
+        n = C.c_int32()
+        self._chk("mdx_kernel_times", self.lib.mdx_kernel_times(self.h, names, _ptr(counts), _ptr(tot), cap, C.byref(n)))
+        res = {}
+        for k in range(n.value):
+            nm = names.raw[48 * k: 48 * k + 48].split(b"\0", 1)[0].decode()
+            res[nm] = (int(counts[k]), float(tot[k]))
+        return res

@@ -48,11 +48,17 @@ __global__ void k_bin(DevState st, const int32_t* __restrict__ item_slot, int32_
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         double x = st.r[d][s];
+        // range-check in floating point first so NaN or huge values never
+        // reach the integer conversion
         if (g.periodic) {
-            int32_t k = (int32_t)(wrap_periodic(x, g.L) / g.h);
+            double y = wrap_periodic(x, g.L) / g.h;
+            ok = ok && (y >= 0.0 && y < (double)g.g[d] + 1.0);
+            int32_t k = ok ? (int32_t)y : 0;
             c[d] = k >= g.g[d] ? g.g[d] - 1 : k;
         } else {
-            c[d] = cell_coord_walled(x, g.npos, g.h);
+            double y = (x + g.npos) / g.h;
+            ok = ok && (y > -1.0 && y < (double)g.g[d]);
+            c[d] = ok ? (int32_t)y : 0;
             ok = ok && (c[d] >= 0 && c[d] < g.g[d]);
         }
     }

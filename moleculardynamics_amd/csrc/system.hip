@@ -1,0 +1,893 @@
+// system.hip -- host side of the C-ABI (include/mdx.h): system lifetime,
+// state transfer, list builds, the device-resident step loop and the
+// diagnostics.  All device work of one system runs on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "mdx_internal.h"
+
+using namespace mdx;
+
+namespace {
+
+constexpr int kNe = 2;
+
+struct Grid {
+    GridView view{};
+    int32_t* count = nullptr;  // ncell (histogram, then fill cursor)
+    int32_t* start = nullptr;  // ncell + 1
+    int64_t ncell = 0, cap = 0;
+    int32_t nlistsize = 0, nlargestsize = 0, largest = 0;
+    int32_t next_build = 0;    // walled schedule (cpu/cpumd/Host.cpp:288-290)
+    bool built = false;
+};
+
+struct EventPair {
+    const char* name;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct mdx_system {
+    mdx_config cfg{};
+    int32_t n = 0;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    DevState st{}, alt{};
+    std::vector<void*> allocs;
+    int32_t* slot_of_id = nullptr;
+    int32_t* cell_of = nullptr;      // primary grid cell per slot
+    int32_t* cell_of_alt = nullptr;
+    int32_t* sorted = nullptr;       // counting-sort output (items)
+    int32_t* cell_by_id = nullptr;   // scratch for list emission
+    double* rref[3] = {nullptr, nullptr, nullptr};
+    Grid lj, mb;
+    int32_t* mb_cell_of = nullptr;   // bond grid cell per id (< n_bond)
+    int32_t* mb_ids = nullptr;       // bond grid members (ids), cell-sorted
+    int32_t* scan_sums = nullptr;
+    int64_t scan_cap = 0;
+    int32_t* xi_j = nullptr;
+    double* xi_v = nullptr;
+    int64_t xi_alloc = 0;            // entries allocated (n_bond * cap)
+    int32_t xi_cap = 0;
+    double* partials = nullptr;
+    int32_t nb = 1;
+    double* dscal = nullptr;         // [0..7] kinetics, [8..9] sums
+    int32_t* dint = nullptr;         // [0] wall hit, [1] bin error, [2] max count, [3] max row
+    uint64_t* dbits = nullptr;       // [0] max disp^2, [1] max v^2, [2] max |coord|
+    double* e_atom = nullptr;        // 2n scratch
+    void* pinned = nullptr;          // 256 bytes
+    LjTable ljt{};
+    MassTable mt{};
+    BondRefParams bp{};
+    GearCoeffs gc{};
+    double vmax = 0, box_size_list = 0;
+    bool have_ready = false;
+    bool initialized = false;
+    bool predicted = false;          // periodic loop: predictor of the next step applied
+    int32_t t = 0;
+    int64_t builds = 0;
+    double disp_limit2 = 0;          // periodic: rebuild at max displacement^2 >= this
+    bool timing = false;
+    std::vector<EventPair> pending;
+    std::vector<hipEvent_t> free_events;
+    std::map<std::string, std::pair<int64_t, double>> times;
+    std::string err;
+};
+
+namespace {
+
+std::string g_err;  // errors before a system exists
+
+int fail(mdx_system* s, int code, const std::string& msg) {
+    if (s) s->err = msg; else g_err = msg;
+    return code;
+}
+
+#define HIPCHK(sys, expr)                                                                    \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(sys, MDX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T> int dalloc(mdx_system* s, T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(s, MDX_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    s->allocs.push_back(*p);
+    return MDX_OK;
+}
+
+void dfree(mdx_system* s, void* p) {
+    if (!p) return;
+    auto it = std::find(s->allocs.begin(), s->allocs.end(), p);
+    if (it != s->allocs.end()) s->allocs.erase(it);
+    hipFree(p);
+}
+
+hipEvent_t get_event(mdx_system* s) {
+    if (!s->free_events.empty()) {
+        hipEvent_t e = s->free_events.back();
+        s->free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+}
+
+// Launch wrapper: brackets the launch with events while timing is on.
+template <class F> hipError_t timed(mdx_system* s, const char* name, F&& launch) {
+    if (!s->timing) return launch();
+    EventPair p{name, get_event(s), get_event(s)};
+    hipEventRecord(p.a, s->stream);
+    hipError_t e = launch();
+    hipEventRecord(p.b, s->stream);
+    s->pending.push_back(p);
+    return e;
+}
+
+#define LAUNCH(sys, name, expr) HIPCHK(sys, timed(sys, name, [&]() { return (expr); }))
+
+int harvest(mdx_system* s) {
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    for (auto& p : s->pending) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, p.a, p.b);
+        auto& slot = s->times[p.name];
+        slot.first += 1;
+        slot.second += ms;
+        s->free_events.push_back(p.a);
+        s->free_events.push_back(p.b);
+    }
+    s->pending.clear();
+    return MDX_OK;
+}
+
+// ---- parameter tables ----
+void setup_tables(mdx_system* s) {
+    const mdx_config& c = s->cfg;
+    LjTable& t = s->ljt;
+    std::memset(&t, 0, sizeof t);
+    MassTable& m = s->mt;
+    m.scale = c.accel_scale > 0 ? c.accel_scale : 1.0;
+    if (c.lj_table == MDX_LJ_REFERENCE) {
+        // common/Constants.h:36-43, 74-76; cpu/common/LennardJones.h:63-88
+        const double sig[3] = {(double)3.015f, (double)2.835f, (double)2.72f};  // partner N, O, Ne
+        const double eps[3] = {(double)0.328e-4f, (double)0.447e-4f, (double)0.391e-4f};
+        for (int ti = 0; ti < kTypes; ti++)
+            for (int tj = 0; tj < kTypes; tj++) {
+                int k = ti * kTypes + tj;
+                if (ti > 2 || tj > 2) continue;
+                if (ti != kNe && tj != kNe) continue;
+                int sel = ti == kNe ? tj : ti;
+                double sg = sig[sel], ep = eps[sel];
+                t.sig2[k] = sg * sg;
+                t.coeff[k] = 24.0f * ep / (sg * sg);
+                t.e4[k] = 4 * ep;
+                t.shift[k] = 0;
+                t.active |= 1u << k;
+            }
+        const double mass[3] = {(double)14.00674f, (double)15.9994f, (double)20.1797f};
+        for (int q = 0; q < kTypes; q++) m.mass[q] = q < 3 ? mass[q] : mass[2];
+        // listSettings.rcutsq is a float (common/NeighborList.h:31, cpu/cpumd/Host.cpp:59)
+        t.rc2 = (double)(float)std::pow((double)c.lj_rcut, 2);
+    } else {
+        double sg = c.lj_sigma > 0 ? c.lj_sigma : 1.0, ep = c.lj_eps > 0 ? c.lj_eps : 1.0;
+        double rc2 = (double)c.lj_rcut * (double)c.lj_rcut;
+        double qc = rc2 / (sg * sg), iq6 = 1.0 / (qc * qc * qc);
+        for (int k = 0; k < kTypes * kTypes; k++) {
+            t.sig2[k] = sg * sg;
+            t.coeff[k] = 24.0 * ep / (sg * sg);
+            t.e4[k] = 4 * ep;
+            t.shift[k] = c.box == MDX_BOX_PERIODIC ? 4 * ep * iq6 * (iq6 - 1) : 0.0;
+            t.active |= 1u << k;
+        }
+        for (int q = 0; q < kTypes; q++) m.mass[q] = c.mass > 0 ? c.mass : 1.0;
+        t.rc2 = rc2;
+    }
+    for (int k = 0; k < kTypes * kTypes; k++) {
+        t.sig2f[k] = (float)t.sig2[k];
+        t.inv_sig2f[k] = t.sig2[k] > 0 ? (float)(1.0 / t.sig2[k]) : 0.f;
+        t.coeff_f[k] = (float)t.coeff[k];
+        t.e4_f[k] = (float)t.e4[k];
+    }
+    t.rc2f = (float)t.rc2;
+    // reference bond order (common/Constants.h:36-61, common/Getters.cpp:24-78):
+    // kinds 0 = N-N, 1 = O-O, 2 = N-O
+    BondRefParams& b = s->bp;
+    const float re[3] = {1.097f, 1.2075f, 1.15f}, de[3] = {941.4e-4f, 493.55309e-4f, 626.734e-4f};
+    const float be[3] = {2.71009f, 2.2478f, 2.24068f}, r0v[3] = {3.72f, 3.31f, 3.51f};
+    for (int k = 0; k < 3; k++) {
+        b.re[k] = re[k]; b.de[k] = de[k]; b.beta[k] = be[k]; b.r0v[k] = r0v[k];
+    }
+    b.rcut2 = (double)(float)std::pow((double)c.mb_rcut, 2);
+    b.pair_cut2 = (double)(225 * 3.31f * 3.31f);
+    b.pair_cut = (double)(15 * 3.31f);
+    s->gc = gear_coeffs(c.dt);
+}
+
+int alloc_state(mdx_system* s, DevState& d) {
+    int n = s->n;
+    for (int q = 0; q < 3; q++) {
+        int rc;
+        if ((rc = dalloc(s, &d.r[q], n)) || (rc = dalloc(s, &d.v[q], n)) || (rc = dalloc(s, &d.a[q], n)) ||
+            (rc = dalloc(s, &d.b[q], n)) || (rc = dalloc(s, &d.c[q], n)) || (rc = dalloc(s, &d.f[q], n)))
+            return rc;
+    }
+    int rc;
+    if ((rc = dalloc(s, &d.type, n)) || (rc = dalloc(s, &d.id, n))) return rc;
+    return MDX_OK;
+}
+
+int ensure_grid(mdx_system* s, Grid& g, int64_t ncell) {
+    if (ncell > (int64_t(1) << 27))
+        return fail(s, MDX_ERR_GRID, "cell grid of " + std::to_string(ncell) + " cells exceeds 2^27");
+    if (ncell > g.cap) {
+        dfree(s, g.count);
+        dfree(s, g.start);
+        int64_t cap = std::max<int64_t>(ncell, 2 * g.cap);
+        if (cap > (int64_t(1) << 27)) cap = ncell;
+        int rc;
+        if ((rc = dalloc(s, &g.count, cap)) || (rc = dalloc(s, &g.start, cap + 1))) return rc;
+        g.cap = cap;
+    }
+    int64_t need = scan_block_sums_needed(ncell + 1);
+    if (need > s->scan_cap) {
+        dfree(s, s->scan_sums);
+        int rc = dalloc(s, &s->scan_sums, need);
+        if (rc) return rc;
+        s->scan_cap = need;
+    }
+    g.ncell = ncell;
+    g.view.start = g.start;
+    return MDX_OK;
+}
+
+// grid geometry: walled uses the reference's rule (common/NeighborList.cpp:21-27),
+// periodic floor(L/rs) cells of edge L/n
+int set_geometry(mdx_system* s, Grid& g, float rs, double box_size_list) {
+    GridView& v = g.view;
+    if (!(rs > 0)) return fail(s, MDX_ERR_ARG, "list radius rs must be positive");
+    if (s->cfg.box == MDX_BOX_PERIODIC) {
+        double L = s->cfg.box_length;
+        int32_t nc = (int32_t)std::floor(L / (double)rs);
+        if (nc < 3) return fail(s, MDX_ERR_ARG, "periodic box must hold at least 3 cells of edge rs per axis");
+        v.g[0] = v.g[1] = v.g[2] = nc;
+        v.cells = nc;
+        v.periodic = 1;
+        v.L = L;
+        v.h = L / nc;
+        v.npos = 0;
+    } else {
+        int32_t cells = (int32_t)(box_size_list / (double)rs + 0.99f);
+        if (cells < 1) cells = 1;
+        v.g[0] = v.g[1] = v.g[2] = cells + 1;
+        v.cells = cells;
+        v.periodic = 0;
+        v.L = 0;
+        v.h = (double)rs;
+        v.npos = box_size_list / 2;
+    }
+    return ensure_grid(s, g, (int64_t)v.g[0] * v.g[1] * v.g[2]);
+}
+
+int read_ints(mdx_system* s, int32_t* host, const int32_t* dev, int count) {
+    HIPCHK(s, hipMemcpyAsync(s->pinned, dev, sizeof(int32_t) * count, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    std::memcpy(host, s->pinned, sizeof(int32_t) * count);
+    return MDX_OK;
+}
+
+// reference list sizes: nlistsize = min(27 * largest, n - 1) (NeighborList.cpp:88-90)
+int finish_sizes(mdx_system* s, Grid& g, const int32_t* cell_by_item, int32_t items) {
+    HIPCHK(s, hipMemsetAsync(s->dint + 2, 0, 2 * sizeof(int32_t), s->stream));
+    LAUNCH(s, "list_max_count", launch_max_count(s->stream, g.count, g.ncell, s->dint + 2));
+    LAUNCH(s, "list_row_lengths", launch_row_lengths(s->stream, g.view, cell_by_item, items, s->dint + 3));
+    int32_t h[2];
+    int rc = read_ints(s, h, s->dint + 2, 2);
+    if (rc) return rc;
+    g.largest = h[0];
+    int64_t nls = 27 * (int64_t)h[0];
+    if (nls > items - 1) nls = items - 1;
+    if (nls < 0) nls = 0;
+    g.nlistsize = (int32_t)nls;
+    g.nlargestsize = h[1];
+    return MDX_OK;
+}
+
+// Primary grid: bins every atom, counting-sorts slots into cell order (ascending
+// id within a cell) and permutes the whole state into that order.
+int build_primary(mdx_system* s, Grid& g, float rs, double box_size_list, bool sizes) {
+    int rc = set_geometry(s, g, rs, box_size_list);
+    if (rc) return rc;
+    hipStream_t st = s->stream;
+    const int32_t n = s->n;
+    HIPCHK(s, hipMemsetAsync(s->dint + 1, 0, sizeof(int32_t), st));
+    LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
+    LAUNCH(s, "cells_bin", launch_bin(st, s->st, nullptr, n, g.view, s->cell_of, g.count, s->dint + 1));
+    LAUNCH(s, "cells_scan", launch_exclusive_scan(st, g.count, g.start, g.ncell, s->scan_sums, s->scan_cap));
+    LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
+    LAUNCH(s, "cells_scatter", launch_scatter(st, s->cell_of, g.start, g.count, s->sorted, n));
+    LAUNCH(s, "cells_sort", launch_sort_cells(st, g.start, g.ncell, s->sorted, s->st.id));
+    const bool periodic = s->cfg.box == MDX_BOX_PERIODIC;
+    double* none[3] = {nullptr, nullptr, nullptr};
+    LAUNCH(s, "cells_permute",
+           launch_permute(st, s->st, s->alt, s->sorted, s->cell_of, s->cell_of_alt, s->slot_of_id,
+                          periodic ? s->rref : none, n, periodic ? 1 : 0, s->cfg.box_length));
+    std::swap(s->st, s->alt);
+    std::swap(s->cell_of, s->cell_of_alt);
+    // the histogram is needed again for the sizes; rebuild it from start[]
+    g.built = true;
+    s->builds++;
+    if (sizes) {
+        // counts = start[c+1] - start[c]; recompute by binning again
+        LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
+        LAUNCH(s, "cells_bin", launch_bin(st, s->st, nullptr, n, g.view, s->cell_by_id, g.count, s->dint + 1));
+        int32_t e;
+        if ((rc = read_ints(s, &e, s->dint + 1, 1))) return rc;
+        if (e) return fail(s, MDX_ERR_ARG, "an atom lies outside the list box (box_size_list too small)");
+        return finish_sizes(s, g, s->cell_of, n);
+    }
+    return MDX_OK;
+}
+
+// Bond-order grid of the walled box: the first n_bond atoms (by id) binned with
+// the bond list radius; members are ids, the state is not permuted.
+int build_bond_walled(mdx_system* s, double box_size_list) {
+    Grid& g = s->mb;
+    int rc = set_geometry(s, g, s->cfg.mb_rs, box_size_list);
+    if (rc) return rc;
+    hipStream_t st = s->stream;
+    const int32_t nb = s->cfg.n_bond;
+    HIPCHK(s, hipMemsetAsync(s->dint + 1, 0, sizeof(int32_t), st));
+    LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
+    LAUNCH(s, "cells_bin", launch_bin(st, s->st, s->slot_of_id, nb, g.view, s->mb_cell_of, g.count, s->dint + 1));
+    LAUNCH(s, "cells_scan", launch_exclusive_scan(st, g.count, g.start, g.ncell, s->scan_sums, s->scan_cap));
+    LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
+    LAUNCH(s, "cells_scatter", launch_scatter(st, s->mb_cell_of, g.start, g.count, s->mb_ids, nb));
+    LAUNCH(s, "cells_sort", launch_sort_cells(st, g.start, g.ncell, s->mb_ids, nullptr));
+    int32_t e;
+    if ((rc = read_ints(s, &e, s->dint + 1, 1))) return rc;
+    if (e) return fail(s, MDX_ERR_ARG, "an atom lies outside the list box (box_size_list too small)");
+    // rebuild the histogram for the sizes
+    LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
+    LAUNCH(s, "cells_bin", launch_bin(st, s->st, s->slot_of_id, nb, g.view, s->cell_by_id, g.count, s->dint + 1));
+    if ((rc = finish_sizes(s, g, s->mb_cell_of, nb))) return rc;
+    // xi table: one entry per candidate within the cutoff, at most the row length
+    int32_t cap = std::max(g.nlargestsize, 1);
+    int64_t need = (int64_t)nb * cap;
+    if (need > s->xi_alloc) {
+        dfree(s, s->xi_j);
+        dfree(s, s->xi_v);
+        if ((rc = dalloc(s, &s->xi_j, need)) || (rc = dalloc(s, &s->xi_v, 2 * need))) return rc;
+        s->xi_alloc = need;
+    }
+    s->xi_cap = cap;
+    g.built = true;
+    return MDX_OK;
+}
+
+int ready_list(mdx_system* s) {
+    HIPCHK(s, hipMemsetAsync(s->dbits + 1, 0, 2 * sizeof(uint64_t), s->stream));
+    LAUNCH(s, "ready_list", launch_ready_list(s->stream, s->st, s->n, s->dbits + 1));
+    HIPCHK(s, hipMemcpyAsync(s->pinned, s->dbits + 1, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    double v2, c;
+    std::memcpy(&v2, (char*)s->pinned, 8);
+    std::memcpy(&c, (char*)s->pinned + 8, 8);
+    s->vmax = std::sqrt(v2);
+    s->box_size_list = 2 * c;
+    s->have_ready = true;
+    return MDX_OK;
+}
+
+// cpu/cpumd/Host.cpp:288-290 (INT32_MAX: never again)
+int32_t next_build(float rs, float rc, double vmax, float dt, int32_t t) {
+    float maxnlmove = (rs - rc) * 0.5f;
+    double q = ((double)maxnlmove / vmax) / std::fabs((double)dt);
+    if (!(q < 2147483647.0 - (double)t)) return INT32_MAX;
+    return (int32_t)q + t;
+}
+
+int build_list(mdx_system* s, int32_t which, double box, bool sizes) {
+    const mdx_config& c = s->cfg;
+    int rc;
+    if (c.box == MDX_BOX_PERIODIC) {
+        float rs = c.use_lj ? c.lj_rs : c.mb_rs;
+        return build_primary(s, s->lj, rs, 0.0, sizes);
+    }
+    if (!s->have_ready && (rc = ready_list(s))) return rc;
+    if (which == MDX_LIST_LJ) {
+        if (!c.use_lj) return fail(s, MDX_ERR_STATE, "UseLennardJones is off");
+        rc = build_primary(s, s->lj, c.lj_rs, box, sizes);
+        if (rc) return rc;
+        s->lj.next_build = next_build(c.lj_rs, c.lj_rcut, s->vmax, c.dt, s->t);
+        return MDX_OK;
+    }
+    if (!c.use_bond) return fail(s, MDX_ERR_STATE, "UseManyBody is off");
+    rc = build_bond_walled(s, box);
+    if (rc) return rc;
+    s->mb.next_build = next_build(c.mb_rs, c.mb_rcut, s->vmax, c.dt, s->t);
+    return MDX_OK;
+}
+
+int lj_forces(mdx_system* s, double* e_atom, const int32_t* gate, const DevState* target = nullptr) {
+    if (!s->lj.built) return fail(s, MDX_ERR_STATE, "Lennard-Jones list not built");
+    const DevState& d = target ? *target : s->st;
+    LAUNCH(s, "lj_forces", launch_lj(s->stream, d, s->cell_of, s->lj.view, s->ljt, s->n,
+                                     s->cfg.precision, e_atom, gate));
+    return MDX_OK;
+}
+
+int bond_forces(mdx_system* s, double* e_atom, const int32_t* gate, const DevState* target = nullptr) {
+    const mdx_config& c = s->cfg;
+    if (c.box == MDX_BOX_PERIODIC || c.bond_form != MDX_BOND_REFERENCE)
+        return fail(s, MDX_ERR_ARG, "this bond-order form is not available in this box");
+    if (!s->mb.built) return fail(s, MDX_ERR_STATE, "bond-order list not built");
+    const DevState& d = target ? *target : s->st;
+    const int assign = c.use_lj ? 0 : 1;  // BondOrderFuncs.h:325-336
+    LAUNCH(s, "bond_forces",
+           launch_bond_ref(s->stream, d, s->slot_of_id, s->mb_cell_of, s->mb_ids, s->mb.view, s->bp,
+                           c.n_bond, s->xi_cap, c.precision, assign, s->xi_j, s->xi_v, e_atom, gate));
+    return MDX_OK;
+}
+
+int forces(mdx_system* s, const int32_t* gate) {
+    int rc;
+    if (s->cfg.use_lj && (rc = lj_forces(s, nullptr, gate))) return rc;
+    if (s->cfg.use_bond && (rc = bond_forces(s, nullptr, gate))) return rc;
+    return MDX_OK;
+}
+
+double npos_of(const mdx_system* s) {
+    return s->cfg.box == MDX_BOX_WALLED ? s->cfg.wall : INFINITY;
+}
+
+// walled box: the reference loop body (cpu/cpumd/Host.cpp:141-195)
+int run_walled(mdx_system* s, int32_t nsteps) {
+    const mdx_config& c = s->cfg;
+    int rc;
+    if (!s->initialized) {
+        if ((c.use_lj && !s->lj.built) || (c.use_bond && !s->mb.built)) {
+            if ((rc = ready_list(s))) return rc;
+            if (c.use_lj && !s->lj.built && (rc = build_list(s, MDX_LIST_LJ, s->box_size_list, true))) return rc;
+            if (c.use_bond && !s->mb.built && (rc = build_list(s, MDX_LIST_BOND, s->box_size_list, true))) return rc;
+        }
+        if ((rc = forces(s, nullptr))) return rc;
+        LAUNCH(s, "accelerations", launch_accelerations(s->stream, s->st, s->n, s->mt, nullptr));
+        s->initialized = true;
+    }
+    for (int32_t k = 0; k < nsteps; k++) {
+        bool lj_due = c.use_lj ? s->lj.next_build == s->t : s->t == 0;
+        bool mb_due = c.use_bond ? s->mb.next_build == s->t : s->t == 0;
+        if ((c.use_lj && lj_due) || (c.use_bond && mb_due)) {
+            if ((rc = ready_list(s))) return rc;
+            if (c.use_lj && lj_due && (rc = build_list(s, MDX_LIST_LJ, s->box_size_list, true))) return rc;
+            if (c.use_bond && mb_due && (rc = build_list(s, MDX_LIST_BOND, s->box_size_list, true))) return rc;
+        }
+        LAUNCH(s, "predict", launch_predict(s->stream, s->st, s->n, s->gc));
+        if ((rc = forces(s, nullptr))) return rc;
+        HIPCHK(s, hipMemsetAsync(s->dint, 0, sizeof(int32_t), s->stream));
+        LAUNCH(s, "correct", launch_correct_walled(s->stream, s->st, s->n, s->gc, s->mt, c.wall,
+                                                   c.energy_loss, s->dint));
+        // wall hit: forces again and a = F/m (Host.cpp:168-178), predicated on the device flag
+        if ((rc = forces(s, s->dint))) return rc;
+        LAUNCH(s, "accelerations", launch_accelerations(s->stream, s->st, s->n, s->mt, s->dint));
+        s->t++;
+    }
+    return MDX_OK;
+}
+
+// periodic box: predictor, [rebuild], forces, then corrector fused with the
+// next predictor; the list is rebuilt when the largest displacement since the
+// last build reaches (h - rc)/2, h the cell edge
+int run_periodic(mdx_system* s, int32_t nsteps) {
+    const mdx_config& c = s->cfg;
+    int rc;
+    const float rcut = c.use_lj ? c.lj_rcut : c.mb_rcut;
+    if (!s->initialized) {
+        if ((rc = build_list(s, MDX_LIST_LJ, 0.0, false))) return rc;
+        if ((rc = forces(s, nullptr))) return rc;
+        LAUNCH(s, "accelerations", launch_accelerations(s->stream, s->st, s->n, s->mt, nullptr));
+        s->initialized = true;
+    }
+    double margin = 0.5 * (s->lj.view.h - (double)rcut);
+    s->disp_limit2 = margin * margin;
+    if (nsteps <= 0) return MDX_OK;
+    // predictor of the first step (with displacement tracking via a zero-force corrector is
+    // not possible, so the plain predictor is followed by an explicit check)
+    LAUNCH(s, "predict", launch_predict(s->stream, s->st, s->n, s->gc));
+    bool check_first = true;
+    for (int32_t k = 0; k < nsteps; k++) {
+        bool rebuild = false;
+        if (check_first) {
+            // displacement after the plain predictor: measured by the ready-list pass
+            rebuild = true;  // conservative: rebuild at the start of every mdx_run call
+            check_first = false;
+        } else {
+            uint64_t bits;
+            HIPCHK(s, hipMemcpyAsync(s->pinned, s->dbits, sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
+            HIPCHK(s, hipStreamSynchronize(s->stream));
+            std::memcpy(&bits, s->pinned, 8);
+            double d2;
+            std::memcpy(&d2, &bits, 8);
+            rebuild = d2 >= s->disp_limit2;
+        }
+        if (rebuild && (rc = build_list(s, MDX_LIST_LJ, 0.0, false))) return rc;
+        if ((rc = forces(s, nullptr))) return rc;
+        HIPCHK(s, hipMemsetAsync(s->dbits, 0, sizeof(uint64_t), s->stream));
+        if (k + 1 < nsteps) {
+            LAUNCH(s, "correct_predict", launch_correct_predict(s->stream, s->st, s->n, s->gc, s->mt, s->rref,
+                                                                reinterpret_cast<uint32_t*>(s->dbits)));
+        } else {
+            LAUNCH(s, "correct", launch_correct_walled(s->stream, s->st, s->n, s->gc, s->mt, INFINITY, 1.0f,
+                                                       s->dint));
+        }
+        s->t++;
+    }
+    return MDX_OK;
+}
+
+bool valid_config(const mdx_config* c, std::string& why) {
+    if (!c) { why = "null config"; return false; }
+    if (c->n_atoms <= 0) { why = "n_atoms must be positive"; return false; }
+    if (c->n_bond < 0 || c->n_bond > c->n_atoms) { why = "n_bond out of range"; return false; }
+    if (c->box != MDX_BOX_WALLED && c->box != MDX_BOX_PERIODIC) { why = "unknown box"; return false; }
+    if (c->precision != MDX_PREC_MIXED && c->precision != MDX_PREC_FP64) { why = "unknown precision"; return false; }
+    if (c->lj_table != MDX_LJ_REFERENCE && c->lj_table != MDX_LJ_UNIFORM) { why = "unknown LJ table"; return false; }
+    if (c->bond_form != MDX_BOND_REFERENCE && c->bond_form != MDX_BOND_TERSOFF) { why = "unknown bond form"; return false; }
+    if (c->use_lj && !(c->lj_rcut > 0 && c->lj_rs >= c->lj_rcut)) { why = "need 0 < LennardJonesRCUT <= LennardJonesRS"; return false; }
+    if (c->use_bond && !(c->mb_rcut > 0 && c->mb_rs >= c->mb_rcut)) { why = "need 0 < ManyBodyRCUT <= ManyBodyRS"; return false; }
+    if (c->box == MDX_BOX_PERIODIC) {
+        if (!(c->box_length > 0)) { why = "periodic box needs box_length > 0"; return false; }
+        if (c->use_lj && c->use_bond) { why = "periodic box runs one potential at a time"; return false; }
+        if (c->use_bond && c->bond_form != MDX_BOND_TERSOFF) { why = "periodic bond order is Tersoff"; return false; }
+    } else if (c->use_bond && c->bond_form != MDX_BOND_REFERENCE) {
+        why = "walled box bond order is the reference form";
+        return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mdx_last_error(const mdx_system* s) { return s ? s->err.c_str() : g_err.c_str(); }
+
+int mdx_create(const mdx_config* cfg, mdx_system** out) {
+    if (!out) return fail(nullptr, MDX_ERR_ARG, "out is null");
+    *out = nullptr;
+    std::string why;
+    if (!valid_config(cfg, why)) return fail(nullptr, MDX_ERR_ARG, why);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(nullptr, MDX_ERR_HIP, "no HIP device available");
+    mdx_system* s = new mdx_system();
+    s->cfg = *cfg;
+    s->n = cfg->n_atoms;
+    hipGetDevice(&s->device);
+    int rc = MDX_OK;
+    auto bail = [&](int code) {
+        mdx_destroy(s);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(nullptr, MDX_ERR_HIP, "hipStreamCreate failed"));
+    const int n = s->n;
+    if ((rc = alloc_state(s, s->st)) || (rc = alloc_state(s, s->alt))) { g_err = s->err; return bail(rc); }
+    if ((rc = dalloc(s, &s->slot_of_id, n)) || (rc = dalloc(s, &s->cell_of, n)) ||
+        (rc = dalloc(s, &s->cell_of_alt, n)) || (rc = dalloc(s, &s->sorted, n)) ||
+        (rc = dalloc(s, &s->cell_by_id, n)) || (rc = dalloc(s, &s->mb_cell_of, std::max(cfg->n_bond, 1))) ||
+        (rc = dalloc(s, &s->mb_ids, std::max(cfg->n_bond, 1))) || (rc = dalloc(s, &s->rref[0], n)) ||
+        (rc = dalloc(s, &s->rref[1], n)) || (rc = dalloc(s, &s->rref[2], n)) ||
+        (rc = dalloc(s, &s->e_atom, 2 * (size_t)n)) || (rc = dalloc(s, &s->dscal, 16)) ||
+        (rc = dalloc(s, &s->dint, 8)) || (rc = dalloc(s, &s->dbits, 4))) {
+        g_err = s->err;
+        return bail(rc);
+    }
+    s->nb = kinetics_blocks(n);
+    if ((rc = dalloc(s, &s->partials, 8 * (size_t)s->nb + 16))) { g_err = s->err; return bail(rc); }
+    if (hipHostMalloc(&s->pinned, 256, hipHostMallocDefault) != hipSuccess)
+        return bail(fail(nullptr, MDX_ERR_ALLOC, "hipHostMalloc failed"));
+    setup_tables(s);
+    hipMemsetAsync(s->dint, 0, 8 * sizeof(int32_t), s->stream);
+    hipMemsetAsync(s->dbits, 0, 4 * sizeof(uint64_t), s->stream);
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(fail(nullptr, MDX_ERR_HIP, "init failed"));
+    *out = s;
+    return MDX_OK;
+}
+
+int mdx_destroy(mdx_system* s) {
+    if (!s) return MDX_OK;
+    if (s->stream) hipStreamSynchronize(s->stream);
+    for (auto& p : s->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
+    for (auto e : s->free_events) hipEventDestroy(e);
+    for (void* p : s->allocs) hipFree(p);
+    if (s->pinned) hipHostFree(s->pinned);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+    return MDX_OK;
+}
+
+int mdx_set_state(mdx_system* s, const double* pos4, const double* vel3, const double* acc3,
+                  const double* b3, const double* c3) {
+    if (!s || !pos4 || !vel3) return fail(s, MDX_ERR_ARG, "pos4 and vel3 are required");
+    const int n = s->n;
+    std::vector<double> buf(3 * (size_t)n);
+    std::vector<int32_t> type(n), id(n);
+    for (int i = 0; i < n; i++) {
+        double w = pos4[4 * i + 3];
+        int t = (int)w;
+        if (t < 0 || t >= kTypes) return fail(s, MDX_ERR_ARG, "species code out of range at atom " + std::to_string(i));
+        if (s->cfg.use_bond && s->cfg.box == MDX_BOX_WALLED && i < s->cfg.n_bond && t == kNe)
+            return fail(s, MDX_ERR_ARG, "bond-order atom " + std::to_string(i) + " is Ne");
+        type[i] = t;
+        id[i] = i;
+    }
+    hipStream_t st = s->stream;
+    // stage in caller order into alt, then scatter into slot order
+    auto put = [&](double* const dst[3], const double* src, int stride) -> int {
+        for (int d = 0; d < 3; d++) {
+            for (int i = 0; i < n; i++) buf[(size_t)d * n + i] = src ? src[(size_t)stride * i + d] : 0.0;
+            HIPCHK(s, hipMemcpyAsync(dst[d], buf.data() + (size_t)d * n, sizeof(double) * n, hipMemcpyHostToDevice, st));
+        }
+        HIPCHK(s, hipStreamSynchronize(st));
+        return MDX_OK;
+    };
+    int rc;
+    if ((rc = put(s->alt.r, pos4, 4)) || (rc = put(s->alt.v, vel3, 3)) || (rc = put(s->alt.a, acc3, 3)) ||
+        (rc = put(s->alt.b, b3, 3)) || (rc = put(s->alt.c, c3, 3)) || (rc = put(s->alt.f, nullptr, 3)))
+        return rc;
+    HIPCHK(s, hipMemcpyAsync(s->alt.type, type.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+    HIPCHK(s, hipMemcpyAsync(s->alt.id, id.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+    if (!s->lj.built) {
+        // identity slots
+        HIPCHK(s, hipMemcpyAsync(s->slot_of_id, id.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+    }
+    LAUNCH(s, "state_in", launch_reorder(st, s->alt, s->st, s->slot_of_id, n, 1));
+    HIPCHK(s, hipStreamSynchronize(st));
+    s->have_ready = false;
+    s->initialized = false;
+    s->t = 0;
+    return MDX_OK;
+}
+
+int mdx_get_state(mdx_system* s, double* pos4, double* vel3, double* acc3, double* b3, double* c3,
+                  double* force3) {
+    if (!s) return fail(s, MDX_ERR_ARG, "null system");
+    const int n = s->n;
+    hipStream_t st = s->stream;
+    LAUNCH(s, "state_out", launch_reorder(st, s->st, s->alt, s->slot_of_id, n, 0));
+    std::vector<double> buf((size_t)n);
+    auto get = [&](double* const src[3], double* dst, int stride) -> int {
+        if (!dst) return MDX_OK;
+        for (int d = 0; d < 3; d++) {
+            HIPCHK(s, hipMemcpyAsync(buf.data(), src[d], sizeof(double) * n, hipMemcpyDeviceToHost, st));
+            HIPCHK(s, hipStreamSynchronize(st));
+            for (int i = 0; i < n; i++) dst[(size_t)stride * i + d] = buf[i];
+        }
+        return MDX_OK;
+    };
+    int rc;
+    if ((rc = get(s->alt.r, pos4, 4)) || (rc = get(s->alt.v, vel3, 3)) || (rc = get(s->alt.a, acc3, 3)) ||
+        (rc = get(s->alt.b, b3, 3)) || (rc = get(s->alt.c, c3, 3)) || (rc = get(s->alt.f, force3, 3)))
+        return rc;
+    if (pos4) {
+        std::vector<int32_t> type(n);
+        HIPCHK(s, hipMemcpy(type.data(), s->alt.type, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; i++) pos4[4 * i + 3] = (double)type[i];
+    }
+    return MDX_OK;
+}
+
+int mdx_ready_list(mdx_system* s, double* vmax, double* box_size_list) {
+    if (!s) return fail(s, MDX_ERR_ARG, "null system");
+    int rc = ready_list(s);
+    if (rc) return rc;
+    if (vmax) *vmax = s->vmax;
+    if (box_size_list) *box_size_list = s->box_size_list;
+    return MDX_OK;
+}
+
+int mdx_build_list(mdx_system* s, int32_t which, double box_size_list, int32_t* nlistsize,
+                   int32_t* nlargestsize) {
+    if (!s) return fail(s, MDX_ERR_ARG, "null system");
+    if (which != MDX_LIST_LJ && which != MDX_LIST_BOND) return fail(s, MDX_ERR_ARG, "unknown list");
+    int rc = build_list(s, which, box_size_list, true);
+    if (rc) return rc;
+    Grid& g = (which == MDX_LIST_LJ || s->cfg.box == MDX_BOX_PERIODIC) ? s->lj : s->mb;
+    if (nlistsize) *nlistsize = g.nlistsize;
+    if (nlargestsize) *nlargestsize = g.nlargestsize;
+    return MDX_OK;
+}
+
+int mdx_copy_neighbor_list(mdx_system* s, int32_t which, int32_t* out, int64_t capacity) {
+    if (!s || !out) return fail(s, MDX_ERR_ARG, "null argument");
+    const bool primary = which == MDX_LIST_LJ || s->cfg.box == MDX_BOX_PERIODIC;
+    Grid& g = primary ? s->lj : s->mb;
+    if (!g.built) return fail(s, MDX_ERR_STATE, "list not built");
+    const int32_t items = primary ? s->n : s->cfg.n_bond;
+    const int64_t need = (int64_t)items * g.nlistsize;
+    if (capacity < need) return fail(s, MDX_ERR_CAPACITY, "output capacity below n * nlistsize");
+    if (need == 0) return MDX_OK;
+    hipStream_t st = s->stream;
+    const int32_t* cell_by_item;
+    const int32_t* members;
+    if (primary) {
+        LAUNCH(s, "list_gather", launch_gather_i32(st, s->cell_of, s->slot_of_id, s->cell_by_id, items));
+        cell_by_item = s->cell_by_id;
+        members = s->st.id;
+    } else {
+        cell_by_item = s->mb_cell_of;
+        members = s->mb_ids;
+    }
+    int32_t* dout;
+    HIPCHK(s, hipMalloc((void**)&dout, sizeof(int32_t) * need));
+    hipError_t e = launch_emit_list(st, g.view, cell_by_item, members, items, g.nlistsize, dout);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(int32_t) * need, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(dout);
+    if (e != hipSuccess) return fail(s, MDX_ERR_HIP, std::string("emit list: ") + hipGetErrorString(e));
+    return MDX_OK;
+}
+
+int mdx_predict(mdx_system* s) {
+    if (!s) return fail(s, MDX_ERR_ARG, "null system");
+    LAUNCH(s, "predict", launch_predict(s->stream, s->st, s->n, s->gc));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return MDX_OK;
+}
+
+int mdx_lj_forces(mdx_system* s) {
+    if (!s) return fail(s, MDX_ERR_ARG, "null system");
+    if (!s->cfg.use_lj) return fail(s, MDX_ERR_STATE, "UseLennardJones is off");
+    int rc = lj_forces(s, nullptr, nullptr);
+    if (rc) return rc;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return MDX_OK;
+}
+
+int mdx_bond_forces(mdx_system* s) {
+    if (!s) return fail(s, MDX_ERR_ARG, "null system");
+    if (!s->cfg.use_bond) return fail(s, MDX_ERR_STATE, "UseManyBody is off");
+    int rc = bond_forces(s, nullptr, nullptr);
+    if (rc) return rc;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return MDX_OK;
+}
+
+int mdx_correct(mdx_system* s, int32_t* wall_hit) {
+    if (!s) return fail(s, MDX_ERR_ARG, "null system");
+    HIPCHK(s, hipMemsetAsync(s->dint, 0, sizeof(int32_t), s->stream));
+    LAUNCH(s, "correct", launch_correct_walled(s->stream, s->st, s->n, s->gc, s->mt, npos_of(s),
+                                               s->cfg.energy_loss, s->dint));
+    int32_t h;
+    int rc = read_ints(s, &h, s->dint, 1);
+    if (rc) return rc;
+    if (wall_hit) *wall_hit = h;
+    return MDX_OK;
+}
+
+int mdx_accelerations(mdx_system* s) {
+    if (!s) return fail(s, MDX_ERR_ARG, "null system");
+    LAUNCH(s, "accelerations", launch_accelerations(s->stream, s->st, s->n, s->mt, nullptr));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return MDX_OK;
+}
+
+int mdx_run(mdx_system* s, int32_t nsteps) {
+    if (!s) return fail(s, MDX_ERR_ARG, "null system");
+    if (nsteps < 0) return fail(s, MDX_ERR_ARG, "nsteps must be >= 0");
+    int rc = s->cfg.box == MDX_BOX_PERIODIC ? run_periodic(s, nsteps) : run_walled(s, nsteps);
+    if (rc) return rc;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return MDX_OK;
+}
+
+int mdx_build_count(mdx_system* s, int64_t* builds) {
+    if (!s || !builds) return fail(s, MDX_ERR_ARG, "null argument");
+    *builds = s->builds;
+    return MDX_OK;
+}
+
+int mdx_kinetics(mdx_system* s, double kboltz, double* out8) {
+    if (!s || !out8) return fail(s, MDX_ERR_ARG, "null argument");
+    LAUNCH(s, "kinetics", launch_kinetics(s->stream, s->st, s->n, s->mt, s->partials, s->nb, s->dscal, kboltz));
+    HIPCHK(s, hipMemcpyAsync(s->pinned, s->dscal, 8 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    std::memcpy(out8, s->pinned, 8 * sizeof(double));
+    return MDX_OK;
+}
+
+int mdx_potential(mdx_system* s, double* out2) {
+    if (!s || !out2) return fail(s, MDX_ERR_ARG, "null argument");
+    out2[0] = out2[1] = 0;
+    // forces of this evaluation go to scratch so the stored force is untouched
+    DevState scratch = s->st;
+    for (int d = 0; d < 3; d++) scratch.f[d] = s->alt.f[d];
+    int rc;
+    hipStream_t st = s->stream;
+    if (s->cfg.box == MDX_BOX_PERIODIC && !s->lj.built && (rc = build_list(s, MDX_LIST_LJ, 0.0, false)))
+        return rc;
+    if (s->cfg.use_lj) {
+        if ((rc = lj_forces(s, s->e_atom, nullptr, &scratch))) return rc;
+        LAUNCH(s, "energy_sum", launch_sum(st, s->e_atom, s->n, s->partials, s->nb, s->dscal + 8));
+    }
+    if (s->cfg.use_bond) {
+        if ((rc = bond_forces(s, s->e_atom + s->n, nullptr, &scratch))) return rc;
+        LAUNCH(s, "energy_sum", launch_sum(st, s->e_atom + s->n, s->cfg.n_bond, s->partials, s->nb, s->dscal + 9));
+    }
+    HIPCHK(s, hipMemcpyAsync(s->pinned, s->dscal + 8, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(s, hipStreamSynchronize(st));
+    double v[2];
+    std::memcpy(v, s->pinned, sizeof v);
+    if (s->cfg.use_lj) out2[0] = v[0];
+    if (s->cfg.use_bond) out2[1] = v[1];
+    return MDX_OK;
+}
+
+int mdx_potential_allpairs(mdx_system* s, double* out2) {
+    if (!s || !out2) return fail(s, MDX_ERR_ARG, "null argument");
+    if (s->cfg.box != MDX_BOX_WALLED) return fail(s, MDX_ERR_ARG, "all-pairs energy is defined for the walled box");
+    hipStream_t st = s->stream;
+    LAUNCH(s, "allpairs", launch_allpairs(st, s->st, s->n, s->cfg.n_bond, s->ljt, s->bp, s->e_atom));
+    LAUNCH(s, "energy_sum", launch_sum(st, s->e_atom, s->n, s->partials, s->nb, s->dscal + 8));
+    LAUNCH(s, "energy_sum", launch_sum(st, s->e_atom + s->n, s->n, s->partials, s->nb, s->dscal + 9));
+    HIPCHK(s, hipMemcpyAsync(s->pinned, s->dscal + 8, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(s, hipStreamSynchronize(st));
+    std::memcpy(out2, s->pinned, 2 * sizeof(double));
+    return MDX_OK;
+}
+
+int mdx_set_timing(mdx_system* s, int32_t enabled) {
+    if (!s) return fail(s, MDX_ERR_ARG, "null system");
+    int rc = harvest(s);
+    if (rc) return rc;
+    s->times.clear();
+    s->timing = enabled != 0;
+    return MDX_OK;
+}
+
+int mdx_kernel_times(mdx_system* s, char* names, int64_t* counts, double* total_ms, int32_t cap,
+                     int32_t* n) {
+    if (!s || !n) return fail(s, MDX_ERR_ARG, "null argument");
+    int rc = harvest(s);
+    if (rc) return rc;
+    int32_t k = 0;
+    for (auto& kv : s->times) {
+        if (k >= cap) break;
+        if (names) {
+            std::strncpy(names + 48 * k, kv.first.c_str(), 47);
+            names[48 * k + 47] = 0;
+        }
+        if (counts) counts[k] = kv.second.first;
+        if (total_ms) total_ms[k] = kv.second.second;
+        k++;
+    }
+    *n = k;
+    return MDX_OK;
+}
+
+int mdx_synchronize(mdx_system* s) {
+    if (!s) return fail(s, MDX_ERR_ARG, "null system");
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return MDX_OK;
+}
+
+}  // extern "C"

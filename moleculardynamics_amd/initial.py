@@ -1,0 +1,58 @@
+"""Seeded initial conditions (lattices + Maxwell-Boltzmann velocities).
+
+The reference's equilibrator seeds with the wall clock (cpu/equi/Host.cpp:83);
+these generators are seeded so two engines can read identical inputs.  They
+write the reference's text input format (common/Loader.cpp:125-155:
+per atom "x y z type vx vy vz").
+"""
+from __future__ import annotations
+
+import numpy as np
+
+FCC_BASIS = np.array([[0, 0, 0], [0, 0.5, 0.5], [0.5, 0, 0.5], [0.5, 0.5, 0]])
+DIAMOND_BASIS = np.array([[0, 0, 0], [0, 0.5, 0.5], [0.5, 0, 0.5], [0.5, 0.5, 0],
+                          [0.25, 0.25, 0.25], [0.25, 0.75, 0.75], [0.75, 0.25, 0.75], [0.75, 0.75, 0.25]])
+
+
+def lattice(ncell: int, a: float, basis: np.ndarray) -> np.ndarray:
+    g = np.stack(np.meshgrid(np.arange(ncell), np.arange(ncell), np.arange(ncell), indexing="ij"), -1).reshape(-1, 3)
+    return ((g[:, None, :] + basis[None]) * a).reshape(-1, 3)
+
+
+def fcc(ncell: int, density: float) -> tuple[np.ndarray, float]:
+    """FCC block of 4 ncell^3 atoms at number density `density`; returns (pos, box edge)."""
+    a = (4.0 / density) ** (1.0 / 3.0)
+    return lattice(ncell, a, FCC_BASIS), a * ncell
+
+
+def diamond(ncell: int, a: float) -> tuple[np.ndarray, float]:
+    return lattice(ncell, a, DIAMOND_BASIS), a * ncell
+
+
+def maxwell_boltzmann(n: int, kT_over_m: float, seed: int) -> np.ndarray:
+    """Gaussian velocities with zero net momentum, rescaled so that
+    sum v^2 / (3 n) equals kT/m exactly (single species)."""
+    rng = np.random.default_rng(seed)
+    v = rng.normal(0.0, np.sqrt(kT_over_m), size=(n, 3))
+    v -= v.mean(axis=0)
+    v *= np.sqrt(3 * n * kT_over_m / np.sum(v * v))
+    return v
+
+
+def lj_fluid(ncell: int, density: float, temperature: float, seed: int):
+    """Reduced-unit LJ start: FCC at density, MB velocities at T* (m = kB = 1)."""
+    pos, L = fcc(ncell, density)
+    vel = maxwell_boltzmann(len(pos), temperature, seed)
+    return pos, vel, L
+
+
+def write_reference_input(path, pos3: np.ndarray, types: np.ndarray, vel3: np.ndarray) -> None:
+    """common/Loader.cpp:125-155 reads x y z type vx vy vz per atom with >>."""
+    with open(path, "w") as f:
+        for p, t, v in zip(pos3, types, vel3):
+            f.write(f"{p[0]:.17g} {p[1]:.17g} {p[2]:.17g} {int(t)} {v[0]:.17g} {v[1]:.17g} {v[2]:.17g}\n")
+
+
+def read_reference_input(path, n: int):
+    vals = np.array(open(path).read().split(), dtype=np.float64)[: 7 * n].reshape(n, 7)
+    return vals[:, :4].copy(), vals[:, 4:7].copy()

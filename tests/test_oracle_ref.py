@@ -1,0 +1,306 @@
+"""Pins for the oracle's restatement of the reference cpumd path.
+
+Each test checks the oracle against something it does not share code with:
+brute-force set definitions, finite differences of the reference's own
+energy definition (common/DebugCalculations.cpp:129-270), closed forms, and
+textbook integrator behaviour.  The reference ships no golden vectors and
+cannot be compiled here without stand-in Windows headers, so these checks are
+the oracle's anchor.
+"""
+import numpy as np
+import pytest
+
+import _oracle as O
+
+N, OX, NE = 0, 1, 2
+
+
+def brute_rows(pos4, rs, box):
+    """Set definition of common/NeighborList.cpp:14-156 by brute force."""
+    rs_d = float(np.float32(rs))
+    cells = int(box / rs_d + float(np.float32(0.99)))
+    cells = max(cells, 1)
+    npos = box / 2
+    c = np.trunc((pos4[:, :3] + npos) / rs_d).astype(np.int64)
+    rows = []
+    for i in range(len(pos4)):
+        row = []
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    t = c[i] + (dx, dy, dz)
+                    if (t < 0).any() or (t >= cells).any():
+                        continue
+                    m = np.nonzero((c == t).all(axis=1))[0]
+                    row.extend(int(j) for j in m if j != i)
+        rows.append(row)
+    occ = {}
+    for k in map(tuple, c):
+        occ[k] = occ.get(k, 0) + 1
+    return rows, max(occ.values()), cells
+
+
+def check_rows(pos4, rs, rc):
+    vmax, box = O.ready_list(pos4, np.zeros((len(pos4), 3)))
+    lst, ls = O.build_neighbor_list(pos4, rs, rc, box)
+    rows, largest, _ = brute_rows(pos4, rs, box)
+    n = len(pos4)
+    assert ls.nlistsize == min(27 * largest, n - 1)
+    assert ls.nlargestsize == max(len(r) for r in rows)
+    for i, r in enumerate(rows):
+        got = lst[i]
+        assert list(got[: len(r)]) == r, i
+        assert (got[len(r):] == -1).all()
+    return lst, ls
+
+
+def test_neighbor_list_matches_set_definition_random():
+    rng = np.random.default_rng(3)
+    pos = np.zeros((300, 4))
+    pos[:, :3] = rng.uniform(-9, 9, (300, 3))
+    pos[:, 3] = rng.integers(0, 3, 300)
+    check_rows(pos, 2.8, 2.5)
+
+
+def test_neighbor_list_edge_cell_is_skipped():
+    # 2 max|coord| / rs = 3.005: cells = 3 and the extreme atom falls in cell 3,
+    # which the gather loop skips (NeighborList.cpp:115)
+    pos = np.array([[3.005, 0, 0, 2], [-3.005, 0, 0, 2], [2.2, 0.1, 0.1, 2], [0, 0, 0, 2], [1.0, -1, 0.5, 2]])
+    lst, _ = check_rows(pos, 2.0, 1.5)
+    assert 0 not in lst[2]          # atom 0 is invisible to its neighbours
+    assert 2 in lst[0]              # but it still sees the cell next to it
+
+
+def test_ready_list_and_schedule():
+    pos = np.array([[1.0, -4.0, 2.0, 0], [0.5, 0.5, 3.5, 1]])
+    vel = np.array([[3.0, 4.0, 0.0], [0, 1, 0]])
+    vmax, box = O.ready_list(pos, vel)
+    assert vmax == 5.0 and box == 8.0
+    assert O.next_build(0.15, 2.0, 0.005, 7) == 7 + 15
+    assert O.next_build(0.15, 0.0, 0.005, 7) == 2**31 - 1
+
+
+def test_arvdwfc_derivatives_are_negative_gradients():
+    for (ti, tj) in [(N, N), (OX, OX), (N, OX)]:
+        de, re, be, r0v = (O.param(k, ti, tj) for k in (2, 1, 3, 4))
+        for r in (0.9, 1.1, 1.6, 2.5, 4.0):
+            h = 1e-6
+            va_p, _ = O.scalar2("orc_ref_attractive", r + h, de, be, re)
+            va_m, _ = O.scalar2("orc_ref_attractive", r - h, de, be, re)
+            _, dva = O.scalar2("orc_ref_attractive", r, de, be, re)
+            assert dva == pytest.approx(-(va_p - va_m) / (2 * h), rel=1e-7)
+            vd_p, _ = O.scalar2("orc_ref_vdw", r + h, r0v)
+            vd_m, _ = O.scalar2("orc_ref_vdw", r - h, r0v)
+            _, dvd = O.scalar2("orc_ref_vdw", r, r0v)
+            assert dvd == pytest.approx(-(vd_p - vd_m) / (2 * h), rel=1e-6, abs=1e-12)
+            fc_p, _ = O.scalar2("orc_ref_cutoff", r + h)
+            fc_m, _ = O.scalar2("orc_ref_cutoff", r - h)
+            _, dfc = O.scalar2("orc_ref_cutoff", r)
+            assert dfc == pytest.approx((fc_p - fc_m) / (2 * h), rel=1e-7, abs=1e-14)
+    # V_A at r = re is S De/(S-1) = 2 De; the cutoff is 1/2 at R0
+    de, re, be = O.param(2, N, N), O.param(1, N, N), O.param(3, N, N)
+    assert O.scalar2("orc_ref_attractive", re, de, be, re)[0] == pytest.approx(2 * de)
+    assert O.scalar2("orc_ref_cutoff", float(np.float32(1.3)))[0] == pytest.approx(0.5)
+
+
+def test_parameter_table_values():
+    # common/Constants.h:42-61 through common/Getters.cpp
+    assert O.param(0, N, 0) == float(np.float32(14.00674))
+    assert O.param(0, NE, 0) == float(np.float32(20.1797))
+    assert O.param(1, N, OX) == float(np.float32(1.15))
+    assert O.param(2, OX, OX) == float(np.float32(493.55309e-4))
+    assert O.param(3, N, N) == float(np.float32(2.71009))
+    assert O.param(4, OX, N) == float(np.float32(3.51))
+
+
+def cluster(rng, kinds, spread=1.4):
+    pos = np.zeros((len(kinds), 4))
+    pos[:, :3] = rng.normal(0, spread, (len(kinds), 3))
+    pos[:, 3] = kinds
+    return pos
+
+
+def fd_grad(fun, pos4, h=1e-5, idx=None):
+    g = np.zeros((len(pos4), 3))
+    for i in (range(len(pos4)) if idx is None else idx):
+        for d in range(3):
+            p = pos4.copy(); p[i, d] += h
+            m = pos4.copy(); m[i, d] -= h
+            g[i, d] = (fun(p) - fun(m)) / (2 * h)
+    return g
+
+
+def test_lj_forces_are_gradient_of_reference_energy():
+    rng = np.random.default_rng(5)
+    # Ne atoms plus N/O partners, spaced so no pair is unphysically close
+    kinds = [NE, NE, NE, N, OX, NE]
+    pos = np.array([[0, 0, 0], [3.1, 0, 0.2], [0.3, 3.0, 0], [-2.9, 0.4, 1.0], [0.2, -0.3, 3.2], [2.5, 2.7, 2.2]], float)
+    pos += rng.normal(0, 0.1, pos.shape)
+    pos4 = np.c_[pos, kinds]
+    rs, rc = 40.0, 30.0
+    lst, ls = O.build_neighbor_list(pos4, rs, rc, O.ready_list(pos4, np.zeros((6, 3)))[1])
+    f = O.lj_forces(pos4, lst, ls.nlargestsize, float(ls.rcutsq))
+    g = fd_grad(lambda p: O.potential(p)[0] / 2 * 2, pos4)
+    # potential() sums ordered pairs at 2 eps(...) each = sum over unordered pairs of 4 eps(...)
+    np.testing.assert_allclose(f, -g, rtol=1e-6, atol=1e-12)
+    assert np.abs(f.sum(axis=0)).max() < 1e-15 * np.abs(f).max() * 10 + 1e-20
+
+
+def test_bond_order_forces_are_gradient_of_reference_energy():
+    """For a cluster where every atom is in every other atom's row and within
+    the cutoff, the reference's hand-derived force (both loops) must equal the
+    negative gradient of its own brute-force bond-order energy."""
+    rng = np.random.default_rng(11)
+    seen_quirk = 0.0
+    for trial in range(6):
+        kinds = rng.integers(0, 2, 6)
+        pos4 = cluster(rng, kinds, spread=0.9)
+        # keep atoms apart so the Morse repulsion stays moderate
+        while True:
+            d = np.linalg.norm(pos4[:, None, :3] - pos4[None, :, :3], axis=-1) + np.eye(6) * 9
+            if d.min() > 0.95:
+                break
+            pos4 = cluster(rng, kinds, spread=0.9)
+        rs, rc = 30.0, 20.0
+        _, box = O.ready_list(pos4, np.zeros((6, 3)))
+        lst, ls = O.build_neighbor_list(pos4, rs, rc, box)
+        assert (lst >= 0).all() and lst.shape[1] == 5
+        f = O.mb_forces(pos4, lst, ls.nlargestsize, float(ls.rcutsq))
+        g = fd_grad(lambda p: O.potential(p)[1], pos4, h=1e-6)
+        # The reference's repulsive() returns dvr = 2 beta V_R for the whole
+        # V_R = De e^{-2 beta (r-re)} + 7e-8 e^{-10 beta (r-re)} (arvdwfc.cpp:29-30),
+        # so the 7e-8 term's force carries 2 beta instead of 10 beta: per pair
+        # F_ref - F_exact = -8 beta 7e-8 e^{-10 beta (r-re)} (r_i - r_j)/r on i.
+        quirk = np.zeros((6, 3))
+        for i in range(6):
+            for j in range(6):
+                if i == j:
+                    continue
+                ti, tj = int(pos4[i, 3]), int(pos4[j, 3])
+                re, be = O.param(1, ti, tj), O.param(3, ti, tj)
+                d = pos4[i, :3] - pos4[j, :3]
+                r = np.linalg.norm(d)
+                quirk[i] += -8 * be * float(np.float32(7e-8)) * np.exp(-10 * be * (r - re)) * d / r
+        seen_quirk = max(seen_quirk, np.abs(quirk).max())
+        np.testing.assert_allclose(f, -g + quirk, rtol=2e-6, atol=1e-9 * np.abs(g).max())
+    assert seen_quirk > 1e-6   # at least one trial has pairs close enough to see the quirk
+
+
+def test_bond_order_two_atoms_closed_form():
+    # no third atom: b = 1, force = -(V_R - V_A + vdW)'
+    for (ti, tj, r) in [(N, N, 1.2), (OX, N, 1.4), (OX, OX, 0.9)]:
+        pos4 = np.array([[0, 0, 0, ti], [r, 0, 0, tj]], float)
+        lst = np.array([[1], [0]], dtype=np.int32)
+        f = O.mb_forces(pos4, lst, 1, 100.0)
+        de, re, be, r0v = (O.param(k, ti, tj) for k in (2, 1, 3, 4))
+        va, dva = O.scalar2("orc_ref_attractive", r, de, be, re)
+        vr, dvr = O.scalar2("orc_ref_repulsive", r, de, be, re)
+        vdw, dvdw = O.scalar2("orc_ref_vdw", r, r0v)
+        # the reference's dvr/dva are beta*sqrt(2S)*V_R and beta*sqrt(2/S)*V_A, i.e. -dV/dr of
+        # the leading exponential; the 7e-8 term shares the first exponent's factor
+        assert f[0, 0] == pytest.approx(-(dvr - dva + dvdw), rel=1e-12)
+        assert f[1, 0] == pytest.approx(dvr - dva + dvdw, rel=1e-12)
+
+
+def quartic_state(coef, t):
+    """x(t) = sum coef[k] t^k and its first four derivatives."""
+    p = np.polynomial.Polynomial(coef)
+    return [p(t), p.deriv(1)(t), p.deriv(2)(t), p.deriv(3)(t), p.deriv(4)(t)]
+
+
+def test_gear_predictor_exact_on_quartic_and_corrector_fixed_point():
+    rng = np.random.default_rng(2)
+    coefs = rng.normal(size=(4, 3, 5))
+    dt = float(np.float32(0.01))
+    t0 = 0.3
+    st = np.array([[quartic_state(coefs[i, d], t0) for d in range(3)] for i in range(4)])  # (n, 3, 5)
+    r = np.c_[st[:, :, 0], np.zeros(4)]
+    v, a, b, c = (st[:, :, k].copy() for k in (1, 2, 3, 4))
+    r2, v2, a2, b2, c2 = O.predict(r, v, a, b, c, dt)
+    ex = np.array([[quartic_state(coefs[i, d], t0 + dt) for d in range(3)] for i in range(4)])
+    np.testing.assert_allclose(r2[:, :3], ex[:, :, 0], rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(v2, ex[:, :, 1], rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(a2, ex[:, :, 2], rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(b2, ex[:, :, 3], rtol=1e-12, atol=1e-12)
+    m = O.param(0, N, 0)
+    out = O.correct(r2, v2, a2 * m, a2, b2, c2, dt, 1e9, 1.0)
+    for got, want in zip(out[:5], (r2, v2, a2, b2, c2)):
+        np.testing.assert_allclose(got, want, rtol=1e-15, atol=1e-15)
+    assert out[5] == 0
+
+
+def harmonic_error(dt, periods=1.0):
+    """Integrate x'' = -x with the reference Gear PC; returns max |x - cos t|."""
+    m = O.param(0, NE, 0)
+    r = np.array([[1.0, 0, 0, NE]])
+    v = np.zeros((1, 3)); a = np.array([[-1.0, 0, 0]]); b = np.zeros((1, 3)); c = np.array([[1.0, 0, 0]])
+    nsteps = int(round(2 * np.pi * periods / dt))
+    err = 0
+    for k in range(nsteps):
+        r, v, a, b, c = O.predict(r, v, a, b, c, dt)
+        f = -r[:, :3] * m
+        r, v, a, b, c, _ = O.correct(r, v, f, a, b, c, dt, 1e9, 1.0)
+        err = max(err, abs(r[0, 0] - np.cos((k + 1) * dt)))
+    return err
+
+
+def test_gear_harmonic_oscillator_accuracy_order():
+    """Gear's 5-value corrector (19/120, 3/4, 1, 1/2, 1/12) is a high-order
+    method: halving dt cuts the phase error by at least 2^3 (a wrong
+    coefficient drops it to first or second order)."""
+    e1 = harmonic_error(0.02)
+    e2 = harmonic_error(0.01)
+    assert e1 < 1e-5
+    assert e1 / e2 > 8
+
+
+def test_walls():
+    m = O.param(0, N, 0)
+    npos = 10.0
+    r = np.array([[10.5, 1.0, 0, N], [10.5, -1.0, 0, N], [10.5, 0.0, 0, N], [-10.5, 0, 0, N],
+                  [0, 11, 0, N], [0, 0, -11, N], [0, 0, 0, N]], float)
+    v = np.tile([1.0, 2.0, 3.0], (7, 1))
+    a = np.zeros((7, 3)); b = np.zeros((7, 3)); c = np.zeros((7, 3))
+    f = a * m
+    eloss = np.float32(np.sqrt(np.float32(0.81)))
+    r2, v2, _, _, _, flag = O.correct(r, v, f, a, b, c, 0.01, npos, eloss)
+    assert flag == 1
+    assert r2[0, 0] == npos and v2[0, 0] == pytest.approx(-1.0 * float(eloss))
+    assert r2[1, 0] == -npos and v2[1, 0] == 1.0        # teleport, velocity kept
+    assert r2[2, 0] == 10.5 and v2[2, 0] == 1.0         # y == 0: no action
+    assert r2[3, 0] == -npos and v2[3, 0] == -1.0
+    assert r2[4, 1] == npos and v2[4, 1] == -2.0
+    assert r2[5, 2] == -npos and v2[5, 2] == -3.0
+    _, _, _, _, _, flag = O.correct(r[6:], v[6:], f[6:], a[6:], b[6:], c[6:], 0.01, npos, eloss)
+    assert flag == 0
+
+
+def test_kinetics_closed_form():
+    pos = np.array([[0, 0, 0, N], [1, 0, 0, OX], [0, 2, 0, NE]], float)
+    vel = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+    out = O.kinetics(pos, vel)
+    m = np.array([O.param(0, k, 0) for k in (N, OX, NE)])
+    assert out[0] == pytest.approx(0.5 * m.sum())
+    P = (m[:, None] * vel).sum(0)
+    np.testing.assert_allclose(out[5:8], P)
+    vcm = P / m.sum()
+    T = (m * ((vel - vcm) ** 2).sum(1)).sum() / (3 * 3 * float(np.float32(8.314e-7)))
+    assert out[1] == pytest.approx(T, rel=1e-12)
+    np.testing.assert_allclose(out[2:5], (m[:, None] * pos[:, :3]).sum(0) / m.sum())
+
+
+def test_reference_run_conserves_energy_for_lj_cluster():
+    from moleculardynamics_amd import initial
+    p, L = initial.fcc(3, 0.8442)
+    sig = float(np.float32(2.72))
+    pos = np.c_[(p - p.mean(0)) * sig, np.full(len(p), NE)]
+    mng = O.param(0, NE, 0)
+    kT = 0.728 * float(np.float32(0.391e-4))
+    vel = initial.maxwell_boltzmann(len(p), kT / mng, 7)
+    r, v, rec, _ = O.ref_run(pos, vel, len(p), 0, 200, 50, 5.0, 2.8 * sig, 2.5 * sig, 1.0, 1.0)
+    # records: before the loop, at t = 50, 100, 150 (Host.cpp:181-185 checks
+    # printout == timesteps for t < 200), and after the loop (t = 200)
+    np.testing.assert_allclose(rec[:, 0], np.array([0, 50, 100, 150, 200]) * float(np.float32(5.0)))
+    # (the reference's printed energy is all-pairs LJ; in a free cluster with the
+    # 2.5 sigma cutoff it drifts slowly; bound the drift relative to EK)
+    assert abs(rec[-1, 2] - rec[0, 2]) < 0.05 * abs(rec[0, 3])
