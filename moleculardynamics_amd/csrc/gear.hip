@@ -113,7 +113,48 @@ k_correct_predict(DevState st, int32_t n, GearCoeffs gc, MassTable m, const doub
     if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(maxdisp2_bits, bits);
 }
 
+__global__ void __launch_bounds__(kBlock)
+k_predict_track(DevState st, int32_t n, GearCoeffs gc, const double* __restrict__ r0x,
+                const double* __restrict__ r0y, const double* __restrict__ r0z,
+                unsigned long long* __restrict__ flags) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double d2 = 0.0, s2 = 0.0;
+    if (i < n) {
+        const double* r0[3] = {r0x, r0y, r0z};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            double r = st.r[d][i], v = st.v[d][i], a = st.a[d][i], b = st.b[d][i], c = st.c[d][i];
+            double rp = r + gc.k[0] * v + gc.k[1] * a + gc.k[2] * b + gc.k[3] * c;
+            st.r[d][i] = rp;
+            st.v[d][i] = v + gc.k[0] * a + gc.k[1] * b + gc.k[2] * c;
+            st.a[d][i] = a + gc.k[0] * b + gc.k[1] * c;
+            st.b[d][i] = b + gc.k[0] * c;
+            double dd = rp - r0[d][i], ds = rp - r;
+            d2 += dd * dd;
+            s2 += ds * ds;
+        }
+    }
+    unsigned long long bd = __double_as_longlong(d2), bs = __double_as_longlong(s2);
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        unsigned long long x = __shfl_xor(bd, o, kWave), y = __shfl_xor(bs, o, kWave);
+        bd = x > bd ? x : bd;
+        bs = y > bs ? y : bs;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        atomicMax(&flags[0], bd);
+        atomicMax(&flags[1], bs);
+    }
+}
+
 }  // namespace
+
+hipError_t launch_predict_track(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
+                                double* const rref[3], uint64_t* flags) {
+    if (n <= 0) return hipSuccess;
+    k_predict_track<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(
+        st, n, gc, rref[0], rref[1], rref[2], reinterpret_cast<unsigned long long*>(flags));
+    return hipGetLastError();
+}
 
 // Device.cpp:19-22 and :48-56: c1 = dt, c2 = c1 dt/2, c3 = c2 dt/3,
 // c4 = c3 dt/4 in real (double) from the float DT; corrector factors from the

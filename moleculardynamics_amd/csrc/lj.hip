@@ -73,19 +73,19 @@ k_lj(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable t, in
                             e += 0.5 * (t.e4[k] * iq6 * (iq6 - 1) - t.shift[k]);
                         }
                     } else {
-                        const float q = (float)r2 * t.inv_sig2f[k];
-                        const float iq = __builtin_amdgcn_rcpf(q);
-                        const float iq2 = iq * iq;
-                        const float iq4 = iq2 * iq2;
-                        const float iq6 = iq4 * iq2;
-                        const float iq8 = iq4 * iq4;
-                        const float fr = __builtin_fmaf(2.0f * iq8, iq6, -iq8) * t.coeff_f[k];
+                        // s2 = (sig/r)^2, s6 = (sig/r)^6, s8 = (sig/r)^8
+                        const float s2 = __builtin_amdgcn_rcpf((float)r2 * t.inv_sig2f[k]);
+                        const float s4 = s2 * s2;
+                        const float s6 = s4 * s2;
+                        const float s8 = s4 * s4;
+                        // 24 eps/sig^2 [2 (sig/r)^14 - (sig/r)^8] = coeff s8 (2 s6 - 1)
+                        const float fr = s8 * __builtin_fmaf(2.0f, s6, -1.0f) * t.coeff_f[k];
                         const double frd = (double)fr;
                         fx = __builtin_fma(frd, ddx, fx);
                         fy = __builtin_fma(frd, ddy, fy);
                         fz = __builtin_fma(frd, ddz, fz);
                         if constexpr (kEnergy) {
-                            e += 0.5 * ((double)(t.e4_f[k] * iq6 * (iq6 - 1.0f)) - t.shift[k]);
+                            e += 0.5 * ((double)(t.e4_f[k] * s6 * (s6 - 1.0f)) - t.shift[k]);
                         }
                     }
                 }

@@ -128,6 +128,20 @@ hipError_t launch_accelerations(hipStream_t s, const DevState& st, int32_t n, co
 hipError_t launch_correct_predict(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
                                   const MassTable& m, double* const rref[3], uint32_t* maxdisp2_bits);
 
+// predictor that also max-reduces |r - rref|^2 into flags[0] and the squared
+// step |r_new - r_old|^2 into flags[1] (bits of non-negative doubles)
+hipError_t launch_predict_track(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
+                                double* const rref[3], uint64_t* flags);
+
+// lj_fused.hip
+hipError_t launch_nlist(hipStream_t s, const DevState& st, const int32_t* cell_of, const GridView& g,
+                        double rl2, int32_t n, int32_t cap, uint32_t* list, int32_t* count,
+                        int32_t* overflow);
+hipError_t launch_step(hipStream_t s, const DevState& st, double* const rn[3], double* const r0[3],
+                       const uint32_t* list, const int32_t* count, const LjTable& t, const MassTable& m,
+                       const GearCoeffs& gc, double L, double hard2, int32_t n, int32_t precision,
+                       bool final_step, bool force_out, uint64_t* flags);
+
 // bond_ref.hip
 hipError_t launch_bond_ref(hipStream_t s, const DevState& st, const int32_t* slot_of_id,
                            const int32_t* mb_cell_of, const int32_t* mb_ids, const GridView& g,

@@ -77,6 +77,15 @@ struct mdx_system {
     int32_t t = 0;
     int64_t builds = 0;
     double disp_limit2 = 0;          // periodic: rebuild at max displacement^2 >= this
+    // periodic fused path
+    double* rnext[3] = {nullptr, nullptr, nullptr};  // r(t+1) written by the step kernel
+    uint32_t* nlist = nullptr;       // column-major neighbour list
+    int32_t* ncount = nullptr;
+    int32_t nl_cap = 0;
+    static constexpr int kRing = 3;
+    uint64_t* dflags = nullptr;      // kRing x 4: max disp^2, max step^2, stale count, spare
+    uint64_t* hflags = nullptr;      // pinned copy of dflags
+    hipEvent_t ev[kRing] = {nullptr, nullptr, nullptr};
     bool timing = false;
     std::vector<EventPair> pending;
     std::vector<hipEvent_t> free_events;
@@ -493,53 +502,109 @@ int run_walled(mdx_system* s, int32_t nsteps) {
     return MDX_OK;
 }
 
-// periodic box: predictor, [rebuild], forces, then corrector fused with the
-// next predictor; the list is rebuilt when the largest displacement since the
-// last build reaches (h - rc)/2, h the cell edge
+// Periodic box, cell grid + neighbour list of radius rs (rebuild of both).
+int rebuild_periodic(mdx_system* s) {
+    const mdx_config& c = s->cfg;
+    int rc = build_list(s, MDX_LIST_LJ, 0.0, false);
+    if (rc) return rc;
+    const int n = s->n;
+    const double rl2 = (double)c.lj_rs * (double)c.lj_rs;
+    if (!s->nlist) {
+        // first guess from the density: 1.5 x the mean count inside rs, plus slack
+        double rho = n / (c.box_length * c.box_length * c.box_length);
+        double est = 4.0 / 3.0 * M_PI * std::pow((double)c.lj_rs, 3) * rho;
+        int32_t cap = c.max_neighbors > 0 ? c.max_neighbors : ((int32_t)(1.5 * est) + 32 + 31) / 32 * 32;
+        if ((rc = dalloc(s, &s->nlist, (size_t)cap * n)) || (rc = dalloc(s, &s->ncount, n))) return rc;
+        s->nl_cap = cap;
+    }
+    for (;;) {
+        HIPCHK(s, hipMemsetAsync(s->dint + 4, 0, sizeof(int32_t), s->stream));
+        LAUNCH(s, "nlist_build", launch_nlist(s->stream, s->st, s->cell_of, s->lj.view, rl2, n, s->nl_cap,
+                                              s->nlist, s->ncount, s->dint + 4));
+        int32_t over;
+        if ((rc = read_ints(s, &over, s->dint + 4, 1))) return rc;
+        if (over == 0) break;
+        // capacity exceeded: grow and build again
+        dfree(s, s->nlist);
+        s->nlist = nullptr;
+        int32_t cap = (over + over / 4 + 31) / 32 * 32;
+        if ((rc = dalloc(s, &s->nlist, (size_t)cap * n))) return rc;
+        s->nl_cap = cap;
+    }
+    return MDX_OK;
+}
+
+double bits_to_double(uint64_t b) {
+    double d;
+    std::memcpy(&d, &b, 8);
+    return d;
+}
+
+// Periodic box, the fused path: one kernel per step computes the LJ force of
+// r(t) from the neighbour list, corrects step t and predicts step t+1.  The
+// rebuild decision runs one step behind the device: before step k is
+// launched the host reads the flags of the positions r(k-1) (largest
+// displacement since the build D and largest single-step move S) and
+// rebuilds when D + 2 S reaches the margin (rs - rc)/2; every step kernel
+// re-checks its own positions against the margin and counts violations, which
+// turn into MDX_ERR_LIST_STALE.
 int run_periodic(mdx_system* s, int32_t nsteps) {
     const mdx_config& c = s->cfg;
     int rc;
-    const float rcut = c.use_lj ? c.lj_rcut : c.mb_rcut;
+    const int n = s->n;
+    const double margin = 0.5 * ((double)c.lj_rs - (double)c.lj_rcut);
+    const double hard2 = margin * margin;
+    hipStream_t st = s->stream;
     if (!s->initialized) {
-        if ((rc = build_list(s, MDX_LIST_LJ, 0.0, false))) return rc;
-        if ((rc = forces(s, nullptr))) return rc;
-        LAUNCH(s, "accelerations", launch_accelerations(s->stream, s->st, s->n, s->mt, nullptr));
+        if ((rc = rebuild_periodic(s))) return rc;
+        if ((rc = lj_forces(s, nullptr, nullptr))) return rc;  // Host.cpp:132-138
+        LAUNCH(s, "accelerations", launch_accelerations(st, s->st, n, s->mt, nullptr));
         s->initialized = true;
     }
-    double margin = 0.5 * (s->lj.view.h - (double)rcut);
-    s->disp_limit2 = margin * margin;
     if (nsteps <= 0) return MDX_OK;
-    // predictor of the first step (with displacement tracking via a zero-force corrector is
-    // not possible, so the plain predictor is followed by an explicit check)
-    LAUNCH(s, "predict", launch_predict(s->stream, s->st, s->n, s->gc));
-    bool check_first = true;
+    const int R = mdx_system::kRing;
+    auto slot_flags = [&](int k) { return s->dflags + 4 * (size_t)(((k % R) + R) % R); };
+    auto host_flags = [&](int k) { return s->hflags + 4 * (size_t)(((k % R) + R) % R); };
+    auto post = [&](int k) -> int {  // copy step flags to the host and mark them
+        HIPCHK(s, hipMemcpyAsync(host_flags(k), slot_flags(k), 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(s, hipEventRecord(s->ev[((k % R) + R) % R], st));
+        return MDX_OK;
+    };
+    // flags of r(0) come from the first predictor (index -1)
+    HIPCHK(s, hipMemsetAsync(slot_flags(-1), 0, 4 * sizeof(uint64_t), st));
+    LAUNCH(s, "predict", launch_predict_track(st, s->st, n, s->gc, s->rref, slot_flags(-1)));
+    if ((rc = post(-1))) return rc;
+    int last_build = -1;   // step index whose positions were used for the last build
     for (int32_t k = 0; k < nsteps; k++) {
+        // Flags index f describes r(f+1) (f = -1: the predictor's r(0)).  Step k
+        // uses r(k); the host reads index k-2, i.e. r(k-1), while launch k-1 runs.
+        const int fk = k >= 1 ? k - 2 : -1;
+        HIPCHK(s, hipEventSynchronize(s->ev[((fk % R) + R) % R]));
+        const uint64_t* h = host_flags(fk);
+        if (h[2] != 0) return fail(s, MDX_ERR_LIST_STALE, "an atom moved past the neighbour-list margin");
+        const double D = std::sqrt(bits_to_double(h[0])), S = std::sqrt(bits_to_double(h[1]));
         bool rebuild = false;
-        if (check_first) {
-            // displacement after the plain predictor: measured by the ready-list pass
-            rebuild = true;  // conservative: rebuild at the start of every mdx_run call
-            check_first = false;
-        } else {
-            uint64_t bits;
-            HIPCHK(s, hipMemcpyAsync(s->pinned, s->dbits, sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
-            HIPCHK(s, hipStreamSynchronize(s->stream));
-            std::memcpy(&bits, s->pinned, 8);
-            double d2;
-            std::memcpy(&d2, &bits, 8);
-            rebuild = d2 >= s->disp_limit2;
+        if (k == 0)
+            rebuild = D >= margin;                 // exact: these flags are of r(0)
+        else if (fk >= last_build)
+            rebuild = D + 2.0 * S >= margin;       // r(k) is at most about one more step away
+        // (fk < last_build: the flags predate the last build, r(k) is one step from it)
+        if (rebuild) {
+            if ((rc = rebuild_periodic(s))) return rc;
+            last_build = k;
         }
-        if (rebuild && (rc = build_list(s, MDX_LIST_LJ, 0.0, false))) return rc;
-        if ((rc = forces(s, nullptr))) return rc;
-        HIPCHK(s, hipMemsetAsync(s->dbits, 0, sizeof(uint64_t), s->stream));
-        if (k + 1 < nsteps) {
-            LAUNCH(s, "correct_predict", launch_correct_predict(s->stream, s->st, s->n, s->gc, s->mt, s->rref,
-                                                                reinterpret_cast<uint32_t*>(s->dbits)));
-        } else {
-            LAUNCH(s, "correct", launch_correct_walled(s->stream, s->st, s->n, s->gc, s->mt, INFINITY, 1.0f,
-                                                       s->dint));
-        }
+        const bool fin = k + 1 == nsteps;
+        HIPCHK(s, hipMemsetAsync(slot_flags(k), 0, 4 * sizeof(uint64_t), st));
+        LAUNCH(s, "md_step", launch_step(st, s->st, s->rnext, s->rref, s->nlist, s->ncount, s->ljt, s->mt, s->gc,
+                                         c.box_length, hard2, n, c.precision, fin, fin, slot_flags(k)));
+        if ((rc = post(k))) return rc;
+        for (int d = 0; d < 3; d++) std::swap(s->st.r[d], s->rnext[d]);
         s->t++;
     }
+    // stale check of the last steps
+    HIPCHK(s, hipStreamSynchronize(st));
+    for (int k = std::max(0, nsteps - R); k < nsteps; k++)
+        if (host_flags(k)[2] != 0) return fail(s, MDX_ERR_LIST_STALE, "an atom moved past the neighbour-list margin");
     return MDX_OK;
 }
 
@@ -602,9 +667,20 @@ int mdx_create(const mdx_config* cfg, mdx_system** out) {
         return bail(rc);
     }
     s->nb = kinetics_blocks(n);
-    if ((rc = dalloc(s, &s->partials, 8 * (size_t)s->nb + 16))) { g_err = s->err; return bail(rc); }
-    if (hipHostMalloc(&s->pinned, 256, hipHostMallocDefault) != hipSuccess)
+    if ((rc = dalloc(s, &s->partials, 8 * (size_t)s->nb + 16)) ||
+        (rc = dalloc(s, &s->dflags, 4 * (size_t)mdx_system::kRing))) {
+        g_err = s->err;
+        return bail(rc);
+    }
+    if (cfg->box == MDX_BOX_PERIODIC)
+        for (int d = 0; d < 3; d++)
+            if ((rc = dalloc(s, &s->rnext[d], n))) { g_err = s->err; return bail(rc); }
+    if (hipHostMalloc(&s->pinned, 256, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&s->hflags, 4 * sizeof(uint64_t) * mdx_system::kRing, hipHostMallocDefault) != hipSuccess)
         return bail(fail(nullptr, MDX_ERR_ALLOC, "hipHostMalloc failed"));
+    for (int k = 0; k < mdx_system::kRing; k++)
+        if (hipEventCreateWithFlags(&s->ev[k], hipEventDisableTiming) != hipSuccess)
+            return bail(fail(nullptr, MDX_ERR_HIP, "hipEventCreate failed"));
     setup_tables(s);
     hipMemsetAsync(s->dint, 0, 8 * sizeof(int32_t), s->stream);
     hipMemsetAsync(s->dbits, 0, 4 * sizeof(uint64_t), s->stream);
@@ -619,7 +695,10 @@ int mdx_destroy(mdx_system* s) {
     for (auto& p : s->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
     for (auto e : s->free_events) hipEventDestroy(e);
     for (void* p : s->allocs) hipFree(p);
+    for (auto e : s->ev)
+        if (e) hipEventDestroy(e);
     if (s->pinned) hipHostFree(s->pinned);
+    if (s->hflags) hipHostFree(s->hflags);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
     return MDX_OK;

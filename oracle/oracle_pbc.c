@@ -189,10 +189,13 @@ int32_t orc_pbc_lj_run(double* r, double* v, int32_t n, double L, double rc, dou
     for (size_t q = 0; q < m; q++) a[q] = f[q];
     if (energies) { energies[0] = u; energies[1] = kinetic(v, n); energies[2] = u + energies[1]; }
     for (int32_t t = 1; t <= nsteps; t++) {
+        orc_gear_predict(r, v, a, b, c, dt, n);
+        /* the list must hold every pair within rc at the positions the force
+         * uses: rebuild when any atom moved skin/2 or more since the build */
         int need = rebuild_every_step;
         for (int32_t i = 0; i < n && !need; i++) {
             double dx = r[3 * i] - r0[3 * i], dy = r[3 * i + 1] - r0[3 * i + 1], dz = r[3 * i + 2] - r0[3 * i + 2];
-            if (dx * dx + dy * dy + dz * dz > half_skin2) need = 1;
+            if (dx * dx + dy * dy + dz * dz >= half_skin2) need = 1;
         }
         if (need) {
             free(list);
@@ -201,7 +204,6 @@ int32_t orc_pbc_lj_run(double* r, double* v, int32_t n, double L, double rc, dou
             memcpy(r0, r, m * sizeof(double));
             builds++;
         }
-        orc_gear_predict(r, v, a, b, c, dt, n);
         u = orc_pbc_lj_forces(r, f, n, L, 1.0, 1.0, rc, nls, list, NULL);
         orc_gear_correct(r, v, f, a, b, c, dt, n, 1.0, 1.0);
         if (energies) {

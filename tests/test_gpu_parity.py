@@ -238,6 +238,23 @@ def test_periodic_list_and_forces(fluid, precision):
         assert e == pytest.approx(eo, rel=1e-6)
 
 
+def test_periodic_fused_steps_bit_exact(fluid):
+    """The fused step kernel (FP64) walks the list in the oracle's candidate
+    order and performs the oracle's operations: a few steps without a rebuild
+    give bit-identical states."""
+    pos, vel, L = fluid
+    n, steps = len(pos), 4
+    ro, vo, _, builds, _ = O.pbc_lj_run(pos, vel, L, 2.5, 2.8, 0.005, steps)
+    assert builds == 1
+    with periodic(n, L, PREC_FP64) as s:
+        s.set_state(np.c_[pos, np.zeros(n)], vel)
+        s.run(steps)
+        st = s.get_state()
+        assert s.build_count() == 1
+    np.testing.assert_array_equal(st["pos"][:, :3], ro)
+    np.testing.assert_array_equal(st["vel"], vo)
+
+
 def test_periodic_run_matches_oracle(fluid):
     pos, vel, L = fluid
     n, steps = len(pos), 200
