@@ -545,7 +545,7 @@ double bits_to_double(uint64_t b) {
 // rebuild decision runs one step behind the device: before step k is
 // launched the host reads the flags of the positions r(k-1) (largest
 // displacement since the build D and largest single-step move S) and
-// rebuilds when D + 2 S reaches the margin (rs - rc)/2; every step kernel
+// rebuilds when D + 2 S reaches 0.8 (rs - rc)/2; every step kernel
 // re-checks its own positions against the margin and counts violations, which
 // turn into MDX_ERR_LIST_STALE.
 int run_periodic(mdx_system* s, int32_t nsteps) {
@@ -583,11 +583,14 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
         const uint64_t* h = host_flags(fk);
         if (h[2] != 0) return fail(s, MDX_ERR_LIST_STALE, "an atom moved past the neighbour-list margin");
         const double D = std::sqrt(bits_to_double(h[0])), S = std::sqrt(bits_to_double(h[1]));
+        // rebuild at 80% of the margin so the one-step-late estimate keeps slack
+        // before the hard check (the step kernel's stale count) could trip
+        const double trigger = 0.8 * margin;
         bool rebuild = false;
         if (k == 0)
-            rebuild = D >= margin;                 // exact: these flags are of r(0)
+            rebuild = D >= trigger;                // exact: these flags are of r(0)
         else if (fk >= last_build)
-            rebuild = D + 2.0 * S >= margin;       // r(k) is at most about one more step away
+            rebuild = D + 2.0 * S >= trigger;      // r(k) is at most about one more step away
         // (fk < last_build: the flags predate the last build, r(k) is one step from it)
         if (rebuild) {
             if ((rc = rebuild_periodic(s))) return rc;

@@ -211,7 +211,7 @@ def periodic(n, L, precision, rc=2.5, rs=2.8, dt=0.005):
 def fluid():
     pos, vel, L = initial.lj_fluid(10, 0.8442, 0.728, seed=3)
     rng = np.random.default_rng(9)
-    pos = np.mod(pos + rng.normal(0, 0.08, pos.shape), L)
+    pos = np.mod(pos + rng.normal(0, 0.02, pos.shape), L)
     return pos, vel, L
 
 
