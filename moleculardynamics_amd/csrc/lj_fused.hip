@@ -3,8 +3,9 @@
 //  k_nlist (at each cell-list rebuild): for every atom, the candidates of the
 //    27-cell neighbourhood within the list radius rs, in the reference's
 //    candidate order (cells in (dx,dy,dz) lexicographic order, ascending slot
-//    within a cell).  Entries are slot | image << 27, image = 9(dx+1) +
-//    3(dy+1) + (dz+1) of the periodic shift; the list is stored
+//    within a cell).  Entries are slot | image << 27, image = 9(wx+1) +
+//    3(wy+1) + (wz+1) for the periodic wrap w = -1, 0, +1 of the neighbour
+//    cell per axis (shift w L); the list is stored
 //    column-major ([k][atom]) so a wave's loads of entry k are coalesced.
 //  k_step (every step): LJ force of atom i over its list (FP64 displacement,
 //    FP32 12-6 force law, FP64 accumulation), then the Gear corrector of step
@@ -36,18 +37,18 @@ k_nlist(const double* __restrict__ rx, const double* __restrict__ ry, const doub
     const int32_t cz = own % g.g[2], cy = (own / g.g[2]) % g.g[1], cx = own / (g.g[2] * g.g[1]);
     int32_t k = 0;
     for (int dx = -1; dx <= 1; dx++) {
-        int32_t x = cx + dx;
-        double sx = 0.0;
-        if (x < 0) { x += g.g[0]; sx = -g.L; } else if (x >= g.g[0]) { x -= g.g[0]; sx = g.L; }
+        int32_t x = cx + dx, wx = 0;   // w*: periodic wrap of the neighbour cell (-1, 0, +1)
+        if (x < 0) { x += g.g[0]; wx = -1; } else if (x >= g.g[0]) { x -= g.g[0]; wx = 1; }
+        const double sx = wx * g.L;
         for (int dy = -1; dy <= 1; dy++) {
-            int32_t y = cy + dy;
-            double sy = 0.0;
-            if (y < 0) { y += g.g[1]; sy = -g.L; } else if (y >= g.g[1]) { y -= g.g[1]; sy = g.L; }
+            int32_t y = cy + dy, wy = 0;
+            if (y < 0) { y += g.g[1]; wy = -1; } else if (y >= g.g[1]) { y -= g.g[1]; wy = 1; }
+            const double sy = wy * g.L;
             for (int dz = -1; dz <= 1; dz++) {
-                int32_t z = cz + dz;
-                double sz = 0.0;
-                if (z < 0) { z += g.g[2]; sz = -g.L; } else if (z >= g.g[2]) { z -= g.g[2]; sz = g.L; }
-                const uint32_t code = (uint32_t)(9 * (dx + 1) + 3 * (dy + 1) + (dz + 1));
+                int32_t z = cz + dz, wz = 0;
+                if (z < 0) { z += g.g[2]; wz = -1; } else if (z >= g.g[2]) { z -= g.g[2]; wz = 1; }
+                const double sz = wz * g.L;
+                const uint32_t code = (uint32_t)(9 * (wx + 1) + 3 * (wy + 1) + (wz + 1));
                 const int32_t c = (x * g.g[1] + y) * g.g[2] + z;
                 const int32_t pe = g.start[c + 1];
                 for (int32_t p = g.start[c]; p < pe; p++) {
