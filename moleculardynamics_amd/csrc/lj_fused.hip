@@ -92,7 +92,7 @@ template <bool kFp64, bool kFinal, bool kForceOut>
 __global__ void __launch_bounds__(kBlock)
 k_step(StepArgs a, int32_t n) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    double d2 = 0.0, s2 = 0.0;
+    double d2 = 0.0, mv2 = 0.0;  // displacement since the build, move in this step
     bool stale = false;
     if (i < n) {
         const double xi = a.st.r[0][i], yi = a.st.r[1][i], zi = a.st.r[2][i];
@@ -149,7 +149,7 @@ k_step(StepArgs a, int32_t n) {
                 b = b + a.gc.k[0] * c;
                 const double dd = rp - a.r0[d][i], ds = rp - rt[d];
                 d2 += dd * dd;
-                s2 += ds * ds;
+                mv2 += ds * ds;
                 r = rp;
             }
             a.rn[d][i] = r;
@@ -164,7 +164,7 @@ k_step(StepArgs a, int32_t n) {
         stale = e0 * e0 + e1 * e1 + e2 * e2 >= a.hard2;
     }
     if constexpr (!kFinal) {
-        unsigned long long bd = __double_as_longlong(d2), bs = __double_as_longlong(s2);
+        unsigned long long bd = __double_as_longlong(d2), bs = __double_as_longlong(mv2);
         for (int o = kWave / 2; o > 0; o >>= 1) {
             unsigned long long x = __shfl_xor(bd, o, kWave), y = __shfl_xor(bs, o, kWave);
             bd = x > bd ? x : bd;
