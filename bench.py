@@ -26,6 +26,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+DT = float(np.float32(0.005))  # DT is a float in the configuration (common/Global.h:79)
 # bytes per atom per step of the whole step, FP64 state: Gear state r,v,a,b,c
 # read+written by the integrator (240), force written by the force kernel and
 # read by the integrator (48), positions read by the force kernel (24)
@@ -54,7 +55,7 @@ def cpu_baseline(ncell: int, steps: int) -> dict:
     from moleculardynamics_amd import initial
 
     pos, vel, L = initial.lj_fluid(ncell, 0.8442, 0.728, seed=12345)
-    _, _, _, builds, elapsed = O.pbc_lj_run(pos, vel, L, 2.5, 2.8, 0.005, steps)
+    _, _, _, builds, elapsed = O.pbc_lj_run(pos, vel, L, 2.5, 2.8, DT, steps)
     return {
         "value": len(pos) * steps / elapsed,
         "unit": "atom-steps/s",

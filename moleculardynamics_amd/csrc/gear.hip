@@ -116,7 +116,7 @@ k_correct_predict(DevState st, int32_t n, GearCoeffs gc, MassTable m, const doub
 __global__ void __launch_bounds__(kBlock)
 k_predict_track(DevState st, int32_t n, GearCoeffs gc, const double* __restrict__ r0x,
                 const double* __restrict__ r0y, const double* __restrict__ r0z,
-                unsigned long long* __restrict__ flags) {
+                unsigned long long* __restrict__ flags, int32_t spread) {
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     double d2 = 0.0, s2 = 0.0;
     if (i < n) {
@@ -134,15 +134,26 @@ k_predict_track(DevState st, int32_t n, GearCoeffs gc, const double* __restrict_
             s2 += ds * ds;
         }
     }
-    unsigned long long bd = __double_as_longlong(d2), bs = __double_as_longlong(s2);
+    // wave shuffles, then LDS over the block; one atomic per block, spread
+    // over flag_spread() addresses so blocks do not serialise on one word
     for (int o = kWave / 2; o > 0; o >>= 1) {
-        unsigned long long x = __shfl_xor(bd, o, kWave), y = __shfl_xor(bs, o, kWave);
-        bd = x > bd ? x : bd;
-        bs = y > bs ? y : bs;
+        d2 = fmax(d2, __shfl_xor(d2, o, kWave));
+        s2 = fmax(s2, __shfl_xor(s2, o, kWave));
     }
+    __shared__ double red[kBlock / kWave][2];
     if ((threadIdx.x & (kWave - 1)) == 0) {
-        atomicMax(&flags[0], bd);
-        atomicMax(&flags[1], bs);
+        red[threadIdx.x / kWave][0] = d2;
+        red[threadIdx.x / kWave][1] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < kBlock / kWave; q++) {
+            d2 = fmax(d2, red[q][0]);
+            s2 = fmax(s2, red[q][1]);
+        }
+        unsigned long long* f = flags + 3 * (blockIdx.x % spread);
+        atomicMax(&f[0], (unsigned long long)__double_as_longlong(d2));
+        atomicMax(&f[1], (unsigned long long)__double_as_longlong(s2));
     }
 }
 
@@ -152,7 +163,7 @@ hipError_t launch_predict_track(hipStream_t s, const DevState& st, int32_t n, co
                                 double* const rref[3], uint64_t* flags) {
     if (n <= 0) return hipSuccess;
     k_predict_track<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(
-        st, n, gc, rref[0], rref[1], rref[2], reinterpret_cast<unsigned long long*>(flags));
+        st, n, gc, rref[0], rref[1], rref[2], reinterpret_cast<unsigned long long*>(flags), flag_spread());
     return hipGetLastError();
 }
 

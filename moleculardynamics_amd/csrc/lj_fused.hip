@@ -1,20 +1,28 @@
-// lj_fused.hip -- periodic-box performance path: the step in one kernel.
+// lj_fused.hip -- periodic-box performance path: LDS-staged neighbourhoods and
+// the whole MD step in one kernel.
 //
-//  k_nlist (at each cell-list rebuild): for every atom, the candidates of the
-//    27-cell neighbourhood within the list radius rs, in the reference's
-//    candidate order (cells in (dx,dy,dz) lexicographic order, ascending slot
-//    within a cell).  Entries are slot | image << 27, image = 9(wx+1) +
-//    3(wy+1) + (wz+1) for the periodic wrap w = -1, 0, +1 of the neighbour
-//    cell per axis (shift w L); the list is stored
-//    column-major ([k][atom]) so a wave's loads of entry k are coalesced.
-//  k_step (every step): LJ force of atom i over its list (FP64 displacement,
-//    FP32 12-6 force law, FP64 accumulation), then the Gear corrector of step
-//    t and the predictor of step t+1, all in registers.  Positions are
-//    double-buffered (read r(t) from A, write r(t+1) to B) so no thread
-//    reads a position another thread has already advanced; the force never
-//    goes to HBM unless asked for.
-// HBM per atom and step: Gear state r,v,a,b,c read and written (240 B), the
-// list-build reference position (24 B) and the list entries (4 B each).
+// Work unit: a "segment" = K consecutive cells of one z-column (x, y fixed).
+// Its atoms are one contiguous slot range (slots are cell-sorted, z fastest).
+// The segment's neighbourhood -- the 9 columns (x+dx, y+dy), dx, dy in
+// {-1,0,1}, each over z0-1 .. z1 -- is staged into LDS as FP64 positions
+// already moved to the periodic image seen from the segment, in the order
+// column (dx,dy lexicographic), then z, then slot; so every atom's 27-cell
+// candidate set is 9 contiguous LDS ranges, visited in the reference's
+// candidate order (cells in (dx,dy,dz) lexicographic order, ascending index
+// within a cell: common/NeighborList.cpp:99-136).
+//
+//  k_nlist_lds (at each rebuild): each atom keeps the LDS indices of its
+//    candidates within the list radius rs (16-bit entries, column-major
+//    [k][slot] so a wave's loads are coalesced).
+//  k_step_lds (every step): LJ force over the list (FP64 displacement, FP32
+//    12-6 force law in the mixed mode, FP64 accumulation), then the Gear
+//    corrector of step t and the predictor of step t+1, in registers.
+//    Positions are double-buffered: r(t) is read (and staged) from A and
+//    r(t+1) written to B, so no workgroup reads a position another one has
+//    already advanced.
+// Workgroup-to-segment mapping is XCD-aware: workgroups are dealt to the 8
+// XCDs round robin, so block b takes segment (b % 8) * per + b / 8 and each
+// XCD's L2 serves one contiguous slab of the box.
 #include "mdx_internal.h"
 
 namespace mdx {
@@ -22,92 +30,179 @@ namespace mdx {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kImageShift = 27;
-constexpr uint32_t kSlotMask = (1u << kImageShift) - 1;
-constexpr uint32_t kNoShift = 13;  // image code of (0,0,0)
+constexpr int kXcd = 8;
+constexpr int kLdsAtoms = 3072;   // staged atoms per segment (3 x 24 KiB of FP64)
+constexpr int kMaxK = 14;         // cells per segment
+constexpr int kSpread = 64;       // flag atomics are spread over this many addresses
+
+struct SegGrid {
+    int32_t n[3];        // cells per axis
+    int32_t nseg;        // segments per column
+    int32_t total;       // n[0] * n[1] * nseg
+    int32_t per_xcd;     // ceil(total / 8)
+    double L;
+    const int32_t* start;
+};
+
+// z range [z0, z1) of segment s of a column with nz cells
+__device__ inline void seg_z(int32_t s, int32_t nseg, int32_t nz, int32_t& z0, int32_t& z1) {
+    z0 = (int32_t)(((int64_t)s * nz) / nseg);
+    z1 = (int32_t)(((int64_t)(s + 1) * nz) / nseg);
+}
+
+__device__ inline int32_t logical_segment(const SegGrid& g) {
+    const int32_t b = blockIdx.x;
+    return (b % kXcd) * g.per_xcd + b / kXcd;
+}
+
+// LDS layout shared by both kernels
+struct Stage {
+    double x[kLdsAtoms], y[kLdsAtoms], z[kLdsAtoms];
+    int32_t colbase[9];                 // LDS start of each column
+    int32_t cellpre[9][kMaxK + 3];      // per column, prefix counts over the K+2 cells
+    int32_t total;
+    double red[kBlock / kWave][3];
+};
+
+// Stage the neighbourhood of segment (cx, cy, z0..z1).  Returns false (for
+// the whole block) when it does not fit in LDS.
+__device__ inline bool stage(Stage& S, const SegGrid& g, int32_t cx, int32_t cy, int32_t z0, int32_t z1,
+                             const double* __restrict__ rx, const double* __restrict__ ry,
+                             const double* __restrict__ rz) {
+    const int32_t K2 = z1 - z0 + 2;    // cells per column in the window
+    const int32_t tid = threadIdx.x;
+    if (tid < 9 * K2) {                // per-cell counts (9 columns x K2 cells)
+        const int32_t col = tid / K2, o = tid % K2;
+        int32_t x = cx + col / 3 - 1, y = cy + col % 3 - 1, z = z0 - 1 + o;
+        x = x < 0 ? x + g.n[0] : (x >= g.n[0] ? x - g.n[0] : x);
+        y = y < 0 ? y + g.n[1] : (y >= g.n[1] ? y - g.n[1] : y);
+        z = z < 0 ? z + g.n[2] : (z >= g.n[2] ? z - g.n[2] : z);
+        const int32_t c = (x * g.n[1] + y) * g.n[2] + z;
+        S.cellpre[col][o + 1] = g.start[c + 1] - g.start[c];
+    }
+    __syncthreads();
+    if (tid < 9) {
+        S.cellpre[tid][0] = 0;
+        for (int32_t o = 1; o <= K2; o++) S.cellpre[tid][o] += S.cellpre[tid][o - 1];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int32_t acc = 0;
+        for (int32_t col = 0; col < 9; col++) {
+            S.colbase[col] = acc;
+            acc += S.cellpre[col][K2];
+        }
+        S.total = acc;
+    }
+    __syncthreads();
+    if (S.total > kLdsAtoms) return false;
+    for (int32_t p = tid; p < S.total; p += kBlock) {
+        int32_t col = 0;
+        while (col < 8 && p >= S.colbase[col + 1]) col++;
+        const int32_t q = p - S.colbase[col];
+        int32_t o = 0;
+        while (S.cellpre[col][o + 1] <= q) o++;
+        int32_t x = cx + col / 3 - 1, y = cy + col % 3 - 1, z = z0 - 1 + o;
+        double sx = 0, sy = 0, sz = 0;
+        if (x < 0) { x += g.n[0]; sx = -g.L; } else if (x >= g.n[0]) { x -= g.n[0]; sx = g.L; }
+        if (y < 0) { y += g.n[1]; sy = -g.L; } else if (y >= g.n[1]) { y -= g.n[1]; sy = g.L; }
+        if (z < 0) { z += g.n[2]; sz = -g.L; } else if (z >= g.n[2]) { z -= g.n[2]; sz = g.L; }
+        const int32_t c = (x * g.n[1] + y) * g.n[2] + z;
+        const int32_t slot = g.start[c] + (q - S.cellpre[col][o]);
+        S.x[p] = rx[slot] + sx;
+        S.y[p] = ry[slot] + sy;
+        S.z[p] = rz[slot] + sz;
+    }
+    __syncthreads();
+    return true;
+}
 
 __global__ void __launch_bounds__(kBlock)
-k_nlist(const double* __restrict__ rx, const double* __restrict__ ry, const double* __restrict__ rz,
-        const int32_t* __restrict__ cell_of, GridView g, double rl2, int32_t n, int32_t cap,
-        uint32_t* __restrict__ list, int32_t* __restrict__ count, int32_t* __restrict__ overflow) {
-    int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const double xi = rx[s], yi = ry[s], zi = rz[s];
-    const int32_t own = cell_of[s];
-    const int32_t cz = own % g.g[2], cy = (own / g.g[2]) % g.g[1], cx = own / (g.g[2] * g.g[1]);
-    int32_t k = 0;
-    for (int dx = -1; dx <= 1; dx++) {
-        int32_t x = cx + dx, wx = 0;   // w*: periodic wrap of the neighbour cell (-1, 0, +1)
-        if (x < 0) { x += g.g[0]; wx = -1; } else if (x >= g.g[0]) { x -= g.g[0]; wx = 1; }
-        const double sx = wx * g.L;
-        for (int dy = -1; dy <= 1; dy++) {
-            int32_t y = cy + dy, wy = 0;
-            if (y < 0) { y += g.g[1]; wy = -1; } else if (y >= g.g[1]) { y -= g.g[1]; wy = 1; }
-            const double sy = wy * g.L;
-            for (int dz = -1; dz <= 1; dz++) {
-                int32_t z = cz + dz, wz = 0;
-                if (z < 0) { z += g.g[2]; wz = -1; } else if (z >= g.g[2]) { z -= g.g[2]; wz = 1; }
-                const double sz = wz * g.L;
-                const uint32_t code = (uint32_t)(9 * (wx + 1) + 3 * (wy + 1) + (wz + 1));
-                const int32_t c = (x * g.g[1] + y) * g.g[2] + z;
-                const int32_t pe = g.start[c + 1];
-                for (int32_t p = g.start[c]; p < pe; p++) {
-                    if (p == s) continue;
-                    const double ddx = (xi - rx[p]) - sx, ddy = (yi - ry[p]) - sy, ddz = (zi - rz[p]) - sz;
-                    const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    if (r2 >= rl2) continue;
-                    if (k < cap) list[(int64_t)k * n + s] = (uint32_t)p | (code << kImageShift);
-                    k++;
-                }
+k_nlist_lds(SegGrid g, const double* __restrict__ rx, const double* __restrict__ ry,
+            const double* __restrict__ rz, double rl2, int32_t n, int32_t cap, uint16_t* __restrict__ list,
+            int32_t* __restrict__ count, int32_t* __restrict__ err) {
+    __shared__ Stage S;
+    const int32_t seg = logical_segment(g);
+    if (seg >= g.total) return;
+    const int32_t col = seg / g.nseg, szi = seg % g.nseg;
+    const int32_t cx = col / g.n[1], cy = col % g.n[1];
+    int32_t z0, z1;
+    seg_z(szi, g.nseg, g.n[2], z0, z1);
+    if (!stage(S, g, cx, cy, z0, z1, rx, ry, rz)) {
+        if (threadIdx.x == 0) atomicMax(&err[0], S.total);
+        return;
+    }
+    const int32_t cbase = (cx * g.n[1] + cy) * g.n[2];
+    const int32_t hbeg = g.start[cbase + z0], hend = g.start[cbase + z1];
+    for (int32_t s = hbeg + threadIdx.x; s < hend; s += kBlock) {
+        // window cell o (1..K) of the atom and its own LDS index
+        const int32_t qc = s - hbeg;
+        int32_t o = 1;
+        while (S.cellpre[4][o + 1] - S.cellpre[4][1] <= qc) o++;
+        const int32_t self = S.colbase[4] + S.cellpre[4][1] + qc;
+        const double xi = S.x[self], yi = S.y[self], zi = S.z[self];
+        int32_t k = 0;
+        for (int32_t c = 0; c < 9; c++) {
+            const int32_t b = S.colbase[c] + S.cellpre[c][o - 1], e = S.colbase[c] + S.cellpre[c][o + 2];
+            for (int32_t p = b; p < e; p++) {
+                if (p == self) continue;
+                const double dx = xi - S.x[p], dy = yi - S.y[p], dz = zi - S.z[p];
+                if (dx * dx + dy * dy + dz * dz >= rl2) continue;
+                if (k < cap) list[(int64_t)k * n + s] = (uint16_t)p;
+                k++;
             }
         }
+        count[s] = k < cap ? k : cap;
+        if (k > cap) atomicMax(&err[1], k);
     }
-    count[s] = k < cap ? k : cap;
-    if (k > cap) atomicMax(overflow, k);
 }
 
 struct StepArgs {
+    SegGrid g;
     DevState st;            // st.r = positions r(t) (read), Gear state v,a,b,c (read/write)
     double* rn[3];          // r(t+1) out
     const double* r0[3];    // positions at the last list build
-    const uint32_t* list;
+    const uint16_t* list;
     const int32_t* count;
-    double L;
     double rc2;
     float inv_sig2f, coeff_f;
     double sig2, coeff;
     double mass, scale;
     GearCoeffs gc;
-    double hard2;           // list validity: displacement^2 must stay below this
-    unsigned long long* flags;  // [0] max disp^2 (bits), [1] max step^2 (bits), [2] stale count
+    double hard2;                // list validity: displacement^2 must stay below this
+    unsigned long long* flags;   // kSpread x {max disp^2 bits, max step^2 bits, stale count}
+    int32_t n;
 };
-
-__device__ inline double image(uint32_t code, int axis, double L) {
-    // code = 9 ix + 3 iy + iz with i* in {0,1,2}; shift = (i* - 1) L
-    const uint32_t q = axis == 0 ? code / 9 : (axis == 1 ? (code / 3) % 3 : code % 3);
-    return ((double)q - 1.0) * L;
-}
 
 template <bool kFp64, bool kFinal, bool kForceOut>
 __global__ void __launch_bounds__(kBlock)
-k_step(StepArgs a, int32_t n) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    double d2 = 0.0, mv2 = 0.0;  // displacement since the build, move in this step
-    bool stale = false;
-    if (i < n) {
+k_step_lds(StepArgs a) {
+    __shared__ Stage S;
+    const SegGrid& g = a.g;
+    const int32_t seg = logical_segment(g);
+    if (seg >= g.total) return;
+    const int32_t col = seg / g.nseg, szi = seg % g.nseg;
+    const int32_t cx = col / g.n[1], cy = col % g.n[1];
+    int32_t z0, z1;
+    seg_z(szi, g.nseg, g.n[2], z0, z1);
+    unsigned long long* fl = a.flags + 3 * (blockIdx.x % kSpread);
+    if (!stage(S, g, cx, cy, z0, z1, a.st.r[0], a.st.r[1], a.st.r[2])) {
+        // cannot happen after a successful list build on the same cells; report it as stale
+        if (threadIdx.x == 0) atomicAdd(&fl[2], 1ull << 32);
+        return;
+    }
+    const int32_t cbase = (cx * g.n[1] + cy) * g.n[2];
+    const int32_t hbeg = g.start[cbase + z0], hend = g.start[cbase + z1];
+    double d2max = 0.0, mv2max = 0.0, stale = 0.0;
+    const int32_t n = a.n;
+    for (int32_t i = hbeg + threadIdx.x; i < hend; i += kBlock) {
         const double xi = a.st.r[0][i], yi = a.st.r[1][i], zi = a.st.r[2][i];
         double fx = 0, fy = 0, fz = 0;
         const int32_t cnt = a.count[i];
-        const uint32_t* e = a.list + i;
+        const uint16_t* e = a.list + i;
         for (int32_t k = 0; k < cnt; k++) {
-            const uint32_t ent = e[(int64_t)k * n];
-            const uint32_t j = ent & kSlotMask, code = ent >> kImageShift;
-            double ddx = xi - a.st.r[0][j], ddy = yi - a.st.r[1][j], ddz = zi - a.st.r[2][j];
-            if (code != kNoShift) {
-                ddx -= image(code, 0, a.L);
-                ddy -= image(code, 1, a.L);
-                ddz -= image(code, 2, a.L);
-            }
+            const int32_t j = e[(int64_t)k * n];
+            const double ddx = xi - S.x[j], ddy = yi - S.y[j], ddz = zi - S.z[j];
             const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
             if (r2 >= a.rc2) continue;
             if constexpr (kFp64) {
@@ -117,12 +212,12 @@ k_step(StepArgs a, int32_t n) {
                 const double fr = (2 * iq7 - iq4) * a.coeff;
                 fx += ddx * fr; fy += ddy * fr; fz += ddz * fr;
             } else {
-                // s2 = (sig/r)^2; F/r = coeff (sig/r)^8 (2 (sig/r)^6 - 1)
-                const float s2 = __builtin_amdgcn_rcpf((float)r2 * a.inv_sig2f);
-                const float s4 = s2 * s2;
-                const float s6 = s4 * s2;
-                const float s8 = s4 * s4;
-                const double fr = (double)(s8 * __builtin_fmaf(2.0f, s6, -1.0f) * a.coeff_f);
+                // u = (sig/r)^2; F/r = coeff u^4 (2 u^3 - 1)
+                const float u = __builtin_amdgcn_rcpf((float)r2 * a.inv_sig2f);
+                const float u2 = u * u;
+                const float u3 = u2 * u;
+                const float u4 = u2 * u2;
+                const double fr = (double)(u4 * __builtin_fmaf(2.0f, u3, -1.0f) * a.coeff_f);
                 fx = __builtin_fma(fr, ddx, fx);
                 fy = __builtin_fma(fr, ddy, fy);
                 fz = __builtin_fma(fr, ddz, fz);
@@ -130,6 +225,7 @@ k_step(StepArgs a, int32_t n) {
         }
         const double f[3] = {fx, fy, fz};
         const double rt[3] = {xi, yi, zi};
+        double d2 = 0.0, mv2 = 0.0;
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             double r = rt[d], v = a.st.v[d][i], ac = a.st.a[d][i], b = a.st.b[d][i], c = a.st.c[d][i];
@@ -159,47 +255,82 @@ k_step(StepArgs a, int32_t n) {
             a.st.c[d][i] = c;
             if constexpr (kForceOut) a.st.f[d][i] = f[d];
         }
-        // the force just used was valid only if r(t) was within the list margin
+        // the force just used is valid only if r(t) was within the list margin
         const double e0 = xi - a.r0[0][i], e1 = yi - a.r0[1][i], e2 = zi - a.r0[2][i];
-        stale = e0 * e0 + e1 * e1 + e2 * e2 >= a.hard2;
+        if (e0 * e0 + e1 * e1 + e2 * e2 >= a.hard2) stale += 1.0;
+        d2max = fmax(d2max, d2);
+        mv2max = fmax(mv2max, mv2);
     }
-    if constexpr (!kFinal) {
-        unsigned long long bd = __double_as_longlong(d2), bs = __double_as_longlong(mv2);
-        for (int o = kWave / 2; o > 0; o >>= 1) {
-            unsigned long long x = __shfl_xor(bd, o, kWave), y = __shfl_xor(bs, o, kWave);
-            bd = x > bd ? x : bd;
-            bs = y > bs ? y : bs;
-        }
-        if ((threadIdx.x & (kWave - 1)) == 0) {
-            atomicMax(&a.flags[0], bd);
-            atomicMax(&a.flags[1], bs);
-        }
+    // block reduction (wave shuffles, then LDS), one spread atomic per block
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        d2max = fmax(d2max, __shfl_xor(d2max, o, kWave));
+        mv2max = fmax(mv2max, __shfl_xor(mv2max, o, kWave));
+        stale += __shfl_xor(stale, o, kWave);
     }
-    if (__any(stale) && (threadIdx.x & (kWave - 1)) == 0) atomicAdd(&a.flags[2], 1ull);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        S.red[w][0] = d2max;
+        S.red[w][1] = mv2max;
+        S.red[w][2] = stale;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < kBlock / kWave; q++) {
+            d2max = fmax(d2max, S.red[q][0]);
+            mv2max = fmax(mv2max, S.red[q][1]);
+            stale += S.red[q][2];
+        }
+        if (!kFinal) {
+            atomicMax(&fl[0], (unsigned long long)__double_as_longlong(d2max));
+            atomicMax(&fl[1], (unsigned long long)__double_as_longlong(mv2max));
+        }
+        if (stale > 0) atomicAdd(&fl[2], (unsigned long long)stale);
+    }
+}
+
+SegGrid make_grid(const GridView& v, int32_t nseg) {
+    SegGrid g;
+    for (int d = 0; d < 3; d++) g.n[d] = v.g[d];
+    g.nseg = nseg;
+    g.total = v.g[0] * v.g[1] * nseg;
+    g.per_xcd = (g.total + kXcd - 1) / kXcd;
+    g.L = v.L;
+    g.start = v.start;
+    return g;
 }
 
 }  // namespace
 
-hipError_t launch_nlist(hipStream_t s, const DevState& st, const int32_t* cell_of, const GridView& g,
-                        double rl2, int32_t n, int32_t cap, uint32_t* list, int32_t* count,
-                        int32_t* overflow) {
+int32_t lds_segments_per_column(int32_t nz, double atoms_per_cell, int32_t kmax) {
+    // about 200 home atoms per workgroup, at most kmax cells per segment
+    int32_t k = (int32_t)(200.0 / (atoms_per_cell > 0 ? atoms_per_cell : 1.0));
+    kmax = kmax < kMaxK ? kmax : kMaxK;
+    k = k < 1 ? 1 : (k > kmax ? kmax : k);
+    return (nz + k - 1) / k;
+}
+
+int32_t lds_capacity() { return kLdsAtoms; }
+int32_t flag_spread() { return kSpread; }
+
+hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg, double rl2,
+                            int32_t n, int32_t cap, uint16_t* list, int32_t* count, int32_t* err) {
     if (n <= 0) return hipSuccess;
-    k_nlist<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(st.r[0], st.r[1], st.r[2], cell_of, g, rl2, n, cap,
-                                                        list, count, overflow);
+    SegGrid g = make_grid(v, nseg);
+    k_nlist_lds<<<g.per_xcd * kXcd, kBlock, 0, s>>>(g, st.r[0], st.r[1], st.r[2], rl2, n, cap, list, count, err);
     return hipGetLastError();
 }
 
-hipError_t launch_step(hipStream_t s, const DevState& st, double* const rn[3], double* const r0[3],
-                       const uint32_t* list, const int32_t* count, const LjTable& t, const MassTable& m,
-                       const GearCoeffs& gc, double L, double hard2, int32_t n, int32_t precision,
-                       bool final_step, bool force_out, uint64_t* flags) {
+hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg,
+                           double* const rn[3], double* const r0[3], const uint16_t* list, const int32_t* count,
+                           const LjTable& t, const MassTable& m, const GearCoeffs& gc, double hard2, int32_t n,
+                           int32_t precision, bool final_step, bool force_out, uint64_t* flags) {
     if (n <= 0) return hipSuccess;
     StepArgs a;
+    a.g = make_grid(v, nseg);
     a.st = st;
     for (int d = 0; d < 3; d++) { a.rn[d] = rn[d]; a.r0[d] = r0[d]; }
     a.list = list;
     a.count = count;
-    a.L = L;
     a.rc2 = t.rc2;
     a.inv_sig2f = t.inv_sig2f[0];
     a.coeff_f = t.coeff_f[0];
@@ -210,9 +341,10 @@ hipError_t launch_step(hipStream_t s, const DevState& st, double* const rn[3], d
     a.gc = gc;
     a.hard2 = hard2;
     a.flags = reinterpret_cast<unsigned long long*>(flags);
-    dim3 grid((n + kBlock - 1) / kBlock);
+    a.n = n;
+    dim3 grid(a.g.per_xcd * kXcd);
     const bool fp64 = precision == MDX_PREC_FP64;
-#define MDX_STEP(F64, FIN, FO) k_step<F64, FIN, FO><<<grid, kBlock, 0, s>>>(a, n)
+#define MDX_STEP(F64, FIN, FO) k_step_lds<F64, FIN, FO><<<grid, kBlock, 0, s>>>(a)
     if (fp64) {
         if (final_step) { if (force_out) MDX_STEP(true, true, true); else MDX_STEP(true, true, false); }
         else { if (force_out) MDX_STEP(true, false, true); else MDX_STEP(true, false, false); }

@@ -128,19 +128,23 @@ hipError_t launch_accelerations(hipStream_t s, const DevState& st, int32_t n, co
 hipError_t launch_correct_predict(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
                                   const MassTable& m, double* const rref[3], uint32_t* maxdisp2_bits);
 
-// predictor that also max-reduces |r - rref|^2 into flags[0] and the squared
-// step |r_new - r_old|^2 into flags[1] (bits of non-negative doubles)
+// predictor that also max-reduces |r - rref|^2 and the squared step
+// |r_new - r_old|^2 (bits of non-negative doubles) into the spread flag
+// layout of the step kernel: flags[3 q + {0, 1}], q = block % flag_spread()
 hipError_t launch_predict_track(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
                                 double* const rref[3], uint64_t* flags);
 
-// lj_fused.hip
-hipError_t launch_nlist(hipStream_t s, const DevState& st, const int32_t* cell_of, const GridView& g,
-                        double rl2, int32_t n, int32_t cap, uint32_t* list, int32_t* count,
-                        int32_t* overflow);
-hipError_t launch_step(hipStream_t s, const DevState& st, double* const rn[3], double* const r0[3],
-                       const uint32_t* list, const int32_t* count, const LjTable& t, const MassTable& m,
-                       const GearCoeffs& gc, double L, double hard2, int32_t n, int32_t precision,
-                       bool final_step, bool force_out, uint64_t* flags);
+// lj_fused.hip: LDS-staged segment kernels of the periodic box
+int32_t lds_segments_per_column(int32_t nz, double atoms_per_cell, int32_t kmax);
+int32_t lds_capacity();
+int32_t flag_spread();
+// err[0]: staged atoms of a segment exceeded lds_capacity(); err[1]: longest list above cap
+hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg, double rl2,
+                            int32_t n, int32_t cap, uint16_t* list, int32_t* count, int32_t* err);
+hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg,
+                           double* const rn[3], double* const r0[3], const uint16_t* list, const int32_t* count,
+                           const LjTable& t, const MassTable& m, const GearCoeffs& gc, double hard2, int32_t n,
+                           int32_t precision, bool final_step, bool force_out, uint64_t* flags);
 
 // bond_ref.hip
 hipError_t launch_bond_ref(hipStream_t s, const DevState& st, const int32_t* slot_of_id,

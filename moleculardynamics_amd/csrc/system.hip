@@ -79,11 +79,14 @@ struct mdx_system {
     double disp_limit2 = 0;          // periodic: rebuild at max displacement^2 >= this
     // periodic fused path
     double* rnext[3] = {nullptr, nullptr, nullptr};  // r(t+1) written by the step kernel
-    uint32_t* nlist = nullptr;       // column-major neighbour list
+    uint16_t* nlist = nullptr;       // column-major neighbour list of LDS indices
     int32_t* ncount = nullptr;
     int32_t nl_cap = 0;
+    int32_t nseg = 0;                // segments per cell column
+    int32_t seg_kmax = 1 << 20;      // cap on cells per segment (lowered if staging overflows)
     static constexpr int kRing = 3;
-    uint64_t* dflags = nullptr;      // kRing x 4: max disp^2, max step^2, stale count, spare
+    static constexpr int kFlagWords = 3 * 64;  // flag_spread() x {disp^2, step^2, stale}
+    uint64_t* dflags = nullptr;      // kRing x kFlagWords
     uint64_t* hflags = nullptr;      // pinned copy of dflags
     hipEvent_t ev[kRing] = {nullptr, nullptr, nullptr};
     bool timing = false;
@@ -502,13 +505,15 @@ int run_walled(mdx_system* s, int32_t nsteps) {
     return MDX_OK;
 }
 
-// Periodic box, cell grid + neighbour list of radius rs (rebuild of both).
+// Periodic box, cell grid + LDS-index neighbour lists of radius rs.
 int rebuild_periodic(mdx_system* s) {
     const mdx_config& c = s->cfg;
     int rc = build_list(s, MDX_LIST_LJ, 0.0, false);
     if (rc) return rc;
     const int n = s->n;
     const double rl2 = (double)c.lj_rs * (double)c.lj_rs;
+    const GridView& v = s->lj.view;
+    const double per_cell = (double)n / ((double)v.g[0] * v.g[1] * v.g[2]);
     if (!s->nlist) {
         // first guess from the density: 1.5 x the mean count inside rs, plus slack
         double rho = n / (c.box_length * c.box_length * c.box_length);
@@ -518,16 +523,24 @@ int rebuild_periodic(mdx_system* s) {
         s->nl_cap = cap;
     }
     for (;;) {
-        HIPCHK(s, hipMemsetAsync(s->dint + 4, 0, sizeof(int32_t), s->stream));
-        LAUNCH(s, "nlist_build", launch_nlist(s->stream, s->st, s->cell_of, s->lj.view, rl2, n, s->nl_cap,
-                                              s->nlist, s->ncount, s->dint + 4));
-        int32_t over;
-        if ((rc = read_ints(s, &over, s->dint + 4, 1))) return rc;
-        if (over == 0) break;
-        // capacity exceeded: grow and build again
+        s->nseg = lds_segments_per_column(v.g[2], per_cell, s->seg_kmax);
+        HIPCHK(s, hipMemsetAsync(s->dint + 4, 0, 2 * sizeof(int32_t), s->stream));
+        LAUNCH(s, "nlist_build", launch_nlist_lds(s->stream, s->st, v, s->nseg, rl2, n, s->nl_cap, s->nlist,
+                                                  s->ncount, s->dint + 4));
+        int32_t e[2];
+        if ((rc = read_ints(s, e, s->dint + 4, 2))) return rc;
+        if (e[0] > 0) {
+            // a segment's neighbourhood did not fit in LDS: shorter segments
+            int32_t k = (v.g[2] + s->nseg - 1) / s->nseg;
+            if (k <= 1) return fail(s, MDX_ERR_CAPACITY, "cell neighbourhood exceeds the LDS staging capacity");
+            s->seg_kmax = k - 1;
+            continue;
+        }
+        if (e[1] == 0) break;
+        // a neighbour list exceeded its capacity: grow and build again
         dfree(s, s->nlist);
         s->nlist = nullptr;
-        int32_t cap = (over + over / 4 + 31) / 32 * 32;
+        int32_t cap = (e[1] + e[1] / 4 + 31) / 32 * 32;
         if ((rc = dalloc(s, &s->nlist, (size_t)cap * n))) return rc;
         s->nl_cap = cap;
     }
@@ -563,15 +576,28 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
     }
     if (nsteps <= 0) return MDX_OK;
     const int R = mdx_system::kRing;
-    auto slot_flags = [&](int k) { return s->dflags + 4 * (size_t)(((k % R) + R) % R); };
-    auto host_flags = [&](int k) { return s->hflags + 4 * (size_t)(((k % R) + R) % R); };
+    const int W = mdx_system::kFlagWords;
+    auto slot_flags = [&](int k) { return s->dflags + W * (size_t)(((k % R) + R) % R); };
+    auto host_flags = [&](int k) { return s->hflags + W * (size_t)(((k % R) + R) % R); };
+    // reduce the spread flags: max displacement^2, max step^2, stale count
+    auto reduce = [&](const uint64_t* h, double& D, double& S, uint64_t& stale) {
+        double d2 = 0, s2 = 0;
+        stale = 0;
+        for (int q = 0; q < W / 3; q++) {
+            d2 = std::max(d2, bits_to_double(h[3 * q]));
+            s2 = std::max(s2, bits_to_double(h[3 * q + 1]));
+            stale += h[3 * q + 2];
+        }
+        D = std::sqrt(d2);
+        S = std::sqrt(s2);
+    };
     auto post = [&](int k) -> int {  // copy step flags to the host and mark them
-        HIPCHK(s, hipMemcpyAsync(host_flags(k), slot_flags(k), 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(s, hipMemcpyAsync(host_flags(k), slot_flags(k), W * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
         HIPCHK(s, hipEventRecord(s->ev[((k % R) + R) % R], st));
         return MDX_OK;
     };
     // flags of r(0) come from the first predictor (index -1)
-    HIPCHK(s, hipMemsetAsync(slot_flags(-1), 0, 4 * sizeof(uint64_t), st));
+    HIPCHK(s, hipMemsetAsync(slot_flags(-1), 0, W * sizeof(uint64_t), st));
     LAUNCH(s, "predict", launch_predict_track(st, s->st, n, s->gc, s->rref, slot_flags(-1)));
     if ((rc = post(-1))) return rc;
     int last_build = -1;   // step index whose positions were used for the last build
@@ -580,9 +606,10 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
         // uses r(k); the host reads index k-2, i.e. r(k-1), while launch k-1 runs.
         const int fk = k >= 1 ? k - 2 : -1;
         HIPCHK(s, hipEventSynchronize(s->ev[((fk % R) + R) % R]));
-        const uint64_t* h = host_flags(fk);
-        if (h[2] != 0) return fail(s, MDX_ERR_LIST_STALE, "an atom moved past the neighbour-list margin");
-        const double D = std::sqrt(bits_to_double(h[0])), S = std::sqrt(bits_to_double(h[1]));
+        double D, S;
+        uint64_t stale;
+        reduce(host_flags(fk), D, S, stale);
+        if (stale != 0) return fail(s, MDX_ERR_LIST_STALE, "an atom moved past the neighbour-list margin");
         // rebuild at 80% of the margin so the one-step-late estimate keeps slack
         // before the hard check (the step kernel's stale count) could trip
         const double trigger = 0.8 * margin;
@@ -597,17 +624,21 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
             last_build = k;
         }
         const bool fin = k + 1 == nsteps;
-        HIPCHK(s, hipMemsetAsync(slot_flags(k), 0, 4 * sizeof(uint64_t), st));
-        LAUNCH(s, "md_step", launch_step(st, s->st, s->rnext, s->rref, s->nlist, s->ncount, s->ljt, s->mt, s->gc,
-                                         c.box_length, hard2, n, c.precision, fin, fin, slot_flags(k)));
+        HIPCHK(s, hipMemsetAsync(slot_flags(k), 0, W * sizeof(uint64_t), st));
+        LAUNCH(s, "md_step", launch_step_lds(st, s->st, s->lj.view, s->nseg, s->rnext, s->rref, s->nlist, s->ncount,
+                                             s->ljt, s->mt, s->gc, hard2, n, c.precision, fin, fin, slot_flags(k)));
         if ((rc = post(k))) return rc;
         for (int d = 0; d < 3; d++) std::swap(s->st.r[d], s->rnext[d]);
         s->t++;
     }
     // stale check of the last steps
     HIPCHK(s, hipStreamSynchronize(st));
-    for (int k = std::max(0, nsteps - R); k < nsteps; k++)
-        if (host_flags(k)[2] != 0) return fail(s, MDX_ERR_LIST_STALE, "an atom moved past the neighbour-list margin");
+    for (int k = std::max(0, nsteps - R); k < nsteps; k++) {
+        double D, S;
+        uint64_t stale;
+        reduce(host_flags(k), D, S, stale);
+        if (stale != 0) return fail(s, MDX_ERR_LIST_STALE, "an atom moved past the neighbour-list margin");
+    }
     return MDX_OK;
 }
 
@@ -671,7 +702,7 @@ int mdx_create(const mdx_config* cfg, mdx_system** out) {
     }
     s->nb = kinetics_blocks(n);
     if ((rc = dalloc(s, &s->partials, 8 * (size_t)s->nb + 16)) ||
-        (rc = dalloc(s, &s->dflags, 4 * (size_t)mdx_system::kRing))) {
+        (rc = dalloc(s, &s->dflags, (size_t)mdx_system::kFlagWords * mdx_system::kRing))) {
         g_err = s->err;
         return bail(rc);
     }
@@ -679,7 +710,8 @@ int mdx_create(const mdx_config* cfg, mdx_system** out) {
         for (int d = 0; d < 3; d++)
             if ((rc = dalloc(s, &s->rnext[d], n))) { g_err = s->err; return bail(rc); }
     if (hipHostMalloc(&s->pinned, 256, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&s->hflags, 4 * sizeof(uint64_t) * mdx_system::kRing, hipHostMallocDefault) != hipSuccess)
+        hipHostMalloc((void**)&s->hflags, sizeof(uint64_t) * mdx_system::kFlagWords * mdx_system::kRing,
+                      hipHostMallocDefault) != hipSuccess)
         return bail(fail(nullptr, MDX_ERR_ALLOC, "hipHostMalloc failed"));
     for (int k = 0; k < mdx_system::kRing; k++)
         if (hipEventCreateWithFlags(&s->ev[k], hipEventDisableTiming) != hipSuccess)
@@ -901,12 +933,13 @@ int mdx_potential(mdx_system* s, double* out2) {
     if (!s || !out2) return fail(s, MDX_ERR_ARG, "null argument");
     out2[0] = out2[1] = 0;
     // forces of this evaluation go to scratch so the stored force is untouched
-    DevState scratch = s->st;
-    for (int d = 0; d < 3; d++) scratch.f[d] = s->alt.f[d];
     int rc;
     hipStream_t st = s->stream;
+    // (a build permutes the state, so it must precede taking the state pointers)
     if (s->cfg.box == MDX_BOX_PERIODIC && !s->lj.built && (rc = build_list(s, MDX_LIST_LJ, 0.0, false)))
         return rc;
+    DevState scratch = s->st;
+    for (int d = 0; d < 3; d++) scratch.f[d] = s->alt.f[d];
     if (s->cfg.use_lj) {
         if ((rc = lj_forces(s, s->e_atom, nullptr, &scratch))) return rc;
         LAUNCH(s, "energy_sum", launch_sum(st, s->e_atom, s->n, s->partials, s->nb, s->dscal + 8));
