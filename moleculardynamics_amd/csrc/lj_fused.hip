@@ -31,7 +31,8 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kXcd = 8;
-constexpr int kLdsAtoms = 3072;   // staged atoms per segment (3 x 24 KiB of FP64)
+constexpr int kLdsAtoms = 3072;   // staged atoms per segment at most (3 x 24 KiB of FP64)
+constexpr int kBatch = 8;         // list entries loaded ahead per thread
 constexpr int kMaxK = 14;         // cells per segment
 constexpr int kSpread = 64;       // flag atomics are spread over this many addresses
 
@@ -55,9 +56,16 @@ __device__ inline int32_t logical_segment(const SegGrid& g) {
     return (b % kXcd) * g.per_xcd + b / kXcd;
 }
 
-// LDS layout shared by both kernels
+// LDS header shared by both kernels; the staged FP64 positions x, y, z of
+// `cap` atoms each live in a separate LDS array (Pos), addressed through
+// pointers derived from the __shared__ array itself so accesses stay ds_*.
+struct Pos {
+    double* x;
+    double* y;
+    double* z;
+    int32_t cap;
+};
 struct Stage {
-    double x[kLdsAtoms], y[kLdsAtoms], z[kLdsAtoms];
     int32_t colbase[9];                 // LDS start of each column
     int32_t cellpre[9][kMaxK + 3];      // per column, prefix counts over the K+2 cells
     int32_t total;
@@ -66,8 +74,8 @@ struct Stage {
 
 // Stage the neighbourhood of segment (cx, cy, z0..z1).  Returns false (for
 // the whole block) when it does not fit in LDS.
-__device__ inline bool stage(Stage& S, const SegGrid& g, int32_t cx, int32_t cy, int32_t z0, int32_t z1,
-                             const double* __restrict__ rx, const double* __restrict__ ry,
+__device__ inline bool stage(Stage& S, const Pos& P, const SegGrid& g, int32_t cx, int32_t cy, int32_t z0,
+                             int32_t z1, const double* __restrict__ rx, const double* __restrict__ ry,
                              const double* __restrict__ rz) {
     const int32_t K2 = z1 - z0 + 2;    // cells per column in the window
     const int32_t tid = threadIdx.x;
@@ -95,7 +103,7 @@ __device__ inline bool stage(Stage& S, const SegGrid& g, int32_t cx, int32_t cy,
         S.total = acc;
     }
     __syncthreads();
-    if (S.total > kLdsAtoms) return false;
+    if (S.total > P.cap) return false;
     for (int32_t p = tid; p < S.total; p += kBlock) {
         int32_t col = 0;
         while (col < 8 && p >= S.colbase[col + 1]) col++;
@@ -109,9 +117,9 @@ __device__ inline bool stage(Stage& S, const SegGrid& g, int32_t cx, int32_t cy,
         if (z < 0) { z += g.n[2]; sz = -g.L; } else if (z >= g.n[2]) { z -= g.n[2]; sz = g.L; }
         const int32_t c = (x * g.n[1] + y) * g.n[2] + z;
         const int32_t slot = g.start[c] + (q - S.cellpre[col][o]);
-        S.x[p] = rx[slot] + sx;
-        S.y[p] = ry[slot] + sy;
-        S.z[p] = rz[slot] + sz;
+        P.x[p] = rx[slot] + sx;
+        P.y[p] = ry[slot] + sy;
+        P.z[p] = rz[slot] + sz;
     }
     __syncthreads();
     return true;
@@ -122,16 +130,19 @@ k_nlist_lds(SegGrid g, const double* __restrict__ rx, const double* __restrict__
             const double* __restrict__ rz, double rl2, int32_t n, int32_t cap, uint16_t* __restrict__ list,
             int32_t* __restrict__ count, int32_t* __restrict__ err) {
     __shared__ Stage S;
+    __shared__ double buf[3 * kLdsAtoms];
     const int32_t seg = logical_segment(g);
     if (seg >= g.total) return;
+    const Pos P{buf, buf + kLdsAtoms, buf + 2 * kLdsAtoms, kLdsAtoms};
     const int32_t col = seg / g.nseg, szi = seg % g.nseg;
     const int32_t cx = col / g.n[1], cy = col % g.n[1];
     int32_t z0, z1;
     seg_z(szi, g.nseg, g.n[2], z0, z1);
-    if (!stage(S, g, cx, cy, z0, z1, rx, ry, rz)) {
+    if (!stage(S, P, g, cx, cy, z0, z1, rx, ry, rz)) {
         if (threadIdx.x == 0) atomicMax(&err[0], S.total);
         return;
     }
+    if (threadIdx.x == 0) atomicMax(&err[2], S.total);   // largest staging, sizes the step kernel's LDS
     const int32_t cbase = (cx * g.n[1] + cy) * g.n[2];
     const int32_t hbeg = g.start[cbase + z0], hend = g.start[cbase + z1];
     for (int32_t s = hbeg + threadIdx.x; s < hend; s += kBlock) {
@@ -140,13 +151,13 @@ k_nlist_lds(SegGrid g, const double* __restrict__ rx, const double* __restrict__
         int32_t o = 1;
         while (S.cellpre[4][o + 1] - S.cellpre[4][1] <= qc) o++;
         const int32_t self = S.colbase[4] + S.cellpre[4][1] + qc;
-        const double xi = S.x[self], yi = S.y[self], zi = S.z[self];
+        const double xi = P.x[self], yi = P.y[self], zi = P.z[self];
         int32_t k = 0;
         for (int32_t c = 0; c < 9; c++) {
             const int32_t b = S.colbase[c] + S.cellpre[c][o - 1], e = S.colbase[c] + S.cellpre[c][o + 2];
             for (int32_t p = b; p < e; p++) {
                 if (p == self) continue;
-                const double dx = xi - S.x[p], dy = yi - S.y[p], dz = zi - S.z[p];
+                const double dx = xi - P.x[p], dy = yi - P.y[p], dz = zi - P.z[p];
                 if (dx * dx + dy * dy + dz * dz >= rl2) continue;
                 if (k < cap) list[(int64_t)k * n + s] = (uint16_t)p;
                 k++;
@@ -172,21 +183,24 @@ struct StepArgs {
     double hard2;                // list validity: displacement^2 must stay below this
     unsigned long long* flags;   // kSpread x {max disp^2 bits, max step^2 bits, stale count}
     int32_t n;
+    int32_t lds_cap;             // staged atoms the dynamic LDS holds
 };
 
 template <bool kFp64, bool kFinal, bool kForceOut>
 __global__ void __launch_bounds__(kBlock)
 k_step_lds(StepArgs a) {
     __shared__ Stage S;
+    extern __shared__ double dyn[];
     const SegGrid& g = a.g;
     const int32_t seg = logical_segment(g);
     if (seg >= g.total) return;
+    const Pos P{dyn, dyn + a.lds_cap, dyn + 2 * a.lds_cap, a.lds_cap};
     const int32_t col = seg / g.nseg, szi = seg % g.nseg;
     const int32_t cx = col / g.n[1], cy = col % g.n[1];
     int32_t z0, z1;
     seg_z(szi, g.nseg, g.n[2], z0, z1);
     unsigned long long* fl = a.flags + 3 * (blockIdx.x % kSpread);
-    if (!stage(S, g, cx, cy, z0, z1, a.st.r[0], a.st.r[1], a.st.r[2])) {
+    if (!stage(S, P, g, cx, cy, z0, z1, a.st.r[0], a.st.r[1], a.st.r[2])) {
         // cannot happen after a successful list build on the same cells; report it as stale
         if (threadIdx.x == 0) atomicAdd(&fl[2], 1ull << 32);
         return;
@@ -200,27 +214,42 @@ k_step_lds(StepArgs a) {
         double fx = 0, fy = 0, fz = 0;
         const int32_t cnt = a.count[i];
         const uint16_t* e = a.list + i;
-        for (int32_t k = 0; k < cnt; k++) {
-            const int32_t j = e[(int64_t)k * n];
-            const double ddx = xi - S.x[j], ddy = yi - S.y[j], ddz = zi - S.z[j];
-            const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            if (r2 >= a.rc2) continue;
-            if constexpr (kFp64) {
-                const double q = r2 / a.sig2;
-                const double iq4 = 1.0 / (q * q * q * q);
-                const double iq7 = iq4 / (q * q * q);
-                const double fr = (2 * iq7 - iq4) * a.coeff;
-                fx += ddx * fr; fy += ddy * fr; fz += ddz * fr;
-            } else {
-                // u = (sig/r)^2; F/r = coeff u^4 (2 u^3 - 1)
-                const float u = __builtin_amdgcn_rcpf((float)r2 * a.inv_sig2f);
-                const float u2 = u * u;
-                const float u3 = u2 * u;
-                const float u4 = u2 * u2;
-                const double fr = (double)(u4 * __builtin_fmaf(2.0f, u3, -1.0f) * a.coeff_f);
-                fx = __builtin_fma(fr, ddx, fx);
-                fy = __builtin_fma(fr, ddy, fy);
-                fz = __builtin_fma(fr, ddz, fz);
+        // list entries are loaded kBatch ahead so several loads are in flight;
+        // the list has 2 kBatch spare rows, so the look-ahead stays in bounds
+        int32_t jn[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) jn[u] = e[(int64_t)u * n];
+        for (int32_t k0 = 0; k0 < cnt; k0 += kBatch) {
+            int32_t jc[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) {
+                jc[u] = jn[u];
+                jn[u] = e[(int64_t)(k0 + kBatch + u) * n];
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) {
+                if (k0 + u >= cnt) break;
+                const int32_t j = jc[u];
+                const double ddx = xi - P.x[j], ddy = yi - P.y[j], ddz = zi - P.z[j];
+                const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                if (r2 >= a.rc2) continue;
+                if constexpr (kFp64) {
+                    const double q = r2 / a.sig2;
+                    const double iq4 = 1.0 / (q * q * q * q);
+                    const double iq7 = iq4 / (q * q * q);
+                    const double fr = (2 * iq7 - iq4) * a.coeff;
+                    fx += ddx * fr; fy += ddy * fr; fz += ddz * fr;
+                } else {
+                    // u = (sig/r)^2; F/r = coeff u^4 (2 u^3 - 1)
+                    const float w = __builtin_amdgcn_rcpf((float)r2 * a.inv_sig2f);
+                    const float w2 = w * w;
+                    const float w3 = w2 * w;
+                    const float w4 = w2 * w2;
+                    const double fr = (double)(w4 * __builtin_fmaf(2.0f, w3, -1.0f) * a.coeff_f);
+                    fx = __builtin_fma(fr, ddx, fx);
+                    fy = __builtin_fma(fr, ddy, fy);
+                    fz = __builtin_fma(fr, ddz, fz);
+                }
             }
         }
         const double f[3] = {fx, fy, fz};
@@ -310,6 +339,7 @@ int32_t lds_segments_per_column(int32_t nz, double atoms_per_cell, int32_t kmax)
 }
 
 int32_t lds_capacity() { return kLdsAtoms; }
+int32_t list_spare_rows() { return 2 * kBatch; }
 int32_t flag_spread() { return kSpread; }
 
 hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg, double rl2,
@@ -323,8 +353,23 @@ hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v
 hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg,
                            double* const rn[3], double* const r0[3], const uint16_t* list, const int32_t* count,
                            const LjTable& t, const MassTable& m, const GearCoeffs& gc, double hard2, int32_t n,
-                           int32_t precision, bool final_step, bool force_out, uint64_t* flags) {
+                           int32_t precision, bool final_step, bool force_out, uint64_t* flags, int32_t lds_atoms) {
     if (n <= 0) return hipSuccess;
+    static bool attr_set = false;
+    if (!attr_set) {
+        // allow dynamic LDS above the 64 KiB default for the largest staging
+        const int bytes = 3 * kLdsAtoms * (int)sizeof(double);
+        (void)hipFuncSetAttribute((const void*)k_step_lds<true, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute((const void*)k_step_lds<true, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute((const void*)k_step_lds<true, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute((const void*)k_step_lds<true, false, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute((const void*)k_step_lds<false, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute((const void*)k_step_lds<false, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute((const void*)k_step_lds<false, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute((const void*)k_step_lds<false, false, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        attr_set = true;
+    }
+    if (lds_atoms < 1 || lds_atoms > kLdsAtoms) return hipErrorInvalidValue;
     StepArgs a;
     a.g = make_grid(v, nseg);
     a.st = st;
@@ -342,9 +387,11 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
     a.hard2 = hard2;
     a.flags = reinterpret_cast<unsigned long long*>(flags);
     a.n = n;
+    a.lds_cap = lds_atoms;
+    const size_t dyn = 3 * (size_t)lds_atoms * sizeof(double);
     dim3 grid(a.g.per_xcd * kXcd);
     const bool fp64 = precision == MDX_PREC_FP64;
-#define MDX_STEP(F64, FIN, FO) k_step_lds<F64, FIN, FO><<<grid, kBlock, 0, s>>>(a)
+#define MDX_STEP(F64, FIN, FO) k_step_lds<F64, FIN, FO><<<grid, kBlock, dyn, s>>>(a)
     if (fp64) {
         if (final_step) { if (force_out) MDX_STEP(true, true, true); else MDX_STEP(true, true, false); }
         else { if (force_out) MDX_STEP(true, false, true); else MDX_STEP(true, false, false); }

@@ -138,13 +138,16 @@ hipError_t launch_predict_track(hipStream_t s, const DevState& st, int32_t n, co
 int32_t lds_segments_per_column(int32_t nz, double atoms_per_cell, int32_t kmax);
 int32_t lds_capacity();
 int32_t flag_spread();
-// err[0]: staged atoms of a segment exceeded lds_capacity(); err[1]: longest list above cap
+int32_t list_spare_rows();   // rows the list needs beyond its capacity (look-ahead loads)
+// err[0]: staged atoms of a segment exceeded lds_capacity(); err[1]: longest list above
+// cap; err[2]: the largest staging (atoms), which sizes the step kernel's LDS
 hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg, double rl2,
                             int32_t n, int32_t cap, uint16_t* list, int32_t* count, int32_t* err);
 hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg,
                            double* const rn[3], double* const r0[3], const uint16_t* list, const int32_t* count,
                            const LjTable& t, const MassTable& m, const GearCoeffs& gc, double hard2, int32_t n,
-                           int32_t precision, bool final_step, bool force_out, uint64_t* flags);
+                           int32_t precision, bool final_step, bool force_out, uint64_t* flags,
+                           int32_t lds_atoms);
 
 // bond_ref.hip
 hipError_t launch_bond_ref(hipStream_t s, const DevState& st, const int32_t* slot_of_id,

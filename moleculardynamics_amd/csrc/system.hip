@@ -84,6 +84,7 @@ struct mdx_system {
     int32_t nl_cap = 0;
     int32_t nseg = 0;                // segments per cell column
     int32_t seg_kmax = 1 << 20;      // cap on cells per segment (lowered if staging overflows)
+    int32_t lds_atoms = 0;           // largest segment staging of the last build
     static constexpr int kRing = 3;
     static constexpr int kFlagWords = 3 * 64;  // flag_spread() x {disp^2, step^2, stale}
     uint64_t* dflags = nullptr;      // kRing x kFlagWords
@@ -519,16 +520,19 @@ int rebuild_periodic(mdx_system* s) {
         double rho = n / (c.box_length * c.box_length * c.box_length);
         double est = 4.0 / 3.0 * M_PI * std::pow((double)c.lj_rs, 3) * rho;
         int32_t cap = c.max_neighbors > 0 ? c.max_neighbors : ((int32_t)(1.5 * est) + 32 + 31) / 32 * 32;
-        if ((rc = dalloc(s, &s->nlist, (size_t)cap * n)) || (rc = dalloc(s, &s->ncount, n))) return rc;
+        if ((rc = dalloc(s, &s->nlist, (size_t)(cap + list_spare_rows()) * n)) || (rc = dalloc(s, &s->ncount, n)))
+            return rc;
         s->nl_cap = cap;
     }
     for (;;) {
         s->nseg = lds_segments_per_column(v.g[2], per_cell, s->seg_kmax);
-        HIPCHK(s, hipMemsetAsync(s->dint + 4, 0, 2 * sizeof(int32_t), s->stream));
+        HIPCHK(s, hipMemsetAsync(s->dint + 4, 0, 3 * sizeof(int32_t), s->stream));
         LAUNCH(s, "nlist_build", launch_nlist_lds(s->stream, s->st, v, s->nseg, rl2, n, s->nl_cap, s->nlist,
                                                   s->ncount, s->dint + 4));
-        int32_t e[2];
-        if ((rc = read_ints(s, e, s->dint + 4, 2))) return rc;
+        int32_t e[3];
+        if ((rc = read_ints(s, e, s->dint + 4, 3))) return rc;
+        s->lds_atoms = (e[2] + 63) / 64 * 64;
+        if (s->lds_atoms > lds_capacity()) s->lds_atoms = lds_capacity();
         if (e[0] > 0) {
             // a segment's neighbourhood did not fit in LDS: shorter segments
             int32_t k = (v.g[2] + s->nseg - 1) / s->nseg;
@@ -541,7 +545,7 @@ int rebuild_periodic(mdx_system* s) {
         dfree(s, s->nlist);
         s->nlist = nullptr;
         int32_t cap = (e[1] + e[1] / 4 + 31) / 32 * 32;
-        if ((rc = dalloc(s, &s->nlist, (size_t)cap * n))) return rc;
+        if ((rc = dalloc(s, &s->nlist, (size_t)(cap + list_spare_rows()) * n))) return rc;
         s->nl_cap = cap;
     }
     return MDX_OK;
@@ -626,7 +630,8 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
         const bool fin = k + 1 == nsteps;
         HIPCHK(s, hipMemsetAsync(slot_flags(k), 0, W * sizeof(uint64_t), st));
         LAUNCH(s, "md_step", launch_step_lds(st, s->st, s->lj.view, s->nseg, s->rnext, s->rref, s->nlist, s->ncount,
-                                             s->ljt, s->mt, s->gc, hard2, n, c.precision, fin, fin, slot_flags(k)));
+                                             s->ljt, s->mt, s->gc, hard2, n, c.precision, fin, fin, slot_flags(k),
+                                             s->lds_atoms));
         if ((rc = post(k))) return rc;
         for (int d = 0; d < 3; d++) std::swap(s->st.r[d], s->rnext[d]);
         s->t++;
@@ -696,7 +701,7 @@ int mdx_create(const mdx_config* cfg, mdx_system** out) {
         (rc = dalloc(s, &s->mb_ids, std::max(cfg->n_bond, 1))) || (rc = dalloc(s, &s->rref[0], n)) ||
         (rc = dalloc(s, &s->rref[1], n)) || (rc = dalloc(s, &s->rref[2], n)) ||
         (rc = dalloc(s, &s->e_atom, 2 * (size_t)n)) || (rc = dalloc(s, &s->dscal, 16)) ||
-        (rc = dalloc(s, &s->dint, 8)) || (rc = dalloc(s, &s->dbits, 4))) {
+        (rc = dalloc(s, &s->dint, 16)) || (rc = dalloc(s, &s->dbits, 4))) {
         g_err = s->err;
         return bail(rc);
     }
