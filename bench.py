@@ -44,7 +44,26 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-ncell", type=int, default=32)
     p.add_argument("--cpu-steps", type=int, default=20)
+    p.add_argument("--copy-bytes", type=int, default=0,
+                   help="only run device-to-device copies of this many bytes (counter calibration)")
     return p.parse_args()
+
+
+def copy_calibration(nbytes: int, reps: int = 3) -> None:
+    """Device-to-device copies of a known size: what FETCH_SIZE / WRITE_SIZE
+    read for them scales the counters of the MD kernels into bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}))
 
 
 def cpu_baseline(ncell: int, steps: int) -> dict:
@@ -71,6 +90,9 @@ def main():
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if a.copy_bytes:
+        copy_calibration(a.copy_bytes)
+        return
     from moleculardynamics_amd import BOX_PERIODIC, LJ_UNIFORM, PREC_MIXED, System, initial
 
     pos, vel, L = initial.lj_fluid(a.ncell, 0.8442, 0.728, seed=2024)

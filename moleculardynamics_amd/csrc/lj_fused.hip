@@ -104,22 +104,45 @@ __device__ inline bool stage(Stage& S, const Pos& P, const SegGrid& g, int32_t c
     }
     __syncthreads();
     if (S.total > P.cap) return false;
-    for (int32_t p = tid; p < S.total; p += kBlock) {
-        int32_t col = 0;
-        while (col < 8 && p >= S.colbase[col + 1]) col++;
-        const int32_t q = p - S.colbase[col];
-        int32_t o = 0;
-        while (S.cellpre[col][o + 1] <= q) o++;
-        int32_t x = cx + col / 3 - 1, y = cy + col % 3 - 1, z = z0 - 1 + o;
-        double sx = 0, sy = 0, sz = 0;
+    // Copy column by column.  A column's window z0-1 .. z1 is one contiguous
+    // slot range unless it wraps in z; then its first (or last) cell comes
+    // from the other end of the column with a shift of -L (or +L).
+    for (int32_t col = 0; col < 9; col++) {
+        int32_t x = cx + col / 3 - 1, y = cy + col % 3 - 1;
+        double sx = 0, sy = 0;
         if (x < 0) { x += g.n[0]; sx = -g.L; } else if (x >= g.n[0]) { x -= g.n[0]; sx = g.L; }
         if (y < 0) { y += g.n[1]; sy = -g.L; } else if (y >= g.n[1]) { y -= g.n[1]; sy = g.L; }
-        if (z < 0) { z += g.n[2]; sz = -g.L; } else if (z >= g.n[2]) { z -= g.n[2]; sz = g.L; }
-        const int32_t c = (x * g.n[1] + y) * g.n[2] + z;
-        const int32_t slot = g.start[c] + (q - S.cellpre[col][o]);
-        P.x[p] = rx[slot] + sx;
-        P.y[p] = ry[slot] + sy;
-        P.z[p] = rz[slot] + sz;
+        const int32_t cb = (x * g.n[1] + y) * g.n[2];
+        const int32_t za = z0 - 1, zb = z1;   // window cells, inclusive
+        // pieces: [za] wrapped, [max(za,0) .. min(zb,nz-1)] contiguous, [zb] wrapped
+        int32_t dst = S.colbase[col];
+        if (za < 0) {
+            const int32_t c = cb + g.n[2] - 1, b = g.start[c], e = g.start[c + 1];
+            for (int32_t q = tid; q < e - b; q += kBlock) {
+                P.x[dst + q] = rx[b + q] + sx;
+                P.y[dst + q] = ry[b + q] + sy;
+                P.z[dst + q] = rz[b + q] - g.L;
+            }
+            dst += e - b;
+        }
+        {
+            const int32_t lo = za < 0 ? 0 : za, hi = zb >= g.n[2] ? g.n[2] - 1 : zb;
+            const int32_t b = g.start[cb + lo], e = g.start[cb + hi + 1];
+            for (int32_t q = tid; q < e - b; q += kBlock) {
+                P.x[dst + q] = rx[b + q] + sx;
+                P.y[dst + q] = ry[b + q] + sy;
+                P.z[dst + q] = rz[b + q];
+            }
+            dst += e - b;
+        }
+        if (zb >= g.n[2]) {
+            const int32_t b = g.start[cb], e = g.start[cb + 1];
+            for (int32_t q = tid; q < e - b; q += kBlock) {
+                P.x[dst + q] = rx[b + q] + sx;
+                P.y[dst + q] = ry[b + q] + sy;
+                P.z[dst + q] = rz[b + q] + g.L;
+            }
+        }
     }
     __syncthreads();
     return true;
