@@ -178,12 +178,21 @@ k_nlist_lds(SegGrid g, const double* __restrict__ rx, const double* __restrict__
         int32_t k = 0;
         for (int32_t c = 0; c < 9; c++) {
             const int32_t b = S.colbase[c] + S.cellpre[c][o - 1], e = S.colbase[c] + S.cellpre[c][o + 2];
-            for (int32_t p = b; p < e; p++) {
-                if (p == self) continue;
-                const double dx = xi - P.x[p], dy = yi - P.y[p], dz = zi - P.z[p];
-                if (dx * dx + dy * dy + dz * dz >= rl2) continue;
-                if (k < cap) list[(int64_t)k * n + s] = (uint16_t)p;
-                k++;
+            for (int32_t p0 = b; p0 < e; p0 += kBatch) {
+                bool hit[kBatch];
+#pragma unroll
+                for (int u = 0; u < kBatch; u++) {
+                    const int32_t p = p0 + u < e ? p0 + u : b;
+                    const double dx = xi - P.x[p], dy = yi - P.y[p], dz = zi - P.z[p];
+                    hit[u] = (p0 + u < e) && (p0 + u != self) && (dx * dx + dy * dy + dz * dz < rl2);
+                }
+#pragma unroll
+                for (int u = 0; u < kBatch; u++) {
+                    if (hit[u]) {
+                        if (k < cap) list[(int64_t)k * n + s] = (uint16_t)(p0 + u);
+                        k++;
+                    }
+                }
             }
         }
         count[s] = k < cap ? k : cap;
@@ -249,26 +258,30 @@ k_step_lds(StepArgs a) {
                 jc[u] = jn[u];
                 jn[u] = e[(int64_t)(k0 + kBatch + u) * n];
             }
+            // straight-line body: every entry of the batch is evaluated and
+            // masked (select, so a 1/0 of a padding entry never reaches the sum),
+            // letting the 8 LDS loads and FP64 chains overlap
 #pragma unroll
             for (int u = 0; u < kBatch; u++) {
-                if (k0 + u >= cnt) break;
-                const int32_t j = jc[u];
+                const bool valid = k0 + u < cnt;
+                const int32_t j = valid ? jc[u] : 0;
                 const double ddx = xi - P.x[j], ddy = yi - P.y[j], ddz = zi - P.z[j];
-                const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                if (r2 >= a.rc2) continue;
                 if constexpr (kFp64) {
+                    const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
                     const double q = r2 / a.sig2;
                     const double iq4 = 1.0 / (q * q * q * q);
                     const double iq7 = iq4 / (q * q * q);
-                    const double fr = (2 * iq7 - iq4) * a.coeff;
+                    const double fr = (valid && r2 < a.rc2) ? (2 * iq7 - iq4) * a.coeff : 0.0;
                     fx += ddx * fr; fy += ddy * fr; fz += ddz * fr;
                 } else {
-                    // u = (sig/r)^2; F/r = coeff u^4 (2 u^3 - 1)
+                    const double r2 = __builtin_fma(ddx, ddx, __builtin_fma(ddy, ddy, ddz * ddz));
+                    // w = (sig/r)^2; F/r = coeff w^4 (2 w^3 - 1)
                     const float w = __builtin_amdgcn_rcpf((float)r2 * a.inv_sig2f);
                     const float w2 = w * w;
                     const float w3 = w2 * w;
                     const float w4 = w2 * w2;
-                    const double fr = (double)(w4 * __builtin_fmaf(2.0f, w3, -1.0f) * a.coeff_f);
+                    const float frf = w4 * __builtin_fmaf(2.0f, w3, -1.0f) * a.coeff_f;
+                    const double fr = (valid && r2 < a.rc2) ? (double)frf : 0.0;
                     fx = __builtin_fma(fr, ddx, fx);
                     fy = __builtin_fma(fr, ddy, fy);
                     fz = __builtin_fma(fr, ddz, fz);
@@ -354,11 +367,14 @@ SegGrid make_grid(const GridView& v, int32_t nseg) {
 }  // namespace
 
 int32_t lds_segments_per_column(int32_t nz, double atoms_per_cell, int32_t kmax) {
-    // about 200 home atoms per workgroup, at most kmax cells per segment
-    int32_t k = (int32_t)(200.0 / (atoms_per_cell > 0 ? atoms_per_cell : 1.0));
+    // about 230 home atoms per 256-thread workgroup, at most kmax cells per segment
     kmax = kmax < kMaxK ? kmax : kMaxK;
-    k = k < 1 ? 1 : (k > kmax ? kmax : k);
-    return (nz + k - 1) / k;
+    kmax = kmax < 1 ? 1 : kmax;
+    const double apc = atoms_per_cell > 0 ? atoms_per_cell : 1.0;
+    int32_t nseg = (int32_t)(nz * apc / 230.0 + 0.5);
+    const int32_t lo = (nz + kmax - 1) / kmax;   // segments no longer than kmax cells
+    nseg = nseg < lo ? lo : nseg;
+    return nseg > nz ? nz : nseg;
 }
 
 int32_t lds_capacity() { return kLdsAtoms; }

@@ -562,7 +562,7 @@ double bits_to_double(uint64_t b) {
 // rebuild decision runs one step behind the device: before step k is
 // launched the host reads the flags of the positions r(k-1) (largest
 // displacement since the build D and largest single-step move S) and
-// rebuilds when D + 2 S reaches 0.8 (rs - rc)/2; every step kernel
+// rebuilds when D + 1.2 S reaches 0.9 (rs - rc)/2; every step kernel
 // re-checks its own positions against the margin and counts violations, which
 // turn into MDX_ERR_LIST_STALE.
 int run_periodic(mdx_system* s, int32_t nsteps) {
@@ -614,14 +614,15 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
         uint64_t stale;
         reduce(host_flags(fk), D, S, stale);
         if (stale != 0) return fail(s, MDX_ERR_LIST_STALE, "an atom moved past the neighbour-list margin");
-        // rebuild at 80% of the margin so the one-step-late estimate keeps slack
-        // before the hard check (the step kernel's stale count) could trip
-        const double trigger = 0.8 * margin;
+        // rebuild at 90% of the margin, estimating r(k) as one more step of at most
+        // 1.2 S away; the step kernel's own check of the full margin (stale count)
+        // catches anything the estimate misses
+        const double trigger = 0.9 * margin;
         bool rebuild = false;
         if (k == 0)
             rebuild = D >= trigger;                // exact: these flags are of r(0)
         else if (fk >= last_build)
-            rebuild = D + 2.0 * S >= trigger;      // r(k) is at most about one more step away
+            rebuild = D + 1.2 * S >= trigger;
         // (fk < last_build: the flags predate the last build, r(k) is one step from it)
         if (rebuild) {
             if ((rc = rebuild_periodic(s))) return rc;
