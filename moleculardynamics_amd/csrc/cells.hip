@@ -36,18 +36,42 @@ __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
     for (; i < n; i += stride) p[i] = v;
 }
 
+// Runs of equal keys among the lanes of a wave (the active lanes must be a
+// prefix of the wave, as at the tail of a 1-D grid).  Consecutive slots are
+// mostly in one cell, so one atomic per run replaces one per item.
+struct Run {
+    int32_t leader;   // first lane of this lane's run
+    int32_t offset;   // lane - leader
+    int32_t len;      // lanes in the run
+};
+__device__ inline Run wave_runs(int32_t key, bool active) {
+    const int32_t lane = threadIdx.x & (kWave - 1);
+    const int32_t prev = __shfl_up(key, 1, kWave);
+    const bool head = active && (lane == 0 || prev != key);
+    const uint64_t heads = __ballot(head);
+    const int32_t nact = __popcll(__ballot(active));
+    const uint64_t upto = (2ull << lane) - 1;          // lanes 0..lane (all 64 at lane 63)
+    Run r;
+    r.leader = 63 - __clzll(heads & upto);
+    const uint64_t later = heads & ~upto;
+    const int32_t end = later ? __ffsll((unsigned long long)later) - 1 : nact;
+    r.offset = lane - r.leader;
+    r.len = end - r.leader;
+    return r;
+}
+
 // One thread per item; item q lives in slot item_slot[q] (or slot q).
 __global__ void k_bin(DevState st, const int32_t* __restrict__ item_slot, int32_t items, GridView g,
                       int32_t* __restrict__ cell_out, int32_t* __restrict__ counts,
                       int32_t* __restrict__ err) {
-    int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= items) return;
-    int32_t s = item_slot ? item_slot[q] : q;
+    const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = q < items;
+    const int32_t s = active ? (item_slot ? item_slot[q] : q) : 0;
     int32_t c[3];
     bool ok = true;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        double x = st.r[d][s];
+        double x = active ? st.r[d][s] : 0.0;
         // range-check in floating point first so NaN or huge values never
         // reach the integer conversion
         if (g.periodic) {
@@ -62,15 +86,14 @@ __global__ void k_bin(DevState st, const int32_t* __restrict__ item_slot, int32_
             ok = ok && (c[d] >= 0 && c[d] < g.g[d]);
         }
     }
-    if (!ok) {
-        atomicOr(err, 1);
-        cell_out[q] = 0;
-        atomicAdd(&counts[0], 1);
-        return;
+    // an atom outside the grid is reported and counted in cell 0
+    const int32_t cell = ok ? (c[0] * g.g[1] + c[1]) * g.g[2] + c[2] : 0;
+    if (active && !ok) atomicOr(err, 1);
+    const Run r = wave_runs(cell, active);
+    if (active) {
+        cell_out[q] = cell;
+        if (r.offset == 0) atomicAdd(&counts[cell], r.len);
     }
-    int32_t cell = (c[0] * g.g[1] + c[1]) * g.g[2] + c[2];
-    cell_out[q] = cell;
-    atomicAdd(&counts[cell], 1);
 }
 
 __device__ inline int32_t wave_incl_scan(int32_t x) {
@@ -143,35 +166,48 @@ __global__ void k_scan_add(int32_t* __restrict__ out, int64_t n, const int32_t* 
     if (i < n) out[i] += sums[i / kScanTile];
 }
 
+// Items into their cells' ranges (order inside a cell is settled by
+// k_sort_cells); one atomic per run of equal cells in a wave.
 __global__ void k_scatter(const int32_t* __restrict__ cell_of, const int32_t* __restrict__ start,
                           int32_t* __restrict__ fill, int32_t* __restrict__ sorted_items,
                           int32_t items) {
-    int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= items) return;
-    int32_t c = cell_of[q];
-    int32_t pos = start[c] + atomicAdd(&fill[c], 1);
-    sorted_items[pos] = q;
+    const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = q < items;
+    const int32_t c = active ? cell_of[q] : -1;
+    const Run r = wave_runs(c, active);
+    int32_t base = 0;
+    if (active && r.offset == 0) base = atomicAdd(&fill[c], r.len);
+    base = __shfl(base, r.leader, kWave);
+    if (active) sorted_items[start[c] + base + r.offset] = q;
 }
 
-// Restore ascending key order inside each cell (the atomic scatter is
-// unordered): insertion sort of one short segment per thread.
-__global__ void k_sort_cells(const int32_t* __restrict__ start, int64_t ncell,
-                             int32_t* __restrict__ items, const int32_t* __restrict__ key_of) {
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    int32_t b = start[c], e = start[c + 1];
-    for (int32_t i = b + 1; i < e; i++) {
-        int32_t it = items[i];
-        int32_t key = key_of ? key_of[it] : it;
-        int32_t j = i - 1;
-        while (j >= b) {
-            int32_t o = items[j];
-            int32_t ko = key_of ? key_of[o] : o;
-            if (ko <= key) break;
-            items[j + 1] = o;
-            j--;
+// Ascending key order inside each cell: one wave per cell, each lane's
+// position is the number of smaller keys in the cell (keys are distinct).
+constexpr int kCellsPerBlock = kBlock / kWave;
+__global__ void __launch_bounds__(kBlock)
+k_sort_cells(const int32_t* __restrict__ start, int64_t ncell, const int32_t* __restrict__ in,
+             int32_t* __restrict__ out, const int32_t* __restrict__ key_of) {
+    const int64_t c = (int64_t)blockIdx.x * kCellsPerBlock + threadIdx.x / kWave;
+    if (c >= ncell) return;   // whole waves
+    const int32_t lane = threadIdx.x & (kWave - 1);
+    const int32_t b = start[c], cnt = start[c + 1] - b;
+    if (cnt <= kWave) {
+        const int32_t it = lane < cnt ? in[b + lane] : 0;
+        const int32_t key = lane < cnt ? (key_of ? key_of[it] : it) : INT32_MAX;
+        int32_t rank = 0;
+        for (int32_t l = 0; l < cnt; l++) rank += __builtin_amdgcn_readlane(key, l) < key ? 1 : 0;
+        if (lane < cnt) out[b + rank] = it;
+    } else {
+        for (int32_t p = lane; p < cnt; p += kWave) {
+            const int32_t it = in[b + p];
+            const int32_t key = key_of ? key_of[it] : it;
+            int32_t rank = 0;
+            for (int32_t l = 0; l < cnt; l++) {
+                const int32_t o = in[b + l];
+                rank += (key_of ? key_of[o] : o) < key ? 1 : 0;
+            }
+            out[b + rank] = it;
         }
-        items[j + 1] = it;
     }
 }
 
@@ -335,9 +371,9 @@ hipError_t launch_scatter(hipStream_t s, const int32_t* cell_of, const int32_t* 
     return hipGetLastError();
 }
 
-hipError_t launch_sort_cells(hipStream_t s, const int32_t* start, int64_t ncell,
-                             int32_t* sorted_items, const int32_t* key_of_item) {
-    k_sort_cells<<<grid_for(ncell), kBlock, 0, s>>>(start, ncell, sorted_items, key_of_item);
+hipError_t launch_sort_cells(hipStream_t s, const int32_t* start, int64_t ncell, const int32_t* items_in,
+                             int32_t* items_out, const int32_t* key_of_item) {
+    k_sort_cells<<<grid_for(ncell, kCellsPerBlock), kBlock, 0, s>>>(start, ncell, items_in, items_out, key_of_item);
     return hipGetLastError();
 }
 

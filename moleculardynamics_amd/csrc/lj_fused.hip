@@ -7,19 +7,28 @@
 // {-1,0,1}, each over z0-1 .. z1 -- is staged into LDS as FP64 positions
 // already moved to the periodic image seen from the segment, in the order
 // column (dx,dy lexicographic), then z, then slot; so every atom's 27-cell
-// candidate set is 9 contiguous LDS ranges, visited in the reference's
-// candidate order (cells in (dx,dy,dz) lexicographic order, ascending index
-// within a cell: common/NeighborList.cpp:99-136).
+// candidate set is 9 contiguous LDS ranges (cells in (dx,dy,dz)
+// lexicographic order, ascending index within a cell, the candidate order of
+// common/NeighborList.cpp:99-136).
 //
-//  k_nlist_lds (at each rebuild): each atom keeps the LDS indices of its
-//    candidates within the list radius rs (16-bit entries, column-major
-//    [k][slot] so a wave's loads are coalesced).
+// Each atom is served by a quad of 4 lanes.  A workgroup of 512 lanes holds
+// 128 atoms per pass, so a segment of about 110 atoms and its ~1400 staged
+// neighbours (34 KiB of LDS) leave room for 4 workgroups (32 waves) per CU:
+// enough waves to cover the HBM and LDS latencies of the gathers.
+//
+//  k_nlist_lds (at each rebuild): the quad scans the atom's candidates, 16 at
+//    a time, and keeps those within the list radius rs.  Hit number k (in the
+//    quad's scan order) goes to lane k % 4 as that lane's entry k / 4, so the
+//    four lanes' lists differ in length by at most one.  Entries are 16-bit
+//    LDS indices; a lane's entries are packed 8 to a 16-byte chunk, and chunk
+//    c of (atom s, lane q) sits at chunk index (c * n + s) * 4 + q, so a
+//    wave's chunk loads cover one contiguous 1 KiB run.
 //  k_step_lds (every step): LJ force over the list (FP64 displacement, FP32
-//    12-6 force law in the mixed mode, FP64 accumulation), then the Gear
-//    corrector of step t and the predictor of step t+1, in registers.
-//    Positions are double-buffered: r(t) is read (and staged) from A and
-//    r(t+1) written to B, so no workgroup reads a position another one has
-//    already advanced.
+//    12-6 force law in the mixed mode, FP64 accumulation), summed over the
+//    quad, then the Gear corrector of step t and the predictor of step t+1,
+//    lane q handling component q.  Positions are double-buffered: r(t) is read
+//    (and staged) from A and r(t+1) written to B, so no workgroup reads a
+//    position another one has already advanced.
 // Workgroup-to-segment mapping is XCD-aware: workgroups are dealt to the 8
 // XCDs round robin, so block b takes segment (b % 8) * per + b / 8 and each
 // XCD's L2 serves one contiguous slab of the box.
@@ -29,47 +38,90 @@ namespace mdx {
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 512;
+constexpr int kLanes = 4;                  // lanes per atom
+constexpr int kApp = kBlock / kLanes;      // atoms per pass
+constexpr int kChunk = 8;                  // 16-bit entries per 16-byte chunk
+constexpr int kScan = 4;                   // candidates per lane per scan step
 constexpr int kXcd = 8;
-constexpr int kLdsAtoms = 3072;   // staged atoms per segment at most (3 x 24 KiB of FP64)
-constexpr int kBatch = 8;         // list entries loaded ahead per thread
-constexpr int kMaxK = 14;         // cells per segment
-constexpr int kSpread = 64;       // flag atomics are spread over this many addresses
+constexpr int kLdsAtoms = 3072;            // staged atoms per segment at most (3 x 24 KiB of FP64)
+constexpr int kMaxK = 14;                  // cells per segment
+constexpr int kSpread = 64;                // flag atomics are spread over this many addresses
+constexpr int kColThreads = kBlock / 9;    // staging: threads per column
+constexpr int kCopyBatch = 4;              // staging: independent loads in flight per thread
 
 struct SegGrid {
     int32_t n[3];        // cells per axis
     int32_t nseg;        // segments per column
     int32_t total;       // n[0] * n[1] * nseg
     int32_t per_xcd;     // ceil(total / 8)
+    float rnseg, rny;    // 1 / nseg, 1 / n[1]
+    int32_t small;       // total and nseg * n[2] below 2^22: div_small applies
     double L;
     const int32_t* start;
 };
 
-// z range [z0, z1) of segment s of a column with nz cells
-__device__ inline void seg_z(int32_t s, int32_t nseg, int32_t nz, int32_t& z0, int32_t& z1) {
-    z0 = (int32_t)(((int64_t)s * nz) / nseg);
-    z1 = (int32_t)(((int64_t)(s + 1) * nz) / nseg);
+// a / b for 0 <= a < 2^22 and b >= 1, given rb ~ 1 / b to a few ulp: the
+// float quotient is then within 1 of the true one, and two integer tests fix
+// it up exactly (an integer division by a run-time value costs ~40 VALU)
+__device__ inline int32_t div_small(int32_t a, int32_t b, float rb) {
+    int32_t q = (int32_t)((float)a * rb);
+    q += (q + 1) * b <= a ? 1 : 0;
+    q -= q * b > a ? 1 : 0;
+    return q;
 }
 
-__device__ inline int32_t logical_segment(const SegGrid& g) {
+// segment number -> column (cx, cy) and z range [z0, z1); the workgroup
+// mapping is XCD-aware (block b is dealt to XCD b % 8)
+__device__ inline int32_t seg_geometry(const SegGrid& g, int32_t& cx, int32_t& cy, int32_t& z0, int32_t& z1) {
     const int32_t b = blockIdx.x;
-    return (b % kXcd) * g.per_xcd + b / kXcd;
+    const int32_t seg = (b % kXcd) * g.per_xcd + b / kXcd;
+    if (seg >= g.total) return -1;
+    int32_t col, szi;
+    if (g.small) {
+        col = div_small(seg, g.nseg, g.rnseg);
+        szi = seg - col * g.nseg;
+        cx = div_small(col, g.n[1], g.rny);
+        z0 = div_small(szi * g.n[2], g.nseg, g.rnseg);
+        z1 = div_small((szi + 1) * g.n[2], g.nseg, g.rnseg);
+    } else {
+        col = seg / g.nseg;
+        szi = seg % g.nseg;
+        cx = col / g.n[1];
+        z0 = (int32_t)(((int64_t)szi * g.n[2]) / g.nseg);
+        z1 = (int32_t)(((int64_t)(szi + 1) * g.n[2]) / g.nseg);
+    }
+    cy = col - cx * g.n[1];
+    return seg;
 }
 
-// LDS header shared by both kernels; the staged FP64 positions x, y, z of
-// `cap` atoms each live in a separate LDS array (Pos), addressed through
-// pointers derived from the __shared__ array itself so accesses stay ds_*.
+// Staged FP64 positions of up to `cap` atoms, x y z side by side (24 bytes an
+// atom, so one address serves the three reads), addressed through a pointer
+// derived from the __shared__ array itself so accesses stay ds_*.  The slot
+// after the last staged atom holds a far-away dummy atom: list padding points
+// at it, and its distance fails every cutoff test.
+constexpr double kFar = 1e100;
 struct Pos {
-    double* x;
-    double* y;
-    double* z;
+    double* p;
     int32_t cap;
+    __device__ double x(int32_t i) const { return p[3 * i]; }
+    __device__ double y(int32_t i) const { return p[3 * i + 1]; }
+    __device__ double z(int32_t i) const { return p[3 * i + 2]; }
+    __device__ void set(int32_t i, double x, double y, double z) const {
+        p[3 * i] = x;
+        p[3 * i + 1] = y;
+        p[3 * i + 2] = z;
+    }
 };
+// LDS header shared by both kernels
 struct Stage {
     int32_t colbase[9];                 // LDS start of each column
     int32_t cellpre[9][kMaxK + 3];      // per column, prefix counts over the K+2 cells
+    int32_t cstart[9][kMaxK + 2];       // per column and window cell, first slot
+    int32_t plen[9][3];                 // per column, lengths of the 3 copy pieces
+    int32_t psrc[9][3];                 // per column, first slot of each piece
     int32_t total;
-    double red[kBlock / kWave][3];
+    unsigned long long red[3];          // step kernel: max disp^2 bits, max step^2 bits, stale count
 };
 
 // Stage the neighbourhood of segment (cx, cy, z0..z1).  Returns false (for
@@ -79,19 +131,32 @@ __device__ inline bool stage(Stage& S, const Pos& P, const SegGrid& g, int32_t c
                              const double* __restrict__ rz) {
     const int32_t K2 = z1 - z0 + 2;    // cells per column in the window
     const int32_t tid = threadIdx.x;
-    if (tid < 9 * K2) {                // per-cell counts (9 columns x K2 cells)
-        const int32_t col = tid / K2, o = tid % K2;
+    if (tid < 3) S.red[tid] = 0;
+    if (tid < 9 * K2) {                // per-cell counts and first slots (9 columns x K2 cells)
+        const int32_t col = div_small(tid, K2, __builtin_amdgcn_rcpf((float)K2)), o = tid - col * K2;
         int32_t x = cx + col / 3 - 1, y = cy + col % 3 - 1, z = z0 - 1 + o;
         x = x < 0 ? x + g.n[0] : (x >= g.n[0] ? x - g.n[0] : x);
         y = y < 0 ? y + g.n[1] : (y >= g.n[1] ? y - g.n[1] : y);
         z = z < 0 ? z + g.n[2] : (z >= g.n[2] ? z - g.n[2] : z);
         const int32_t c = (x * g.n[1] + y) * g.n[2] + z;
-        S.cellpre[col][o + 1] = g.start[c + 1] - g.start[c];
+        const int32_t b = g.start[c], e = g.start[c + 1];
+        S.cellpre[col][o + 1] = e - b;
+        S.cstart[col][o] = b;
     }
     __syncthreads();
     if (tid < 9) {
+        // prefix over the window, then the copy pieces: the window z0-1 .. z1 is
+        // one contiguous slot range except for a first (last) cell that wraps in z
         S.cellpre[tid][0] = 0;
         for (int32_t o = 1; o <= K2; o++) S.cellpre[tid][o] += S.cellpre[tid][o - 1];
+        const bool wlo = z0 == 0, whi = z1 == g.n[2];
+        const int32_t olo = wlo ? 1 : 0, ohi = whi ? K2 - 2 : K2 - 1;
+        S.plen[tid][0] = wlo ? S.cellpre[tid][1] : 0;
+        S.psrc[tid][0] = S.cstart[tid][0];
+        S.plen[tid][1] = S.cellpre[tid][ohi + 1] - S.cellpre[tid][olo];
+        S.psrc[tid][1] = S.cstart[tid][olo];
+        S.plen[tid][2] = whi ? S.cellpre[tid][K2] - S.cellpre[tid][K2 - 1] : 0;
+        S.psrc[tid][2] = S.cstart[tid][K2 - 1];
     }
     __syncthreads();
     if (tid == 0) {
@@ -103,100 +168,127 @@ __device__ inline bool stage(Stage& S, const Pos& P, const SegGrid& g, int32_t c
         S.total = acc;
     }
     __syncthreads();
-    if (S.total > P.cap) return false;
-    // Copy column by column.  A column's window z0-1 .. z1 is one contiguous
-    // slot range unless it wraps in z; then its first (or last) cell comes
-    // from the other end of the column with a shift of -L (or +L).
-    for (int32_t col = 0; col < 9; col++) {
+    if (S.total + 1 > P.cap) return false;   // + the dummy atom
+    // copy: kColThreads threads per column, kCopyBatch independent loads each
+    if (tid < 9 * kColThreads) {
+        const int32_t col = tid / kColThreads, l = tid % kColThreads;
         int32_t x = cx + col / 3 - 1, y = cy + col % 3 - 1;
-        double sx = 0, sy = 0;
-        if (x < 0) { x += g.n[0]; sx = -g.L; } else if (x >= g.n[0]) { x -= g.n[0]; sx = g.L; }
-        if (y < 0) { y += g.n[1]; sy = -g.L; } else if (y >= g.n[1]) { y -= g.n[1]; sy = g.L; }
-        const int32_t cb = (x * g.n[1] + y) * g.n[2];
-        const int32_t za = z0 - 1, zb = z1;   // window cells, inclusive
-        // pieces: [za] wrapped, [max(za,0) .. min(zb,nz-1)] contiguous, [zb] wrapped
-        int32_t dst = S.colbase[col];
-        if (za < 0) {
-            const int32_t c = cb + g.n[2] - 1, b = g.start[c], e = g.start[c + 1];
-            for (int32_t q = tid; q < e - b; q += kBlock) {
-                P.x[dst + q] = rx[b + q] + sx;
-                P.y[dst + q] = ry[b + q] + sy;
-                P.z[dst + q] = rz[b + q] - g.L;
+        const double sx = x < 0 ? -g.L : (x >= g.n[0] ? g.L : 0.0);
+        const double sy = y < 0 ? -g.L : (y >= g.n[1] ? g.L : 0.0);
+        const int32_t l0 = S.plen[col][0], l1 = S.plen[col][1], l2 = S.plen[col][2];
+        const int32_t s0 = S.psrc[col][0], s1 = S.psrc[col][1], s2 = S.psrc[col][2];
+        const int32_t len = l0 + l1 + l2, dst = S.colbase[col];
+        for (int32_t w0 = l; w0 < len; w0 += kCopyBatch * kColThreads) {
+            double bx[kCopyBatch], by[kCopyBatch], bz[kCopyBatch];
+#pragma unroll
+            for (int u = 0; u < kCopyBatch; u++) {
+                const int32_t w = w0 + u * kColThreads;
+                if (w < len) {
+                    int32_t src;
+                    double sz;
+                    if (w < l0) { src = s0 + w; sz = -g.L; }
+                    else if (w < l0 + l1) { src = s1 + (w - l0); sz = 0.0; }
+                    else { src = s2 + (w - l0 - l1); sz = g.L; }
+                    bx[u] = rx[src] + sx;
+                    by[u] = ry[src] + sy;
+                    bz[u] = rz[src] + sz;
+                }
             }
-            dst += e - b;
-        }
-        {
-            const int32_t lo = za < 0 ? 0 : za, hi = zb >= g.n[2] ? g.n[2] - 1 : zb;
-            const int32_t b = g.start[cb + lo], e = g.start[cb + hi + 1];
-            for (int32_t q = tid; q < e - b; q += kBlock) {
-                P.x[dst + q] = rx[b + q] + sx;
-                P.y[dst + q] = ry[b + q] + sy;
-                P.z[dst + q] = rz[b + q];
-            }
-            dst += e - b;
-        }
-        if (zb >= g.n[2]) {
-            const int32_t b = g.start[cb], e = g.start[cb + 1];
-            for (int32_t q = tid; q < e - b; q += kBlock) {
-                P.x[dst + q] = rx[b + q] + sx;
-                P.y[dst + q] = ry[b + q] + sy;
-                P.z[dst + q] = rz[b + q] + g.L;
+#pragma unroll
+            for (int u = 0; u < kCopyBatch; u++) {
+                const int32_t w = w0 + u * kColThreads;
+                if (w < len) P.set(dst + w, bx[u], by[u], bz[u]);
             }
         }
     }
+    if (tid == kBlock - 1) P.set(S.total, kFar, kFar, kFar);
     __syncthreads();
     return true;
 }
 
-__global__ void __launch_bounds__(kBlock)
+// window cell (1..K) of home atom number qc of the segment
+__device__ inline int32_t home_cell(const Stage& S, int32_t qc) {
+    int32_t o = 1;
+    while (S.cellpre[4][o + 1] - S.cellpre[4][1] <= qc) o++;
+    return o;
+}
+
+__global__ void __launch_bounds__(kBlock, 8)
 k_nlist_lds(SegGrid g, const double* __restrict__ rx, const double* __restrict__ ry,
             const double* __restrict__ rz, double rl2, int32_t n, int32_t cap, uint16_t* __restrict__ list,
-            int32_t* __restrict__ count, int32_t* __restrict__ err) {
+            int32_t* __restrict__ count, int32_t* __restrict__ err, int32_t lds_cap) {
     __shared__ Stage S;
-    __shared__ double buf[3 * kLdsAtoms];
-    const int32_t seg = logical_segment(g);
-    if (seg >= g.total) return;
-    const Pos P{buf, buf + kLdsAtoms, buf + 2 * kLdsAtoms, kLdsAtoms};
-    const int32_t col = seg / g.nseg, szi = seg % g.nseg;
-    const int32_t cx = col / g.n[1], cy = col % g.n[1];
-    int32_t z0, z1;
-    seg_z(szi, g.nseg, g.n[2], z0, z1);
+    extern __shared__ double dyn[];
+    int32_t cx, cy, z0, z1;
+    if (seg_geometry(g, cx, cy, z0, z1) < 0) return;
+    const Pos P{dyn, lds_cap};
     if (!stage(S, P, g, cx, cy, z0, z1, rx, ry, rz)) {
-        if (threadIdx.x == 0) atomicMax(&err[0], S.total);
+        if (threadIdx.x == 0) atomicMax(&err[0], S.total + 1);
         return;
     }
-    if (threadIdx.x == 0) atomicMax(&err[2], S.total);   // largest staging, sizes the step kernel's LDS
+    // largest staging (with the dummy atom), sizes the step kernel's LDS
+    if (threadIdx.x == 0) atomicMax(&err[2], S.total + 1);
     const int32_t cbase = (cx * g.n[1] + cy) * g.n[2];
     const int32_t hbeg = g.start[cbase + z0], hend = g.start[cbase + z1];
-    for (int32_t s = hbeg + threadIdx.x; s < hend; s += kBlock) {
-        // window cell o (1..K) of the atom and its own LDS index
+    const int32_t q = threadIdx.x & (kLanes - 1);
+    for (int32_t base = hbeg; base < hend; base += kApp) {
+        const int32_t s = base + threadIdx.x / kLanes;
+        if (s >= hend) continue;             // whole quads drop out together
         const int32_t qc = s - hbeg;
-        int32_t o = 1;
-        while (S.cellpre[4][o + 1] - S.cellpre[4][1] <= qc) o++;
+        const int32_t o = home_cell(S, qc);
         const int32_t self = S.colbase[4] + S.cellpre[4][1] + qc;
-        const double xi = P.x[self], yi = P.y[self], zi = P.z[self];
-        int32_t k = 0;
+        const double xi = P.x(self), yi = P.y(self), zi = P.z(self);
+        int32_t k = 0;                        // hits so far, the same in all 4 lanes
         for (int32_t c = 0; c < 9; c++) {
             const int32_t b = S.colbase[c] + S.cellpre[c][o - 1], e = S.colbase[c] + S.cellpre[c][o + 2];
-            for (int32_t p0 = b; p0 < e; p0 += kBatch) {
-                bool hit[kBatch];
+            for (int32_t p0 = b; p0 < e; p0 += kLanes * kScan) {
+                // lane q tests candidates p0 + kLanes u + q
+                uint32_t mask = 0;
 #pragma unroll
-                for (int u = 0; u < kBatch; u++) {
-                    const int32_t p = p0 + u < e ? p0 + u : b;
-                    const double dx = xi - P.x[p], dy = yi - P.y[p], dz = zi - P.z[p];
-                    hit[u] = (p0 + u < e) && (p0 + u != self) && (dx * dx + dy * dy + dz * dz < rl2);
+                for (int u = 0; u < kScan; u++) {
+                    const int32_t p = p0 + kLanes * u + q;
+                    const int32_t pp = p < e ? p : b;
+                    const double dx = xi - P.x(pp), dy = yi - P.y(pp), dz = zi - P.z(pp);
+                    const bool hit = (p < e) && (p != self) && (dx * dx + dy * dy + dz * dz < rl2);
+                    mask |= (uint32_t)hit << u;
                 }
+                // hits are numbered lane-major within the quad: exclusive prefix of the counts
+                const int32_t h = __popc(mask);
+                int32_t incl = h;
+                int32_t t = __shfl_up(incl, 1, kLanes);
+                incl += q >= 1 ? t : 0;
+                t = __shfl_up(incl, 2, kLanes);
+                incl += q >= 2 ? t : 0;
+                const int32_t total = __shfl(incl, kLanes - 1, kLanes);
+                int32_t kk = k + incl - h;
 #pragma unroll
-                for (int u = 0; u < kBatch; u++) {
-                    if (hit[u]) {
-                        if (k < cap) list[(int64_t)k * n + s] = (uint16_t)(p0 + u);
-                        k++;
+                for (int u = 0; u < kScan; u++) {
+                    if ((mask >> u) & 1u) {
+                        if (kk < cap) {
+                            // entry kk -> lane kk % 4, that lane's entry kk / 4
+                            const int32_t ln = kk & (kLanes - 1), jj = kk / kLanes;
+                            const int64_t at = (((int64_t)(jj / kChunk) * n + s) * kLanes + ln) * kChunk + jj % kChunk;
+                            list[at] = (uint16_t)(p0 + kLanes * u + q);
+                        }
+                        kk++;
                     }
                 }
+                k += total;
             }
         }
-        count[s] = k < cap ? k : cap;
-        if (k > cap) atomicMax(&err[1], k);
+        const int32_t kc = k < cap ? k : cap;
+        if (q == 0) {
+            count[s] = kc;
+            if (k > cap) atomicMax(&err[1], k);
+        }
+        // pad the rest of this lane's last chunk with the dummy atom, so the
+        // step kernel evaluates whole chunks without a validity test
+        const int32_t mq = (kc - q + kLanes - 1) / kLanes;
+        const int32_t mend = (mq + kChunk - 1) / kChunk * kChunk;
+        for (int32_t jj = mq; jj < mend; jj++) {
+            const int64_t at = (((int64_t)(jj / kChunk) * n + s) * kLanes + q) * kChunk + jj % kChunk;
+            list[at] = (uint16_t)S.total;
+        }
     }
 }
 
@@ -208,29 +300,45 @@ struct StepArgs {
     const uint16_t* list;
     const int32_t* count;
     double rc2;
-    float inv_sig2f, coeff_f;
+    float f12, f6;          // F/r = x^4 (f12 x^3 + f6), x = 1/r^2: 48 eps sig^12, -24 eps sig^6
     double sig2, coeff;
     double mass, scale;
     GearCoeffs gc;
     double hard2;                // list validity: displacement^2 must stay below this
     unsigned long long* flags;   // kSpread x {max disp^2 bits, max step^2 bits, stale count}
+    int64_t cs;                  // component stride: component d of a vector at [0] + d * cs
     int32_t n;
     int32_t lds_cap;             // staged atoms the dynamic LDS holds
 };
 
+// a double from the lane given by a quad permutation (DPP, no LDS traffic)
+template <int kCtrl>
+__device__ inline double quad_perm(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), kCtrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), kCtrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+// sum over the 4 lanes of a quad, (v0 + v1) + (v2 + v3): every lane gets the
+// same bits.  Quad permutations [1,0,3,2] = 0xB1 and [2,3,0,1] = 0x4E.
+__device__ inline double quad_sum(double v) {
+    v += quad_perm<0xB1>(v);
+    v += quad_perm<0x4E>(v);
+    return v;
+}
+
+__device__ inline double pick3(int32_t q, double a, double b, double c) { return q == 0 ? a : (q == 1 ? b : c); }
+
+// 8 waves per SIMD (64 VGPRs) for the mixed body; the FP64 body's divisions
+// need more registers, so it runs at 4
 template <bool kFp64, bool kFinal, bool kForceOut>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, kFp64 ? 4 : 8)
 k_step_lds(StepArgs a) {
     __shared__ Stage S;
     extern __shared__ double dyn[];
     const SegGrid& g = a.g;
-    const int32_t seg = logical_segment(g);
-    if (seg >= g.total) return;
-    const Pos P{dyn, dyn + a.lds_cap, dyn + 2 * a.lds_cap, a.lds_cap};
-    const int32_t col = seg / g.nseg, szi = seg % g.nseg;
-    const int32_t cx = col / g.n[1], cy = col % g.n[1];
-    int32_t z0, z1;
-    seg_z(szi, g.nseg, g.n[2], z0, z1);
+    int32_t cx, cy, z0, z1;
+    if (seg_geometry(g, cx, cy, z0, z1) < 0) return;
+    const Pos P{dyn, a.lds_cap};
     unsigned long long* fl = a.flags + 3 * (blockIdx.x % kSpread);
     if (!stage(S, P, g, cx, cy, z0, z1, a.st.r[0], a.st.r[1], a.st.r[2])) {
         // cannot happen after a successful list build on the same cells; report it as stale
@@ -239,63 +347,75 @@ k_step_lds(StepArgs a) {
     }
     const int32_t cbase = (cx * g.n[1] + cy) * g.n[2];
     const int32_t hbeg = g.start[cbase + z0], hend = g.start[cbase + z1];
+    const int32_t q = threadIdx.x & (kLanes - 1);
     double d2max = 0.0, mv2max = 0.0, stale = 0.0;
     const int32_t n = a.n;
-    for (int32_t i = hbeg + threadIdx.x; i < hend; i += kBlock) {
-        const double xi = a.st.r[0][i], yi = a.st.r[1][i], zi = a.st.r[2][i];
+    for (int32_t base = hbeg; base < hend; base += kApp) {
+        const int32_t i = base + threadIdx.x / kLanes;
+        if (i >= hend) continue;              // whole quads drop out together
+        // r(t) of the home atom: its staged copy (centre column, never shifted)
+        const int32_t self = S.colbase[4] + S.cellpre[4][1] + (i - hbeg);
+        const double xi = P.x(self), yi = P.y(self), zi = P.z(self);
         double fx = 0, fy = 0, fz = 0;
-        const int32_t cnt = a.count[i];
-        const uint16_t* e = a.list + i;
-        // list entries are loaded kBatch ahead so several loads are in flight;
-        // the list has 2 kBatch spare rows, so the look-ahead stays in bounds
-        int32_t jn[kBatch];
+        const int32_t m = (a.count[i] - q + kLanes - 1) / kLanes;   // this lane's entries
+        // chunks of this lane are (n * kLanes) uint4 apart; the next chunk is
+        // loaded while the current one is evaluated (the list has one spare
+        // chunk per lane, so the look-ahead stays in bounds); the last chunk is
+        // padded with the dummy atom
+        const uint4* lp = reinterpret_cast<const uint4*>(a.list) + (int64_t)i * kLanes + q;
+        const int64_t cstride = (int64_t)n * kLanes;
+        uint4 nx = lp[0];
+        for (int32_t k0 = 0; k0 < m; k0 += kChunk) {
+            const uint4 cu = nx;
+            lp += cstride;
+            // the empty asm hides that this address is the next iteration's
+            // chunk, so the compiler keeps the load here (one chunk ahead)
+            // instead of merging it into a load at the top of the next iteration
+            uint64_t pa = reinterpret_cast<uint64_t>(lp);
+            asm volatile("" : "+v"(pa));
+            nx = *reinterpret_cast<const uint4*>(pa);
+            const uint32_t wv[4] = {cu.x, cu.y, cu.z, cu.w};
+            // straight-line body over the whole chunk: entries past the lane's
+            // count are the dummy atom, which fails the cutoff test (a select,
+            // so the 1/0 of its force law never reaches the sum)
 #pragma unroll
-        for (int u = 0; u < kBatch; u++) jn[u] = e[(int64_t)u * n];
-        for (int32_t k0 = 0; k0 < cnt; k0 += kBatch) {
-            int32_t jc[kBatch];
-#pragma unroll
-            for (int u = 0; u < kBatch; u++) {
-                jc[u] = jn[u];
-                jn[u] = e[(int64_t)(k0 + kBatch + u) * n];
-            }
-            // straight-line body: every entry of the batch is evaluated and
-            // masked (select, so a 1/0 of a padding entry never reaches the sum),
-            // letting the 8 LDS loads and FP64 chains overlap
-#pragma unroll
-            for (int u = 0; u < kBatch; u++) {
-                const bool valid = k0 + u < cnt;
-                const int32_t j = valid ? jc[u] : 0;
-                const double ddx = xi - P.x[j], ddy = yi - P.y[j], ddz = zi - P.z[j];
+            for (int u = 0; u < kChunk; u++) {
+                const int32_t j = (int32_t)((wv[u >> 1] >> (16 * (u & 1))) & 0xffffu);
+                const double ddx = xi - P.x(j), ddy = yi - P.y(j), ddz = zi - P.z(j);
                 if constexpr (kFp64) {
                     const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    const double q = r2 / a.sig2;
-                    const double iq4 = 1.0 / (q * q * q * q);
-                    const double iq7 = iq4 / (q * q * q);
-                    const double fr = (valid && r2 < a.rc2) ? (2 * iq7 - iq4) * a.coeff : 0.0;
+                    const double qq = r2 / a.sig2;
+                    const double iq4 = 1.0 / (qq * qq * qq * qq);
+                    const double iq7 = iq4 / (qq * qq * qq);
+                    const double fr = r2 < a.rc2 ? (2 * iq7 - iq4) * a.coeff : 0.0;
                     fx += ddx * fr; fy += ddy * fr; fz += ddz * fr;
                 } else {
                     const double r2 = __builtin_fma(ddx, ddx, __builtin_fma(ddy, ddy, ddz * ddz));
-                    // w = (sig/r)^2; F/r = coeff w^4 (2 w^3 - 1)
-                    const float w = __builtin_amdgcn_rcpf((float)r2 * a.inv_sig2f);
-                    const float w2 = w * w;
-                    const float w3 = w2 * w;
-                    const float w4 = w2 * w2;
-                    const float frf = w4 * __builtin_fmaf(2.0f, w3, -1.0f) * a.coeff_f;
-                    const double fr = (valid && r2 < a.rc2) ? (double)frf : 0.0;
+                    const float x = __builtin_amdgcn_rcpf((float)r2);
+                    const float x2 = x * x;
+                    const float x3 = x2 * x;
+                    // the cutoff masks x^4 (one 32-bit select) before the product
+                    const float x4 = r2 < a.rc2 ? x2 * x2 : 0.0f;
+                    const double fr = (double)(x4 * __builtin_fmaf(a.f12, x3, a.f6));
                     fx = __builtin_fma(fr, ddx, fx);
                     fy = __builtin_fma(fr, ddy, fy);
                     fz = __builtin_fma(fr, ddz, fz);
                 }
             }
         }
-        const double f[3] = {fx, fy, fz};
-        const double rt[3] = {xi, yi, zi};
-        double d2 = 0.0, mv2 = 0.0;
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            double r = rt[d], v = a.st.v[d][i], ac = a.st.a[d][i], b = a.st.b[d][i], c = a.st.c[d][i];
+        fx = quad_sum(fx);
+        fy = quad_sum(fy);
+        fz = quad_sum(fz);
+        // lane q < 3 integrates component q
+        double d2 = 0.0, mv2 = 0.0, e2 = 0.0;
+        if (q < 3) {
+            const double fd = pick3(q, fx, fy, fz);
+            const double rt = pick3(q, xi, yi, zi);
+            const int64_t o = q * a.cs + i;
+            const double r0 = a.r0[0][o];
+            double r = rt, v = a.st.v[0][o], ac = a.st.a[0][o], b = a.st.b[0][o], c = a.st.c[0][o];
             // corrector (cpu/common/Device.cpp:58-84)
-            const double anew = f[d] / a.mass * a.scale;
+            const double anew = fd / a.mass * a.scale;
             const double corr = anew - ac;
             r = r + a.gc.kc[0] * corr;
             v = v + a.gc.kc[1] * corr;
@@ -308,48 +428,44 @@ k_step_lds(StepArgs a) {
                 v = v + a.gc.k[0] * ac + a.gc.k[1] * b + a.gc.k[2] * c;
                 ac = ac + a.gc.k[0] * b + a.gc.k[1] * c;
                 b = b + a.gc.k[0] * c;
-                const double dd = rp - a.r0[d][i], ds = rp - rt[d];
-                d2 += dd * dd;
-                mv2 += ds * ds;
+                const double dd = rp - r0, ds = rp - rt;
+                d2 = dd * dd;
+                mv2 = ds * ds;
                 r = rp;
             }
-            a.rn[d][i] = r;
-            a.st.v[d][i] = v;
-            a.st.a[d][i] = ac;
-            a.st.b[d][i] = b;
-            a.st.c[d][i] = c;
-            if constexpr (kForceOut) a.st.f[d][i] = f[d];
+            a.rn[0][o] = r;
+            a.st.v[0][o] = v;
+            a.st.a[0][o] = ac;
+            a.st.b[0][o] = b;
+            a.st.c[0][o] = c;
+            if constexpr (kForceOut) a.st.f[0][o] = fd;
+            // the force just used is valid only if r(t) was within the list margin
+            const double e = rt - r0;
+            e2 = e * e;
         }
-        // the force just used is valid only if r(t) was within the list margin
-        const double e0 = xi - a.r0[0][i], e1 = yi - a.r0[1][i], e2 = zi - a.r0[2][i];
-        if (e0 * e0 + e1 * e1 + e2 * e2 >= a.hard2) stale += 1.0;
+        d2 = quad_sum(d2);
+        mv2 = quad_sum(mv2);
+        e2 = quad_sum(e2);
+        if (q == 0 && e2 >= a.hard2) stale += 1.0;
         d2max = fmax(d2max, d2);
         mv2max = fmax(mv2max, mv2);
     }
-    // block reduction (wave shuffles, then LDS), one spread atomic per block
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-        d2max = fmax(d2max, __shfl_xor(d2max, o, kWave));
-        mv2max = fmax(mv2max, __shfl_xor(mv2max, o, kWave));
-        stale += __shfl_xor(stale, o, kWave);
-    }
-    const int w = threadIdx.x / kWave;
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-        S.red[w][0] = d2max;
-        S.red[w][1] = mv2max;
-        S.red[w][2] = stale;
+    // block reduction through LDS atomics (non-negative doubles order like
+    // their bits), then one spread global atomic per block
+    if (q == 0) {
+        if (!kFinal) {
+            atomicMax(&S.red[0], (unsigned long long)__double_as_longlong(d2max));
+            atomicMax(&S.red[1], (unsigned long long)__double_as_longlong(mv2max));
+        }
+        if (stale > 0) atomicAdd(&S.red[2], (unsigned long long)stale);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int q = 1; q < kBlock / kWave; q++) {
-            d2max = fmax(d2max, S.red[q][0]);
-            mv2max = fmax(mv2max, S.red[q][1]);
-            stale += S.red[q][2];
-        }
         if (!kFinal) {
-            atomicMax(&fl[0], (unsigned long long)__double_as_longlong(d2max));
-            atomicMax(&fl[1], (unsigned long long)__double_as_longlong(mv2max));
+            atomicMax(&fl[0], S.red[0]);
+            atomicMax(&fl[1], S.red[1]);
         }
-        if (stale > 0) atomicAdd(&fl[2], (unsigned long long)stale);
+        if (S.red[2] > 0) atomicAdd(&fl[2], S.red[2]);
     }
 }
 
@@ -359,53 +475,74 @@ SegGrid make_grid(const GridView& v, int32_t nseg) {
     g.nseg = nseg;
     g.total = v.g[0] * v.g[1] * nseg;
     g.per_xcd = (g.total + kXcd - 1) / kXcd;
+    g.rnseg = 1.0f / (float)nseg;
+    g.rny = 1.0f / (float)v.g[1];
+    g.small = g.total < (1 << 22) && (int64_t)nseg * v.g[2] < (1 << 22);
     g.L = v.L;
     g.start = v.start;
     return g;
 }
 
+template <typename K>
+void allow_lds(K kernel) {
+    // dynamic LDS above the 64 KiB default, up to the largest staging
+    (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              3 * kLdsAtoms * (int)sizeof(double));
+}
+
 }  // namespace
 
 int32_t lds_segments_per_column(int32_t nz, double atoms_per_cell, int32_t kmax) {
-    // about 230 home atoms per 256-thread workgroup, at most kmax cells per segment
+    // about 0.85 x 128 home atoms per segment (one pass of the workgroup's
+    // quads), at most kmax cells per segment
     kmax = kmax < kMaxK ? kmax : kMaxK;
     kmax = kmax < 1 ? 1 : kmax;
     const double apc = atoms_per_cell > 0 ? atoms_per_cell : 1.0;
-    int32_t nseg = (int32_t)(nz * apc / 230.0 + 0.5);
+    int32_t nseg = (int32_t)std::ceil(nz * apc / (0.85 * kApp));
     const int32_t lo = (nz + kmax - 1) / kmax;   // segments no longer than kmax cells
     nseg = nseg < lo ? lo : nseg;
     return nseg > nz ? nz : nseg;
 }
 
 int32_t lds_capacity() { return kLdsAtoms; }
-int32_t list_spare_rows() { return 2 * kBatch; }
+int32_t list_spare_rows() { return kLanes * kChunk; }
+int32_t list_cap_multiple() { return kLanes * kChunk; }
 int32_t flag_spread() { return kSpread; }
 
 hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg, double rl2,
-                            int32_t n, int32_t cap, uint16_t* list, int32_t* count, int32_t* err) {
+                            int32_t n, int32_t cap, uint16_t* list, int32_t* count, int32_t* err,
+                            int32_t lds_atoms) {
     if (n <= 0) return hipSuccess;
+    if (cap % (kLanes * kChunk) != 0) return hipErrorInvalidValue;
+    if (lds_atoms < 1 || lds_atoms > kLdsAtoms) return hipErrorInvalidValue;
+    static bool attr_set = false;
+    if (!attr_set) {
+        allow_lds(k_nlist_lds);
+        attr_set = true;
+    }
     SegGrid g = make_grid(v, nseg);
-    k_nlist_lds<<<g.per_xcd * kXcd, kBlock, 0, s>>>(g, st.r[0], st.r[1], st.r[2], rl2, n, cap, list, count, err);
+    const size_t dyn = 3 * (size_t)lds_atoms * sizeof(double);
+    k_nlist_lds<<<g.per_xcd * kXcd, kBlock, dyn, s>>>(g, st.r[0], st.r[1], st.r[2], rl2, n, cap, list, count, err,
+                                                      lds_atoms);
     return hipGetLastError();
 }
 
 hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg,
                            double* const rn[3], double* const r0[3], const uint16_t* list, const int32_t* count,
                            const LjTable& t, const MassTable& m, const GearCoeffs& gc, double hard2, int32_t n,
-                           int32_t precision, bool final_step, bool force_out, uint64_t* flags, int32_t lds_atoms) {
+                           int64_t cs, int32_t precision, bool final_step, bool force_out, uint64_t* flags,
+                           int32_t lds_atoms) {
     if (n <= 0) return hipSuccess;
     static bool attr_set = false;
     if (!attr_set) {
-        // allow dynamic LDS above the 64 KiB default for the largest staging
-        const int bytes = 3 * kLdsAtoms * (int)sizeof(double);
-        (void)hipFuncSetAttribute((const void*)k_step_lds<true, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        (void)hipFuncSetAttribute((const void*)k_step_lds<true, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        (void)hipFuncSetAttribute((const void*)k_step_lds<true, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        (void)hipFuncSetAttribute((const void*)k_step_lds<true, false, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        (void)hipFuncSetAttribute((const void*)k_step_lds<false, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        (void)hipFuncSetAttribute((const void*)k_step_lds<false, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        (void)hipFuncSetAttribute((const void*)k_step_lds<false, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        (void)hipFuncSetAttribute((const void*)k_step_lds<false, false, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        allow_lds(k_step_lds<true, true, true>);
+        allow_lds(k_step_lds<true, true, false>);
+        allow_lds(k_step_lds<true, false, true>);
+        allow_lds(k_step_lds<true, false, false>);
+        allow_lds(k_step_lds<false, true, true>);
+        allow_lds(k_step_lds<false, true, false>);
+        allow_lds(k_step_lds<false, false, true>);
+        allow_lds(k_step_lds<false, false, false>);
         attr_set = true;
     }
     if (lds_atoms < 1 || lds_atoms > kLdsAtoms) return hipErrorInvalidValue;
@@ -416,8 +553,10 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
     a.list = list;
     a.count = count;
     a.rc2 = t.rc2;
-    a.inv_sig2f = t.inv_sig2f[0];
-    a.coeff_f = t.coeff_f[0];
+    // coeff = 24 eps / sig^2: 48 eps sig^12 = 2 coeff sig^14, -24 eps sig^6 = -coeff sig^8
+    const double s4 = t.sig2[0] * t.sig2[0] * t.sig2[0] * t.sig2[0];
+    a.f12 = (float)(2.0 * t.coeff[0] * s4 * t.sig2[0] * t.sig2[0] * t.sig2[0]);
+    a.f6 = (float)(-t.coeff[0] * s4);
     a.sig2 = t.sig2[0];
     a.coeff = t.coeff[0];
     a.mass = m.mass[0];
@@ -425,6 +564,11 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
     a.gc = gc;
     a.hard2 = hard2;
     a.flags = reinterpret_cast<unsigned long long*>(flags);
+    a.cs = cs;
+    for (int d = 1; d < 3; d++)
+        if (st.v[d] != st.v[0] + d * cs || st.r[d] != st.r[0] + d * cs || rn[d] != rn[0] + d * cs ||
+            r0[d] != r0[0] + d * cs)
+            return hipErrorInvalidValue;
     a.n = n;
     a.lds_cap = lds_atoms;
     const size_t dyn = 3 * (size_t)lds_atoms * sizeof(double);

@@ -89,8 +89,10 @@ hipError_t launch_exclusive_scan(hipStream_t s, const int32_t* in, int32_t* out,
 int64_t scan_block_sums_needed(int64_t n);
 hipError_t launch_scatter(hipStream_t s, const int32_t* cell_of, const int32_t* start,
                           int32_t* fill, int32_t* sorted_items, int32_t items);
-hipError_t launch_sort_cells(hipStream_t s, const int32_t* start, int64_t ncell,
-                             int32_t* sorted_items, const int32_t* key_of_item);
+// items_in grouped by cell -> items_out with ascending key inside each cell
+// (key_of_item null: the item is its own key)
+hipError_t launch_sort_cells(hipStream_t s, const int32_t* start, int64_t ncell, const int32_t* items_in,
+                             int32_t* items_out, const int32_t* key_of_item);
 hipError_t launch_permute(hipStream_t s, const DevState& src, const DevState& dst,
                           const int32_t* sorted_src, const int32_t* cell_of_src,
                           int32_t* cell_of_dst, int32_t* slot_of_id, double* const rref[3],
@@ -139,14 +141,18 @@ int32_t lds_segments_per_column(int32_t nz, double atoms_per_cell, int32_t kmax)
 int32_t lds_capacity();
 int32_t flag_spread();
 int32_t list_spare_rows();   // rows the list needs beyond its capacity (look-ahead loads)
-// err[0]: staged atoms of a segment exceeded lds_capacity(); err[1]: longest list above
+int32_t list_cap_multiple(); // the list capacity (entries per atom) must be a multiple of this
+// err[0]: staged atoms of a segment exceeded lds_atoms; err[1]: longest list above
 // cap; err[2]: the largest staging (atoms), which sizes the step kernel's LDS
 hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg, double rl2,
-                            int32_t n, int32_t cap, uint16_t* list, int32_t* count, int32_t* err);
+                            int32_t n, int32_t cap, uint16_t* list, int32_t* count, int32_t* err,
+                            int32_t lds_atoms);
+// st, rn and r0 hold each vector's three components in one block, component d
+// at [0] + d * cs
 hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg,
                            double* const rn[3], double* const r0[3], const uint16_t* list, const int32_t* count,
                            const LjTable& t, const MassTable& m, const GearCoeffs& gc, double hard2, int32_t n,
-                           int32_t precision, bool final_step, bool force_out, uint64_t* flags,
+                           int64_t cs, int32_t precision, bool final_step, bool force_out, uint64_t* flags,
                            int32_t lds_atoms);
 
 // bond_ref.hip

@@ -79,12 +79,13 @@ struct mdx_system {
     double disp_limit2 = 0;          // periodic: rebuild at max displacement^2 >= this
     // periodic fused path
     double* rnext[3] = {nullptr, nullptr, nullptr};  // r(t+1) written by the step kernel
-    uint16_t* nlist = nullptr;       // column-major neighbour list of LDS indices
+    uint16_t* nlist = nullptr;       // chunked per-lane neighbour list of LDS indices
     int32_t* ncount = nullptr;
     int32_t nl_cap = 0;
     int32_t nseg = 0;                // segments per cell column
     int32_t seg_kmax = 1 << 20;      // cap on cells per segment (lowered if staging overflows)
     int32_t lds_atoms = 0;           // largest segment staging of the last build
+    int32_t build_lds = 0;           // staging capacity the list build runs with
     static constexpr int kRing = 3;
     static constexpr int kFlagWords = 3 * 64;  // flag_spread() x {disp^2, step^2, stale}
     uint64_t* dflags = nullptr;      // kRing x kFlagWords
@@ -122,6 +123,19 @@ template <class T> int dalloc(mdx_system* s, T** p, size_t count) {
         return fail(s, MDX_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
     }
     s->allocs.push_back(*p);
+    return MDX_OK;
+}
+
+// three components of one per-atom vector quantity in one block: component d
+// at p[0] + d * component_stride(n), so a kernel can address component d by an
+// offset instead of selecting a pointer
+int64_t component_stride(int n) { return ((int64_t)n + 63) / 64 * 64; }
+int dalloc3(mdx_system* s, double* p[3], int n) {
+    const int64_t cs = component_stride(n);
+    int rc = dalloc(s, &p[0], 3 * (size_t)cs);
+    if (rc) return rc;
+    p[1] = p[0] + cs;
+    p[2] = p[0] + 2 * cs;
     return MDX_OK;
 }
 
@@ -236,13 +250,10 @@ void setup_tables(mdx_system* s) {
 
 int alloc_state(mdx_system* s, DevState& d) {
     int n = s->n;
-    for (int q = 0; q < 3; q++) {
-        int rc;
-        if ((rc = dalloc(s, &d.r[q], n)) || (rc = dalloc(s, &d.v[q], n)) || (rc = dalloc(s, &d.a[q], n)) ||
-            (rc = dalloc(s, &d.b[q], n)) || (rc = dalloc(s, &d.c[q], n)) || (rc = dalloc(s, &d.f[q], n)))
-            return rc;
-    }
     int rc;
+    if ((rc = dalloc3(s, d.r, n)) || (rc = dalloc3(s, d.v, n)) || (rc = dalloc3(s, d.a, n)) ||
+        (rc = dalloc3(s, d.b, n)) || (rc = dalloc3(s, d.c, n)) || (rc = dalloc3(s, d.f, n)))
+        return rc;
     if ((rc = dalloc(s, &d.type, n)) || (rc = dalloc(s, &d.id, n))) return rc;
     return MDX_OK;
 }
@@ -335,8 +346,10 @@ int build_primary(mdx_system* s, Grid& g, float rs, double box_size_list, bool s
     LAUNCH(s, "cells_bin", launch_bin(st, s->st, nullptr, n, g.view, s->cell_of, g.count, s->dint + 1));
     LAUNCH(s, "cells_scan", launch_exclusive_scan(st, g.count, g.start, g.ncell, s->scan_sums, s->scan_cap));
     LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
-    LAUNCH(s, "cells_scatter", launch_scatter(st, s->cell_of, g.start, g.count, s->sorted, n));
-    LAUNCH(s, "cells_sort", launch_sort_cells(st, g.start, g.ncell, s->sorted, s->st.id));
+    // scatter into scratch (cell_by_id is free until the sizes pass), then
+    // order each cell by id into sorted
+    LAUNCH(s, "cells_scatter", launch_scatter(st, s->cell_of, g.start, g.count, s->cell_by_id, n));
+    LAUNCH(s, "cells_sort", launch_sort_cells(st, g.start, g.ncell, s->cell_by_id, s->sorted, s->st.id));
     const bool periodic = s->cfg.box == MDX_BOX_PERIODIC;
     double* none[3] = {nullptr, nullptr, nullptr};
     LAUNCH(s, "cells_permute",
@@ -372,8 +385,8 @@ int build_bond_walled(mdx_system* s, double box_size_list) {
     LAUNCH(s, "cells_bin", launch_bin(st, s->st, s->slot_of_id, nb, g.view, s->mb_cell_of, g.count, s->dint + 1));
     LAUNCH(s, "cells_scan", launch_exclusive_scan(st, g.count, g.start, g.ncell, s->scan_sums, s->scan_cap));
     LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
-    LAUNCH(s, "cells_scatter", launch_scatter(st, s->mb_cell_of, g.start, g.count, s->mb_ids, nb));
-    LAUNCH(s, "cells_sort", launch_sort_cells(st, g.start, g.ncell, s->mb_ids, nullptr));
+    LAUNCH(s, "cells_scatter", launch_scatter(st, s->mb_cell_of, g.start, g.count, s->cell_by_id, nb));
+    LAUNCH(s, "cells_sort", launch_sort_cells(st, g.start, g.ncell, s->cell_by_id, s->mb_ids, nullptr));
     int32_t e;
     if ((rc = read_ints(s, &e, s->dint + 1, 1))) return rc;
     if (e) return fail(s, MDX_ERR_ARG, "an atom lies outside the list box (box_size_list too small)");
@@ -515,26 +528,37 @@ int rebuild_periodic(mdx_system* s) {
     const double rl2 = (double)c.lj_rs * (double)c.lj_rs;
     const GridView& v = s->lj.view;
     const double per_cell = (double)n / ((double)v.g[0] * v.g[1] * v.g[2]);
+    const int32_t cm = list_cap_multiple();
     if (!s->nlist) {
         // first guess from the density: 1.5 x the mean count inside rs, plus slack
         double rho = n / (c.box_length * c.box_length * c.box_length);
         double est = 4.0 / 3.0 * M_PI * std::pow((double)c.lj_rs, 3) * rho;
-        int32_t cap = c.max_neighbors > 0 ? c.max_neighbors : ((int32_t)(1.5 * est) + 32 + 31) / 32 * 32;
+        int32_t cap = c.max_neighbors > 0 ? c.max_neighbors : (int32_t)(1.5 * est) + 32;
+        cap = (cap + cm - 1) / cm * cm;
         if ((rc = dalloc(s, &s->nlist, (size_t)(cap + list_spare_rows()) * n)) || (rc = dalloc(s, &s->ncount, n)))
             return rc;
         s->nl_cap = cap;
     }
+    auto round64 = [](double a) { return ((int32_t)std::ceil(a) + 63) / 64 * 64; };
     for (;;) {
         s->nseg = lds_segments_per_column(v.g[2], per_cell, s->seg_kmax);
+        if (s->build_lds == 0) {
+            // staging estimate: 9 columns of K + 2 cells at 1.3 x the mean occupancy
+            const int32_t kcells = (v.g[2] + s->nseg - 1) / s->nseg;
+            s->build_lds = std::min(lds_capacity(), round64(9.0 * (kcells + 2) * per_cell * 1.3 + 64));
+        }
         HIPCHK(s, hipMemsetAsync(s->dint + 4, 0, 3 * sizeof(int32_t), s->stream));
         LAUNCH(s, "nlist_build", launch_nlist_lds(s->stream, s->st, v, s->nseg, rl2, n, s->nl_cap, s->nlist,
-                                                  s->ncount, s->dint + 4));
+                                                  s->ncount, s->dint + 4, s->build_lds));
         int32_t e[3];
         if ((rc = read_ints(s, e, s->dint + 4, 3))) return rc;
-        s->lds_atoms = (e[2] + 63) / 64 * 64;
-        if (s->lds_atoms > lds_capacity()) s->lds_atoms = lds_capacity();
+        s->lds_atoms = std::min(lds_capacity(), round64(e[2]));
         if (e[0] > 0) {
-            // a segment's neighbourhood did not fit in LDS: shorter segments
+            // a segment's neighbourhood did not fit: more LDS, else shorter segments
+            if (s->build_lds < lds_capacity()) {
+                s->build_lds = std::min(lds_capacity(), round64(1.1 * e[0]));
+                continue;
+            }
             int32_t k = (v.g[2] + s->nseg - 1) / s->nseg;
             if (k <= 1) return fail(s, MDX_ERR_CAPACITY, "cell neighbourhood exceeds the LDS staging capacity");
             s->seg_kmax = k - 1;
@@ -544,7 +568,7 @@ int rebuild_periodic(mdx_system* s) {
         // a neighbour list exceeded its capacity: grow and build again
         dfree(s, s->nlist);
         s->nlist = nullptr;
-        int32_t cap = (e[1] + e[1] / 4 + 31) / 32 * 32;
+        int32_t cap = (e[1] + e[1] / 4 + cm - 1) / cm * cm;
         if ((rc = dalloc(s, &s->nlist, (size_t)(cap + list_spare_rows()) * n))) return rc;
         s->nl_cap = cap;
     }
@@ -631,8 +655,8 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
         const bool fin = k + 1 == nsteps;
         HIPCHK(s, hipMemsetAsync(slot_flags(k), 0, W * sizeof(uint64_t), st));
         LAUNCH(s, "md_step", launch_step_lds(st, s->st, s->lj.view, s->nseg, s->rnext, s->rref, s->nlist, s->ncount,
-                                             s->ljt, s->mt, s->gc, hard2, n, c.precision, fin, fin, slot_flags(k),
-                                             s->lds_atoms));
+                                             s->ljt, s->mt, s->gc, hard2, n, component_stride(n), c.precision, fin,
+                                             fin, slot_flags(k), s->lds_atoms));
         if ((rc = post(k))) return rc;
         for (int d = 0; d < 3; d++) std::swap(s->st.r[d], s->rnext[d]);
         s->t++;
@@ -699,8 +723,7 @@ int mdx_create(const mdx_config* cfg, mdx_system** out) {
     if ((rc = dalloc(s, &s->slot_of_id, n)) || (rc = dalloc(s, &s->cell_of, n)) ||
         (rc = dalloc(s, &s->cell_of_alt, n)) || (rc = dalloc(s, &s->sorted, n)) ||
         (rc = dalloc(s, &s->cell_by_id, n)) || (rc = dalloc(s, &s->mb_cell_of, std::max(cfg->n_bond, 1))) ||
-        (rc = dalloc(s, &s->mb_ids, std::max(cfg->n_bond, 1))) || (rc = dalloc(s, &s->rref[0], n)) ||
-        (rc = dalloc(s, &s->rref[1], n)) || (rc = dalloc(s, &s->rref[2], n)) ||
+        (rc = dalloc(s, &s->mb_ids, std::max(cfg->n_bond, 1))) || (rc = dalloc3(s, s->rref, n)) ||
         (rc = dalloc(s, &s->e_atom, 2 * (size_t)n)) || (rc = dalloc(s, &s->dscal, 16)) ||
         (rc = dalloc(s, &s->dint, 16)) || (rc = dalloc(s, &s->dbits, 4))) {
         g_err = s->err;
@@ -713,8 +736,7 @@ int mdx_create(const mdx_config* cfg, mdx_system** out) {
         return bail(rc);
     }
     if (cfg->box == MDX_BOX_PERIODIC)
-        for (int d = 0; d < 3; d++)
-            if ((rc = dalloc(s, &s->rnext[d], n))) { g_err = s->err; return bail(rc); }
+        if ((rc = dalloc3(s, s->rnext, n))) { g_err = s->err; return bail(rc); }
     if (hipHostMalloc(&s->pinned, 256, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&s->hflags, sizeof(uint64_t) * mdx_system::kFlagWords * mdx_system::kRing,
                       hipHostMallocDefault) != hipSuccess)
