@@ -16,13 +16,15 @@
 // neighbours (34 KiB of LDS) leave room for 4 workgroups (32 waves) per CU:
 // enough waves to cover the HBM and LDS latencies of the gathers.
 //
-//  k_nlist_lds (at each rebuild): the quad scans the atom's candidates, 16 at
+//  k_nlist_lds (at each rebuild): the quad scans the atom's candidates, 32 at
 //    a time, and keeps those within the list radius rs.  Hit number k (in the
 //    quad's scan order) goes to lane k % 4 as that lane's entry k / 4, so the
 //    four lanes' lists differ in length by at most one.  Entries are 16-bit
-//    LDS indices; a lane's entries are packed 8 to a 16-byte chunk, and chunk
-//    c of (atom s, lane q) sits at chunk index (c * n + s) * 4 + q, so a
-//    wave's chunk loads cover one contiguous 1 KiB run.
+//    LDS indices; a lane's entries are packed 8 to a 16-byte chunk.  Atoms
+//    are grouped 16 to a block; block B holds its chunks c = 0 .. NC-1 one
+//    after the other, each as [16 atoms][4 lanes] 16-byte chunks (1 KiB), so a
+//    wave's chunk loads cover one or two contiguous runs and an atom's chunks
+//    are 1 KiB apart.
 //  k_step_lds (every step): LJ force over the list (FP64 displacement, FP32
 //    12-6 force law in the mixed mode, FP64 accumulation), summed over the
 //    quad, then the Gear corrector of step t and the predictor of step t+1,
@@ -42,7 +44,9 @@ constexpr int kBlock = 512;
 constexpr int kLanes = 4;                  // lanes per atom
 constexpr int kApp = kBlock / kLanes;      // atoms per pass
 constexpr int kChunk = 8;                  // 16-bit entries per 16-byte chunk
-constexpr int kScan = 4;                   // candidates per lane per scan step
+constexpr int kScan = 8;                   // candidates per lane per scan step
+constexpr int kListBlock = 16;             // atoms per list block
+constexpr int kChunkRow = kListBlock * kLanes * kChunk;   // 16-bit entries in one chunk of a block (512)
 constexpr int kXcd = 8;
 constexpr int kLdsAtoms = 3072;            // staged atoms per segment at most (3 x 24 KiB of FP64)
 constexpr int kMaxK = 14;                  // cells per segment
@@ -217,6 +221,7 @@ __global__ void __launch_bounds__(kBlock, 8)
 k_nlist_lds(SegGrid g, const double* __restrict__ rx, const double* __restrict__ ry,
             const double* __restrict__ rz, double rl2, int32_t n, int32_t cap, uint16_t* __restrict__ list,
             int32_t* __restrict__ count, int32_t* __restrict__ err, int32_t lds_cap) {
+    const int32_t nchunk = cap / (kLanes * kChunk) + 1;   // chunks per lane with the spare
     __shared__ Stage S;
     extern __shared__ double dyn[];
     int32_t cx, cy, z0, z1;
@@ -238,40 +243,46 @@ k_nlist_lds(SegGrid g, const double* __restrict__ rx, const double* __restrict__
         const int32_t o = home_cell(S, qc);
         const int32_t self = S.colbase[4] + S.cellpre[4][1] + qc;
         const double xi = P.x(self), yi = P.y(self), zi = P.z(self);
+        // entry kk of atom s: lane kk % 4, that lane's entry jj = kk / 4, i.e.
+        // chunk jj / 8 (= kk >> 5), place jj % 8 of the lane's 8
+        uint16_t* const la = list + ((int64_t)(s / kListBlock) * nchunk * kListBlock + s % kListBlock) * (kLanes * kChunk);
         int32_t k = 0;                        // hits so far, the same in all 4 lanes
         for (int32_t c = 0; c < 9; c++) {
             const int32_t b = S.colbase[c] + S.cellpre[c][o - 1], e = S.colbase[c] + S.cellpre[c][o + 2];
             for (int32_t p0 = b; p0 < e; p0 += kLanes * kScan) {
-                // lane q tests candidates p0 + kLanes u + q
+                // lane q tests candidates p0 + kLanes u + q (the list radius test
+                // may round differently from the reference's; the list only has to
+                // hold every pair that can come within rc before the next build)
                 uint32_t mask = 0;
 #pragma unroll
                 for (int u = 0; u < kScan; u++) {
                     const int32_t p = p0 + kLanes * u + q;
                     const int32_t pp = p < e ? p : b;
                     const double dx = xi - P.x(pp), dy = yi - P.y(pp), dz = zi - P.z(pp);
-                    const bool hit = (p < e) && (p != self) && (dx * dx + dy * dy + dz * dz < rl2);
+                    const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
+                    const bool hit = (p < e) & (p != self) & (r2 < rl2);
                     mask |= (uint32_t)hit << u;
                 }
-                // hits are numbered lane-major within the quad: exclusive prefix of the counts
+                // hits are numbered lane-major within the quad: exclusive prefix
+                // of the lane counts by quad permutations [0,0,1,2], [0,1,0,1]
+                // and the total from lane 3 ([3,3,3,3])
+                // (each permutation must run in all 4 lanes: a lane reading a
+                // disabled lane gets a stale value, so the empty asm pins the moves
+                // ahead of the per-lane select instead of under it)
                 const int32_t h = __popc(mask);
-                int32_t incl = h;
-                int32_t t = __shfl_up(incl, 1, kLanes);
-                incl += q >= 1 ? t : 0;
-                t = __shfl_up(incl, 2, kLanes);
-                incl += q >= 2 ? t : 0;
-                const int32_t total = __shfl(incl, kLanes - 1, kLanes);
+                int32_t t1 = __builtin_amdgcn_mov_dpp(h, 0x90, 0xF, 0xF, false);
+                asm volatile("" : "+v"(t1));
+                int32_t incl = h + (q >= 1 ? t1 : 0);
+                int32_t t2 = __builtin_amdgcn_mov_dpp(incl, 0x44, 0xF, 0xF, false);
+                asm volatile("" : "+v"(t2));
+                incl += q >= 2 ? t2 : 0;
+                const int32_t total = __builtin_amdgcn_mov_dpp(incl, 0xFF, 0xF, 0xF, false);
                 int32_t kk = k + incl - h;
-#pragma unroll
-                for (int u = 0; u < kScan; u++) {
-                    if ((mask >> u) & 1u) {
-                        if (kk < cap) {
-                            // entry kk -> lane kk % 4, that lane's entry kk / 4
-                            const int32_t ln = kk & (kLanes - 1), jj = kk / kLanes;
-                            const int64_t at = (((int64_t)(jj / kChunk) * n + s) * kLanes + ln) * kChunk + jj % kChunk;
-                            list[at] = (uint16_t)(p0 + kLanes * u + q);
-                        }
-                        kk++;
-                    }
+                // store the set bits only (about 1.5 of 8 per lane)
+                for (uint32_t mb = mask; mb; mb &= mb - 1, kk++) {
+                    const int32_t u = __builtin_ctz(mb);
+                    if (kk < cap)
+                        la[(kk >> 5) * kChunkRow + (((kk & 3) << 3) | ((kk >> 2) & 7))] = (uint16_t)(p0 + kLanes * u + q);
                 }
                 k += total;
             }
@@ -285,10 +296,7 @@ k_nlist_lds(SegGrid g, const double* __restrict__ rx, const double* __restrict__
         // step kernel evaluates whole chunks without a validity test
         const int32_t mq = (kc - q + kLanes - 1) / kLanes;
         const int32_t mend = (mq + kChunk - 1) / kChunk * kChunk;
-        for (int32_t jj = mq; jj < mend; jj++) {
-            const int64_t at = (((int64_t)(jj / kChunk) * n + s) * kLanes + q) * kChunk + jj % kChunk;
-            list[at] = (uint16_t)S.total;
-        }
+        for (int32_t jj = mq; jj < mend; jj++) la[(jj / kChunk) * kChunkRow + q * kChunk + jj % kChunk] = (uint16_t)S.total;
     }
 }
 
@@ -299,6 +307,7 @@ struct StepArgs {
     const double* r0[3];    // positions at the last list build
     const uint16_t* list;
     const int32_t* count;
+    int32_t nchunk;         // list chunks per lane (capacity / 32 + the spare)
     double rc2;
     float f12, f6;          // F/r = x^4 (f12 x^3 + f6), x = 1/r^2: 48 eps sig^12, -24 eps sig^6
     double sig2, coeff;
@@ -358,12 +367,13 @@ k_step_lds(StepArgs a) {
         const double xi = P.x(self), yi = P.y(self), zi = P.z(self);
         double fx = 0, fy = 0, fz = 0;
         const int32_t m = (a.count[i] - q + kLanes - 1) / kLanes;   // this lane's entries
-        // chunks of this lane are (n * kLanes) uint4 apart; the next chunk is
-        // loaded while the current one is evaluated (the list has one spare
-        // chunk per lane, so the look-ahead stays in bounds); the last chunk is
-        // padded with the dummy atom
-        const uint4* lp = reinterpret_cast<const uint4*>(a.list) + (int64_t)i * kLanes + q;
-        const int64_t cstride = (int64_t)n * kLanes;
+        // chunks of this lane are 16 x 4 uint4 apart; the next chunk is loaded
+        // while the current one is evaluated (the list has one spare chunk per
+        // lane, so the look-ahead stays in bounds); the last chunk is padded
+        // with the dummy atom
+        const uint4* lp = reinterpret_cast<const uint4*>(a.list) +
+                          ((int64_t)(i / kListBlock) * a.nchunk * kListBlock + i % kListBlock) * kLanes + q;
+        constexpr int32_t cstride = kListBlock * kLanes;
         uint4 nx = lp[0];
         for (int32_t k0 = 0; k0 < m; k0 += kChunk) {
             const uint4 cu = nx;
@@ -505,7 +515,10 @@ int32_t lds_segments_per_column(int32_t nz, double atoms_per_cell, int32_t kmax)
 }
 
 int32_t lds_capacity() { return kLdsAtoms; }
-int32_t list_spare_rows() { return kLanes * kChunk; }
+int64_t list_entries(int32_t n, int32_t cap) {
+    const int64_t blocks = ((int64_t)n + kListBlock - 1) / kListBlock;
+    return blocks * kListBlock * (cap + kLanes * kChunk);
+}
 int32_t list_cap_multiple() { return kLanes * kChunk; }
 int32_t flag_spread() { return kSpread; }
 
@@ -529,7 +542,7 @@ hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v
 
 hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg,
                            double* const rn[3], double* const r0[3], const uint16_t* list, const int32_t* count,
-                           const LjTable& t, const MassTable& m, const GearCoeffs& gc, double hard2, int32_t n,
+                           int32_t cap, const LjTable& t, const MassTable& m, const GearCoeffs& gc, double hard2, int32_t n,
                            int64_t cs, int32_t precision, bool final_step, bool force_out, uint64_t* flags,
                            int32_t lds_atoms) {
     if (n <= 0) return hipSuccess;
@@ -552,6 +565,8 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
     for (int d = 0; d < 3; d++) { a.rn[d] = rn[d]; a.r0[d] = r0[d]; }
     a.list = list;
     a.count = count;
+    if (cap % (kLanes * kChunk) != 0) return hipErrorInvalidValue;
+    a.nchunk = cap / (kLanes * kChunk) + 1;
     a.rc2 = t.rc2;
     // coeff = 24 eps / sig^2: 48 eps sig^12 = 2 coeff sig^14, -24 eps sig^6 = -coeff sig^8
     const double s4 = t.sig2[0] * t.sig2[0] * t.sig2[0] * t.sig2[0];

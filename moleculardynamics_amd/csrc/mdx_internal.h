@@ -140,7 +140,7 @@ hipError_t launch_predict_track(hipStream_t s, const DevState& st, int32_t n, co
 int32_t lds_segments_per_column(int32_t nz, double atoms_per_cell, int32_t kmax);
 int32_t lds_capacity();
 int32_t flag_spread();
-int32_t list_spare_rows();   // rows the list needs beyond its capacity (look-ahead loads)
+int64_t list_entries(int32_t n, int32_t cap);   // 16-bit entries to allocate for the list
 int32_t list_cap_multiple(); // the list capacity (entries per atom) must be a multiple of this
 // err[0]: staged atoms of a segment exceeded lds_atoms; err[1]: longest list above
 // cap; err[2]: the largest staging (atoms), which sizes the step kernel's LDS
@@ -151,8 +151,8 @@ hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v
 // at [0] + d * cs
 hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg,
                            double* const rn[3], double* const r0[3], const uint16_t* list, const int32_t* count,
-                           const LjTable& t, const MassTable& m, const GearCoeffs& gc, double hard2, int32_t n,
-                           int64_t cs, int32_t precision, bool final_step, bool force_out, uint64_t* flags,
+                           int32_t cap, const LjTable& t, const MassTable& m, const GearCoeffs& gc, double hard2,
+                           int32_t n, int64_t cs, int32_t precision, bool final_step, bool force_out, uint64_t* flags,
                            int32_t lds_atoms);
 
 // bond_ref.hip

@@ -535,7 +535,7 @@ int rebuild_periodic(mdx_system* s) {
         double est = 4.0 / 3.0 * M_PI * std::pow((double)c.lj_rs, 3) * rho;
         int32_t cap = c.max_neighbors > 0 ? c.max_neighbors : (int32_t)(1.5 * est) + 32;
         cap = (cap + cm - 1) / cm * cm;
-        if ((rc = dalloc(s, &s->nlist, (size_t)(cap + list_spare_rows()) * n)) || (rc = dalloc(s, &s->ncount, n)))
+        if ((rc = dalloc(s, &s->nlist, (size_t)list_entries(n, cap))) || (rc = dalloc(s, &s->ncount, n)))
             return rc;
         s->nl_cap = cap;
     }
@@ -569,7 +569,7 @@ int rebuild_periodic(mdx_system* s) {
         dfree(s, s->nlist);
         s->nlist = nullptr;
         int32_t cap = (e[1] + e[1] / 4 + cm - 1) / cm * cm;
-        if ((rc = dalloc(s, &s->nlist, (size_t)(cap + list_spare_rows()) * n))) return rc;
+        if ((rc = dalloc(s, &s->nlist, (size_t)list_entries(n, cap)))) return rc;
         s->nl_cap = cap;
     }
     return MDX_OK;
@@ -654,7 +654,7 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
         }
         const bool fin = k + 1 == nsteps;
         HIPCHK(s, hipMemsetAsync(slot_flags(k), 0, W * sizeof(uint64_t), st));
-        LAUNCH(s, "md_step", launch_step_lds(st, s->st, s->lj.view, s->nseg, s->rnext, s->rref, s->nlist, s->ncount,
+        LAUNCH(s, "md_step", launch_step_lds(st, s->st, s->lj.view, s->nseg, s->rnext, s->rref, s->nlist, s->ncount, s->nl_cap,
                                              s->ljt, s->mt, s->gc, hard2, n, component_stride(n), c.precision, fin,
                                              fin, slot_flags(k), s->lds_atoms));
         if ((rc = post(k))) return rc;
