@@ -33,6 +33,19 @@ DT = float(np.float32(0.005))  # DT is a float in the configuration (common/Glob
 STEP_BYTES_PER_ATOM = 312
 # the dominant kernel (md_step) does the whole step: LJ force, corrector and the
 # next predictor; its algorithmic bytes per atom are the step's 312 B
+# measured HBM bytes per md_step launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
+# of this workload, as recorded by the committed rocprofv3 summary
+TRAFFIC_PROFILE = ROOT / "profiles" / "md_step_lj1m.md"
+
+
+def measured_traffic() -> int | None:
+    try:
+        for line in TRAFFIC_PROFILE.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def parse():
@@ -142,7 +155,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": None,
+            "traffic": measured_traffic(),
+            "traffic_source": str(TRAFFIC_PROFILE.relative_to(ROOT)),
             "avg_launch_ms": k_ms,
             "bytes_per_atom": STEP_BYTES_PER_ATOM,
             "step_gbs_at_312B_per_atom": step_gbs,

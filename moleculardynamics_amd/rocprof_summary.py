@@ -1,0 +1,133 @@
+"""Summarise rocprofv3 result databases (rocpd SQLite) of the MD kernels.
+
+Reads one kernel-trace database (--stats) and optionally two counter runs
+(FETCH_SIZE, WRITE_SIZE) of the same command plus the same two counters for a
+device-to-device copy of known size; the copy fixes the scale of each counter
+(bytes moved / counter reading), which turns the MD kernel's readings into HBM
+bytes per launch.  Prints a short markdown report, including the line
+
+    traffic_bytes_per_launch: <int>
+
+that bench.py reads for its roofline record.
+
+    python -m moleculardynamics_amd.rocprof_summary --trace T.db \
+        --fetch F.db --write W.db --copy-fetch CF.db --copy-write CW.db \
+        --copy-bytes 268435456 --kernel "k_step_lds<false, false, false>" \
+        --units 1048576 --bytes-per-unit 312 --command "..."
+"""
+from __future__ import annotations
+
+import argparse
+import glob
+import sqlite3
+from collections import defaultdict
+from pathlib import Path
+
+
+def _db(path: str) -> sqlite3.Connection:
+    p = Path(path)
+    if p.is_dir():
+        hits = sorted(glob.glob(str(p / "**" / "*.db"), recursive=True))
+        if not hits:
+            raise FileNotFoundError(f"no .db under {p}")
+        p = Path(hits[0])
+    return sqlite3.connect(str(p))
+
+
+def short(name: str) -> str:
+    n = name.replace("void ", "").replace("mdx::(anonymous namespace)::", "").replace("mdx::", "")
+    depth, out = 0, []
+    for ch in n:   # drop the argument list, keep template arguments
+        if ch == "(" and depth == 0:
+            break
+        depth += ch == "<"
+        depth -= ch == ">"
+        out.append(ch)
+    return "".join(out)
+
+
+def kernel_stats(con: sqlite3.Connection):
+    rows = defaultdict(list)
+    meta = {}
+    for name, dur, gx, wx, lds, vgpr, agpr, sgpr in con.execute(
+            "select name, duration, grid_x, workgroup_x, lds_size, vgpr_count, accum_vgpr_count, sgpr_count "
+            "from kernels"):
+        rows[short(name)].append(dur)
+        meta[short(name)] = (gx, wx, lds, vgpr, agpr, sgpr)
+    return rows, meta
+
+
+def counter_means(con: sqlite3.Connection, counter: str):
+    vals = defaultdict(list)
+    for name, cn, v in con.execute("select name, counter_name, counter_value from pmc_events"):
+        if cn == counter:
+            vals[short(name)].append(v)
+    return {k: sum(v) / len(v) for k, v in vals.items()}
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--trace", required=True)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--copy-fetch")
+    ap.add_argument("--copy-write")
+    ap.add_argument("--copy-bytes", type=int, default=0)
+    ap.add_argument("--kernel", required=True, help="short name of the dominant kernel")
+    ap.add_argument("--units", type=int, required=True, help="work units (atoms) per launch")
+    ap.add_argument("--bytes-per-unit", type=float, required=True)
+    ap.add_argument("--command", default="")
+    ap.add_argument("--title", default="")
+    a = ap.parse_args()
+
+    rows, meta = kernel_stats(_db(a.trace))
+    total = sum(sum(v) for v in rows.values())
+    print(f"# {a.title or a.kernel}\n")
+    if a.command:
+        print("Commands (run from the repository root on one MI355X):\n")
+        for c in a.command.split(" ;; "):
+            print(f"    {c}")
+        print()
+    print("## Kernel trace (rocprofv3 --kernel-trace --stats)\n")
+    print("| kernel | calls | total ms | avg us | % | grid | wg | LDS B | VGPR | SGPR |")
+    print("|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|")
+    for k, v in sorted(rows.items(), key=lambda kv: -sum(kv[1])):
+        if sum(v) < 0.001 * total:
+            continue
+        gx, wx, lds, vg, ag, sg = meta[k]
+        print(f"| `{k}` | {len(v)} | {sum(v) / 1e6:.3f} | {sum(v) / len(v) / 1e3:.1f} | "
+              f"{100 * sum(v) / total:.1f} | {gx} | {wx} | {lds} | {vg} | {sg} |")
+    durs = rows.get(a.kernel)
+    if not durs:
+        raise SystemExit(f"kernel {a.kernel!r} not in the trace")
+    avg_s = sum(durs) / len(durs) / 1e9
+    alg = a.units * a.bytes_per_unit
+    print(f"\n## Roofline of `{a.kernel}`\n")
+    print(f"- average launch: {avg_s * 1e6:.1f} us over {len(durs)} launches")
+    print(f"- algorithmic bytes per launch: {a.units} units x {a.bytes_per_unit:g} B = {alg / 1e6:.1f} MB")
+    print(f"- achieved (algorithmic / average launch): {alg / avg_s / 1e9:.0f} GB/s "
+          f"= {alg / avg_s / 8e12:.3f} of 8 TB/s")
+    if a.fetch and a.write and a.copy_fetch and a.copy_write and a.copy_bytes:
+        cf = counter_means(_db(a.copy_fetch), "FETCH_SIZE")
+        cw = counter_means(_db(a.copy_write), "WRITE_SIZE")
+        ck = [k for k in cf if "copyBuffer" in k or "copy" in k.lower()]
+        if not ck:
+            raise SystemExit("no copy kernel in the calibration runs")
+        f_copy, w_copy = cf[ck[0]], cw[ck[0]]
+        kib = a.copy_bytes / 1024
+        f_scale, w_scale = kib / f_copy, kib / w_copy
+        fk = counter_means(_db(a.fetch), "FETCH_SIZE")[a.kernel]
+        wk = counter_means(_db(a.write), "WRITE_SIZE")[a.kernel]
+        traffic = (fk * f_scale + wk * w_scale) * 1024
+        print("\n## HBM traffic (rocprofv3 --pmc, separate runs)\n")
+        print(f"- copy calibration: a {a.copy_bytes} B device copy reads FETCH_SIZE {f_copy:.0f} KiB and "
+              f"WRITE_SIZE {w_copy:.0f} KiB, so the scale factors are {f_scale:.3f} (fetch) and "
+              f"{w_scale:.3f} (write)")
+        print(f"- `{a.kernel}`: FETCH_SIZE {fk:.0f} KiB x {f_scale:.3f} + WRITE_SIZE {wk:.0f} KiB x "
+              f"{w_scale:.3f} per launch")
+        print(f"- traffic per launch: {traffic / 1e6:.1f} MB = {traffic / alg:.2f} x the algorithmic bytes")
+        print(f"\ntraffic_bytes_per_launch: {int(round(traffic))}")
+
+
+if __name__ == "__main__":
+    main()
