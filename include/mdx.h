@@ -122,6 +122,14 @@ int mdx_set_state(mdx_system* sys, const double* pos4, const double* vel3,
 int mdx_get_state(mdx_system* sys, double* pos4, double* vel3, double* acc3,
                   double* b3, double* c3, double* force3);
 
+/* Forces (n*3 doubles, caller order) into the device state, replacing the
+ * last computed force.  The reference hands forceArray to correct and
+ * calculateAccelerations (cpu/common/Device.cpp:44,125) and, when
+ * Lennard-Jones is on, manyBodyForces adds to it
+ * (cpu/common/BondOrderFuncs.h:283-397); callers that keep forces on the host
+ * (the reference-signature adapter) upload them here. */
+int mdx_set_forces(mdx_system* sys, const double* force3);
+
 /* cpu/cpumd/Host.cpp:245-271: vmax = max |v|, box_size_list = 2 max |coordinate| */
 int mdx_ready_list(mdx_system* sys, double* vmax, double* box_size_list);
 

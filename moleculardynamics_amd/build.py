@@ -19,6 +19,8 @@ HOST = PKG / "host"
 BUILD = PKG / "_build"
 LIB = PKG / "libmdx.so"
 HOSTLIB = PKG / "libmdx_host.so"
+REFLIB = PKG / "libmdx_ref.so"       # the reference's free functions over the C-ABI
+REFABI = PKG / "refabi"
 CPUMD = PKG / "mdx_cpumd"
 ORACLE_DIR = ROOT / "oracle"
 
@@ -90,6 +92,28 @@ def build_host() -> None:
         os.replace(tmpx, CPUMD)
 
 
+def build_refabi() -> None:
+    """libmdx_ref.so: predict/correct/lennardJonesForces/... with the
+    reference's C++ signatures (include/mdx_reference.h)."""
+    srcs = sorted(REFABI.glob("*.cpp"))
+    tmp = REFLIB.with_suffix(".so.tmp")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{ROOT / 'include'}", "-shared", "-o", str(tmp),
+          *map(str, srcs), f"-L{PKG}", "-lmdx", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, REFLIB)
+
+
+def build_test_tools() -> None:
+    """Test drivers under tests/ (they link the oracle: test infrastructure)."""
+    out = ROOT / "tests" / "_bin"
+    out.mkdir(exist_ok=True)
+    exe = out / "ref_adapter_harness"
+    tmp = exe.with_suffix(".tmp")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", str(ROOT / "tests" / "ref_adapter_harness.cpp"), "-o", str(tmp),
+          f"-L{PKG}", "-lmdx_ref", "-lmdx", f"-L{ORACLE_DIR}", "-loracle",
+          "-Wl,-rpath,$ORIGIN/../../moleculardynamics_amd", "-Wl,-rpath,$ORIGIN/../../oracle"])
+    os.replace(tmp, exe)
+
+
 def build_oracle() -> Path:
     _run(["make", "-s", "-C", str(ORACLE_DIR), "-B"])
     return ORACLE_DIR / "liboracle.so"
@@ -98,7 +122,9 @@ def build_oracle() -> Path:
 def build_all() -> None:
     build_library()
     build_host()
+    build_refabi()
     build_oracle()
+    build_test_tools()
 
 
 if __name__ == "__main__":

@@ -334,7 +334,24 @@ __global__ void k_gather_i32(const int32_t* __restrict__ src, const int32_t* __r
     if (i < n) dst[i] = src[idx[i]];
 }
 
+__global__ void k_scatter3(const double* __restrict__ s0, const double* __restrict__ s1,
+                           const double* __restrict__ s2, const int32_t* __restrict__ idx, double* __restrict__ d0,
+                           double* __restrict__ d1, double* __restrict__ d2, int32_t n) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t k = idx[i];
+    d0[k] = s0[i];
+    d1[k] = s1[i];
+    d2[k] = s2[i];
+}
+
 }  // namespace
+
+hipError_t launch_scatter3(hipStream_t s, double* const src[3], const int32_t* idx, double* const dst[3], int32_t n) {
+    if (n <= 0) return hipSuccess;
+    k_scatter3<<<grid_for(n), kBlock, 0, s>>>(src[0], src[1], src[2], idx, dst[0], dst[1], dst[2], n);
+    return hipGetLastError();
+}
 
 hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v) {
     if (n <= 0) return hipSuccess;

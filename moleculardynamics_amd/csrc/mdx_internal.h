@@ -105,6 +105,8 @@ hipError_t launch_emit_list(hipStream_t s, const GridView& g, const int32_t* cel
                             int32_t* out);
 hipError_t launch_gather_i32(hipStream_t s, const int32_t* src, const int32_t* idx, int32_t* dst,
                              int32_t n);
+// dst[d][idx[i]] = src[d][i] for the three components
+hipError_t launch_scatter3(hipStream_t s, double* const src[3], const int32_t* idx, double* const dst[3], int32_t n);
 // to_slots: dst[idx[i]] = src[i] (caller order -> slots); else dst[i] = src[idx[i]]
 hipError_t launch_reorder(hipStream_t s, const DevState& src, const DevState& dst,
                           const int32_t* idx, int32_t n, int32_t to_slots);

@@ -838,6 +838,21 @@ int mdx_get_state(mdx_system* s, double* pos4, double* vel3, double* acc3, doubl
     return MDX_OK;
 }
 
+int mdx_set_forces(mdx_system* s, const double* force3) {
+    if (!s || !force3) return fail(s, MDX_ERR_ARG, "force3 is required");
+    const int n = s->n;
+    hipStream_t st = s->stream;
+    std::vector<double> buf(3 * (size_t)n);
+    for (int d = 0; d < 3; d++)
+        for (int i = 0; i < n; i++) buf[(size_t)d * n + i] = force3[3 * (size_t)i + d];
+    for (int d = 0; d < 3; d++)
+        HIPCHK(s, hipMemcpyAsync(s->alt.f[d], buf.data() + (size_t)d * n, sizeof(double) * n, hipMemcpyHostToDevice,
+                                 st));
+    LAUNCH(s, "forces_in", launch_scatter3(st, s->alt.f, s->slot_of_id, s->st.f, n));
+    HIPCHK(s, hipStreamSynchronize(st));
+    return MDX_OK;
+}
+
 int mdx_ready_list(mdx_system* s, double* vmax, double* box_size_list) {
     if (!s) return fail(s, MDX_ERR_ARG, "null system");
     int rc = ready_list(s);
