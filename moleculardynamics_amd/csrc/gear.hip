@@ -157,7 +157,74 @@ k_predict_track(DevState st, int32_t n, GearCoeffs gc, const double* __restrict_
     }
 }
 
+// Periodic box, forces from a separate kernel: corrector of step t, then
+// (unless final) the predictor of step t+1 with the displacement from the
+// list-build positions and the single-step move max-reduced into the spread
+// flags, as k_predict_track does.
+template <bool kFinal>
+__global__ void __launch_bounds__(kBlock)
+k_correct_predict_track(DevState st, int32_t n, GearCoeffs gc, MassTable m, const double* __restrict__ r0x,
+                        const double* __restrict__ r0y, const double* __restrict__ r0z,
+                        unsigned long long* __restrict__ flags, int32_t spread) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double d2 = 0.0, s2 = 0.0;
+    if (i < n) {
+        const double mass = m.mass[st.type[i] & (kTypes - 1)];
+        const double* r0[3] = {r0x, r0y, r0z};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            double r = st.r[d][i], v = st.v[d][i], a = st.a[d][i], b = st.b[d][i], c = st.c[d][i];
+            correct1(st.f[d][i], mass, m.scale, gc, r, v, a, b, c);
+            if constexpr (!kFinal) {
+                const double rp = r + gc.k[0] * v + gc.k[1] * a + gc.k[2] * b + gc.k[3] * c;
+                v = v + gc.k[0] * a + gc.k[1] * b + gc.k[2] * c;
+                a = a + gc.k[0] * b + gc.k[1] * c;
+                b = b + gc.k[0] * c;
+                const double dd = rp - r0[d][i], ds = rp - r;
+                d2 += dd * dd;
+                s2 += ds * ds;
+                r = rp;
+            }
+            st.r[d][i] = r; st.v[d][i] = v; st.a[d][i] = a; st.b[d][i] = b; st.c[d][i] = c;
+        }
+    }
+    if constexpr (!kFinal) {
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            d2 = fmax(d2, __shfl_xor(d2, o, kWave));
+            s2 = fmax(s2, __shfl_xor(s2, o, kWave));
+        }
+        __shared__ double red[kBlock / kWave][2];
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+            red[threadIdx.x / kWave][0] = d2;
+            red[threadIdx.x / kWave][1] = s2;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int q = 1; q < kBlock / kWave; q++) {
+                d2 = fmax(d2, red[q][0]);
+                s2 = fmax(s2, red[q][1]);
+            }
+            unsigned long long* f = flags + 3 * (blockIdx.x % spread);
+            atomicMax(&f[0], (unsigned long long)__double_as_longlong(d2));
+            atomicMax(&f[1], (unsigned long long)__double_as_longlong(s2));
+        }
+    }
+}
+
 }  // namespace
+
+hipError_t launch_correct_predict_track(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
+                                        const MassTable& m, double* const rref[3], uint64_t* flags, bool final_step) {
+    if (n <= 0) return hipSuccess;
+    auto* f = reinterpret_cast<unsigned long long*>(flags);
+    if (final_step)
+        k_correct_predict_track<true><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(st, n, gc, m, rref[0], rref[1],
+                                                                                 rref[2], f, flag_spread());
+    else
+        k_correct_predict_track<false><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(st, n, gc, m, rref[0], rref[1],
+                                                                                  rref[2], f, flag_spread());
+    return hipGetLastError();
+}
 
 hipError_t launch_predict_track(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
                                 double* const rref[3], uint64_t* flags) {

@@ -74,6 +74,7 @@ struct BondRefParams {
     double pair_cut;     // CUTOFF_RIJ
 };
 
+// Tersoff silicon: c2 = c^2, d2 = d^2; zeta term exp((lambda3 (r_ij - r_ik))^m)
 struct TersoffParams {
     double A, B, lambda1, lambda2, lambda3, beta, n, c2, d2, h, R, D;
     int32_t m;
@@ -137,6 +138,10 @@ hipError_t launch_correct_predict(hipStream_t s, const DevState& st, int32_t n, 
 // layout of the step kernel: flags[3 q + {0, 1}], q = block % flag_spread()
 hipError_t launch_predict_track(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
                                 double* const rref[3], uint64_t* flags);
+// corrector of step t and (unless final_step) predictor of step t+1 with the
+// same flag tracking, in place (forces computed by a separate kernel)
+hipError_t launch_correct_predict_track(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
+                                        const MassTable& m, double* const rref[3], uint64_t* flags, bool final_step);
 
 // lj_fused.hip: LDS-staged segment kernels of the periodic box
 int32_t lds_segments_per_column(int32_t nz, double atoms_per_cell, int32_t kmax);
@@ -156,6 +161,19 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
                            int32_t cap, const LjTable& t, const MassTable& m, const GearCoeffs& gc, double hard2,
                            int32_t n, int64_t cs, int32_t precision, bool final_step, bool force_out, uint64_t* flags,
                            int32_t lds_atoms);
+
+// tersoff.hip: Tersoff silicon in the periodic box (compact [k][slot] lists)
+TersoffParams tersoff_si_params();
+// list of the atoms within rs of each slot; err: longest list when above cap
+hipError_t launch_ts_list(hipStream_t s, const DevState& st, const int32_t* cell_of, const GridView& g, double rs,
+                          int32_t n, int32_t cap, int32_t* list, int32_t* count, int32_t* err);
+// forces of st's positions into out.f; per-atom energy into e_atom (may be
+// null); with rref, atoms farther than sqrt(hard2) from it are counted into
+// the spread flags as stale
+hipError_t launch_ts_forces(hipStream_t s, const DevState& st, double L, const TersoffParams& p, int32_t precision,
+                            const int32_t* list, const int32_t* count, int32_t n, double* dedr, double* dedz,
+                            double* e_atom, double* const rref[3], double hard2, uint64_t* flags,
+                            const DevState& out);
 
 // bond_ref.hip
 hipError_t launch_bond_ref(hipStream_t s, const DevState& st, const int32_t* slot_of_id,

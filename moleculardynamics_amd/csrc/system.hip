@@ -86,6 +86,14 @@ struct mdx_system {
     int32_t seg_kmax = 1 << 20;      // cap on cells per segment (lowered if staging overflows)
     int32_t lds_atoms = 0;           // largest segment staging of the last build
     int32_t build_lds = 0;           // staging capacity the list build runs with
+    // periodic Tersoff silicon: compact lists [k][slot] and per-bond derivatives
+    TersoffParams tp{};
+    int32_t* ts_list = nullptr;
+    int32_t* ts_count = nullptr;
+    double* ts_dedr = nullptr;       // dE/dr_ij at fixed zeta, per bond
+    double* ts_dedz = nullptr;       // dE/dzeta_ij, per bond
+    int32_t ts_cap = 0;
+    bool ts_built = false;
     static constexpr int kRing = 3;
     static constexpr int kFlagWords = 3 * 64;  // flag_spread() x {disp^2, step^2, stale}
     uint64_t* dflags = nullptr;      // kRing x kFlagWords
@@ -430,12 +438,43 @@ int32_t next_build(float rs, float rc, double vmax, float dt, int32_t t) {
     return (int32_t)q + t;
 }
 
+bool is_tersoff(const mdx_system* s) { return s->cfg.box == MDX_BOX_PERIODIC && s->cfg.use_bond; }
+
+// Tersoff compact lists over the primary grid (cells of edge mb_rs); grows the
+// capacity and builds again when a list overflows
+int ts_build_list(mdx_system* s) {
+    const int n = s->n;
+    for (;;) {
+        if (!s->ts_list) {
+            int rc;
+            const size_t e = (size_t)s->ts_cap * n;
+            if ((rc = dalloc(s, &s->ts_list, e)) || (rc = dalloc(s, &s->ts_dedr, e)) || (rc = dalloc(s, &s->ts_dedz, e)))
+                return rc;
+        }
+        HIPCHK(s, hipMemsetAsync(s->dint + 4, 0, sizeof(int32_t), s->stream));
+        LAUNCH(s, "ts_list", launch_ts_list(s->stream, s->st, s->cell_of, s->lj.view, (double)s->cfg.mb_rs, n,
+                                            s->ts_cap, s->ts_list, s->ts_count, s->dint + 4));
+        int32_t longest;
+        int rc = read_ints(s, &longest, s->dint + 4, 1);
+        if (rc) return rc;
+        if (longest == 0) break;
+        dfree(s, s->ts_list);
+        dfree(s, s->ts_dedr);
+        dfree(s, s->ts_dedz);
+        s->ts_list = nullptr;
+        s->ts_cap = longest + longest / 4 + 4;
+    }
+    s->ts_built = true;
+    return MDX_OK;
+}
+
 int build_list(mdx_system* s, int32_t which, double box, bool sizes) {
     const mdx_config& c = s->cfg;
     int rc;
     if (c.box == MDX_BOX_PERIODIC) {
         float rs = c.use_lj ? c.lj_rs : c.mb_rs;
-        return build_primary(s, s->lj, rs, 0.0, sizes);
+        if ((rc = build_primary(s, s->lj, rs, 0.0, sizes))) return rc;
+        return is_tersoff(s) ? ts_build_list(s) : MDX_OK;
     }
     if (!s->have_ready && (rc = ready_list(s))) return rc;
     if (which == MDX_LIST_LJ) {
@@ -462,6 +501,14 @@ int lj_forces(mdx_system* s, double* e_atom, const int32_t* gate, const DevState
 
 int bond_forces(mdx_system* s, double* e_atom, const int32_t* gate, const DevState* target = nullptr) {
     const mdx_config& c = s->cfg;
+    if (is_tersoff(s)) {
+        if (!s->ts_built) return fail(s, MDX_ERR_STATE, "Tersoff list not built");
+        const DevState& d = target ? *target : s->st;
+        LAUNCH(s, "ts_forces", launch_ts_forces(s->stream, s->st, c.box_length, s->tp, c.precision, s->ts_list,
+                                                s->ts_count, s->n, s->ts_dedr, s->ts_dedz, e_atom, nullptr, 0.0,
+                                                nullptr, d));
+        return MDX_OK;
+    }
     if (c.box == MDX_BOX_PERIODIC || c.bond_form != MDX_BOND_REFERENCE)
         return fail(s, MDX_ERR_ARG, "this bond-order form is not available in this box");
     if (!s->mb.built) return fail(s, MDX_ERR_STATE, "bond-order list not built");
@@ -588,17 +635,22 @@ double bits_to_double(uint64_t b) {
 // displacement since the build D and largest single-step move S) and
 // rebuilds when D + 1.2 S reaches 0.9 (rs - rc)/2; every step kernel
 // re-checks its own positions against the margin and counts violations, which
-// turn into MDX_ERR_LIST_STALE.
+// turn into MDX_ERR_LIST_STALE.  Tersoff silicon runs the same loop with its
+// two force kernels (which count the stale atoms) and a separate corrector /
+// next-predictor kernel per step.
 int run_periodic(mdx_system* s, int32_t nsteps) {
     const mdx_config& c = s->cfg;
     int rc;
     const int n = s->n;
-    const double margin = 0.5 * ((double)c.lj_rs - (double)c.lj_rcut);
+    const bool ts = is_tersoff(s);
+    const double margin = ts ? 0.5 * ((double)c.mb_rs - (s->tp.R + s->tp.D))
+                             : 0.5 * ((double)c.lj_rs - (double)c.lj_rcut);
     const double hard2 = margin * margin;
     hipStream_t st = s->stream;
+    auto rebuild_lists = [&]() { return ts ? build_list(s, MDX_LIST_BOND, 0.0, false) : rebuild_periodic(s); };
     if (!s->initialized) {
-        if ((rc = rebuild_periodic(s))) return rc;
-        if ((rc = lj_forces(s, nullptr, nullptr))) return rc;  // Host.cpp:132-138
+        if ((rc = rebuild_lists())) return rc;
+        if ((rc = ts ? bond_forces(s, nullptr, nullptr) : lj_forces(s, nullptr, nullptr))) return rc;  // Host.cpp:132-138
         LAUNCH(s, "accelerations", launch_accelerations(st, s->st, n, s->mt, nullptr));
         s->initialized = true;
     }
@@ -649,11 +701,20 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
             rebuild = D + 1.2 * S >= trigger;
         // (fk < last_build: the flags predate the last build, r(k) is one step from it)
         if (rebuild) {
-            if ((rc = rebuild_periodic(s))) return rc;
+            if ((rc = rebuild_lists())) return rc;
             last_build = k;
         }
         const bool fin = k + 1 == nsteps;
         HIPCHK(s, hipMemsetAsync(slot_flags(k), 0, W * sizeof(uint64_t), st));
+        if (ts) {
+            LAUNCH(s, "ts_forces", launch_ts_forces(st, s->st, c.box_length, s->tp, c.precision, s->ts_list,
+                                                    s->ts_count, n, s->ts_dedr, s->ts_dedz, nullptr, s->rref, hard2,
+                                                    slot_flags(k), s->st));
+            LAUNCH(s, "gear_step", launch_correct_predict_track(st, s->st, n, s->gc, s->mt, s->rref, slot_flags(k), fin));
+            if ((rc = post(k))) return rc;
+            s->t++;
+            continue;
+        }
         LAUNCH(s, "md_step", launch_step_lds(st, s->st, s->lj.view, s->nseg, s->rnext, s->rref, s->nlist, s->ncount, s->nl_cap,
                                              s->ljt, s->mt, s->gc, hard2, n, component_stride(n), c.precision, fin,
                                              fin, slot_flags(k), s->lds_atoms));
@@ -686,6 +747,8 @@ bool valid_config(const mdx_config* c, std::string& why) {
         if (!(c->box_length > 0)) { why = "periodic box needs box_length > 0"; return false; }
         if (c->use_lj && c->use_bond) { why = "periodic box runs one potential at a time"; return false; }
         if (c->use_bond && c->bond_form != MDX_BOND_TERSOFF) { why = "periodic bond order is Tersoff"; return false; }
+        if (c->use_bond && c->n_bond != c->n_atoms) { why = "periodic Tersoff needs n_bond == n_atoms"; return false; }
+        if (c->use_bond && !(c->mb_rs >= 3.0f)) { why = "Tersoff list radius mb_rs must be >= R + D = 3.0"; return false; }
     } else if (c->use_bond && c->bond_form != MDX_BOND_REFERENCE) {
         why = "walled box bond order is the reference form";
         return false;
@@ -737,6 +800,11 @@ int mdx_create(const mdx_config* cfg, mdx_system** out) {
     }
     if (cfg->box == MDX_BOX_PERIODIC)
         if ((rc = dalloc3(s, s->rnext, n))) { g_err = s->err; return bail(rc); }
+    if (cfg->box == MDX_BOX_PERIODIC && cfg->use_bond) {
+        s->tp = tersoff_si_params();
+        s->ts_cap = cfg->max_neighbors > 0 ? cfg->max_neighbors : 32;
+        if ((rc = dalloc(s, &s->ts_count, n))) { g_err = s->err; return bail(rc); }
+    }
     if (hipHostMalloc(&s->pinned, 256, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&s->hflags, sizeof(uint64_t) * mdx_system::kFlagWords * mdx_system::kRing,
                       hipHostMallocDefault) != hipSuccess)

@@ -56,3 +56,22 @@ def write_reference_input(path, pos3: np.ndarray, types: np.ndarray, vel3: np.nd
 def read_reference_input(path, n: int):
     vals = np.array(open(path).read().split(), dtype=np.float64)[: 7 * n].reshape(n, 7)
     return vals[:, :4].copy(), vals[:, 4:7].copy()
+
+
+# silicon in metal-style units: A, fs, amu, eV
+SI_LATTICE = 5.431          # A
+SI_MASS = 28.0855           # amu
+KB_EV = 8.617333262e-5      # eV / K
+ACCEL_EV_A_AMU = 9.6485332e-3   # 1 eV / (A amu) in A / fs^2: a = F / m * this
+
+
+def si_diamond(ncell: int, temperature_k: float, seed: int, disp: float = 0.05):
+    """Diamond silicon, 8 ncell^3 atoms: Gaussian displacements of `disp` A per
+    axis and Maxwell-Boltzmann velocities (A/fs) at temperature_k, from one seed.
+    Returns (positions wrapped into [0, L), velocities, L)."""
+    pos, L = diamond(ncell, SI_LATTICE)
+    rng = np.random.default_rng(seed)
+    pos = np.mod(pos + rng.normal(0.0, disp, pos.shape), L)
+    kT_over_m = KB_EV * temperature_k / SI_MASS * ACCEL_EV_A_AMU   # A^2 / fs^2
+    vel = maxwell_boltzmann(len(pos), kT_over_m, seed + 1)
+    return pos, vel, L

@@ -249,3 +249,17 @@ def tersoff(pos3, L, p=None, forces=True):
     f = np.zeros_like(pos3) if forces else None
     e = lib().orc_tersoff_energy_forces(C.byref(p), ptr(pos3), len(pos3), L, ptr(f))
     return e, f
+
+
+def tersoff_run(pos3, vel3, L, dt, nsteps, mass, scale):
+    """Tersoff silicon, FP64 Gear PC with forces recomputed by the oracle at
+    every step (no list): the trajectory the engine's periodic run must follow."""
+    r, v = f64(pos3).copy(), f64(vel3).copy()
+    a, b, c = np.zeros_like(r), np.zeros_like(r), np.zeros_like(r)
+    _, f = tersoff(r, L)
+    a[:] = f / mass * scale
+    for _ in range(nsteps):
+        r, v, a, b, c = gear_predict(r, v, a, b, c, dt)
+        _, f = tersoff(r, L)
+        r, v, a, b, c = gear_correct(r, v, f, a, b, c, dt, mass, scale)
+    return r, v

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import _oracle as O
-from moleculardynamics_amd import (BOND_REFERENCE, BOX_PERIODIC, BOX_WALLED, LIST_BOND, LIST_LJ,
+from moleculardynamics_amd import (BOND_REFERENCE, BOND_TERSOFF, BOX_PERIODIC, BOX_WALLED, LIST_BOND, LIST_LJ,
                                    LJ_REFERENCE, LJ_UNIFORM, PREC_FP64, PREC_MIXED, System, initial)
 
 pytestmark = pytest.mark.gpu
@@ -287,3 +287,67 @@ def test_periodic_mixed_energy_drift(fluid):
     drift = abs(e1 - e0) / n
     assert e0 == pytest.approx(en[0, 2], rel=1e-6)
     assert drift <= drift_o + 1e-4
+
+
+# ---------------- periodic Tersoff silicon ----------------
+def tersoff_system(n, L, precision, dt=1.0, mb_rs=3.3):
+    return System(n_atoms=n, n_bond=n, box=BOX_PERIODIC, precision=precision, use_lj=0, use_bond=1,
+                  lj_table=LJ_UNIFORM, bond_form=BOND_TERSOFF, dt=dt, mb_rs=mb_rs, mb_rcut=3.0, box_length=L,
+                  mass=initial.SI_MASS, accel_scale=initial.ACCEL_EV_A_AMU)
+
+
+@pytest.fixture(scope="module")
+def silicon():
+    # 3^3 diamond cells (216 atoms, L = 16.3 A: 4 cells of 3.3 A per axis), 1000 K
+    return initial.si_diamond(3, 1000.0, seed=5, disp=0.08)
+
+
+@pytest.mark.parametrize("precision", [PREC_FP64, PREC_MIXED])
+def test_tersoff_forces_and_energy(silicon, precision):
+    pos, vel, L = silicon
+    n = len(pos)
+    with tersoff_system(n, L, precision) as s:
+        s.set_state(np.c_[pos, np.zeros(n)], vel)
+        s.build_list(LIST_BOND)
+        s.bond_forces()
+        f = s.get_state()["force"]
+        e = s.potential()[1]
+    eo, fo = O.tersoff(pos, L)
+    if precision == PREC_FP64:
+        assert rel_rms(f, fo) < 1e-12
+        assert e == pytest.approx(eo, rel=1e-13)
+    else:
+        assert rel_rms(f, fo) < 1e-5
+        assert e == pytest.approx(eo, rel=1e-6)
+
+
+def test_tersoff_run_matches_oracle(silicon):
+    pos, vel, L = silicon
+    n, steps = len(pos), 40
+    ro, vo = O.tersoff_run(pos, vel, L, 1.0, steps, initial.SI_MASS, initial.ACCEL_EV_A_AMU)
+    with tersoff_system(n, L, PREC_FP64) as s:
+        s.set_state(np.c_[pos, np.zeros(n)], vel)
+        s.run(steps)
+        st = s.get_state()
+        assert s.build_count() >= 1
+    d = st["pos"][:, :3] - ro
+    d -= L * np.round(d / L)
+    assert np.abs(d).max() < 1e-9
+    assert rel_rms(st["vel"], vo) < 1e-9
+
+
+def test_tersoff_mixed_energy_drift(silicon):
+    """Mixed precision, 400 steps of 1 fs at 1000 K: total energy holds to
+    1e-4 of the potential energy scale."""
+    pos, vel, L = silicon
+    n = len(pos)
+    with tersoff_system(n, L, PREC_MIXED) as s:
+        s.set_state(np.c_[pos, np.zeros(n)], vel)
+        s.run(1)
+        e = []
+        for _ in range(4):
+            st = s.get_state()
+            ek = 0.5 * initial.SI_MASS * (st["vel"] ** 2).sum() / initial.ACCEL_EV_A_AMU
+            e.append(s.potential()[1] + ek)
+            s.run(100)
+    assert max(e) - min(e) < 1e-4 * abs(e[0])
