@@ -9,6 +9,11 @@ when due], LJ forces, corrector (fused with the next predictor).
 
 Prints one JSON line (see the fields below).  --gpus > 1 is not supported:
 this build runs on one GPU.
+
+--workload tersoff measures the bond-order configuration instead: Tersoff
+silicon (R = 2.7 A, D = 0.3 A), 262,144 atoms of diamond (32^3 cells of
+5.431 A) with 0.05 A Gaussian displacements and 1000 K velocities, list
+radius 3.3 A, dt = 1 fs, mixed precision.
 """
 from __future__ import annotations
 
@@ -57,6 +62,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-ncell", type=int, default=32)
     p.add_argument("--cpu-steps", type=int, default=20)
+    p.add_argument("--workload", choices=["lj", "tersoff"], default="lj")
     p.add_argument("--copy-bytes", type=int, default=0,
                    help="only run device-to-device copies of this many bytes (counter calibration)")
     return p.parse_args()
@@ -98,6 +104,97 @@ def cpu_baseline(ncell: int, steps: int) -> dict:
     }
 
 
+# Tersoff forces (k_ts_bond + k_ts_force) per atom: positions read (24 B), the
+# list read twice (2 x 4 entries x 4 B) with its count (2 x 4 B), the per-bond
+# derivatives written and read back (4 bonds x 2 x 8 B, twice) and the force
+# written (24 B)
+TS_FORCE_BYTES_PER_ATOM = 24 + 32 + 8 + 128 + 24
+
+
+def cpu_baseline_tersoff(ncell: int, steps: int) -> dict:
+    """The FP64 oracle's Tersoff forces (cell search, O(N L^2)) with its Gear
+    predictor-corrector, single-threaded, on a sub-box sample."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import _oracle as O
+    from moleculardynamics_amd import initial
+
+    pos, vel, L = initial.si_diamond(ncell, 1000.0, seed=12345, disp=0.05)
+    t0 = time.perf_counter()
+    O.tersoff_run(pos, vel, L, 1.0, steps, initial.SI_MASS, initial.ACCEL_EV_A_AMU)
+    elapsed = time.perf_counter() - t0
+    return {
+        "value": len(pos) * steps / elapsed,
+        "unit": "atom-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{len(pos)} atoms ({ncell}^3 diamond x8, same T/displacements), {steps} steps, "
+                  f"forces every step, {elapsed:.1f} s on {platform.processor() or platform.machine()}",
+    }
+
+
+def bench_tersoff(a) -> None:
+    from moleculardynamics_amd import BOND_TERSOFF, BOX_PERIODIC, LJ_UNIFORM, PREC_MIXED, System, initial
+
+    ncell = 32 if a.ncell == 64 else a.ncell
+    pos, vel, L = initial.si_diamond(ncell, 1000.0, seed=2024, disp=0.05)
+    n = len(pos)
+    print(f"bench: Tersoff Si {n} atoms, L={L:.4f}; warmup {a.warmup}, steps {a.steps}", file=sys.stderr, flush=True)
+    s = System(n_atoms=n, n_bond=n, box=BOX_PERIODIC, precision=PREC_MIXED, use_lj=0, use_bond=1,
+               lj_table=LJ_UNIFORM, bond_form=BOND_TERSOFF, dt=1.0, mb_rs=3.3, mb_rcut=3.0, box_length=L,
+               mass=initial.SI_MASS, accel_scale=initial.ACCEL_EV_A_AMU)
+    s.set_state(np.c_[pos, np.zeros(n)], vel)
+    s.run(a.warmup)
+    s.synchronize()
+    b0 = s.build_count()
+    s.set_timing(True)
+    t0 = time.perf_counter()
+    s.run(a.steps)
+    s.synchronize()
+    t1 = time.perf_counter()
+    times = s.kernel_times()
+    s.set_timing(False)
+    builds = s.build_count() - b0
+    s.close()
+    elapsed = t1 - t0
+    value = n * a.steps / elapsed
+    print(f"bench: {elapsed:.3f} s, {builds} list builds; kernels {times}", file=sys.stderr, flush=True)
+    cnt, ms = times.get("ts_forces", (0, 0.0))
+    k_ms = ms / max(cnt, 1)
+    achieved = n * TS_FORCE_BYTES_PER_ATOM / (k_ms * 1e-3) / 1e9 if cnt else None
+    out = {
+        "metric": "atom-steps/s per GPU (Tersoff Si 262k, mixed precision)",
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * elapsed / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-terms/fp64-accum",
+        "data": "synthetic (seeded diamond lattice + Gaussian displacements + Maxwell-Boltzmann velocities)",
+        "config": {"workload": f"periodic Tersoff Si {n} atoms (diamond {ncell}^3 x8, a 5.431 A), 1000 K, "
+                               f"R 2.7 D 0.3, list radius 3.3 A, dt 1 fs", "atoms": n, "list_builds": builds},
+        "roofline": {
+            "kernel": "ts_forces (k_ts_bond + k_ts_force)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": None,
+            "avg_launch_ms": k_ms,
+            "bytes_per_atom": TS_FORCE_BYTES_PER_ATOM,
+        },
+        "kernel_ms": {k: v[1] / max(v[0], 1) for k, v in times.items()},
+    }
+    if not a.no_cpu:
+        print("bench: cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline_tersoff(16, 200)
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -105,6 +202,9 @@ def main():
         sys.exit(2)
     if a.copy_bytes:
         copy_calibration(a.copy_bytes)
+        return
+    if a.workload == "tersoff":
+        bench_tersoff(a)
         return
     from moleculardynamics_amd import BOX_PERIODIC, LJ_UNIFORM, PREC_MIXED, System, initial
 

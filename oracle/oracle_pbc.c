@@ -257,10 +257,86 @@ static void ts_ex(const orc_tersoff_params* p, double x, double* e, double* de) 
 
 typedef struct { int32_t j; double d[3], r; } ts_nb;
 
-/* neighbours within R + D by brute force over all pairs (minimum image when L > 0) */
+/* neighbours within R + D of each atom, appended in candidate order j */
+static int ts_push(ts_nb** nb, size_t* cap, size_t* cnt, const double* x, int32_t i, int32_t j, double L,
+                   double rc2) {
+    double d[3];
+    for (int q = 0; q < 3; q++) {
+        d[q] = x[3 * j + q] - x[3 * i + q];
+        if (L > 0) d[q] = min_image(d[q], L);
+    }
+    double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (r2 >= rc2) return 0;
+    if (*cnt == *cap) {
+        *cap *= 2;
+        *nb = (ts_nb*)realloc(*nb, *cap * sizeof(ts_nb));
+        if (!*nb) return -1;
+    }
+    (*nb)[*cnt].j = j;
+    memcpy((*nb)[*cnt].d, d, sizeof d);
+    (*nb)[*cnt].r = sqrt(r2);
+    (*cnt)++;
+    return 0;
+}
+
+/* Large periodic systems: cells of edge >= R + D (at least 3 per axis), the
+ * 27 surrounding cells scanned per atom; the same neighbour set as the brute
+ * force, in a different order. */
+static ts_nb* ts_neighbors_cells(const orc_tersoff_params* p, const double* x, int32_t n, double L,
+                                 int32_t nc, int32_t** start_out) {
+    double rc = p->R + p->D, rc2 = rc * rc, h = L / nc;
+    int32_t ncell = nc * nc * nc;
+    int32_t* head = (int32_t*)malloc(sizeof(int32_t) * ((size_t)ncell + 1));
+    int32_t* cell = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    int32_t* start = (int32_t*)calloc((size_t)n + 1, sizeof(int32_t));
+    size_t cap = 16 * (size_t)n + 16, cnt = 0;
+    ts_nb* nb = (ts_nb*)malloc(cap * sizeof(ts_nb));
+    if (!head || !cell || !order || !start || !nb) return NULL;
+    memset(head, 0, sizeof(int32_t) * ((size_t)ncell + 1));
+    for (int32_t i = 0; i < n; i++) {
+        int32_t c[3];
+        for (int q = 0; q < 3; q++) {
+            double y = x[3 * i + q] - L * floor(x[3 * i + q] / L);
+            c[q] = (int32_t)(y / h);
+            if (c[q] >= nc) c[q] = nc - 1;
+        }
+        cell[i] = (c[0] * nc + c[1]) * nc + c[2];
+        head[cell[i] + 1]++;
+    }
+    for (int32_t c = 0; c < ncell; c++) head[c + 1] += head[c];
+    int32_t* fill = (int32_t*)calloc((size_t)ncell, sizeof(int32_t));
+    if (!fill) return NULL;
+    for (int32_t i = 0; i < n; i++) order[head[cell[i]] + fill[cell[i]]++] = i;
+    free(fill);
+    for (int32_t i = 0; i < n; i++) {
+        start[i] = (int32_t)cnt;
+        int32_t ci = cell[i], cz = ci % nc, cy = (ci / nc) % nc, cx = ci / (nc * nc);
+        for (int dx = -1; dx <= 1; dx++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dz = -1; dz <= 1; dz++) {
+                    int32_t c = (((cx + dx + nc) % nc) * nc + (cy + dy + nc) % nc) * nc + (cz + dz + nc) % nc;
+                    for (int32_t q = head[c]; q < head[c + 1]; q++) {
+                        int32_t j = order[q];
+                        if (j != i && ts_push(&nb, &cap, &cnt, x, i, j, L, rc2)) return NULL;
+                    }
+                }
+    }
+    start[n] = (int32_t)cnt;
+    free(head);
+    free(cell);
+    free(order);
+    *start_out = start;
+    return nb;
+}
+
+/* neighbours within R + D by brute force over all pairs (minimum image when L > 0);
+ * periodic systems above 2048 atoms use the cell search */
 static ts_nb* ts_neighbors(const orc_tersoff_params* p, const double* x, int32_t n, double L,
                            int32_t** start_out) {
     double rc = p->R + p->D, rc2 = rc * rc;
+    if (L > 0 && n > 2048 && (int32_t)floor(L / rc) >= 3)
+        return ts_neighbors_cells(p, x, n, L, (int32_t)floor(L / rc), start_out);
     int32_t* start = (int32_t*)calloc((size_t)n + 1, sizeof(int32_t));
     size_t cap = 16 * (size_t)n + 16, cnt = 0;
     ts_nb* nb = (ts_nb*)malloc(cap * sizeof(ts_nb));

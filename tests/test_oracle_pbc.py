@@ -162,3 +162,30 @@ def test_tersoff_cutoff_region_gradient():
                 mm = pos.copy(); mm[i, d] -= h
                 g = (O.tersoff(pp, 0.0, p, False)[0] - O.tersoff(mm, 0.0, p, False)[0]) / (2 * h)
                 assert f[i, d] == pytest.approx(-g, rel=1e-6, abs=1e-7)
+
+
+def test_tersoff_cell_search_matches_brute_force():
+    """Above 2048 atoms the oracle finds Tersoff neighbours through cells.
+    Pinned by exact properties: perfect diamond gives the brute-force energy
+    per atom, a rigid translation (which moves atoms across cell boundaries)
+    changes nothing, and the forces are the negative gradient."""
+    p = O.tersoff_params()
+    pos, L = initial.diamond(8, 5.431)                      # 4096 atoms: cell search
+    e_big, f_big = O.tersoff(pos, L, p)
+    small, Ls = initial.diamond(3, 5.431)                   # 216 atoms: brute force
+    e_small, _ = O.tersoff(small, Ls, p)
+    assert e_big / len(pos) == pytest.approx(e_small / len(small), rel=1e-12)
+    assert np.abs(f_big).max() < 1e-10
+    rng = np.random.default_rng(11)
+    x = np.mod(pos + rng.normal(0, 0.1, pos.shape), L)
+    e1, f1 = O.tersoff(x, L, p)
+    e2, f2 = O.tersoff(np.mod(x + np.array([1.37, -2.11, 0.61]), L), L, p)
+    assert e2 == pytest.approx(e1, rel=1e-12)
+    np.testing.assert_allclose(f2, f1, rtol=0, atol=1e-10)
+    # forces against finite differences of the cell-search energy
+    h = 1e-5
+    for i, d in [(5, 0), (1000, 2), (4000, 1)]:
+        pp = x.copy(); pp[i, d] += h
+        mm = x.copy(); mm[i, d] -= h
+        g = (O.tersoff(pp, L, p, False)[0] - O.tersoff(mm, L, p, False)[0]) / (2 * h)
+        assert f1[i, d] == pytest.approx(-g, rel=1e-5, abs=1e-6)
