@@ -189,6 +189,42 @@ int mdx_kernel_times(mdx_system* sys, char* names, int64_t* counts, double* tota
 /* blocks until all work on the system's stream is done */
 int mdx_synchronize(mdx_system* sys);
 
+/*
+ * Batched replicas.  The reference runs independent systems one after
+ * another (the vxcm sweep restarts the step loop per velocity offset,
+ * cpu/cpumd/Host.cpp:75-216); a batch holds n_replicas independent periodic
+ * Lennard-Jones boxes of different sizes in one set of device arrays and
+ * advances all of them with one graph launch per two steps.  From `common`:
+ * precision, dt, lj_rs, lj_rcut, lj_sigma, lj_eps, mass, accel_scale,
+ * max_neighbors (0: from the densest replica); the other fields are ignored.
+ * Every box must be at least 2 lj_rs wide (minimum image).  Lists are rebuilt
+ * per replica, decided on the device from the predicted positions: exactly
+ * when an atom moved (lj_rs - lj_rcut)/2 since its replica's last build.
+ * State arrays hold the replicas one after another, 3 doubles per atom.
+ */
+typedef struct mdx_batch mdx_batch;
+int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t* n_atoms,
+                     const double* box_length, mdx_batch** out);
+int mdx_batch_destroy(mdx_batch* batch);
+const char* mdx_batch_last_error(const mdx_batch* batch);  /* batch may be NULL */
+/* (re)starts the batch: positions, velocities; Gear history cleared */
+int mdx_batch_set_state(mdx_batch* batch, const double* pos3, const double* vel3);
+/* any output may be NULL; force3 is the last computed force */
+int mdx_batch_get_state(mdx_batch* batch, double* pos3, double* vel3, double* force3);
+/* nsteps steps of every replica; the first call after set_state computes the
+ * initial forces and a = F/m (cpu/cpumd/Host.cpp:132-138) */
+int mdx_batch_run(mdx_batch* batch, int32_t nsteps);
+/* list rebuilds per replica since creation (n_replicas entries) */
+int mdx_batch_build_counts(mdx_batch* batch, int64_t* builds);
+/* per replica {kinetic, potential (LJ shifted to zero at rc)}: 2 * n_replicas */
+int mdx_batch_energies(mdx_batch* batch, double* out2);
+/* per-kernel timing with events; while enabled, mdx_batch_run launches the
+ * kernels one by one instead of through the graph */
+int mdx_batch_set_timing(mdx_batch* batch, int32_t enabled);
+int mdx_batch_kernel_times(mdx_batch* batch, char* names, int64_t* counts, double* total_ms,
+                           int32_t cap, int32_t* n);
+int mdx_batch_synchronize(mdx_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,11 +4,11 @@ The compute path is libmdx.so (hand-written HIP kernels behind the C-ABI in
 include/mdx.h); this package only loads it and moves arrays across it.
 """
 from ._lib import (BOND_REFERENCE, BOND_TERSOFF, BOX_PERIODIC, BOX_WALLED, LIST_BOND, LIST_LJ,
-                   LJ_REFERENCE, LJ_UNIFORM, PREC_FP64, PREC_MIXED, MdxConfig, MdxError, System,
+                   LJ_REFERENCE, LJ_UNIFORM, PREC_FP64, PREC_MIXED, Batch, MdxConfig, MdxError, System,
                    header_symbols, load)
 
 __all__ = [
-    "System", "MdxConfig", "MdxError", "load", "header_symbols",
+    "System", "Batch", "MdxConfig", "MdxError", "load", "header_symbols",
     "BOX_WALLED", "BOX_PERIODIC", "PREC_MIXED", "PREC_FP64", "LJ_REFERENCE", "LJ_UNIFORM",
     "BOND_REFERENCE", "BOND_TERSOFF", "LIST_LJ", "LIST_BOND",
 ]

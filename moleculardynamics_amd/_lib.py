@@ -76,7 +76,12 @@ def load() -> C.CDLL:
         "mdx_run": [P, I32], "mdx_build_count": [P, C.POINTER(I64)],
         "mdx_kinetics": [P, D, P], "mdx_potential": [P, P], "mdx_potential_allpairs": [P, P],
         "mdx_set_timing": [P, I32], "mdx_kernel_times": [P, P, P, P, I32, C.POINTER(I32)],
-        "mdx_synchronize": [P],
+        "mdx_synchronize": [P], "mdx_set_forces": [P, P],
+        "mdx_batch_create": [C.POINTER(MdxConfig), I32, P, P, C.POINTER(P)],
+        "mdx_batch_destroy": [P], "mdx_batch_set_state": [P, P, P], "mdx_batch_get_state": [P, P, P, P],
+        "mdx_batch_run": [P, I32], "mdx_batch_build_counts": [P, P], "mdx_batch_energies": [P, P],
+        "mdx_batch_set_timing": [P, I32], "mdx_batch_kernel_times": [P, P, P, P, I32, C.POINTER(I32)],
+        "mdx_batch_synchronize": [P],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -84,6 +89,8 @@ def load() -> C.CDLL:
         f.restype = C.c_int
     lib.mdx_last_error.argtypes = [P]
     lib.mdx_last_error.restype = C.c_char_p
+    lib.mdx_batch_last_error.argtypes = [P]
+    lib.mdx_batch_last_error.restype = C.c_char_p
     _lib = lib
     return lib
 
@@ -220,3 +227,94 @@ class System:
             nm = names.raw[48 * k: 48 * k + 48].split(b"\0", 1)[0].decode()
             res[nm] = (int(counts[k]), float(tot[k]))
         return res
+
+
+def _kernel_table(names, counts, tot, n):
+    res = {}
+    for k in range(n):
+        nm = names.raw[48 * k: 48 * k + 48].split(b"\0", 1)[0].decode()
+        res[nm] = (int(counts[k]), float(tot[k]))
+    return res
+
+
+class Batch:
+    """Independent periodic Lennard-Jones replicas advanced together
+    (mdx_batch_*).  State arrays hold the replicas one after another."""
+
+    def __init__(self, n_atoms, box_length, **cfg):
+        self.lib = load()
+        c = MdxConfig()
+        c.accel_scale = 1.0
+        c.lj_sigma = c.lj_eps = c.mass = 1.0
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        self.n_atoms = np.ascontiguousarray(n_atoms, dtype=np.int32)
+        self.box = np.ascontiguousarray(box_length, dtype=np.float64)
+        self.R = len(self.n_atoms)
+        self.n = int(self.n_atoms.sum())
+        h = C.c_void_p()
+        rc = self.lib.mdx_batch_create(C.byref(c), self.R, _ptr(self.n_atoms), _ptr(self.box), C.byref(h))
+        if rc != MDX_OK:
+            raise MdxError("mdx_batch_create", rc, self.lib.mdx_batch_last_error(None).decode())
+        self.h = h
+
+    def _chk(self, fn: str, rc: int):
+        if rc != MDX_OK:
+            raise MdxError(fn, rc, self.lib.mdx_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mdx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_state(self, pos3, vel3):
+        p = np.ascontiguousarray(pos3, dtype=np.float64)
+        v = np.ascontiguousarray(vel3, dtype=np.float64)
+        assert p.shape == (self.n, 3) and v.shape == (self.n, 3)
+        self._chk("mdx_batch_set_state", self.lib.mdx_batch_set_state(self.h, _ptr(p), _ptr(v)))
+
+    def get_state(self):
+        p, v, f = (np.zeros((self.n, 3)) for _ in range(3))
+        self._chk("mdx_batch_get_state", self.lib.mdx_batch_get_state(self.h, _ptr(p), _ptr(v), _ptr(f)))
+        return {"pos": p, "vel": v, "force": f}
+
+    def run(self, nsteps):
+        self._chk("mdx_batch_run", self.lib.mdx_batch_run(self.h, int(nsteps)))
+
+    def build_counts(self):
+        b = np.zeros(self.R, dtype=np.int64)
+        self._chk("mdx_batch_build_counts", self.lib.mdx_batch_build_counts(self.h, _ptr(b)))
+        return b
+
+    def energies(self):
+        e = np.zeros((self.R, 2))
+        self._chk("mdx_batch_energies", self.lib.mdx_batch_energies(self.h, _ptr(e)))
+        return e
+
+    def synchronize(self):
+        self._chk("mdx_batch_synchronize", self.lib.mdx_batch_synchronize(self.h))
+
+    def set_timing(self, on: bool):
+        self._chk("mdx_batch_set_timing", self.lib.mdx_batch_set_timing(self.h, 1 if on else 0))
+
+    def kernel_times(self):
+        cap = 64
+        names = C.create_string_buffer(48 * cap)
+        counts = np.zeros(cap, dtype=np.int64)
+        tot = np.zeros(cap)
+        n = C.c_int32()
+        self._chk("mdx_batch_kernel_times",
+                  self.lib.mdx_batch_kernel_times(self.h, names, _ptr(counts), _ptr(tot), cap, C.byref(n)))
+        return _kernel_table(names, counts, tot, n.value)

@@ -1,0 +1,735 @@
+// replicas.hip -- batched independent periodic Lennard-Jones replicas, the
+// whole batch advanced by one graph launch per two steps.
+//
+// The reference runs independent systems one after another: its vxcm sweep
+// (cpu/cpumd/Host.cpp:75-216) restarts the same step loop for every velocity
+// offset.  Here R replicas of different sizes and boxes share one set of
+// device arrays and every kernel covers all of them:
+//
+//  * layout: replica r owns atoms [off[r], off[r] + n[r]) of the SoA state;
+//    off[r] is a multiple of 64, so every wave belongs to exactly one replica
+//    (padding atoms are inactive);
+//  * lists are rebuilt per replica, gated by a device flag, so one static
+//    graph serves all steps: k_rep_bin (cell of each atom), k_rep_cells (the
+//    members of each cell in ascending atom order: deterministic), k_rep_list
+//    (compact list within rs from the 27 cells, or from all atoms of the
+//    replica when its box holds fewer than 3 cells per axis; minimum image);
+//  * k_rep_step: LJ force over the list (minimum image), Gear corrector of
+//    step t and predictor of step t+1; positions are double-buffered; the
+//    largest displacement of r(t+1) from the list-build positions is
+//    max-reduced per replica;
+//  * k_rep_decide: a replica rebuilds before the next step exactly when that
+//    displacement reached (rs - rc)/2 -- the decision reads the positions the
+//    next step will use, so it needs no margin for lag and no host round trip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mdx_internal.h"
+
+using namespace mdx;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kCellCap = 64;   // members per cell (a cell of edge <= 3.7 sigma at rho <= 1.2 holds < 64)
+
+struct RepView {
+    const int32_t* rep_of;     // replica of each atom slot (-1: padding)
+    const int32_t* off;        // R + 1 atom offsets
+    const int32_t* nat;        // R atom counts
+    const int32_t* coff;       // R + 1 cell offsets
+    const int32_t* nc;         // R cells per axis (0: all pairs)
+    const double* L;           // R box edges
+    const int32_t* rebuild;    // R flags
+};
+
+__device__ inline double mi(double d, double L, double invL) { return d - L * rint(d * invL); }
+
+__device__ inline int32_t wrap_cell(double x, double L, int32_t nc) {
+    double y = x - L * floor(x / L);
+    int32_t c = (int32_t)(y * nc / L);
+    return c >= nc ? nc - 1 : (c < 0 ? 0 : c);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_rep_bin(RepView v, const double* __restrict__ rx, const double* __restrict__ ry, const double* __restrict__ rz,
+          int32_t ntot, int32_t* __restrict__ cell_of) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntot) return;
+    const int32_t r = v.rep_of[i];
+    if (r < 0 || !v.rebuild[r]) return;
+    const int32_t nc = v.nc[r];
+    if (nc == 0) return;
+    const double L = v.L[r];
+    const int32_t cx = wrap_cell(rx[i], L, nc), cy = wrap_cell(ry[i], L, nc), cz = wrap_cell(rz[i], L, nc);
+    cell_of[i] = v.coff[r] + (cx * nc + cy) * nc + cz;
+}
+
+// one thread per cell: the replica's atoms of this cell in ascending order
+__global__ void __launch_bounds__(kBlock)
+k_rep_cells(RepView v, const int32_t* __restrict__ cell_rep, int32_t ncell_tot, const int32_t* __restrict__ cell_of,
+            int32_t* __restrict__ members, int32_t* __restrict__ mcount, int32_t* __restrict__ err) {
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell_tot) return;
+    const int32_t r = cell_rep[c];
+    if (!v.rebuild[r]) return;
+    const int32_t b = v.off[r], e = b + v.nat[r];
+    int32_t k = 0;
+    for (int32_t i = b; i < e; i++) {
+        if (cell_of[i] == c) {
+            if (k < kCellCap) members[(int64_t)c * kCellCap + k] = i;
+            k++;
+        }
+    }
+    mcount[c] = k < kCellCap ? k : kCellCap;
+    if (k > kCellCap) atomicMax(&err[0], k);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_rep_list(RepView v, double* const* __restrict__ pos, double* __restrict__ r0x, double* __restrict__ r0y,
+           double* __restrict__ r0z, int32_t ntot, const int32_t* __restrict__ members,
+           const int32_t* __restrict__ mcount, double rs2, int32_t cap, int32_t* __restrict__ list,
+           int32_t* __restrict__ count, int32_t* __restrict__ err) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntot) return;
+    const int32_t r = v.rep_of[i];
+    if (r < 0 || !v.rebuild[r]) return;
+    const double* rx = pos[0];
+    const double* ry = pos[1];
+    const double* rz = pos[2];
+    const double L = v.L[r], invL = 1.0 / L;
+    const double xi = rx[i], yi = ry[i], zi = rz[i];
+    r0x[i] = xi;
+    r0y[i] = yi;
+    r0z[i] = zi;
+    int32_t k = 0;
+    auto test = [&](int32_t j) {
+        const double dx = mi(xi - rx[j], L, invL), dy = mi(yi - ry[j], L, invL), dz = mi(zi - rz[j], L, invL);
+        if (dx * dx + dy * dy + dz * dz < rs2) {
+            if (k < cap) list[(int64_t)k * ntot + i] = j;
+            k++;
+        }
+    };
+    const int32_t nc = v.nc[r];
+    if (nc == 0) {
+        const int32_t b = v.off[r], e = b + v.nat[r];
+        for (int32_t j = b; j < e; j++)
+            if (j != i) test(j);
+    } else {
+        const int32_t cx = wrap_cell(xi, L, nc), cy = wrap_cell(yi, L, nc), cz = wrap_cell(zi, L, nc);
+        for (int dx = -1; dx <= 1; dx++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dz = -1; dz <= 1; dz++) {
+                    const int32_t x = (cx + dx + nc) % nc, y = (cy + dy + nc) % nc, z = (cz + dz + nc) % nc;
+                    const int32_t c = v.coff[r] + (x * nc + y) * nc + z;
+                    const int32_t m = mcount[c];
+                    for (int32_t q = 0; q < m; q++) {
+                        const int32_t j = members[(int64_t)c * kCellCap + q];
+                        if (j != i) test(j);
+                    }
+                }
+    }
+    count[i] = k < cap ? k : cap;
+    if (k > cap) atomicMax(&err[1], k);
+}
+
+struct RepStep {
+    RepView v;
+    const double* r[3];   // r(t), read
+    double* rn[3];        // r(t+1), written
+    double* vel[3];
+    double* acc[3];
+    double* b3[3];
+    double* c3[3];
+    double* f[3];
+    const double* r0[3];
+    const int32_t* list;
+    const int32_t* count;
+    int32_t ntot;
+    double rc2, sig2, coeff;
+    float inv_sig2f, coeff_f;
+    double mass, scale;
+    GearCoeffs gc;
+    unsigned long long* dmax;   // R: max displacement^2 bits of r(t+1)
+};
+
+// kMode: 0 = corrector + next predictor, 1 = corrector only (last step),
+// 2 = forces and a = F/m only (start of a run)
+template <bool kFp64, int kMode>
+__global__ void __launch_bounds__(kBlock)
+k_rep_step(RepStep a) {
+    constexpr bool kFinal = kMode != 0;
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t r = i < a.ntot ? a.v.rep_of[i] : -1;
+    double d2 = 0.0;
+    if (r >= 0) {
+        const double L = a.v.L[r], invL = 1.0 / L;
+        const double xi = a.r[0][i], yi = a.r[1][i], zi = a.r[2][i];
+        double fx = 0, fy = 0, fz = 0;
+        const int32_t cnt = a.count[i];
+        for (int32_t k = 0; k < cnt; k++) {
+            const int32_t j = a.list[(int64_t)k * a.ntot + i];
+            const double dx = mi(xi - a.r[0][j], L, invL), dy = mi(yi - a.r[1][j], L, invL),
+                         dz = mi(zi - a.r[2][j], L, invL);
+            if constexpr (kFp64) {
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 < a.rc2) {
+                    const double q = r2 / a.sig2;
+                    const double iq4 = 1.0 / (q * q * q * q);
+                    const double iq7 = iq4 / (q * q * q);
+                    const double fr = (2 * iq7 - iq4) * a.coeff;
+                    fx += dx * fr; fy += dy * fr; fz += dz * fr;
+                }
+            } else {
+                const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
+                const float w = __builtin_amdgcn_rcpf((float)r2 * a.inv_sig2f);
+                const float w2 = w * w, w3 = w2 * w;
+                const float frf = r2 < a.rc2 ? w2 * w2 * __builtin_fmaf(2.0f, w3, -1.0f) * a.coeff_f : 0.0f;
+                const double fr = (double)frf;
+                fx = __builtin_fma(fr, dx, fx);
+                fy = __builtin_fma(fr, dy, fy);
+                fz = __builtin_fma(fr, dz, fz);
+            }
+        }
+        const double F[3] = {fx, fy, fz}, R0[3] = {xi, yi, zi};
+        if constexpr (kMode == 2) {
+            for (int d = 0; d < 3; d++) {
+                a.acc[d][i] = F[d] / a.mass * a.scale;
+                a.f[d][i] = F[d];
+            }
+            return;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            double rr = R0[d], v = a.vel[d][i], ac = a.acc[d][i], b = a.b3[d][i], c = a.c3[d][i];
+            // corrector (cpu/common/Device.cpp:58-84)
+            const double anew = F[d] / a.mass * a.scale;
+            const double corr = anew - ac;
+            rr = rr + a.gc.kc[0] * corr;
+            v = v + a.gc.kc[1] * corr;
+            ac = anew;
+            b = b + a.gc.kc[2] * corr;
+            c = c + a.gc.kc[3] * corr;
+            if constexpr (!kFinal) {
+                // predictor of the next step (cpu/common/Device.cpp:24-38)
+                const double rp = rr + a.gc.k[0] * v + a.gc.k[1] * ac + a.gc.k[2] * b + a.gc.k[3] * c;
+                v = v + a.gc.k[0] * ac + a.gc.k[1] * b + a.gc.k[2] * c;
+                ac = ac + a.gc.k[0] * b + a.gc.k[1] * c;
+                b = b + a.gc.k[0] * c;
+                const double dd = rp - a.r0[d][i];
+                d2 += dd * dd;
+                rr = rp;
+            }
+            a.rn[d][i] = rr;
+            a.vel[d][i] = v;
+            a.acc[d][i] = ac;
+            a.b3[d][i] = b;
+            a.c3[d][i] = c;
+            a.f[d][i] = F[d];
+        }
+    }
+    if constexpr (!kFinal) {
+        // a wave never spans two replicas (offsets are multiples of 64)
+        for (int o = kWave / 2; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o, kWave));
+        const int32_t r0 = __shfl(r, 0, kWave);
+        if ((threadIdx.x & (kWave - 1)) == 0 && r0 >= 0)
+            atomicMax(&a.dmax[r0], (unsigned long long)__double_as_longlong(d2));
+    }
+}
+
+// predictor (cpu/common/Device.cpp:17-39) with the per-replica largest
+// displacement of the predicted positions from the list-build positions
+__global__ void __launch_bounds__(kBlock)
+k_rep_predict(RepStep a) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t r = i < a.ntot ? a.v.rep_of[i] : -1;
+    double d2 = 0.0;
+    if (r >= 0) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            double* rp = const_cast<double*>(a.r[d]);
+            const double x = rp[i], v = a.vel[d][i], ac = a.acc[d][i], b = a.b3[d][i], c = a.c3[d][i];
+            const double xn = x + a.gc.k[0] * v + a.gc.k[1] * ac + a.gc.k[2] * b + a.gc.k[3] * c;
+            rp[i] = xn;
+            a.vel[d][i] = v + a.gc.k[0] * ac + a.gc.k[1] * b + a.gc.k[2] * c;
+            a.acc[d][i] = ac + a.gc.k[0] * b + a.gc.k[1] * c;
+            a.b3[d][i] = b + a.gc.k[0] * c;
+            const double dd = xn - a.r0[d][i];
+            d2 += dd * dd;
+        }
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o, kWave));
+    const int32_t r0 = __shfl(r, 0, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0 && r0 >= 0) atomicMax(&a.dmax[r0], (unsigned long long)__double_as_longlong(d2));
+}
+
+// a replica rebuilds before the next step when r(t+1) moved (rs - rc)/2 from
+// its list-build positions; clears the accumulator for the next step
+__global__ void k_rep_decide(int32_t R, unsigned long long* __restrict__ dmax, int32_t* __restrict__ rebuild,
+                             long long* __restrict__ builds, double margin2) {
+    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const double d2 = __longlong_as_double((long long)dmax[r]);
+    const int32_t f = d2 >= margin2 ? 1 : 0;
+    rebuild[r] = f;
+    builds[r] += f;
+    dmax[r] = 0ull;
+}
+
+__global__ void k_rep_set(int32_t R, int32_t* __restrict__ rebuild, int32_t v) {
+    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < R) rebuild[r] = v;
+}
+
+// per-replica kinetic and shifted-LJ potential energy (validation; FP64 atomics)
+__global__ void __launch_bounds__(kBlock)
+k_rep_energy(RepStep a, double shift, double e4, double* __restrict__ out) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.ntot) return;
+    const int32_t r = a.v.rep_of[i];
+    if (r < 0) return;
+    const double L = a.v.L[r], invL = 1.0 / L;
+    const double xi = a.r[0][i], yi = a.r[1][i], zi = a.r[2][i];
+    double u = 0;
+    const int32_t cnt = a.count[i];
+    for (int32_t k = 0; k < cnt; k++) {
+        const int32_t j = a.list[(int64_t)k * a.ntot + i];
+        const double dx = mi(xi - a.r[0][j], L, invL), dy = mi(yi - a.r[1][j], L, invL), dz = mi(zi - a.r[2][j], L, invL);
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < a.rc2) {
+            const double q = a.sig2 / r2, q3 = q * q * q;
+            u += 0.5 * (e4 * q3 * (q3 - 1) - shift);
+        }
+    }
+    const double ek = 0.5 * a.mass * (a.vel[0][i] * a.vel[0][i] + a.vel[1][i] * a.vel[1][i] + a.vel[2][i] * a.vel[2][i]);
+    atomicAdd(&out[2 * r], ek);
+    atomicAdd(&out[2 * r + 1], u);
+}
+
+}  // namespace
+
+struct mdx_batch {
+    mdx_config cfg{};
+    int32_t R = 0, ntot = 0, ncell_tot = 0;
+    hipStream_t stream = nullptr;
+    std::vector<int32_t> off, nat, coff, nc;
+    std::vector<double> L;
+    std::vector<void*> allocs;
+    int32_t *d_rep_of = nullptr, *d_off = nullptr, *d_nat = nullptr, *d_coff = nullptr, *d_nc = nullptr;
+    double* d_L = nullptr;
+    int32_t *d_rebuild = nullptr, *d_cell_rep = nullptr, *d_cell_of = nullptr, *d_members = nullptr, *d_mcount = nullptr;
+    int32_t *d_list = nullptr, *d_count = nullptr, *d_err = nullptr;
+    int32_t cap = 0;
+    long long* d_builds = nullptr;
+    unsigned long long* d_dmax = nullptr;
+    double* r[3] = {}; double* rn[3] = {}; double* v[3] = {}; double* a[3] = {}; double* b[3] = {}; double* c[3] = {};
+    double* f[3] = {}; double* r0[3] = {};
+    double** d_pos = nullptr;      // 2 x 3 device pointers: the position buffers
+    double* d_energy = nullptr;
+    GearCoeffs gc{};
+    bool initialized = false, flip = false;
+    hipGraphExec_t graph = nullptr;  // two steps (A -> B, B -> A)
+    bool graph_flip = false;
+    bool timing = false;
+    std::vector<std::pair<const char*, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<std::pair<std::string, std::pair<int64_t, double>>> times;
+    std::string err;
+};
+
+namespace {
+
+std::string g_batch_err;
+
+int bfail(mdx_batch* b, int code, const std::string& m) {
+    if (b) b->err = m; else g_batch_err = m;
+    return code;
+}
+
+#define BCHK(b, expr)                                                                            \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess) return bfail(b, MDX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T> int balloc(mdx_batch* b, T** p, size_t n) {
+    *p = nullptr;
+    if (hipMalloc((void**)p, (n ? n : 1) * sizeof(T)) != hipSuccess) return bfail(b, MDX_ERR_ALLOC, "hipMalloc failed");
+    b->allocs.push_back(*p);
+    return MDX_OK;
+}
+
+int balloc3(mdx_batch* b, double* p[3], int32_t n) {
+    int rc = balloc(b, &p[0], 3 * (size_t)n);
+    if (rc) return rc;
+    p[1] = p[0] + n;
+    p[2] = p[0] + 2 * (size_t)n;
+    return MDX_OK;
+}
+
+RepView view(const mdx_batch* b) {
+    return RepView{b->d_rep_of, b->d_off, b->d_nat, b->d_coff, b->d_nc, b->d_L, b->d_rebuild};
+}
+
+RepStep step_args(const mdx_batch* b, bool flip) {
+    RepStep s{};
+    s.v = view(b);
+    for (int d = 0; d < 3; d++) {
+        s.r[d] = flip ? b->rn[d] : b->r[d];
+        s.rn[d] = flip ? b->r[d] : b->rn[d];
+        s.vel[d] = b->v[d]; s.acc[d] = b->a[d]; s.b3[d] = b->b[d]; s.c3[d] = b->c[d]; s.f[d] = b->f[d];
+        s.r0[d] = b->r0[d];
+    }
+    s.list = b->d_list;
+    s.count = b->d_count;
+    s.ntot = b->ntot;
+    const mdx_config& c = b->cfg;
+    const double sg = c.lj_sigma > 0 ? c.lj_sigma : 1.0, ep = c.lj_eps > 0 ? c.lj_eps : 1.0;
+    s.rc2 = (double)c.lj_rcut * (double)c.lj_rcut;
+    s.sig2 = sg * sg;
+    s.coeff = 24.0 * ep / (sg * sg);
+    s.inv_sig2f = (float)(1.0 / (sg * sg));
+    s.coeff_f = (float)s.coeff;
+    s.mass = c.mass > 0 ? c.mass : 1.0;
+    s.scale = c.accel_scale > 0 ? c.accel_scale : 1.0;
+    s.gc = b->gc;
+    s.dmax = b->d_dmax;
+    return s;
+}
+
+inline int grid_for(int64_t n) { return (int)std::max<int64_t>(1, (n + kBlock - 1) / kBlock); }
+
+template <class F> hipError_t btimed(mdx_batch* b, const char* name, F&& launch) {
+    if (!b->timing) return launch();
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, b->stream);
+    hipError_t e = launch();
+    hipEventRecord(e1, b->stream);
+    b->pending.push_back({name, {e0, e1}});
+    return e;
+}
+#define BLAUNCH(b, name, expr) BCHK(b, btimed(b, name, [&]() { (expr); return hipGetLastError(); }))
+
+// gated list rebuild of the flagged replicas
+int enqueue_rebuild(mdx_batch* b, bool flip) {
+    hipStream_t s = b->stream;
+    const RepView v = view(b);
+    double* const* pos = b->d_pos + (flip ? 3 : 0);
+    double* const* hp = flip ? b->rn : b->r;
+    const double rs = b->cfg.lj_rs;
+    BLAUNCH(b, "rep_bin", (k_rep_bin<<<grid_for(b->ntot), kBlock, 0, s>>>(v, hp[0], hp[1], hp[2], b->ntot, b->d_cell_of)));
+    BLAUNCH(b, "rep_cells", (k_rep_cells<<<grid_for(b->ncell_tot), kBlock, 0, s>>>(v, b->d_cell_rep, b->ncell_tot,
+                                                                                   b->d_cell_of, b->d_members,
+                                                                                   b->d_mcount, b->d_err)));
+    BLAUNCH(b, "rep_list", (k_rep_list<<<grid_for(b->ntot), kBlock, 0, s>>>(v, pos, b->r0[0], b->r0[1], b->r0[2], b->ntot,
+                                                                            b->d_members, b->d_mcount, rs * rs, b->cap,
+                                                                            b->d_list, b->d_count, b->d_err)));
+    return MDX_OK;
+}
+
+int enqueue_decide(mdx_batch* b) {
+    const double margin = 0.5 * ((double)b->cfg.lj_rs - (double)b->cfg.lj_rcut);
+    BLAUNCH(b, "rep_decide", (k_rep_decide<<<(b->R + 255) / 256, 256, 0, b->stream>>>(b->R, b->d_dmax, b->d_rebuild,
+                                                                                       b->d_builds, margin * margin)));
+    return MDX_OK;
+}
+
+int enqueue_step(mdx_batch* b, bool flip, bool fin) {
+    hipStream_t s = b->stream;
+    int rc = enqueue_rebuild(b, flip);
+    if (rc) return rc;
+    const RepStep a = step_args(b, flip);
+    const bool fp64 = b->cfg.precision == MDX_PREC_FP64;
+    const int g = grid_for(b->ntot);
+    if (fin) {
+        if (fp64) BLAUNCH(b, "rep_step", (k_rep_step<true, 1><<<g, kBlock, 0, s>>>(a)));
+        else BLAUNCH(b, "rep_step", (k_rep_step<false, 1><<<g, kBlock, 0, s>>>(a)));
+        return MDX_OK;
+    }
+    if (fp64) BLAUNCH(b, "rep_step", (k_rep_step<true, 0><<<g, kBlock, 0, s>>>(a)));
+    else BLAUNCH(b, "rep_step", (k_rep_step<false, 0><<<g, kBlock, 0, s>>>(a)));
+    return enqueue_decide(b);
+}
+
+int check_err(mdx_batch* b) {
+    int32_t e[2];
+    BCHK(b, hipMemcpyAsync(e, b->d_err, sizeof e, hipMemcpyDeviceToHost, b->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    if (e[0]) return bfail(b, MDX_ERR_CAPACITY, "a replica cell holds more than 64 atoms");
+    if (e[1]) return bfail(b, MDX_ERR_CAPACITY, "a neighbour list exceeded max_neighbors (longest " + std::to_string(e[1]) + ")");
+    return MDX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t* n_atoms, const double* box_length,
+                     mdx_batch** out) {
+    if (!out) return bfail(nullptr, MDX_ERR_ARG, "null output");
+    *out = nullptr;
+    if (!common || n_replicas <= 0 || !n_atoms || !box_length) return bfail(nullptr, MDX_ERR_ARG, "bad arguments");
+    const mdx_config& c = *common;
+    if (!(c.lj_rcut > 0 && c.lj_rs >= c.lj_rcut)) return bfail(nullptr, MDX_ERR_ARG, "need 0 < lj_rcut <= lj_rs");
+    if (c.precision != MDX_PREC_MIXED && c.precision != MDX_PREC_FP64) return bfail(nullptr, MDX_ERR_ARG, "unknown precision");
+    mdx_batch* b = new mdx_batch;
+    b->cfg = c;
+    b->R = n_replicas;
+    int64_t off = 0, coff = 0;
+    for (int32_t r = 0; r < n_replicas; r++) {
+        const double L = box_length[r];
+        if (n_atoms[r] <= 0 || !(L >= 2.0 * c.lj_rs)) {
+            delete b;
+            return bfail(nullptr, MDX_ERR_ARG, "replica " + std::to_string(r) + ": need n > 0 and box >= 2 rs");
+        }
+        b->off.push_back((int32_t)off);
+        b->nat.push_back(n_atoms[r]);
+        b->L.push_back(L);
+        const int32_t nc = (int32_t)std::floor(L / (double)c.lj_rs);
+        b->nc.push_back(nc >= 3 ? nc : 0);
+        b->coff.push_back((int32_t)coff);
+        coff += nc >= 3 ? (int64_t)nc * nc * nc : 0;
+        off += ((int64_t)n_atoms[r] + kWave - 1) / kWave * kWave;
+    }
+    if (off > INT32_MAX / 2 || coff > (int64_t(1) << 27)) {
+        delete b;
+        return bfail(nullptr, MDX_ERR_ARG, "batch too large");
+    }
+    b->off.push_back((int32_t)off);
+    b->coff.push_back((int32_t)coff);
+    b->ntot = (int32_t)off;
+    b->ncell_tot = (int32_t)coff;
+    auto bail = [&](int rc) { mdx_batch_destroy(b); return rc; };
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bail(bfail(nullptr, MDX_ERR_HIP, "stream"));
+    // a list capacity from the densest replica's mean count inside rs, with slack
+    double rho_max = 0;
+    for (int32_t r = 0; r < n_replicas; r++) rho_max = std::max(rho_max, n_atoms[r] / std::pow(b->L[r], 3));
+    b->cap = c.max_neighbors > 0 ? c.max_neighbors
+                                 : (int32_t)(1.5 * 4.0 / 3.0 * M_PI * std::pow((double)c.lj_rs, 3) * rho_max) + 32;
+    std::vector<int32_t> rep_of(b->ntot, -1), cell_rep(std::max<int32_t>(b->ncell_tot, 1), 0);
+    for (int32_t r = 0; r < n_replicas; r++) {
+        for (int32_t i = 0; i < n_atoms[r]; i++) rep_of[b->off[r] + i] = r;
+        for (int32_t q = b->coff[r]; q < b->coff[r + 1]; q++) cell_rep[q] = r;
+    }
+    const int32_t n = b->ntot;
+    int rc;
+    if ((rc = balloc(b, &b->d_rep_of, n)) || (rc = balloc(b, &b->d_off, n_replicas + 1)) ||
+        (rc = balloc(b, &b->d_nat, n_replicas)) || (rc = balloc(b, &b->d_coff, n_replicas + 1)) ||
+        (rc = balloc(b, &b->d_nc, n_replicas)) || (rc = balloc(b, &b->d_L, n_replicas)) ||
+        (rc = balloc(b, &b->d_rebuild, n_replicas)) || (rc = balloc(b, &b->d_builds, n_replicas)) ||
+        (rc = balloc(b, &b->d_dmax, n_replicas)) || (rc = balloc(b, &b->d_cell_rep, std::max<int32_t>(b->ncell_tot, 1))) ||
+        (rc = balloc(b, &b->d_cell_of, n)) ||
+        (rc = balloc(b, &b->d_members, (size_t)std::max<int32_t>(b->ncell_tot, 1) * kCellCap)) ||
+        (rc = balloc(b, &b->d_mcount, std::max<int32_t>(b->ncell_tot, 1))) ||
+        (rc = balloc(b, &b->d_list, (size_t)b->cap * n)) || (rc = balloc(b, &b->d_count, n)) ||
+        (rc = balloc(b, &b->d_err, 2)) || (rc = balloc(b, &b->d_pos, 6)) ||
+        (rc = balloc(b, &b->d_energy, 2 * (size_t)n_replicas)) ||
+        (rc = balloc3(b, b->r, n)) || (rc = balloc3(b, b->rn, n)) || (rc = balloc3(b, b->v, n)) ||
+        (rc = balloc3(b, b->a, n)) || (rc = balloc3(b, b->b, n)) || (rc = balloc3(b, b->c, n)) ||
+        (rc = balloc3(b, b->f, n)) || (rc = balloc3(b, b->r0, n))) {
+        g_batch_err = b->err;
+        return bail(rc);
+    }
+    double* hp[6] = {b->r[0], b->r[1], b->r[2], b->rn[0], b->rn[1], b->rn[2]};
+    hipStream_t s = b->stream;
+    bool ok = hipMemcpyAsync(b->d_rep_of, rep_of.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(b->d_off, b->off.data(), sizeof(int32_t) * (n_replicas + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(b->d_nat, b->nat.data(), sizeof(int32_t) * n_replicas, hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(b->d_coff, b->coff.data(), sizeof(int32_t) * (n_replicas + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(b->d_nc, b->nc.data(), sizeof(int32_t) * n_replicas, hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(b->d_L, b->L.data(), sizeof(double) * n_replicas, hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(b->d_cell_rep, cell_rep.data(), sizeof(int32_t) * cell_rep.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(b->d_pos, hp, sizeof hp, hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemsetAsync(b->d_builds, 0, sizeof(long long) * n_replicas, s) == hipSuccess &&
+              hipMemsetAsync(b->d_dmax, 0, sizeof(unsigned long long) * n_replicas, s) == hipSuccess &&
+              hipMemsetAsync(b->d_err, 0, 2 * sizeof(int32_t), s) == hipSuccess &&
+              hipMemsetAsync(b->d_count, 0, sizeof(int32_t) * n, s) == hipSuccess &&
+              hipStreamSynchronize(s) == hipSuccess;
+    if (!ok) return bail(bfail(nullptr, MDX_ERR_HIP, "batch upload failed"));
+    b->gc = gear_coeffs(c.dt);
+    *out = b;
+    return MDX_OK;
+}
+
+int mdx_batch_destroy(mdx_batch* b) {
+    if (!b) return MDX_OK;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->graph) hipGraphExecDestroy(b->graph);
+    for (void* p : b->allocs) hipFree(p);
+    for (auto& p : b->pending) { hipEventDestroy(p.second.first); hipEventDestroy(p.second.second); }
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return MDX_OK;
+}
+
+const char* mdx_batch_last_error(const mdx_batch* b) { return b ? b->err.c_str() : g_batch_err.c_str(); }
+
+int mdx_batch_set_state(mdx_batch* b, const double* pos3, const double* vel3) {
+    if (!b || !pos3 || !vel3) return bfail(b, MDX_ERR_ARG, "pos3 and vel3 are required");
+    const int32_t n = b->ntot;
+    std::vector<double> buf(3 * (size_t)n, 0.0), vb(3 * (size_t)n, 0.0);
+    int64_t src = 0;
+    for (int32_t r = 0; r < b->R; r++)
+        for (int32_t i = 0; i < b->nat[r]; i++, src++)
+            for (int d = 0; d < 3; d++) {
+                buf[(size_t)d * n + b->off[r] + i] = pos3[3 * src + d];
+                vb[(size_t)d * n + b->off[r] + i] = vel3[3 * src + d];
+            }
+    hipStream_t s = b->stream;
+    // a fresh start: the state is (re)loaded into buffer A, Gear history cleared
+    BCHK(b, hipMemcpyAsync(b->r[0], buf.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+    BCHK(b, hipMemcpyAsync(b->v[0], vb.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+    for (double* p : {b->a[0], b->b[0], b->c[0], b->f[0]}) BCHK(b, hipMemsetAsync(p, 0, sizeof(double) * 3 * n, s));
+    BCHK(b, hipStreamSynchronize(s));
+    b->flip = false;
+    b->initialized = false;
+    return MDX_OK;
+}
+
+int mdx_batch_get_state(mdx_batch* b, double* pos3, double* vel3, double* force3) {
+    if (!b) return bfail(b, MDX_ERR_ARG, "null batch");
+    const int32_t n = b->ntot;
+    hipStream_t s = b->stream;
+    auto get = [&](double* const* src, double* dst) -> int {
+        if (!dst) return MDX_OK;
+        std::vector<double> buf(3 * (size_t)n);
+        BCHK(b, hipMemcpyAsync(buf.data(), src[0], sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+        BCHK(b, hipStreamSynchronize(s));
+        int64_t q = 0;
+        for (int32_t r = 0; r < b->R; r++)
+            for (int32_t i = 0; i < b->nat[r]; i++, q++)
+                for (int d = 0; d < 3; d++) dst[3 * q + d] = buf[(size_t)d * n + b->off[r] + i];
+        return MDX_OK;
+    };
+    int rc;
+    if ((rc = get(b->flip ? b->rn : b->r, pos3)) || (rc = get(b->v, vel3)) || (rc = get(b->f, force3))) return rc;
+    return MDX_OK;
+}
+
+// The first call after set_state builds all lists and computes a = F/m.  Each
+// call then runs the predictor of its first step and nsteps steps of [gated
+// rebuild] + step kernel + decision (the last one corrector-only, leaving the
+// corrected state).  Without timing, pairs of steps replay one captured graph.
+int mdx_batch_run(mdx_batch* b, int32_t nsteps) {
+    if (!b) return bfail(b, MDX_ERR_ARG, "null batch");
+    if (nsteps < 0) return bfail(b, MDX_ERR_ARG, "nsteps must be >= 0");
+    hipStream_t s = b->stream;
+    int rc;
+    if (!b->initialized) {
+        // first call after set_state: lists for all replicas, forces, a = F/m
+        // (cpu/cpumd/Host.cpp:132-138)
+        BLAUNCH(b, "rep_set", (k_rep_set<<<(b->R + 255) / 256, 256, 0, s>>>(b->R, b->d_rebuild, 1)));
+        if ((rc = enqueue_rebuild(b, b->flip))) return rc;
+        if ((rc = check_err(b))) return rc;
+        const RepStep a = step_args(b, b->flip);
+        if (b->cfg.precision == MDX_PREC_FP64)
+            BLAUNCH(b, "rep_force0", (k_rep_step<true, 2><<<grid_for(b->ntot), kBlock, 0, s>>>(a)));
+        else
+            BLAUNCH(b, "rep_force0", (k_rep_step<false, 2><<<grid_for(b->ntot), kBlock, 0, s>>>(a)));
+        b->initialized = true;
+    }
+    if (nsteps == 0) { BCHK(b, hipStreamSynchronize(s)); return MDX_OK; }
+    // predictor of this call's first step (in place), then the exact rebuild decision
+    BLAUNCH(b, "rep_predict", (k_rep_predict<<<grid_for(b->ntot), kBlock, 0, s>>>(step_args(b, b->flip))));
+    if ((rc = enqueue_decide(b))) return rc;
+    int32_t k = 0;
+    const int32_t body = nsteps - 1;   // the last step is the corrector-only variant
+    if (!b->timing) {
+        if (!b->graph || b->graph_flip != b->flip) {
+            if (b->graph) hipGraphExecDestroy(b->graph);
+            b->graph = nullptr;
+            hipGraph_t g;
+            BCHK(b, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int r1 = enqueue_step(b, b->flip, false);
+            int r2 = r1 ? r1 : enqueue_step(b, !b->flip, false);
+            hipError_t e = hipStreamEndCapture(s, &g);
+            if (r1 || r2) return r1 ? r1 : r2;
+            if (e != hipSuccess) return bfail(b, MDX_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (e != hipSuccess) return bfail(b, MDX_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+            b->graph_flip = b->flip;
+        }
+        for (; k + 2 <= body; k += 2) BCHK(b, hipGraphLaunch(b->graph, s));
+    }
+    for (; k < body; k++) {
+        if ((rc = enqueue_step(b, b->flip, false))) return rc;
+        b->flip = !b->flip;
+    }
+    if ((rc = enqueue_step(b, b->flip, true))) return rc;
+    b->flip = !b->flip;
+    // the state is corrected r(t); the next call continues with the predictor
+    return check_err(b);
+}
+
+int mdx_batch_build_counts(mdx_batch* b, int64_t* builds) {
+    if (!b || !builds) return bfail(b, MDX_ERR_ARG, "null argument");
+    std::vector<long long> h(b->R);
+    BCHK(b, hipMemcpyAsync(h.data(), b->d_builds, sizeof(long long) * b->R, hipMemcpyDeviceToHost, b->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    for (int32_t r = 0; r < b->R; r++) builds[r] = h[r];
+    return MDX_OK;
+}
+
+int mdx_batch_energies(mdx_batch* b, double* out2) {
+    if (!b || !out2) return bfail(b, MDX_ERR_ARG, "null argument");
+    hipStream_t s = b->stream;
+    BCHK(b, hipMemsetAsync(b->d_energy, 0, sizeof(double) * 2 * b->R, s));
+    const RepStep a = step_args(b, b->flip);
+    const mdx_config& c = b->cfg;
+    const double sg = c.lj_sigma > 0 ? c.lj_sigma : 1.0, ep = c.lj_eps > 0 ? c.lj_eps : 1.0;
+    const double qc = a.rc2 / (sg * sg), iq6 = 1.0 / (qc * qc * qc);
+    BLAUNCH(b, "rep_energy", (k_rep_energy<<<grid_for(b->ntot), kBlock, 0, s>>>(a, 4 * ep * iq6 * (iq6 - 1), 4 * ep,
+                                                                                 b->d_energy)));
+    BCHK(b, hipMemcpyAsync(out2, b->d_energy, sizeof(double) * 2 * b->R, hipMemcpyDeviceToHost, s));
+    BCHK(b, hipStreamSynchronize(s));
+    return MDX_OK;
+}
+
+int mdx_batch_set_timing(mdx_batch* b, int32_t enabled) {
+    if (!b) return bfail(b, MDX_ERR_ARG, "null batch");
+    b->timing = enabled != 0;
+    return MDX_OK;
+}
+
+int mdx_batch_kernel_times(mdx_batch* b, char* names, int64_t* counts, double* total_ms, int32_t cap, int32_t* n) {
+    if (!b) return bfail(b, MDX_ERR_ARG, "null batch");
+    BCHK(b, hipStreamSynchronize(b->stream));
+    for (auto& p : b->pending) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, p.second.first, p.second.second);
+        auto it = std::find_if(b->times.begin(), b->times.end(), [&](auto& t) { return t.first == p.first; });
+        if (it == b->times.end()) { b->times.push_back({p.first, {0, 0.0}}); it = b->times.end() - 1; }
+        it->second.first += 1;
+        it->second.second += ms;
+        hipEventDestroy(p.second.first);
+        hipEventDestroy(p.second.second);
+    }
+    b->pending.clear();
+    int32_t k = 0;
+    for (auto& t : b->times) {
+        if (k >= cap) break;
+        std::strncpy(names + 48 * k, t.first.c_str(), 47);
+        names[48 * k + 47] = 0;
+        counts[k] = t.second.first;
+        total_ms[k] = t.second.second;
+        k++;
+    }
+    if (n) *n = k;
+    b->times.clear();
+    return MDX_OK;
+}
+
+int mdx_batch_synchronize(mdx_batch* b) {
+    if (!b) return bfail(b, MDX_ERR_ARG, "null batch");
+    BCHK(b, hipStreamSynchronize(b->stream));
+    return MDX_OK;
+}
+
+}  // extern "C"

@@ -75,3 +75,24 @@ def si_diamond(ncell: int, temperature_k: float, seed: int, disp: float = 0.05):
     kT_over_m = KB_EV * temperature_k / SI_MASS * ACCEL_EV_A_AMU   # A^2 / fs^2
     vel = maxwell_boltzmann(len(pos), kT_over_m, seed + 1)
     return pos, vel, L
+
+
+def replica_batch(n_replicas: int, seed: int, n_min: int = 256, n_max: int = 8192):
+    """Replica sizes log-uniform in [n_min, n_max], each rounded to an FCC block
+    of 4 k^3 atoms, density U[0.70, 0.95] and T* U[0.5, 2.0], all from one seed.
+    Returns (n_atoms, box_lengths, positions, velocities) with the replicas'
+    atoms one after another."""
+    rng = np.random.default_rng(seed)
+    ns, boxes, pos, vel = [], [], [], []
+    for r in range(n_replicas):
+        target = np.exp(rng.uniform(np.log(n_min), np.log(n_max)))
+        k = max(4, int(round((target / 4) ** (1 / 3))))
+        rho = rng.uniform(0.70, 0.95)
+        temp = rng.uniform(0.5, 2.0)
+        p, L = fcc(k, rho)
+        v = maxwell_boltzmann(len(p), temp, int(rng.integers(1 << 30)))
+        ns.append(len(p))
+        boxes.append(L)
+        pos.append(p)
+        vel.append(v)
+    return np.array(ns, dtype=np.int32), np.array(boxes), np.concatenate(pos), np.concatenate(vel)
