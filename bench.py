@@ -14,6 +14,11 @@ this build runs on one GPU.
 silicon (R = 2.7 A, D = 0.3 A), 262,144 atoms of diamond (32^3 cells of
 5.431 A) with 0.05 A Gaussian displacements and 1000 K velocities, list
 radius 3.3 A, dt = 1 fs, mixed precision.
+
+--workload replicas measures the batched configuration: 512 independent
+periodic LJ replicas of 256-8788 atoms (log-uniform sizes, density
+U[0.70, 0.95], T* U[0.5, 2.0]; 1.13M atoms together), rc 2.5, rs 2.8,
+dt 0.005, mixed precision, advanced by graph replays.
 """
 from __future__ import annotations
 
@@ -62,7 +67,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-ncell", type=int, default=32)
     p.add_argument("--cpu-steps", type=int, default=20)
-    p.add_argument("--workload", choices=["lj", "tersoff"], default="lj")
+    p.add_argument("--workload", choices=["lj", "tersoff", "replicas"], default="lj")
     p.add_argument("--copy-bytes", type=int, default=0,
                    help="only run device-to-device copies of this many bytes (counter calibration)")
     return p.parse_args()
@@ -195,6 +200,103 @@ def bench_tersoff(a) -> None:
     print(json.dumps(out))
 
 
+# one batched LJ step per atom (k_rep_step): the Gear state r, v, a, b, c read
+# and written (240 B), positions written to the second buffer (24 B is in
+# that), the force written (24 B), list-build positions read (24 B), the
+# replica index (4 B); neighbour positions are L2 hits and not counted
+REP_STEP_BYTES_PER_ATOM = 240 + 24 + 24 + 4
+
+
+def cpu_baseline_replicas(ns, Ls, pos, vel, max_atoms: int, steps: int) -> dict:
+    """The FP64 oracle's periodic LJ loop (27-cell lists, Gear PC) on the
+    first replicas wide enough for its 3-cell grid, about max_atoms atoms."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import _oracle as O
+
+    off, done, atoms, elapsed = 0, 0, 0, 0.0
+    for n, L in zip(ns, Ls):
+        if atoms >= max_atoms:
+            break
+        if int(np.floor(L / 2.8)) >= 3:
+            _, _, _, _, el = O.pbc_lj_run(pos[off: off + n], vel[off: off + n], L, 2.5, 2.8, DT, steps)
+            elapsed += el
+            atoms += n
+            done += 1
+        off += n
+    return {
+        "value": atoms * steps / elapsed,
+        "unit": "atom-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{done} replicas of the batch ({atoms} atoms), {steps} steps each, {elapsed:.1f} s on "
+                  f"{platform.processor() or platform.machine()}",
+    }
+
+
+def bench_replicas(a) -> None:
+    from moleculardynamics_amd import PREC_MIXED, Batch, initial
+
+    ns, Ls, pos, vel = initial.replica_batch(512, seed=2024)
+    n = int(ns.sum())
+    print(f"bench: {len(ns)} LJ replicas, {n} atoms; warmup {a.warmup}, steps {a.steps}", file=sys.stderr, flush=True)
+    b = Batch(ns, Ls, precision=PREC_MIXED, dt=0.005, lj_rs=2.8, lj_rcut=2.5)
+    b.set_state(pos, vel)
+    b.run(a.warmup)
+    b.synchronize()
+    builds0 = b.build_counts().sum()
+    t0 = time.perf_counter()
+    b.run(a.steps)                      # graph replays
+    b.synchronize()
+    t1 = time.perf_counter()
+    builds = int(b.build_counts().sum() - builds0)
+    elapsed = t1 - t0
+    value = n * a.steps / elapsed
+    # the same steps launched one by one with events around each kernel, for the per-kernel record
+    b.set_timing(True)
+    b.run(a.steps)
+    b.synchronize()
+    times = b.kernel_times()
+    b.set_timing(False)
+    b.close()
+    print(f"bench: {elapsed:.3f} s (graph), {builds} replica list builds; kernels {times}", file=sys.stderr, flush=True)
+    cnt, ms = times.get("rep_step", (0, 0.0))
+    k_ms = ms / max(cnt, 1)
+    achieved = n * REP_STEP_BYTES_PER_ATOM / (k_ms * 1e-3) / 1e9 if cnt else None
+    out = {
+        "metric": "atom-steps/s per GPU (512 LJ replicas, mixed precision)",
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * elapsed / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-pair/fp64-accum",
+        "data": "synthetic (seeded FCC blocks + Maxwell-Boltzmann velocities per replica)",
+        "config": {"workload": f"{len(ns)} periodic LJ replicas, {n} atoms ({int(ns.min())}-{int(ns.max())} per replica), "
+                               f"rho U[0.70,0.95], T* U[0.5,2.0], rc 2.5 skin 0.3, dt 0.005", "atoms": n,
+                   "replica_list_builds": builds},
+        "roofline": {
+            "kernel": "rep_step (k_rep_step)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": None,
+            "avg_launch_ms": k_ms,
+            "bytes_per_atom": REP_STEP_BYTES_PER_ATOM,
+        },
+        "kernel_ms": {k: v[1] / max(v[0], 1) for k, v in times.items()},
+    }
+    if not a.no_cpu:
+        print("bench: cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline_replicas(ns, Ls, pos, vel, 60000, 20)
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -205,6 +307,9 @@ def main():
         return
     if a.workload == "tersoff":
         bench_tersoff(a)
+        return
+    if a.workload == "replicas":
+        bench_replicas(a)
         return
     from moleculardynamics_amd import BOX_PERIODIC, LJ_UNIFORM, PREC_MIXED, System, initial
 

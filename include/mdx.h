@@ -197,7 +197,8 @@ int mdx_synchronize(mdx_system* sys);
  * advances all of them with one graph launch per two steps.  From `common`:
  * precision, dt, lj_rs, lj_rcut, lj_sigma, lj_eps, mass, accel_scale,
  * max_neighbors (0: from the densest replica); the other fields are ignored.
- * Every box must be at least 2 lj_rs wide (minimum image).  Lists are rebuilt
+ * Every box must be at least 2 lj_rs wide (minimum image) and hold at most
+ * 16384 atoms.  Lists are rebuilt
  * per replica, decided on the device from the predicted positions: exactly
  * when an atom moved (lj_rs - lj_rcut)/2 since its replica's last build.
  * State arrays hold the replicas one after another, 3 doubles per atom.

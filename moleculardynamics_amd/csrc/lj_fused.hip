@@ -263,21 +263,10 @@ k_nlist_lds(SegGrid g, const double* __restrict__ rx, const double* __restrict__
                     const bool hit = (p < e) & (p != self) & (r2 < rl2);
                     mask |= (uint32_t)hit << u;
                 }
-                // hits are numbered lane-major within the quad: exclusive prefix
-                // of the lane counts by quad permutations [0,0,1,2], [0,1,0,1]
-                // and the total from lane 3 ([3,3,3,3])
-                // (each permutation must run in all 4 lanes: a lane reading a
-                // disabled lane gets a stale value, so the empty asm pins the moves
-                // ahead of the per-lane select instead of under it)
+                // hits are numbered lane-major within the quad
                 const int32_t h = __popc(mask);
-                int32_t t1 = __builtin_amdgcn_mov_dpp(h, 0x90, 0xF, 0xF, false);
-                asm volatile("" : "+v"(t1));
-                int32_t incl = h + (q >= 1 ? t1 : 0);
-                int32_t t2 = __builtin_amdgcn_mov_dpp(incl, 0x44, 0xF, 0xF, false);
-                asm volatile("" : "+v"(t2));
-                incl += q >= 2 ? t2 : 0;
-                const int32_t total = __builtin_amdgcn_mov_dpp(incl, 0xFF, 0xF, 0xF, false);
-                int32_t kk = k + incl - h;
+                int32_t total;
+                int32_t kk = k + quad_excl(h, q, total);
                 // store the set bits only (about 1.5 of 8 per lane)
                 for (uint32_t mb = mask; mb; mb &= mb - 1, kk++) {
                     const int32_t u = __builtin_ctz(mb);
@@ -319,21 +308,6 @@ struct StepArgs {
     int32_t n;
     int32_t lds_cap;             // staged atoms the dynamic LDS holds
 };
-
-// a double from the lane given by a quad permutation (DPP, no LDS traffic)
-template <int kCtrl>
-__device__ inline double quad_perm(double v) {
-    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), kCtrl, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), kCtrl, 0xF, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-// sum over the 4 lanes of a quad, (v0 + v1) + (v2 + v3): every lane gets the
-// same bits.  Quad permutations [1,0,3,2] = 0xB1 and [2,3,0,1] = 0x4E.
-__device__ inline double quad_sum(double v) {
-    v += quad_perm<0xB1>(v);
-    v += quad_perm<0x4E>(v);
-    return v;
-}
 
 __device__ inline double pick3(int32_t q, double a, double b, double c) { return q == 0 ? a : (q == 1 ? b : c); }
 

@@ -21,6 +21,35 @@ namespace mdx {
 
 constexpr int kWave = 64;
 
+// Quads of 4 consecutive lanes serving one atom.  All four lanes must be
+// active: a DPP move reading a disabled lane gets a stale value.
+template <int kCtrl>
+__device__ inline double quad_perm(double v) {   // kCtrl: DPP quad permutation
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), kCtrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), kCtrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+// (v0 + v1) + (v2 + v3) in every lane, the same bits in all four:
+// permutations [1,0,3,2] = 0xB1, [2,3,0,1] = 0x4E
+__device__ inline double quad_sum(double v) {
+    v += quad_perm<0xB1>(v);
+    v += quad_perm<0x4E>(v);
+    return v;
+}
+// exclusive prefix of h over the quad in lane order (q = lane & 3) and the
+// quad total; the empty asm keeps each permutation ahead of the per-lane
+// select, executed by all four lanes
+__device__ inline int32_t quad_excl(int32_t h, int32_t q, int32_t& total) {
+    int32_t t1 = __builtin_amdgcn_mov_dpp(h, 0x90, 0xF, 0xF, false);   // [0,0,1,2]
+    asm volatile("" : "+v"(t1));
+    int32_t incl = h + (q >= 1 ? t1 : 0);
+    int32_t t2 = __builtin_amdgcn_mov_dpp(incl, 0x44, 0xF, 0xF, false);   // [0,1,0,1]
+    asm volatile("" : "+v"(t2));
+    incl += q >= 2 ? t2 : 0;
+    total = __builtin_amdgcn_mov_dpp(incl, 0xFF, 0xF, 0xF, false);     // [3,3,3,3]
+    return incl - h;
+}
+
 // Per-atom SoA state (slot order).
 struct DevState {
     double* r[3];

@@ -10,10 +10,12 @@
 //    off[r] is a multiple of 64, so every wave belongs to exactly one replica
 //    (padding atoms are inactive);
 //  * lists are rebuilt per replica, gated by a device flag, so one static
-//    graph serves all steps: k_rep_bin (cell of each atom), k_rep_cells (the
-//    members of each cell in ascending atom order: deterministic), k_rep_list
-//    (compact list within rs from the 27 cells, or from all atoms of the
-//    replica when its box holds fewer than 3 cells per axis; minimum image);
+//    graph serves all steps: k_rep_sort (one workgroup per replica: cell
+//    histogram and scan in LDS, scatter, each cell sorted by atom index --
+//    deterministic -- and a cell-ordered FP32 copy of the wrapped positions),
+//    then k_rep_list (compact list within rs from the 27 neighbouring cells'
+//    contiguous position ranges, or from the whole replica when its box holds
+//    fewer than 3 cells per axis; minimum image);
 //  * k_rep_step: LJ force over the list (minimum image), Gear corrector of
 //    step t and predictor of step t+1; positions are double-buffered; the
 //    largest displacement of r(t+1) from the list-build positions is
@@ -36,106 +38,175 @@ using namespace mdx;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kCellCap = 64;   // members per cell (a cell of edge <= 3.7 sigma at rho <= 1.2 holds < 64)
+constexpr int kSortBlock = 256;
+constexpr int kMaxNc = 10;                               // cells per axis at most (larger cells stay correct)
+constexpr int kMaxCells = kMaxNc * kMaxNc * kMaxNc;
+constexpr int kMaxReplica = 16384;                       // atoms per replica (the LDS sort holds them all)
 
 struct RepView {
     const int32_t* rep_of;     // replica of each atom slot (-1: padding)
     const int32_t* off;        // R + 1 atom offsets
     const int32_t* nat;        // R atom counts
-    const int32_t* coff;       // R + 1 cell offsets
-    const int32_t* nc;         // R cells per axis (0: all pairs)
+    const int32_t* coff;       // R + 1 offsets of the per-replica cell-start arrays (ncell + 1 entries each)
+    const int32_t* nc;         // R cells per axis (0: one cell, all pairs)
     const double* L;           // R box edges
     const int32_t* rebuild;    // R flags
 };
 
 __device__ inline double mi(double d, double L, double invL) { return d - L * rint(d * invL); }
 
-__device__ inline int32_t wrap_cell(double x, double L, int32_t nc) {
-    double y = x - L * floor(x / L);
+__device__ inline double wrap(double x, double L) { return x - L * floor(x / L); }
+
+__device__ inline int32_t cell_index(double y, double L, int32_t nc) {   // y already wrapped into [0, L)
     int32_t c = (int32_t)(y * nc / L);
     return c >= nc ? nc - 1 : (c < 0 ? 0 : c);
 }
 
-__global__ void __launch_bounds__(kBlock)
-k_rep_bin(RepView v, const double* __restrict__ rx, const double* __restrict__ ry, const double* __restrict__ rz,
-          int32_t ntot, int32_t* __restrict__ cell_of) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ntot) return;
-    const int32_t r = v.rep_of[i];
-    if (r < 0 || !v.rebuild[r]) return;
-    const int32_t nc = v.nc[r];
-    if (nc == 0) return;
-    const double L = v.L[r];
-    const int32_t cx = wrap_cell(rx[i], L, nc), cy = wrap_cell(ry[i], L, nc), cz = wrap_cell(rz[i], L, nc);
-    cell_of[i] = v.coff[r] + (cx * nc + cy) * nc + cz;
-}
-
-// one thread per cell: the replica's atoms of this cell in ascending order
-__global__ void __launch_bounds__(kBlock)
-k_rep_cells(RepView v, const int32_t* __restrict__ cell_rep, int32_t ncell_tot, const int32_t* __restrict__ cell_of,
-            int32_t* __restrict__ members, int32_t* __restrict__ mcount, int32_t* __restrict__ err) {
-    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell_tot) return;
-    const int32_t r = cell_rep[c];
+// One workgroup per replica that needs a new list: counting sort of its atoms
+// into cells (histogram, scan and scatter in LDS), each cell then ordered by
+// atom index in LDS, and a cell-ordered copy (x, y, z wrapped into [0, L) as
+// FP32, atom index in w).  tpos is scratch for the unsorted copy.
+__global__ void __launch_bounds__(kSortBlock)
+k_rep_sort(RepView v, const double* __restrict__ rx, const double* __restrict__ ry, const double* __restrict__ rz,
+           int32_t* __restrict__ cell_of, int32_t* __restrict__ cstart, int32_t* __restrict__ sidx,
+           float4* __restrict__ tpos, float4* __restrict__ spos) {
+    const int32_t r = blockIdx.x;
     if (!v.rebuild[r]) return;
-    const int32_t b = v.off[r], e = b + v.nat[r];
-    int32_t k = 0;
-    for (int32_t i = b; i < e; i++) {
-        if (cell_of[i] == c) {
-            if (k < kCellCap) members[(int64_t)c * kCellCap + k] = i;
-            k++;
+    __shared__ int32_t cnt[kMaxCells + 1];
+    __shared__ int32_t beg[kMaxCells + 1];
+    __shared__ int32_t part[kSortBlock];
+    __shared__ int32_t lidx[kMaxReplica];
+    const int32_t tid = threadIdx.x;
+    const int32_t n = v.nat[r], base = v.off[r], nc = v.nc[r], cb = v.coff[r];
+    const int32_t ncl = nc ? nc * nc * nc : 1;
+    const double L = v.L[r];
+    for (int32_t c = tid; c < ncl; c += kSortBlock) cnt[c] = 0;
+    __syncthreads();
+    for (int32_t i = tid; i < n; i += kSortBlock) {
+        const double x = wrap(rx[base + i], L), y = wrap(ry[base + i], L), z = wrap(rz[base + i], L);
+        const int32_t c = nc ? (cell_index(x, L, nc) * nc + cell_index(y, L, nc)) * nc + cell_index(z, L, nc) : 0;
+        cell_of[base + i] = c;
+        tpos[base + i] = make_float4((float)x, (float)y, (float)z, __int_as_float(i));
+        atomicAdd(&cnt[c], 1);
+    }
+    __syncthreads();
+    // exclusive scan over the cells: contiguous chunks per thread, then the
+    // chunk sums by one thread
+    const int32_t chunk = (ncl + kSortBlock - 1) / kSortBlock;
+    const int32_t lo = tid * chunk < ncl ? tid * chunk : ncl, hi = lo + chunk < ncl ? lo + chunk : ncl;
+    int32_t sum = 0;
+    for (int32_t c = lo; c < hi; c++) sum += cnt[c];
+    part[tid] = sum;
+    __syncthreads();
+    if (tid == 0) {
+        int32_t run = 0;
+        for (int32_t t = 0; t < kSortBlock; t++) { const int32_t x = part[t]; part[t] = run; run += x; }
+    }
+    __syncthreads();
+    int32_t run = part[tid];
+    for (int32_t c = lo; c < hi; c++) {
+        const int32_t x = cnt[c];
+        beg[c] = run;
+        cnt[c] = run;   // becomes the fill cursor
+        cstart[cb + c] = run;
+        run += x;
+    }
+    if (tid == 0) { beg[ncl] = n; cstart[cb + ncl] = n; }
+    __syncthreads();
+    if (nc == 0) {
+        // one cell: ascending index order is the identity
+        for (int32_t i = tid; i < n; i += kSortBlock) lidx[i] = i;
+    } else {
+        for (int32_t i = tid; i < n; i += kSortBlock) lidx[atomicAdd(&cnt[cell_of[base + i]], 1)] = i;
+    }
+    __syncthreads();
+    for (int32_t c = tid; c < ncl && nc; c += kSortBlock) {   // insertion sort per cell (~20 atoms) in LDS
+        const int32_t b = beg[c], e = beg[c + 1];
+        for (int32_t p = b + 1; p < e; p++) {
+            const int32_t x = lidx[p];
+            int32_t q = p - 1;
+            while (q >= b && lidx[q] > x) { lidx[q + 1] = lidx[q]; q--; }
+            lidx[q + 1] = x;
         }
     }
-    mcount[c] = k < kCellCap ? k : kCellCap;
-    if (k > kCellCap) atomicMax(&err[0], k);
+    __syncthreads();
+    for (int32_t p = tid; p < n; p += kSortBlock) {
+        const int32_t i = lidx[p];
+        sidx[base + p] = i;
+        spos[base + p] = tpos[base + i];
+    }
 }
 
+// Compact list of each atom of a flagged replica, a quad of 4 lanes per atom:
+// candidates from the 27 cells around it (or the whole replica), the lanes
+// testing interleaved candidates and numbering their hits lane-major, so the
+// list order is deterministic.  The radius test runs on the FP32 copy with rs
+// widened by 1e-4, so the list holds every pair within rs.
 __global__ void __launch_bounds__(kBlock)
-k_rep_list(RepView v, double* const* __restrict__ pos, double* __restrict__ r0x, double* __restrict__ r0y,
-           double* __restrict__ r0z, int32_t ntot, const int32_t* __restrict__ members,
-           const int32_t* __restrict__ mcount, double rs2, int32_t cap, int32_t* __restrict__ list,
-           int32_t* __restrict__ count, int32_t* __restrict__ err) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ntot) return;
+k_rep_list(RepView v, const double* __restrict__ rx, const double* __restrict__ ry, const double* __restrict__ rz,
+           double* __restrict__ r0x, double* __restrict__ r0y, double* __restrict__ r0z, int32_t ntot,
+           const int32_t* __restrict__ cell_of, const int32_t* __restrict__ cstart, const float4* __restrict__ spos,
+           float rs2, int32_t cap, int32_t* __restrict__ list, int32_t* __restrict__ count, int32_t* __restrict__ err) {
+    const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t i = t >> 2, q = t & 3;
+    if (i >= ntot) return;   // whole quads
     const int32_t r = v.rep_of[i];
     if (r < 0 || !v.rebuild[r]) return;
-    const double* rx = pos[0];
-    const double* ry = pos[1];
-    const double* rz = pos[2];
-    const double L = v.L[r], invL = 1.0 / L;
-    const double xi = rx[i], yi = ry[i], zi = rz[i];
-    r0x[i] = xi;
-    r0y[i] = yi;
-    r0z[i] = zi;
-    int32_t k = 0;
-    auto test = [&](int32_t j) {
-        const double dx = mi(xi - rx[j], L, invL), dy = mi(yi - ry[j], L, invL), dz = mi(zi - rz[j], L, invL);
-        if (dx * dx + dy * dy + dz * dz < rs2) {
-            if (k < cap) list[(int64_t)k * ntot + i] = j;
-            k++;
+    const int32_t base = v.off[r], nc = v.nc[r], cb = v.coff[r];
+    const double L = v.L[r];
+    const double x = rx[i], y = ry[i], z = rz[i];
+    if (q == 0) {
+        r0x[i] = x;
+        r0y[i] = y;
+        r0z[i] = z;
+    }
+    const float Lf = (float)L, iLf = 1.0f / Lf;
+    const float xi = (float)wrap(x, L), yi = (float)wrap(y, L), zi = (float)wrap(z, L);
+    const int32_t li = i - base;
+    int32_t k = 0;   // hits so far, the same in the 4 lanes
+    auto scan = [&](int32_t c) {
+        const int32_t b = cstart[cb + c], e = cstart[cb + c + 1];
+        for (int32_t p0 = b; p0 < e; p0 += 16) {
+            uint32_t mask = 0;
+            int32_t js[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int32_t p = p0 + 4 * u + q;
+                const float4 w = spos[base + (p < e ? p : b)];
+                js[u] = __float_as_int(w.w);
+                float dx = xi - w.x, dy = yi - w.y, dz = zi - w.z;
+                dx -= Lf * rintf(dx * iLf);
+                dy -= Lf * rintf(dy * iLf);
+                dz -= Lf * rintf(dz * iLf);
+                const bool hit = (p < e) & (js[u] != li) & (dx * dx + dy * dy + dz * dz < rs2);
+                mask |= (uint32_t)hit << u;
+            }
+            int32_t total;
+            int32_t kk = k + quad_excl(__popc(mask), q, total);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if ((mask >> u) & 1u) {
+                    if (kk < cap) list[(int64_t)kk * ntot + i] = base + js[u];
+                    kk++;
+                }
+            }
+            k += total;
         }
     };
-    const int32_t nc = v.nc[r];
     if (nc == 0) {
-        const int32_t b = v.off[r], e = b + v.nat[r];
-        for (int32_t j = b; j < e; j++)
-            if (j != i) test(j);
+        scan(0);
     } else {
-        const int32_t cx = wrap_cell(xi, L, nc), cy = wrap_cell(yi, L, nc), cz = wrap_cell(zi, L, nc);
+        const int32_t c = cell_of[i];
+        const int32_t cz = c % nc, cy = (c / nc) % nc, cx = c / (nc * nc);
         for (int dx = -1; dx <= 1; dx++)
             for (int dy = -1; dy <= 1; dy++)
-                for (int dz = -1; dz <= 1; dz++) {
-                    const int32_t x = (cx + dx + nc) % nc, y = (cy + dy + nc) % nc, z = (cz + dz + nc) % nc;
-                    const int32_t c = v.coff[r] + (x * nc + y) * nc + z;
-                    const int32_t m = mcount[c];
-                    for (int32_t q = 0; q < m; q++) {
-                        const int32_t j = members[(int64_t)c * kCellCap + q];
-                        if (j != i) test(j);
-                    }
-                }
+                for (int dz = -1; dz <= 1; dz++)
+                    scan((((cx + dx + nc) % nc) * nc + (cy + dy + nc) % nc) * nc + (cz + dz + nc) % nc);
     }
-    count[i] = k < cap ? k : cap;
-    if (k > cap) atomicMax(&err[1], k);
+    if (q == 0) {
+        count[i] = k < cap ? k : cap;
+        if (k > cap) atomicMax(&err[1], k);
+    }
 }
 
 struct RepStep {
@@ -148,6 +219,8 @@ struct RepStep {
     double* c3[3];
     double* f[3];
     const double* r0[3];
+    const double4* rq;    // r(t) again, one 32-byte record per atom (the neighbour gathers)
+    double4* rqn;         // r(t+1) as records
     const int32_t* list;
     const int32_t* count;
     int32_t ntot;
@@ -159,29 +232,41 @@ struct RepStep {
 };
 
 // kMode: 0 = corrector + next predictor, 1 = corrector only (last step),
-// 2 = forces and a = F/m only (start of a run)
+// 2 = forces and a = F/m only (start of a run).  A quad of 4 lanes per atom:
+// lane q sums list entries q, q + 4, ...; the quad's sums are added by DPP and
+// lane q < 3 integrates component q (the state arrays hold the 3 components of
+// a vector in one block of 3 x ntot).
 template <bool kFp64, int kMode>
 __global__ void __launch_bounds__(kBlock)
 k_rep_step(RepStep a) {
     constexpr bool kFinal = kMode != 0;
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t i = t >> 2, q = t & 3;
     const int32_t r = i < a.ntot ? a.v.rep_of[i] : -1;
     double d2 = 0.0;
     if (r >= 0) {
         const double L = a.v.L[r], invL = 1.0 / L;
-        const double xi = a.r[0][i], yi = a.r[1][i], zi = a.r[2][i];
+        const double4 pi = a.rq[i];
+        const double xi = pi.x, yi = pi.y, zi = pi.z;
         double fx = 0, fy = 0, fz = 0;
         const int32_t cnt = a.count[i];
-        for (int32_t k = 0; k < cnt; k++) {
-            const int32_t j = a.list[(int64_t)k * a.ntot + i];
-            const double dx = mi(xi - a.r[0][j], L, invL), dy = mi(yi - a.r[1][j], L, invL),
-                         dz = mi(zi - a.r[2][j], L, invL);
+        const int32_t* lp = a.list + i;
+        int32_t jn = q < cnt ? lp[(int64_t)q * a.ntot] : 0;
+        for (int32_t k = q; k < cnt; k += 4) {
+            const int32_t j = jn;
+            // the next entry is loaded one iteration ahead; the empty asm keeps
+            // the compiler from moving the load to the top of the next iteration
+            uint64_t pa = reinterpret_cast<uint64_t>(lp + (int64_t)(k + 4 < cnt ? k + 4 : k) * a.ntot);
+            asm volatile("" : "+v"(pa));
+            jn = *reinterpret_cast<const int32_t*>(pa);
+            const double4 pj = a.rq[j];
+            const double dx = mi(xi - pj.x, L, invL), dy = mi(yi - pj.y, L, invL), dz = mi(zi - pj.z, L, invL);
             if constexpr (kFp64) {
                 const double r2 = dx * dx + dy * dy + dz * dz;
                 if (r2 < a.rc2) {
-                    const double q = r2 / a.sig2;
-                    const double iq4 = 1.0 / (q * q * q * q);
-                    const double iq7 = iq4 / (q * q * q);
+                    const double qq = r2 / a.sig2;
+                    const double iq4 = 1.0 / (qq * qq * qq * qq);
+                    const double iq7 = iq4 / (qq * qq * qq);
                     const double fr = (2 * iq7 - iq4) * a.coeff;
                     fx += dx * fr; fy += dy * fr; fz += dz * fr;
                 }
@@ -196,45 +281,48 @@ k_rep_step(RepStep a) {
                 fz = __builtin_fma(fr, dz, fz);
             }
         }
-        const double F[3] = {fx, fy, fz}, R0[3] = {xi, yi, zi};
-        if constexpr (kMode == 2) {
-            for (int d = 0; d < 3; d++) {
-                a.acc[d][i] = F[d] / a.mass * a.scale;
-                a.f[d][i] = F[d];
+        fx = quad_sum(fx);
+        fy = quad_sum(fy);
+        fz = quad_sum(fz);
+        if (q < 3) {
+            const double F = q == 0 ? fx : (q == 1 ? fy : fz);
+            const int64_t o = (int64_t)q * a.ntot + i;   // component q of atom i
+            if constexpr (kMode == 2) {
+                a.acc[0][o] = F / a.mass * a.scale;
+                a.f[0][o] = F;
+            } else {
+                double rr = a.r[0][o], v = a.vel[0][o], ac = a.acc[0][o], b = a.b3[0][o], c = a.c3[0][o];
+                // corrector (cpu/common/Device.cpp:58-84)
+                const double anew = F / a.mass * a.scale;
+                const double corr = anew - ac;
+                rr = rr + a.gc.kc[0] * corr;
+                v = v + a.gc.kc[1] * corr;
+                ac = anew;
+                b = b + a.gc.kc[2] * corr;
+                c = c + a.gc.kc[3] * corr;
+                if constexpr (!kFinal) {
+                    // predictor of the next step (cpu/common/Device.cpp:24-38)
+                    const double rp = rr + a.gc.k[0] * v + a.gc.k[1] * ac + a.gc.k[2] * b + a.gc.k[3] * c;
+                    v = v + a.gc.k[0] * ac + a.gc.k[1] * b + a.gc.k[2] * c;
+                    ac = ac + a.gc.k[0] * b + a.gc.k[1] * c;
+                    b = b + a.gc.k[0] * c;
+                    const double dd = rp - a.r0[0][o];
+                    d2 = dd * dd;
+                    rr = rp;
+                }
+                a.rn[0][o] = rr;
+                reinterpret_cast<double*>(a.rqn + i)[q] = rr;
+                a.vel[0][o] = v;
+                a.acc[0][o] = ac;
+                a.b3[0][o] = b;
+                a.c3[0][o] = c;
+                a.f[0][o] = F;
             }
-            return;
         }
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            double rr = R0[d], v = a.vel[d][i], ac = a.acc[d][i], b = a.b3[d][i], c = a.c3[d][i];
-            // corrector (cpu/common/Device.cpp:58-84)
-            const double anew = F[d] / a.mass * a.scale;
-            const double corr = anew - ac;
-            rr = rr + a.gc.kc[0] * corr;
-            v = v + a.gc.kc[1] * corr;
-            ac = anew;
-            b = b + a.gc.kc[2] * corr;
-            c = c + a.gc.kc[3] * corr;
-            if constexpr (!kFinal) {
-                // predictor of the next step (cpu/common/Device.cpp:24-38)
-                const double rp = rr + a.gc.k[0] * v + a.gc.k[1] * ac + a.gc.k[2] * b + a.gc.k[3] * c;
-                v = v + a.gc.k[0] * ac + a.gc.k[1] * b + a.gc.k[2] * c;
-                ac = ac + a.gc.k[0] * b + a.gc.k[1] * c;
-                b = b + a.gc.k[0] * c;
-                const double dd = rp - a.r0[d][i];
-                d2 += dd * dd;
-                rr = rp;
-            }
-            a.rn[d][i] = rr;
-            a.vel[d][i] = v;
-            a.acc[d][i] = ac;
-            a.b3[d][i] = b;
-            a.c3[d][i] = c;
-            a.f[d][i] = F[d];
-        }
+        if constexpr (!kFinal) d2 = quad_sum(d2);
     }
     if constexpr (!kFinal) {
-        // a wave never spans two replicas (offsets are multiples of 64)
+        // a wave (16 atoms) never spans two replicas (offsets are multiples of 64)
         for (int o = kWave / 2; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o, kWave));
         const int32_t r0 = __shfl(r, 0, kWave);
         if ((threadIdx.x & (kWave - 1)) == 0 && r0 >= 0)
@@ -256,6 +344,7 @@ k_rep_predict(RepStep a) {
             const double x = rp[i], v = a.vel[d][i], ac = a.acc[d][i], b = a.b3[d][i], c = a.c3[d][i];
             const double xn = x + a.gc.k[0] * v + a.gc.k[1] * ac + a.gc.k[2] * b + a.gc.k[3] * c;
             rp[i] = xn;
+            reinterpret_cast<double*>(const_cast<double4*>(a.rq) + i)[d] = xn;
             a.vel[d][i] = v + a.gc.k[0] * ac + a.gc.k[1] * b + a.gc.k[2] * c;
             a.acc[d][i] = ac + a.gc.k[0] * b + a.gc.k[1] * c;
             a.b3[d][i] = b + a.gc.k[0] * c;
@@ -279,6 +368,13 @@ __global__ void k_rep_decide(int32_t R, unsigned long long* __restrict__ dmax, i
     rebuild[r] = f;
     builds[r] += f;
     dmax[r] = 0ull;
+}
+
+// the position records of one buffer from its three component arrays
+__global__ void k_rep_records(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+                              double4* __restrict__ rq, int32_t ntot) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ntot) rq[i] = make_double4(x[i], y[i], z[i], 0.0);
 }
 
 __global__ void k_rep_set(int32_t R, int32_t* __restrict__ rebuild, int32_t v) {
@@ -322,14 +418,16 @@ struct mdx_batch {
     std::vector<void*> allocs;
     int32_t *d_rep_of = nullptr, *d_off = nullptr, *d_nat = nullptr, *d_coff = nullptr, *d_nc = nullptr;
     double* d_L = nullptr;
-    int32_t *d_rebuild = nullptr, *d_cell_rep = nullptr, *d_cell_of = nullptr, *d_members = nullptr, *d_mcount = nullptr;
+    int32_t *d_rebuild = nullptr, *d_cell_of = nullptr, *d_cstart = nullptr, *d_sidx = nullptr;
+    float4* d_spos = nullptr;
+    float4* d_tpos = nullptr;
+    double4* d_rq[2] = {nullptr, nullptr};   // position records of buffers A (r) and B (rn)
     int32_t *d_list = nullptr, *d_count = nullptr, *d_err = nullptr;
     int32_t cap = 0;
     long long* d_builds = nullptr;
     unsigned long long* d_dmax = nullptr;
     double* r[3] = {}; double* rn[3] = {}; double* v[3] = {}; double* a[3] = {}; double* b[3] = {}; double* c[3] = {};
     double* f[3] = {}; double* r0[3] = {};
-    double** d_pos = nullptr;      // 2 x 3 device pointers: the position buffers
     double* d_energy = nullptr;
     GearCoeffs gc{};
     bool initialized = false, flip = false;
@@ -384,6 +482,8 @@ RepStep step_args(const mdx_batch* b, bool flip) {
         s.vel[d] = b->v[d]; s.acc[d] = b->a[d]; s.b3[d] = b->b[d]; s.c3[d] = b->c[d]; s.f[d] = b->f[d];
         s.r0[d] = b->r0[d];
     }
+    s.rq = flip ? b->d_rq[1] : b->d_rq[0];
+    s.rqn = flip ? b->d_rq[0] : b->d_rq[1];
     s.list = b->d_list;
     s.count = b->d_count;
     s.ntot = b->ntot;
@@ -420,16 +520,14 @@ template <class F> hipError_t btimed(mdx_batch* b, const char* name, F&& launch)
 int enqueue_rebuild(mdx_batch* b, bool flip) {
     hipStream_t s = b->stream;
     const RepView v = view(b);
-    double* const* pos = b->d_pos + (flip ? 3 : 0);
     double* const* hp = flip ? b->rn : b->r;
-    const double rs = b->cfg.lj_rs;
-    BLAUNCH(b, "rep_bin", (k_rep_bin<<<grid_for(b->ntot), kBlock, 0, s>>>(v, hp[0], hp[1], hp[2], b->ntot, b->d_cell_of)));
-    BLAUNCH(b, "rep_cells", (k_rep_cells<<<grid_for(b->ncell_tot), kBlock, 0, s>>>(v, b->d_cell_rep, b->ncell_tot,
-                                                                                   b->d_cell_of, b->d_members,
-                                                                                   b->d_mcount, b->d_err)));
-    BLAUNCH(b, "rep_list", (k_rep_list<<<grid_for(b->ntot), kBlock, 0, s>>>(v, pos, b->r0[0], b->r0[1], b->r0[2], b->ntot,
-                                                                            b->d_members, b->d_mcount, rs * rs, b->cap,
-                                                                            b->d_list, b->d_count, b->d_err)));
+    const float rs = (float)b->cfg.lj_rs + 1e-4f;
+    BLAUNCH(b, "rep_sort", (k_rep_sort<<<b->R, kSortBlock, 0, s>>>(v, hp[0], hp[1], hp[2], b->d_cell_of, b->d_cstart,
+                                                                     b->d_sidx, b->d_tpos, b->d_spos)));
+    BLAUNCH(b, "rep_list", (k_rep_list<<<grid_for(4 * (int64_t)b->ntot), kBlock, 0, s>>>(v, hp[0], hp[1], hp[2], b->r0[0], b->r0[1],
+                                                                            b->r0[2], b->ntot, b->d_cell_of, b->d_cstart,
+                                                                            b->d_spos, rs * rs, b->cap, b->d_list,
+                                                                            b->d_count, b->d_err)));
     return MDX_OK;
 }
 
@@ -446,7 +544,7 @@ int enqueue_step(mdx_batch* b, bool flip, bool fin) {
     if (rc) return rc;
     const RepStep a = step_args(b, flip);
     const bool fp64 = b->cfg.precision == MDX_PREC_FP64;
-    const int g = grid_for(b->ntot);
+    const int g = grid_for(4 * (int64_t)b->ntot);
     if (fin) {
         if (fp64) BLAUNCH(b, "rep_step", (k_rep_step<true, 1><<<g, kBlock, 0, s>>>(a)));
         else BLAUNCH(b, "rep_step", (k_rep_step<false, 1><<<g, kBlock, 0, s>>>(a)));
@@ -461,7 +559,6 @@ int check_err(mdx_batch* b) {
     int32_t e[2];
     BCHK(b, hipMemcpyAsync(e, b->d_err, sizeof e, hipMemcpyDeviceToHost, b->stream));
     BCHK(b, hipStreamSynchronize(b->stream));
-    if (e[0]) return bfail(b, MDX_ERR_CAPACITY, "a replica cell holds more than 64 atoms");
     if (e[1]) return bfail(b, MDX_ERR_CAPACITY, "a neighbour list exceeded max_neighbors (longest " + std::to_string(e[1]) + ")");
     return MDX_OK;
 }
@@ -484,17 +581,19 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
     int64_t off = 0, coff = 0;
     for (int32_t r = 0; r < n_replicas; r++) {
         const double L = box_length[r];
-        if (n_atoms[r] <= 0 || !(L >= 2.0 * c.lj_rs)) {
+        if (n_atoms[r] <= 0 || n_atoms[r] > kMaxReplica || !(L >= 2.0 * c.lj_rs)) {
             delete b;
-            return bfail(nullptr, MDX_ERR_ARG, "replica " + std::to_string(r) + ": need n > 0 and box >= 2 rs");
+            return bfail(nullptr, MDX_ERR_ARG, "replica " + std::to_string(r) + ": need 0 < n <= " +
+                                                   std::to_string(kMaxReplica) + " and box >= 2 rs");
         }
         b->off.push_back((int32_t)off);
         b->nat.push_back(n_atoms[r]);
         b->L.push_back(L);
-        const int32_t nc = (int32_t)std::floor(L / (double)c.lj_rs);
-        b->nc.push_back(nc >= 3 ? nc : 0);
+        int32_t nc = (int32_t)std::floor(L / (double)c.lj_rs);
+        nc = nc < 3 ? 0 : (nc > kMaxNc ? kMaxNc : nc);
+        b->nc.push_back(nc);
         b->coff.push_back((int32_t)coff);
-        coff += nc >= 3 ? (int64_t)nc * nc * nc : 0;
+        coff += (nc ? (int64_t)nc * nc * nc : 1) + 1;
         off += ((int64_t)n_atoms[r] + kWave - 1) / kWave * kWave;
     }
     if (off > INT32_MAX / 2 || coff > (int64_t(1) << 27)) {
@@ -512,23 +611,21 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
     for (int32_t r = 0; r < n_replicas; r++) rho_max = std::max(rho_max, n_atoms[r] / std::pow(b->L[r], 3));
     b->cap = c.max_neighbors > 0 ? c.max_neighbors
                                  : (int32_t)(1.5 * 4.0 / 3.0 * M_PI * std::pow((double)c.lj_rs, 3) * rho_max) + 32;
-    std::vector<int32_t> rep_of(b->ntot, -1), cell_rep(std::max<int32_t>(b->ncell_tot, 1), 0);
-    for (int32_t r = 0; r < n_replicas; r++) {
+    std::vector<int32_t> rep_of(b->ntot, -1);
+    for (int32_t r = 0; r < n_replicas; r++)
         for (int32_t i = 0; i < n_atoms[r]; i++) rep_of[b->off[r] + i] = r;
-        for (int32_t q = b->coff[r]; q < b->coff[r + 1]; q++) cell_rep[q] = r;
-    }
     const int32_t n = b->ntot;
     int rc;
     if ((rc = balloc(b, &b->d_rep_of, n)) || (rc = balloc(b, &b->d_off, n_replicas + 1)) ||
         (rc = balloc(b, &b->d_nat, n_replicas)) || (rc = balloc(b, &b->d_coff, n_replicas + 1)) ||
         (rc = balloc(b, &b->d_nc, n_replicas)) || (rc = balloc(b, &b->d_L, n_replicas)) ||
         (rc = balloc(b, &b->d_rebuild, n_replicas)) || (rc = balloc(b, &b->d_builds, n_replicas)) ||
-        (rc = balloc(b, &b->d_dmax, n_replicas)) || (rc = balloc(b, &b->d_cell_rep, std::max<int32_t>(b->ncell_tot, 1))) ||
-        (rc = balloc(b, &b->d_cell_of, n)) ||
-        (rc = balloc(b, &b->d_members, (size_t)std::max<int32_t>(b->ncell_tot, 1) * kCellCap)) ||
-        (rc = balloc(b, &b->d_mcount, std::max<int32_t>(b->ncell_tot, 1))) ||
+        (rc = balloc(b, &b->d_dmax, n_replicas)) || (rc = balloc(b, &b->d_cell_of, n)) ||
+        (rc = balloc(b, &b->d_cstart, b->ncell_tot)) || (rc = balloc(b, &b->d_sidx, n)) ||
+        (rc = balloc(b, &b->d_spos, n)) || (rc = balloc(b, &b->d_tpos, n)) ||
+        (rc = balloc(b, &b->d_rq[0], n)) || (rc = balloc(b, &b->d_rq[1], n)) ||
         (rc = balloc(b, &b->d_list, (size_t)b->cap * n)) || (rc = balloc(b, &b->d_count, n)) ||
-        (rc = balloc(b, &b->d_err, 2)) || (rc = balloc(b, &b->d_pos, 6)) ||
+        (rc = balloc(b, &b->d_err, 2)) ||
         (rc = balloc(b, &b->d_energy, 2 * (size_t)n_replicas)) ||
         (rc = balloc3(b, b->r, n)) || (rc = balloc3(b, b->rn, n)) || (rc = balloc3(b, b->v, n)) ||
         (rc = balloc3(b, b->a, n)) || (rc = balloc3(b, b->b, n)) || (rc = balloc3(b, b->c, n)) ||
@@ -536,7 +633,6 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
         g_batch_err = b->err;
         return bail(rc);
     }
-    double* hp[6] = {b->r[0], b->r[1], b->r[2], b->rn[0], b->rn[1], b->rn[2]};
     hipStream_t s = b->stream;
     bool ok = hipMemcpyAsync(b->d_rep_of, rep_of.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s) == hipSuccess &&
               hipMemcpyAsync(b->d_off, b->off.data(), sizeof(int32_t) * (n_replicas + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
@@ -544,8 +640,6 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
               hipMemcpyAsync(b->d_coff, b->coff.data(), sizeof(int32_t) * (n_replicas + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
               hipMemcpyAsync(b->d_nc, b->nc.data(), sizeof(int32_t) * n_replicas, hipMemcpyHostToDevice, s) == hipSuccess &&
               hipMemcpyAsync(b->d_L, b->L.data(), sizeof(double) * n_replicas, hipMemcpyHostToDevice, s) == hipSuccess &&
-              hipMemcpyAsync(b->d_cell_rep, cell_rep.data(), sizeof(int32_t) * cell_rep.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
-              hipMemcpyAsync(b->d_pos, hp, sizeof hp, hipMemcpyHostToDevice, s) == hipSuccess &&
               hipMemsetAsync(b->d_builds, 0, sizeof(long long) * n_replicas, s) == hipSuccess &&
               hipMemsetAsync(b->d_dmax, 0, sizeof(unsigned long long) * n_replicas, s) == hipSuccess &&
               hipMemsetAsync(b->d_err, 0, 2 * sizeof(int32_t), s) == hipSuccess &&
@@ -627,11 +721,14 @@ int mdx_batch_run(mdx_batch* b, int32_t nsteps) {
         BLAUNCH(b, "rep_set", (k_rep_set<<<(b->R + 255) / 256, 256, 0, s>>>(b->R, b->d_rebuild, 1)));
         if ((rc = enqueue_rebuild(b, b->flip))) return rc;
         if ((rc = check_err(b))) return rc;
+        double* const* cur = b->flip ? b->rn : b->r;
+        BLAUNCH(b, "rep_records", (k_rep_records<<<grid_for(b->ntot), kBlock, 0, s>>>(cur[0], cur[1], cur[2],
+                                                                                       b->d_rq[b->flip ? 1 : 0], b->ntot)));
         const RepStep a = step_args(b, b->flip);
         if (b->cfg.precision == MDX_PREC_FP64)
-            BLAUNCH(b, "rep_force0", (k_rep_step<true, 2><<<grid_for(b->ntot), kBlock, 0, s>>>(a)));
+            BLAUNCH(b, "rep_force0", (k_rep_step<true, 2><<<grid_for(4 * (int64_t)b->ntot), kBlock, 0, s>>>(a)));
         else
-            BLAUNCH(b, "rep_force0", (k_rep_step<false, 2><<<grid_for(b->ntot), kBlock, 0, s>>>(a)));
+            BLAUNCH(b, "rep_force0", (k_rep_step<false, 2><<<grid_for(4 * (int64_t)b->ntot), kBlock, 0, s>>>(a)));
         b->initialized = true;
     }
     if (nsteps == 0) { BCHK(b, hipStreamSynchronize(s)); return MDX_OK; }
