@@ -72,6 +72,14 @@ extern "C" {
 /* bond-order functional form */
 #define MDX_BOND_REFERENCE 0  /* the reference's N/O potential, cpu/common/BondOrderFuncs.h */
 #define MDX_BOND_TERSOFF 1    /* Tersoff silicon, PRB 38, 9902 (1988), R = 2.7 A, D = 0.3 A */
+#define MDX_BOND_SURROGATE 2  /* learned surrogate of MDX_BOND_TERSOFF, network set by mdx_set_surrogate */
+
+/* surrogate network: a flat float blob, in order W1[64][48], b1[64], W2[64][64],
+ * b2[64], w3[64], b3, then per descriptor component mean[48], istd[48] and the
+ * accepted range lo[48], hi[48] of the standardized value (G - mean) * istd */
+#define MDX_SG_NFEAT 48
+#define MDX_SG_NTRAIN 7361   /* trainable part: W1 .. b3 */
+#define MDX_SG_NPARAM 7553   /* whole blob */
 
 /* list selector for mdx_build_list / mdx_copy_neighbor_list */
 #define MDX_LIST_LJ 0
@@ -188,6 +196,47 @@ int mdx_kernel_times(mdx_system* sys, char* names, int64_t* counts, double* tota
                      int32_t cap, int32_t* n);
 /* blocks until all work on the system's stream is done */
 int mdx_synchronize(mdx_system* sys);
+
+/*
+ * Learned surrogate of Tersoff silicon (bond_form MDX_BOND_SURROGATE, periodic
+ * box; same list settings as MDX_BOND_TERSOFF).  No counterpart in the
+ * reference: it takes the place of the bond-order force (manyBodyForces,
+ * cpu/common/BondOrderFuncs.h:283-397) as E = sum_i NN(G_i), with the
+ * descriptor G_i of 32 radial and 16 bond-resolved angular terms inside 3.0 A
+ * and a 48-64-64-1 tanh network run on BF16 matrix cores.  Atoms with more
+ * than 6 list entries, or a standardized descriptor outside [lo, hi], take
+ * the Tersoff per-atom energy 1/2 sum_j V_ij instead (fallback).
+ */
+int mdx_set_surrogate(mdx_system* sys, const float* params, int64_t len);   /* len = MDX_SG_NPARAM */
+/* {atom evaluations that took the Tersoff fallback, all atom evaluations} since creation */
+int mdx_surrogate_counts(mdx_system* sys, int64_t* out2);
+
+/*
+ * Training of the surrogate on the GPU: n_frames periodic frames of n_atoms
+ * atoms in one cubic box (pos3, force3: 3 doubles per atom, frame after
+ * frame; energy: one per frame, eV).  Loss = mean over force components of
+ * (F - F_ref)^2 + w_energy * mean over frames of ((E - E_ref) / n_atoms)^2.
+ * Every atom may have at most 8 neighbours inside 3.0 A.
+ */
+typedef struct mdx_sg_trainer mdx_sg_trainer;
+int mdx_sg_trainer_create(int32_t n_frames, int32_t n_atoms, double box_length, const double* pos3,
+                          const double* force3, const double* energy, mdx_sg_trainer** out);
+int mdx_sg_trainer_destroy(mdx_sg_trainer* t);
+const char* mdx_sg_trainer_last_error(const mdx_sg_trainer* t);   /* t may be NULL */
+/* per descriptor component over all atoms: mean[48], std[48], min[48], max[48] */
+int mdx_sg_trainer_feature_stats(mdx_sg_trainer* t, double* out4x48);
+/* descriptors of one frame, [atom][48] */
+int mdx_sg_trainer_descriptors(mdx_sg_trainer* t, int32_t frame, float* out);
+/* the network being trained (MDX_SG_NPARAM floats); set clears the Adam state */
+int mdx_sg_trainer_set_params(mdx_sg_trainer* t, const float* params, int64_t len);
+int mdx_sg_trainer_get_params(mdx_sg_trainer* t, float* params, int64_t len);
+/* loss of the minibatch frames[0..n) into out3 = {loss, force mse, energy mse per atom}
+ * (may be NULL) and its gradient in the MDX_SG_NTRAIN trainable weights (may be NULL) */
+int mdx_sg_trainer_gradient(mdx_sg_trainer* t, const int32_t* frames, int32_t n, double w_energy, float* grad,
+                            double* out3);
+/* one Adam step (beta 0.9, 0.999, eps 1e-8) on the minibatch; out3 as above, before the step */
+int mdx_sg_trainer_step(mdx_sg_trainer* t, const int32_t* frames, int32_t n, double w_energy, double lr,
+                        double* out3);
 
 /*
  * Batched replicas.  The reference runs independent systems one after

@@ -24,7 +24,8 @@ ERRORS = {
 BOX_WALLED, BOX_PERIODIC = 0, 1
 PREC_MIXED, PREC_FP64 = 0, 1
 LJ_REFERENCE, LJ_UNIFORM = 0, 1
-BOND_REFERENCE, BOND_TERSOFF = 0, 1
+BOND_REFERENCE, BOND_TERSOFF, BOND_SURROGATE = 0, 1, 2
+SG_NFEAT, SG_NTRAIN, SG_NPARAM = 48, 7361, 7553
 LIST_LJ, LIST_BOND = 0, 1
 
 
@@ -82,6 +83,11 @@ def load() -> C.CDLL:
         "mdx_batch_run": [P, I32], "mdx_batch_build_counts": [P, P], "mdx_batch_energies": [P, P],
         "mdx_batch_set_timing": [P, I32], "mdx_batch_kernel_times": [P, P, P, P, I32, C.POINTER(I32)],
         "mdx_batch_synchronize": [P],
+        "mdx_set_surrogate": [P, P, I64], "mdx_surrogate_counts": [P, P],
+        "mdx_sg_trainer_create": [I32, I32, D, P, P, P, C.POINTER(P)], "mdx_sg_trainer_destroy": [P],
+        "mdx_sg_trainer_feature_stats": [P, P], "mdx_sg_trainer_descriptors": [P, I32, P],
+        "mdx_sg_trainer_set_params": [P, P, I64], "mdx_sg_trainer_get_params": [P, P, I64],
+        "mdx_sg_trainer_gradient": [P, P, I32, D, P, P], "mdx_sg_trainer_step": [P, P, I32, D, D, P],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -91,6 +97,8 @@ def load() -> C.CDLL:
     lib.mdx_last_error.restype = C.c_char_p
     lib.mdx_batch_last_error.argtypes = [P]
     lib.mdx_batch_last_error.restype = C.c_char_p
+    lib.mdx_sg_trainer_last_error.argtypes = [P]
+    lib.mdx_sg_trainer_last_error.restype = C.c_char_p
     _lib = lib
     return lib
 
@@ -212,6 +220,18 @@ class System:
     def synchronize(self):
         self._chk("mdx_synchronize", self.lib.mdx_synchronize(self.h))
 
+    def set_surrogate(self, params):
+        """Upload the surrogate network (MDX_SG_NPARAM floats, layout of include/mdx.h)."""
+        b = np.ascontiguousarray(params, dtype=np.float32)
+        assert b.shape == (SG_NPARAM,)
+        self._chk("mdx_set_surrogate", self.lib.mdx_set_surrogate(self.h, _ptr(b), SG_NPARAM))
+
+    def surrogate_counts(self):
+        """(atom evaluations by the Tersoff fallback, all atom evaluations)."""
+        out = np.zeros(2, dtype=np.int64)
+        self._chk("mdx_surrogate_counts", self.lib.mdx_surrogate_counts(self.h, _ptr(out)))
+        return int(out[0]), int(out[1])
+
     def set_timing(self, on: bool):
         self._chk("mdx_set_timing", self.lib.mdx_set_timing(self.h, 1 if on else 0))
 
@@ -318,3 +338,80 @@ class Batch:
         self._chk("mdx_batch_kernel_times",
                   self.lib.mdx_batch_kernel_times(self.h, names, _ptr(counts), _ptr(tot), cap, C.byref(n)))
         return _kernel_table(names, counts, tot, n.value)
+
+
+class SgTrainer:
+    """Training set of the Tersoff surrogate on the device (mdx_sg_trainer_*):
+    frames of n_atoms atoms in one periodic box with reference forces and
+    energies; minibatch gradients and Adam steps run on the GPU."""
+
+    def __init__(self, pos, force, energy, box_length):
+        self.lib = load()
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        force = np.ascontiguousarray(force, dtype=np.float64)
+        energy = np.ascontiguousarray(energy, dtype=np.float64)
+        self.nf, self.na = pos.shape[:2]
+        assert pos.shape == (self.nf, self.na, 3) and force.shape == pos.shape and energy.shape == (self.nf,)
+        h = C.c_void_p()
+        rc = self.lib.mdx_sg_trainer_create(self.nf, self.na, float(box_length), _ptr(pos), _ptr(force),
+                                            _ptr(energy), C.byref(h))
+        if rc != MDX_OK:
+            raise MdxError("mdx_sg_trainer_create", rc, self.lib.mdx_sg_trainer_last_error(None).decode())
+        self.h = h
+
+    def _chk(self, fn: str, rc: int):
+        if rc != MDX_OK:
+            raise MdxError(fn, rc, self.lib.mdx_sg_trainer_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mdx_sg_trainer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def feature_stats(self):
+        out = np.zeros((4, SG_NFEAT))
+        self._chk("mdx_sg_trainer_feature_stats", self.lib.mdx_sg_trainer_feature_stats(self.h, _ptr(out)))
+        return {"mean": out[0], "std": out[1], "min": out[2], "max": out[3]}
+
+    def descriptors(self, frame):
+        out = np.zeros((self.na, SG_NFEAT), dtype=np.float32)
+        self._chk("mdx_sg_trainer_descriptors", self.lib.mdx_sg_trainer_descriptors(self.h, int(frame), _ptr(out)))
+        return out
+
+    def set_params(self, params):
+        b = np.ascontiguousarray(params, dtype=np.float32)
+        assert b.shape == (SG_NPARAM,)
+        self._chk("mdx_sg_trainer_set_params", self.lib.mdx_sg_trainer_set_params(self.h, _ptr(b), SG_NPARAM))
+
+    def get_params(self):
+        b = np.zeros(SG_NPARAM, dtype=np.float32)
+        self._chk("mdx_sg_trainer_get_params", self.lib.mdx_sg_trainer_get_params(self.h, _ptr(b), SG_NPARAM))
+        return b
+
+    def gradient(self, frames, w_energy=1.0, want_grad=True):
+        """(loss, force mse, energy mse per atom) and the gradient of the trainable weights."""
+        fr = np.ascontiguousarray(frames, dtype=np.int32)
+        out = np.zeros(3)
+        g = np.zeros(SG_NTRAIN, dtype=np.float32) if want_grad else None
+        self._chk("mdx_sg_trainer_gradient",
+                  self.lib.mdx_sg_trainer_gradient(self.h, _ptr(fr), len(fr), float(w_energy), _ptr(g), _ptr(out)))
+        return out, g
+
+    def step(self, frames, lr, w_energy=1.0, report=False):
+        fr = np.ascontiguousarray(frames, dtype=np.int32)
+        out = np.zeros(3) if report else None
+        self._chk("mdx_sg_trainer_step",
+                  self.lib.mdx_sg_trainer_step(self.h, _ptr(fr), len(fr), float(w_energy), float(lr), _ptr(out)))
+        return out

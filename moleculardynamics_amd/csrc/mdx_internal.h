@@ -204,6 +204,23 @@ hipError_t launch_ts_forces(hipStream_t s, const DevState& st, double L, const T
                             double* e_atom, double* const rref[3], double hard2, uint64_t* flags,
                             const DevState& out);
 
+// surrogate.hip: learned surrogate of Tersoff silicon over the same lists
+// FP32 blob (MDX_SG_NPARAM) -> BF16 MFMA fragments and FP32 vectors
+void sg_pack(const float* blob, uint16_t* frag, float* vec);
+size_t sg_frag_u16();
+size_t sg_vec_len();
+int32_t sg_slots();
+// rev[k][i]: the slot of i in the list of its k-th partner; err set if absent
+hipError_t launch_sg_rev(hipStream_t s, const int32_t* list, const int32_t* count, int32_t n, uint8_t* rev,
+                         int32_t* err);
+// forces of st's positions into out.f (and per-atom energies into e_atom if
+// non-null); nflag counts the atoms that took the Tersoff fallback; stale
+// atoms against rref are counted into the spread flags as in launch_ts_forces
+hipError_t launch_sg_forces(hipStream_t s, const DevState& st, double L, const TersoffParams& p, const int32_t* list,
+                            const int32_t* count, const uint8_t* rev, int32_t n, const uint16_t* frag,
+                            const float* vec, float* w, double* e_atom, int32_t* nflag, double* const rref[3],
+                            double hard2, uint64_t* flags, const DevState& out);
+
 // bond_ref.hip
 hipError_t launch_bond_ref(hipStream_t s, const DevState& st, const int32_t* slot_of_id,
                            const int32_t* mb_cell_of, const int32_t* mb_ids, const GridView& g,

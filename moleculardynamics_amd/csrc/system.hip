@@ -94,6 +94,14 @@ struct mdx_system {
     double* ts_dedz = nullptr;       // dE/dzeta_ij, per bond
     int32_t ts_cap = 0;
     bool ts_built = false;
+    // learned surrogate (MDX_BOND_SURROGATE) over the same compact lists
+    uint16_t* sg_frag = nullptr;     // BF16 MFMA weight fragments
+    float* sg_vec = nullptr;         // biases, output weights, feature scaling and range
+    float* sg_w = nullptr;           // dE_i/d(x_j - x_i) per list slot, [slot][3][n]
+    uint8_t* sg_rev = nullptr;       // slot of i in the list of its partner, [slot][n]
+    int32_t* sg_nflag = nullptr;     // atoms evaluated by the Tersoff fallback (cumulative)
+    bool sg_set = false;
+    int64_t sg_evals = 0;            // force evaluations
     static constexpr int kRing = 3;
     static constexpr int kFlagWords = 3 * 64;  // flag_spread() x {disp^2, step^2, stale}
     uint64_t* dflags = nullptr;      // kRing x kFlagWords
@@ -439,6 +447,7 @@ int32_t next_build(float rs, float rc, double vmax, float dt, int32_t t) {
 }
 
 bool is_tersoff(const mdx_system* s) { return s->cfg.box == MDX_BOX_PERIODIC && s->cfg.use_bond; }
+bool is_surrogate(const mdx_system* s) { return is_tersoff(s) && s->cfg.bond_form == MDX_BOND_SURROGATE; }
 
 // Tersoff compact lists over the primary grid (cells of edge mb_rs); grows the
 // capacity and builds again when a list overflows
@@ -448,8 +457,12 @@ int ts_build_list(mdx_system* s) {
         if (!s->ts_list) {
             int rc;
             const size_t e = (size_t)s->ts_cap * n;
-            if ((rc = dalloc(s, &s->ts_list, e)) || (rc = dalloc(s, &s->ts_dedr, e)) || (rc = dalloc(s, &s->ts_dedz, e)))
+            if ((rc = dalloc(s, &s->ts_list, e))) return rc;
+            if (is_surrogate(s)) {
+                if ((rc = dalloc(s, &s->sg_w, 3 * e)) || (rc = dalloc(s, &s->sg_rev, e))) return rc;
+            } else if ((rc = dalloc(s, &s->ts_dedr, e)) || (rc = dalloc(s, &s->ts_dedz, e))) {
                 return rc;
+            }
         }
         HIPCHK(s, hipMemsetAsync(s->dint + 4, 0, sizeof(int32_t), s->stream));
         LAUNCH(s, "ts_list", launch_ts_list(s->stream, s->st, s->cell_of, s->lj.view, (double)s->cfg.mb_rs, n,
@@ -461,8 +474,22 @@ int ts_build_list(mdx_system* s) {
         dfree(s, s->ts_list);
         dfree(s, s->ts_dedr);
         dfree(s, s->ts_dedz);
+        dfree(s, s->sg_w);
+        dfree(s, s->sg_rev);
         s->ts_list = nullptr;
+        s->ts_dedr = s->ts_dedz = nullptr;
+        s->sg_w = nullptr;
+        s->sg_rev = nullptr;
         s->ts_cap = longest + longest / 4 + 4;
+    }
+    if (is_surrogate(s)) {
+        if (s->ts_cap > 255) return fail(s, MDX_ERR_CAPACITY, "surrogate lists hold at most 255 neighbours");
+        HIPCHK(s, hipMemsetAsync(s->dint + 4, 0, sizeof(int32_t), s->stream));
+        LAUNCH(s, "sg_rev", launch_sg_rev(s->stream, s->ts_list, s->ts_count, n, s->sg_rev, s->dint + 4));
+        int32_t asym;
+        int rc = read_ints(s, &asym, s->dint + 4, 1);
+        if (rc) return rc;
+        if (asym) return fail(s, MDX_ERR_STATE, "asymmetric neighbour lists");
     }
     s->ts_built = true;
     return MDX_OK;
@@ -501,6 +528,16 @@ int lj_forces(mdx_system* s, double* e_atom, const int32_t* gate, const DevState
 
 int bond_forces(mdx_system* s, double* e_atom, const int32_t* gate, const DevState* target = nullptr) {
     const mdx_config& c = s->cfg;
+    if (is_surrogate(s)) {
+        if (!s->ts_built) return fail(s, MDX_ERR_STATE, "Tersoff list not built");
+        if (!s->sg_set) return fail(s, MDX_ERR_STATE, "surrogate network not set (mdx_set_surrogate)");
+        const DevState& d = target ? *target : s->st;
+        LAUNCH(s, "sg_forces", launch_sg_forces(s->stream, s->st, c.box_length, s->tp, s->ts_list, s->ts_count,
+                                                s->sg_rev, s->n, s->sg_frag, s->sg_vec, s->sg_w, e_atom,
+                                                s->sg_nflag, nullptr, 0.0, nullptr, d));
+        s->sg_evals++;
+        return MDX_OK;
+    }
     if (is_tersoff(s)) {
         if (!s->ts_built) return fail(s, MDX_ERR_STATE, "Tersoff list not built");
         const DevState& d = target ? *target : s->st;
@@ -707,9 +744,16 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
         const bool fin = k + 1 == nsteps;
         HIPCHK(s, hipMemsetAsync(slot_flags(k), 0, W * sizeof(uint64_t), st));
         if (ts) {
-            LAUNCH(s, "ts_forces", launch_ts_forces(st, s->st, c.box_length, s->tp, c.precision, s->ts_list,
-                                                    s->ts_count, n, s->ts_dedr, s->ts_dedz, nullptr, s->rref, hard2,
-                                                    slot_flags(k), s->st));
+            if (is_surrogate(s)) {
+                LAUNCH(s, "sg_forces", launch_sg_forces(st, s->st, c.box_length, s->tp, s->ts_list, s->ts_count,
+                                                        s->sg_rev, n, s->sg_frag, s->sg_vec, s->sg_w, nullptr,
+                                                        s->sg_nflag, s->rref, hard2, slot_flags(k), s->st));
+                s->sg_evals++;
+            } else {
+                LAUNCH(s, "ts_forces", launch_ts_forces(st, s->st, c.box_length, s->tp, c.precision, s->ts_list,
+                                                        s->ts_count, n, s->ts_dedr, s->ts_dedz, nullptr, s->rref,
+                                                        hard2, slot_flags(k), s->st));
+            }
             LAUNCH(s, "gear_step", launch_correct_predict_track(st, s->st, n, s->gc, s->mt, s->rref, slot_flags(k), fin));
             if ((rc = post(k))) return rc;
             s->t++;
@@ -740,13 +784,16 @@ bool valid_config(const mdx_config* c, std::string& why) {
     if (c->box != MDX_BOX_WALLED && c->box != MDX_BOX_PERIODIC) { why = "unknown box"; return false; }
     if (c->precision != MDX_PREC_MIXED && c->precision != MDX_PREC_FP64) { why = "unknown precision"; return false; }
     if (c->lj_table != MDX_LJ_REFERENCE && c->lj_table != MDX_LJ_UNIFORM) { why = "unknown LJ table"; return false; }
-    if (c->bond_form != MDX_BOND_REFERENCE && c->bond_form != MDX_BOND_TERSOFF) { why = "unknown bond form"; return false; }
+    if (c->bond_form != MDX_BOND_REFERENCE && c->bond_form != MDX_BOND_TERSOFF && c->bond_form != MDX_BOND_SURROGATE) {
+        why = "unknown bond form";
+        return false;
+    }
     if (c->use_lj && !(c->lj_rcut > 0 && c->lj_rs >= c->lj_rcut)) { why = "need 0 < LennardJonesRCUT <= LennardJonesRS"; return false; }
     if (c->use_bond && !(c->mb_rcut > 0 && c->mb_rs >= c->mb_rcut)) { why = "need 0 < ManyBodyRCUT <= ManyBodyRS"; return false; }
     if (c->box == MDX_BOX_PERIODIC) {
         if (!(c->box_length > 0)) { why = "periodic box needs box_length > 0"; return false; }
         if (c->use_lj && c->use_bond) { why = "periodic box runs one potential at a time"; return false; }
-        if (c->use_bond && c->bond_form != MDX_BOND_TERSOFF) { why = "periodic bond order is Tersoff"; return false; }
+        if (c->use_bond && c->bond_form == MDX_BOND_REFERENCE) { why = "periodic bond order is Tersoff or its surrogate"; return false; }
         if (c->use_bond && c->n_bond != c->n_atoms) { why = "periodic Tersoff needs n_bond == n_atoms"; return false; }
         if (c->use_bond && !(c->mb_rs >= 3.0f)) { why = "Tersoff list radius mb_rs must be >= R + D = 3.0"; return false; }
     } else if (c->use_bond && c->bond_form != MDX_BOND_REFERENCE) {
@@ -804,6 +851,13 @@ int mdx_create(const mdx_config* cfg, mdx_system** out) {
         s->tp = tersoff_si_params();
         s->ts_cap = cfg->max_neighbors > 0 ? cfg->max_neighbors : 32;
         if ((rc = dalloc(s, &s->ts_count, n))) { g_err = s->err; return bail(rc); }
+        if (cfg->bond_form == MDX_BOND_SURROGATE &&
+            ((rc = dalloc(s, &s->sg_frag, sg_frag_u16())) || (rc = dalloc(s, &s->sg_vec, sg_vec_len())) ||
+             (rc = dalloc(s, &s->sg_nflag, 1)))) {
+            g_err = s->err;
+            return bail(rc);
+        }
+        if (s->sg_nflag) hipMemsetAsync(s->sg_nflag, 0, sizeof(int32_t), s->stream);
     }
     if (hipHostMalloc(&s->pinned, 256, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&s->hflags, sizeof(uint64_t) * mdx_system::kFlagWords * mdx_system::kRing,
@@ -918,6 +972,35 @@ int mdx_set_forces(mdx_system* s, const double* force3) {
                                  st));
     LAUNCH(s, "forces_in", launch_scatter3(st, s->alt.f, s->slot_of_id, s->st.f, n));
     HIPCHK(s, hipStreamSynchronize(st));
+    return MDX_OK;
+}
+
+int mdx_set_surrogate(mdx_system* s, const float* params, int64_t len) {
+    if (!s || !params) return fail(s, MDX_ERR_ARG, "params is required");
+    if (s->cfg.bond_form != MDX_BOND_SURROGATE || !is_tersoff(s))
+        return fail(s, MDX_ERR_STATE, "the system's bond form is not MDX_BOND_SURROGATE");
+    if (len != MDX_SG_NPARAM) return fail(s, MDX_ERR_ARG, "parameter blob must hold MDX_SG_NPARAM floats");
+    for (int64_t k = 0; k < len; k++)
+        if (!std::isfinite(params[k]) && !(k >= MDX_SG_NTRAIN + 2 * MDX_SG_NFEAT))   // lo / hi may be infinite
+            return fail(s, MDX_ERR_ARG, "non-finite network parameter " + std::to_string(k));
+    std::vector<uint16_t> frag(sg_frag_u16());
+    std::vector<float> vec(sg_vec_len());
+    sg_pack(params, frag.data(), vec.data());
+    HIPCHK(s, hipMemcpyAsync(s->sg_frag, frag.data(), sizeof(uint16_t) * frag.size(), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(s, hipMemcpyAsync(s->sg_vec, vec.data(), sizeof(float) * vec.size(), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    s->sg_set = true;
+    return MDX_OK;
+}
+
+int mdx_surrogate_counts(mdx_system* s, int64_t* out2) {
+    if (!s || !out2) return fail(s, MDX_ERR_ARG, "null argument");
+    if (!is_surrogate(s)) return fail(s, MDX_ERR_STATE, "the system's bond form is not MDX_BOND_SURROGATE");
+    int32_t nf;
+    int rc = read_ints(s, &nf, s->sg_nflag, 1);
+    if (rc) return rc;
+    out2[0] = nf;
+    out2[1] = s->sg_evals * (int64_t)s->n;
     return MDX_OK;
 }
 

@@ -150,6 +150,11 @@ void orc_tersoff_si_params(orc_tersoff_params* p);  /* Si(C) set with R=2.7, D=0
  * brute force over all pairs with the minimum image.  f may be NULL. */
 double orc_tersoff_energy_forces(const orc_tersoff_params* p, const double* pos3, int32_t n,
                                  double L, double* f3);
+/* Per-centre form: only centres with centre[i] != 0 (all if NULL) contribute
+ * their e_i = 1/2 sum_j V_ij; f3 = minus the gradient of that partial sum,
+ * e_atom (if non-null) = e_i.  The surrogate's plain-Tersoff fallback. */
+double orc_tersoff_centres(const orc_tersoff_params* p, const double* pos3, int32_t n, double L,
+                           const uint8_t* centre, double* f3, double* e_atom);
 
 #ifdef __cplusplus
 }

@@ -372,12 +372,24 @@ static ts_nb* ts_neighbors(const orc_tersoff_params* p, const double* x, int32_t
  */
 double orc_tersoff_energy_forces(const orc_tersoff_params* p, const double* x, int32_t n,
                                  double L, double* f) {
+    return orc_tersoff_centres(p, x, n, L, NULL, f, NULL);
+}
+
+/*
+ * The same sum restricted to the centres i with centre[i] != 0 (all centres
+ * when centre is NULL): the energy sum_{i in centres} e_i with
+ * e_i = 1/2 sum_j V_ij, its negative gradient in f and e_i in e_atom.
+ */
+double orc_tersoff_centres(const orc_tersoff_params* p, const double* x, int32_t n, double L,
+                           const uint8_t* centre, double* f, double* e_atom) {
     int32_t* st;
     ts_nb* nb = ts_neighbors(p, x, n, L, &st);
     if (!nb) return NAN;
     if (f) memset(f, 0, sizeof(double) * 3 * (size_t)n);
+    if (e_atom) memset(e_atom, 0, sizeof(double) * (size_t)n);
     double E = 0;
     for (int32_t i = 0; i < n; i++) {
+        if (centre && !centre[i]) continue;
         for (int32_t a = st[i]; a < st[i + 1]; a++) {
             const ts_nb* J = &nb[a];
             double rij = J->r, u[3] = {J->d[0] / rij, J->d[1] / rij, J->d[2] / rij};
@@ -404,6 +416,7 @@ double orc_tersoff_energy_forces(const orc_tersoff_params* p, const double* x, i
                 dbz = -0.5 * bz / (1 + t) * t / zeta;   /* db/dzeta */
             }
             E += 0.5 * fc * (fr + bz * fa);
+            if (e_atom) e_atom[i] += 0.5 * fc * (fr + bz * fa);
             if (!f) continue;
             /* pair part: d/d r_ij at fixed zeta */
             double dEdr = 0.5 * (dfc * (fr + bz * fa) + fc * (dfr + bz * dfa));

@@ -74,6 +74,8 @@ def lib() -> C.CDLL:
         L.orc_tersoff_si_params.argtypes = [C.POINTER(TersoffParams)]
         L.orc_tersoff_energy_forces.restype = D
         L.orc_tersoff_energy_forces.argtypes = [C.POINTER(TersoffParams), P, I, D, P]
+        L.orc_tersoff_centres.restype = D
+        L.orc_tersoff_centres.argtypes = [C.POINTER(TersoffParams), P, I, D, P, P, P]
         _lib = L
     return _lib
 
@@ -249,6 +251,18 @@ def tersoff(pos3, L, p=None, forces=True):
     f = np.zeros_like(pos3) if forces else None
     e = lib().orc_tersoff_energy_forces(C.byref(p), ptr(pos3), len(pos3), L, ptr(f))
     return e, f
+
+
+def tersoff_centres(pos3, L, centre, p=None):
+    """Per-centre Tersoff: (energy of the selected centres, minus its
+    gradient, per-atom energies e_i of the selected centres)."""
+    p = p or tersoff_params()
+    pos3 = f64(pos3)
+    mask = np.ascontiguousarray(np.asarray(centre, dtype=np.uint8))
+    f = np.zeros_like(pos3)
+    e_atom = np.zeros(len(pos3))
+    e = lib().orc_tersoff_centres(C.byref(p), ptr(pos3), len(pos3), L, ptr(mask), ptr(f), ptr(e_atom))
+    return e, f, e_atom
 
 
 def tersoff_run(pos3, vel3, L, dt, nsteps, mass, scale):

@@ -1,0 +1,168 @@
+"""The learned Tersoff surrogate on the GPU against the oracle
+(oracle/surrogate.py): descriptors and the training gradient of the FP32
+training kernels, the BF16 MFMA inference forces and energies (the oracle
+emulates the kernel's BF16 rounding points), and the per-atom plain-Tersoff
+fallback (against the oracle's per-centre Tersoff)."""
+import numpy as np
+import pytest
+
+import _oracle as O
+from moleculardynamics_amd import (BOND_SURROGATE, BOND_TERSOFF, BOX_PERIODIC, LIST_BOND, LJ_UNIFORM, PREC_FP64,
+                                   PREC_MIXED, SgTrainer, System, initial)
+from oracle import surrogate as S
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_rms(a, b):
+    return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
+def frames(n_frames=4, ncell=2, seed=20):
+    ps, fs, es = [], [], []
+    for k in range(n_frames):
+        pos, _, L = initial.si_diamond(ncell, 300.0, seed + k, disp=0.06 + 0.02 * k)
+        e, f = O.tersoff(pos, L)
+        ps.append(pos)
+        fs.append(f)
+        es.append(e)
+    return np.array(ps), np.array(fs), np.array(es), L
+
+
+def random_params(G, seed=0):
+    rng = np.random.default_rng(seed)
+    p = {
+        "W1": rng.normal(0, 0.3, (S.NHID, S.NFEAT)), "b1": rng.normal(0, 0.1, S.NHID),
+        "W2": rng.normal(0, 0.15, (S.NHID, S.NHID)), "b2": rng.normal(0, 0.1, S.NHID),
+        "w3": rng.normal(0, 0.3, S.NHID), "b3": np.array([-4.6]),
+        "mean": G.mean(0), "istd": 1.0 / (G.std(0) + 1e-2),
+        "lo": np.full(S.NFEAT, -np.inf), "hi": np.full(S.NFEAT, np.inf),
+    }
+    # the blob is float32: the oracle works with the same rounded values
+    return S.unpack(S.pack(p).astype(np.float32).astype(np.float64))
+
+
+def surrogate_system(n, L):
+    return System(n_atoms=n, n_bond=n, box=BOX_PERIODIC, precision=PREC_MIXED, use_lj=0, use_bond=1,
+                  lj_table=LJ_UNIFORM, bond_form=BOND_SURROGATE, dt=1.0, mb_rs=3.3, mb_rcut=3.0, box_length=L,
+                  mass=initial.SI_MASS, accel_scale=initial.ACCEL_EV_A_AMU)
+
+
+def sg_forces(pos, L, blob):
+    """Forces and energy of the surrogate system (two evaluations: forces,
+    then the energy) and its fallback counts."""
+    n = len(pos)
+    with surrogate_system(n, L) as s:
+        s.set_surrogate(blob.astype(np.float32))
+        s.set_state(np.c_[pos, np.zeros(n)], np.zeros((n, 3)))
+        s.build_list(LIST_BOND)
+        s.bond_forces()
+        f = s.get_state()["force"]
+        e = s.potential()[1]
+        counts = s.surrogate_counts()
+    return f, e, counts
+
+
+@pytest.fixture(scope="module")
+def silicon216():
+    return initial.si_diamond(3, 1000.0, seed=31, disp=0.09)
+
+
+def test_trainer_descriptors_match_oracle():
+    P, F, E, L = frames()
+    with SgTrainer(P, F, E, L) as t:
+        for k in range(len(P)):
+            G = t.descriptors(k).astype(np.float64)
+            Go = S.frame_descriptors(P[k], L)[0]
+            scale = np.abs(Go).max(0) + 1e-3
+            assert np.abs((G - Go) / scale).max() < 2e-5, k
+        st = t.feature_stats()
+    Gall = np.concatenate([S.frame_descriptors(P[k], L)[0] for k in range(len(P))])
+    np.testing.assert_allclose(st["mean"], Gall.mean(0), rtol=1e-4, atol=1e-5)
+    np.testing.assert_allclose(st["std"], Gall.std(0), rtol=1e-3, atol=1e-5)
+
+
+def test_trainer_gradient_matches_oracle():
+    P, F, E, L = frames()
+    G = np.concatenate([S.frame_descriptors(P[k], L)[0] for k in range(len(P))])
+    p = random_params(G, seed=3)
+    blob = S.pack(p).astype(np.float32)
+    sel = [2, 0, 3]
+    fr = []
+    for k in sel:
+        Gk, Jk, idx, _ = S.frame_descriptors(P[k], L, slots=8)
+        fr.append((Gk, Jk, idx, F[k], E[k]))
+    loss, g = S.train_loss_gradient(p, fr, w_energy=0.5)
+    with SgTrainer(P, F, E, L) as t:
+        t.set_params(blob)
+        out, gg = t.gradient(sel, w_energy=0.5)
+    assert out[0] == pytest.approx(loss, rel=1e-4)
+    flat = np.concatenate([g[k].reshape(-1) for k in ("W1", "b1", "W2", "b2", "w3", "b3")])
+    assert np.abs(gg - flat).max() < 1e-4 * np.abs(flat).max()
+    assert rel_rms(gg, flat) < 1e-5
+
+
+def test_trainer_steps_reduce_loss():
+    P, F, E, L = frames(6)
+    G = np.concatenate([S.frame_descriptors(P[k], L)[0] for k in range(len(P))])
+    p = random_params(G, seed=4)
+    with SgTrainer(P, F, E, L) as t:
+        t.set_params(S.pack(p).astype(np.float32))
+        first = t.gradient(range(6), want_grad=False)[0][0]
+        for _ in range(60):
+            t.step(range(6), lr=3e-3)
+        last = t.gradient(range(6), want_grad=False)[0][0]
+    assert last < 0.3 * first
+
+
+def test_surrogate_forces_match_bf16_oracle(silicon216):
+    pos, _, L = silicon216
+    G = S.frame_descriptors(pos, L)[0]
+    p = random_params(G, seed=5)
+    f, e, counts = sg_forces(pos, L, S.pack(p))
+    Eo, Fo, _, fl = S.energy_forces(p, pos, L, emulate_bf16=True)
+    E64, F64, _, _ = S.energy_forces(p, pos, L)
+    assert not fl.any() and counts == (0, 2 * len(pos))
+    assert rel_rms(f, Fo) < 2e-3
+    assert e == pytest.approx(Eo, rel=1e-5, abs=1e-3)
+    # BF16 inference stays near the FP64 network, and the emulation is closer
+    assert rel_rms(f, F64) < 0.05
+    assert rel_rms(f, Fo) < rel_rms(f, F64)
+    np.testing.assert_allclose(f.sum(0), 0, atol=1e-6 * np.abs(f).max() * len(f))
+
+
+def test_surrogate_full_fallback_is_plain_tersoff(silicon216):
+    pos, _, L = silicon216
+    G = S.frame_descriptors(pos, L)[0]
+    p = random_params(G, seed=6)
+    p["hi"] = np.full(S.NFEAT, -np.inf)      # every atom out of range
+    f, e, counts = sg_forces(pos, L, S.pack(p))
+    eo, fo = O.tersoff(pos, L)
+    assert counts == (2 * len(pos), 2 * len(pos))
+    assert rel_rms(f, fo) < 1e-5
+    # FP32 Tersoff terms: the float32-rounded parameters alone shift this
+    # configuration's energy by +1.0e-6 relative (a float32 emulation of the
+    # pair terms on the CPU gives the same +9.4e-4 eV)
+    assert e == pytest.approx(eo, rel=2e-6)
+
+
+def test_surrogate_partial_fallback_matches_oracle(silicon216):
+    pos, _, L = silicon216
+    G = S.frame_descriptors(pos, L)[0]
+    p = random_params(G, seed=7)
+    gs = np.sort(S.standardize(p, G)[:, 1])
+    k = int(0.7 * len(gs))
+    gap = np.argmax(np.diff(gs[k - 20: k + 20])) + k - 20     # a threshold well away from every value
+    p["hi"][1] = 0.5 * (gs[gap] + gs[gap + 1])
+    assert gs[gap + 1] - gs[gap] > 1e-4
+
+    def fallback(mask):
+        _, ft, e_atom = O.tersoff_centres(pos, L, mask.astype(np.uint8))
+        return e_atom, ft
+
+    Eo, Fo, _, fl = S.energy_forces(p, pos, L, emulate_bf16=True, tersoff_fallback=fallback)
+    f, e, counts = sg_forces(pos, L, S.pack(p))
+    assert 0 < fl.sum() < len(pos)
+    assert counts == (2 * int(fl.sum()), 2 * len(pos))
+    assert rel_rms(f, Fo) < 2e-3
+    assert e == pytest.approx(Eo, rel=1e-5, abs=1e-3)
