@@ -15,6 +15,13 @@ silicon (R = 2.7 A, D = 0.3 A), 262,144 atoms of diamond (32^3 cells of
 5.431 A) with 0.05 A Gaussian displacements and 1000 K velocities, list
 radius 3.3 A, dt = 1 fs, mixed precision.
 
+--workload surrogate measures the learned surrogate of the Tersoff
+configuration (same system and initial state): descriptors, the 48-64-64-1
+network on BF16 MFMA and its analytic forces, with plain-Tersoff fallback for
+out-of-range atoms; plain Tersoff is timed beside it on the same state, and
+the surrogate's force error is taken against plain Tersoff on the same
+configuration.
+
 --workload replicas measures the batched configuration: 512 independent
 periodic LJ replicas of 256-8788 atoms (log-uniform sizes, density
 U[0.70, 0.95], T* U[0.5, 2.0]; 1.13M atoms together), rc 2.5, rs 2.8,
@@ -67,7 +74,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-ncell", type=int, default=32)
     p.add_argument("--cpu-steps", type=int, default=20)
-    p.add_argument("--workload", choices=["lj", "tersoff", "replicas"], default="lj")
+    p.add_argument("--workload", choices=["lj", "tersoff", "replicas", "surrogate"], default="lj")
+    p.add_argument("--weights", default=None, help="surrogate network (default: the shipped one)")
+    p.add_argument("--mb-rs", type=float, default=3.3, help="Tersoff / surrogate list radius (A)")
     p.add_argument("--copy-bytes", type=int, default=0,
                    help="only run device-to-device copies of this many bytes (counter calibration)")
     return p.parse_args()
@@ -200,6 +209,152 @@ def bench_tersoff(a) -> None:
     print(json.dumps(out))
 
 
+# the surrogate's force kernel (k_sg_eval) per atom: positions read (24 B;
+# neighbour positions are cache hits), the list (4 x 4 B) and its count (4 B)
+# read, the per-slot gradients written (4 slots x 12 B)
+SG_EVAL_BYTES_PER_ATOM = 24 + 16 + 4 + 48
+# network work per atom as the kernel issues it (16x16x32 BF16 MFMA tiles,
+# K of the first layer padded 48 -> 64): forward 2 layers, backward 2 layers
+SG_MFMA_FLOP_PER_ATOM = 2 * (64 * 64 + 64 * 64 + 64 * 64 + 48 * 64)
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def cpu_baseline_surrogate(ncell: int, steps: int, blob) -> dict:
+    """The FP64 oracle's surrogate (numpy: descriptors, network, analytic
+    forces) with the oracle's Gear predictor-corrector, on a sub-box sample."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import _oracle as O
+    from moleculardynamics_amd import initial
+    from oracle import surrogate as S
+
+    p = S.unpack(np.asarray(blob, dtype=np.float64))
+    pos, vel, L = initial.si_diamond(ncell, 1000.0, seed=12345, disp=0.05)
+    m, sc = initial.SI_MASS, initial.ACCEL_EV_A_AMU
+    r, v = pos.copy(), vel.copy()
+
+    def fb(mask):   # plain-Tersoff per-atom terms of out-of-range atoms
+        _, f_t, e_t = O.tersoff_centres(r, L, mask.astype(np.uint8))
+        return e_t, f_t
+
+    a_, b_, c_ = np.zeros_like(r), np.zeros_like(r), np.zeros_like(r)
+    from threadpoolctl import threadpool_limits
+
+    with threadpool_limits(1):   # one core, as stated below
+        t0 = time.perf_counter()
+        _, f, _, _ = S.energy_forces(p, r, L, tersoff_fallback=fb)
+        a_[:] = f / m * sc
+        for _ in range(steps):
+            r, v, a_, b_, c_ = O.gear_predict(r, v, a_, b_, c_, 1.0)
+            _, f, _, _ = S.energy_forces(p, np.mod(r, L), L, tersoff_fallback=fb)
+            r, v, a_, b_, c_ = O.gear_correct(r, v, f, a_, b_, c_, 1.0, m, sc)
+        elapsed = time.perf_counter() - t0
+    return {
+        "value": len(pos) * steps / elapsed,
+        "unit": "atom-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{len(pos)} atoms ({ncell}^3 diamond x8, 1000 K), {steps} steps of the numpy FP64 surrogate "
+                  f"(brute-force neighbours), {elapsed:.1f} s on {platform.processor() or platform.machine()}",
+    }
+
+
+def bench_surrogate(a) -> None:
+    from moleculardynamics_amd import BOND_SURROGATE, BOND_TERSOFF, LIST_BOND, initial
+    from moleculardynamics_amd import surrogate as SG
+
+    blob = SG.load_weights(a.weights) if a.weights else SG.load_weights()
+    ncell = 32 if a.ncell == 64 else a.ncell
+    pos, vel, L = initial.si_diamond(ncell, 1000.0, seed=2024, disp=0.05)
+    n = len(pos)
+    print(f"bench: surrogate Si {n} atoms, L={L:.4f}; warmup {a.warmup}, steps {a.steps}", file=sys.stderr,
+          flush=True)
+
+    def timed(form):
+        s = SG.silicon_system(n, L, bond_form=form, mb_rs=a.mb_rs)
+        if form == BOND_SURROGATE:
+            s.set_surrogate(blob)
+        s.set_state(np.c_[pos, np.zeros(n)], vel)
+        s.run(a.warmup)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.run(a.steps)
+        s.synchronize()
+        elapsed = time.perf_counter() - t0
+        s.set_timing(True)
+        s.run(min(a.steps, 50))
+        s.synchronize()
+        times = s.kernel_times()
+        s.set_timing(False)
+        return s, elapsed, times
+
+    sp, el_plain, t_plain = timed(BOND_TERSOFF)
+    final = np.mod(sp.get_state()["pos"][:, :3], L)
+    sp.close()
+    print(f"bench: plain Tersoff {el_plain:.3f} s; kernels {t_plain}", file=sys.stderr, flush=True)
+    ss, elapsed, times = timed(BOND_SURROGATE)
+    fallback, evals = ss.surrogate_counts()
+    ss.close()
+    print(f"bench: surrogate {elapsed:.3f} s; kernels {times}", file=sys.stderr, flush=True)
+    # force error on the configuration plain Tersoff reached
+    def forces(form):
+        with SG.silicon_system(n, L, bond_form=form) as s:
+            if form == BOND_SURROGATE:
+                s.set_surrogate(blob)
+            s.set_state(np.c_[final, np.zeros(n)], np.zeros((n, 3)))
+            s.build_list(LIST_BOND)
+            s.bond_forces()
+            return s.get_state()["force"]
+    f_plain, f_sg = forces(BOND_TERSOFF), forces(BOND_SURROGATE)
+    rmse = float(np.sqrt(np.mean((f_sg - f_plain) ** 2)))
+    value = n * a.steps / elapsed
+    value_plain = n * a.steps / el_plain
+    cnt, ms = times.get("sg_eval", (0, 0.0))
+    k_ms = ms / max(cnt, 1)
+    achieved = n * SG_EVAL_BYTES_PER_ATOM / (k_ms * 1e-3) / 1e9 if cnt else None
+    out = {
+        "metric": "atom-steps/s per GPU (Tersoff-Si surrogate 262k, BF16 MFMA network) vs plain Tersoff",
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * elapsed / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16-mfma-network/fp32-descriptors/fp64-accum",
+        "data": "synthetic (seeded diamond lattice + Gaussian displacements + Maxwell-Boltzmann velocities); "
+                "network trained on GPU plain-Tersoff frames",
+        "config": {"workload": f"periodic Tersoff-Si surrogate {n} atoms (diamond {ncell}^3 x8, a 5.431 A), "
+                               f"1000 K, list radius {a.mb_rs} A, dt 1 fs", "atoms": n},
+        "plain_tersoff": {"value": value_plain, "ms_per_step": 1e3 * el_plain / a.steps,
+                          "kernel_ms": {k: v[1] / max(v[0], 1) for k, v in t_plain.items()}},
+        "speedup_vs_plain": value / value_plain,
+        "force_rmse_vs_plain_eV_per_A": rmse,
+        "force_rms_plain_eV_per_A": float(np.sqrt(np.mean(f_plain ** 2))),
+        "fallback_fraction": fallback / max(evals, 1),
+        "roofline": {
+            "kernel": "k_sg_eval (descriptors, BF16 MFMA network, per-slot gradients)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": None,
+            "avg_launch_ms": k_ms,
+            "bytes_per_atom": SG_EVAL_BYTES_PER_ATOM,
+        },
+        "mfma": {"flop_per_atom": SG_MFMA_FLOP_PER_ATOM,
+                 "achieved_tflops": n * SG_MFMA_FLOP_PER_ATOM / (k_ms * 1e-3) / 1e12 if cnt else None,
+                 "peak_tflops": MFMA_BF16_PEAK_TFLOPS},
+        "kernel_ms": {k: v[1] / max(v[0], 1) for k, v in times.items()},
+    }
+    if not a.no_cpu:
+        print("bench: cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline_surrogate(4, 240, blob)
+    print(json.dumps(out))
+
+
 # one batched LJ step per atom (k_rep_step): the Gear state r, v, a, b, c read
 # and written (240 B), positions written to the second buffer (24 B is in
 # that), the force written (24 B), list-build positions read (24 B), the
@@ -310,6 +465,9 @@ def main():
         return
     if a.workload == "replicas":
         bench_replicas(a)
+        return
+    if a.workload == "surrogate":
+        bench_surrogate(a)
         return
     from moleculardynamics_amd import BOX_PERIODIC, LJ_UNIFORM, PREC_MIXED, System, initial
 

@@ -161,20 +161,48 @@ k_predict_track(DevState st, int32_t n, GearCoeffs gc, const double* __restrict_
 // (unless final) the predictor of step t+1 with the displacement from the
 // list-build positions and the single-step move max-reduced into the spread
 // flags, as k_predict_track does.
-template <bool kFinal>
+// With kGather the force is first gathered from the per-slot gradients of
+// the surrogate (written to st.f) and atoms at least sqrt(hard2) from the
+// list-build positions are counted as stale, as the bond-order kernels do.
+template <bool kFinal, bool kGather>
 __global__ void __launch_bounds__(kBlock)
 k_correct_predict_track(DevState st, int32_t n, GearCoeffs gc, MassTable m, const double* __restrict__ r0x,
                         const double* __restrict__ r0y, const double* __restrict__ r0z,
-                        unsigned long long* __restrict__ flags, int32_t spread) {
+                        unsigned long long* __restrict__ flags, int32_t spread, SlotGather g, double hard2) {
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     double d2 = 0.0, s2 = 0.0;
+    const double* r0[3] = {r0x, r0y, r0z};
+    double F[3] = {0.0, 0.0, 0.0};
+    if constexpr (kGather) {
+        bool stale = false;
+        if (i < n) {
+            const int32_t cnt = g.count[i];
+            for (int32_t s = 0; s < cnt; s++) {
+                const int32_t j = g.list[(int64_t)s * n + i];
+                const int32_t b = g.rev[(int64_t)s * n + i];
+#pragma unroll
+                for (int d = 0; d < 3; d++)
+                    F[d] += (double)g.w[(int64_t)(s * 3 + d) * n + i] - (double)g.w[(int64_t)(b * 3 + d) * n + j];
+            }
+            double e2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                st.f[d][i] = F[d];
+                const double e = st.r[d][i] - r0[d][i];
+                e2 += e * e;
+            }
+            stale = e2 >= hard2;
+        }
+        const unsigned long long nst = __popcll(__ballot(stale));
+        if ((threadIdx.x & (kWave - 1)) == 0 && nst)
+            atomicAdd(&flags[3 * ((blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) % spread) + 2], nst);
+    }
     if (i < n) {
         const double mass = m.mass[st.type[i] & (kTypes - 1)];
-        const double* r0[3] = {r0x, r0y, r0z};
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             double r = st.r[d][i], v = st.v[d][i], a = st.a[d][i], b = st.b[d][i], c = st.c[d][i];
-            correct1(st.f[d][i], mass, m.scale, gc, r, v, a, b, c);
+            correct1(kGather ? F[d] : st.f[d][i], mass, m.scale, gc, r, v, a, b, c);
             if constexpr (!kFinal) {
                 const double rp = r + gc.k[0] * v + gc.k[1] * a + gc.k[2] * b + gc.k[3] * c;
                 v = v + gc.k[0] * a + gc.k[1] * b + gc.k[2] * c;
@@ -217,12 +245,27 @@ hipError_t launch_correct_predict_track(hipStream_t s, const DevState& st, int32
                                         const MassTable& m, double* const rref[3], uint64_t* flags, bool final_step) {
     if (n <= 0) return hipSuccess;
     auto* f = reinterpret_cast<unsigned long long*>(flags);
+    const SlotGather none{nullptr, nullptr, nullptr, nullptr};
     if (final_step)
-        k_correct_predict_track<true><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(st, n, gc, m, rref[0], rref[1],
-                                                                                 rref[2], f, flag_spread());
+        k_correct_predict_track<true, false><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(
+            st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), none, 0.0);
     else
-        k_correct_predict_track<false><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(st, n, gc, m, rref[0], rref[1],
-                                                                                  rref[2], f, flag_spread());
+        k_correct_predict_track<false, false><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(
+            st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), none, 0.0);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_correct_predict_track(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
+                                               const MassTable& m, double* const rref[3], uint64_t* flags,
+                                               bool final_step, const SlotGather& g, double hard2) {
+    if (n <= 0) return hipSuccess;
+    auto* f = reinterpret_cast<unsigned long long*>(flags);
+    if (final_step)
+        k_correct_predict_track<true, true><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(
+            st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), g, hard2);
+    else
+        k_correct_predict_track<false, true><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(
+            st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), g, hard2);
     return hipGetLastError();
 }
 

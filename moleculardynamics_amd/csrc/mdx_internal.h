@@ -171,6 +171,19 @@ hipError_t launch_predict_track(hipStream_t s, const DevState& st, int32_t n, co
 // same flag tracking, in place (forces computed by a separate kernel)
 hipError_t launch_correct_predict_track(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
                                         const MassTable& m, double* const rref[3], uint64_t* flags, bool final_step);
+// Forces from per-slot gradients (the surrogate): F_i = sum_s w[s][d][i] -
+// w[rev[s][i]][d][list[s][i]] in FP64, written to st.f.
+struct SlotGather {
+    const int32_t* list;
+    const int32_t* count;
+    const uint8_t* rev;
+    const float* w;
+};
+// the gather, the stale count of st's positions against rref (sqrt(hard2)),
+// then launch_correct_predict_track's corrector and predictor, in one kernel
+hipError_t launch_gather_correct_predict_track(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
+                                               const MassTable& m, double* const rref[3], uint64_t* flags,
+                                               bool final_step, const SlotGather& g, double hard2);
 
 // lj_fused.hip: LDS-staged segment kernels of the periodic box
 int32_t lds_segments_per_column(int32_t nz, double atoms_per_cell, int32_t kmax);
@@ -216,6 +229,10 @@ hipError_t launch_sg_rev(hipStream_t s, const int32_t* list, const int32_t* coun
 // forces of st's positions into out.f (and per-atom energies into e_atom if
 // non-null); nflag counts the atoms that took the Tersoff fallback; stale
 // atoms against rref are counted into the spread flags as in launch_ts_forces
+// descriptors, network and per-slot gradients only (w), for the fused step
+hipError_t launch_sg_eval(hipStream_t s, const DevState& st, double L, const TersoffParams& p, const int32_t* list,
+                          const int32_t* count, int32_t n, const uint16_t* frag, const float* vec, float* w,
+                          double* e_atom, int32_t* nflag);
 hipError_t launch_sg_forces(hipStream_t s, const DevState& st, double L, const TersoffParams& p, const int32_t* list,
                             const int32_t* count, const uint8_t* rev, int32_t n, const uint16_t* frag,
                             const float* vec, float* w, double* e_atom, int32_t* nflag, double* const rref[3],

@@ -56,7 +56,17 @@ __device__ inline bf16x8 as_bf16x8(uint4 v) {
     return r;
 }
 
-__device__ inline uint32_t pack2(float a, float b) { return bf16_bits(a) | (bf16_bits(b) << 16); }
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// two floats -> two BF16 (round to nearest even, v_cvt_pk_bf16_f32), a low
+__device__ inline uint32_t pack2(float a, float b) {
+    const f32x2 v = {a, b};
+    const bf16x2 h = __builtin_convertvector(v, bf16x2);
+    uint32_t u;
+    __builtin_memcpy(&u, &h, 4);
+    return u;
+}
 
 // 8 values a[2ks + t/4][t%4], t = 0..7, as one BF16 B operand
 __device__ inline bf16x8 operand(const float (&a)[4][4], int ks) {
@@ -388,10 +398,9 @@ hipError_t launch_sg_rev(hipStream_t s, const int32_t* list, const int32_t* coun
     return hipGetLastError();
 }
 
-hipError_t launch_sg_forces(hipStream_t s, const DevState& st, double L, const TersoffParams& p, const int32_t* list,
-                            const int32_t* count, const uint8_t* rev, int32_t n, const uint16_t* frag,
-                            const float* vec, float* w, double* e_atom, int32_t* nflag, double* const rref[3],
-                            double hard2, uint64_t* flags, const DevState& out) {
+hipError_t launch_sg_eval(hipStream_t s, const DevState& st, double L, const TersoffParams& p, const int32_t* list,
+                          const int32_t* count, int32_t n, const uint16_t* frag, const float* vec, float* w,
+                          double* e_atom, int32_t* nflag) {
     if (n <= 0) return hipSuccess;
     static bool attr = false;
     if (!attr) {
@@ -399,9 +408,19 @@ hipError_t launch_sg_forces(hipStream_t s, const DevState& st, double L, const T
         if (e != hipSuccess) return e;
         attr = true;
     }
+    k_sg_eval<<<grid_for(n), kBlock, kLds, s>>>(pos3(st, L), list, count, n, reinterpret_cast<const uint4*>(frag),
+                                                vec, cast_params<float>(p), w, e_atom, nflag);
+    return hipGetLastError();
+}
+
+hipError_t launch_sg_forces(hipStream_t s, const DevState& st, double L, const TersoffParams& p, const int32_t* list,
+                            const int32_t* count, const uint8_t* rev, int32_t n, const uint16_t* frag,
+                            const float* vec, float* w, double* e_atom, int32_t* nflag, double* const rref[3],
+                            double hard2, uint64_t* flags, const DevState& out) {
+    if (n <= 0) return hipSuccess;
+    hipError_t e = launch_sg_eval(s, st, L, p, list, count, n, frag, vec, w, e_atom, nflag);
+    if (e != hipSuccess) return e;
     const Pos3 P = pos3(st, L);
-    k_sg_eval<<<grid_for(n), kBlock, kLds, s>>>(P, list, count, n, reinterpret_cast<const uint4*>(frag), vec,
-                                                cast_params<float>(p), w, e_atom, nflag);
     const double* r0[3] = {rref ? rref[0] : nullptr, rref ? rref[1] : nullptr, rref ? rref[2] : nullptr};
     k_sg_gather<<<grid_for(n), kBlock, 0, s>>>(P, list, count, rev, n, w, out.f[0], out.f[1], out.f[2], r0[0], r0[1],
                                                r0[2], hard2, reinterpret_cast<unsigned long long*>(flags),

@@ -745,10 +745,15 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
         HIPCHK(s, hipMemsetAsync(slot_flags(k), 0, W * sizeof(uint64_t), st));
         if (ts) {
             if (is_surrogate(s)) {
-                LAUNCH(s, "sg_forces", launch_sg_forces(st, s->st, c.box_length, s->tp, s->ts_list, s->ts_count,
-                                                        s->sg_rev, n, s->sg_frag, s->sg_vec, s->sg_w, nullptr,
-                                                        s->sg_nflag, s->rref, hard2, slot_flags(k), s->st));
+                LAUNCH(s, "sg_eval", launch_sg_eval(st, s->st, c.box_length, s->tp, s->ts_list, s->ts_count, n,
+                                                    s->sg_frag, s->sg_vec, s->sg_w, nullptr, s->sg_nflag));
                 s->sg_evals++;
+                const SlotGather g{s->ts_list, s->ts_count, s->sg_rev, s->sg_w};
+                LAUNCH(s, "gather_gear_step", launch_gather_correct_predict_track(st, s->st, n, s->gc, s->mt, s->rref,
+                                                                                  slot_flags(k), fin, g, hard2));
+                if ((rc = post(k))) return rc;
+                s->t++;
+                continue;
             } else {
                 LAUNCH(s, "ts_forces", launch_ts_forces(st, s->st, c.box_length, s->tp, c.precision, s->ts_list,
                                                         s->ts_count, n, s->ts_dedr, s->ts_dedz, nullptr, s->rref,
