@@ -180,9 +180,10 @@ k_correct_predict_track(DevState st, int32_t n, GearCoeffs gc, MassTable m, cons
             for (int32_t s = 0; s < cnt; s++) {
                 const int32_t j = g.list[(int64_t)s * n + i];
                 const int32_t b = g.rev[(int64_t)s * n + i];
-#pragma unroll
-                for (int d = 0; d < 3; d++)
-                    F[d] += (double)g.w[(int64_t)(s * 3 + d) * n + i] - (double)g.w[(int64_t)(b * 3 + d) * n + j];
+                const float4 u = g.w[(int64_t)s * n + i], v = g.w[(int64_t)b * n + j];
+                F[0] += (double)u.x - (double)v.x;
+                F[1] += (double)u.y - (double)v.y;
+                F[2] += (double)u.z - (double)v.z;
             }
             double e2 = 0.0;
 #pragma unroll

@@ -177,7 +177,7 @@ struct SlotGather {
     const int32_t* list;
     const int32_t* count;
     const uint8_t* rev;
-    const float* w;
+    const float4* w;   // [slot][atom], x y z used
 };
 // the gather, the stale count of st's positions against rref (sqrt(hard2)),
 // then launch_correct_predict_track's corrector and predictor, in one kernel
@@ -231,11 +231,11 @@ hipError_t launch_sg_rev(hipStream_t s, const int32_t* list, const int32_t* coun
 // atoms against rref are counted into the spread flags as in launch_ts_forces
 // descriptors, network and per-slot gradients only (w), for the fused step
 hipError_t launch_sg_eval(hipStream_t s, const DevState& st, double L, const TersoffParams& p, const int32_t* list,
-                          const int32_t* count, int32_t n, const uint16_t* frag, const float* vec, float* w,
+                          const int32_t* count, int32_t n, const uint16_t* frag, const float* vec, float4* w,
                           double* e_atom, int32_t* nflag);
 hipError_t launch_sg_forces(hipStream_t s, const DevState& st, double L, const TersoffParams& p, const int32_t* list,
                             const int32_t* count, const uint8_t* rev, int32_t n, const uint16_t* frag,
-                            const float* vec, float* w, double* e_atom, int32_t* nflag, double* const rref[3],
+                            const float* vec, float4* w, double* e_atom, int32_t* nflag, double* const rref[3],
                             double hard2, uint64_t* flags, const DevState& out);
 
 // bond_ref.hip

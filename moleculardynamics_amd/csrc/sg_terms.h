@@ -30,12 +30,29 @@ constexpr int kOffW1 = 0, kOffB1 = kNH * kNF, kOffW2 = kOffB1 + kNH, kOffB2 = kO
 constexpr float kRc = 3.0f, kRin = 2.4f, kRmid = 2.7f, kRhalf = 0.3f;
 constexpr float kEta = 0.5 * (31.0 / 1.5) * (31.0 / 1.5);
 constexpr float kEtaB = 0.5 / (0.2 * 0.2);
+constexpr float kLog2e = 1.4426950408889634f;
+// radial Gaussians by recurrence from two anchor centres a = 8 and 24, each
+// for the 16 centres around it:
+//   e_{k+1} = e_k up_k, up_{k+1} = up_k q;  e_{k-1} = e_k dn_k, dn_{k-1} = dn_k q
+//   up_a = exp(2 eta D x - eta D^2), dn_a = exp(-2 eta D x - eta D^2), q = exp(-2 eta D^2)
+// with x = r - mu_a, D = 1/31.  An anchor underflows FP32 only when |x| > 0.64 A,
+// and then every centre of its half (at most 8 D = 0.26 A away) is below 1e-13.
+// A float32 emulation over r in [1.5, 3] A stays within 2.7e-5 relative on every
+// term above 1e-8.
+constexpr int kMid = 16;
+constexpr float kDelta = 1.0f / 31.0f;
+constexpr float kMuMid = 2.0 + 16.0 / 31.0;
+constexpr float kQ = 0.6411803884299546f;
+constexpr float kE2 = -308.0955409542875f;      // -eta log2(e)
+constexpr float kE2U = 19.877131674470164f;     // 2 eta D log2(e)
+constexpr float kE2C = -0.3205988979753252f;    // -eta D^2 log2(e)
+constexpr float kEB2 = -18.03368801111204f;     // -eta_b log2(e)
 
-__device__ inline float mu(int k) { return 2.0f + (float)k * (1.0f / 31.0f); }
 __device__ inline float nu(int m) { return m == 0 ? 2.25f : 2.55f; }
+__device__ inline float exp2_raw(float x) { return __builtin_amdgcn_exp2f(x); }
 
 struct Nb {   // one neighbour slot; fc = dfc = 0 marks an empty slot
-    float ux, uy, uz, r, fc, dfc;
+    float ux, uy, uz, r, ir, fc, dfc;
 };
 
 __device__ inline void cutoff(float r, float& fc, float& dfc) {
@@ -47,20 +64,22 @@ __device__ inline void cutoff(float r, float& fc, float& dfc) {
         dfc = 0.0f;
     } else {
         float sx, cx;
-        sincosf((float)M_PI_2 * (r - kRmid) / kRhalf, &sx, &cx);
+        sincosf((float)M_PI_2 * (1.0f / kRhalf) * (r - kRmid), &sx, &cx);
         fc = 0.5f - 0.5f * sx;
-        dfc = -0.25f * (float)M_PI / kRhalf * cx;
+        dfc = -0.25f * (float)M_PI * (1.0f / kRhalf) * cx;
     }
 }
 
-// slot from a displacement (FP64 minimum image)
+// slot from a displacement (FP64 minimum image): r^2 in FP64 (the Gaussian
+// tails amplify an error in r by 2 eta |r - mu|), the rest in FP32
 __device__ inline Nb make_nb(double dx, double dy, double dz) {
     Nb b;
-    const double rr = sqrt(dx * dx + dy * dy + dz * dz);
-    b.r = (float)rr;
-    b.ux = (float)(dx / rr);
-    b.uy = (float)(dy / rr);
-    b.uz = (float)(dz / rr);
+    const float x = (float)dx, y = (float)dy, z = (float)dz;
+    b.r = __builtin_sqrtf((float)(dx * dx + dy * dy + dz * dz));
+    b.ir = __builtin_amdgcn_rcpf(b.r);
+    b.ux = x * b.ir;
+    b.uy = y * b.ir;
+    b.uz = z * b.ir;
     cutoff(b.r, b.fc, b.dfc);
     if (!(b.fc > 0.0f)) {   // outside rc: no contribution, no derivative
         b.fc = 0.0f;
@@ -69,7 +88,33 @@ __device__ inline Nb make_nb(double dx, double dy, double dz) {
     return b;
 }
 
-__device__ inline Nb empty_nb() { return Nb{0.0f, 0.0f, 0.0f, 10.0f, 0.0f, 0.0f}; }
+__device__ inline Nb empty_nb() { return Nb{0.0f, 0.0f, 0.0f, 10.0f, 0.1f, 0.0f, 0.0f}; }
+
+// calls f(k, e_k) for the 32 radial Gaussians of distance r
+template <typename F>
+__device__ inline void radial(float r, F&& f) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int ka = 8 + 16 * h;
+        const float x = r - (h == 0 ? (float)(2.0 + 8.0 / 31.0) : (float)(2.0 + 24.0 / 31.0));
+        const float e = exp2_raw(kE2 * x * x);
+        float up = exp2_raw(fmaf(kE2U, x, kE2C)), dn = exp2_raw(fmaf(-kE2U, x, kE2C));
+        f(ka, e);
+        float eu = e, ed = e;
+#pragma unroll
+        for (int k = ka + 1; k < ka + 8; k++) {
+            eu *= up;
+            up *= kQ;
+            f(k, eu);
+        }
+#pragma unroll
+        for (int k = ka - 1; k >= ka - 8; k--) {
+            ed *= dn;
+            dn *= kQ;
+            f(k, ed);
+        }
+    }
+}
 
 // bond-resolved angular moments A[s][0..3]
 template <int kS>
@@ -114,11 +159,8 @@ __device__ inline void forward(const Nb (&nb)[kS], float (&G)[kNF]) {
 #pragma unroll
     for (int s = 0; s < kS; s++) {
         if (nb[s].fc > 0.0f) {
-#pragma unroll
-            for (int k = 0; k < kNR; k++) {
-                const float d = nb[s].r - mu(k);
-                G[k] = fmaf(nb[s].fc, __expf(-kEta * d * d), G[k]);
-            }
+            const float fc = nb[s].fc;
+            radial(nb[s].r, [&](int k, float e) { G[k] = fmaf(fc, e, G[k]); });
         }
     }
     float A[kS][4];
@@ -131,7 +173,7 @@ __device__ inline void forward(const Nb (&nb)[kS], float (&G)[kNF]) {
 #pragma unroll
             for (int m = 0; m < 2; m++) {
                 const float d = nb[s].r - nu(m);
-                const float psi = __expf(-kEtaB * d * d) * nb[s].fc;
+                const float psi = exp2_raw(kEB2 * d * d) * nb[s].fc;
 #pragma unroll
                 for (int q = 0; q < 8; q++) G[kNR + 8 * m + q] = fmaf(psi, mo[q], G[kNR + 8 * m + q]);
             }
@@ -151,13 +193,15 @@ __device__ inline void backward(const Nb (&nb)[kS], const float (&h)[kNF], float
         Lam[s][0] = Lam[s][1] = Lam[s][2] = Lam[s][3] = 0.0f;
         if (nb[s].fc > 0.0f) {
             const float r = nb[s].r, fc = nb[s].fc, dfc = nb[s].dfc;
-            float coef = 0.0f;
-#pragma unroll
-            for (int k = 0; k < kNR; k++) {
-                const float d = r - mu(k);
-                const float e = __expf(-kEta * d * d);
-                coef = fmaf(h[k] * e, fmaf(-2.0f * kEta * d, fc, dfc), coef);
-            }
+            // sum_k h_k e_k (dfc - 2 eta (r - mu_k) fc), r - mu_k = x - (k - 16) D
+            float s0 = 0.0f, s1 = 0.0f;
+            radial(r, [&](int k, float e) {
+                const float he = h[k] * e;
+                s0 += he;
+                s1 = fmaf(he, (float)(k - kMid), s1);
+            });
+            const float x = r - kMuMid;
+            float coef = dfc * s0 - 2.0f * kEta * fc * fmaf(x, s0, -kDelta * s1);
             float mo[8];
             monomials(A[s], mo);
             float P[8];
@@ -166,7 +210,7 @@ __device__ inline void backward(const Nb (&nb)[kS], const float (&h)[kNF], float
 #pragma unroll
             for (int m = 0; m < 2; m++) {
                 const float d = r - nu(m);
-                const float g = __expf(-kEtaB * d * d);
+                const float g = exp2_raw(kEB2 * d * d);
                 const float psi = g * fc, dpsi = g * fmaf(-2.0f * kEtaB * d, fc, dfc);
                 float Hm = 0.0f;
 #pragma unroll
@@ -199,7 +243,7 @@ __device__ inline void backward(const Nb (&nb)[kS], const float (&h)[kNF], float
                 const float bs = Lam[s][0] + Lam[s][1] * c + Lam[s][2] * T2 + Lam[s][3] * T3;
                 const float bt = Lam[t][0] + Lam[t][1] * c + Lam[t][2] * T2 + Lam[t][3] * T3;
                 const float k = nb[t].fc * as + nb[s].fc * at;
-                const float ks = k / nb[s].r, kt = k / nb[t].r;
+                const float ks = k * nb[s].ir, kt = k * nb[t].ir;
                 const float es = nb[s].dfc * bt, et = nb[t].dfc * bs;
                 w[s][0] += ks * (nb[t].ux - c * nb[s].ux) + es * nb[s].ux;
                 w[s][1] += ks * (nb[t].uy - c * nb[s].uy) + es * nb[s].uy;
@@ -213,8 +257,8 @@ __device__ inline void backward(const Nb (&nb)[kS], const float (&h)[kNF], float
 }
 
 __device__ inline float tanh_f(float x) {
-    // 1 - 2 / (e^{2x} + 1): exact limits for large |x|
-    return 1.0f - 2.0f / (__expf(2.0f * x) + 1.0f);
+    // 1 - 2 / (e^{2x} + 1) with the hardware exp2 and reciprocal: limits +-1 for large |x|
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(exp2_raw(2.0f * kLog2e * x) + 1.0f);
 }
 
 // float -> bfloat16 bits, round to nearest, ties to even (finite inputs)

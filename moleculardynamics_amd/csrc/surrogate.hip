@@ -82,9 +82,9 @@ __device__ inline bf16x8 operand(const float (&a)[4][4], int ks) {
 // slot, FP32 terms (e_i accumulated in FP64), straight from the list in
 // memory (any count)
 __device__ double tersoff_centre(const Pos3& P, const Tp<float>& p, const int32_t* __restrict__ list, int32_t n,
-                                int32_t i, int32_t cnt, float* __restrict__ w) {
-    for (int32_t a = 0; a < cnt; a++)
-        for (int d = 0; d < 3; d++) w[(int64_t)(a * 3 + d) * n + i] = 0.0f;
+                                int32_t i, int32_t cnt, float4* __restrict__ w4) {
+    float* w = reinterpret_cast<float*>(w4);   // slot a of atom i at [4 (a n + i) + d]
+    for (int32_t a = 0; a < cnt; a++) w4[(int64_t)a * n + i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     double e = 0.0;
     for (int32_t a = 0; a < cnt; a++) {
         double dj[3];
@@ -133,18 +133,18 @@ __device__ double tersoff_centre(const Pos3& P, const Tp<float>& p, const int32_
                 const ZTerm<float> z = zterm(p, u, rij, v, rik);
                 for (int q = 0; q < 3; q++) {
                     gj[q] += de_z * (z.d_rij * u[q] + z.d_cs * (v[q] - z.cs * u[q]) / rij);
-                    w[(int64_t)(b * 3 + q) * n + i] += de_z * (z.d_rik * v[q] + z.d_cs * (u[q] - z.cs * v[q]) / rik);
+                    w[4 * ((int64_t)b * n + i) + q] += de_z * (z.d_rik * v[q] + z.d_cs * (u[q] - z.cs * v[q]) / rik);
                 }
             }
         }
-        for (int q = 0; q < 3; q++) w[(int64_t)(a * 3 + q) * n + i] += gj[q];
+        for (int q = 0; q < 3; q++) w[4 * ((int64_t)a * n + i) + q] += gj[q];
     }
     return e;
 }
 
 __global__ void __launch_bounds__(kBlock)
 k_sg_eval(Pos3 P, const int32_t* __restrict__ list, const int32_t* __restrict__ count, int32_t n,
-          const uint4* __restrict__ frag, const float* __restrict__ vec, Tp<float> tp, float* __restrict__ w,
+          const uint4* __restrict__ frag, const float* __restrict__ vec, Tp<float> tp, float4* __restrict__ w,
           double* __restrict__ e_atom, int32_t* __restrict__ nflag) {
     extern __shared__ __align__(16) unsigned char lds[];
     uint4* sfrag = reinterpret_cast<uint4*>(lds);
@@ -169,12 +169,14 @@ k_sg_eval(Pos3 P, const int32_t* __restrict__ list, const int32_t* __restrict__ 
     }
     float G[kNF];
     forward(nb, G);
+    bool out = false;   // a standardized value outside [lo, hi] (or NaN); no branches
 #pragma unroll
     for (int f = 0; f < kNF; f++) {
         const float gs = (G[f] - vec[kVMean + f]) * vec[kVIstd + f];
-        flag |= !(gs >= vec[kVLo + f] && gs <= vec[kVHi + f]);
+        out |= !(gs >= vec[kVLo + f]) | !(gs <= vec[kVHi + f]);
         G[f] = gs;
     }
+    flag |= out;
     if (flag || !live) {
 #pragma unroll
         for (int f = 0; f < kNF; f++) G[f] = 0.0f;
@@ -277,9 +279,7 @@ k_sg_eval(Pos3 P, const int32_t* __restrict__ list, const int32_t* __restrict__ 
             backward(nb, h, ws);
 #pragma unroll
             for (int s = 0; s < kS; s++)
-                if (s < cnt)
-#pragma unroll
-                    for (int d = 0; d < 3; d++) w[(int64_t)(s * 3 + d) * n + i] = ws[s][d];
+                if (s < cnt) w[(int64_t)s * n + i] = make_float4(ws[s][0], ws[s][1], ws[s][2], 0.0f);
         } else {
             e = tersoff_centre(P, tp, list, n, i, cnt, w);
         }
@@ -310,7 +310,7 @@ k_sg_rev(const int32_t* __restrict__ list, const int32_t* __restrict__ count, in
 
 __global__ void __launch_bounds__(kBlock)
 k_sg_gather(Pos3 P, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
-            const uint8_t* __restrict__ rev, int32_t n, const float* __restrict__ w, double* __restrict__ fx,
+            const uint8_t* __restrict__ rev, int32_t n, const float4* __restrict__ w, double* __restrict__ fx,
             double* __restrict__ fy, double* __restrict__ fz, const double* __restrict__ r0x,
             const double* __restrict__ r0y, const double* __restrict__ r0z, double hard2,
             unsigned long long* __restrict__ flags, int32_t spread) {
@@ -322,9 +322,10 @@ k_sg_gather(Pos3 P, const int32_t* __restrict__ list, const int32_t* __restrict_
         for (int32_t a = 0; a < cnt; a++) {
             const int32_t j = list[(int64_t)a * n + i];
             const int32_t b = rev[(int64_t)a * n + i];
-#pragma unroll
-            for (int d = 0; d < 3; d++)
-                F[d] += (double)w[(int64_t)(a * 3 + d) * n + i] - (double)w[(int64_t)(b * 3 + d) * n + j];
+            const float4 u = w[(int64_t)a * n + i], v = w[(int64_t)b * n + j];
+            F[0] += (double)u.x - (double)v.x;
+            F[1] += (double)u.y - (double)v.y;
+            F[2] += (double)u.z - (double)v.z;
         }
         fx[i] = F[0];
         fy[i] = F[1];
@@ -399,7 +400,7 @@ hipError_t launch_sg_rev(hipStream_t s, const int32_t* list, const int32_t* coun
 }
 
 hipError_t launch_sg_eval(hipStream_t s, const DevState& st, double L, const TersoffParams& p, const int32_t* list,
-                          const int32_t* count, int32_t n, const uint16_t* frag, const float* vec, float* w,
+                          const int32_t* count, int32_t n, const uint16_t* frag, const float* vec, float4* w,
                           double* e_atom, int32_t* nflag) {
     if (n <= 0) return hipSuccess;
     static bool attr = false;
@@ -415,7 +416,7 @@ hipError_t launch_sg_eval(hipStream_t s, const DevState& st, double L, const Ter
 
 hipError_t launch_sg_forces(hipStream_t s, const DevState& st, double L, const TersoffParams& p, const int32_t* list,
                             const int32_t* count, const uint8_t* rev, int32_t n, const uint16_t* frag,
-                            const float* vec, float* w, double* e_atom, int32_t* nflag, double* const rref[3],
+                            const float* vec, float4* w, double* e_atom, int32_t* nflag, double* const rref[3],
                             double hard2, uint64_t* flags, const DevState& out) {
     if (n <= 0) return hipSuccess;
     hipError_t e = launch_sg_eval(s, st, L, p, list, count, n, frag, vec, w, e_atom, nflag);

@@ -97,7 +97,7 @@ struct mdx_system {
     // learned surrogate (MDX_BOND_SURROGATE) over the same compact lists
     uint16_t* sg_frag = nullptr;     // BF16 MFMA weight fragments
     float* sg_vec = nullptr;         // biases, output weights, feature scaling and range
-    float* sg_w = nullptr;           // dE_i/d(x_j - x_i) per list slot, [slot][3][n]
+    float4* sg_w = nullptr;          // dE_i/d(x_j - x_i) per list slot, [slot][n] (x, y, z, 0)
     uint8_t* sg_rev = nullptr;       // slot of i in the list of its partner, [slot][n]
     int32_t* sg_nflag = nullptr;     // atoms evaluated by the Tersoff fallback (cumulative)
     bool sg_set = false;
@@ -459,7 +459,7 @@ int ts_build_list(mdx_system* s) {
             const size_t e = (size_t)s->ts_cap * n;
             if ((rc = dalloc(s, &s->ts_list, e))) return rc;
             if (is_surrogate(s)) {
-                if ((rc = dalloc(s, &s->sg_w, 3 * e)) || (rc = dalloc(s, &s->sg_rev, e))) return rc;
+                if ((rc = dalloc(s, &s->sg_w, e)) || (rc = dalloc(s, &s->sg_rev, e))) return rc;
             } else if ((rc = dalloc(s, &s->ts_dedr, e)) || (rc = dalloc(s, &s->ts_dedz, e))) {
                 return rc;
             }
