@@ -24,7 +24,7 @@ configuration.
 
 --workload replicas measures the batched configuration: 512 independent
 periodic LJ replicas of 256-8788 atoms (log-uniform sizes, density
-U[0.70, 0.95], T* U[0.5, 2.0]; 1.13M atoms together), rc 2.5, rs 2.8,
+U[0.70, 0.95], T* U[0.5, 2.0]; 1,234,468 atoms together), rc 2.5, rs 2.8,
 dt 0.005, mixed precision, advanced by graph replays.
 """
 from __future__ import annotations
@@ -55,9 +55,9 @@ STEP_BYTES_PER_ATOM = 312
 TRAFFIC_PROFILE = ROOT / "profiles" / "md_step_lj1m.md"
 
 
-def measured_traffic() -> int | None:
+def measured_traffic(profile: Path = TRAFFIC_PROFILE) -> int | None:
     try:
-        for line in TRAFFIC_PROFILE.read_text().splitlines():
+        for line in profile.read_text().splitlines():
             if line.startswith("traffic_bytes_per_launch:"):
                 return int(line.split(":", 1)[1])
     except OSError:
@@ -340,7 +340,8 @@ def bench_surrogate(a) -> None:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": None,
+            # copy-calibrated FETCH_SIZE + WRITE_SIZE per launch (profiles/sg_eval_si262k.md)
+            "traffic": measured_traffic(ROOT / "profiles" / "sg_eval_si262k.md"),
             "avg_launch_ms": k_ms,
             "bytes_per_atom": SG_EVAL_BYTES_PER_ATOM,
         },

@@ -35,7 +35,7 @@ def _db(path: str) -> sqlite3.Connection:
 
 
 def short(name: str) -> str:
-    n = name.replace("void ", "").replace("mdx::(anonymous namespace)::", "").replace("mdx::", "")
+    n = name.replace("void ", "").replace("mdx::", "").replace("(anonymous namespace)::", "")
     depth, out = 0, []
     for ch in n:   # drop the argument list, keep template arguments
         if ch == "(" and depth == 0:
