@@ -117,6 +117,22 @@ struct Pos {
         p[3 * i + 2] = z;
     }
 };
+// The list build's staging: FP32 positions relative to the segment's corner
+// (12 bytes an atom).  Within a segment's neighbourhood the coordinates stay
+// below ~30 sigma, so a distance is good to ~1e-5 sigma^2 in r^2; the build
+// tests against a radius that covers that, keeping the list a superset.
+struct PosF {
+    float* p;
+    int32_t cap;
+    __device__ float x(int32_t i) const { return p[3 * i]; }
+    __device__ float y(int32_t i) const { return p[3 * i + 1]; }
+    __device__ float z(int32_t i) const { return p[3 * i + 2]; }
+    __device__ void set(int32_t i, double x, double y, double z) const {
+        p[3 * i] = (float)x;
+        p[3 * i + 1] = (float)y;
+        p[3 * i + 2] = (float)z;
+    }
+};
 // LDS header shared by both kernels
 struct Stage {
     int32_t colbase[9];                 // LDS start of each column
@@ -128,11 +144,13 @@ struct Stage {
     unsigned long long red[3];          // step kernel: max disp^2 bits, max step^2 bits, stale count
 };
 
-// Stage the neighbourhood of segment (cx, cy, z0..z1).  Returns false (for
-// the whole block) when it does not fit in LDS.
-__device__ inline bool stage(Stage& S, const Pos& P, const SegGrid& g, int32_t cx, int32_t cy, int32_t z0,
+// Stage the neighbourhood of segment (cx, cy, z0..z1), positions minus the
+// origin o (zero for the FP64 staging: the values are then exactly the
+// images).  Returns false (for the whole block) when it does not fit in LDS.
+template <typename PosT>
+__device__ inline bool stage(Stage& S, const PosT& P, const SegGrid& g, int32_t cx, int32_t cy, int32_t z0,
                              int32_t z1, const double* __restrict__ rx, const double* __restrict__ ry,
-                             const double* __restrict__ rz) {
+                             const double* __restrict__ rz, double ox = 0.0, double oy = 0.0, double oz = 0.0) {
     const int32_t K2 = z1 - z0 + 2;    // cells per column in the window
     const int32_t tid = threadIdx.x;
     if (tid < 3) S.red[tid] = 0;
@@ -193,9 +211,9 @@ __device__ inline bool stage(Stage& S, const Pos& P, const SegGrid& g, int32_t c
                     if (w < l0) { src = s0 + w; sz = -g.L; }
                     else if (w < l0 + l1) { src = s1 + (w - l0); sz = 0.0; }
                     else { src = s2 + (w - l0 - l1); sz = g.L; }
-                    bx[u] = rx[src] + sx;
-                    by[u] = ry[src] + sy;
-                    bz[u] = rz[src] + sz;
+                    bx[u] = rx[src] + sx - ox;
+                    by[u] = ry[src] + sy - oy;
+                    bz[u] = rz[src] + sz - oz;
                 }
             }
 #pragma unroll
@@ -226,11 +244,12 @@ k_nlist_lds(SegGrid g, const double* __restrict__ rx, const double* __restrict__
     extern __shared__ double dyn[];
     int32_t cx, cy, z0, z1;
     if (seg_geometry(g, cx, cy, z0, z1) < 0) return;
-    const Pos P{dyn, lds_cap};
-    if (!stage(S, P, g, cx, cy, z0, z1, rx, ry, rz)) {
+    const PosF P{reinterpret_cast<float*>(dyn), lds_cap};
+    if (!stage(S, P, g, cx, cy, z0, z1, rx, ry, rz, cx * (g.L / g.n[0]), cy * (g.L / g.n[1]), z0 * (g.L / g.n[2]))) {
         if (threadIdx.x == 0) atomicMax(&err[0], S.total + 1);
         return;
     }
+    const float rlf = (float)(rl2 + 1e-4);   // superset of the FP64 test (see PosF)
     // largest staging (with the dummy atom), sizes the step kernel's LDS
     if (threadIdx.x == 0) atomicMax(&err[2], S.total + 1);
     const int32_t cbase = (cx * g.n[1] + cy) * g.n[2];
@@ -242,7 +261,7 @@ k_nlist_lds(SegGrid g, const double* __restrict__ rx, const double* __restrict__
         const int32_t qc = s - hbeg;
         const int32_t o = home_cell(S, qc);
         const int32_t self = S.colbase[4] + S.cellpre[4][1] + qc;
-        const double xi = P.x(self), yi = P.y(self), zi = P.z(self);
+        const float xi = P.x(self), yi = P.y(self), zi = P.z(self);
         // entry kk of atom s: lane kk % 4, that lane's entry jj = kk / 4, i.e.
         // chunk jj / 8 (= kk >> 5), place jj % 8 of the lane's 8
         uint16_t* const la = list + ((int64_t)(s / kListBlock) * nchunk * kListBlock + s % kListBlock) * (kLanes * kChunk);
@@ -258,9 +277,9 @@ k_nlist_lds(SegGrid g, const double* __restrict__ rx, const double* __restrict__
                 for (int u = 0; u < kScan; u++) {
                     const int32_t p = p0 + kLanes * u + q;
                     const int32_t pp = p < e ? p : b;
-                    const double dx = xi - P.x(pp), dy = yi - P.y(pp), dz = zi - P.z(pp);
-                    const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
-                    const bool hit = (p < e) & (p != self) & (r2 < rl2);
+                    const float dx = xi - P.x(pp), dy = yi - P.y(pp), dz = zi - P.z(pp);
+                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    const bool hit = (p < e) & (p != self) & (r2 < rlf);
                     mask |= (uint32_t)hit << u;
                 }
                 // hits are numbered lane-major within the quad
@@ -508,7 +527,7 @@ hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v
         attr_set = true;
     }
     SegGrid g = make_grid(v, nseg);
-    const size_t dyn = 3 * (size_t)lds_atoms * sizeof(double);
+    const size_t dyn = 3 * (size_t)lds_atoms * sizeof(float);
     k_nlist_lds<<<g.per_xcd * kXcd, kBlock, dyn, s>>>(g, st.r[0], st.r[1], st.r[2], rl2, n, cap, list, count, err,
                                                       lds_atoms);
     return hipGetLastError();
