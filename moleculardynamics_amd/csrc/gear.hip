@@ -168,7 +168,10 @@ template <bool kFinal, bool kGather>
 __global__ void __launch_bounds__(kBlock)
 k_correct_predict_track(DevState st, int32_t n, GearCoeffs gc, MassTable m, const double* __restrict__ r0x,
                         const double* __restrict__ r0y, const double* __restrict__ r0z,
-                        unsigned long long* __restrict__ flags, int32_t spread, SlotGather g, double hard2) {
+                        unsigned long long* __restrict__ flags, int32_t spread, SlotGather g, double hard2,
+                        unsigned long long* __restrict__ clear) {
+    if (blockIdx.x == 0 && clear)   // the next step's flag words
+        for (int32_t t = threadIdx.x; t < 3 * spread; t += blockDim.x) clear[t] = 0;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     double d2 = 0.0, s2 = 0.0;
     const double* r0[3] = {r0x, r0y, r0z};
@@ -243,30 +246,34 @@ k_correct_predict_track(DevState st, int32_t n, GearCoeffs gc, MassTable m, cons
 }  // namespace
 
 hipError_t launch_correct_predict_track(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
-                                        const MassTable& m, double* const rref[3], uint64_t* flags, bool final_step) {
+                                        const MassTable& m, double* const rref[3], uint64_t* flags, bool final_step,
+                                        uint64_t* clear_next) {
     if (n <= 0) return hipSuccess;
     auto* f = reinterpret_cast<unsigned long long*>(flags);
+    auto* c = reinterpret_cast<unsigned long long*>(clear_next);
     const SlotGather none{nullptr, nullptr, nullptr, nullptr};
     if (final_step)
         k_correct_predict_track<true, false><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(
-            st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), none, 0.0);
+            st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), none, 0.0, c);
     else
         k_correct_predict_track<false, false><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(
-            st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), none, 0.0);
+            st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), none, 0.0, c);
     return hipGetLastError();
 }
 
 hipError_t launch_gather_correct_predict_track(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
                                                const MassTable& m, double* const rref[3], uint64_t* flags,
-                                               bool final_step, const SlotGather& g, double hard2) {
+                                               bool final_step, const SlotGather& g, double hard2,
+                                               uint64_t* clear_next) {
     if (n <= 0) return hipSuccess;
     auto* f = reinterpret_cast<unsigned long long*>(flags);
+    auto* c = reinterpret_cast<unsigned long long*>(clear_next);
     if (final_step)
         k_correct_predict_track<true, true><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(
-            st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), g, hard2);
+            st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), g, hard2, c);
     else
         k_correct_predict_track<false, true><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(
-            st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), g, hard2);
+            st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), g, hard2, c);
     return hipGetLastError();
 }
 

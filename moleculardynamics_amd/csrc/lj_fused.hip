@@ -323,6 +323,7 @@ struct StepArgs {
     GearCoeffs gc;
     double hard2;                // list validity: displacement^2 must stay below this
     unsigned long long* flags;   // kSpread x {max disp^2 bits, max step^2 bits, stale count}
+    unsigned long long* clear;   // the next step's flag words, zeroed by block 0 (may be null)
     int64_t cs;                  // component stride: component d of a vector at [0] + d * cs
     int32_t n;
     int32_t lds_cap;             // staged atoms the dynamic LDS holds
@@ -338,6 +339,8 @@ k_step_lds(StepArgs a) {
     __shared__ Stage S;
     extern __shared__ double dyn[];
     const SegGrid& g = a.g;
+    if (blockIdx.x == 0 && a.clear)
+        for (int32_t t = threadIdx.x; t < 3 * kSpread; t += kBlock) a.clear[t] = 0;
     int32_t cx, cy, z0, z1;
     if (seg_geometry(g, cx, cy, z0, z1) < 0) return;
     const Pos P{dyn, a.lds_cap};
@@ -537,7 +540,7 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
                            double* const rn[3], double* const r0[3], const uint16_t* list, const int32_t* count,
                            int32_t cap, const LjTable& t, const MassTable& m, const GearCoeffs& gc, double hard2, int32_t n,
                            int64_t cs, int32_t precision, bool final_step, bool force_out, uint64_t* flags,
-                           int32_t lds_atoms) {
+                           int32_t lds_atoms, uint64_t* clear_next) {
     if (n <= 0) return hipSuccess;
     static bool attr_set = false;
     if (!attr_set) {
@@ -572,6 +575,7 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
     a.gc = gc;
     a.hard2 = hard2;
     a.flags = reinterpret_cast<unsigned long long*>(flags);
+    a.clear = reinterpret_cast<unsigned long long*>(clear_next);
     a.cs = cs;
     for (int d = 1; d < 3; d++)
         if (st.v[d] != st.v[0] + d * cs || st.r[d] != st.r[0] + d * cs || rn[d] != rn[0] + d * cs ||

@@ -169,8 +169,10 @@ hipError_t launch_predict_track(hipStream_t s, const DevState& st, int32_t n, co
                                 double* const rref[3], uint64_t* flags);
 // corrector of step t and (unless final_step) predictor of step t+1 with the
 // same flag tracking, in place (forces computed by a separate kernel)
+// clear_next (may be null): the next step's flag words, zeroed by the kernel
 hipError_t launch_correct_predict_track(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
-                                        const MassTable& m, double* const rref[3], uint64_t* flags, bool final_step);
+                                        const MassTable& m, double* const rref[3], uint64_t* flags, bool final_step,
+                                        uint64_t* clear_next);
 // Forces from per-slot gradients (the surrogate): F_i = sum_s w[s][d][i] -
 // w[rev[s][i]][d][list[s][i]] in FP64, written to st.f.
 struct SlotGather {
@@ -183,7 +185,8 @@ struct SlotGather {
 // then launch_correct_predict_track's corrector and predictor, in one kernel
 hipError_t launch_gather_correct_predict_track(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
                                                const MassTable& m, double* const rref[3], uint64_t* flags,
-                                               bool final_step, const SlotGather& g, double hard2);
+                                               bool final_step, const SlotGather& g, double hard2,
+                                               uint64_t* clear_next);
 
 // lj_fused.hip: LDS-staged segment kernels of the periodic box
 int32_t lds_segments_per_column(int32_t nz, double atoms_per_cell, int32_t kmax);
@@ -197,12 +200,12 @@ hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v
                             int32_t n, int32_t cap, uint16_t* list, int32_t* count, int32_t* err,
                             int32_t lds_atoms);
 // st, rn and r0 hold each vector's three components in one block, component d
-// at [0] + d * cs
+// at [0] + d * cs; clear_next (may be null): the next step's flag words, zeroed
 hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg,
                            double* const rn[3], double* const r0[3], const uint16_t* list, const int32_t* count,
                            int32_t cap, const LjTable& t, const MassTable& m, const GearCoeffs& gc, double hard2,
                            int32_t n, int64_t cs, int32_t precision, bool final_step, bool force_out, uint64_t* flags,
-                           int32_t lds_atoms);
+                           int32_t lds_atoms, uint64_t* clear_next);
 
 // tersoff.hip: Tersoff silicon in the periodic box (compact [k][slot] lists)
 TersoffParams tersoff_si_params();

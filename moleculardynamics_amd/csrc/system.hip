@@ -714,7 +714,11 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
         return MDX_OK;
     };
     // flags of r(0) come from the first predictor (index -1)
+    // the predictor's slot and step 0's slot start cleared; from then on each
+    // step's last kernel clears the next step's slot (its host copy, two steps
+    // back in the ring, was enqueued before)
     HIPCHK(s, hipMemsetAsync(slot_flags(-1), 0, W * sizeof(uint64_t), st));
+    HIPCHK(s, hipMemsetAsync(slot_flags(0), 0, W * sizeof(uint64_t), st));
     LAUNCH(s, "predict", launch_predict_track(st, s->st, n, s->gc, s->rref, slot_flags(-1)));
     if ((rc = post(-1))) return rc;
     int last_build = -1;   // step index whose positions were used for the last build
@@ -742,7 +746,6 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
             last_build = k;
         }
         const bool fin = k + 1 == nsteps;
-        HIPCHK(s, hipMemsetAsync(slot_flags(k), 0, W * sizeof(uint64_t), st));
         if (ts) {
             if (is_surrogate(s)) {
                 LAUNCH(s, "sg_eval", launch_sg_eval(st, s->st, c.box_length, s->tp, s->ts_list, s->ts_count, n,
@@ -750,7 +753,8 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
                 s->sg_evals++;
                 const SlotGather g{s->ts_list, s->ts_count, s->sg_rev, s->sg_w};
                 LAUNCH(s, "gather_gear_step", launch_gather_correct_predict_track(st, s->st, n, s->gc, s->mt, s->rref,
-                                                                                  slot_flags(k), fin, g, hard2));
+                                                                                  slot_flags(k), fin, g, hard2,
+                                                                                  slot_flags(k + 1)));
                 if ((rc = post(k))) return rc;
                 s->t++;
                 continue;
@@ -759,14 +763,16 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
                                                         s->ts_count, n, s->ts_dedr, s->ts_dedz, nullptr, s->rref,
                                                         hard2, slot_flags(k), s->st));
             }
-            LAUNCH(s, "gear_step", launch_correct_predict_track(st, s->st, n, s->gc, s->mt, s->rref, slot_flags(k), fin));
+            LAUNCH(s, "gear_step", launch_correct_predict_track(st, s->st, n, s->gc, s->mt, s->rref, slot_flags(k), fin,
+                                                                slot_flags(k + 1)));
             if ((rc = post(k))) return rc;
             s->t++;
             continue;
         }
         LAUNCH(s, "md_step", launch_step_lds(st, s->st, s->lj.view, s->nseg, s->rnext, s->rref, s->nlist, s->ncount, s->nl_cap,
                                              s->ljt, s->mt, s->gc, hard2, n, component_stride(n), c.precision, fin,
-                                             fin, slot_flags(k), s->lds_atoms));
+                                             fin, slot_flags(k), s->lds_atoms,
+                                             slot_flags(k + 1)));
         if ((rc = post(k))) return rc;
         for (int d = 0; d < 3; d++) std::swap(s->st.r[d], s->rnext[d]);
         s->t++;
