@@ -183,10 +183,17 @@ k_correct_predict_track(DevState st, int32_t n, GearCoeffs gc, MassTable m, cons
             for (int32_t s = 0; s < cnt; s++) {
                 const int32_t j = g.list[(int64_t)s * n + i];
                 const int32_t b = g.rev[(int64_t)s * n + i];
-                const float4 u = g.w[(int64_t)s * n + i], v = g.w[(int64_t)b * n + j];
-                F[0] += (double)u.x - (double)v.x;
-                F[1] += (double)u.y - (double)v.y;
-                F[2] += (double)u.z - (double)v.z;
+                if (g.wd) {
+                    const double4 u = g.wd[(int64_t)s * n + i], v = g.wd[(int64_t)b * n + j];
+                    F[0] += u.x - v.x;
+                    F[1] += u.y - v.y;
+                    F[2] += u.z - v.z;
+                } else {
+                    const float4 u = g.w[(int64_t)s * n + i], v = g.w[(int64_t)b * n + j];
+                    F[0] += (double)u.x - (double)v.x;
+                    F[1] += (double)u.y - (double)v.y;
+                    F[2] += (double)u.z - (double)v.z;
+                }
             }
             double e2 = 0.0;
 #pragma unroll
@@ -251,7 +258,7 @@ hipError_t launch_correct_predict_track(hipStream_t s, const DevState& st, int32
     if (n <= 0) return hipSuccess;
     auto* f = reinterpret_cast<unsigned long long*>(flags);
     auto* c = reinterpret_cast<unsigned long long*>(clear_next);
-    const SlotGather none{nullptr, nullptr, nullptr, nullptr};
+    const SlotGather none{nullptr, nullptr, nullptr, nullptr, nullptr};
     if (final_step)
         k_correct_predict_track<true, false><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(
             st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), none, 0.0, c);

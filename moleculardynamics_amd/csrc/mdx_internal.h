@@ -179,7 +179,8 @@ struct SlotGather {
     const int32_t* list;
     const int32_t* count;
     const uint8_t* rev;
-    const float4* w;   // [slot][atom], x y z used
+    const float4* w;    // [slot][atom], x y z used
+    const double4* wd;  // the same in FP64 (used instead of w when non-null)
 };
 // the gather, the stale count of st's positions against rref (sqrt(hard2)),
 // then launch_correct_predict_track's corrector and predictor, in one kernel
@@ -212,13 +213,11 @@ TersoffParams tersoff_si_params();
 // list of the atoms within rs of each slot; err: longest list when above cap
 hipError_t launch_ts_list(hipStream_t s, const DevState& st, const int32_t* cell_of, const GridView& g, double rs,
                           int32_t n, int32_t cap, int32_t* list, int32_t* count, int32_t* err);
-// forces of st's positions into out.f; per-atom energy into e_atom (may be
-// null); with rref, atoms farther than sqrt(hard2) from it are counted into
-// the spread flags as stale
-hipError_t launch_ts_forces(hipStream_t s, const DevState& st, double L, const TersoffParams& p, int32_t precision,
-                            const int32_t* list, const int32_t* count, int32_t n, double* dedr, double* dedz,
-                            double* e_atom, double* const rref[3], double hard2, uint64_t* flags,
-                            const DevState& out);
+// e_i and the per-slot vectors w_is = de_i/d(x_j - x_i) of every atom: FP32
+// terms into w (MDX_PREC_MIXED) or FP64 into wd (MDX_PREC_FP64); e_atom may be null
+hipError_t launch_ts_atoms(hipStream_t s, const DevState& st, double L, const TersoffParams& p, int32_t precision,
+                           const int32_t* list, const int32_t* count, int32_t n, float4* w, double4* wd,
+                           double* e_atom);
 
 // surrogate.hip: learned surrogate of Tersoff silicon over the same lists
 // FP32 blob (MDX_SG_NPARAM) -> BF16 MFMA fragments and FP32 vectors
@@ -231,7 +230,12 @@ hipError_t launch_sg_rev(hipStream_t s, const int32_t* list, const int32_t* coun
                          int32_t* err);
 // forces of st's positions into out.f (and per-atom energies into e_atom if
 // non-null); nflag counts the atoms that took the Tersoff fallback; stale
-// atoms against rref are counted into the spread flags as in launch_ts_forces
+// atoms against rref are counted into the spread flags
+// F = the gather of per-slot vectors (w, or wd when non-null) into out.f;
+// atoms farther than sqrt(hard2) from rref are counted as stale into flags
+hipError_t launch_slot_gather(hipStream_t s, const DevState& st, double L, const int32_t* list, const int32_t* count,
+                              const uint8_t* rev, int32_t n, const float4* w, const double4* wd,
+                              double* const rref[3], double hard2, uint64_t* flags, const DevState& out);
 // descriptors, network and per-slot gradients only (w), for the fused step
 hipError_t launch_sg_eval(hipStream_t s, const DevState& st, double L, const TersoffParams& p, const int32_t* list,
                           const int32_t* count, int32_t n, const uint16_t* frag, const float* vec, float4* w,

@@ -78,70 +78,6 @@ __device__ inline bf16x8 operand(const float (&a)[4][4], int ks) {
     return as_bf16x8(v);
 }
 
-// Tersoff per-atom term of a flagged centre i: e_i and w_is for every list
-// slot, FP32 terms (e_i accumulated in FP64), straight from the list in
-// memory (any count)
-__device__ double tersoff_centre(const Pos3& P, const Tp<float>& p, const int32_t* __restrict__ list, int32_t n,
-                                int32_t i, int32_t cnt, float4* __restrict__ w4) {
-    float* w = reinterpret_cast<float*>(w4);   // slot a of atom i at [4 (a n + i) + d]
-    for (int32_t a = 0; a < cnt; a++) w4[(int64_t)a * n + i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    double e = 0.0;
-    for (int32_t a = 0; a < cnt; a++) {
-        double dj[3];
-        P.d(i, list[(int64_t)a * n + i], dj);
-        float rij, u[3];
-        unit(dj, rij, u);
-        if (!(rij < p.rc)) continue;
-        const double rj = sqrt(dj[0] * dj[0] + dj[1] * dj[1] + dj[2] * dj[2]);
-        float zeta = 0.0f;
-        for (int32_t b = 0; b < cnt; b++) {
-            if (b == a) continue;
-            double dk[3];
-            P.d(i, list[(int64_t)b * n + i], dk);
-            const double rk = sqrt(dk[0] * dk[0] + dk[1] * dk[1] + dk[2] * dk[2]);
-            const float rik = (float)rk;
-            if (!(rik < p.rc)) continue;
-            float fk, dfk, gg, dg, ee, dee;
-            fcut(p, rik, fk, dfk);
-            // the cosine from the FP64 displacements, as the plain Tersoff bond pass takes it
-            gang(p, (float)((dj[0] * dk[0] + dj[1] * dk[1] + dj[2] * dk[2]) / (rj * rk)), gg, dg);
-            ex3(p, rij - rik, ee, dee);
-            zeta += fk * gg * ee;
-        }
-        float fc, dfc;
-        fcut(p, rij, fc, dfc);
-        const float fr = p.A * expf(-p.l1 * rij), dfr = -p.l1 * fr;
-        const float fa = -p.B * expf(-p.l2 * rij), dfa = -p.l2 * fa;
-        float bz = 1.0f, dbz = 0.0f;
-        if (zeta > 0.0f) {
-            const float t = powf(p.beta * zeta, p.n);
-            bz = powf(1.0f + t, -0.5f / p.n);
-            dbz = -0.5f * bz / (1.0f + t) * t / zeta;
-        }
-        e += 0.5 * (double)fc * ((double)fr + (double)bz * (double)fa);
-        const float de_r = 0.5f * (dfc * (fr + bz * fa) + fc * (dfr + bz * dfa));
-        const float de_z = 0.5f * fc * fa * dbz;
-        float gj[3] = {de_r * u[0], de_r * u[1], de_r * u[2]};
-        if (de_z != 0.0f) {
-            for (int32_t b = 0; b < cnt; b++) {
-                if (b == a) continue;
-                double dk[3];
-                P.d(i, list[(int64_t)b * n + i], dk);
-                float rik, v[3];
-                unit(dk, rik, v);
-                if (!(rik < p.rc)) continue;
-                const ZTerm<float> z = zterm(p, u, rij, v, rik);
-                for (int q = 0; q < 3; q++) {
-                    gj[q] += de_z * (z.d_rij * u[q] + z.d_cs * (v[q] - z.cs * u[q]) / rij);
-                    w[4 * ((int64_t)b * n + i) + q] += de_z * (z.d_rik * v[q] + z.d_cs * (u[q] - z.cs * v[q]) / rik);
-                }
-            }
-        }
-        for (int q = 0; q < 3; q++) w[4 * ((int64_t)a * n + i) + q] += gj[q];
-    }
-    return e;
-}
-
 __global__ void __launch_bounds__(kBlock)
 k_sg_eval(Pos3 P, const int32_t* __restrict__ list, const int32_t* __restrict__ count, int32_t n,
           const uint4* __restrict__ frag, const float* __restrict__ vec, Tp<float> tp, float4* __restrict__ w,
@@ -280,8 +216,14 @@ k_sg_eval(Pos3 P, const int32_t* __restrict__ list, const int32_t* __restrict__ 
 #pragma unroll
             for (int s = 0; s < kS; s++)
                 if (s < cnt) w[(int64_t)s * n + i] = make_float4(ws[s][0], ws[s][1], ws[s][2], 0.0f);
+        } else if (cnt <= kS) {   // plain Tersoff for this centre (tersoff_terms.h)
+            float wr[kS][3];
+            e = centre_reg<float, kS>(P, tp, list, n, i, cnt, wr);
+#pragma unroll
+            for (int s = 0; s < kS; s++)
+                if (s < cnt) w[(int64_t)s * n + i] = make_float4(wr[s][0], wr[s][1], wr[s][2], 0.0f);
         } else {
-            e = tersoff_centre(P, tp, list, n, i, cnt, w);
+            e = centre_mem<float>(P, tp, list, n, i, cnt, w);
         }
         if (e_atom) e_atom[i] = e;
     }
@@ -308,9 +250,11 @@ k_sg_rev(const int32_t* __restrict__ list, const int32_t* __restrict__ count, in
     }
 }
 
+// forces from per-slot vectors: FP32 (w) or FP64 (wd, when non-null)
 __global__ void __launch_bounds__(kBlock)
 k_sg_gather(Pos3 P, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
-            const uint8_t* __restrict__ rev, int32_t n, const float4* __restrict__ w, double* __restrict__ fx,
+            const uint8_t* __restrict__ rev, int32_t n, const float4* __restrict__ w,
+            const double4* __restrict__ wd, double* __restrict__ fx,
             double* __restrict__ fy, double* __restrict__ fz, const double* __restrict__ r0x,
             const double* __restrict__ r0y, const double* __restrict__ r0z, double hard2,
             unsigned long long* __restrict__ flags, int32_t spread) {
@@ -322,10 +266,17 @@ k_sg_gather(Pos3 P, const int32_t* __restrict__ list, const int32_t* __restrict_
         for (int32_t a = 0; a < cnt; a++) {
             const int32_t j = list[(int64_t)a * n + i];
             const int32_t b = rev[(int64_t)a * n + i];
-            const float4 u = w[(int64_t)a * n + i], v = w[(int64_t)b * n + j];
-            F[0] += (double)u.x - (double)v.x;
-            F[1] += (double)u.y - (double)v.y;
-            F[2] += (double)u.z - (double)v.z;
+            if (wd) {
+                const double4 u = wd[(int64_t)a * n + i], v = wd[(int64_t)b * n + j];
+                F[0] += u.x - v.x;
+                F[1] += u.y - v.y;
+                F[2] += u.z - v.z;
+            } else {
+                const float4 u = w[(int64_t)a * n + i], v = w[(int64_t)b * n + j];
+                F[0] += (double)u.x - (double)v.x;
+                F[1] += (double)u.y - (double)v.y;
+                F[2] += (double)u.z - (double)v.z;
+            }
         }
         fx[i] = F[0];
         fy[i] = F[1];
@@ -421,11 +372,17 @@ hipError_t launch_sg_forces(hipStream_t s, const DevState& st, double L, const T
     if (n <= 0) return hipSuccess;
     hipError_t e = launch_sg_eval(s, st, L, p, list, count, n, frag, vec, w, e_atom, nflag);
     if (e != hipSuccess) return e;
-    const Pos3 P = pos3(st, L);
+    return launch_slot_gather(s, st, L, list, count, rev, n, w, nullptr, rref, hard2, flags, out);
+}
+
+hipError_t launch_slot_gather(hipStream_t s, const DevState& st, double L, const int32_t* list, const int32_t* count,
+                              const uint8_t* rev, int32_t n, const float4* w, const double4* wd,
+                              double* const rref[3], double hard2, uint64_t* flags, const DevState& out) {
+    if (n <= 0) return hipSuccess;
     const double* r0[3] = {rref ? rref[0] : nullptr, rref ? rref[1] : nullptr, rref ? rref[2] : nullptr};
-    k_sg_gather<<<grid_for(n), kBlock, 0, s>>>(P, list, count, rev, n, w, out.f[0], out.f[1], out.f[2], r0[0], r0[1],
-                                               r0[2], hard2, reinterpret_cast<unsigned long long*>(flags),
-                                               flag_spread());
+    k_sg_gather<<<grid_for(n), kBlock, 0, s>>>(pos3(st, L), list, count, rev, n, w, wd, out.f[0], out.f[1], out.f[2],
+                                               r0[0], r0[1], r0[2], hard2,
+                                               reinterpret_cast<unsigned long long*>(flags), flag_spread());
     return hipGetLastError();
 }
 

@@ -90,14 +90,13 @@ struct mdx_system {
     TersoffParams tp{};
     int32_t* ts_list = nullptr;
     int32_t* ts_count = nullptr;
-    double* ts_dedr = nullptr;       // dE/dr_ij at fixed zeta, per bond
-    double* ts_dedz = nullptr;       // dE/dzeta_ij, per bond
+    double4* ts_wd = nullptr;        // FP64 Tersoff: de_i/d(x_j - x_i) per list slot, [slot][n]
     int32_t ts_cap = 0;
     bool ts_built = false;
     // learned surrogate (MDX_BOND_SURROGATE) over the same compact lists
     uint16_t* sg_frag = nullptr;     // BF16 MFMA weight fragments
     float* sg_vec = nullptr;         // biases, output weights, feature scaling and range
-    float4* sg_w = nullptr;          // dE_i/d(x_j - x_i) per list slot, [slot][n] (x, y, z, 0)
+    float4* sg_w = nullptr;          // dE_i/d(x_j - x_i) per list slot, [slot][n] (x, y, z, 0); mixed Tersoff too
     uint8_t* sg_rev = nullptr;       // slot of i in the list of its partner, [slot][n]
     int32_t* sg_nflag = nullptr;     // atoms evaluated by the Tersoff fallback (cumulative)
     bool sg_set = false;
@@ -448,6 +447,8 @@ int32_t next_build(float rs, float rc, double vmax, float dt, int32_t t) {
 
 bool is_tersoff(const mdx_system* s) { return s->cfg.box == MDX_BOX_PERIODIC && s->cfg.use_bond; }
 bool is_surrogate(const mdx_system* s) { return is_tersoff(s) && s->cfg.bond_form == MDX_BOND_SURROGATE; }
+// plain Tersoff in FP64 keeps FP64 per-slot vectors; every other Tersoff-family path FP32
+bool ts_wide(const mdx_system* s) { return !is_surrogate(s) && s->cfg.precision == MDX_PREC_FP64; }
 
 // Tersoff compact lists over the primary grid (cells of edge mb_rs); grows the
 // capacity and builds again when a list overflows
@@ -457,10 +458,10 @@ int ts_build_list(mdx_system* s) {
         if (!s->ts_list) {
             int rc;
             const size_t e = (size_t)s->ts_cap * n;
-            if ((rc = dalloc(s, &s->ts_list, e))) return rc;
-            if (is_surrogate(s)) {
-                if ((rc = dalloc(s, &s->sg_w, e)) || (rc = dalloc(s, &s->sg_rev, e))) return rc;
-            } else if ((rc = dalloc(s, &s->ts_dedr, e)) || (rc = dalloc(s, &s->ts_dedz, e))) {
+            if ((rc = dalloc(s, &s->ts_list, e)) || (rc = dalloc(s, &s->sg_rev, e))) return rc;
+            if (ts_wide(s)) {
+                if ((rc = dalloc(s, &s->ts_wd, e))) return rc;
+            } else if ((rc = dalloc(s, &s->sg_w, e))) {
                 return rc;
             }
         }
@@ -472,18 +473,17 @@ int ts_build_list(mdx_system* s) {
         if (rc) return rc;
         if (longest == 0) break;
         dfree(s, s->ts_list);
-        dfree(s, s->ts_dedr);
-        dfree(s, s->ts_dedz);
+        dfree(s, s->ts_wd);
         dfree(s, s->sg_w);
         dfree(s, s->sg_rev);
         s->ts_list = nullptr;
-        s->ts_dedr = s->ts_dedz = nullptr;
+        s->ts_wd = nullptr;
         s->sg_w = nullptr;
         s->sg_rev = nullptr;
         s->ts_cap = longest + longest / 4 + 4;
     }
-    if (is_surrogate(s)) {
-        if (s->ts_cap > 255) return fail(s, MDX_ERR_CAPACITY, "surrogate lists hold at most 255 neighbours");
+    {   // the slot of each atom in its partners' lists (for the force gather)
+        if (s->ts_cap > 255) return fail(s, MDX_ERR_CAPACITY, "Tersoff lists hold at most 255 neighbours");
         HIPCHK(s, hipMemsetAsync(s->dint + 4, 0, sizeof(int32_t), s->stream));
         LAUNCH(s, "sg_rev", launch_sg_rev(s->stream, s->ts_list, s->ts_count, n, s->sg_rev, s->dint + 4));
         int32_t asym;
@@ -541,9 +541,10 @@ int bond_forces(mdx_system* s, double* e_atom, const int32_t* gate, const DevSta
     if (is_tersoff(s)) {
         if (!s->ts_built) return fail(s, MDX_ERR_STATE, "Tersoff list not built");
         const DevState& d = target ? *target : s->st;
-        LAUNCH(s, "ts_forces", launch_ts_forces(s->stream, s->st, c.box_length, s->tp, c.precision, s->ts_list,
-                                                s->ts_count, s->n, s->ts_dedr, s->ts_dedz, e_atom, nullptr, 0.0,
-                                                nullptr, d));
+        LAUNCH(s, "ts_atoms", launch_ts_atoms(s->stream, s->st, c.box_length, s->tp, c.precision, s->ts_list,
+                                              s->ts_count, s->n, s->sg_w, s->ts_wd, e_atom));
+        LAUNCH(s, "ts_gather", launch_slot_gather(s->stream, s->st, c.box_length, s->ts_list, s->ts_count, s->sg_rev,
+                                                  s->n, s->sg_w, s->ts_wd, nullptr, 0.0, nullptr, d));
         return MDX_OK;
     }
     if (c.box == MDX_BOX_PERIODIC || c.bond_form != MDX_BOND_REFERENCE)
@@ -751,20 +752,20 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
                 LAUNCH(s, "sg_eval", launch_sg_eval(st, s->st, c.box_length, s->tp, s->ts_list, s->ts_count, n,
                                                     s->sg_frag, s->sg_vec, s->sg_w, nullptr, s->sg_nflag));
                 s->sg_evals++;
-                const SlotGather g{s->ts_list, s->ts_count, s->sg_rev, s->sg_w};
+                const SlotGather g{s->ts_list, s->ts_count, s->sg_rev, s->sg_w, nullptr};
                 LAUNCH(s, "gather_gear_step", launch_gather_correct_predict_track(st, s->st, n, s->gc, s->mt, s->rref,
                                                                                   slot_flags(k), fin, g, hard2,
                                                                                   slot_flags(k + 1)));
                 if ((rc = post(k))) return rc;
                 s->t++;
                 continue;
-            } else {
-                LAUNCH(s, "ts_forces", launch_ts_forces(st, s->st, c.box_length, s->tp, c.precision, s->ts_list,
-                                                        s->ts_count, n, s->ts_dedr, s->ts_dedz, nullptr, s->rref,
-                                                        hard2, slot_flags(k), s->st));
             }
-            LAUNCH(s, "gear_step", launch_correct_predict_track(st, s->st, n, s->gc, s->mt, s->rref, slot_flags(k), fin,
-                                                                slot_flags(k + 1)));
+            LAUNCH(s, "ts_atoms", launch_ts_atoms(st, s->st, c.box_length, s->tp, c.precision, s->ts_list, s->ts_count,
+                                                  n, s->sg_w, s->ts_wd, nullptr));
+            const SlotGather g{s->ts_list, s->ts_count, s->sg_rev, s->sg_w, s->ts_wd};
+            LAUNCH(s, "gather_gear_step", launch_gather_correct_predict_track(st, s->st, n, s->gc, s->mt, s->rref,
+                                                                              slot_flags(k), fin, g, hard2,
+                                                                              slot_flags(k + 1)));
             if ((rc = post(k))) return rc;
             s->t++;
             continue;

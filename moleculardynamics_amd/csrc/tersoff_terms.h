@@ -115,5 +115,135 @@ __device__ inline void unit(const double d[3], T& r, T u[3]) {
     for (int q = 0; q < 3; q++) u[q] = (T)(d[q] / rr);
 }
 
+
+// ---- one centre's terms: e_i = 1/2 sum_j V_ij and w_is = de_i/d(x_j - x_i) ----
+// e_i depends only on the displacements from i to its neighbours, so its
+// gradient is fully described by one vector per list slot; the force on atom
+// a is then sum_s w_as - sum_{(i, s) -> a} w_is (a gather, no atomics).
+
+// pair and bond-order part of bond (centre, partner at distance rij): returns
+// the bond's energy and dE/dr at fixed zeta (de_r) and dE/dzeta (de_z)
+template <typename T>
+__device__ inline double bond_terms(const Tp<T>& p, T rij, T zeta, T& de_r, T& de_z) {
+    T fc, dfc;
+    fcut(p, rij, fc, dfc);
+    const T fr = p.A * t_exp<T>(-p.l1 * rij), dfr = -p.l1 * fr;
+    const T fa = -p.B * t_exp<T>(-p.l2 * rij), dfa = -p.l2 * fa;
+    T bz = 1, dbz = 0;
+    if (zeta > 0) {
+        const T t = t_pow<T>(p.beta * zeta, p.n);
+        bz = t_pow<T>(1 + t, (T)-0.5 / p.n);
+        dbz = (T)-0.5 * bz / (1 + t) * t / zeta;
+    }
+    de_r = (T)0.5 * (dfc * (fr + bz * fa) + fc * (dfr + bz * dfa));
+    de_z = (T)0.5 * fc * fa * dbz;
+    return 0.5 * (double)fc * ((double)fr + (double)bz * (double)fa);
+}
+
+// Up to kS list entries held in registers; w[s] for s < cnt.
+template <typename T, int kS>
+__device__ inline double centre_reg(const Pos3& P, const Tp<T>& p, const int32_t* __restrict__ list, int32_t n,
+                                    int32_t i, int32_t cnt, T (&w)[kS][3]) {
+    T r[kS], u[kS][3];
+    bool act[kS];
+#pragma unroll
+    for (int s = 0; s < kS; s++) {
+        w[s][0] = w[s][1] = w[s][2] = 0;
+        act[s] = false;
+        r[s] = 0;
+        u[s][0] = u[s][1] = u[s][2] = 0;
+        if (s < cnt) {
+            double d[3];
+            P.d(i, list[(int64_t)s * n + i], d);
+            unit(d, r[s], u[s]);
+            act[s] = r[s] < p.rc;
+        }
+    }
+    double e = 0.0;
+#pragma unroll
+    for (int a = 0; a < kS; a++) {
+        if (!act[a]) continue;
+        T zeta = 0;
+#pragma unroll
+        for (int b = 0; b < kS; b++) {
+            if (b == a || !act[b]) continue;
+            T fk, dfk, gg, dg, ee, dee;
+            fcut(p, r[b], fk, dfk);
+            gang(p, u[a][0] * u[b][0] + u[a][1] * u[b][1] + u[a][2] * u[b][2], gg, dg);
+            ex3(p, r[a] - r[b], ee, dee);
+            zeta += fk * gg * ee;
+        }
+        T de_r, de_z;
+        e += bond_terms(p, r[a], zeta, de_r, de_z);
+#pragma unroll
+        for (int q = 0; q < 3; q++) w[a][q] += de_r * u[a][q];
+        if (de_z != 0) {
+#pragma unroll
+            for (int b = 0; b < kS; b++) {
+                if (b == a || !act[b]) continue;
+                const ZTerm<T> z = zterm(p, u[a], r[a], u[b], r[b]);
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    w[a][q] += de_z * (z.d_rij * u[a][q] + z.d_cs * (u[b][q] - z.cs * u[a][q]) / r[a]);
+                    w[b][q] += de_z * (z.d_rik * u[b][q] + z.d_cs * (u[a][q] - z.cs * u[b][q]) / r[b]);
+                }
+            }
+        }
+    }
+    return e;
+}
+
+// Any count: the list read from memory, w accumulated in place ([slot][n]
+// vectors of 4 T, x y z used).
+template <typename T, typename W4>
+__device__ inline double centre_mem(const Pos3& P, const Tp<T>& p, const int32_t* __restrict__ list, int32_t n,
+                                    int32_t i, int32_t cnt, W4* __restrict__ w4) {
+    T* w = reinterpret_cast<T*>(w4);
+    for (int32_t a = 0; a < cnt; a++)
+        for (int q = 0; q < 4; q++) w[4 * ((int64_t)a * n + i) + q] = 0;
+    double e = 0.0;
+    for (int32_t a = 0; a < cnt; a++) {
+        double dj[3];
+        P.d(i, list[(int64_t)a * n + i], dj);
+        T rij, u[3];
+        unit(dj, rij, u);
+        if (!(rij < p.rc)) continue;
+        T zeta = 0;
+        for (int32_t b = 0; b < cnt; b++) {
+            if (b == a) continue;
+            double dk[3];
+            P.d(i, list[(int64_t)b * n + i], dk);
+            T rik, v[3];
+            unit(dk, rik, v);
+            if (!(rik < p.rc)) continue;
+            T fk, dfk, gg, dg, ee, dee;
+            fcut(p, rik, fk, dfk);
+            gang(p, u[0] * v[0] + u[1] * v[1] + u[2] * v[2], gg, dg);
+            ex3(p, rij - rik, ee, dee);
+            zeta += fk * gg * ee;
+        }
+        T de_r, de_z;
+        e += bond_terms(p, rij, zeta, de_r, de_z);
+        T gj[3] = {de_r * u[0], de_r * u[1], de_r * u[2]};
+        if (de_z != 0) {
+            for (int32_t b = 0; b < cnt; b++) {
+                if (b == a) continue;
+                double dk[3];
+                P.d(i, list[(int64_t)b * n + i], dk);
+                T rik, v[3];
+                unit(dk, rik, v);
+                if (!(rik < p.rc)) continue;
+                const ZTerm<T> z = zterm(p, u, rij, v, rik);
+                for (int q = 0; q < 3; q++) {
+                    gj[q] += de_z * (z.d_rij * u[q] + z.d_cs * (v[q] - z.cs * u[q]) / rij);
+                    w[4 * ((int64_t)b * n + i) + q] += de_z * (z.d_rik * v[q] + z.d_cs * (u[q] - z.cs * v[q]) / rik);
+                }
+            }
+        }
+        for (int q = 0; q < 3; q++) w[4 * ((int64_t)a * n + i) + q] += gj[q];
+    }
+    return e;
+}
+
 }  // namespace ts
 }  // namespace mdx
