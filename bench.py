@@ -153,7 +153,7 @@ def bench_tersoff(a) -> None:
     n = len(pos)
     print(f"bench: Tersoff Si {n} atoms, L={L:.4f}; warmup {a.warmup}, steps {a.steps}", file=sys.stderr, flush=True)
     s = System(n_atoms=n, n_bond=n, box=BOX_PERIODIC, precision=PREC_MIXED, use_lj=0, use_bond=1,
-               lj_table=LJ_UNIFORM, bond_form=BOND_TERSOFF, dt=1.0, mb_rs=3.3, mb_rcut=3.0, box_length=L,
+               lj_table=LJ_UNIFORM, bond_form=BOND_TERSOFF, dt=1.0, mb_rs=a.mb_rs, mb_rcut=3.0, box_length=L,
                mass=initial.SI_MASS, accel_scale=initial.ACCEL_EV_A_AMU)
     s.set_state(np.c_[pos, np.zeros(n)], vel)
     s.run(a.warmup)
@@ -191,7 +191,7 @@ def bench_tersoff(a) -> None:
         "dtype": "fp32-terms/fp64-accum",
         "data": "synthetic (seeded diamond lattice + Gaussian displacements + Maxwell-Boltzmann velocities)",
         "config": {"workload": f"periodic Tersoff Si {n} atoms (diamond {ncell}^3 x8, a 5.431 A), 1000 K, "
-                               f"R 2.7 D 0.3, list radius 3.3 A, dt 1 fs", "atoms": n, "list_builds": builds},
+                               f"R 2.7 D 0.3, list radius {a.mb_rs} A, dt 1 fs", "atoms": n, "list_builds": builds},
         "roofline": {
             "kernel": "k_ts_atoms (per-atom Tersoff terms and per-slot gradients)",
             "bound": "hbm",
@@ -199,7 +199,8 @@ def bench_tersoff(a) -> None:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": None,
+            # copy-calibrated FETCH_SIZE + WRITE_SIZE per launch (profiles/tersoff_si262k.md)
+            "traffic": measured_traffic(ROOT / "profiles" / "tersoff_si262k.md"),
             "avg_launch_ms": k_ms,
             "bytes_per_atom": TS_ATOMS_BYTES_PER_ATOM,
         },
