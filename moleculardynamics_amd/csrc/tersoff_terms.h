@@ -18,24 +18,34 @@ struct Tp {   // parameters in the arithmetic type
     int32_t m;
 };
 
+// FP32 (mixed precision) uses the hardware exp2, log2 and reciprocal (about
+// 1 ulp each); FP64 the correctly rounded library functions.
 template <typename T>
 __device__ inline T t_exp(T x) {
-    if constexpr (sizeof(T) == 4) return expf(x); else return exp(x);
+    if constexpr (sizeof(T) == 4) return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); else return exp(x);
 }
 template <typename T>
-__device__ inline T t_pow(T x, T y) {
-    if constexpr (sizeof(T) == 4) return powf(x, y); else return pow(x, y);
+__device__ inline T t_pow(T x, T y) {   // x > 0
+    if constexpr (sizeof(T) == 4) return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); else return pow(x, y);
+}
+template <typename T>
+__device__ inline T t_rcp(T x) {
+    if constexpr (sizeof(T) == 4) return __builtin_amdgcn_rcpf(x); else return 1.0 / x;
+}
+template <typename T>
+__device__ inline T t_div(T a, T b) {
+    if constexpr (sizeof(T) == 4) return a * __builtin_amdgcn_rcpf(b); else return a / b;
 }
 
 template <typename T>
 __device__ inline void fcut(const Tp<T>& p, T r, T& fc, T& dfc) {
     if (r < p.R - p.D) { fc = 1; dfc = 0; return; }
     if (r > p.R + p.D) { fc = 0; dfc = 0; return; }
-    const T x = (T)M_PI_2 * (r - p.R) / p.D;
+    const T x = t_div<T>((T)M_PI_2 * (r - p.R), p.D);
     T sx, cx;
     if constexpr (sizeof(T) == 4) { sincosf(x, &sx, &cx); } else { sincos(x, &sx, &cx); }
     fc = (T)0.5 - (T)0.5 * sx;
-    dfc = -(T)0.25 * (T)M_PI / p.D * cx;
+    dfc = t_div<T>(-(T)0.25 * (T)M_PI, p.D) * cx;
 }
 
 template <typename T>
@@ -44,11 +54,11 @@ __device__ inline void gang(const Tp<T>& p, T cs, T& g, T& dg) {
     if constexpr (sizeof(T) == 4) {
         // c^2/d^2 - c^2/den = c^2 u^2 / (d^2 den): for Si c^2/d^2 = 3.8e7 while g
         // is ~1e5, so the difference form would lose 6e-6 in FP32
-        g = 1 + p.c2 * (u * u) / (p.d2 * den);
+        g = 1 + t_div<T>(p.c2 * (u * u), p.d2 * den);
     } else {
         g = 1 + p.c2 / p.d2 - p.c2 / den;
     }
-    dg = -2 * p.c2 * u / (den * den);   // dg/dcos
+    dg = t_div<T>(-2 * p.c2 * u, den * den);   // dg/dcos
 }
 
 template <typename T>
@@ -110,9 +120,16 @@ __device__ inline ZTerm<T> zterm(const Tp<T>& p, const T u[3], T rij, const T w[
 
 template <typename T>
 __device__ inline void unit(const double d[3], T& r, T u[3]) {
-    const double rr = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    r = (T)rr;
-    for (int q = 0; q < 3; q++) u[q] = (T)(d[q] / rr);
+    const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if constexpr (sizeof(T) == 4) {   // r^2 in FP64, the rest in FP32
+        r = __builtin_sqrtf((float)r2);
+        const float ir = __builtin_amdgcn_rcpf(r);
+        for (int q = 0; q < 3; q++) u[q] = (float)d[q] * ir;
+    } else {
+        const double rr = sqrt(r2);
+        r = rr;
+        for (int q = 0; q < 3; q++) u[q] = d[q] / rr;
+    }
 }
 
 
@@ -133,7 +150,7 @@ __device__ inline double bond_terms(const Tp<T>& p, T rij, T zeta, T& de_r, T& d
     if (zeta > 0) {
         const T t = t_pow<T>(p.beta * zeta, p.n);
         bz = t_pow<T>(1 + t, (T)-0.5 / p.n);
-        dbz = (T)-0.5 * bz / (1 + t) * t / zeta;
+        dbz = (T)-0.5 * t_div<T>(bz, 1 + t) * t_div<T>(t, zeta);
     }
     de_r = (T)0.5 * (dfc * (fr + bz * fa) + fc * (dfr + bz * dfa));
     de_z = (T)0.5 * fc * fa * dbz;
@@ -144,18 +161,20 @@ __device__ inline double bond_terms(const Tp<T>& p, T rij, T zeta, T& de_r, T& d
 template <typename T, int kS>
 __device__ inline double centre_reg(const Pos3& P, const Tp<T>& p, const int32_t* __restrict__ list, int32_t n,
                                     int32_t i, int32_t cnt, T (&w)[kS][3]) {
-    T r[kS], u[kS][3];
+    T r[kS], ir[kS], u[kS][3];
     bool act[kS];
 #pragma unroll
     for (int s = 0; s < kS; s++) {
         w[s][0] = w[s][1] = w[s][2] = 0;
         act[s] = false;
-        r[s] = 0;
+        r[s] = 1;
+        ir[s] = 1;
         u[s][0] = u[s][1] = u[s][2] = 0;
         if (s < cnt) {
             double d[3];
             P.d(i, list[(int64_t)s * n + i], d);
             unit(d, r[s], u[s]);
+            ir[s] = t_rcp<T>(r[s]);
             act[s] = r[s] < p.rc;
         }
     }
@@ -184,8 +203,8 @@ __device__ inline double centre_reg(const Pos3& P, const Tp<T>& p, const int32_t
                 const ZTerm<T> z = zterm(p, u[a], r[a], u[b], r[b]);
 #pragma unroll
                 for (int q = 0; q < 3; q++) {
-                    w[a][q] += de_z * (z.d_rij * u[a][q] + z.d_cs * (u[b][q] - z.cs * u[a][q]) / r[a]);
-                    w[b][q] += de_z * (z.d_rik * u[b][q] + z.d_cs * (u[a][q] - z.cs * u[b][q]) / r[b]);
+                    w[a][q] += de_z * (z.d_rij * u[a][q] + z.d_cs * (u[b][q] - z.cs * u[a][q]) * ir[a]);
+                    w[b][q] += de_z * (z.d_rik * u[b][q] + z.d_cs * (u[a][q] - z.cs * u[b][q]) * ir[b]);
                 }
             }
         }
