@@ -211,6 +211,26 @@ k_sort_cells(const int32_t* __restrict__ start, int64_t ncell, const int32_t* __
     }
 }
 
+// The same order for sparse grids (a few items per cell): one thread per cell
+// ranks its items by comparing keys.
+__global__ void __launch_bounds__(kBlock)
+k_sort_cells_sparse(const int32_t* __restrict__ start, int64_t ncell, const int32_t* __restrict__ in,
+                    int32_t* __restrict__ out, const int32_t* __restrict__ key_of) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= ncell) return;
+    const int32_t b = start[c], cnt = start[c + 1] - b;
+    for (int32_t p = 0; p < cnt; p++) {
+        const int32_t it = in[b + p];
+        const int32_t key = key_of ? key_of[it] : it;
+        int32_t rank = 0;
+        for (int32_t q = 0; q < cnt; q++) {
+            const int32_t o = in[b + q];
+            rank += (key_of ? key_of[o] : o) < key ? 1 : 0;
+        }
+        out[b + rank] = it;
+    }
+}
+
 __global__ void k_permute(DevState src, DevState dst, const int32_t* __restrict__ sorted_src,
                           const int32_t* __restrict__ cell_of_src, int32_t* __restrict__ cell_of_dst,
                           int32_t* __restrict__ slot_of_id, double* rr0, double* rr1, double* rr2,
@@ -389,8 +409,12 @@ hipError_t launch_scatter(hipStream_t s, const int32_t* cell_of, const int32_t* 
 }
 
 hipError_t launch_sort_cells(hipStream_t s, const int32_t* start, int64_t ncell, const int32_t* items_in,
-                             int32_t* items_out, const int32_t* key_of_item) {
-    k_sort_cells<<<grid_for(ncell, kCellsPerBlock), kBlock, 0, s>>>(start, ncell, items_in, items_out, key_of_item);
+                             int32_t* items_out, const int32_t* key_of_item, int64_t items) {
+    if (ncell <= 0) return hipSuccess;
+    if (items < 4 * ncell)   // sparse: a thread per cell
+        k_sort_cells_sparse<<<grid_for(ncell, kBlock), kBlock, 0, s>>>(start, ncell, items_in, items_out, key_of_item);
+    else
+        k_sort_cells<<<grid_for(ncell, kCellsPerBlock), kBlock, 0, s>>>(start, ncell, items_in, items_out, key_of_item);
     return hipGetLastError();
 }
 

@@ -122,7 +122,8 @@ hipError_t launch_scatter(hipStream_t s, const int32_t* cell_of, const int32_t* 
 // items_in grouped by cell -> items_out with ascending key inside each cell
 // (key_of_item null: the item is its own key)
 hipError_t launch_sort_cells(hipStream_t s, const int32_t* start, int64_t ncell, const int32_t* items_in,
-                             int32_t* items_out, const int32_t* key_of_item);
+                             int32_t* items_out, const int32_t* key_of_item,
+                             int64_t items);
 hipError_t launch_permute(hipStream_t s, const DevState& src, const DevState& dst,
                           const int32_t* sorted_src, const int32_t* cell_of_src,
                           int32_t* cell_of_dst, int32_t* slot_of_id, double* const rref[3],

@@ -364,7 +364,7 @@ int build_primary(mdx_system* s, Grid& g, float rs, double box_size_list, bool s
     // scatter into scratch (cell_by_id is free until the sizes pass), then
     // order each cell by id into sorted
     LAUNCH(s, "cells_scatter", launch_scatter(st, s->cell_of, g.start, g.count, s->cell_by_id, n));
-    LAUNCH(s, "cells_sort", launch_sort_cells(st, g.start, g.ncell, s->cell_by_id, s->sorted, s->st.id));
+    LAUNCH(s, "cells_sort", launch_sort_cells(st, g.start, g.ncell, s->cell_by_id, s->sorted, s->st.id, n));
     const bool periodic = s->cfg.box == MDX_BOX_PERIODIC;
     double* none[3] = {nullptr, nullptr, nullptr};
     LAUNCH(s, "cells_permute",
@@ -401,7 +401,7 @@ int build_bond_walled(mdx_system* s, double box_size_list) {
     LAUNCH(s, "cells_scan", launch_exclusive_scan(st, g.count, g.start, g.ncell, s->scan_sums, s->scan_cap));
     LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
     LAUNCH(s, "cells_scatter", launch_scatter(st, s->mb_cell_of, g.start, g.count, s->cell_by_id, nb));
-    LAUNCH(s, "cells_sort", launch_sort_cells(st, g.start, g.ncell, s->cell_by_id, s->mb_ids, nullptr));
+    LAUNCH(s, "cells_sort", launch_sort_cells(st, g.start, g.ncell, s->cell_by_id, s->mb_ids, nullptr, nb));
     int32_t e;
     if ((rc = read_ints(s, &e, s->dint + 1, 1))) return rc;
     if (e) return fail(s, MDX_ERR_ARG, "an atom lies outside the list box (box_size_list too small)");
@@ -465,13 +465,22 @@ int ts_build_list(mdx_system* s) {
                 return rc;
             }
         }
-        HIPCHK(s, hipMemsetAsync(s->dint + 4, 0, sizeof(int32_t), s->stream));
+        if (s->ts_cap > 255) return fail(s, MDX_ERR_CAPACITY, "Tersoff lists hold at most 255 neighbours");
+        // the list, then the slot of each atom in its partners' lists (for the
+        // force gather); one read-back covers both: dint[4] the longest list
+        // when one overflowed, dint[5] a partner list without the atom
+        HIPCHK(s, hipMemsetAsync(s->dint + 4, 0, 2 * sizeof(int32_t), s->stream));
         LAUNCH(s, "ts_list", launch_ts_list(s->stream, s->st, s->cell_of, s->lj.view, (double)s->cfg.mb_rs, n,
                                             s->ts_cap, s->ts_list, s->ts_count, s->dint + 4));
-        int32_t longest;
-        int rc = read_ints(s, &longest, s->dint + 4, 1);
+        LAUNCH(s, "sg_rev", launch_sg_rev(s->stream, s->ts_list, s->ts_count, n, s->sg_rev, s->dint + 5));
+        int32_t e[2];
+        int rc = read_ints(s, e, s->dint + 4, 2);
         if (rc) return rc;
-        if (longest == 0) break;
+        if (e[0] == 0) {
+            if (e[1]) return fail(s, MDX_ERR_STATE, "asymmetric neighbour lists");
+            break;
+        }
+        const int32_t longest = e[0];
         dfree(s, s->ts_list);
         dfree(s, s->ts_wd);
         dfree(s, s->sg_w);
@@ -481,15 +490,6 @@ int ts_build_list(mdx_system* s) {
         s->sg_w = nullptr;
         s->sg_rev = nullptr;
         s->ts_cap = longest + longest / 4 + 4;
-    }
-    {   // the slot of each atom in its partners' lists (for the force gather)
-        if (s->ts_cap > 255) return fail(s, MDX_ERR_CAPACITY, "Tersoff lists hold at most 255 neighbours");
-        HIPCHK(s, hipMemsetAsync(s->dint + 4, 0, sizeof(int32_t), s->stream));
-        LAUNCH(s, "sg_rev", launch_sg_rev(s->stream, s->ts_list, s->ts_count, n, s->sg_rev, s->dint + 4));
-        int32_t asym;
-        int rc = read_ints(s, &asym, s->dint + 4, 1);
-        if (rc) return rc;
-        if (asym) return fail(s, MDX_ERR_STATE, "asymmetric neighbour lists");
     }
     s->ts_built = true;
     return MDX_OK;

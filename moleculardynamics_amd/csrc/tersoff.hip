@@ -31,32 +31,62 @@ using namespace ts;
 
 constexpr int kBlock = 256;
 
+// a / b for 0 <= a < 2^22, b >= 1: float quotient, fixed up by one step
+__device__ inline int32_t idiv(int32_t a, int32_t b) {
+    int32_t q = (int32_t)((float)a * __builtin_amdgcn_rcpf((float)b));
+    q += (q + 1) * b <= a ? 1 : 0;
+    q -= q * b > a ? 1 : 0;
+    return q;
+}
+
+// The 27 neighbour cells as 9 z-columns: cells z-1 .. z+1 of a column are one
+// contiguous slot range, or two where the column wraps (the grid has at least
+// 3 cells per axis).  The distance test is FP32 on the FP64 minimum-image
+// difference, which is exactly antisymmetric, so the lists stay symmetric;
+// rl2 is rs^2 enlarged by a few ulp, so the list holds every atom within rs.
 __global__ void __launch_bounds__(kBlock)
-k_ts_list(Pos3 P, const int32_t* __restrict__ cell_of, GridView g, double rs2, int32_t n, int32_t cap,
-          int32_t* __restrict__ list, int32_t* __restrict__ count, int32_t* __restrict__ err) {
+k_ts_list(Pos3 P, const int32_t* __restrict__ cell_of, GridView g, float rl2, int32_t n, int32_t cap,
+          int32_t* __restrict__ list, int32_t* __restrict__ count, int32_t* __restrict__ err, int32_t small) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int32_t g1 = g.g[1], g2 = g.g[2];
     const int32_t c = cell_of[i];
-    const int32_t cz = c % g.g[2], cy = (c / g.g[2]) % g.g[1], cx = c / (g.g[1] * g.g[2]);
+    const int32_t t = small ? idiv(c, g2) : c / g2, cz = c - t * g2;
+    const int32_t cx = small ? idiv(t, g1) : t / g1, cy = t - cx * g1;
+    const double xi = P.x[i], yi = P.y[i], zi = P.z[i];
     int32_t k = 0;
-    for (int dx = -1; dx <= 1; dx++)
-        for (int dy = -1; dy <= 1; dy++)
-            for (int dz = -1; dz <= 1; dz++) {
-                int32_t x = cx + dx, y = cy + dy, z = cz + dz;
-                x = x < 0 ? x + g.g[0] : (x >= g.g[0] ? x - g.g[0] : x);
-                y = y < 0 ? y + g.g[1] : (y >= g.g[1] ? y - g.g[1] : y);
-                z = z < 0 ? z + g.g[2] : (z >= g.g[2] ? z - g.g[2] : z);
-                const int32_t cc = (x * g.g[1] + y) * g.g[2] + z;
-                for (int32_t j = g.start[cc]; j < g.start[cc + 1]; j++) {
+    for (int dx = -1; dx <= 1; dx++) {
+        int32_t x = cx + dx;
+        x = x < 0 ? x + g.g[0] : (x >= g.g[0] ? x - g.g[0] : x);
+        for (int dy = -1; dy <= 1; dy++) {
+            int32_t y = cy + dy;
+            y = y < 0 ? y + g1 : (y >= g1 ? y - g1 : y);
+            const int32_t col = (x * g1 + y) * g2;
+            int32_t rb[2], re[2];
+            if (cz == 0) {
+                rb[0] = g.start[col]; re[0] = g.start[col + 2];
+                rb[1] = g.start[col + g2 - 1]; re[1] = g.start[col + g2];
+            } else if (cz == g2 - 1) {
+                rb[0] = g.start[col + g2 - 2]; re[0] = g.start[col + g2];
+                rb[1] = g.start[col]; re[1] = g.start[col + 1];
+            } else {
+                rb[0] = g.start[col + cz - 1]; re[0] = g.start[col + cz + 2];
+                rb[1] = re[1] = 0;
+            }
+#pragma unroll
+            for (int r = 0; r < 2; r++)
+                for (int32_t j = rb[r]; j < re[r]; j++) {
                     if (j == i) continue;
-                    double d[3];
-                    P.d(i, j, d);
-                    if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < rs2) {
+                    const float ex = (float)min_image(P.x[j] - xi, P.L, P.invL);
+                    const float ey = (float)min_image(P.y[j] - yi, P.L, P.invL);
+                    const float ez = (float)min_image(P.z[j] - zi, P.L, P.invL);
+                    if (fmaf(ex, ex, fmaf(ey, ey, ez * ez)) < rl2) {
                         if (k < cap) list[(int64_t)k * n + i] = j;
                         k++;
                     }
                 }
-            }
+        }
+    }
     count[i] = k < cap ? k : cap;
     if (k > cap) atomicMax(err, k);
 }
@@ -106,7 +136,9 @@ TersoffParams tersoff_si_params() {
 hipError_t launch_ts_list(hipStream_t s, const DevState& st, const int32_t* cell_of, const GridView& g, double rs,
                           int32_t n, int32_t cap, int32_t* list, int32_t* count, int32_t* err) {
     if (n <= 0) return hipSuccess;
-    k_ts_list<<<grid_for(n), kBlock, 0, s>>>(pos3(st, g.L), cell_of, g, rs * rs, n, cap, list, count, err);
+    const float rl2 = (float)(rs * rs * (1.0 + 1e-6));
+    const int32_t small = (int64_t)g.g[0] * g.g[1] * g.g[2] < (1 << 22) ? 1 : 0;
+    k_ts_list<<<grid_for(n), kBlock, 0, s>>>(pos3(st, g.L), cell_of, g, rl2, n, cap, list, count, err, small);
     return hipGetLastError();
 }
 
