@@ -161,8 +161,8 @@ k_predict_track(DevState st, int32_t n, GearCoeffs gc, const double* __restrict_
 // (unless final) the predictor of step t+1 with the displacement from the
 // list-build positions and the single-step move max-reduced into the spread
 // flags, as k_predict_track does.
-// With kGather the force is first gathered from the per-slot gradients of
-// the surrogate (written to st.f) and atoms at least sqrt(hard2) from the
+// With kGather the force is first gathered from the per-slot gradients (and
+// written to st.f on the final step of a run) and atoms at least sqrt(hard2) from the
 // list-build positions are counted as stale, as the bond-order kernels do.
 template <bool kFinal, bool kGather>
 __global__ void __launch_bounds__(kBlock)
@@ -198,7 +198,7 @@ k_correct_predict_track(DevState st, int32_t n, GearCoeffs gc, MassTable m, cons
             double e2 = 0.0;
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                st.f[d][i] = F[d];
+                if constexpr (kFinal) st.f[d][i] = F[d];   // read back only after a run
                 const double e = st.r[d][i] - r0[d][i];
                 e2 += e * e;
             }
