@@ -38,7 +38,7 @@ using namespace mdx;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kSortBlock = 256;
+constexpr int kSortBlock = 1024;
 constexpr int kMaxNc = 10;                               // cells per axis at most (larger cells stay correct)
 constexpr int kMaxCells = kMaxNc * kMaxNc * kMaxNc;
 constexpr int kMaxReplica = 16384;                       // atoms per replica (the LDS sort holds them all)
