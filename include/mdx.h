@@ -35,7 +35,9 @@
  * order.  Every entry point returns MDX_OK (0) or a negative MDX_ERR_* code
  * in place of the reference's print-and-exit (common/Global.h:116-117);
  * mdx_last_error() gives the message.  A system is single-threaded: calls on
- * one system must not overlap.  Different systems are independent.
+ * one system must not overlap.  Different systems are independent, each with
+ * its own HIP stream, and may be driven from different host threads at once
+ * (the cpumd driver runs vxcm sweep members that way).
  */
 #ifndef MDX_H
 #define MDX_H
@@ -120,7 +122,8 @@ typedef struct mdx_config {
 /* lifetime (cpu/cpumd/Host.cpp:43-48, 218-237) */
 int mdx_create(const mdx_config* cfg, mdx_system** out);
 int mdx_destroy(mdx_system* sys);
-const char* mdx_last_error(const mdx_system* sys);  /* sys may be NULL */
+const char* mdx_last_error(const mdx_system* sys);  /* sys NULL: the calling thread's last
+                                                       creation error */
 
 /* state transfer (common/Loader.cpp:88-155 layout).  pos4: n*4 doubles (x, y,
  * z, species), vel3/acc3/b3/c3: n*3 doubles; acc3/b3/c3 may be NULL (zero). */
@@ -222,7 +225,7 @@ typedef struct mdx_sg_trainer mdx_sg_trainer;
 int mdx_sg_trainer_create(int32_t n_frames, int32_t n_atoms, double box_length, const double* pos3,
                           const double* force3, const double* energy, mdx_sg_trainer** out);
 int mdx_sg_trainer_destroy(mdx_sg_trainer* t);
-const char* mdx_sg_trainer_last_error(const mdx_sg_trainer* t);   /* t may be NULL */
+const char* mdx_sg_trainer_last_error(const mdx_sg_trainer* t);   /* t NULL: this thread's */
 /* per descriptor component over all atoms: mean[48], std[48], min[48], max[48] */
 int mdx_sg_trainer_feature_stats(mdx_sg_trainer* t, double* out4x48);
 /* descriptors of one frame, [atom][48] */
@@ -256,7 +259,7 @@ typedef struct mdx_batch mdx_batch;
 int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t* n_atoms,
                      const double* box_length, mdx_batch** out);
 int mdx_batch_destroy(mdx_batch* batch);
-const char* mdx_batch_last_error(const mdx_batch* batch);  /* batch may be NULL */
+const char* mdx_batch_last_error(const mdx_batch* batch);  /* batch NULL: this thread's */
 /* (re)starts the batch: positions, velocities; Gear history cleared */
 int mdx_batch_set_state(mdx_batch* batch, const double* pos3, const double* vel3);
 /* any output may be NULL; force3 is the last computed force */

@@ -79,7 +79,7 @@ def build_host() -> None:
     if not srcs:
         return
     lib_srcs = [s for s in srcs if s.name != "cpumd_main.cpp"]
-    common = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{ROOT / 'include'}", f"-I{HOST}"]
+    common = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-pthread", f"-I{ROOT / 'include'}", f"-I{HOST}"]
     tmp = HOSTLIB.with_suffix(".so.tmp")
     _run(["g++", *common, "-shared", "-o", str(tmp), *map(str, lib_srcs),
           f"-L{PKG}", "-lmdx", f"-Wl,-rpath,$ORIGIN"])

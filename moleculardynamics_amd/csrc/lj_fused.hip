@@ -524,11 +524,8 @@ hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v
     if (n <= 0) return hipSuccess;
     if (cap % (kLanes * kChunk) != 0) return hipErrorInvalidValue;
     if (lds_atoms < 1 || lds_atoms > kLdsAtoms) return hipErrorInvalidValue;
-    static bool attr_set = false;
-    if (!attr_set) {
-        allow_lds(k_nlist_lds);
-        attr_set = true;
-    }
+    static const bool attr_set = (allow_lds(k_nlist_lds), true);  // once per process, thread-safe
+    (void)attr_set;
     SegGrid g = make_grid(v, nseg);
     const size_t dyn = 3 * (size_t)lds_atoms * sizeof(float);
     k_nlist_lds<<<g.per_xcd * kXcd, kBlock, dyn, s>>>(g, st.r[0], st.r[1], st.r[2], rl2, n, cap, list, count, err,
@@ -542,8 +539,7 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
                            int64_t cs, int32_t precision, bool final_step, bool force_out, uint64_t* flags,
                            int32_t lds_atoms, uint64_t* clear_next) {
     if (n <= 0) return hipSuccess;
-    static bool attr_set = false;
-    if (!attr_set) {
+    static const bool attr_set = [] {  // once per process, thread-safe
         allow_lds(k_step_lds<true, true, true>);
         allow_lds(k_step_lds<true, true, false>);
         allow_lds(k_step_lds<true, false, true>);
@@ -552,8 +548,9 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
         allow_lds(k_step_lds<false, true, false>);
         allow_lds(k_step_lds<false, false, true>);
         allow_lds(k_step_lds<false, false, false>);
-        attr_set = true;
-    }
+        return true;
+    }();
+    (void)attr_set;
     if (lds_atoms < 1 || lds_atoms > kLdsAtoms) return hipErrorInvalidValue;
     StepArgs a;
     a.g = make_grid(v, nseg);

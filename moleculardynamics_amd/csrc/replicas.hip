@@ -441,7 +441,7 @@ struct mdx_batch {
 
 namespace {
 
-std::string g_batch_err;
+thread_local std::string g_batch_err;  // errors before a batch exists, per calling thread
 
 int bfail(mdx_batch* b, int code, const std::string& m) {
     if (b) b->err = m; else g_batch_err = m;

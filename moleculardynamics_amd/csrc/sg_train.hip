@@ -469,7 +469,7 @@ struct mdx_sg_trainer {
 
 namespace {
 
-std::string g_terr;
+thread_local std::string g_terr;  // errors before a trainer exists, per calling thread
 
 int tfail(mdx_sg_trainer* t, int code, const std::string& msg) {
     if (t) t->err = msg; else g_terr = msg;

@@ -354,12 +354,10 @@ hipError_t launch_sg_eval(hipStream_t s, const DevState& st, double L, const Ter
                           const int32_t* count, int32_t n, const uint16_t* frag, const float* vec, float4* w,
                           double* e_atom, int32_t* nflag) {
     if (n <= 0) return hipSuccess;
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_sg_eval, hipFuncAttributeMaxDynamicSharedMemorySize, kLds);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
+    // once per process, thread-safe
+    static const hipError_t attr =
+        hipFuncSetAttribute((const void*)k_sg_eval, hipFuncAttributeMaxDynamicSharedMemorySize, kLds);
+    if (attr != hipSuccess) return attr;
     k_sg_eval<<<grid_for(n), kBlock, kLds, s>>>(pos3(st, L), list, count, n, reinterpret_cast<const uint4*>(frag),
                                                 vec, cast_params<float>(p), w, e_atom, nflag);
     return hipGetLastError();

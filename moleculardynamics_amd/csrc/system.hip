@@ -115,7 +115,7 @@ struct mdx_system {
 
 namespace {
 
-std::string g_err;  // errors before a system exists
+thread_local std::string g_err;  // errors before a system exists, per calling thread
 
 int fail(mdx_system* s, int code, const std::string& msg) {
     if (s) s->err = msg; else g_err = msg;
