@@ -291,6 +291,295 @@ k_bond_pass2(Ctx c, BondRefParams p, int32_t n_bond, int32_t cap, const int32_t*
     c.st.f[2][sk] += Fz;
 }
 
+// ---- one wavefront per bond-order atom (few atoms, short rows) ----
+//
+// The atom's candidate row (9 contiguous runs of the cell-sorted member
+// array, in the reference's row order) is staged once into LDS with FP64
+// positions, so the nested loops over the row read LDS instead of chasing
+// ids through memory.  The outer loops (j in pass 1, i in pass 2) run in row
+// order over the ballot of the lanes whose entry qualifies; the inner loop
+// over the row is spread over the lanes and summed by a fixed butterfly, so
+// results are deterministic and differ from the thread-per-atom kernels only
+// in the association of those inner sums.
+constexpr int kWaveBlock = 256;
+constexpr int kRowMax = 256;   // row entries staged per wavefront
+
+struct RowEntry {
+    double x, y, z;
+    int32_t id;     // -1 for the atom itself (not a candidate)
+    int32_t type;
+};
+
+template <class T> __device__ inline T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// stage the row of `self` into row[0..len); the host guarantees len <= kRowMax
+__device__ inline int32_t stage_row(const Ctx& c, int32_t self, RowEntry* row, int lane) {
+    const int32_t own = c.cell_of[self];
+    const int32_t cz = own % c.g.g[2], cy = (own / c.g.g[2]) % c.g.g[1], cx = own / (c.g.g[2] * c.g.g[1]);
+    int32_t rb = 0, rn = 0;
+    if (lane < 9) {
+        const int32_t x = cx + lane / 3 - 1, y = cy + lane % 3 - 1;
+        if (x >= 0 && x < c.g.cells && y >= 0 && y < c.g.cells) {
+            const int32_t z0 = max(cz - 1, 0), z1 = min(cz + 1, c.g.cells - 1);
+            if (z0 <= z1) {
+                const int32_t col = (x * c.g.g[1] + y) * c.g.g[2];
+                rb = c.g.start[col + z0];
+                rn = c.g.start[col + z1 + 1] - rb;
+            }
+        }
+    }
+    int32_t base[9], off[9], len = 0;
+#pragma unroll
+    for (int r = 0; r < 9; r++) {
+        base[r] = __builtin_amdgcn_readlane(rb, r);
+        off[r] = len;
+        len += __builtin_amdgcn_readlane(rn, r);
+    }
+    len = min(len, kRowMax);
+    for (int32_t q = lane; q < len; q += 64) {
+        int32_t p = 0;
+#pragma unroll
+        for (int r = 0; r < 9; r++)
+            if (q >= off[r]) p = base[r] + (q - off[r]);
+        const int32_t m = c.ids[p];
+        const int32_t sm = c.slot_of_id[m];
+        row[q] = RowEntry{c.st.r[0][sm], c.st.r[1][sm], c.st.r[2][sm], m == self ? -1 : m, c.st.type[sm]};
+    }
+    return len;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kWaveBlock)
+k_bond_wave1(Ctx c, BondRefParams p, int32_t n_bond, int32_t cap, int32_t assign,
+             int32_t* __restrict__ xi_j, double* __restrict__ xi_v, double* __restrict__ e_atom,
+             const int32_t* __restrict__ gate) {
+    __shared__ RowEntry rows[kWaveBlock / 64][kRowMax];
+    if (gate && *gate == 0) return;   // the same for every workgroup
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int32_t i = (int32_t)blockIdx.x * (kWaveBlock / 64) + w;
+    RowEntry* row = rows[w];
+    const int32_t len = i < n_bond ? stage_row(c, i, row, lane) : 0;
+    __syncthreads();
+    if (i >= n_bond) return;
+    const int32_t si = c.slot_of_id[i];
+    const double px = c.st.r[0][si], py = c.st.r[1][si], pz = c.st.r[2][si];
+    const int32_t ti = c.st.type[si];
+    double Fx = 0, Fy = 0, Fz = 0, e = 0;
+    int32_t nx = 0;
+    for (int32_t q0 = 0; q0 < len; q0 += 64) {
+        bool cand = false;
+        if (q0 + lane < len) {
+            const RowEntry a = row[q0 + lane];
+            const double dx = px - a.x, dy = py - a.y, dz = pz - a.z;
+            const double r2d = dx * dx + dy * dy + dz * dz;
+            cand = a.id >= 0 && !(r2d >= p.rcut2 || r2d > p.pair_cut2) && a.type != kNe;
+        }
+        uint64_t mj = __ballot(cand);
+        while (mj) {
+            const int l = __builtin_ctzll(mj);
+            mj &= mj - 1;
+            const RowEntry bj = row[q0 + l];
+            const int32_t j = bj.id;
+            const double djx = px - bj.x, djy = py - bj.y, djz = pz - bj.z;
+            const double r2d = djx * djx + djy * djy + djz * djz;
+            const V3<T> ij = mk((T)djx, (T)djy, (T)djz);
+            const T rij = d_sqrt((T)r2d);
+            const int kd = pair_kind(ti, bj.type);
+            T va = 0, dva = 0, vr = 0, dvr = 0, vdw, dvdw, bo = 0;
+            V3<T> dbo = mk<T>(0, 0, 0);
+            T fcut, dfcut;
+            cutoff<T>(rij / (T)10, fcut, dfcut);
+            if (fcut > 0) {
+                const T re = (T)p.re[kd], de = (T)p.de[kd], be = (T)p.beta[kd];
+                attractive<T>(rij, de, be, re, va, dva);
+                repulsive<T>(rij, de, be, re, vr, dvr);
+                const V3<T> eij = ij * ((T)1 / rij);
+                const T ex = d_exp(K<T>::LAMDA3 * (rij - re));
+                T xi = 0, xj = 0;
+                V3<T> dxi = mk<T>(0, 0, 0), dxj = mk<T>(0, 0, 0);
+                for (int32_t m0 = 0; m0 < len; m0 += 64) {
+                    if (m0 + lane >= len) continue;
+                    const RowEntry bm = row[m0 + lane];
+                    if (bm.id < 0 || bm.id == j || bm.type == kNe) continue;
+                    const double ax = px - bm.x, ay = py - bm.y, az = pz - bm.z;
+                    const double rim2 = ax * ax + ay * ay + az * az;
+                    if (rim2 >= p.rcut2) continue;
+                    const V3<T> im = mk((T)ax, (T)ay, (T)az);
+                    const T rim = d_sqrt((T)rim2);
+                    const V3<T> jm = mk((T)(bj.x - bm.x), (T)(bj.y - bm.y), (T)(bj.z - bm.z));
+                    const T rjm = d_sqrt(dot(jm, jm));
+                    T fci, dfci, fcj, dfcj;
+                    cutoff<T>(rim, fci, dfci);
+                    cutoff<T>(rjm, fcj, dfcj);
+                    const V3<T> eim = im * ((T)1 / rim);
+                    if (fci > 0) {
+                        T cs = (rij * rij + rim * rim - rjm * rjm) / ((T)2 * rij * rim);
+                        T g, dg;
+                        ang<T>(cs, g, dg);
+                        xi += fci * g * ex;
+                        V3<T> dcs = (eim - eij * cs) * ((T)1 / rij) + (eij - eim * cs) * ((T)1 / rim);
+                        dxi = dxi + eim * (dfci * g * ex) + dcs * (fci * dg * ex) + eij * (fci * g * K<T>::LAMDA3 * ex);
+                    }
+                    if (fcj > 0) {
+                        T cs = (rij * rij + rjm * rjm - rim * rim) / ((T)2 * rij * rjm);
+                        T g, dg;
+                        ang<T>(cs, g, dg);
+                        xj += fcj * g * ex;
+                        V3<T> dcs = (jm * ((T)-1 / rjm) - eij * cs) * ((T)1 / rij);
+                        dxj = dxj + dcs * (fcj * dg * ex) + eij * (fcj * g * K<T>::LAMDA3 * ex);
+                    }
+                }
+                xi = wave_sum(xi);
+                xj = wave_sum(xj);
+                dxi = mk(wave_sum(dxi.x), wave_sum(dxi.y), wave_sum(dxi.z));
+                dxj = mk(wave_sum(dxj.x), wave_sum(dxj.y), wave_sum(dxj.z));
+                T qi = (T)1 / (((T)1 + xi) * ((T)1 + xi)), qj = (T)1 / (((T)1 + xj) * ((T)1 + xj));
+                bo = (T)0.5 * ((T)1 / ((T)1 + xi) + (T)1 / ((T)1 + xj));
+                dbo = (dxi * qi + dxj * qj) * (T)0.5;
+                if (nx < cap && lane == 0) {
+                    int64_t q = (int64_t)i * cap + nx;
+                    xi_j[q] = j;
+                    xi_v[2 * q] = (double)xi;
+                    xi_v[2 * q + 1] = (double)xj;
+                }
+                nx++;
+            }
+            vanderwaals<T>(rij, (T)p.r0v[kd], vdw, dvdw);
+            T fij = dvr - bo * dva + dvdw;
+            T s = fij / rij;
+            Fx += (double)(ij.x * s - dbo.x * va);
+            Fy += (double)(ij.y * s - dbo.y * va);
+            Fz += (double)(ij.z * s - dbo.z * va);
+            e += 0.5 * (double)((vr - bo * va) + vdw);
+        }
+    }
+    if (lane != 0) return;
+    if (nx < cap) xi_j[(int64_t)i * cap + nx] = -1;
+    if (assign) {
+        c.st.f[0][si] = Fx; c.st.f[1][si] = Fy; c.st.f[2][si] = Fz;
+    } else {
+        c.st.f[0][si] += Fx; c.st.f[1][si] += Fy; c.st.f[2][si] += Fz;
+    }
+    if (e_atom) e_atom[i] = e;
+}
+
+// pass 2 per wavefront: atom k; i runs over k's row in order, j over k's row
+// on the lanes; the xi table of i (at most 64 entries) is held one entry per lane
+template <class T>
+__global__ void __launch_bounds__(kWaveBlock)
+k_bond_wave2(Ctx c, BondRefParams p, int32_t n_bond, int32_t cap, const int32_t* __restrict__ xi_j,
+             const double* __restrict__ xi_v, const int32_t* __restrict__ gate) {
+    __shared__ RowEntry rows[kWaveBlock / 64][kRowMax];
+    if (gate && *gate == 0) return;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int32_t k = (int32_t)blockIdx.x * (kWaveBlock / 64) + w;
+    RowEntry* row = rows[w];
+    const int32_t len = k < n_bond ? stage_row(c, k, row, lane) : 0;
+    __syncthreads();
+    if (k >= n_bond) return;
+    const int32_t sk = c.slot_of_id[k];
+    const double px = c.st.r[0][sk], py = c.st.r[1][sk], pz = c.st.r[2][sk];
+    double Fx = 0, Fy = 0, Fz = 0;
+    for (int32_t q0 = 0; q0 < len; q0 += 64) {
+        bool cand = false;
+        if (q0 + lane < len) {
+            const RowEntry a = row[q0 + lane];
+            const double dx = px - a.x, dy = py - a.y, dz = pz - a.z;
+            cand = a.id >= 0 && dx * dx + dy * dy + dz * dz < p.rcut2;
+        }
+        uint64_t mi = __ballot(cand);
+        while (mi) {
+            const int l = __builtin_ctzll(mi);
+            mi &= mi - 1;
+            const RowEntry bi = row[q0 + l];
+            const int32_t i = bi.id;
+            const double kx = px - bi.x, ky = py - bi.y, kz = pz - bi.z;
+            const V3<T> ik = mk((T)kx, (T)ky, (T)kz);
+            const T rik = d_sqrt((T)(kx * kx + ky * ky + kz * kz));
+            // i's xi table up to its first -1 (the reference scans no further)
+            int32_t tj = -1;
+            double t0 = 0, t1 = 0;
+            if (lane < cap) {
+                const int64_t q = (int64_t)i * cap + lane;
+                tj = xi_j[q];
+                t0 = xi_v[2 * q];
+                t1 = xi_v[2 * q + 1];
+            }
+            const uint64_t ends = __ballot(lane < cap && tj < 0);
+            const int nt = ends ? __builtin_ctzll(ends) : cap;
+            for (int32_t m0 = 0; m0 < len; m0 += 64) {
+                const bool have = m0 + lane < len;
+                const RowEntry bj = row[have ? m0 + lane : 0];
+                const int32_t j = have ? bj.id : -1;
+                T xi = 0, xj = 0;
+                for (int t = 0; t < nt; t++) {
+                    const int32_t jj = __builtin_amdgcn_readlane(tj, t);
+                    const double v0 = lane_double(t0, t), v1 = lane_double(t1, t);
+                    if (jj == j) {
+                        xi = (T)v0;
+                        xj = (T)v1;
+                    }
+                }
+                if (j < 0 || j == i) continue;
+                const double ux = bi.x - bj.x, uy = bi.y - bj.y, uz = bi.z - bj.z;
+                const double r2d = ux * ux + uy * uy + uz * uz;
+                if (r2d >= p.rcut2 || r2d > p.pair_cut2) continue;
+                const T rij = d_sqrt((T)r2d);
+                const int kd = pair_kind(bi.type, bj.type);
+                const T re = (T)p.re[kd], de = (T)p.de[kd], be = (T)p.beta[kd];
+                T va, dva;
+                attractive<T>(rij, de, be, re, va, dva);
+                // bondOrder2 (BondOrderFuncs.h:148-237): -db_ij/dx_k, halves included
+                const V3<T> kj = mk((T)(px - bj.x), (T)(py - bj.y), (T)(pz - bj.z));
+                T rjk = d_sqrt(dot(kj, kj));
+                V3<T> ejk = kj * ((T)1 / rjk);
+                V3<T> eik = ik * ((T)1 / rik);
+                V3<T> uij = mk((T)-ux, (T)-uy, (T)-uz) * ((T)1 / rij);
+                T ex = d_exp(K<T>::LAMDA3 * (rij - re));
+                T fci, dfci, fcj, dfcj;
+                cutoff<T>(rik, fci, dfci);
+                cutoff<T>(rjk, fcj, dfcj);
+                V3<T> d = mk<T>(0, 0, 0);
+                if (fci > 0) {
+                    T cs = (rij * rij + rik * rik - rjk * rjk) / ((T)2 * rij * rik);
+                    T g, dg;
+                    ang<T>(cs, g, dg);
+                    V3<T> dcs = (uij - eik * cs) * ((T)1 / rik);
+                    V3<T> dx_ = eik * (dfci * g * ex) + dcs * (fci * dg * ex);
+                    d = d + dx_ * ((T)0.5 / (((T)1 + xi) * ((T)1 + xi)));
+                }
+                if (fcj > 0) {
+                    T cs = (rij * rij + rjk * rjk - rik * rik) / ((T)2 * rij * rjk);
+                    T g, dg;
+                    ang<T>(cs, g, dg);
+                    V3<T> dcs = (uij * (T)-1 - ejk * cs) * ((T)1 / rjk);
+                    V3<T> dx_ = ejk * (dfcj * g * ex) + dcs * (fcj * dg * ex);
+                    d = d + dx_ * ((T)0.5 / (((T)1 + xj) * ((T)1 + xj)));
+                }
+                Fx -= (double)((T)0.5 * d.x * va);
+                Fy -= (double)((T)0.5 * d.y * va);
+                Fz -= (double)((T)0.5 * d.z * va);
+            }
+        }
+    }
+    Fx = wave_sum(Fx);
+    Fy = wave_sum(Fy);
+    Fz = wave_sum(Fz);
+    if (lane == 0) {
+        c.st.f[0][sk] += Fx;
+        c.st.f[1][sk] += Fy;
+        c.st.f[2][sk] += Fz;
+    }
+}
+
+// below this many bond-order atoms (with rows and xi tables that fit) the
+// wavefront-per-atom kernels run
+constexpr int32_t kWaveAtoms = 32768;
+
 // common/DebugCalculations.cpp:129-270: per-atom share of the brute-force
 // potential.  Ordered pairs with a Ne atom add 2 eps [(s/r)^12 - (s/r)^6]
 // (no cutoff); other pairs with r <= CUTOFF_RIJ add (V_R - b V_A + vdW)/2
@@ -362,9 +651,22 @@ hipError_t launch_bond_ref(hipStream_t s, const DevState& st, const int32_t* slo
                            const int32_t* mb_cell_of, const int32_t* mb_ids, const GridView& g,
                            const BondRefParams& p, int32_t n_bond, int32_t cap,
                            int32_t precision, int32_t assign, int32_t* xi_j, double* xi_v,
-                           double* energy_per_atom, const int32_t* gate) {
+                           double* energy_per_atom, const int32_t* gate, int64_t row_bound) {
     if (n_bond <= 0) return hipSuccess;
     Ctx c{st, slot_of_id, mb_cell_of, mb_ids, g};
+    if (n_bond <= kWaveAtoms && row_bound <= kRowMax && cap <= 64) {
+        dim3 wgrid((n_bond + kWaveBlock / 64 - 1) / (kWaveBlock / 64));
+        if (precision == MDX_PREC_FP64) {
+            k_bond_wave1<double><<<wgrid, kWaveBlock, 0, s>>>(c, p, n_bond, cap, assign, xi_j, xi_v,
+                                                             energy_per_atom, gate);
+            k_bond_wave2<double><<<wgrid, kWaveBlock, 0, s>>>(c, p, n_bond, cap, xi_j, xi_v, gate);
+        } else {
+            k_bond_wave1<float><<<wgrid, kWaveBlock, 0, s>>>(c, p, n_bond, cap, assign, xi_j, xi_v,
+                                                            energy_per_atom, gate);
+            k_bond_wave2<float><<<wgrid, kWaveBlock, 0, s>>>(c, p, n_bond, cap, xi_j, xi_v, gate);
+        }
+        return hipGetLastError();
+    }
     dim3 grid((n_bond + kBlock - 1) / kBlock);
     if (precision == MDX_PREC_FP64) {
         k_bond_pass1<double><<<grid, kBlock, 0, s>>>(c, p, n_bond, cap, assign, xi_j, xi_v,

@@ -98,14 +98,138 @@ k_lj(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable t, in
     if constexpr (kEnergy) e_atom[s] = e;
 }
 
+// Walled box, few atoms: one wavefront per atom.  The 27 candidate cells of
+// the reference's row are 9 z-runs of up to 3 consecutive cells, i.e. 9
+// contiguous slot ranges in row order; the 64 lanes take 64 consecutive row
+// entries at a time and evaluate their pairs, then every lane adds the
+// in-cutoff terms in row order (read back lane by lane), so the sums are the
+// thread-per-atom kernel's, operation for operation.
+constexpr int kWaveBlock = 256;
+
+template <bool kFp64, bool kEnergy>
+__global__ void __launch_bounds__(kWaveBlock)
+k_lj_wave(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable t, int32_t n,
+          double* __restrict__ e_atom, const int32_t* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    const int32_t s = (int32_t)((blockIdx.x * kWaveBlock + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (s >= n) return;
+    const double xi = st.r[0][s], yi = st.r[1][s], zi = st.r[2][s];
+    const int32_t ti = st.type[s] & (kTypes - 1);
+    const int32_t own = cell_of[s];
+    const int32_t cz = own % g.g[2], cy = (own / g.g[2]) % g.g[1], cx = own / (g.g[2] * g.g[1]);
+    // lane r < 9: slot range of the column (cx + r/3 - 1, cy + r%3 - 1), z clipped to the grid
+    int32_t rb = 0, rn = 0;
+    if (lane < 9) {
+        const int32_t x = cx + lane / 3 - 1, y = cy + lane % 3 - 1;
+        if (x >= 0 && x < g.cells && y >= 0 && y < g.cells) {
+            const int32_t z0 = max(cz - 1, 0), z1 = min(cz + 1, g.cells - 1);
+            if (z0 <= z1) {
+                const int32_t col = (x * g.g[1] + y) * g.g[2];
+                rb = g.start[col + z0];
+                rn = g.start[col + z1 + 1] - rb;
+            }
+        }
+    }
+    int32_t base[9], off[9], len = 0;
+#pragma unroll
+    for (int r = 0; r < 9; r++) {
+        base[r] = __builtin_amdgcn_readlane(rb, r);
+        off[r] = len;
+        len += __builtin_amdgcn_readlane(rn, r);
+    }
+    double fx = 0, fy = 0, fz = 0, e = 0;
+    for (int32_t q0 = 0; q0 < len; q0 += 64) {
+        const int32_t q = q0 + lane;
+        int32_t p = -1;
+        if (q < len) {
+            // the last range starting at or before q holds q (an empty range
+            // shares its start with the next one)
+#pragma unroll
+            for (int r = 0; r < 9; r++)
+                if (q >= off[r]) p = base[r] + (q - off[r]);
+        }
+        bool in = false;
+        double ddx = 0, ddy = 0, ddz = 0, term_e = 0, c0 = 0, c1 = 0, c2 = 0;
+        [[maybe_unused]] double frd = 0;
+        if (p >= 0 && p != s) {
+            ddx = xi - st.r[0][p];
+            ddy = yi - st.r[1][p];
+            ddz = zi - st.r[2][p];
+            const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            const int32_t k = ti * kTypes + (st.type[p] & (kTypes - 1));
+            in = r2 < t.rc2 && ((t.active >> k) & 1u);
+            if (in) {
+                if constexpr (kFp64) {
+                    const double qq = r2 / t.sig2[k];
+                    const double iq4 = 1.0 / (qq * qq * qq * qq);
+                    const double iq7 = iq4 / (qq * qq * qq);
+                    const double fr = (2 * iq7 - iq4) * t.coeff[k];
+                    c0 = ddx * fr;
+                    c1 = ddy * fr;
+                    c2 = ddz * fr;
+                    if constexpr (kEnergy) {
+                        const double iq6 = 1.0 / (qq * qq * qq);
+                        term_e = 0.5 * (t.e4[k] * iq6 * (iq6 - 1) - t.shift[k]);
+                    }
+                } else {
+                    const float s2 = __builtin_amdgcn_rcpf((float)r2 * t.inv_sig2f[k]);
+                    const float s4 = s2 * s2;
+                    const float s6 = s4 * s2;
+                    const float s8 = s4 * s4;
+                    frd = (double)(s8 * __builtin_fmaf(2.0f, s6, -1.0f) * t.coeff_f[k]);
+                    if constexpr (kEnergy) term_e = 0.5 * ((double)(t.e4_f[k] * s6 * (s6 - 1.0f)) - t.shift[k]);
+                }
+            }
+        }
+        uint64_t m = __ballot(in);
+        while (m) {
+            const int l = __builtin_ctzll(m);
+            m &= m - 1;
+            if constexpr (kFp64) {
+                fx += lane_double(c0, l);
+                fy += lane_double(c1, l);
+                fz += lane_double(c2, l);
+            } else {
+                const double f = lane_double(frd, l);
+                fx = __builtin_fma(f, lane_double(ddx, l), fx);
+                fy = __builtin_fma(f, lane_double(ddy, l), fy);
+                fz = __builtin_fma(f, lane_double(ddz, l), fz);
+            }
+            if constexpr (kEnergy) e += lane_double(term_e, l);
+        }
+    }
+    if (lane == 0) {
+        st.f[0][s] = fx;
+        st.f[1][s] = fy;
+        st.f[2][s] = fz;
+        if constexpr (kEnergy) e_atom[s] = e;
+    }
+}
+
+// below this many atoms a walled box takes the wavefront-per-atom kernel
+constexpr int32_t kWaveAtoms = 32768;
+
 }  // namespace
 
 hipError_t launch_lj(hipStream_t s, const DevState& st, const int32_t* cell_of, const GridView& g,
                      const LjTable& t, int32_t n, int32_t precision, double* energy_per_atom,
                      const int32_t* gate) {
     if (n <= 0) return hipSuccess;
-    dim3 grid((n + kBlock - 1) / kBlock);
     const bool fp64 = precision == MDX_PREC_FP64;
+    if (!g.periodic && n <= kWaveAtoms) {
+        dim3 wgrid((n + kWaveBlock / 64 - 1) / (kWaveBlock / 64));
+        if (fp64 && energy_per_atom)
+            k_lj_wave<true, true><<<wgrid, kWaveBlock, 0, s>>>(st, cell_of, g, t, n, energy_per_atom, gate);
+        else if (fp64)
+            k_lj_wave<true, false><<<wgrid, kWaveBlock, 0, s>>>(st, cell_of, g, t, n, nullptr, gate);
+        else if (energy_per_atom)
+            k_lj_wave<false, true><<<wgrid, kWaveBlock, 0, s>>>(st, cell_of, g, t, n, energy_per_atom, gate);
+        else
+            k_lj_wave<false, false><<<wgrid, kWaveBlock, 0, s>>>(st, cell_of, g, t, n, nullptr, gate);
+        return hipGetLastError();
+    }
+    dim3 grid((n + kBlock - 1) / kBlock);
     if (fp64 && energy_per_atom)
         k_lj<true, true><<<grid, kBlock, 0, s>>>(st, cell_of, g, t, n, energy_per_atom, gate);
     else if (fp64)

@@ -50,6 +50,12 @@ __device__ inline int32_t quad_excl(int32_t h, int32_t q, int32_t& total) {
     return incl - h;
 }
 
+// lane l's value of v in every lane (l uniform across the wavefront)
+__device__ inline double lane_double(double v, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
 // Per-atom SoA state (slot order).
 struct DevState {
     double* r[3];
@@ -251,7 +257,7 @@ hipError_t launch_bond_ref(hipStream_t s, const DevState& st, const int32_t* slo
                            const int32_t* mb_cell_of, const int32_t* mb_ids, const GridView& g,
                            const BondRefParams& p, int32_t n_bond, int32_t nlistsize,
                            int32_t precision, int32_t assign, int32_t* xi_j, double* xi_v,
-                           double* energy_per_atom, const int32_t* gate);
+                           double* energy_per_atom, const int32_t* gate, int64_t row_bound);
 
 // reduce.hip
 hipError_t launch_kinetics(hipStream_t s, const DevState& st, int32_t n, const MassTable& m,

@@ -554,7 +554,8 @@ int bond_forces(mdx_system* s, double* e_atom, const int32_t* gate, const DevSta
     const int assign = c.use_lj ? 0 : 1;  // BondOrderFuncs.h:325-336
     LAUNCH(s, "bond_forces",
            launch_bond_ref(s->stream, d, s->slot_of_id, s->mb_cell_of, s->mb_ids, s->mb.view, s->bp,
-                           c.n_bond, s->xi_cap, c.precision, assign, s->xi_j, s->xi_v, e_atom, gate));
+                           c.n_bond, s->xi_cap, c.precision, assign, s->xi_j, s->xi_v, e_atom, gate,
+                           std::min<int64_t>(27 * (int64_t)s->mb.largest, c.n_bond)));
     return MDX_OK;
 }
 

@@ -128,6 +128,40 @@ def test_bond_forces_walled(mixture, precision):
     assert rel_rms(f, fo) < tol
 
 
+def test_lj_forces_walled_thread_per_atom():
+    """33,000 Ne atoms: above the size where the walled LJ force runs one
+    wavefront per atom, so this covers the thread-per-atom kernel (same bar:
+    FP64 bit for bit)."""
+    pos4, vel = gas_mixture(0, 33000, seed=8)
+    with walled(pos4, 0, PREC_FP64, use_bond=False) as s:
+        s.set_state(pos4, vel)
+        _, box = s.ready_list()
+        s.build_list(LIST_LJ, box)
+        s.lj_forces()
+        f = s.get_state()["force"]
+    ref, ls = O.build_neighbor_list(pos4, 7.6, 6.8, box)
+    fo = O.lj_forces(pos4, ref, ls.nlargestsize, float(ls.rcutsq))
+    np.testing.assert_array_equal(f, fo)
+
+
+@pytest.mark.parametrize("precision", [PREC_FP64, PREC_MIXED])
+def test_bond_forces_walled_long_rows(precision):
+    """Molecules 2 A apart: the bond-order rows exceed the 256 entries one
+    wavefront stages, so the thread-per-atom bond kernels run."""
+    pos4, vel = gas_mixture(400, 0, seed=9, spacing=2.0)
+    nb = len(pos4)
+    with walled(pos4, nb, precision, use_lj=False) as s:
+        s.set_state(pos4, vel)
+        _, box = s.ready_list()
+        nls, _ = s.build_list(LIST_BOND, box)
+        assert nls > 256
+        s.bond_forces()
+        f = s.get_state()["force"]
+    mb, lsb = O.build_neighbor_list(pos4, 3.6, 3.0, box)
+    fo = O.mb_forces(pos4, mb, lsb.nlargestsize, float(lsb.rcutsq), f_in=np.zeros((nb, 3)), ljexist=False)
+    assert rel_rms(f, fo) < (1e-12 if precision == PREC_FP64 else 1e-5)
+
+
 def test_bond_only_assigns(mixture):
     pos4, vel = mixture
     nb = 60
