@@ -45,6 +45,7 @@ constexpr int kLanes = 4;                  // lanes per atom
 constexpr int kApp = kBlock / kLanes;      // atoms per pass
 constexpr int kChunk = 8;                  // 16-bit entries per 16-byte chunk
 constexpr int kScan = 8;                   // candidates per lane per scan step
+constexpr int kScanSlack = kLanes * kScan; // staged slots past the last atom the scan may read
 constexpr int kListBlock = 16;             // atoms per list block
 constexpr int kChunkRow = kListBlock * kLanes * kChunk;   // 16-bit entries in one chunk of a block (512)
 constexpr int kXcd = 8;
@@ -124,9 +125,9 @@ struct Pos {
 struct PosF {
     float* p;
     int32_t cap;
-    __device__ float x(int32_t i) const { return p[3 * i]; }
-    __device__ float y(int32_t i) const { return p[3 * i + 1]; }
-    __device__ float z(int32_t i) const { return p[3 * i + 2]; }
+    __device__ float x(uint32_t i) const { return p[3 * i]; }
+    __device__ float y(uint32_t i) const { return p[3 * i + 1]; }
+    __device__ float z(uint32_t i) const { return p[3 * i + 2]; }
     __device__ void set(int32_t i, double x, double y, double z) const {
         p[3 * i] = (float)x;
         p[3 * i + 1] = (float)y;
@@ -271,17 +272,30 @@ k_nlist_lds(SegGrid g, const double* __restrict__ rx, const double* __restrict__
             for (int32_t p0 = b; p0 < e; p0 += kLanes * kScan) {
                 // lane q tests candidates p0 + kLanes u + q (the list radius test
                 // may round differently from the reference's; the list only has to
-                // hold every pair that can come within rc before the next build)
+                // hold every pair that can come within rc before the next build).
+                // All 8 are read first, from one base address; reads past e land
+                // in the staging's slack (slots < cap + kScanSlack, so a 16-bit
+                // index) and are masked.
+                float cx8[kScan], cy8[kScan], cz8[kScan];
+                const float* pb = P.p + 3 * (uint32_t)(uint16_t)(p0 + q);
+#pragma unroll
+                for (int u = 0; u < kScan; u++) {
+                    cx8[u] = pb[3 * kLanes * u];
+                    cy8[u] = pb[3 * kLanes * u + 1];
+                    cz8[u] = pb[3 * kLanes * u + 2];
+                }
                 uint32_t mask = 0;
 #pragma unroll
                 for (int u = 0; u < kScan; u++) {
-                    const int32_t p = p0 + kLanes * u + q;
-                    const int32_t pp = p < e ? p : b;
-                    const float dx = xi - P.x(pp), dy = yi - P.y(pp), dz = zi - P.z(pp);
+                    const float dx = xi - cx8[u], dy = yi - cy8[u], dz = zi - cz8[u];
                     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    const bool hit = (p < e) & (p != self) & (r2 < rlf);
-                    mask |= (uint32_t)hit << u;
+                    mask |= (uint32_t)(r2 < rlf) << u;
                 }
+                // candidate u is p0 + kLanes u + q: keep u below the range end,
+                // drop the atom itself
+                const int32_t left = e - p0 - q, ds = self - p0 - q;
+                mask &= (1u << (left <= 0 ? 0 : min((left + kLanes - 1) / kLanes, kScan))) - 1u;
+                if (ds >= 0 && ds < kLanes * kScan && ds % kLanes == 0) mask &= ~(1u << (ds / kLanes));
                 // hits are numbered lane-major within the quad
                 const int32_t h = __popc(mask);
                 int32_t total;
@@ -527,7 +541,7 @@ hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v
     static const bool attr_set = (allow_lds(k_nlist_lds), true);  // once per process, thread-safe
     (void)attr_set;
     SegGrid g = make_grid(v, nseg);
-    const size_t dyn = 3 * (size_t)lds_atoms * sizeof(float);
+    const size_t dyn = 3 * ((size_t)lds_atoms + kScanSlack) * sizeof(float);
     k_nlist_lds<<<g.per_xcd * kXcd, kBlock, dyn, s>>>(g, st.r[0], st.r[1], st.r[2], rl2, n, cap, list, count, err,
                                                       lds_atoms);
     return hipGetLastError();
