@@ -17,10 +17,11 @@ def rel_rms(a, b):
 
 
 def make_batch():
-    """Four replicas: 256 atoms at rho 0.95 (L 6.46, all pairs), 500 at 0.80
-    (L 8.55, 3 cells), 256 at 0.7 (L 7.15, all pairs) and 864 at 0.9 (L 9.86, 3 cells)."""
+    """Five replicas: 256 atoms at rho 0.95 (L 6.46, all pairs), 500 at 0.80
+    (L 8.55, 3 cells), 256 at 0.7 (L 7.15, all pairs), 864 at 0.9 (L 9.86,
+    3 cells) and 6912 at 0.8 (L 20.5, 7 cells)."""
     rng = np.random.default_rng(8)
-    spec = [(4, 0.95, 1.5), (5, 0.80, 0.728), (4, 0.70, 2.0), (6, 0.90, 1.0)]
+    spec = [(4, 0.95, 1.5), (5, 0.80, 0.728), (4, 0.70, 2.0), (6, 0.90, 1.0), (12, 0.80, 1.2)]
     ns, Ls, ps, vs = [], [], [], []
     for k, rho, T in spec:
         p, L = initial.fcc(k, rho)
