@@ -501,7 +501,8 @@ def main():
     achieved = n * STEP_BYTES_PER_ATOM / (k_ms * 1e-3) / 1e9 if cnt else None
     step_gbs = value * STEP_BYTES_PER_ATOM / 1e9
     out = {
-        "metric": "atom-steps/s per GPU (LJ 1M, mixed precision)",
+        "metric": "atom-steps/s per GPU (LJ 1M, mixed precision)" if n == 1 << 20 else
+                  f"atom-steps/s per GPU (LJ {n} atoms, mixed precision)",
         "value": value,
         "unit": "atom-steps/s",
         "n_gpus": 1,
