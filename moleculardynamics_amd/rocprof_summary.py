@@ -14,6 +14,15 @@ that bench.py reads for its roofline record.
         --fetch F.db --write W.db --copy-fetch CF.db --copy-write CW.db \
         --copy-bytes 268435456 --kernel "k_step_lds<false, false, false>" \
         --units 1048576 --bytes-per-unit 312 --command "..."
+
+With --counters, tabulates every counter of one or more --pmc runs for the
+named kernels instead: per-instance rows summed per dispatch, then averaged
+over the dispatches, plus the ratios the counters give directly (the trace,
+if given, supplies the kernel duration for the VALU issue fraction).
+
+    python -m moleculardynamics_amd.rocprof_summary --counters P1 P2 \
+        --kernel k_nlist_lds --kernel "k_step_lds<false, false, false>" \
+        [--trace T.db]
 """
 from __future__ import annotations
 
@@ -65,20 +74,94 @@ def counter_means(con: sqlite3.Connection, counter: str):
     return {k: sum(v) / len(v) for k, v in vals.items()}
 
 
+def counter_totals(con: sqlite3.Connection):
+    """{kernel: {counter: mean over dispatches of the per-dispatch sum}}"""
+    per = defaultdict(lambda: defaultdict(float))
+    names = {}
+    for name, did, cn, v in con.execute("select name, dispatch_id, counter_name, counter_value from pmc_events"):
+        per[(did, cn)][None] += v
+        names[did] = short(name)
+    acc = defaultdict(lambda: defaultdict(list))
+    for (did, cn), d in per.items():
+        acc[names[did]][cn].append(d[None])
+    return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in acc.items()}
+
+
+# MI355X: 256 CUs x 4 SIMDs; a wave64 VALU instruction occupies a 16-lane SIMD
+# for 4 cycles; nominal engine clock 2.4 GHz
+SIMDS, VALU_CYCLES, CLOCK_HZ = 1024, 4, 2.4e9
+
+
+def counters_report(a) -> None:
+    merged = defaultdict(dict)
+    for path in a.counters:
+        for k, cs in counter_totals(_db(path)).items():
+            merged[k].update(cs)
+    durs = kernel_stats(_db(a.trace))[0] if a.trace else {}
+    print(f"# {a.title or 'Counters'}\n")
+    if a.command:
+        print("Commands (run from the repository root on one MI355X):\n")
+        for c in a.command.split(" ;; "):
+            print(f"    {c}")
+        print()
+    for k in a.kernel:
+        cs = merged.get(k)
+        if not cs:
+            raise SystemExit(f"kernel {k!r} not in the counter runs")
+        print(f"## `{k}`\n")
+        print("| counter | per dispatch |")
+        print("|---|---:|")
+        for c in sorted(cs):
+            print(f"| {c} | {cs[c]:.0f} |")
+        waves = cs.get("SQ_WAVES")
+        lines = []
+        if waves:
+            for c, what in (("SQ_INSTS_VALU", "VALU instructions"), ("SQ_INSTS_LDS", "LDS instructions"),
+                            ("SQ_INSTS_VMEM_RD", "vector memory reads"), ("SQ_INSTS_SALU", "scalar instructions"),
+                            ("TCP_TOTAL_CACHE_ACCESSES_sum", "vector L1 accesses")):
+                if c in cs:
+                    lines.append(f"- {what} per wave: {cs[c] / waves:.0f}")
+        if "SQ_WAIT_ANY" in cs and "SQ_WAVE_CYCLES" in cs:
+            lines.append(f"- waves waiting (SQ_WAIT_ANY / SQ_WAVE_CYCLES): {cs['SQ_WAIT_ANY'] / cs['SQ_WAVE_CYCLES']:.2f}")
+        if "TCC_HIT_sum" in cs and "TCC_MISS_sum" in cs:
+            lines.append(f"- L2 hit rate: {cs['TCC_HIT_sum'] / (cs['TCC_HIT_sum'] + cs['TCC_MISS_sum']):.2f}")
+        if "SQ_LDS_BANK_CONFLICT" in cs and "SQ_ACTIVE_INST_LDS" in cs:
+            lines.append(f"- LDS bank-conflict cycles per active LDS cycle: "
+                         f"{cs['SQ_LDS_BANK_CONFLICT'] / cs['SQ_ACTIVE_INST_LDS']:.2f}")
+        if "SQ_INSTS_VALU" in cs and k in durs:
+            d = durs[k]
+            dur_s = sum(d) / len(d) / 1e9
+            frac = cs["SQ_INSTS_VALU"] / SIMDS * VALU_CYCLES / (dur_s * CLOCK_HZ)
+            lines.append(f"- VALU issue fraction: SQ_INSTS_VALU / {SIMDS} SIMDs x {VALU_CYCLES} cycles / "
+                         f"({dur_s * 1e6:.1f} us x {CLOCK_HZ / 1e9:.1f} GHz) = {frac:.2f}")
+        if lines:
+            print()
+            print("\n".join(lines))
+        print()
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--trace", required=True)
+    ap.add_argument("--counters", nargs="+", help="--pmc result folders to tabulate (counters mode)")
+    ap.add_argument("--trace", required=False)
     ap.add_argument("--fetch")
     ap.add_argument("--write")
     ap.add_argument("--copy-fetch")
     ap.add_argument("--copy-write")
     ap.add_argument("--copy-bytes", type=int, default=0)
-    ap.add_argument("--kernel", required=True, help="short name of the dominant kernel")
-    ap.add_argument("--units", type=int, required=True, help="work units (atoms) per launch")
-    ap.add_argument("--bytes-per-unit", type=float, required=True)
+    ap.add_argument("--kernel", action="append", required=True,
+                    help="short name of the dominant kernel (counters mode: repeat for several)")
+    ap.add_argument("--units", type=int, help="work units (atoms) per launch")
+    ap.add_argument("--bytes-per-unit", type=float)
     ap.add_argument("--command", default="")
     ap.add_argument("--title", default="")
     a = ap.parse_args()
+    if a.counters:
+        counters_report(a)
+        return
+    if not a.trace or a.units is None or a.bytes_per_unit is None:
+        ap.error("--trace, --units and --bytes-per-unit are required without --counters")
+    a.kernel = a.kernel[0]
 
     rows, meta = kernel_stats(_db(a.trace))
     total = sum(sum(v) for v in rows.values())
