@@ -332,20 +332,24 @@ int read_ints(mdx_system* s, int32_t* host, const int32_t* dev, int count) {
     return MDX_OK;
 }
 
-// reference list sizes: nlistsize = min(27 * largest, n - 1) (NeighborList.cpp:88-90)
+// reference list sizes: nlistsize = min(27 * largest, n - 1) (NeighborList.cpp:88-90).
+// One read-back brings the binning's out-of-box flag (dint[1]; an atom outside
+// the grid is counted in cell 0, so the kernels before stay in bounds) with
+// the two sizes.
 int finish_sizes(mdx_system* s, Grid& g, const int32_t* cell_by_item, int32_t items) {
     HIPCHK(s, hipMemsetAsync(s->dint + 2, 0, 2 * sizeof(int32_t), s->stream));
     LAUNCH(s, "list_max_count", launch_max_count(s->stream, g.count, g.ncell, s->dint + 2));
     LAUNCH(s, "list_row_lengths", launch_row_lengths(s->stream, g.view, cell_by_item, items, s->dint + 3));
-    int32_t h[2];
-    int rc = read_ints(s, h, s->dint + 2, 2);
+    int32_t h[3];
+    int rc = read_ints(s, h, s->dint + 1, 3);
     if (rc) return rc;
-    g.largest = h[0];
-    int64_t nls = 27 * (int64_t)h[0];
+    if (h[0]) return fail(s, MDX_ERR_ARG, "an atom lies outside the list box (box_size_list too small)");
+    g.largest = h[1];
+    int64_t nls = 27 * (int64_t)h[1];
     if (nls > items - 1) nls = items - 1;
     if (nls < 0) nls = 0;
     g.nlistsize = (int32_t)nls;
-    g.nlargestsize = h[1];
+    g.nlargestsize = h[2];
     return MDX_OK;
 }
 
@@ -379,9 +383,6 @@ int build_primary(mdx_system* s, Grid& g, float rs, double box_size_list, bool s
         // counts = start[c+1] - start[c]; recompute by binning again
         LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
         LAUNCH(s, "cells_bin", launch_bin(st, s->st, nullptr, n, g.view, s->cell_by_id, g.count, s->dint + 1));
-        int32_t e;
-        if ((rc = read_ints(s, &e, s->dint + 1, 1))) return rc;
-        if (e) return fail(s, MDX_ERR_ARG, "an atom lies outside the list box (box_size_list too small)");
         return finish_sizes(s, g, s->cell_of, n);
     }
     return MDX_OK;
@@ -402,10 +403,7 @@ int build_bond_walled(mdx_system* s, double box_size_list) {
     LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
     LAUNCH(s, "cells_scatter", launch_scatter(st, s->mb_cell_of, g.start, g.count, s->cell_by_id, nb));
     LAUNCH(s, "cells_sort", launch_sort_cells(st, g.start, g.ncell, s->cell_by_id, s->mb_ids, nullptr, nb));
-    int32_t e;
-    if ((rc = read_ints(s, &e, s->dint + 1, 1))) return rc;
-    if (e) return fail(s, MDX_ERR_ARG, "an atom lies outside the list box (box_size_list too small)");
-    // rebuild the histogram for the sizes
+    // rebuild the histogram for the sizes (the out-of-box flag is read with them)
     LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
     LAUNCH(s, "cells_bin", launch_bin(st, s->st, s->slot_of_id, nb, g.view, s->cell_by_id, g.count, s->dint + 1));
     if ((rc = finish_sizes(s, g, s->mb_cell_of, nb))) return rc;
