@@ -155,9 +155,11 @@ bool run_member(mdx_system* s, const Configuration& c, int n, double boxSize, Me
     return true;
 }
 
+}  // namespace
+
 // The sweep's vxcm values, by the reference's float loop (Host.cpp:75); a
 // step that is not positive gives one member instead of looping forever.
-std::vector<float> sweep_values(const Configuration& c) {
+std::vector<float> sweepValues(const Configuration& c) {
     std::vector<float> v;
     for (float vxcm = c.vxcmFrom; vxcm <= c.vxcmTo; vxcm += c.vxcmStep) {
         v.push_back(vxcm);
@@ -166,14 +168,12 @@ std::vector<float> sweep_values(const Configuration& c) {
     return v;
 }
 
-}  // namespace
-
 int hostMain(Configuration* c) {
     const int n = c->LennardJonesParticles + c->ManyBodyParticles;
     const double boxSize = std::pow((double)n, 1.0 / 3.0) * 52.8;  // Host.cpp:31
     c->energyLoss = std::sqrt(c->energyLoss);  // Host.cpp:73 (float sqrt)
     const mdx_config mc = make_config(*c, n, boxSize);
-    const std::vector<float> values = sweep_values(*c);
+    const std::vector<float> values = sweepValues(*c);
     const int count = (int)values.size();
     const int workers = std::max(1, std::min(c->SweepStreams, count));
     const Configuration& cfg = *c;

@@ -28,7 +28,8 @@ std::string to_json(const mdxh::Configuration& c) {
       << ",\"vxcmFrom\":" << c.vxcmFrom << ",\"vxcmTo\":" << c.vxcmTo << ",\"vxcmStep\":" << c.vxcmStep
       << ",\"energyLoss\":" << c.energyLoss << ",\"animts\":" << c.animts << ",\"Input\":\"" << c.Input
       << "\",\"Output\":\"" << c.Output << "\",\"Filename\":\"" << c.Filename << "\",\"AnimFile\":\"" << c.AnimFile
-      << "\",\"BoxLength\":" << c.BoxLength << ",\"Precision\":" << c.Precision << "}";
+      << "\",\"BoxLength\":" << c.BoxLength << ",\"Precision\":" << c.Precision
+      << ",\"SweepStreams\":" << c.SweepStreams << "}";
     return j.str();
 }
 
@@ -45,6 +46,18 @@ int mdxh_read_configuration(const char* path, char* out, int cap) {
         return 1;
     }
     return put(to_json(c), out, cap);
+}
+
+// the vxcm sweep's member values for from/to/step; returns the count (values
+// beyond cap are not written)
+int mdxh_sweep_values(float from, float to, float step, float* out, int cap) {
+    mdxh::Configuration c;
+    c.vxcmFrom = from;
+    c.vxcmTo = to;
+    c.vxcmStep = step;
+    const std::vector<float> v = mdxh::sweepValues(c);
+    for (int i = 0; i < (int)v.size() && i < cap; i++) out[i] = v[i];
+    return (int)v.size();
 }
 
 // ConfigParser on text: returns 0 and the value of `key` as a string, 1 when missing, 2 on parse error

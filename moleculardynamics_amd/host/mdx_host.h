@@ -86,6 +86,9 @@ bool writeResults(const std::vector<Results>& res, const Configuration& cfg, flo
 // "<stem>(vxcm-<v>).<ext>" as the reference's stringstream prints a float
 std::string vxcmName(const std::string& stem, float vxcm, const char* ext);
 
+// the vxcm values of the sweep (cpu/cpumd/Host.cpp:75, float accumulation);
+// one value when the step is not positive
+std::vector<float> sweepValues(const Configuration& cfg);
 // cpu/cpumd/Host.cpp:14-240 on the engine: returns the process exit status.
 int hostMain(Configuration* cfg);
 

@@ -30,6 +30,7 @@ def host():
     L.mdxh_config_lookup.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]
     L.mdxh_write_output.argtypes = [C.c_char_p, C.c_int, C.c_void_p, C.c_void_p]
     L.mdxh_read_input.argtypes = [C.c_char_p, C.c_int, C.c_void_p, C.c_void_p]
+    L.mdxh_sweep_values.argtypes = [C.c_float, C.c_float, C.c_float, C.c_void_p, C.c_int]
     L.mdxh_write_results.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, C.c_float,
                                      C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_void_p, C.c_int,
                                      C.c_void_p, C.c_int]
@@ -76,6 +77,37 @@ def test_read_configuration_all_keys(host, tmp_path):
     assert c["useLennardJones"] == 1 and c["Fallback"] == 1 and c["animts"] == -1
     assert c["vxcmTo"] == pytest.approx(0.02) and c["Input"] == "in.txt" and c["AnimFile"] == "anim"
     assert c["BoxLength"] == pytest.approx(107.49) and c["Precision"] == 0
+    assert c["SweepStreams"] == 4                  # the default: all of HIP's 4 hardware queues
+
+
+def test_sweep_streams_key(host, tmp_path):
+    p = tmp_path / "default.cfg"
+    write_cfg(p, {**KEYS, "SweepStreams": "2"})
+    buf = C.create_string_buffer(8192)
+    assert host.mdxh_read_configuration(str(p).encode(), buf, 8192) == 0
+    assert json.loads(buf.value.decode())["SweepStreams"] == 2
+
+
+def float32_sweep(a, b, step):
+    """cpu/cpumd/Host.cpp:75: for (float vxcm = from; vxcm <= to; vxcm += step)"""
+    a, b, step = np.float32(a), np.float32(b), np.float32(step)
+    out, v = [], a
+    while v <= b:
+        out.append(float(v))
+        if not step > 0:
+            break
+        v = np.float32(v + step)
+    return out
+
+
+@pytest.mark.parametrize("a,b,step", [(0, 0.003, 0.0015), (0, 0.3, 0.1), (-0.05, 0.05, 0.01), (0.2, 0.1, 0.05),
+                                      (0.1, 0.1, 0.0), (0, 1, 0.07)])
+def test_sweep_members_follow_the_float_loop(host, a, b, step):
+    out = np.zeros(64, dtype=np.float32)
+    k = host.mdxh_sweep_values(a, b, step, out.ctypes.data, len(out))
+    want = float32_sweep(a, b, step)
+    assert k == len(want)
+    np.testing.assert_array_equal(out[:k], np.array(want, dtype=np.float32))
 
 
 @pytest.mark.parametrize("drop", ["DT", "AnimationData", "Many Body Particles"])
