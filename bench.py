@@ -8,7 +8,7 @@ predictor-corrector, dt = 0.005.  One step = predictor, [cell-list rebuild
 when due], LJ forces, corrector (fused with the next predictor).
 
 Prints one JSON line (see the fields below).  --gpus > 1 is not supported:
-this build runs on one GPU.
+the engine runs on one GPU.
 
 --workload tersoff measures the bond-order configuration instead: Tersoff
 silicon (R = 2.7 A, D = 0.3 A), 262,144 atoms of diamond (32^3 cells of
@@ -459,7 +459,7 @@ def bench_replicas(a) -> None:
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "the engine runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
     if a.copy_bytes:
         copy_calibration(a.copy_bytes)
