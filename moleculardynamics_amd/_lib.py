@@ -80,7 +80,8 @@ def load() -> C.CDLL:
         "mdx_synchronize": [P], "mdx_set_forces": [P, P],
         "mdx_batch_create": [C.POINTER(MdxConfig), I32, P, P, C.POINTER(P)],
         "mdx_batch_destroy": [P], "mdx_batch_set_state": [P, P, P], "mdx_batch_get_state": [P, P, P, P],
-        "mdx_batch_run": [P, I32], "mdx_batch_build_counts": [P, P], "mdx_batch_energies": [P, P],
+        "mdx_batch_run": [P, I32], "mdx_batch_build_counts": [P, P], "mdx_batch_staged": [P, P],
+        "mdx_batch_energies": [P, P],
         "mdx_batch_set_timing": [P, I32], "mdx_batch_kernel_times": [P, P, P, P, I32, C.POINTER(I32)],
         "mdx_batch_synchronize": [P],
         "mdx_set_surrogate": [P, P, I64], "mdx_surrogate_counts": [P, P],
@@ -317,6 +318,12 @@ class Batch:
         b = np.zeros(self.R, dtype=np.int64)
         self._chk("mdx_batch_build_counts", self.lib.mdx_batch_build_counts(self.h, _ptr(b)))
         return b
+
+    def staged(self):
+        """True for the replicas on the LDS-staged path (fixed at creation)."""
+        s = np.zeros(self.R, dtype=np.int32)
+        self._chk("mdx_batch_staged", self.lib.mdx_batch_staged(self.h, _ptr(s)))
+        return s.astype(bool)
 
     def energies(self):
         e = np.zeros((self.R, 2))

@@ -100,6 +100,51 @@ __device__ inline int32_t seg_geometry(const SegGrid& g, int32_t& cx, int32_t& c
     return seg;
 }
 
+// Batched replicas (replicas.hip): segment b covers cells z0 = seg[4b+2] to
+// z1 = seg[4b+3] (exclusive) of z-column seg[4b+1] of replica seg[4b]; the
+// replica's cell starts are global slots.  The step kernel takes every
+// segment, the list build the work list wl[0 .. *wcount) of the segments of
+// the replicas that rebuild.
+struct RepGeo {
+    const int32_t* seg;
+    int32_t nblocks;         // segments in the table
+    const int32_t* nc;       // cells per axis of each replica (>= 3 here)
+    const double* L;
+    const int32_t* cstart;   // per replica nc^3 + 1 global slots, at coff[r]
+    const int32_t* coff;
+    const int32_t* wl;       // list build: segment numbers
+    const int32_t* wcount;
+};
+
+// the step kernel's segment of this block: the 8 XCDs take contiguous eighths
+// of the table (a replica's segments share their neighbour columns in L2)
+__device__ inline int32_t rep_block(const RepGeo& rg) {
+    const int32_t per = (rg.nblocks + kXcd - 1) / kXcd;
+    const int32_t b = (int32_t)(blockIdx.x % kXcd) * per + (int32_t)(blockIdx.x / kXcd);
+    return b < rg.nblocks ? b : -1;
+}
+
+__device__ inline int32_t rep_geometry(const RepGeo& rg, int32_t b, SegGrid& g, int32_t& cx, int32_t& cy,
+                                       int32_t& z0, int32_t& z1) {
+    const int4 sg = reinterpret_cast<const int4*>(rg.seg)[b];
+    const int32_t r = sg.x;
+    const int32_t m = rg.nc[r], col = sg.y;
+    g.n[0] = g.n[1] = g.n[2] = m;
+    g.nseg = 1;
+    g.total = m * m;
+    g.per_xcd = 0;
+    g.rnseg = 1.0f;
+    g.rny = 1.0f / (float)m;
+    g.small = 1;
+    g.L = rg.L[r];
+    g.start = rg.cstart + rg.coff[r];
+    cx = col / m;
+    cy = col - cx * m;
+    z0 = sg.z;
+    z1 = sg.w;
+    return r;
+}
+
 // Staged FP64 positions of up to `cap` atoms, x y z side by side (24 bytes an
 // atom, so one address serves the three reads), addressed through a pointer
 // derived from the __shared__ array itself so accesses stay ds_*.  The slot
@@ -236,15 +281,14 @@ __device__ inline int32_t home_cell(const Stage& S, int32_t qc) {
     return o;
 }
 
-__global__ void __launch_bounds__(kBlock, 8)
-k_nlist_lds(SegGrid g, const double* __restrict__ rx, const double* __restrict__ ry,
-            const double* __restrict__ rz, double rl2, int32_t n, int32_t cap, uint16_t* __restrict__ list,
-            int32_t* __restrict__ count, int32_t* __restrict__ err, int32_t lds_cap) {
+// the lists of one segment's home atoms (the whole block)
+__device__ __forceinline__ void nlist_segment(Stage& S, double* dyn, const SegGrid& g, int32_t cx, int32_t cy,
+                                              int32_t z0, int32_t z1, const double* __restrict__ rx,
+                                              const double* __restrict__ ry, const double* __restrict__ rz,
+                                              double rl2, int32_t cap, uint16_t* __restrict__ list,
+                                              int32_t* __restrict__ count, int32_t* __restrict__ err,
+                                              int32_t lds_cap) {
     const int32_t nchunk = cap / (kLanes * kChunk) + 1;   // chunks per lane with the spare
-    __shared__ Stage S;
-    extern __shared__ double dyn[];
-    int32_t cx, cy, z0, z1;
-    if (seg_geometry(g, cx, cy, z0, z1) < 0) return;
     const PosF P{reinterpret_cast<float*>(dyn), lds_cap};
     if (!stage(S, P, g, cx, cy, z0, z1, rx, ry, rz, cx * (g.L / g.n[0]), cy * (g.L / g.n[1]), z0 * (g.L / g.n[2]))) {
         if (threadIdx.x == 0) atomicMax(&err[0], S.total + 1);
@@ -322,6 +366,31 @@ k_nlist_lds(SegGrid g, const double* __restrict__ rx, const double* __restrict__
     }
 }
 
+// The single box: one segment per block.  Replicas: a fixed grid over the work
+// list, each XCD taking a contiguous eighth of it.
+template <bool kRep>
+__global__ void __launch_bounds__(kBlock, 8)
+k_nlist_lds(SegGrid g, RepGeo rg, const double* __restrict__ rx, const double* __restrict__ ry,
+            const double* __restrict__ rz, double rl2, int32_t cap, uint16_t* __restrict__ list,
+            int32_t* __restrict__ count, int32_t* __restrict__ err, int32_t lds_cap) {
+    __shared__ Stage S;
+    extern __shared__ double dyn[];
+    int32_t cx, cy, z0, z1;
+    if constexpr (kRep) {
+        const int32_t nw = *rg.wcount;
+        const int32_t per_x = (int32_t)gridDim.x / kXcd, x = (int32_t)(blockIdx.x % kXcd);
+        const int32_t w1 = (int32_t)((int64_t)nw * (x + 1) / kXcd);
+        for (int32_t w = (int32_t)((int64_t)nw * x / kXcd) + (int32_t)(blockIdx.x / kXcd); w < w1; w += per_x) {
+            rep_geometry(rg, rg.wl[w], g, cx, cy, z0, z1);
+            nlist_segment(S, dyn, g, cx, cy, z0, z1, rx, ry, rz, rl2, cap, list, count, err, lds_cap);
+            __syncthreads();   // the staging is reused by the next segment
+        }
+    } else {
+        if (seg_geometry(g, cx, cy, z0, z1) < 0) return;
+        nlist_segment(S, dyn, g, cx, cy, z0, z1, rx, ry, rz, rl2, cap, list, count, err, lds_cap);
+    }
+}
+
 struct StepArgs {
     SegGrid g;
     DevState st;            // st.r = positions r(t) (read), Gear state v,a,b,c (read/write)
@@ -341,27 +410,40 @@ struct StepArgs {
     int64_t cs;                  // component stride: component d of a vector at [0] + d * cs
     int32_t n;
     int32_t lds_cap;             // staged atoms the dynamic LDS holds
+    RepGeo rg;                   // batched replicas: geometry,
+    unsigned long long* dmax;    // the replicas' largest displacement^2 bits of r(t+1),
+    int32_t* err;                // and a staging overflow word
 };
 
 __device__ inline double pick3(int32_t q, double a, double b, double c) { return q == 0 ? a : (q == 1 ? b : c); }
 
 // 8 waves per SIMD (64 VGPRs) for the mixed body; the FP64 body's divisions
 // need more registers, so it runs at 4
-template <bool kFp64, bool kFinal, bool kForceOut>
+template <bool kRep, bool kFp64, bool kFinal, bool kForceOut>
 __global__ void __launch_bounds__(kBlock, kFp64 ? 4 : 8)
 k_step_lds(StepArgs a) {
     __shared__ Stage S;
     extern __shared__ double dyn[];
-    const SegGrid& g = a.g;
+    SegGrid g = a.g;
     if (blockIdx.x == 0 && a.clear)
         for (int32_t t = threadIdx.x; t < 3 * kSpread; t += kBlock) a.clear[t] = 0;
-    int32_t cx, cy, z0, z1;
-    if (seg_geometry(g, cx, cy, z0, z1) < 0) return;
+    int32_t cx, cy, z0, z1, rep = 0;
+    if constexpr (kRep) {
+        const int32_t b = rep_block(a.rg);
+        if (b < 0) return;
+        rep = rep_geometry(a.rg, b, g, cx, cy, z0, z1);
+    } else if (seg_geometry(g, cx, cy, z0, z1) < 0) {
+        return;
+    }
     const Pos P{dyn, a.lds_cap};
-    unsigned long long* fl = a.flags + 3 * (blockIdx.x % kSpread);
+    unsigned long long* fl = kRep ? nullptr : a.flags + 3 * (blockIdx.x % kSpread);
     if (!stage(S, P, g, cx, cy, z0, z1, a.st.r[0], a.st.r[1], a.st.r[2])) {
-        // cannot happen after a successful list build on the same cells; report it as stale
-        if (threadIdx.x == 0) atomicAdd(&fl[2], 1ull << 32);
+        // cannot happen after a successful list build on the same cells: the
+        // single box reports it as stale, the replicas through their error word
+        if (threadIdx.x == 0) {
+            if constexpr (kRep) atomicMax(&a.err[0], S.total + 1);
+            else atomicAdd(&fl[2], 1ull << 32);
+        }
         return;
     }
     const int32_t cbase = (cx * g.n[1] + cy) * g.n[2];
@@ -471,7 +553,9 @@ k_step_lds(StepArgs a) {
         mv2max = fmax(mv2max, mv2);
     }
     // block reduction through LDS atomics (non-negative doubles order like
-    // their bits), then one spread global atomic per block
+    // their bits), then one spread global atomic per block (the single box)
+    // or one into the replica's displacement accumulator (the exact rebuild
+    // decision of replicas.hip; stale atoms cannot occur there)
     if (q == 0) {
         if (!kFinal) {
             atomicMax(&S.red[0], (unsigned long long)__double_as_longlong(d2max));
@@ -481,11 +565,15 @@ k_step_lds(StepArgs a) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        if (!kFinal) {
-            atomicMax(&fl[0], S.red[0]);
-            atomicMax(&fl[1], S.red[1]);
+        if constexpr (kRep) {
+            if (!kFinal) atomicMax(&a.dmax[rep], S.red[0]);
+        } else {
+            if (!kFinal) {
+                atomicMax(&fl[0], S.red[0]);
+                atomicMax(&fl[1], S.red[1]);
+            }
+            if (S.red[2] > 0) atomicAdd(&fl[2], S.red[2]);
         }
-        if (S.red[2] > 0) atomicAdd(&fl[2], S.red[2]);
     }
 }
 
@@ -538,12 +626,12 @@ hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v
     if (n <= 0) return hipSuccess;
     if (cap % (kLanes * kChunk) != 0) return hipErrorInvalidValue;
     if (lds_atoms < 1 || lds_atoms > kLdsAtoms) return hipErrorInvalidValue;
-    static const bool attr_set = (allow_lds(k_nlist_lds), true);  // once per process, thread-safe
+    static const bool attr_set = (allow_lds(k_nlist_lds<false>), true);  // once per process, thread-safe
     (void)attr_set;
     SegGrid g = make_grid(v, nseg);
     const size_t dyn = 3 * ((size_t)lds_atoms + kScanSlack) * sizeof(float);
-    k_nlist_lds<<<g.per_xcd * kXcd, kBlock, dyn, s>>>(g, st.r[0], st.r[1], st.r[2], rl2, n, cap, list, count, err,
-                                                      lds_atoms);
+    k_nlist_lds<false><<<g.per_xcd * kXcd, kBlock, dyn, s>>>(g, RepGeo{}, st.r[0], st.r[1], st.r[2], rl2, cap,
+                                                             list, count, err, lds_atoms);
     return hipGetLastError();
 }
 
@@ -554,19 +642,19 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
                            int32_t lds_atoms, uint64_t* clear_next) {
     if (n <= 0) return hipSuccess;
     static const bool attr_set = [] {  // once per process, thread-safe
-        allow_lds(k_step_lds<true, true, true>);
-        allow_lds(k_step_lds<true, true, false>);
-        allow_lds(k_step_lds<true, false, true>);
-        allow_lds(k_step_lds<true, false, false>);
-        allow_lds(k_step_lds<false, true, true>);
-        allow_lds(k_step_lds<false, true, false>);
-        allow_lds(k_step_lds<false, false, true>);
-        allow_lds(k_step_lds<false, false, false>);
+        allow_lds(k_step_lds<false, true, true, true>);
+        allow_lds(k_step_lds<false, true, true, false>);
+        allow_lds(k_step_lds<false, true, false, true>);
+        allow_lds(k_step_lds<false, true, false, false>);
+        allow_lds(k_step_lds<false, false, true, true>);
+        allow_lds(k_step_lds<false, false, true, false>);
+        allow_lds(k_step_lds<false, false, false, true>);
+        allow_lds(k_step_lds<false, false, false, false>);
         return true;
     }();
     (void)attr_set;
     if (lds_atoms < 1 || lds_atoms > kLdsAtoms) return hipErrorInvalidValue;
-    StepArgs a;
+    StepArgs a{};
     a.g = make_grid(v, nseg);
     a.st = st;
     for (int d = 0; d < 3; d++) { a.rn[d] = rn[d]; a.r0[d] = r0[d]; }
@@ -597,7 +685,7 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
     const size_t dyn = 3 * (size_t)lds_atoms * sizeof(double);
     dim3 grid(a.g.per_xcd * kXcd);
     const bool fp64 = precision == MDX_PREC_FP64;
-#define MDX_STEP(F64, FIN, FO) k_step_lds<F64, FIN, FO><<<grid, kBlock, dyn, s>>>(a)
+#define MDX_STEP(F64, FIN, FO) k_step_lds<false, F64, FIN, FO><<<grid, kBlock, dyn, s>>>(a)
     if (fp64) {
         if (final_step) { if (force_out) MDX_STEP(true, true, true); else MDX_STEP(true, true, false); }
         else { if (force_out) MDX_STEP(true, false, true); else MDX_STEP(true, false, false); }
@@ -606,6 +694,89 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
         else { if (force_out) MDX_STEP(false, false, true); else MDX_STEP(false, false, false); }
     }
 #undef MDX_STEP
+    return hipGetLastError();
+}
+
+// ---- batched replicas: one workgroup per z-segment of each staged replica ----
+
+namespace {
+RepGeo rep_geo(const RepSegs& rs, const int32_t* wl, const int32_t* wcount) {
+    return RepGeo{rs.seg, rs.nblocks, rs.nc, rs.L, rs.cstart, rs.coff, wl, wcount};
+}
+}  // namespace
+
+hipError_t launch_rep_nlist(hipStream_t s, const RepSegs& rs, const int32_t* wl, const int32_t* wcount,
+                            double* const r[3], double rl2, int32_t cap, uint16_t* list, int32_t* count,
+                            int32_t* err, int32_t lds_atoms) {
+    if (rs.nblocks <= 0) return hipSuccess;
+    if (cap % (kLanes * kChunk) != 0) return hipErrorInvalidValue;
+    if (lds_atoms < 1 || lds_atoms > kLdsAtoms) return hipErrorInvalidValue;
+    static const bool attr_set = (allow_lds(k_nlist_lds<true>), true);  // once per process, thread-safe
+    (void)attr_set;
+    const size_t dyn = 3 * ((size_t)lds_atoms + kScanSlack) * sizeof(float);
+    // a grid of resident blocks (4 a CU at the full staging), a multiple of 8
+    const int32_t all = (rs.nblocks + kXcd - 1) / kXcd * kXcd;
+    const int32_t grid = all < 1024 ? all : 1024;
+    k_nlist_lds<true><<<grid, kBlock, dyn, s>>>(SegGrid{}, rep_geo(rs, wl, wcount), r[0], r[1], r[2], rl2, cap, list,
+                                                count, err, lds_atoms);
+    return hipGetLastError();
+}
+
+hipError_t launch_rep_step(hipStream_t s, const RepSegs& rs, const DevState& st, double* const rn[3],
+                           double* const r0[3], const uint16_t* list, const int32_t* count, int32_t cap, double rc2,
+                           double sig2, double coeff, double mass, double scale, const GearCoeffs& gc, int64_t cs,
+                           int32_t precision, bool final_step, unsigned long long* dmax, int32_t* err,
+                           int32_t lds_atoms) {
+    if (rs.nblocks <= 0) return hipSuccess;
+    static const bool attr_set = [] {  // once per process, thread-safe
+        allow_lds(k_step_lds<true, true, true, true>);
+        allow_lds(k_step_lds<true, true, false, false>);
+        allow_lds(k_step_lds<true, false, true, true>);
+        allow_lds(k_step_lds<true, false, false, false>);
+        return true;
+    }();
+    (void)attr_set;
+    if (lds_atoms < 1 || lds_atoms > kLdsAtoms) return hipErrorInvalidValue;
+    if (cap % (kLanes * kChunk) != 0) return hipErrorInvalidValue;
+    for (int d = 1; d < 3; d++)
+        if (st.v[d] != st.v[0] + d * cs || st.r[d] != st.r[0] + d * cs || rn[d] != rn[0] + d * cs ||
+            r0[d] != r0[0] + d * cs || st.a[d] != st.a[0] + d * cs || st.b[d] != st.b[0] + d * cs ||
+            st.c[d] != st.c[0] + d * cs || st.f[d] != st.f[0] + d * cs)
+            return hipErrorInvalidValue;
+    StepArgs a{};
+    a.st = st;
+    for (int d = 0; d < 3; d++) { a.rn[d] = rn[d]; a.r0[d] = r0[d]; }
+    a.list = list;
+    a.count = count;
+    a.nchunk = cap / (kLanes * kChunk) + 1;
+    a.rc2 = rc2;
+    // coeff = 24 eps / sig^2: 48 eps sig^12 = 2 coeff sig^14, -24 eps sig^6 = -coeff sig^8
+    const double s4 = sig2 * sig2 * sig2 * sig2;
+    a.f12 = (float)(2.0 * coeff * s4 * sig2 * sig2 * sig2);
+    a.f6 = (float)(-coeff * s4);
+    a.sig2 = sig2;
+    a.coeff = coeff;
+    a.mass = mass;
+    a.scale = scale;
+    a.gc = gc;
+    a.hard2 = INFINITY;   // the replicas' rebuild decision is exact
+    a.cs = cs;
+    a.lds_cap = lds_atoms;
+    a.rg = rep_geo(rs, nullptr, nullptr);
+    a.dmax = dmax;
+    a.err = err;
+    const size_t dyn = 3 * (size_t)lds_atoms * sizeof(double);
+    const bool fp64 = precision == MDX_PREC_FP64;
+    const dim3 grid((rs.nblocks + kXcd - 1) / kXcd * kXcd);   // rep_block: XCD-major order
+    // a step: corrector and the next predictor; the last step of a run:
+    // corrector only, with the force written for read-back
+    if (fp64) {
+        if (final_step) k_step_lds<true, true, true, true><<<grid, kBlock, dyn, s>>>(a);
+        else k_step_lds<true, true, false, false><<<grid, kBlock, dyn, s>>>(a);
+    } else {
+        if (final_step) k_step_lds<true, false, true, true><<<grid, kBlock, dyn, s>>>(a);
+        else k_step_lds<true, false, false, false><<<grid, kBlock, dyn, s>>>(a);
+    }
     return hipGetLastError();
 }
 

@@ -252,6 +252,28 @@ hipError_t launch_sg_forces(hipStream_t s, const DevState& st, double L, const T
                             const float* vec, float4* w, double* e_atom, int32_t* nflag, double* const rref[3],
                             double hard2, uint64_t* flags, const DevState& out);
 
+// lj_fused.hip, batched replicas: block b serves cells seg[4b+2] .. seg[4b+3]
+// (exclusive) of z-column seg[4b+1] of replica seg[4b] (every replica here has
+// nc >= 3 cells per axis); cstart holds the replicas' cell starts as global
+// slots, nc^3 + 1 per replica at coff[r]
+struct RepSegs {
+    const int32_t* seg;   // 4 per block, 16-byte aligned
+    int32_t nblocks;
+    const int32_t* nc;
+    const double* L;
+    const int32_t* cstart;
+    const int32_t* coff;
+};
+// the lists of the segments wl[0 .. *wcount) (a fixed grid loops over them)
+hipError_t launch_rep_nlist(hipStream_t s, const RepSegs& rs, const int32_t* wl, const int32_t* wcount,
+                            double* const r[3], double rl2, int32_t cap, uint16_t* list, int32_t* count,
+                            int32_t* err, int32_t lds_atoms);
+hipError_t launch_rep_step(hipStream_t s, const RepSegs& rs, const DevState& st, double* const rn[3],
+                           double* const r0[3], const uint16_t* list, const int32_t* count, int32_t cap, double rc2,
+                           double sig2, double coeff, double mass, double scale, const GearCoeffs& gc, int64_t cs,
+                           int32_t precision, bool final_step, unsigned long long* dmax, int32_t* err,
+                           int32_t lds_atoms);
+
 // bond_ref.hip
 hipError_t launch_bond_ref(hipStream_t s, const DevState& st, const int32_t* slot_of_id,
                            const int32_t* mb_cell_of, const int32_t* mb_ids, const GridView& g,

@@ -20,6 +20,14 @@
 //    step t and predictor of step t+1; positions are double-buffered; the
 //    largest displacement of r(t+1) from the list-build positions is
 //    max-reduced per replica;
+//  * staged replicas (at least 3 cells per axis and a neighbourhood that fits
+//    LDS) take the single box's LDS path instead: at each rebuild their state
+//    is permuted into the new cell order with positions wrapped into the box
+//    (k_rep_perm_*), their lists are built and their steps run by
+//    k_nlist_lds / k_step_lds of lj_fused.hip, one workgroup per z-segment of
+//    a cell column; an atom-id map keeps the caller's order.  They occupy the
+//    slots below ubeg, so the gather path's kernels (k_rep_list after the
+//    first build, k_rep_step) run over the slots from ubeg on only;
 //  * k_rep_decide: a replica rebuilds before the next step exactly when that
 //    displacement reached (rs - rc)/2 -- the decision reads the positions the
 //    next step will use, so it needs no margin for lag and no host round trip.
@@ -42,6 +50,9 @@ constexpr int kSortBlock = 1024;
 constexpr int kMaxNc = 10;                               // cells per axis at most (larger cells stay correct)
 constexpr int kMaxCells = kMaxNc * kMaxNc * kMaxNc;
 constexpr int kMaxReplica = 16384;                       // atoms per replica (the LDS sort holds them all)
+// staged atoms a segment of a staged replica aims at: the step kernel's LDS,
+// 1.2 x the largest staging, then stays within a quarter of a CU's 160 KiB
+constexpr double kStageAim = 1250.0;
 
 struct RepView {
     const int32_t* rep_of;     // replica of each atom slot (-1: padding)
@@ -55,89 +66,192 @@ struct RepView {
 
 __device__ inline double mi(double d, double L, double invL) { return d - L * rint(d * invL); }
 
-__device__ inline double wrap(double x, double L) { return x - L * floor(x / L); }
+// into [0, L) up to rounding at the ends (cell_index clamps those)
+__device__ inline double wrap(double x, double L, double invL) { return x - L * floor(x * invL); }
 
-__device__ inline int32_t cell_index(double y, double L, int32_t nc) {   // y already wrapped into [0, L)
-    int32_t c = (int32_t)(y * nc / L);
+__device__ inline int32_t cell_index(double y, double cells_per_length, int32_t nc) {   // y wrapped
+    int32_t c = (int32_t)(y * cells_per_length);
     return c >= nc ? nc - 1 : (c < 0 ? 0 : c);
 }
 
 // One workgroup per replica that needs a new list: counting sort of its atoms
 // into cells (histogram, scan and scatter in LDS), each cell then ordered by
-// atom index in LDS, and a cell-ordered copy (x, y, z wrapped into [0, L) as
-// FP32, atom index in w).  tpos is scratch for the unsorted copy.
+// atom index (an atom's rank among its cell's atoms), and a cell-ordered copy
+// (x, y, z wrapped into [0, L) as FP32, atom index in w).  tpos is scratch for
+// the unsorted copy.
 __global__ void __launch_bounds__(kSortBlock)
 k_rep_sort(RepView v, const double* __restrict__ rx, const double* __restrict__ ry, const double* __restrict__ rz,
-           int32_t* __restrict__ cell_of, int32_t* __restrict__ cstart, int32_t* __restrict__ sidx,
-           float4* __restrict__ tpos, float4* __restrict__ spos) {
+           int32_t* __restrict__ cell_of, int32_t* __restrict__ cstart, int32_t* __restrict__ cstart_g,
+           int32_t* __restrict__ sidx, float4* __restrict__ tpos, float4* __restrict__ spos,
+           const uint8_t* __restrict__ staged) {
     const int32_t r = blockIdx.x;
     if (!v.rebuild[r]) return;
     __shared__ int32_t cnt[kMaxCells + 1];
     __shared__ int32_t beg[kMaxCells + 1];
-    __shared__ int32_t part[kSortBlock];
-    __shared__ int32_t lidx[kMaxReplica];
+    __shared__ int32_t wsum[kSortBlock / kWave];
+    __shared__ int32_t lidx[kMaxReplica];   // atoms in scatter order
+    __shared__ int32_t sorted[kMaxReplica]; // atoms in cell order, ascending index within a cell
     const int32_t tid = threadIdx.x;
     const int32_t n = v.nat[r], base = v.off[r], nc = v.nc[r], cb = v.coff[r];
     const int32_t ncl = nc ? nc * nc * nc : 1;
-    const double L = v.L[r];
+    const double L = v.L[r], invL = 1.0 / L, cpl = nc * invL;
     for (int32_t c = tid; c < ncl; c += kSortBlock) cnt[c] = 0;
     __syncthreads();
     for (int32_t i = tid; i < n; i += kSortBlock) {
-        const double x = wrap(rx[base + i], L), y = wrap(ry[base + i], L), z = wrap(rz[base + i], L);
-        const int32_t c = nc ? (cell_index(x, L, nc) * nc + cell_index(y, L, nc)) * nc + cell_index(z, L, nc) : 0;
+        const double x = wrap(rx[base + i], L, invL), y = wrap(ry[base + i], L, invL), z = wrap(rz[base + i], L, invL);
+        const int32_t c = nc ? (cell_index(x, cpl, nc) * nc + cell_index(y, cpl, nc)) * nc + cell_index(z, cpl, nc) : 0;
         cell_of[base + i] = c;
         tpos[base + i] = make_float4((float)x, (float)y, (float)z, __int_as_float(i));
         atomicAdd(&cnt[c], 1);
     }
     __syncthreads();
-    // exclusive scan over the cells: contiguous chunks per thread, then the
-    // chunk sums by one thread
+    // exclusive scan over the cells: contiguous chunks per thread, the chunk
+    // sums scanned within each wave (shuffles) and across the waves (LDS)
     const int32_t chunk = (ncl + kSortBlock - 1) / kSortBlock;
     const int32_t lo = tid * chunk < ncl ? tid * chunk : ncl, hi = lo + chunk < ncl ? lo + chunk : ncl;
     int32_t sum = 0;
     for (int32_t c = lo; c < hi; c++) sum += cnt[c];
-    part[tid] = sum;
-    __syncthreads();
-    if (tid == 0) {
-        int32_t run = 0;
-        for (int32_t t = 0; t < kSortBlock; t++) { const int32_t x = part[t]; part[t] = run; run += x; }
+    const int32_t lane = tid & (kWave - 1), wv = tid / kWave;
+    int32_t inc = sum;
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int32_t y = __shfl_up(inc, o, kWave);
+        if (lane >= o) inc += y;
     }
+    if (lane == kWave - 1) wsum[wv] = inc;
     __syncthreads();
-    int32_t run = part[tid];
+    int32_t run = inc - sum;
+    for (int32_t w = 0; w < wv; w++) run += wsum[w];
     for (int32_t c = lo; c < hi; c++) {
         const int32_t x = cnt[c];
         beg[c] = run;
         cnt[c] = run;   // becomes the fill cursor
         cstart[cb + c] = run;
+        cstart_g[cb + c] = base + run;
         run += x;
     }
-    if (tid == 0) { beg[ncl] = n; cstart[cb + ncl] = n; }
+    if (tid == 0) { beg[ncl] = n; cstart[cb + ncl] = n; cstart_g[cb + ncl] = base + n; }
     __syncthreads();
     if (nc == 0) {
         // one cell: ascending index order is the identity
-        for (int32_t i = tid; i < n; i += kSortBlock) lidx[i] = i;
+        for (int32_t i = tid; i < n; i += kSortBlock) sorted[i] = i;
     } else {
         for (int32_t i = tid; i < n; i += kSortBlock) lidx[atomicAdd(&cnt[cell_of[base + i]], 1)] = i;
-    }
-    __syncthreads();
-    for (int32_t c = tid; c < ncl && nc; c += kSortBlock) {   // insertion sort per cell (~20 atoms) in LDS
-        const int32_t b = beg[c], e = beg[c + 1];
-        for (int32_t p = b + 1; p < e; p++) {
-            const int32_t x = lidx[p];
-            int32_t q = p - 1;
-            while (q >= b && lidx[q] > x) { lidx[q + 1] = lidx[q]; q--; }
-            lidx[q + 1] = x;
+        __syncthreads();
+        // an atom's slot: its cell's start plus the number of the cell's atoms
+        // with a smaller index (independent reads, ~20-50 per atom)
+        for (int32_t p = tid; p < n; p += kSortBlock) {
+            const int32_t i = lidx[p], c = cell_of[base + i];
+            const int32_t b = beg[c], e = beg[c + 1];
+            int32_t rank = 0;
+            for (int32_t q = b; q < e; q++) rank += lidx[q] < i ? 1 : 0;
+            sorted[b + rank] = i;
         }
     }
     __syncthreads();
+    // a staged replica is permuted into this order next (k_rep_perm_*): its
+    // records name the new slots and its cells follow the new order
+    const bool perm = nc != 0 && staged[r];
     for (int32_t p = tid; p < n; p += kSortBlock) {
-        const int32_t i = lidx[p];
+        const int32_t i = sorted[p];
         sidx[base + p] = i;
-        spos[base + p] = tpos[base + i];
+        float4 w = tpos[base + i];
+        if (perm) w.w = __int_as_float(p);
+        spos[base + p] = w;
+    }
+    if (!perm) return;
+    constexpr int kPer = kMaxReplica / kSortBlock;
+    int32_t hold[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        const int32_t p = tid + k * kSortBlock;
+        if (p < n) hold[k] = cell_of[base + sorted[p]];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        const int32_t p = tid + k * kSortBlock;
+        if (p < n) cell_of[base + p] = hold[k];
     }
 }
 
-// Compact list of each atom of a flagged replica, a quad of 4 lanes per atom:
+// The segments of the staged replicas that rebuild, compacted in replica
+// order into wl[0 .. *count): the list build's work.  One workgroup takes the
+// replicas in rounds of kSortBlock (a block scan of their segment counts).
+__global__ void __launch_bounds__(kSortBlock)
+k_rep_worklist(int32_t R, const int32_t* __restrict__ rebuild, const int32_t* __restrict__ sbeg,
+               int32_t* __restrict__ wl, int32_t* __restrict__ count) {
+    __shared__ int32_t wsum[kSortBlock / kWave];
+    __shared__ int32_t carry;
+    const int32_t tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (int32_t r0 = 0; r0 < R; r0 += kSortBlock) {
+        const int32_t r = r0 + tid;
+        const int32_t m = r < R && rebuild[r] ? sbeg[r + 1] - sbeg[r] : 0;
+        int32_t inc = m;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int32_t y = __shfl_up(inc, o, kWave);
+            if (lane >= o) inc += y;
+        }
+        if (lane == kWave - 1) wsum[wv] = inc;
+        __syncthreads();
+        int32_t run = carry + inc - m;
+        for (int32_t w = 0; w < wv; w++) run += wsum[w];
+        for (int32_t k = 0; k < m; k++) wl[run + k] = sbeg[r] + k;
+        __syncthreads();   // carry and wsum read by every thread
+        if (tid == kSortBlock - 1) carry = run + m;
+        __syncthreads();
+    }
+    if (tid == 0) *count = carry;
+}
+
+// The state of a rebuilt staged replica into the new cell order: slot p
+// takes old slot sidx[p] (15 components: positions of the current buffer,
+// v, a, b, c), first gathered into tmp, then copied back with the positions
+// wrapped into the box; the list-build positions r0 and the atom-id map
+// follow.
+struct RepPerm {
+    double* comp[15];   // x y z of r, v, a, b, c
+    int32_t* atom;
+    double* r0[3];
+};
+
+__global__ void __launch_bounds__(kBlock)
+k_rep_perm_gather(RepView v, const uint8_t* __restrict__ staged, const int32_t* __restrict__ sidx, RepPerm pm,
+                  double* __restrict__ tmp, int32_t* __restrict__ tmp_atom, int32_t ntot) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntot) return;
+    const int32_t r = v.rep_of[i];
+    if (r < 0 || !v.rebuild[r] || !staged[r] || v.nc[r] == 0) return;
+    const int32_t src = v.off[r] + sidx[i];
+#pragma unroll
+    for (int k = 0; k < 15; k++) tmp[(int64_t)k * ntot + i] = pm.comp[k][src];
+    tmp_atom[i] = pm.atom[src];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_rep_perm_scatter(RepView v, const uint8_t* __restrict__ staged, RepPerm pm, const double* __restrict__ tmp,
+                   const int32_t* __restrict__ tmp_atom, int32_t ntot) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntot) return;
+    const int32_t r = v.rep_of[i];
+    if (r < 0 || !v.rebuild[r] || !staged[r] || v.nc[r] == 0) return;
+    // positions wrapped as k_rep_sort wrapped them to bin them: the staging
+    // adds each cell column's image shift to the stored value
+    const double L = v.L[r], invL = 1.0 / L;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double x = wrap(tmp[(int64_t)d * ntot + i], L, invL);
+        pm.comp[d][i] = x;
+        pm.r0[d][i] = x;
+    }
+#pragma unroll
+    for (int k = 3; k < 15; k++) pm.comp[k][i] = tmp[(int64_t)k * ntot + i];
+    pm.atom[i] = tmp_atom[i];
+}
+
+// Compact list of each atom of a flagged replica from slot `first` on, a quad
+// of 4 lanes per atom:
 // candidates from the 27 cells around it (or the whole replica), the lanes
 // testing interleaved candidates and numbering their hits lane-major, so the
 // list order is deterministic.  The radius test runs on the FP32 copy with rs
@@ -146,22 +260,24 @@ __global__ void __launch_bounds__(kBlock)
 k_rep_list(RepView v, const double* __restrict__ rx, const double* __restrict__ ry, const double* __restrict__ rz,
            double* __restrict__ r0x, double* __restrict__ r0y, double* __restrict__ r0z, int32_t ntot,
            const int32_t* __restrict__ cell_of, const int32_t* __restrict__ cstart, const float4* __restrict__ spos,
-           float rs2, int32_t cap, int32_t* __restrict__ list, int32_t* __restrict__ count, int32_t* __restrict__ err) {
+           float rs2, int32_t cap, int32_t* __restrict__ list, int32_t* __restrict__ count, int32_t* __restrict__ err,
+           const int32_t* __restrict__ gate, int32_t first, int32_t write_r0) {
     const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int32_t i = t >> 2, q = t & 3;
+    const int32_t i = first + (t >> 2), q = t & 3;
     if (i >= ntot) return;   // whole quads
     const int32_t r = v.rep_of[i];
-    if (r < 0 || !v.rebuild[r]) return;
+    if (r < 0 || !gate[r]) return;
     const int32_t base = v.off[r], nc = v.nc[r], cb = v.coff[r];
     const double L = v.L[r];
     const double x = rx[i], y = ry[i], z = rz[i];
-    if (q == 0) {
+    if (q == 0 && write_r0) {
         r0x[i] = x;
         r0y[i] = y;
         r0z[i] = z;
     }
     const float Lf = (float)L, iLf = 1.0f / Lf;
-    const float xi = (float)wrap(x, L), yi = (float)wrap(y, L), zi = (float)wrap(z, L);
+    const double invL = 1.0 / L;
+    const float xi = (float)wrap(x, L, invL), yi = (float)wrap(y, L, invL), zi = (float)wrap(z, L, invL);
     const int32_t li = i - base;
     int32_t k = 0;   // hits so far, the same in the 4 lanes
     auto scan = [&](int32_t c) {
@@ -229,10 +345,11 @@ struct RepStep {
     double mass, scale;
     GearCoeffs gc;
     unsigned long long* dmax;   // R: max displacement^2 bits of r(t+1)
+    int32_t first;              // k_rep_step: first slot (the staged replicas lie below it)
 };
 
 // kMode: 0 = corrector + next predictor, 1 = corrector only (last step),
-// 2 = forces and a = F/m only (start of a run).  A quad of 4 lanes per atom:
+// 2 = forces and a = F/m only (start of a run); atoms from slot a.first on.  A quad of 4 lanes per atom:
 // lane q sums list entries q, q + 4, ...; the quad's sums are added by DPP and
 // lane q < 3 integrates component q (the state arrays hold the 3 components of
 // a vector in one block of 3 x ntot).
@@ -241,7 +358,7 @@ __global__ void __launch_bounds__(kBlock)
 k_rep_step(RepStep a) {
     constexpr bool kFinal = kMode != 0;
     const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int32_t i = t >> 2, q = t & 3;
+    const int32_t i = a.first + (t >> 2), q = t & 3;
     const int32_t r = i < a.ntot ? a.v.rep_of[i] : -1;
     double d2 = 0.0;
     if (r >= 0) {
@@ -414,6 +531,7 @@ struct mdx_batch {
     int32_t R = 0, ntot = 0, ncell_tot = 0;
     hipStream_t stream = nullptr;
     std::vector<int32_t> off, nat, coff, nc;
+    std::vector<uint8_t> staged;   // the LDS path's replicas
     std::vector<double> L;
     std::vector<void*> allocs;
     int32_t *d_rep_of = nullptr, *d_off = nullptr, *d_nat = nullptr, *d_coff = nullptr, *d_nc = nullptr;
@@ -424,6 +542,24 @@ struct mdx_batch {
     double4* d_rq[2] = {nullptr, nullptr};   // position records of buffers A (r) and B (rn)
     int32_t *d_list = nullptr, *d_count = nullptr, *d_err = nullptr;
     int32_t cap = 0;
+    // staged replicas (the single box's LDS path, see the file header)
+    uint8_t* d_staged = nullptr;          // R flags
+    int32_t* d_staged_i = nullptr;        // the same as int32 (a list gate)
+    int32_t* d_seg = nullptr;             // (replica, column, z0, z1) per segment
+    int32_t* d_sbeg = nullptr;            // R + 1: first segment of each replica
+    int32_t* d_wl = nullptr;              // the list build's work list and its length
+    int32_t* d_wcount = nullptr;
+    int32_t nseg_blocks = 0;
+    int32_t* d_cstart_g = nullptr;        // cell starts as global slots
+    int32_t* d_atom = nullptr;            // replica-local atom id of each slot
+    double* d_ptmp = nullptr;             // permutation scratch, 15 x ntot
+    int32_t* d_patom = nullptr;
+    uint16_t* d_rs_list = nullptr;        // LDS-index lists of the staged replicas
+    int32_t* d_rs_count = nullptr;
+    int32_t* d_rs_err = nullptr;          // staging overflow, list overflow, largest staging
+    int32_t rs_cap = 0, rs_lds = 0;
+    int32_t n_unstaged = 0;               // replicas on the gather path,
+    int32_t ubeg = 0;                     // laid out from this slot on
     long long* d_builds = nullptr;
     unsigned long long* d_dmax = nullptr;
     double* r[3] = {}; double* rn[3] = {}; double* v[3] = {}; double* a[3] = {}; double* b[3] = {}; double* c[3] = {};
@@ -498,7 +634,23 @@ RepStep step_args(const mdx_batch* b, bool flip) {
     s.scale = c.accel_scale > 0 ? c.accel_scale : 1.0;
     s.gc = b->gc;
     s.dmax = b->d_dmax;
+    s.first = 0;
     return s;
+}
+
+RepSegs rep_segs(const mdx_batch* b) {
+    return RepSegs{b->d_seg, b->nseg_blocks, b->d_nc, b->d_L, b->d_cstart_g, b->d_coff};
+}
+
+RepPerm rep_perm(const mdx_batch* b, bool flip) {
+    RepPerm pm{};
+    double* const* cur = flip ? b->rn : b->r;
+    double* const* arr[5] = {cur, b->v, b->a, b->b, b->c};
+    for (int k = 0; k < 5; k++)
+        for (int d = 0; d < 3; d++) pm.comp[3 * k + d] = arr[k][d];
+    pm.atom = b->d_atom;
+    for (int d = 0; d < 3; d++) pm.r0[d] = b->r0[d];
+    return pm;
 }
 
 inline int grid_for(int64_t n) { return (int)std::max<int64_t>(1, (n + kBlock - 1) / kBlock); }
@@ -516,18 +668,48 @@ template <class F> hipError_t btimed(mdx_batch* b, const char* name, F&& launch)
 }
 #define BLAUNCH(b, name, expr) BCHK(b, btimed(b, name, [&]() { (expr); return hipGetLastError(); }))
 
-// gated list rebuild of the flagged replicas
-int enqueue_rebuild(mdx_batch* b, bool flip) {
+// the int32 lists of k_rep_list for the replicas gate[] names, from slot
+// `first` on; write_r0 records the list-build positions
+int enqueue_int_lists(mdx_batch* b, bool flip, const int32_t* gate, int32_t first, int32_t write_r0) {
+    if (first >= b->ntot) return MDX_OK;
     hipStream_t s = b->stream;
     const RepView v = view(b);
     double* const* hp = flip ? b->rn : b->r;
     const float rs = (float)b->cfg.lj_rs + 1e-4f;
+    BLAUNCH(b, "rep_list", (k_rep_list<<<grid_for(4 * (int64_t)(b->ntot - first)), kBlock, 0, s>>>(
+                               v, hp[0], hp[1], hp[2], b->r0[0], b->r0[1], b->r0[2], b->ntot, b->d_cell_of,
+                               b->d_cstart, b->d_spos, rs * rs, b->cap, b->d_list, b->d_count, b->d_err, gate, first,
+                               write_r0)));
+    return MDX_OK;
+}
+
+// gated list rebuild of the flagged replicas; at the start of a run (all
+// flagged) the staged replicas get int32 lists too, for the first forces
+int enqueue_rebuild(mdx_batch* b, bool flip, bool start = false) {
+    hipStream_t s = b->stream;
+    const RepView v = view(b);
+    double* const* hp = flip ? b->rn : b->r;
     BLAUNCH(b, "rep_sort", (k_rep_sort<<<b->R, kSortBlock, 0, s>>>(v, hp[0], hp[1], hp[2], b->d_cell_of, b->d_cstart,
-                                                                     b->d_sidx, b->d_tpos, b->d_spos)));
-    BLAUNCH(b, "rep_list", (k_rep_list<<<grid_for(4 * (int64_t)b->ntot), kBlock, 0, s>>>(v, hp[0], hp[1], hp[2], b->r0[0], b->r0[1],
-                                                                            b->r0[2], b->ntot, b->d_cell_of, b->d_cstart,
-                                                                            b->d_spos, rs * rs, b->cap, b->d_list,
-                                                                            b->d_count, b->d_err)));
+                                                                     b->d_cstart_g, b->d_sidx, b->d_tpos, b->d_spos,
+                                                                     b->d_staged)));
+    const bool staged = b->nseg_blocks > 0;
+    if (staged) {
+        const RepPerm pm = rep_perm(b, flip);
+        BLAUNCH(b, "rep_perm", (k_rep_perm_gather<<<grid_for(b->ntot), kBlock, 0, s>>>(v, b->d_staged, b->d_sidx, pm,
+                                                                                        b->d_ptmp, b->d_patom, b->ntot)));
+        BLAUNCH(b, "rep_perm", (k_rep_perm_scatter<<<grid_for(b->ntot), kBlock, 0, s>>>(v, b->d_staged, pm, b->d_ptmp,
+                                                                                         b->d_patom, b->ntot)));
+    }
+    // the gather path's replicas lie from ubeg on
+    int rc = enqueue_int_lists(b, flip, b->d_rebuild, start ? 0 : b->ubeg, 1);
+    if (rc) return rc;
+    if (staged) {
+        const double rl2 = (double)b->cfg.lj_rs * (double)b->cfg.lj_rs;
+        BLAUNCH(b, "rep_worklist", (k_rep_worklist<<<1, kSortBlock, 0, s>>>(b->R, b->d_rebuild, b->d_sbeg, b->d_wl,
+                                                                             b->d_wcount)));
+        BLAUNCH(b, "rep_nlist", launch_rep_nlist(s, rep_segs(b), b->d_wl, b->d_wcount, hp, rl2, b->rs_cap,
+                                                 b->d_rs_list, b->d_rs_count, b->d_rs_err, lds_capacity()));
+    }
     return MDX_OK;
 }
 
@@ -544,22 +726,53 @@ int enqueue_step(mdx_batch* b, bool flip, bool fin) {
     if (rc) return rc;
     const RepStep a = step_args(b, flip);
     const bool fp64 = b->cfg.precision == MDX_PREC_FP64;
-    const int g = grid_for(4 * (int64_t)b->ntot);
-    if (fin) {
-        if (fp64) BLAUNCH(b, "rep_step", (k_rep_step<true, 1><<<g, kBlock, 0, s>>>(a)));
-        else BLAUNCH(b, "rep_step", (k_rep_step<false, 1><<<g, kBlock, 0, s>>>(a)));
-        return MDX_OK;
+    if (b->nseg_blocks > 0) {
+        DevState st{};
+        double* const* cur = flip ? b->rn : b->r;
+        for (int d = 0; d < 3; d++) {
+            st.r[d] = cur[d]; st.v[d] = b->v[d]; st.a[d] = b->a[d]; st.b[d] = b->b[d]; st.c[d] = b->c[d];
+            st.f[d] = b->f[d];
+        }
+        double* const* nxt = flip ? b->r : b->rn;
+        BLAUNCH(b, "rep_step_lds", launch_rep_step(s, rep_segs(b), st, nxt, b->r0, b->d_rs_list, b->d_rs_count,
+                                                   b->rs_cap, a.rc2, a.sig2, a.coeff, a.mass, a.scale, b->gc, b->ntot,
+                                                   b->cfg.precision, fin, b->d_dmax, b->d_rs_err, b->rs_lds));
     }
-    if (fp64) BLAUNCH(b, "rep_step", (k_rep_step<true, 0><<<g, kBlock, 0, s>>>(a)));
-    else BLAUNCH(b, "rep_step", (k_rep_step<false, 0><<<g, kBlock, 0, s>>>(a)));
-    return enqueue_decide(b);
+    if (b->n_unstaged > 0) {
+        RepStep ag = a;
+        ag.first = b->ubeg;
+        const int g = grid_for(4 * (int64_t)(b->ntot - b->ubeg));
+        if (fin) {
+            if (fp64) BLAUNCH(b, "rep_step", (k_rep_step<true, 1><<<g, kBlock, 0, s>>>(ag)));
+            else BLAUNCH(b, "rep_step", (k_rep_step<false, 1><<<g, kBlock, 0, s>>>(ag)));
+        } else {
+            if (fp64) BLAUNCH(b, "rep_step", (k_rep_step<true, 0><<<g, kBlock, 0, s>>>(ag)));
+            else BLAUNCH(b, "rep_step", (k_rep_step<false, 0><<<g, kBlock, 0, s>>>(ag)));
+        }
+    }
+    return fin ? MDX_OK : enqueue_decide(b);
 }
 
 int check_err(mdx_batch* b) {
-    int32_t e[2];
+    int32_t e[2], f[3] = {0, 0, 0};
     BCHK(b, hipMemcpyAsync(e, b->d_err, sizeof e, hipMemcpyDeviceToHost, b->stream));
+    if (b->nseg_blocks > 0) BCHK(b, hipMemcpyAsync(f, b->d_rs_err, sizeof f, hipMemcpyDeviceToHost, b->stream));
     BCHK(b, hipStreamSynchronize(b->stream));
     if (e[1]) return bfail(b, MDX_ERR_CAPACITY, "a neighbour list exceeded max_neighbors (longest " + std::to_string(e[1]) + ")");
+    if (f[1]) return bfail(b, MDX_ERR_CAPACITY, "a neighbour list exceeded max_neighbors (longest " + std::to_string(f[1]) + ")");
+    if (f[0]) return bfail(b, MDX_ERR_CAPACITY, "a replica column's neighbourhood exceeds the LDS staging (" +
+                                                    std::to_string(f[0]) + " atoms)");
+    // the staged step's LDS: the largest staging so far (f[2], from every list
+    // build) with a fifth to grow; sized again, with a new graph, once that
+    // staging comes within a tenth of it (a step staging more than it holds
+    // fails the run with the error above)
+    if (b->nseg_blocks > 0 && (b->rs_lds == 0 || 1.1 * f[2] > b->rs_lds)) {
+        b->rs_lds = std::min<int32_t>(lds_capacity(), ((int32_t)(1.2 * f[2]) + 64 + 63) / 64 * 64);
+        if (b->graph) {
+            hipGraphExecDestroy(b->graph);
+            b->graph = nullptr;
+        }
+    }
     return MDX_OK;
 }
 
@@ -578,7 +791,7 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
     mdx_batch* b = new mdx_batch;
     b->cfg = c;
     b->R = n_replicas;
-    int64_t off = 0, coff = 0;
+    int64_t coff = 0;
     for (int32_t r = 0; r < n_replicas; r++) {
         const double L = box_length[r];
         if (n_atoms[r] <= 0 || n_atoms[r] > kMaxReplica || !(L >= 2.0 * c.lj_rs)) {
@@ -586,7 +799,6 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
             return bfail(nullptr, MDX_ERR_ARG, "replica " + std::to_string(r) + ": need 0 < n <= " +
                                                    std::to_string(kMaxReplica) + " and box >= 2 rs");
         }
-        b->off.push_back((int32_t)off);
         b->nat.push_back(n_atoms[r]);
         b->L.push_back(L);
         int32_t nc = (int32_t)std::floor(L / (double)c.lj_rs);
@@ -594,13 +806,50 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
         b->nc.push_back(nc);
         b->coff.push_back((int32_t)coff);
         coff += (nc ? (int64_t)nc * nc * nc : 1) + 1;
-        off += ((int64_t)n_atoms[r] + kWave - 1) / kWave * kWave;
+    }
+    // staged replicas: at least 3 cells per axis, the z-columns cut into
+    // segments of about 0.85 x 128 home atoms (as the single box cuts them),
+    // shorter where their neighbourhood, 9 columns of K + 2 cells, would stage
+    // more than kStageAim atoms; a replica whose segments cannot fit the LDS
+    // staging with a fifth to spare stays on the gather path
+    std::vector<uint8_t> staged(n_replicas, 0);
+    std::vector<int32_t> staged_i(n_replicas, 0), seg, sbeg(n_replicas + 1, 0);
+    for (int32_t r = 0; r < n_replicas; r++) {
+        const int32_t nc = b->nc[r];
+        sbeg[r] = (int32_t)(seg.size() / 4);
+        if (nc == 0) continue;
+        const double per_cell = (double)n_atoms[r] / ((double)nc * nc * nc);
+        int32_t nseg = lds_segments_per_column(nc, per_cell, nc);
+        while (nseg < nc && 9.0 * ((nc + nseg - 1) / nseg + 2) * per_cell > kStageAim) nseg++;
+        const int32_t K = (nc + nseg - 1) / nseg;
+        if (1.2 * 9.0 * (K + 2) * per_cell + 64 > lds_capacity()) continue;
+        staged[r] = 1;
+        staged_i[r] = 1;
+        for (int32_t col = 0; col < nc * nc; col++)
+            for (int32_t k = 0; k < nseg; k++) {
+                seg.push_back(r);
+                seg.push_back(col);
+                seg.push_back(k * nc / nseg);
+                seg.push_back((k + 1) * nc / nseg);
+            }
+    }
+    // layout: the staged replicas first, the gather path's from ubeg on
+    b->off.assign(n_replicas + 1, 0);
+    int64_t off = 0, ubeg = 0;
+    for (int pass = 1; pass >= 0; pass--) {
+        if (pass == 0) ubeg = off;
+        for (int32_t r = 0; r < n_replicas; r++)
+            if (staged[r] == pass) {
+                b->off[r] = (int32_t)std::min<int64_t>(off, INT32_MAX);
+                off += ((int64_t)n_atoms[r] + kWave - 1) / kWave * kWave;
+            }
     }
     if (off > INT32_MAX / 2 || coff > (int64_t(1) << 27)) {
         delete b;
         return bfail(nullptr, MDX_ERR_ARG, "batch too large");
     }
-    b->off.push_back((int32_t)off);
+    b->off[n_replicas] = (int32_t)off;
+    b->ubeg = (int32_t)ubeg;
     b->coff.push_back((int32_t)coff);
     b->ntot = (int32_t)off;
     b->ncell_tot = (int32_t)coff;
@@ -633,7 +882,43 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
         g_batch_err = b->err;
         return bail(rc);
     }
+    b->nseg_blocks = (int32_t)(seg.size() / 4);
+    sbeg[n_replicas] = b->nseg_blocks;
+    b->staged = staged;
+    for (int32_t r = 0; r < n_replicas; r++) b->n_unstaged += staged[r] ? 0 : 1;
+    if (b->nseg_blocks > 0) {
+        b->rs_cap = (b->cap + list_cap_multiple() - 1) / list_cap_multiple() * list_cap_multiple();
+        if ((rc = balloc(b, &b->d_staged, n_replicas)) || (rc = balloc(b, &b->d_staged_i, n_replicas)) ||
+            (rc = balloc(b, &b->d_seg, seg.size())) || (rc = balloc(b, &b->d_cstart_g, b->ncell_tot)) ||
+            (rc = balloc(b, &b->d_sbeg, sbeg.size())) || (rc = balloc(b, &b->d_wl, b->nseg_blocks)) ||
+            (rc = balloc(b, &b->d_wcount, 1)) ||
+            (rc = balloc(b, &b->d_ptmp, 15 * (size_t)n)) || (rc = balloc(b, &b->d_patom, n)) ||
+            (rc = balloc(b, &b->d_rs_list, (size_t)list_entries(n, b->rs_cap))) ||
+            (rc = balloc(b, &b->d_rs_count, n)) || (rc = balloc(b, &b->d_rs_err, 4))) {
+            g_batch_err = b->err;
+            return bail(rc);
+        }
+    } else if ((rc = balloc(b, &b->d_staged, n_replicas)) || (rc = balloc(b, &b->d_cstart_g, b->ncell_tot))) {
+        g_batch_err = b->err;
+        return bail(rc);
+    }
+    if ((rc = balloc(b, &b->d_atom, n))) {
+        g_batch_err = b->err;
+        return bail(rc);
+    }
     hipStream_t s = b->stream;
+    bool staged_ok = hipMemcpyAsync(b->d_staged, staged.data(), n_replicas, hipMemcpyHostToDevice, s) == hipSuccess;
+    if (b->nseg_blocks > 0)
+        staged_ok = staged_ok &&
+                    hipMemcpyAsync(b->d_staged_i, staged_i.data(), sizeof(int32_t) * n_replicas,
+                                   hipMemcpyHostToDevice, s) == hipSuccess &&
+                    hipMemcpyAsync(b->d_seg, seg.data(), sizeof(int32_t) * seg.size(), hipMemcpyHostToDevice, s) ==
+                        hipSuccess &&
+                    hipMemcpyAsync(b->d_sbeg, sbeg.data(), sizeof(int32_t) * sbeg.size(), hipMemcpyHostToDevice, s) ==
+                        hipSuccess &&
+                    hipMemsetAsync(b->d_rs_err, 0, 4 * sizeof(int32_t), s) == hipSuccess &&
+                    hipMemsetAsync(b->d_rs_count, 0, sizeof(int32_t) * n, s) == hipSuccess;
+    if (!staged_ok) return bail(bfail(nullptr, MDX_ERR_HIP, "batch upload failed"));
     bool ok = hipMemcpyAsync(b->d_rep_of, rep_of.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s) == hipSuccess &&
               hipMemcpyAsync(b->d_off, b->off.data(), sizeof(int32_t) * (n_replicas + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
               hipMemcpyAsync(b->d_nat, b->nat.data(), sizeof(int32_t) * n_replicas, hipMemcpyHostToDevice, s) == hipSuccess &&
@@ -668,14 +953,24 @@ int mdx_batch_set_state(mdx_batch* b, const double* pos3, const double* vel3) {
     if (!b || !pos3 || !vel3) return bfail(b, MDX_ERR_ARG, "pos3 and vel3 are required");
     const int32_t n = b->ntot;
     std::vector<double> buf(3 * (size_t)n, 0.0), vb(3 * (size_t)n, 0.0);
+    std::vector<int32_t> atom(n, -1);
     int64_t src = 0;
     for (int32_t r = 0; r < b->R; r++)
-        for (int32_t i = 0; i < b->nat[r]; i++, src++)
+        for (int32_t i = 0; i < b->nat[r]; i++, src++) {
+            atom[b->off[r] + i] = i;
             for (int d = 0; d < 3; d++) {
                 buf[(size_t)d * n + b->off[r] + i] = pos3[3 * src + d];
                 vb[(size_t)d * n + b->off[r] + i] = vel3[3 * src + d];
             }
+        }
     hipStream_t s = b->stream;
+    BCHK(b, hipMemcpyAsync(b->d_atom, atom.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+    // the staged step's LDS is measured again at the first run; its graph goes
+    b->rs_lds = 0;
+    if (b->graph) {
+        hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
     // a fresh start: the state is (re)loaded into buffer A, Gear history cleared
     BCHK(b, hipMemcpyAsync(b->r[0], buf.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
     BCHK(b, hipMemcpyAsync(b->v[0], vb.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
@@ -690,15 +985,23 @@ int mdx_batch_get_state(mdx_batch* b, double* pos3, double* vel3, double* force3
     if (!b) return bfail(b, MDX_ERR_ARG, "null batch");
     const int32_t n = b->ntot;
     hipStream_t s = b->stream;
+    // staged replicas keep their atoms in the cell order of their last list build
+    std::vector<int32_t> atom(n);
+    BCHK(b, hipMemcpyAsync(atom.data(), b->d_atom, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+    BCHK(b, hipStreamSynchronize(s));
     auto get = [&](double* const* src, double* dst) -> int {
         if (!dst) return MDX_OK;
         std::vector<double> buf(3 * (size_t)n);
         BCHK(b, hipMemcpyAsync(buf.data(), src[0], sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
         BCHK(b, hipStreamSynchronize(s));
-        int64_t q = 0;
-        for (int32_t r = 0; r < b->R; r++)
-            for (int32_t i = 0; i < b->nat[r]; i++, q++)
-                for (int d = 0; d < 3; d++) dst[3 * q + d] = buf[(size_t)d * n + b->off[r] + i];
+        int64_t q0 = 0;
+        for (int32_t r = 0; r < b->R; r++) {
+            for (int32_t p = 0; p < b->nat[r]; p++) {
+                const int64_t q = q0 + atom[b->off[r] + p];
+                for (int d = 0; d < 3; d++) dst[3 * q + d] = buf[(size_t)d * n + b->off[r] + p];
+            }
+            q0 += b->nat[r];
+        }
         return MDX_OK;
     };
     int rc;
@@ -719,12 +1022,12 @@ int mdx_batch_run(mdx_batch* b, int32_t nsteps) {
         // first call after set_state: lists for all replicas, forces, a = F/m
         // (cpu/cpumd/Host.cpp:132-138)
         BLAUNCH(b, "rep_set", (k_rep_set<<<(b->R + 255) / 256, 256, 0, s>>>(b->R, b->d_rebuild, 1)));
-        if ((rc = enqueue_rebuild(b, b->flip))) return rc;
+        if ((rc = enqueue_rebuild(b, b->flip, true))) return rc;
         if ((rc = check_err(b))) return rc;
         double* const* cur = b->flip ? b->rn : b->r;
         BLAUNCH(b, "rep_records", (k_rep_records<<<grid_for(b->ntot), kBlock, 0, s>>>(cur[0], cur[1], cur[2],
                                                                                        b->d_rq[b->flip ? 1 : 0], b->ntot)));
-        const RepStep a = step_args(b, b->flip);
+        RepStep a = step_args(b, b->flip);   // the first forces of every replica, from the int32 lists
         if (b->cfg.precision == MDX_PREC_FP64)
             BLAUNCH(b, "rep_force0", (k_rep_step<true, 2><<<grid_for(4 * (int64_t)b->ntot), kBlock, 0, s>>>(a)));
         else
@@ -774,10 +1077,22 @@ int mdx_batch_build_counts(mdx_batch* b, int64_t* builds) {
     return MDX_OK;
 }
 
+int mdx_batch_staged(mdx_batch* b, int32_t* staged) {
+    if (!b || !staged) return bfail(b, MDX_ERR_ARG, "null argument");
+    for (int32_t r = 0; r < b->R; r++) staged[r] = b->staged[r];
+    return MDX_OK;
+}
+
 int mdx_batch_energies(mdx_batch* b, double* out2) {
     if (!b || !out2) return bfail(b, MDX_ERR_ARG, "null argument");
     hipStream_t s = b->stream;
     BCHK(b, hipMemsetAsync(b->d_energy, 0, sizeof(double) * 2 * b->R, s));
+    if (b->nseg_blocks > 0) {
+        // int32 lists of the staged replicas from their last build (the list-build
+        // positions stay as they are, so the rebuild schedule is unchanged)
+        int rc = enqueue_int_lists(b, b->flip, b->d_staged_i, 0, 0);
+        if (rc) return rc;
+    }
     const RepStep a = step_args(b, b->flip);
     const mdx_config& c = b->cfg;
     const double sg = c.lj_sigma > 0 ? c.lj_sigma : 1.0, ep = c.lj_eps > 0 ? c.lj_eps : 1.0;

@@ -60,6 +60,15 @@ def oracle_run(p, v, L, steps):
 
 
 @pytest.mark.gpu
+def test_batch_paths():
+    """Boxes of 3 or more cells of rs take the LDS-staged path, narrower ones
+    the gather path."""
+    ns, Ls, _, _ = make_batch()
+    with batch(ns, Ls, PREC_MIXED) as b:
+        assert b.staged().tolist() == [False, True, False, True, True]
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("precision", [PREC_FP64, PREC_MIXED])
 def test_batch_forces_match_oracle(precision):
     ns, Ls, ps, vs = make_batch()
