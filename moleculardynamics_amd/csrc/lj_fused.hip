@@ -437,6 +437,23 @@ k_step_lds(StepArgs a) {
     }
     const Pos P{dyn, a.lds_cap};
     unsigned long long* fl = kRep ? nullptr : a.flags + 3 * (blockIdx.x % kSpread);
+    const int32_t cbase = (cx * g.n[1] + cy) * g.n[2];
+    const int32_t hbeg = g.start[cbase + z0], hend = g.start[cbase + z1];
+    const int32_t q = threadIdx.x & (kLanes - 1);
+    // the first pass's list count and first chunk, loaded before the staging so
+    // that their latency passes while it runs
+    constexpr int32_t cstride = kListBlock * kLanes;   // uint4 between a lane's chunks
+    auto list_of = [&](int32_t i) {
+        return reinterpret_cast<const uint4*>(a.list) +
+               ((int64_t)(i / kListBlock) * a.nchunk * kListBlock + i % kListBlock) * kLanes + q;
+    };
+    const int32_t i_first = hbeg + (int32_t)threadIdx.x / kLanes;
+    int32_t count_first = 0;
+    uint4 chunk_first = make_uint4(0u, 0u, 0u, 0u);
+    if (i_first < hend) {
+        count_first = a.count[i_first];
+        chunk_first = list_of(i_first)[0];
+    }
     if (!stage(S, P, g, cx, cy, z0, z1, a.st.r[0], a.st.r[1], a.st.r[2])) {
         // cannot happen after a successful list build on the same cells: the
         // single box reports it as stale, the replicas through their error word
@@ -446,11 +463,7 @@ k_step_lds(StepArgs a) {
         }
         return;
     }
-    const int32_t cbase = (cx * g.n[1] + cy) * g.n[2];
-    const int32_t hbeg = g.start[cbase + z0], hend = g.start[cbase + z1];
-    const int32_t q = threadIdx.x & (kLanes - 1);
     double d2max = 0.0, mv2max = 0.0, stale = 0.0;
-    const int32_t n = a.n;
     for (int32_t base = hbeg; base < hend; base += kApp) {
         const int32_t i = base + threadIdx.x / kLanes;
         if (i >= hend) continue;              // whole quads drop out together
@@ -458,15 +471,14 @@ k_step_lds(StepArgs a) {
         const int32_t self = S.colbase[4] + S.cellpre[4][1] + (i - hbeg);
         const double xi = P.x(self), yi = P.y(self), zi = P.z(self);
         double fx = 0, fy = 0, fz = 0;
-        const int32_t m = (a.count[i] - q + kLanes - 1) / kLanes;   // this lane's entries
+        const bool first = base == hbeg;
+        const int32_t m = ((first ? count_first : a.count[i]) - q + kLanes - 1) / kLanes;   // this lane's entries
         // chunks of this lane are 16 x 4 uint4 apart; the next chunk is loaded
         // while the current one is evaluated (the list has one spare chunk per
         // lane, so the look-ahead stays in bounds); the last chunk is padded
         // with the dummy atom
-        const uint4* lp = reinterpret_cast<const uint4*>(a.list) +
-                          ((int64_t)(i / kListBlock) * a.nchunk * kListBlock + i % kListBlock) * kLanes + q;
-        constexpr int32_t cstride = kListBlock * kLanes;
-        uint4 nx = lp[0];
+        const uint4* lp = list_of(i);
+        uint4 nx = first ? chunk_first : lp[0];
         for (int32_t k0 = 0; k0 < m; k0 += kChunk) {
             const uint4 cu = nx;
             lp += cstride;
