@@ -413,6 +413,7 @@ struct StepArgs {
     RepGeo rg;                   // batched replicas: geometry,
     unsigned long long* dmax;    // the replicas' largest displacement^2 bits of r(t+1),
     int32_t* err;                // and a staging overflow word
+    FlagPublish pub;             // the single box's flag hand-over
 };
 
 __device__ inline double pick3(int32_t q, double a, double b, double c) { return q == 0 ? a : (q == 1 ? b : c); }
@@ -427,12 +428,12 @@ k_step_lds(StepArgs a) {
     SegGrid g = a.g;
     if (blockIdx.x == 0 && a.clear)
         for (int32_t t = threadIdx.x; t < 3 * kSpread; t += kBlock) a.clear[t] = 0;
-    int32_t cx, cy, z0, z1, rep = 0;
+    int32_t cx, cy, z0, z1, rep = 0, seg = 0;
     if constexpr (kRep) {
         const int32_t b = rep_block(a.rg);
         if (b < 0) return;
         rep = rep_geometry(a.rg, b, g, cx, cy, z0, z1);
-    } else if (seg_geometry(g, cx, cy, z0, z1) < 0) {
+    } else if ((seg = seg_geometry(g, cx, cy, z0, z1)) < 0) {
         return;
     }
     const Pos P{dyn, a.lds_cap};
@@ -461,6 +462,7 @@ k_step_lds(StepArgs a) {
             if constexpr (kRep) atomicMax(&a.err[0], S.total + 1);
             else atomicAdd(&fl[2], 1ull << 32);
         }
+        if constexpr (!kRep) publish_flags(a.pub, seg, a.flags, 3 * kSpread);
         return;
     }
     double d2max = 0.0, mv2max = 0.0, stale = 0.0;
@@ -587,6 +589,7 @@ k_step_lds(StepArgs a) {
             if (S.red[2] > 0) atomicAdd(&fl[2], S.red[2]);
         }
     }
+    if constexpr (!kRep) publish_flags(a.pub, seg, a.flags, 3 * kSpread);
 }
 
 SegGrid make_grid(const GridView& v, int32_t nseg) {
@@ -651,7 +654,7 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
                            double* const rn[3], double* const r0[3], const uint16_t* list, const int32_t* count,
                            int32_t cap, const LjTable& t, const MassTable& m, const GearCoeffs& gc, double hard2, int32_t n,
                            int64_t cs, int32_t precision, bool final_step, bool force_out, uint64_t* flags,
-                           int32_t lds_atoms, uint64_t* clear_next) {
+                           int32_t lds_atoms, uint64_t* clear_next, FlagPublish pub) {
     if (n <= 0) return hipSuccess;
     static const bool attr_set = [] {  // once per process, thread-safe
         allow_lds(k_step_lds<false, true, true, true>);
@@ -694,6 +697,8 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
             return hipErrorInvalidValue;
     a.n = n;
     a.lds_cap = lds_atoms;
+    a.pub = pub;
+    a.pub.blocks = a.g.total;   // the workgroups past the segments return before counting
     const size_t dyn = 3 * (size_t)lds_atoms * sizeof(double);
     dim3 grid(a.g.per_xcd * kXcd);
     const bool fp64 = precision == MDX_PREC_FP64;

@@ -56,6 +56,60 @@ __device__ inline double lane_double(double v, int l) {
                             __builtin_amdgcn_readlane(__double2loint(v), l));
 }
 
+// The per-step flag hand-over of the periodic LJ step kernel (k_step_lds), in
+// place of a device-to-host copy after each launch.  Every workgroup that takes part
+// arrives, once its flag atomics have completed, at one of kPubSub counters
+// (its number modulo kPubSub); the last arrival at each of those arrives at
+// the top counter, and the last arrival there copies the step's flag words to
+// a slot of pinned host memory and then writes the step's sequence number
+// after them, so a host that sees the number sees the words.  Each counter is
+// reset by its last arrival for the next launch.  Two levels keep a kernel's
+// workgroups, which may all finish at once, from queueing on one address.
+// The flags are only touched by device-scope atomics and every access here is
+// an atomic that waits for the previous ones, so no fence (which would write
+// back the XCD's L2 after a step kernel's stores) is needed.
+// host == nullptr: nothing is published.
+constexpr int kPubSub = 32;
+struct FlagPublish {
+    unsigned int* done;         // kPubSub + 1 device counters, 0 between launches
+    unsigned long long* host;   // flag words, then the sequence number
+    unsigned long long seq;
+    int32_t blocks;             // workgroups that take part, numbered 0 .. blocks - 1
+};
+
+// every thread of a workgroup that takes part (number id), after the
+// workgroup's flag atomics
+__device__ inline void publish_flags(const FlagPublish& p, int32_t id, const unsigned long long* flags,
+                                     int32_t words) {
+    if (!p.host) return;
+    __shared__ int32_t last;
+    __builtin_amdgcn_s_waitcnt(0);   // this wave's flag atomics have completed
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int32_t sub = id % kPubSub;
+        const unsigned int n_sub = (unsigned int)(p.blocks / kPubSub + (sub < p.blocks % kPubSub ? 1 : 0));
+        const unsigned int n_top = (unsigned int)(p.blocks < kPubSub ? p.blocks : kPubSub);
+        int32_t l = 0;
+        if (__hip_atomic_fetch_add(p.done + sub, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u == n_sub) {
+            __hip_atomic_store(p.done + sub, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            l = __hip_atomic_fetch_add(p.done + kPubSub, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u ==
+                n_top;
+        }
+        last = l;
+    }
+    __syncthreads();
+    if (!last) return;
+    for (int32_t w = threadIdx.x; w < words; w += blockDim.x)
+        __hip_atomic_store(p.host + w, __hip_atomic_load(flags + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __builtin_amdgcn_s_waitcnt(0);   // the words have been written
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(p.done + kPubSub, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(p.host + words, p.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 // Per-atom SoA state (slot order).
 struct DevState {
     double* r[3];
@@ -208,12 +262,13 @@ hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v
                             int32_t n, int32_t cap, uint16_t* list, int32_t* count, int32_t* err,
                             int32_t lds_atoms);
 // st, rn and r0 hold each vector's three components in one block, component d
-// at [0] + d * cs; clear_next (may be null): the next step's flag words, zeroed
+// at [0] + d * cs; clear_next (may be null): the next step's flag words, zeroed;
+// pub: the flag hand-over (blocks is set here)
 hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg,
                            double* const rn[3], double* const r0[3], const uint16_t* list, const int32_t* count,
                            int32_t cap, const LjTable& t, const MassTable& m, const GearCoeffs& gc, double hard2,
                            int32_t n, int64_t cs, int32_t precision, bool final_step, bool force_out, uint64_t* flags,
-                           int32_t lds_atoms, uint64_t* clear_next);
+                           int32_t lds_atoms, uint64_t* clear_next, FlagPublish pub);
 
 // tersoff.hip: Tersoff silicon in the periodic box (compact [k][slot] lists)
 TersoffParams tersoff_si_params();

@@ -104,7 +104,12 @@ struct mdx_system {
     static constexpr int kRing = 3;
     static constexpr int kFlagWords = 3 * 64;  // flag_spread() x {disp^2, step^2, stale}
     uint64_t* dflags = nullptr;      // kRing x kFlagWords
-    uint64_t* hflags = nullptr;      // pinned copy of dflags
+    // pinned: kRing slots of kFlagWords copied from dflags and the sequence
+    // number of the step whose words they are (written by the step kernels)
+    static constexpr int kHostWords = kFlagWords + 1;
+    uint64_t* hflags = nullptr;
+    unsigned int* dpub = nullptr;    // the step kernels' arrival counters (FlagPublish)
+    uint64_t pub_seq = 0;            // last sequence number handed out
     hipEvent_t ev[kRing] = {nullptr, nullptr, nullptr};
     bool timing = false;
     std::vector<EventPair> pending;
@@ -694,8 +699,10 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
     if (nsteps <= 0) return MDX_OK;
     const int R = mdx_system::kRing;
     const int W = mdx_system::kFlagWords;
-    auto slot_flags = [&](int k) { return s->dflags + W * (size_t)(((k % R) + R) % R); };
-    auto host_flags = [&](int k) { return s->hflags + W * (size_t)(((k % R) + R) % R); };
+    const int HW = mdx_system::kHostWords;
+    auto ring = [&](int k) { return ((k % R) + R) % R; };
+    auto slot_flags = [&](int k) { return s->dflags + W * (size_t)ring(k); };
+    auto host_flags = [&](int k) { return s->hflags + HW * (size_t)ring(k); };
     // reduce the spread flags: max displacement^2, max step^2, stale count
     auto reduce = [&](const uint64_t* h, double& D, double& S, uint64_t& stale) {
         double d2 = 0, s2 = 0;
@@ -708,9 +715,39 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
         D = std::sqrt(d2);
         S = std::sqrt(s2);
     };
+    // The LJ step kernel hands its flag words to the host slot itself and
+    // writes the step's sequence number after them (FlagPublish); the other
+    // steps' and the predictor's flags are copied and marked with an event
+    // (for the short Tersoff and surrogate step kernels the hand-over at
+    // their end costs as much as the copy it saves).
+    uint64_t expect[mdx_system::kRing] = {0, 0, 0};   // per slot: sequence number awaited (0: event)
+    auto publish = [&](int k) {
+        expect[ring(k)] = ++s->pub_seq;
+        return FlagPublish{s->dpub, reinterpret_cast<unsigned long long*>(host_flags(k)),
+                           (unsigned long long)s->pub_seq, 0};
+    };
     auto post = [&](int k) -> int {  // copy step flags to the host and mark them
         HIPCHK(s, hipMemcpyAsync(host_flags(k), slot_flags(k), W * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-        HIPCHK(s, hipEventRecord(s->ev[((k % R) + R) % R], st));
+        HIPCHK(s, hipEventRecord(s->ev[ring(k)], st));
+        expect[ring(k)] = 0;
+        return MDX_OK;
+    };
+    auto wait_flags = [&](int k) -> int {  // the host slot of step k holds its words
+        const int r = ring(k);
+        if (expect[r] == 0) {
+            HIPCHK(s, hipEventSynchronize(s->ev[r]));
+            return MDX_OK;
+        }
+        const uint64_t* seq = host_flags(k) + W;   // written by the device; read with acquire
+        for (uint32_t spin = 1; __atomic_load_n(seq, __ATOMIC_ACQUIRE) != expect[r]; spin++) {
+            if ((spin & 255u) != 0) continue;
+            // every 256 polls: a stream that has finished (or failed) ends the wait
+            const hipError_t e = hipStreamQuery(st);
+            if (e == hipErrorNotReady) continue;
+            if (e != hipSuccess) return fail(s, MDX_ERR_HIP, std::string("step: ") + hipGetErrorString(e));
+            if (__atomic_load_n(seq, __ATOMIC_ACQUIRE) != expect[r])
+                return fail(s, MDX_ERR_HIP, "a step kernel finished without handing over its flags");
+        }
         return MDX_OK;
     };
     // flags of r(0) come from the first predictor (index -1)
@@ -726,7 +763,7 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
         // Flags index f describes r(f+1) (f = -1: the predictor's r(0)).  Step k
         // uses r(k); the host reads index k-2, i.e. r(k-1), while launch k-1 runs.
         const int fk = k >= 1 ? k - 2 : -1;
-        HIPCHK(s, hipEventSynchronize(s->ev[((fk % R) + R) % R]));
+        if ((rc = wait_flags(fk))) return rc;
         double D, S;
         uint64_t stale;
         reduce(host_flags(fk), D, S, stale);
@@ -772,8 +809,7 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
         LAUNCH(s, "md_step", launch_step_lds(st, s->st, s->lj.view, s->nseg, s->rnext, s->rref, s->nlist, s->ncount, s->nl_cap,
                                              s->ljt, s->mt, s->gc, hard2, n, component_stride(n), c.precision, fin,
                                              fin, slot_flags(k), s->lds_atoms,
-                                             slot_flags(k + 1)));
-        if ((rc = post(k))) return rc;
+                                             slot_flags(k + 1), publish(k)));
         for (int d = 0; d < 3; d++) std::swap(s->st.r[d], s->rnext[d]);
         s->t++;
     }
@@ -852,7 +888,8 @@ int mdx_create(const mdx_config* cfg, mdx_system** out) {
     }
     s->nb = kinetics_blocks(n);
     if ((rc = dalloc(s, &s->partials, 8 * (size_t)s->nb + 16)) ||
-        (rc = dalloc(s, &s->dflags, (size_t)mdx_system::kFlagWords * mdx_system::kRing))) {
+        (rc = dalloc(s, &s->dflags, (size_t)mdx_system::kFlagWords * mdx_system::kRing)) ||
+        (rc = dalloc(s, &s->dpub, kPubSub + 1))) {
         g_err = s->err;
         return bail(rc);
     }
@@ -871,9 +908,12 @@ int mdx_create(const mdx_config* cfg, mdx_system** out) {
         if (s->sg_nflag) hipMemsetAsync(s->sg_nflag, 0, sizeof(int32_t), s->stream);
     }
     if (hipHostMalloc(&s->pinned, 256, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&s->hflags, sizeof(uint64_t) * mdx_system::kFlagWords * mdx_system::kRing,
+        hipHostMalloc((void**)&s->hflags, sizeof(uint64_t) * mdx_system::kHostWords * mdx_system::kRing,
                       hipHostMallocDefault) != hipSuccess)
         return bail(fail(nullptr, MDX_ERR_ALLOC, "hipHostMalloc failed"));
+    std::memset(s->hflags, 0, sizeof(uint64_t) * mdx_system::kHostWords * mdx_system::kRing);
+    if (hipMemsetAsync(s->dpub, 0, sizeof(unsigned int) * (kPubSub + 1), s->stream) != hipSuccess)
+        return bail(fail(nullptr, MDX_ERR_HIP, "init failed"));
     for (int k = 0; k < mdx_system::kRing; k++)
         if (hipEventCreateWithFlags(&s->ev[k], hipEventDisableTiming) != hipSuccess)
             return bail(fail(nullptr, MDX_ERR_HIP, "hipEventCreate failed"));

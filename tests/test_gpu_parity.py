@@ -306,6 +306,26 @@ def test_periodic_run_matches_oracle(fluid):
     assert rel_rms(st["vel"], vo) < 1e-8
 
 
+def test_periodic_split_runs_match_oracle(fluid):
+    """The same 200 steps as above in calls of 1, 2, 3, 47 and 147 steps: each
+    call starts with the predictor's flags (copied to the host) and hands the
+    steps' flags over from the step kernel, so the rebuild decisions made
+    across calls must keep the trajectory on the oracle's."""
+    pos, vel, L = fluid
+    n, chunks = len(pos), [1, 2, 3, 47, 147]
+    ro, vo, _, _, _ = O.pbc_lj_run(pos, vel, L, 2.5, 2.8, DT, sum(chunks))
+    with periodic(n, L, PREC_FP64) as s:
+        s.set_state(np.c_[pos, np.zeros(n)], vel)
+        for k in chunks:
+            s.run(k)
+        st = s.get_state()
+        assert s.build_count() >= 2
+    d = st["pos"][:, :3] - ro
+    d -= L * np.round(d / L)
+    assert np.abs(d).max() < 1e-8
+    assert rel_rms(st["vel"], vo) < 1e-8
+
+
 def test_periodic_mixed_energy_drift(fluid):
     """1000 steps of the 4000-atom fluid: total energy (shifted LJ) drift of the
     mixed-precision run stays within that of the FP64 oracle run plus 1e-4."""
