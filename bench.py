@@ -400,6 +400,7 @@ def bench_replicas(a) -> None:
     b = Batch(ns, Ls, precision=PREC_MIXED, dt=0.005, lj_rs=2.8, lj_rcut=2.5)
     staged = b.staged()
     n_staged = int(ns[staged].sum())
+    n_small = int(ns[b.paths() == 2].sum())
     b.set_state(pos, vel)
     b.run(a.warmup)
     b.synchronize()
@@ -437,7 +438,8 @@ def bench_replicas(a) -> None:
         "data": "synthetic (seeded FCC blocks + Maxwell-Boltzmann velocities per replica)",
         "config": {"workload": f"{len(ns)} periodic LJ replicas, {n} atoms ({int(ns.min())}-{int(ns.max())} per replica), "
                                f"rho U[0.70,0.95], T* U[0.5,2.0], rc 2.5 skin 0.3, dt 0.005", "atoms": n,
-                   "replica_list_builds": builds, "staged_replicas": int(staged.sum()), "staged_atoms": n_staged},
+                   "replica_list_builds": builds, "staged_replicas": int(staged.sum()), "staged_atoms": n_staged,
+                   "all_pairs_atoms": n_small},
         "roofline": {
             "kernel": "rep_step_lds (k_step_lds over the staged replicas' segments)",
             "bound": "hbm",

@@ -272,10 +272,12 @@ int mdx_batch_build_counts(mdx_batch* batch, int64_t* builds);
 /* the path of each replica (n_replicas entries), fixed at creation: 1 for a
  * box of at least 3 cells of lj_rs per axis whose lists and steps run through
  * the single box's LDS-staged kernels (its atoms kept in cell order inside the
- * batch; get_state returns them in the caller's order), 0 for the per-atom
- * gather kernels.  No counterpart in the reference, which runs one system at
- * a time; for callers that size or account work per replica. */
-int mdx_batch_staged(mdx_batch* batch, int32_t* staged);
+ * batch; get_state returns them in the caller's order), 2 for a narrower box
+ * of at most 2048 atoms, stepped by one workgroup per 64 atoms over all pairs
+ * staged in LDS, 0 for the per-atom gather kernels over neighbour lists.  No
+ * counterpart in the reference, which runs one system at a time; for callers
+ * that size or account work per replica. */
+int mdx_batch_staged(mdx_batch* batch, int32_t* path);
 /* per replica {kinetic, potential (LJ shifted to zero at rc)}: 2 * n_replicas */
 int mdx_batch_energies(mdx_batch* batch, double* out2);
 /* per-kernel timing with events; while enabled, mdx_batch_run launches the

@@ -319,11 +319,17 @@ class Batch:
         self._chk("mdx_batch_build_counts", self.lib.mdx_batch_build_counts(self.h, _ptr(b)))
         return b
 
-    def staged(self):
-        """True for the replicas on the LDS-staged path (fixed at creation)."""
+    def paths(self):
+        """Per replica, fixed at creation: 1 the LDS-staged list and step
+        kernels, 2 the one-workgroup all-pairs kernel of narrow boxes, 0 the
+        gather kernels."""
         s = np.zeros(self.R, dtype=np.int32)
         self._chk("mdx_batch_staged", self.lib.mdx_batch_staged(self.h, _ptr(s)))
-        return s.astype(bool)
+        return s
+
+    def staged(self):
+        """True for the replicas on the LDS-staged path."""
+        return self.paths() == 1
 
     def energies(self):
         e = np.zeros((self.R, 2))

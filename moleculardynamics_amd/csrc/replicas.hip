@@ -53,6 +53,9 @@ constexpr int kMaxReplica = 16384;                       // atoms per replica (t
 // staged atoms a segment of a staged replica aims at: the step kernel's LDS,
 // 1.2 x the largest staging, then stays within a quarter of a CU's 160 KiB
 constexpr double kStageAim = 1250.0;
+// the paths of a replica (mdx_batch_staged reports them)
+constexpr uint8_t kGather = 0, kStaged = 1, kSmall = 2;
+constexpr int kSmallMax = 2048;   // atoms of a small replica (its FP64 positions, 48 KiB of LDS)
 
 struct RepView {
     const int32_t* rep_of;     // replica of each atom slot (-1: padding)
@@ -250,8 +253,8 @@ k_rep_perm_scatter(RepView v, const uint8_t* __restrict__ staged, RepPerm pm, co
     pm.atom[i] = tmp_atom[i];
 }
 
-// Compact list of each atom of a flagged replica from slot `first` on, a quad
-// of 4 lanes per atom:
+// Compact list of each atom of a flagged replica in slots [first, last), a
+// quad of 4 lanes per atom:
 // candidates from the 27 cells around it (or the whole replica), the lanes
 // testing interleaved candidates and numbering their hits lane-major, so the
 // list order is deterministic.  The radius test runs on the FP32 copy with rs
@@ -261,10 +264,10 @@ k_rep_list(RepView v, const double* __restrict__ rx, const double* __restrict__ 
            double* __restrict__ r0x, double* __restrict__ r0y, double* __restrict__ r0z, int32_t ntot,
            const int32_t* __restrict__ cell_of, const int32_t* __restrict__ cstart, const float4* __restrict__ spos,
            float rs2, int32_t cap, int32_t* __restrict__ list, int32_t* __restrict__ count, int32_t* __restrict__ err,
-           const int32_t* __restrict__ gate, int32_t first, int32_t write_r0) {
+           const int32_t* __restrict__ gate, int32_t first, int32_t last, int32_t write_r0) {
     const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t i = first + (t >> 2), q = t & 3;
-    if (i >= ntot) return;   // whole quads
+    if (i >= last) return;   // whole quads
     const int32_t r = v.rep_of[i];
     if (r < 0 || !gate[r]) return;
     const int32_t base = v.off[r], nc = v.nc[r], cb = v.coff[r];
@@ -345,11 +348,11 @@ struct RepStep {
     double mass, scale;
     GearCoeffs gc;
     unsigned long long* dmax;   // R: max displacement^2 bits of r(t+1)
-    int32_t first;              // k_rep_step: first slot (the staged replicas lie below it)
+    int32_t first, last;        // k_rep_step: its slots (the gather path's replicas)
 };
 
 // kMode: 0 = corrector + next predictor, 1 = corrector only (last step),
-// 2 = forces and a = F/m only (start of a run); atoms from slot a.first on.  A quad of 4 lanes per atom:
+// 2 = forces and a = F/m only (start of a run); atoms in slots [a.first, a.last).  A quad of 4 lanes per atom:
 // lane q sums list entries q, q + 4, ...; the quad's sums are added by DPP and
 // lane q < 3 integrates component q (the state arrays hold the 3 components of
 // a vector in one block of 3 x ntot).
@@ -359,7 +362,7 @@ k_rep_step(RepStep a) {
     constexpr bool kFinal = kMode != 0;
     const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t i = a.first + (t >> 2), q = t & 3;
-    const int32_t r = i < a.ntot ? a.v.rep_of[i] : -1;
+    const int32_t r = i < a.last ? a.v.rep_of[i] : -1;
     double d2 = 0.0;
     if (r >= 0) {
         const double L = a.v.L[r], invL = 1.0 / L;
@@ -447,6 +450,106 @@ k_rep_step(RepStep a) {
     }
 }
 
+// Replicas narrower than 3 cells of rs per axis (all pairs), one workgroup per
+// kSmallAtoms of a replica's atoms (block table: replica, first atom): the
+// replica's positions r(t) staged in LDS, a quad of lanes per atom testing
+// every other atom by minimum image (lane q takes atoms q, q + 4, ...; the
+// quads of a wave read the same 4 atoms, which the LDS broadcasts), then the
+// Gear step of k_rep_step.  The pairs within rc are those of the list path;
+// only the order of the sum differs.  No list is kept: a replica that rebuilds
+// records its list-build positions here (its rebuild decision, count and sort
+// still run, for the energies' lists and the build counts).
+constexpr int kSmallAtoms = kBlock / 4;
+template <bool kFp64, bool kFinal>
+__global__ void __launch_bounds__(kBlock)
+k_rep_small(RepStep a, const int2* __restrict__ blocks) {
+    extern __shared__ double stg[];   // x y z of each atom of the replica
+    const int2 bk = blocks[blockIdx.x];
+    const int32_t r = bk.x;
+    const int32_t base = a.v.off[r], n = a.v.nat[r];
+    const double L = a.v.L[r], invL = 1.0 / L;
+    for (int32_t t = threadIdx.x; t < 3 * n; t += kBlock) {
+        const int32_t j = t / 3, d = t - 3 * j;
+        stg[t] = a.r[d][base + j];
+    }
+    __syncthreads();
+    const int32_t li = bk.y + (int32_t)threadIdx.x / 4, q = threadIdx.x & 3;
+    const bool on = li < n;   // whole quads
+    double fx = 0, fy = 0, fz = 0;
+    if (on) {
+        const double xi = stg[3 * li], yi = stg[3 * li + 1], zi = stg[3 * li + 2];
+        // the atom itself fails the cutoff test by a select (no branch, so the
+        // loop unrolls and its LDS reads overlap)
+#pragma unroll 4
+        for (int32_t j = q; j < n; j += 4) {
+            const double dx = mi(xi - stg[3 * j], L, invL), dy = mi(yi - stg[3 * j + 1], L, invL),
+                         dz = mi(zi - stg[3 * j + 2], L, invL);
+            const double rc2 = j == li ? -1.0 : a.rc2;
+            if constexpr (kFp64) {
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 < rc2) {
+                    const double qq = r2 / a.sig2;
+                    const double iq4 = 1.0 / (qq * qq * qq * qq);
+                    const double iq7 = iq4 / (qq * qq * qq);
+                    const double fr = (2 * iq7 - iq4) * a.coeff;
+                    fx += dx * fr; fy += dy * fr; fz += dz * fr;
+                }
+            } else {
+                const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
+                const float w = __builtin_amdgcn_rcpf((float)r2 * a.inv_sig2f);
+                const float w2 = w * w, w3 = w2 * w;
+                const float frf = r2 < rc2 ? w2 * w2 * __builtin_fmaf(2.0f, w3, -1.0f) * a.coeff_f : 0.0f;
+                const double fr = (double)frf;
+                fx = __builtin_fma(fr, dx, fx);
+                fy = __builtin_fma(fr, dy, fy);
+                fz = __builtin_fma(fr, dz, fz);
+            }
+        }
+    }
+    fx = quad_sum(fx);
+    fy = quad_sum(fy);
+    fz = quad_sum(fz);
+    double d2 = 0.0;
+    if (on && q < 3) {
+        const double F = q == 0 ? fx : (q == 1 ? fy : fz);
+        const int64_t o = (int64_t)q * a.ntot + base + li;   // component q of the atom
+        const double rt = stg[3 * li + q];
+        double r0 = rt;
+        if (a.v.rebuild[r]) const_cast<double*>(a.r0[0])[o] = rt;   // this step's positions are the build's
+        else r0 = a.r0[0][o];
+        double rr = rt, v = a.vel[0][o], ac = a.acc[0][o], b = a.b3[0][o], c = a.c3[0][o];
+        // corrector (cpu/common/Device.cpp:58-84)
+        const double anew = F / a.mass * a.scale;
+        const double corr = anew - ac;
+        rr = rr + a.gc.kc[0] * corr;
+        v = v + a.gc.kc[1] * corr;
+        ac = anew;
+        b = b + a.gc.kc[2] * corr;
+        c = c + a.gc.kc[3] * corr;
+        if constexpr (!kFinal) {
+            // predictor of the next step (cpu/common/Device.cpp:24-38)
+            const double rp = rr + a.gc.k[0] * v + a.gc.k[1] * ac + a.gc.k[2] * b + a.gc.k[3] * c;
+            v = v + a.gc.k[0] * ac + a.gc.k[1] * b + a.gc.k[2] * c;
+            ac = ac + a.gc.k[0] * b + a.gc.k[1] * c;
+            b = b + a.gc.k[0] * c;
+            const double dd = rp - r0;
+            d2 = dd * dd;
+            rr = rp;
+        }
+        a.rn[0][o] = rr;
+        a.vel[0][o] = v;
+        a.acc[0][o] = ac;
+        a.b3[0][o] = b;
+        a.c3[0][o] = c;
+        a.f[0][o] = F;
+    }
+    if constexpr (!kFinal) {
+        d2 = quad_sum(d2);
+        for (int o = kWave / 2; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o, kWave));
+        if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(&a.dmax[r], (unsigned long long)__double_as_longlong(d2));
+    }
+}
+
 // predictor (cpu/common/Device.cpp:17-39) with the per-replica largest
 // displacement of the predicted positions from the list-build positions
 __global__ void __launch_bounds__(kBlock)
@@ -531,7 +634,7 @@ struct mdx_batch {
     int32_t R = 0, ntot = 0, ncell_tot = 0;
     hipStream_t stream = nullptr;
     std::vector<int32_t> off, nat, coff, nc;
-    std::vector<uint8_t> staged;   // the LDS path's replicas
+    std::vector<uint8_t> path;     // per replica: kGather, kStaged or kSmall
     std::vector<double> L;
     std::vector<void*> allocs;
     int32_t *d_rep_of = nullptr, *d_off = nullptr, *d_nat = nullptr, *d_coff = nullptr, *d_nc = nullptr;
@@ -544,7 +647,7 @@ struct mdx_batch {
     int32_t cap = 0;
     // staged replicas (the single box's LDS path, see the file header)
     uint8_t* d_staged = nullptr;          // R flags
-    int32_t* d_staged_i = nullptr;        // the same as int32 (a list gate)
+    int32_t* d_staged_i = nullptr;        // staged and small replicas (int32 lists only for the energies)
     int32_t* d_seg = nullptr;             // (replica, column, z0, z1) per segment
     int32_t* d_sbeg = nullptr;            // R + 1: first segment of each replica
     int32_t* d_wl = nullptr;              // the list build's work list and its length
@@ -559,7 +662,9 @@ struct mdx_batch {
     int32_t* d_rs_err = nullptr;          // staging overflow, list overflow, largest staging
     int32_t rs_cap = 0, rs_lds = 0;
     int32_t n_unstaged = 0;               // replicas on the gather path,
-    int32_t ubeg = 0;                     // laid out from this slot on
+    int32_t ubeg = 0, sfirst = 0;         // in slots [ubeg, sfirst); the small replicas from sfirst on
+    int2* d_small_blocks = nullptr;       // k_rep_small's workgroups: (replica, first atom)
+    int32_t nsmall_blocks = 0, small_lds = 0;
     long long* d_builds = nullptr;
     unsigned long long* d_dmax = nullptr;
     double* r[3] = {}; double* rn[3] = {}; double* v[3] = {}; double* a[3] = {}; double* b[3] = {}; double* c[3] = {};
@@ -635,6 +740,7 @@ RepStep step_args(const mdx_batch* b, bool flip) {
     s.gc = b->gc;
     s.dmax = b->d_dmax;
     s.first = 0;
+    s.last = b->ntot;
     return s;
 }
 
@@ -668,18 +774,18 @@ template <class F> hipError_t btimed(mdx_batch* b, const char* name, F&& launch)
 }
 #define BLAUNCH(b, name, expr) BCHK(b, btimed(b, name, [&]() { (expr); return hipGetLastError(); }))
 
-// the int32 lists of k_rep_list for the replicas gate[] names, from slot
-// `first` on; write_r0 records the list-build positions
-int enqueue_int_lists(mdx_batch* b, bool flip, const int32_t* gate, int32_t first, int32_t write_r0) {
-    if (first >= b->ntot) return MDX_OK;
+// the int32 lists of k_rep_list for the replicas gate[] names, in slots
+// [first, last); write_r0 records the list-build positions
+int enqueue_int_lists(mdx_batch* b, bool flip, const int32_t* gate, int32_t first, int32_t last, int32_t write_r0) {
+    if (first >= last) return MDX_OK;
     hipStream_t s = b->stream;
     const RepView v = view(b);
     double* const* hp = flip ? b->rn : b->r;
     const float rs = (float)b->cfg.lj_rs + 1e-4f;
-    BLAUNCH(b, "rep_list", (k_rep_list<<<grid_for(4 * (int64_t)(b->ntot - first)), kBlock, 0, s>>>(
+    BLAUNCH(b, "rep_list", (k_rep_list<<<grid_for(4 * (int64_t)(last - first)), kBlock, 0, s>>>(
                                v, hp[0], hp[1], hp[2], b->r0[0], b->r0[1], b->r0[2], b->ntot, b->d_cell_of,
                                b->d_cstart, b->d_spos, rs * rs, b->cap, b->d_list, b->d_count, b->d_err, gate, first,
-                               write_r0)));
+                               last, write_r0)));
     return MDX_OK;
 }
 
@@ -700,8 +806,10 @@ int enqueue_rebuild(mdx_batch* b, bool flip, bool start = false) {
         BLAUNCH(b, "rep_perm", (k_rep_perm_scatter<<<grid_for(b->ntot), kBlock, 0, s>>>(v, b->d_staged, pm, b->d_ptmp,
                                                                                          b->d_patom, b->ntot)));
     }
-    // the gather path's replicas lie from ubeg on
-    int rc = enqueue_int_lists(b, flip, b->d_rebuild, start ? 0 : b->ubeg, 1);
+    // the gather path's replicas lie in [ubeg, sfirst); the first forces of a
+    // run come from the int32 lists of every replica
+    int rc = start ? enqueue_int_lists(b, flip, b->d_rebuild, 0, b->ntot, 1)
+                   : enqueue_int_lists(b, flip, b->d_rebuild, b->ubeg, b->sfirst, 1);
     if (rc) return rc;
     if (staged) {
         const double rl2 = (double)b->cfg.lj_rs * (double)b->cfg.lj_rs;
@@ -738,10 +846,21 @@ int enqueue_step(mdx_batch* b, bool flip, bool fin) {
                                                    b->rs_cap, a.rc2, a.sig2, a.coeff, a.mass, a.scale, b->gc, b->ntot,
                                                    b->cfg.precision, fin, b->d_dmax, b->d_rs_err, b->rs_lds));
     }
+    if (b->nsmall_blocks > 0) {
+        const size_t dyn = (size_t)b->small_lds;
+        if (fin) {
+            if (fp64) BLAUNCH(b, "rep_small", (k_rep_small<true, true><<<b->nsmall_blocks, kBlock, dyn, s>>>(a, b->d_small_blocks)));
+            else BLAUNCH(b, "rep_small", (k_rep_small<false, true><<<b->nsmall_blocks, kBlock, dyn, s>>>(a, b->d_small_blocks)));
+        } else {
+            if (fp64) BLAUNCH(b, "rep_small", (k_rep_small<true, false><<<b->nsmall_blocks, kBlock, dyn, s>>>(a, b->d_small_blocks)));
+            else BLAUNCH(b, "rep_small", (k_rep_small<false, false><<<b->nsmall_blocks, kBlock, dyn, s>>>(a, b->d_small_blocks)));
+        }
+    }
     if (b->n_unstaged > 0) {
         RepStep ag = a;
         ag.first = b->ubeg;
-        const int g = grid_for(4 * (int64_t)(b->ntot - b->ubeg));
+        ag.last = b->sfirst;
+        const int g = grid_for(4 * (int64_t)(b->sfirst - b->ubeg));
         if (fin) {
             if (fp64) BLAUNCH(b, "rep_step", (k_rep_step<true, 1><<<g, kBlock, 0, s>>>(ag)));
             else BLAUNCH(b, "rep_step", (k_rep_step<false, 1><<<g, kBlock, 0, s>>>(ag)));
@@ -811,13 +930,20 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
     // segments of about 0.85 x 128 home atoms (as the single box cuts them),
     // shorter where their neighbourhood, 9 columns of K + 2 cells, would stage
     // more than kStageAim atoms; a replica whose segments cannot fit the LDS
-    // staging with a fifth to spare stays on the gather path
-    std::vector<uint8_t> staged(n_replicas, 0);
+    // staging with a fifth to spare stays on the gather path.  Boxes narrower
+    // than 3 cells of up to kSmallMax atoms take k_rep_small.
+    std::vector<uint8_t> staged(n_replicas, 0), path(n_replicas, kGather);
     std::vector<int32_t> staged_i(n_replicas, 0), seg, sbeg(n_replicas + 1, 0);
     for (int32_t r = 0; r < n_replicas; r++) {
         const int32_t nc = b->nc[r];
         sbeg[r] = (int32_t)(seg.size() / 4);
-        if (nc == 0) continue;
+        if (nc == 0) {
+            if (n_atoms[r] <= kSmallMax) {
+                path[r] = kSmall;
+                staged_i[r] = 1;
+            }
+            continue;
+        }
         const double per_cell = (double)n_atoms[r] / ((double)nc * nc * nc);
         int32_t nseg = lds_segments_per_column(nc, per_cell, nc);
         while (nseg < nc && 9.0 * ((nc + nseg - 1) / nseg + 2) * per_cell > kStageAim) nseg++;
@@ -825,6 +951,7 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
         if (1.2 * 9.0 * (K + 2) * per_cell + 64 > lds_capacity()) continue;
         staged[r] = 1;
         staged_i[r] = 1;
+        path[r] = kStaged;
         for (int32_t col = 0; col < nc * nc; col++)
             for (int32_t k = 0; k < nseg; k++) {
                 seg.push_back(r);
@@ -833,15 +960,23 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
                 seg.push_back((k + 1) * nc / nseg);
             }
     }
-    // layout: the staged replicas first, the gather path's from ubeg on
+    // layout: the staged replicas first, the gather path's from ubeg on, the
+    // small ones from sfirst on
     b->off.assign(n_replicas + 1, 0);
-    int64_t off = 0, ubeg = 0;
-    for (int pass = 1; pass >= 0; pass--) {
-        if (pass == 0) ubeg = off;
+    int64_t off = 0, ubeg = 0, sfirst = 0;
+    std::vector<int2> small_blocks;
+    int32_t small_max = 0;
+    for (const uint8_t kind : {kStaged, kGather, kSmall}) {
+        if (kind == kGather) ubeg = off;
+        if (kind == kSmall) sfirst = off;
         for (int32_t r = 0; r < n_replicas; r++)
-            if (staged[r] == pass) {
+            if (path[r] == kind) {
                 b->off[r] = (int32_t)std::min<int64_t>(off, INT32_MAX);
                 off += ((int64_t)n_atoms[r] + kWave - 1) / kWave * kWave;
+                if (kind == kSmall) {
+                    for (int32_t a0 = 0; a0 < n_atoms[r]; a0 += kSmallAtoms) small_blocks.push_back(make_int2(r, a0));
+                    small_max = std::max(small_max, n_atoms[r]);
+                }
             }
     }
     if (off > INT32_MAX / 2 || coff > (int64_t(1) << 27)) {
@@ -850,6 +985,9 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
     }
     b->off[n_replicas] = (int32_t)off;
     b->ubeg = (int32_t)ubeg;
+    b->sfirst = (int32_t)sfirst;
+    b->nsmall_blocks = (int32_t)small_blocks.size();
+    b->small_lds = 3 * (int32_t)sizeof(double) * small_max;
     b->coff.push_back((int32_t)coff);
     b->ntot = (int32_t)off;
     b->ncell_tot = (int32_t)coff;
@@ -884,11 +1022,18 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
     }
     b->nseg_blocks = (int32_t)(seg.size() / 4);
     sbeg[n_replicas] = b->nseg_blocks;
-    b->staged = staged;
-    for (int32_t r = 0; r < n_replicas; r++) b->n_unstaged += staged[r] ? 0 : 1;
+    b->path = path;
+    for (int32_t r = 0; r < n_replicas; r++) b->n_unstaged += path[r] == kGather ? 1 : 0;
+    if (b->nsmall_blocks > 0 &&
+        ((rc = balloc(b, &b->d_small_blocks, small_blocks.size())) ||
+         hipMemcpyAsync(b->d_small_blocks, small_blocks.data(), sizeof(int2) * small_blocks.size(),
+                        hipMemcpyHostToDevice, b->stream) != hipSuccess)) {
+        g_batch_err = b->err;
+        return bail(rc ? rc : bfail(nullptr, MDX_ERR_HIP, "batch upload failed"));
+    }
     if (b->nseg_blocks > 0) {
         b->rs_cap = (b->cap + list_cap_multiple() - 1) / list_cap_multiple() * list_cap_multiple();
-        if ((rc = balloc(b, &b->d_staged, n_replicas)) || (rc = balloc(b, &b->d_staged_i, n_replicas)) ||
+        if ((rc = balloc(b, &b->d_staged, n_replicas)) ||
             (rc = balloc(b, &b->d_seg, seg.size())) || (rc = balloc(b, &b->d_cstart_g, b->ncell_tot)) ||
             (rc = balloc(b, &b->d_sbeg, sbeg.size())) || (rc = balloc(b, &b->d_wl, b->nseg_blocks)) ||
             (rc = balloc(b, &b->d_wcount, 1)) ||
@@ -902,16 +1047,16 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
         g_batch_err = b->err;
         return bail(rc);
     }
-    if ((rc = balloc(b, &b->d_atom, n))) {
+    if ((rc = balloc(b, &b->d_atom, n)) || (rc = balloc(b, &b->d_staged_i, n_replicas))) {
         g_batch_err = b->err;
         return bail(rc);
     }
     hipStream_t s = b->stream;
-    bool staged_ok = hipMemcpyAsync(b->d_staged, staged.data(), n_replicas, hipMemcpyHostToDevice, s) == hipSuccess;
+    bool staged_ok = hipMemcpyAsync(b->d_staged, staged.data(), n_replicas, hipMemcpyHostToDevice, s) == hipSuccess &&
+                     hipMemcpyAsync(b->d_staged_i, staged_i.data(), sizeof(int32_t) * n_replicas,
+                                    hipMemcpyHostToDevice, s) == hipSuccess;
     if (b->nseg_blocks > 0)
         staged_ok = staged_ok &&
-                    hipMemcpyAsync(b->d_staged_i, staged_i.data(), sizeof(int32_t) * n_replicas,
-                                   hipMemcpyHostToDevice, s) == hipSuccess &&
                     hipMemcpyAsync(b->d_seg, seg.data(), sizeof(int32_t) * seg.size(), hipMemcpyHostToDevice, s) ==
                         hipSuccess &&
                     hipMemcpyAsync(b->d_sbeg, sbeg.data(), sizeof(int32_t) * sbeg.size(), hipMemcpyHostToDevice, s) ==
@@ -1077,9 +1222,9 @@ int mdx_batch_build_counts(mdx_batch* b, int64_t* builds) {
     return MDX_OK;
 }
 
-int mdx_batch_staged(mdx_batch* b, int32_t* staged) {
-    if (!b || !staged) return bfail(b, MDX_ERR_ARG, "null argument");
-    for (int32_t r = 0; r < b->R; r++) staged[r] = b->staged[r];
+int mdx_batch_staged(mdx_batch* b, int32_t* path) {
+    if (!b || !path) return bfail(b, MDX_ERR_ARG, "null argument");
+    for (int32_t r = 0; r < b->R; r++) path[r] = b->path[r];
     return MDX_OK;
 }
 
@@ -1087,10 +1232,10 @@ int mdx_batch_energies(mdx_batch* b, double* out2) {
     if (!b || !out2) return bfail(b, MDX_ERR_ARG, "null argument");
     hipStream_t s = b->stream;
     BCHK(b, hipMemsetAsync(b->d_energy, 0, sizeof(double) * 2 * b->R, s));
-    if (b->nseg_blocks > 0) {
-        // int32 lists of the staged replicas from their last build (the list-build
-        // positions stay as they are, so the rebuild schedule is unchanged)
-        int rc = enqueue_int_lists(b, b->flip, b->d_staged_i, 0, 0);
+    if (b->nseg_blocks > 0 || b->nsmall_blocks > 0) {
+        // int32 lists of the staged and small replicas from their last build (the
+        // list-build positions stay as they are, so the rebuild schedule is unchanged)
+        int rc = enqueue_int_lists(b, b->flip, b->d_staged_i, 0, b->ntot, 0);
         if (rc) return rc;
     }
     const RepStep a = step_args(b, b->flip);

@@ -61,10 +61,11 @@ def oracle_run(p, v, L, steps):
 
 @pytest.mark.gpu
 def test_batch_paths():
-    """Boxes of 3 or more cells of rs take the LDS-staged path, narrower ones
-    the gather path."""
+    """Boxes of 3 or more cells of rs take the LDS-staged path (1), narrower
+    ones the all-pairs path (2); the gather path (0) keeps what neither holds."""
     ns, Ls, _, _ = make_batch()
     with batch(ns, Ls, PREC_MIXED) as b:
+        assert b.paths().tolist() == [2, 1, 2, 1, 1]
         assert b.staged().tolist() == [False, True, False, True, True]
 
 
