@@ -26,8 +26,12 @@
 //    (k_rep_perm_*), their lists are built and their steps run by
 //    k_nlist_lds / k_step_lds of lj_fused.hip, one workgroup per z-segment of
 //    a cell column; an atom-id map keeps the caller's order.  They occupy the
-//    slots below ubeg, so the gather path's kernels (k_rep_list after the
-//    first build, k_rep_step) run over the slots from ubeg on only;
+//    slots below ubeg;
+//  * small replicas (boxes narrower than 3 cells, up to kSmallMax atoms) sit
+//    from slot sfirst on and keep no list: k_rep_small steps them over all
+//    pairs staged in LDS;
+//  * the gather path (k_rep_list after the first build, k_rep_step) runs over
+//    the slots [ubeg, sfirst) only, what neither of the above holds;
 //  * k_rep_decide: a replica rebuilds before the next step exactly when that
 //    displacement reached (rs - rc)/2 -- the decision reads the positions the
 //    next step will use, so it needs no margin for lag and no host round trip.
