@@ -582,16 +582,45 @@ k_rep_predict(RepStep a) {
 }
 
 // a replica rebuilds before the next step when r(t+1) moved (rs - rc)/2 from
-// its list-build positions; clears the accumulator for the next step
-__global__ void k_rep_decide(int32_t R, unsigned long long* __restrict__ dmax, int32_t* __restrict__ rebuild,
-                             long long* __restrict__ builds, double margin2) {
-    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= R) return;
-    const double d2 = __longlong_as_double((long long)dmax[r]);
-    const int32_t f = d2 >= margin2 ? 1 : 0;
-    rebuild[r] = f;
-    builds[r] += f;
-    dmax[r] = 0ull;
+// its list-build positions; clears the accumulator for the next step.  With
+// staged replicas (sbeg non-null) it also writes the next list build's work
+// list, as k_rep_worklist does.  One workgroup, the replicas in rounds.
+__global__ void __launch_bounds__(kSortBlock)
+k_rep_decide(int32_t R, unsigned long long* __restrict__ dmax, int32_t* __restrict__ rebuild,
+             long long* __restrict__ builds, double margin2, const int32_t* __restrict__ sbeg,
+             int32_t* __restrict__ wl, int32_t* __restrict__ count) {
+    __shared__ int32_t wsum[kSortBlock / kWave];
+    __shared__ int32_t carry;
+    const int32_t tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (int32_t r0 = 0; r0 < R; r0 += kSortBlock) {
+        const int32_t r = r0 + tid;
+        int32_t f = 0;
+        if (r < R) {
+            const double d2 = __longlong_as_double((long long)dmax[r]);
+            f = d2 >= margin2 ? 1 : 0;
+            rebuild[r] = f;
+            builds[r] += f;
+            dmax[r] = 0ull;
+        }
+        if (!sbeg) continue;   // uniform
+        const int32_t m = f ? sbeg[r + 1] - sbeg[r] : 0;
+        int32_t inc = m;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int32_t y = __shfl_up(inc, o, kWave);
+            if (lane >= o) inc += y;
+        }
+        if (lane == kWave - 1) wsum[wv] = inc;
+        __syncthreads();
+        int32_t run = carry + inc - m;
+        for (int32_t w = 0; w < wv; w++) run += wsum[w];
+        for (int32_t k = 0; k < m; k++) wl[run + k] = sbeg[r] + k;
+        __syncthreads();   // carry and wsum read by every thread
+        if (tid == kSortBlock - 1) carry = run + m;
+        __syncthreads();
+    }
+    if (sbeg && tid == 0) *count = carry;
 }
 
 // the position records of one buffer from its three component arrays
@@ -817,8 +846,10 @@ int enqueue_rebuild(mdx_batch* b, bool flip, bool start = false) {
     if (rc) return rc;
     if (staged) {
         const double rl2 = (double)b->cfg.lj_rs * (double)b->cfg.lj_rs;
-        BLAUNCH(b, "rep_worklist", (k_rep_worklist<<<1, kSortBlock, 0, s>>>(b->R, b->d_rebuild, b->d_sbeg, b->d_wl,
-                                                                             b->d_wcount)));
+        // after the first build k_rep_decide writes the work list with the flags
+        if (start)
+            BLAUNCH(b, "rep_worklist", (k_rep_worklist<<<1, kSortBlock, 0, s>>>(b->R, b->d_rebuild, b->d_sbeg,
+                                                                                 b->d_wl, b->d_wcount)));
         BLAUNCH(b, "rep_nlist", launch_rep_nlist(s, rep_segs(b), b->d_wl, b->d_wcount, hp, rl2, b->rs_cap,
                                                  b->d_rs_list, b->d_rs_count, b->d_rs_err, lds_capacity()));
     }
@@ -827,8 +858,10 @@ int enqueue_rebuild(mdx_batch* b, bool flip, bool start = false) {
 
 int enqueue_decide(mdx_batch* b) {
     const double margin = 0.5 * ((double)b->cfg.lj_rs - (double)b->cfg.lj_rcut);
-    BLAUNCH(b, "rep_decide", (k_rep_decide<<<(b->R + 255) / 256, 256, 0, b->stream>>>(b->R, b->d_dmax, b->d_rebuild,
-                                                                                       b->d_builds, margin * margin)));
+    const bool staged = b->nseg_blocks > 0;
+    BLAUNCH(b, "rep_decide", (k_rep_decide<<<1, kSortBlock, 0, b->stream>>>(
+                                 b->R, b->d_dmax, b->d_rebuild, b->d_builds, margin * margin,
+                                 staged ? b->d_sbeg : nullptr, b->d_wl, b->d_wcount)));
     return MDX_OK;
 }
 
