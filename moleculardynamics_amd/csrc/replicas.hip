@@ -85,31 +85,51 @@ __device__ inline int32_t cell_index(double y, double cells_per_length, int32_t 
 // into cells (histogram, scan and scatter in LDS), each cell then ordered by
 // atom index (an atom's rank among its cell's atoms), and a cell-ordered copy
 // (x, y, z wrapped into [0, L) as FP32, atom index in w).  tpos is scratch for
-// the unsorted copy.
+// the unsorted copy.  Every atom's cell stays in LDS (16 bits; the LDS holds
+// 6 bytes per atom of the batch's largest replica, cap), so after the first
+// pass no loop waits on a global read per iteration; the global loads of a
+// thread's atoms are issued kBatch at a time.
+constexpr int kPer = kMaxReplica / kSortBlock;   // atoms per thread at most
+constexpr int kBatch = 8;
 __global__ void __launch_bounds__(kSortBlock)
 k_rep_sort(RepView v, const double* __restrict__ rx, const double* __restrict__ ry, const double* __restrict__ rz,
            int32_t* __restrict__ cell_of, int32_t* __restrict__ cstart, int32_t* __restrict__ cstart_g,
            int32_t* __restrict__ sidx, float4* __restrict__ tpos, float4* __restrict__ spos,
-           const uint8_t* __restrict__ staged) {
+           const uint8_t* __restrict__ staged, int32_t cap) {
     const int32_t r = blockIdx.x;
     if (!v.rebuild[r]) return;
     __shared__ int32_t cnt[kMaxCells + 1];
     __shared__ int32_t beg[kMaxCells + 1];
     __shared__ int32_t wsum[kSortBlock / kWave];
-    __shared__ int32_t lidx[kMaxReplica];   // atoms in scatter order
-    __shared__ int32_t sorted[kMaxReplica]; // atoms in cell order, ascending index within a cell
+    extern __shared__ int32_t sort_lds[];
+    int32_t* lidx = sort_lds;                                      // atoms in scatter order
+    uint16_t* cell = reinterpret_cast<uint16_t*>(sort_lds + cap);  // cell of each atom
     const int32_t tid = threadIdx.x;
     const int32_t n = v.nat[r], base = v.off[r], nc = v.nc[r], cb = v.coff[r];
     const int32_t ncl = nc ? nc * nc * nc : 1;
     const double L = v.L[r], invL = 1.0 / L, cpl = nc * invL;
     for (int32_t c = tid; c < ncl; c += kSortBlock) cnt[c] = 0;
     __syncthreads();
-    for (int32_t i = tid; i < n; i += kSortBlock) {
-        const double x = wrap(rx[base + i], L, invL), y = wrap(ry[base + i], L, invL), z = wrap(rz[base + i], L, invL);
-        const int32_t c = nc ? (cell_index(x, cpl, nc) * nc + cell_index(y, cpl, nc)) * nc + cell_index(z, cpl, nc) : 0;
-        cell_of[base + i] = c;
-        tpos[base + i] = make_float4((float)x, (float)y, (float)z, __int_as_float(i));
-        atomicAdd(&cnt[c], 1);
+    for (int32_t k0 = 0; k0 < kPer; k0 += kBatch) {
+        if (tid + k0 * kSortBlock >= n) break;
+        double px[kBatch], py[kBatch], pz[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; k++) {
+            const int32_t i = tid + (k0 + k) * kSortBlock;
+            if (i < n) { px[k] = rx[base + i]; py[k] = ry[base + i]; pz[k] = rz[base + i]; }
+        }
+#pragma unroll
+        for (int k = 0; k < kBatch; k++) {
+            const int32_t i = tid + (k0 + k) * kSortBlock;
+            if (i >= n) continue;
+            const double x = wrap(px[k], L, invL), y = wrap(py[k], L, invL), z = wrap(pz[k], L, invL);
+            const int32_t c =
+                nc ? (cell_index(x, cpl, nc) * nc + cell_index(y, cpl, nc)) * nc + cell_index(z, cpl, nc) : 0;
+            cell_of[base + i] = c;
+            cell[i] = (uint16_t)c;
+            tpos[base + i] = make_float4((float)x, (float)y, (float)z, __int_as_float(i));
+            atomicAdd(&cnt[c], 1);
+        }
     }
     __syncthreads();
     // exclusive scan over the cells: contiguous chunks per thread, the chunk
@@ -140,44 +160,43 @@ k_rep_sort(RepView v, const double* __restrict__ rx, const double* __restrict__ 
     __syncthreads();
     if (nc == 0) {
         // one cell: ascending index order is the identity
-        for (int32_t i = tid; i < n; i += kSortBlock) sorted[i] = i;
+        for (int32_t i = tid; i < n; i += kSortBlock) sidx[base + i] = i;
     } else {
-        for (int32_t i = tid; i < n; i += kSortBlock) lidx[atomicAdd(&cnt[cell_of[base + i]], 1)] = i;
+        for (int32_t i = tid; i < n; i += kSortBlock) lidx[atomicAdd(&cnt[cell[i]], 1)] = i;
         __syncthreads();
         // an atom's slot: its cell's start plus the number of the cell's atoms
-        // with a smaller index (independent reads, ~20-50 per atom)
+        // with a smaller index (independent LDS reads, ~20-50 per atom)
         for (int32_t p = tid; p < n; p += kSortBlock) {
-            const int32_t i = lidx[p], c = cell_of[base + i];
+            const int32_t i = lidx[p], c = cell[i];
             const int32_t b = beg[c], e = beg[c + 1];
             int32_t rank = 0;
             for (int32_t q = b; q < e; q++) rank += lidx[q] < i ? 1 : 0;
-            sorted[b + rank] = i;
+            sidx[base + b + rank] = i;
         }
     }
-    __syncthreads();
+    __syncthreads();   // this workgroup's sidx writes are visible to it
     // a staged replica is permuted into this order next (k_rep_perm_*): its
     // records name the new slots and its cells follow the new order
     const bool perm = nc != 0 && staged[r];
-    for (int32_t p = tid; p < n; p += kSortBlock) {
-        const int32_t i = sorted[p];
-        sidx[base + p] = i;
-        float4 w = tpos[base + i];
-        if (perm) w.w = __int_as_float(p);
-        spos[base + p] = w;
-    }
-    if (!perm) return;
-    constexpr int kPer = kMaxReplica / kSortBlock;
-    int32_t hold[kPer];
+    for (int32_t k0 = 0; k0 < kPer; k0 += kBatch) {
+        if (tid + k0 * kSortBlock >= n) break;
+        int32_t at[kBatch];
 #pragma unroll
-    for (int k = 0; k < kPer; k++) {
-        const int32_t p = tid + k * kSortBlock;
-        if (p < n) hold[k] = cell_of[base + sorted[p]];
-    }
-    __syncthreads();
+        for (int k = 0; k < kBatch; k++) {
+            const int32_t p = tid + (k0 + k) * kSortBlock;
+            at[k] = p < n ? sidx[base + p] : 0;
+        }
 #pragma unroll
-    for (int k = 0; k < kPer; k++) {
-        const int32_t p = tid + k * kSortBlock;
-        if (p < n) cell_of[base + p] = hold[k];
+        for (int k = 0; k < kBatch; k++) {
+            const int32_t p = tid + (k0 + k) * kSortBlock;
+            if (p >= n) continue;
+            float4 w = tpos[base + at[k]];
+            if (perm) {
+                w.w = __int_as_float(p);
+                cell_of[base + p] = cell[at[k]];
+            }
+            spos[base + p] = w;
+        }
     }
 }
 
@@ -697,6 +716,7 @@ struct mdx_batch {
     int32_t n_unstaged = 0;               // replicas on the gather path,
     int32_t ubeg = 0, sfirst = 0;         // in slots [ubeg, sfirst); the small replicas from sfirst on
     int2* d_small_blocks = nullptr;       // k_rep_small's workgroups: (replica, first atom)
+    int32_t sort_cap = 0;                 // atoms of the largest replica (k_rep_sort's LDS)
     int32_t nsmall_blocks = 0, small_lds = 0;
     long long* d_builds = nullptr;
     unsigned long long* d_dmax = nullptr;
@@ -828,9 +848,15 @@ int enqueue_rebuild(mdx_batch* b, bool flip, bool start = false) {
     hipStream_t s = b->stream;
     const RepView v = view(b);
     double* const* hp = flip ? b->rn : b->r;
-    BLAUNCH(b, "rep_sort", (k_rep_sort<<<b->R, kSortBlock, 0, s>>>(v, hp[0], hp[1], hp[2], b->d_cell_of, b->d_cstart,
-                                                                     b->d_cstart_g, b->d_sidx, b->d_tpos, b->d_spos,
-                                                                     b->d_staged)));
+    static const bool attr_set = [] {   // once per process, thread-safe
+        hipFuncSetAttribute(reinterpret_cast<const void*>(k_rep_sort), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            kMaxReplica * (int)(sizeof(int32_t) + sizeof(uint16_t)));
+        return true;
+    }();
+    (void)attr_set;
+    BLAUNCH(b, "rep_sort", (k_rep_sort<<<b->R, kSortBlock, (sizeof(int32_t) + sizeof(uint16_t)) * b->sort_cap, s>>>(
+                               v, hp[0], hp[1], hp[2], b->d_cell_of, b->d_cstart, b->d_cstart_g, b->d_sidx, b->d_tpos,
+                               b->d_spos, b->d_staged, b->sort_cap)));
     const bool staged = b->nseg_blocks > 0;
     if (staged) {
         const RepPerm pm = rep_perm(b, flip);
@@ -1021,6 +1047,7 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
         return bfail(nullptr, MDX_ERR_ARG, "batch too large");
     }
     b->off[n_replicas] = (int32_t)off;
+    b->sort_cap = *std::max_element(n_atoms, n_atoms + n_replicas);
     b->ubeg = (int32_t)ubeg;
     b->sfirst = (int32_t)sfirst;
     b->nsmall_blocks = (int32_t)small_blocks.size();
