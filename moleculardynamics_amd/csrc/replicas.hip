@@ -475,14 +475,16 @@ k_rep_step(RepStep a) {
 
 // Replicas narrower than 3 cells of rs per axis (all pairs), one workgroup per
 // kSmallAtoms of a replica's atoms (block table: replica, first atom): the
-// replica's positions r(t) staged in LDS, a quad of lanes per atom testing
-// every other atom by minimum image (lane q takes atoms q, q + 4, ...; the
-// quads of a wave read the same 4 atoms, which the LDS broadcasts), then the
-// Gear step of k_rep_step.  The pairs within rc are those of the list path;
+// replica's positions r(t) staged in LDS, 8 lanes per atom testing every
+// other atom by minimum image (lane q takes atoms q, q + 8, ...; the groups of
+// a wave read the same 8 atoms, which the LDS broadcasts; the two quads' sums
+// are added by one shuffle, the same in all 8 lanes), then the Gear step of
+// k_rep_step.  The pairs within rc are those of the list path;
 // only the order of the sum differs.  No list is kept: a replica that rebuilds
 // records its list-build positions here (its rebuild decision, count and sort
 // still run, for the energies' lists and the build counts).
-constexpr int kSmallAtoms = kBlock / 4;
+constexpr int kSmallLanes = 8;
+constexpr int kSmallAtoms = kBlock / kSmallLanes;
 template <bool kFp64, bool kFinal>
 __global__ void __launch_bounds__(kBlock)
 k_rep_small(RepStep a, const int2* __restrict__ blocks) {
@@ -496,15 +498,15 @@ k_rep_small(RepStep a, const int2* __restrict__ blocks) {
         stg[t] = a.r[d][base + j];
     }
     __syncthreads();
-    const int32_t li = bk.y + (int32_t)threadIdx.x / 4, q = threadIdx.x & 3;
-    const bool on = li < n;   // whole quads
+    const int32_t li = bk.y + (int32_t)threadIdx.x / kSmallLanes, q = threadIdx.x & (kSmallLanes - 1);
+    const bool on = li < n;   // whole groups
     double fx = 0, fy = 0, fz = 0;
     if (on) {
         const double xi = stg[3 * li], yi = stg[3 * li + 1], zi = stg[3 * li + 2];
         // the atom itself fails the cutoff test by a select (no branch, so the
         // loop unrolls and its LDS reads overlap)
 #pragma unroll 4
-        for (int32_t j = q; j < n; j += 4) {
+        for (int32_t j = q; j < n; j += kSmallLanes) {
             const double dx = mi(xi - stg[3 * j], L, invL), dy = mi(yi - stg[3 * j + 1], L, invL),
                          dz = mi(zi - stg[3 * j + 2], L, invL);
             const double rc2 = j == li ? -1.0 : a.rc2;
@@ -532,6 +534,9 @@ k_rep_small(RepStep a, const int2* __restrict__ blocks) {
     fx = quad_sum(fx);
     fy = quad_sum(fy);
     fz = quad_sum(fz);
+    fx += __shfl_xor(fx, 4, kWave);
+    fy += __shfl_xor(fy, 4, kWave);
+    fz += __shfl_xor(fz, 4, kWave);
     double d2 = 0.0;
     if (on && q < 3) {
         const double F = q == 0 ? fx : (q == 1 ? fy : fz);
