@@ -756,6 +756,9 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
     // back in the ring, was enqueued before)
     HIPCHK(s, hipMemsetAsync(slot_flags(-1), 0, W * sizeof(uint64_t), st));
     HIPCHK(s, hipMemsetAsync(slot_flags(0), 0, W * sizeof(uint64_t), st));
+    // the hand-over's arrival counters start each call at zero, whatever an
+    // earlier failed launch left in them
+    HIPCHK(s, hipMemsetAsync(s->dpub, 0, sizeof(unsigned int) * (kPubSub + 1), st));
     LAUNCH(s, "predict", launch_predict_track(st, s->st, n, s->gc, s->rref, slot_flags(-1)));
     if ((rc = post(-1))) return rc;
     int last_build = -1;   // step index whose positions were used for the last build
