@@ -493,9 +493,19 @@ k_rep_small(RepStep a, const int2* __restrict__ blocks) {
     const int32_t r = bk.x;
     const int32_t base = a.v.off[r], n = a.v.nat[r];
     const double L = a.v.L[r], invL = 1.0 / L;
-    for (int32_t t = threadIdx.x; t < 3 * n; t += kBlock) {
-        const int32_t j = t / 3, d = t - 3 * j;
-        stg[t] = a.r[d][base + j];
+    // staging: a thread's atoms' loads (4 atoms, 3 components) go out together
+    for (int32_t j0 = 0; j0 < n; j0 += 4 * kBlock) {
+        double bx[4], by[4], bz[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int32_t j = j0 + u * kBlock + (int32_t)threadIdx.x;
+            if (j < n) { bx[u] = a.r[0][base + j]; by[u] = a.r[1][base + j]; bz[u] = a.r[2][base + j]; }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int32_t j = j0 + u * kBlock + (int32_t)threadIdx.x;
+            if (j < n) { stg[3 * j] = bx[u]; stg[3 * j + 1] = by[u]; stg[3 * j + 2] = bz[u]; }
+        }
     }
     __syncthreads();
     const int32_t li = bk.y + (int32_t)threadIdx.x / kSmallLanes, q = threadIdx.x & (kSmallLanes - 1);
