@@ -134,3 +134,40 @@ def test_batch_energy_conservation_and_rebuilds():
         builds = b.build_counts()
     assert np.all(builds >= 1)          # every replica rebuilt its list at least once
     assert np.all(np.abs(e1 - e0) < 2e-3 * np.abs(e0) + 0.05)
+
+
+@pytest.mark.gpu
+def test_batch_gather_path_matches_oracle():
+    """A long cutoff (rc 4.5, rs 5.0) puts ~150 atoms in a cell of the 3-cell box,
+    more than the LDS staging takes, so that box runs on the gather kernels
+    (k_rep_list, k_rep_step); the 2-cell box takes the all-pairs kernel.  20 FP64
+    steps of each against the oracle's Gear run with the exact pair set."""
+    rc, rs, steps = 4.5, 5.0, 20
+    rng = np.random.default_rng(21)
+    ns, Ls, ps, vs = [], [], [], []
+    for k, T in [(10, 1.0), (6, 1.5)]:
+        p, L = initial.fcc(k, 0.8)
+        p = np.mod(p + rng.normal(0, 0.03, p.shape), L)
+        ns.append(len(p)); Ls.append(L); ps.append(p)
+        vs.append(initial.maxwell_boltzmann(len(p), T, int(rng.integers(1 << 30))))
+    with Batch(np.array(ns, np.int32), np.array(Ls), precision=PREC_FP64, dt=0.005, lj_rs=rs, lj_rcut=rc) as b:
+        assert b.paths().tolist() == [0, 2]
+        b.set_state(np.concatenate(ps), np.concatenate(vs))
+        b.run(steps)
+        st = b.get_state()
+    off = 0
+    for n, L, p, v in zip(ns, Ls, ps, vs):
+        def lst(x):
+            return O.pbc_list(np.mod(x, L), L, rs)[0] if int(np.floor(L / rs)) >= 3 else full_list(n)
+        r, vel = p.copy(), v.copy()
+        a, b3, c = np.zeros_like(r), np.zeros_like(r), np.zeros_like(r)
+        a[:] = O.pbc_lj_forces(r, L, rc, lst(r))[0]
+        for _ in range(steps):
+            r, vel, a, b3, c = O.gear_predict(r, vel, a, b3, c, DT)
+            f = O.pbc_lj_forces(r, L, rc, lst(r))[0]
+            r, vel, a, b3, c = O.gear_correct(r, vel, f, a, b3, c, DT)
+        d = st["pos"][off: off + n] - r
+        d -= L * np.round(d / L)
+        assert np.abs(d).max() < 1e-10
+        assert rel_rms(st["vel"][off: off + n], vel) < 1e-10
+        off += n
