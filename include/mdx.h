@@ -158,6 +158,21 @@ int mdx_build_list(mdx_system* sys, int32_t which, double box_size_list,
  * ints and must be at least n * nlistsize (n = n_atoms or n_bond). */
 int mdx_copy_neighbor_list(mdx_system* sys, int32_t which, int32_t* out, int64_t capacity);
 
+/* The pair lists the periodic force kernels read, as caller atom indices:
+ * the atoms found within the list radius (lj_rs; mb_rs for Tersoff and its
+ * surrogate) at the last list build, which mdx_run makes.  Row i of out
+ * (width ints; -1 after counts[i] entries) holds atom i's partners in the
+ * list's own order.  *width receives the longest row; with out NULL nothing
+ * else is written (the call that sizes out).  capacity is in ints, at least
+ * n * width.  build_pos3 (may be NULL): the positions the lists were built
+ * from, 3 doubles per atom.  The reference's list
+ * (common/NeighborList.cpp:88-145) is the padded candidate list of
+ * mdx_copy_neighbor_list, which its force loops filter by distance
+ * (cpu/common/LennardJones.h:35-93); here the filtered list is what the
+ * kernels consume, so this export is what parity checks as a set. */
+int mdx_copy_pair_list(mdx_system* sys, int32_t* out, int64_t capacity, int32_t* counts, int32_t* width,
+                       double* build_pos3);
+
 /* cpu/common/Device.cpp:17-39 */
 int mdx_predict(mdx_system* sys);
 /* cpu/common/LennardJones.h:35-93: assigns the force */

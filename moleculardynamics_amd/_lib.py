@@ -72,6 +72,7 @@ def load() -> C.CDLL:
         "mdx_ready_list": [P, C.POINTER(D), C.POINTER(D)],
         "mdx_build_list": [P, I32, D, C.POINTER(I32), C.POINTER(I32)],
         "mdx_copy_neighbor_list": [P, I32, P, I64],
+        "mdx_copy_pair_list": [P, P, I64, P, C.POINTER(I32), P],
         "mdx_predict": [P], "mdx_lj_forces": [P], "mdx_bond_forces": [P],
         "mdx_correct": [P, C.POINTER(I32)], "mdx_accelerations": [P],
         "mdx_run": [P, I32], "mdx_build_count": [P, C.POINTER(I64)],
@@ -177,6 +178,19 @@ class System:
         out = np.zeros(max(items * nlistsize, 1), dtype=np.int32)
         self._chk("mdx_copy_neighbor_list", self.lib.mdx_copy_neighbor_list(self.h, which, _ptr(out), out.size))
         return out[: items * nlistsize].reshape(items, nlistsize)
+
+    def pair_list(self):
+        """(rows, counts, build positions) of the list the periodic force
+        kernels read: rows[i, :counts[i]] are atom i's partners (caller
+        indices), -1 after them."""
+        w = C.c_int32()
+        pos = np.zeros((self.n, 3))
+        self._chk("mdx_copy_pair_list", self.lib.mdx_copy_pair_list(self.h, None, 0, None, C.byref(w), _ptr(pos)))
+        rows = np.zeros((self.n, max(w.value, 1)), dtype=np.int32)
+        counts = np.zeros(self.n, dtype=np.int32)
+        self._chk("mdx_copy_pair_list", self.lib.mdx_copy_pair_list(self.h, _ptr(rows), rows.size, _ptr(counts),
+                                                                    C.byref(w), None))
+        return rows[:, : w.value], counts, pos
 
     def predict(self):
         self._chk("mdx_predict", self.lib.mdx_predict(self.h))

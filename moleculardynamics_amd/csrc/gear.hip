@@ -162,7 +162,7 @@ k_predict_track(DevState st, int32_t n, GearCoeffs gc, const double* __restrict_
 // list-build positions and the single-step move max-reduced into the spread
 // flags, as k_predict_track does.
 // With kGather the force is first gathered from the per-slot gradients (and
-// written to st.f on the final step of a run) and atoms at least sqrt(hard2) from the
+// written to st.f on the final step of a run) and atoms farther than sqrt(hard2) from the
 // list-build positions are counted as stale, as the bond-order kernels do.
 template <bool kFinal, bool kGather>
 __global__ void __launch_bounds__(kBlock)
@@ -202,7 +202,7 @@ k_correct_predict_track(DevState st, int32_t n, GearCoeffs gc, MassTable m, cons
                 const double e = st.r[d][i] - r0[d][i];
                 e2 += e * e;
             }
-            stale = e2 >= hard2;
+            stale = e2 > hard2;
         }
         const unsigned long long nst = __popcll(__ballot(stale));
         if ((threadIdx.x & (kWave - 1)) == 0 && nst)

@@ -190,13 +190,10 @@ struct Stage {
     unsigned long long red[3];          // step kernel: max disp^2 bits, max step^2 bits, stale count
 };
 
-// Stage the neighbourhood of segment (cx, cy, z0..z1), positions minus the
-// origin o (zero for the FP64 staging: the values are then exactly the
-// images).  Returns false (for the whole block) when it does not fit in LDS.
-template <typename PosT>
-__device__ inline bool stage(Stage& S, const PosT& P, const SegGrid& g, int32_t cx, int32_t cy, int32_t z0,
-                             int32_t z1, const double* __restrict__ rx, const double* __restrict__ ry,
-                             const double* __restrict__ rz, double ox = 0.0, double oy = 0.0, double oz = 0.0) {
+// The staging layout of segment (cx, cy, z0..z1): per column its LDS start,
+// the prefix counts of its window cells and the slot ranges of its copy
+// pieces (the whole block takes part; S is complete on return).
+__device__ inline void stage_header(Stage& S, const SegGrid& g, int32_t cx, int32_t cy, int32_t z0, int32_t z1) {
     const int32_t K2 = z1 - z0 + 2;    // cells per column in the window
     const int32_t tid = threadIdx.x;
     if (tid < 3) S.red[tid] = 0;
@@ -236,6 +233,26 @@ __device__ inline bool stage(Stage& S, const PosT& P, const SegGrid& g, int32_t 
         S.total = acc;
     }
     __syncthreads();
+}
+
+// the slot of staged atom w (LDS index) of a segment whose header is S
+__device__ inline int32_t staged_slot(const Stage& S, int32_t w) {
+    int32_t col = 0;
+    while (col < 8 && S.colbase[col + 1] <= w) col++;
+    w -= S.colbase[col];
+    const int32_t l0 = S.plen[col][0], l1 = S.plen[col][1];
+    return w < l0 ? S.psrc[col][0] + w : (w < l0 + l1 ? S.psrc[col][1] + (w - l0) : S.psrc[col][2] + (w - l0 - l1));
+}
+
+// Stage the neighbourhood of segment (cx, cy, z0..z1), positions minus the
+// origin o (zero for the FP64 staging: the values are then exactly the
+// images).  Returns false (for the whole block) when it does not fit in LDS.
+template <typename PosT>
+__device__ inline bool stage(Stage& S, const PosT& P, const SegGrid& g, int32_t cx, int32_t cy, int32_t z0,
+                             int32_t z1, const double* __restrict__ rx, const double* __restrict__ ry,
+                             const double* __restrict__ rz, double ox = 0.0, double oy = 0.0, double oz = 0.0) {
+    const int32_t tid = threadIdx.x;
+    stage_header(S, g, cx, cy, z0, z1);
     if (S.total + 1 > P.cap) return false;   // + the dummy atom
     // copy: kColThreads threads per column, kCopyBatch independent loads each
     if (tid < 9 * kColThreads) {
@@ -388,6 +405,41 @@ k_nlist_lds(SegGrid g, RepGeo rg, const double* __restrict__ rx, const double* _
     } else {
         if (seg_geometry(g, cx, cy, z0, z1) < 0) return;
         nlist_segment(S, dyn, g, cx, cy, z0, z1, rx, ry, rz, rl2, cap, list, count, err, lds_cap);
+    }
+}
+
+// The lists as caller atom indices (parity exports): entry kk of home slot i
+// becomes id[slot of its staged atom] in row id[i] of rows (width ints a
+// row); a count above width is reported in *err.  One block per segment.
+template <bool kRep>
+__global__ void __launch_bounds__(kBlock)
+k_nlist_decode(SegGrid g, RepGeo rg, const uint16_t* __restrict__ list, const int32_t* __restrict__ count,
+               int32_t nchunk, const int32_t* __restrict__ id, int32_t width, int32_t* __restrict__ rows,
+               int32_t* __restrict__ counts, int32_t* __restrict__ err) {
+    __shared__ Stage S;
+    int32_t cx, cy, z0, z1;
+    if constexpr (kRep) {
+        if ((int32_t)blockIdx.x >= rg.nblocks) return;
+        rep_geometry(rg, blockIdx.x, g, cx, cy, z0, z1);
+    } else if (seg_geometry(g, cx, cy, z0, z1) < 0) {
+        return;
+    }
+    stage_header(S, g, cx, cy, z0, z1);
+    const int32_t cbase = (cx * g.n[1] + cy) * g.n[2];
+    const int32_t hbeg = g.start[cbase + z0], hend = g.start[cbase + z1];
+    for (int32_t i = hbeg + (int32_t)threadIdx.x; i < hend; i += kBlock) {
+        const uint16_t* la = list + ((int64_t)(i / kListBlock) * nchunk * kListBlock + i % kListBlock) * (kLanes * kChunk);
+        const int32_t c = count[i], me = id[i];
+        counts[me] = c;
+        if (c > width) {
+            atomicMax(err, c);
+            continue;
+        }
+        for (int32_t kk = 0; kk < c; kk++) {
+            const int32_t w = la[(kk >> 5) * kChunkRow + (((kk & 3) << 3) | ((kk >> 2) & 7))];
+            // an index past the staged atoms (the padding's dummy) inside the count would be a bug: -2
+            rows[(int64_t)me * width + kk] = w < S.total ? id[staged_slot(S, w)] : -2;
+        }
     }
 }
 
@@ -562,7 +614,7 @@ k_step_lds(StepArgs a) {
         d2 = quad_sum(d2);
         mv2 = quad_sum(mv2);
         e2 = quad_sum(e2);
-        if (q == 0 && e2 >= a.hard2) stale += 1.0;
+        if (q == 0 && e2 > a.hard2) stale += 1.0;
         d2max = fmax(d2max, d2);
         mv2max = fmax(mv2max, mv2);
     }
@@ -711,6 +763,16 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
         else { if (force_out) MDX_STEP(false, false, true); else MDX_STEP(false, false, false); }
     }
 #undef MDX_STEP
+    return hipGetLastError();
+}
+
+hipError_t launch_nlist_decode(hipStream_t s, const GridView& v, int32_t nseg, const uint16_t* list,
+                               const int32_t* count, int32_t cap, const int32_t* id, int32_t width, int32_t* rows,
+                               int32_t* counts, int32_t* err) {
+    if (cap % (kLanes * kChunk) != 0) return hipErrorInvalidValue;
+    SegGrid g = make_grid(v, nseg);
+    k_nlist_decode<false><<<g.per_xcd * kXcd, kBlock, 0, s>>>(g, RepGeo{}, list, count, cap / (kLanes * kChunk) + 1,
+                                                              id, width, rows, counts, err);
     return hipGetLastError();
 }
 

@@ -270,6 +270,16 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
                            int32_t n, int64_t cs, int32_t precision, bool final_step, bool force_out, uint64_t* flags,
                            int32_t lds_atoms, uint64_t* clear_next, FlagPublish pub);
 
+// the lists of launch_nlist_lds as caller indices: row id[i] of rows (width
+// entries) holds atom i's partners in list order, counts[id[i]] their number;
+// *err receives a count above width
+hipError_t launch_nlist_decode(hipStream_t s, const GridView& v, int32_t nseg, const uint16_t* list,
+                               const int32_t* count, int32_t cap, const int32_t* id, int32_t width, int32_t* rows,
+                               int32_t* counts, int32_t* err);
+// the same for compact [k][slot] lists of slots (Tersoff)
+hipError_t launch_compact_decode(hipStream_t s, const int32_t* list, const int32_t* count, int32_t n,
+                                 const int32_t* id, int32_t width, int32_t* rows, int32_t* counts, int32_t* err);
+
 // tersoff.hip: Tersoff silicon in the periodic box (compact [k][slot] lists)
 TersoffParams tersoff_si_params();
 // list of the atoms within rs of each slot; err: longest list when above cap

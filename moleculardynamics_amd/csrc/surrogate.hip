@@ -283,7 +283,7 @@ k_sg_gather(Pos3 P, const int32_t* __restrict__ list, const int32_t* __restrict_
         fz[i] = F[2];
         if (r0x) {
             const double e0 = P.x[i] - r0x[i], e1 = P.y[i] - r0y[i], e2 = P.z[i] - r0z[i];
-            stale = e0 * e0 + e1 * e1 + e2 * e2 >= hard2;
+            stale = e0 * e0 + e1 * e1 + e2 * e2 > hard2;
         }
     }
     if (flags) {

@@ -62,7 +62,8 @@ struct mdx_system {
     double* partials = nullptr;
     int32_t nb = 1;
     double* dscal = nullptr;         // [0..7] kinetics, [8..9] sums
-    int32_t* dint = nullptr;         // [0] wall hit, [1] bin error, [2] max count, [3] max row
+    int32_t* dint = nullptr;         // [0] wall hit, [1] bin error, [2] max count, [3] max row,
+                                     // [4..6] list-build error words, [8..9] pair-list export
     uint64_t* dbits = nullptr;       // [0] max disp^2, [1] max v^2, [2] max |coord|
     double* e_atom = nullptr;        // 2n scratch
     void* pinned = nullptr;          // 256 bytes
@@ -701,6 +702,12 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
     const int W = mdx_system::kFlagWords;
     const int HW = mdx_system::kHostWords;
     auto ring = [&](int k) { return ((k % R) + R) % R; };
+    // the stale count in step k's flags is of that step's own positions
+    const int32_t t0 = s->t;
+    auto stale_error = [&](int k) {
+        return fail(s, MDX_ERR_LIST_STALE, "an atom moved past the neighbour-list margin: step " +
+                                               std::to_string((int64_t)t0 + k) + " used a stale list");
+    };
     auto slot_flags = [&](int k) { return s->dflags + W * (size_t)ring(k); };
     auto host_flags = [&](int k) { return s->hflags + HW * (size_t)ring(k); };
     // reduce the spread flags: max displacement^2, max step^2, stale count
@@ -770,7 +777,7 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
         double D, S;
         uint64_t stale;
         reduce(host_flags(fk), D, S, stale);
-        if (stale != 0) return fail(s, MDX_ERR_LIST_STALE, "an atom moved past the neighbour-list margin");
+        if (stale != 0) return stale_error(fk);
         // rebuild at 90% of the margin, estimating r(k) as one more step of at most
         // 1.2 S away; the step kernel's own check of the full margin (stale count)
         // catches anything the estimate misses
@@ -780,7 +787,10 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
             rebuild = D >= trigger;                // exact: these flags are of r(0)
         else if (fk >= last_build)
             rebuild = D + 1.2 * S >= trigger;
-        // (fk < last_build: the flags predate the last build, r(k) is one step from it)
+        else
+            // the flags predate the last build and r(k) is one step from it: only
+            // a step as long as the trigger (every step when rs = rc) forces a build
+            rebuild = 1.2 * S >= trigger;
         if (rebuild) {
             if ((rc = rebuild_lists())) return rc;
             last_build = k;
@@ -822,7 +832,7 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
         double D, S;
         uint64_t stale;
         reduce(host_flags(k), D, S, stale);
-        if (stale != 0) return fail(s, MDX_ERR_LIST_STALE, "an atom moved past the neighbour-list margin");
+        if (stale != 0) return stale_error(k);
     }
     return MDX_OK;
 }
@@ -1106,6 +1116,59 @@ int mdx_copy_neighbor_list(mdx_system* s, int32_t which, int32_t* out, int64_t c
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     hipFree(dout);
     if (e != hipSuccess) return fail(s, MDX_ERR_HIP, std::string("emit list: ") + hipGetErrorString(e));
+    return MDX_OK;
+}
+
+int mdx_copy_pair_list(mdx_system* s, int32_t* out, int64_t capacity, int32_t* counts, int32_t* width,
+                       double* build_pos3) {
+    if (!s || !width) return fail(s, MDX_ERR_ARG, "null argument");
+    if (s->cfg.box != MDX_BOX_PERIODIC)
+        return fail(s, MDX_ERR_ARG, "the walled box's kernels read the candidate rows (mdx_copy_neighbor_list)");
+    const bool ts = is_tersoff(s);
+    if (ts ? !s->ts_built : !(s->nlist && s->initialized))
+        return fail(s, MDX_ERR_STATE, "no pair list built (mdx_run builds it)");
+    const int32_t n = s->n;
+    hipStream_t st = s->stream;
+    const int32_t* cnt = ts ? s->ts_count : s->ncount;
+    HIPCHK(s, hipMemsetAsync(s->dint + 8, 0, 2 * sizeof(int32_t), st));
+    LAUNCH(s, "list_max_count", launch_max_count(st, cnt, n, s->dint + 8));
+    int32_t w;
+    int rc = read_ints(s, &w, s->dint + 8, 1);
+    if (rc) return rc;
+    *width = w;
+    if (build_pos3) {
+        // the list-build positions, slot order -> caller order (alt.f is scratch here)
+        LAUNCH(s, "list_positions", launch_scatter3(st, s->rref, s->st.id, s->alt.f, n));
+        std::vector<double> buf((size_t)n);
+        for (int d = 0; d < 3; d++) {
+            HIPCHK(s, hipMemcpyAsync(buf.data(), s->alt.f[d], sizeof(double) * n, hipMemcpyDeviceToHost, st));
+            HIPCHK(s, hipStreamSynchronize(st));
+            for (int i = 0; i < n; i++) build_pos3[3 * (size_t)i + d] = buf[i];
+        }
+    }
+    if (!out) return MDX_OK;
+    if (!counts) return fail(s, MDX_ERR_ARG, "counts is required with out");
+    const int64_t need = (int64_t)n * w;
+    if (capacity < need) return fail(s, MDX_ERR_CAPACITY, "output capacity below n * width");
+    int32_t* drows = nullptr;
+    int32_t* dcounts = nullptr;
+    HIPCHK(s, hipMalloc((void**)&drows, sizeof(int32_t) * std::max<int64_t>(need, 1)));
+    hipError_t e = hipMalloc((void**)&dcounts, sizeof(int32_t) * n);
+    if (e == hipSuccess) e = hipMemsetAsync(drows, 0xff, sizeof(int32_t) * std::max<int64_t>(need, 1), st);
+    if (e == hipSuccess)
+        e = ts ? launch_compact_decode(st, s->ts_list, s->ts_count, n, s->st.id, w, drows, dcounts, s->dint + 9)
+               : launch_nlist_decode(st, s->lj.view, s->nseg, s->nlist, s->ncount, s->nl_cap, s->st.id, w, drows,
+                                     dcounts, s->dint + 9);
+    if (e == hipSuccess && need > 0)
+        e = hipMemcpyAsync(out, drows, sizeof(int32_t) * need, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(counts, dcounts, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(drows);
+    hipFree(dcounts);
+    if (e != hipSuccess) return fail(s, MDX_ERR_HIP, std::string("pair list: ") + hipGetErrorString(e));
+    int32_t over;
+    if ((rc = read_ints(s, &over, s->dint + 9, 1))) return rc;
+    if (over) return fail(s, MDX_ERR_STATE, "a list count changed during the copy");
     return MDX_OK;
 }
 

@@ -142,6 +142,29 @@ hipError_t launch_ts_list(hipStream_t s, const DevState& st, const int32_t* cell
     return hipGetLastError();
 }
 
+// compact [k][slot] list of slots -> rows of caller indices (parity export)
+__global__ void __launch_bounds__(kBlock)
+k_compact_decode(const int32_t* __restrict__ list, const int32_t* __restrict__ count, int32_t n,
+                 const int32_t* __restrict__ id, int32_t width, int32_t* __restrict__ rows,
+                 int32_t* __restrict__ counts, int32_t* __restrict__ err) {
+    const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int32_t c = count[i], me = id[i];
+    counts[me] = c;
+    if (c > width) {
+        atomicMax(err, c);
+        return;
+    }
+    for (int32_t k = 0; k < c; k++) rows[(int64_t)me * width + k] = id[list[(int64_t)k * n + i]];
+}
+
+hipError_t launch_compact_decode(hipStream_t s, const int32_t* list, const int32_t* count, int32_t n,
+                                 const int32_t* id, int32_t width, int32_t* rows, int32_t* counts, int32_t* err) {
+    if (n <= 0) return hipSuccess;
+    k_compact_decode<<<grid_for(n), kBlock, 0, s>>>(list, count, n, id, width, rows, counts, err);
+    return hipGetLastError();
+}
+
 hipError_t launch_ts_atoms(hipStream_t s, const DevState& st, double L, const TersoffParams& p, int32_t precision,
                            const int32_t* list, const int32_t* count, int32_t n, float4* w, double4* wd,
                            double* e_atom) {

@@ -95,12 +95,21 @@ double orc_ref_param(int32_t which, int32_t ti, int32_t tj);
  * Host.cpp:94-106); a/b/c start at zero as at Host.cpp:86-88.  Writes
  * the energy record (t, T, E, EK, EU, COM, P) at each output step into
  * records (11 doubles each, capacity max_records); returns the number of
- * records written, or -1 on allocation failure. */
+ * records written, or -1 on allocation failure.
+ * wall: 0 puts the walls at +-box (box = N^(1/3) * 52.8, Host.cpp:31,166);
+ * a positive value puts them there instead (tests that drive atoms into the
+ * walls).  stats (may be NULL) receives ORC_RUN_STATS counts: [0..6] atoms
+ * turned back per wall branch (x+ reflect, x+ teleport, x-, y+, y-, z+, z-),
+ * [7] steps with a wall hit (forces recomputed), [8] Lennard-Jones list
+ * builds in the loop, [9] of those the builds that came after a teleport. */
+#define ORC_RUN_STATS 10
 typedef struct {
     int32_t n_lj, n_mb, timesteps, output_timesteps;
     float dt, lj_rs, lj_rcut, mb_rs, mb_rcut, energy_loss /* raw, sqrt applied inside */;
     float vxcm;
     int32_t use_lj, use_mb;
+    double wall;
+    int64_t* stats;
 } orc_ref_run_cfg;
 int32_t orc_ref_run(const orc_ref_run_cfg* cfg, orc_real4* r, orc_real3* v,
                     double* records, int32_t max_records, double* elapsed_s);
@@ -113,6 +122,11 @@ int32_t orc_ref_run(const orc_ref_run_cfg* cfg, orc_real4* r, orc_real3* v,
  * cell, self excluded.  Returns malloc'ed n*nlistsize ints (pad -1). */
 int32_t* orc_pbc_build_list(const double* pos3, int32_t n, double L, double rs,
                             int32_t* nlistsize, int32_t* nlargestsize);
+
+/* Check of an exported pair list against the exact FP64 sets within rs and
+ * rc at the build positions; see oracle_pbc.c for the 8 counts in stats. */
+int32_t orc_pbc_pair_list_check(const double* pos3, int32_t n, double L, double rs, double rc, double slack2,
+                                const int32_t* rows, const int32_t* counts, int32_t width, int64_t* stats);
 
 /* LJ forces over a candidate list with the minimum image; single species
  * (sigma, eps).  f is assigned.  Returns the truncated, shifted potential

@@ -24,6 +24,7 @@ static double wrap01(double x, double L) {
 }
 
 static double min_image(double d, double L) { return d - L * nearbyint(d / L); }
+static double sq3(double x, double y, double z) { return x * x + y * y + z * z; }
 
 /* ---------------- periodic candidate list ---------------- */
 /*
@@ -84,6 +85,87 @@ int32_t* orc_pbc_build_list(const double* pos3, int32_t n, double L, double rs,
     *nlargestsize = longest;
     free(cell); free(start); free(members); free(fill);
     return out;
+}
+
+/* Check of an exported pair list (rows of width ints, counts[i] partners of
+ * atom i) against the exact FP64 sets at the build positions: every j != i
+ * with |x_i - x_j|^2 < rs^2 (minimum image) must be in row i; no entry may
+ * lie at rs^2 + slack2 or beyond; no entry may repeat, be i or be out of
+ * range.  The exact sets come from a periodic cell grid of floor(L/rs) cells
+ * per axis (the 27-cell stencil holds every atom within rs).  stats:
+ * [0] entries, [1] pairs within rs missing from the list, [2] entries at or
+ * beyond rs^2 + slack2, [3] entries in [rs^2, rs^2 + slack2), [4] repeated,
+ * self or out-of-range entries, [5] pairs within rc, [6] of those in the
+ * list, [7] pairs within rs.  Returns 0, or -1 (grid narrower than 3 cells,
+ * allocation failure). */
+int32_t orc_pbc_pair_list_check(const double* pos3, int32_t n, double L, double rs, double rc, double slack2,
+                                const int32_t* rows, const int32_t* counts, int32_t width, int64_t* stats) {
+    int32_t nc = (int32_t)floor(L / rs);
+    if (nc < 3) return -1;
+    double h = L / nc;
+    size_t ncell = (size_t)nc * nc * nc;
+    int32_t* cell = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    int32_t* start = (int32_t*)calloc(ncell + 1, sizeof(int32_t));
+    int32_t* members = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    int32_t* fill = (int32_t*)malloc(sizeof(int32_t) * ncell);
+    int32_t* sorted = (int32_t*)malloc(sizeof(int32_t) * ((size_t)width + 1));
+    if (!cell || !start || !members || !fill || !sorted) return -1;
+    for (int32_t i = 0; i < n; i++) {
+        int32_t c[3];
+        for (int d = 0; d < 3; d++) {
+            c[d] = (int32_t)(wrap01(pos3[3 * i + d], L) / h);
+            if (c[d] >= nc) c[d] = nc - 1;
+        }
+        cell[i] = (c[0] * nc + c[1]) * nc + c[2];
+        start[cell[i] + 1]++;
+    }
+    for (size_t c = 0; c < ncell; c++) start[c + 1] += start[c];
+    memcpy(fill, start, sizeof(int32_t) * ncell);
+    for (int32_t i = 0; i < n; i++) members[fill[cell[i]]++] = i;
+    for (int k = 0; k < 8; k++) stats[k] = 0;
+    const double rs2 = rs * rs, rc2 = rc * rc;
+#define R2(i, j) sq3(min_image(pos3[3 * (i)] - pos3[3 * (j)], L), min_image(pos3[3 * (i) + 1] - pos3[3 * (j) + 1], L), \
+                     min_image(pos3[3 * (i) + 2] - pos3[3 * (j) + 2], L))
+    for (int32_t i = 0; i < n; i++) {
+        const int32_t m = counts[i];
+        if (m < 0 || m > width) { stats[4]++; continue; }
+        /* the row's entries, sorted (insertion sort: rows are short) */
+        for (int32_t k = 0; k < m; k++) {
+            int32_t v = rows[(size_t)i * width + k], q = k;
+            while (q > 0 && sorted[q - 1] > v) { sorted[q] = sorted[q - 1]; q--; }
+            sorted[q] = v;
+        }
+        stats[0] += m;
+        for (int32_t k = 0; k < m; k++) {
+            int32_t j = sorted[k];
+            if (j < 0 || j >= n || j == i || (k > 0 && sorted[k - 1] == j)) { stats[4]++; continue; }
+            double r2 = R2(i, j);
+            if (r2 >= rs2 + slack2) stats[2]++;
+            else if (r2 >= rs2) stats[3]++;
+        }
+        int32_t cz = cell[i] % nc, cy = (cell[i] / nc) % nc, cx = cell[i] / (nc * nc);
+        for (int dx = -1; dx <= 1; dx++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dz = -1; dz <= 1; dz++) {
+                    int32_t x = (cx + dx + nc) % nc, y = (cy + dy + nc) % nc, z = (cz + dz + nc) % nc;
+                    int32_t c = (x * nc + y) * nc + z;
+                    for (int32_t p = start[c]; p < start[c + 1]; p++) {
+                        int32_t j = members[p];
+                        if (j == i) continue;
+                        double r2 = R2(i, j);
+                        if (!(r2 < rs2)) continue;
+                        stats[7]++;
+                        int32_t lo = 0, hi = m;   /* binary search in the sorted row */
+                        while (lo < hi) { int32_t mid = (lo + hi) / 2; if (sorted[mid] < j) lo = mid + 1; else hi = mid; }
+                        int found = lo < m && sorted[lo] == j;
+                        if (!found) stats[1]++;
+                        if (r2 < rc2) { stats[5]++; if (found) stats[6]++; }
+                    }
+                }
+    }
+#undef R2
+    free(cell); free(start); free(members); free(fill); free(sorted);
+    return 0;
 }
 
 /* 12-6 LJ over candidate rows, minimum image, r < rc; energy shifted so the

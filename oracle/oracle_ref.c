@@ -253,9 +253,11 @@ void orc_ref_predict(orc_real4* r, orc_real3* v, orc_real3* a, orc_real3* b, orc
  * +x wall reflects with restitution energy_loss for y > 0 and teleports the
  * atom to -npos for y < 0 (y == 0 exactly: no action); -x, +-y, +-z reflect.
  * Any wall action sets *flag. */
-void orc_ref_correct(orc_real4* r, orc_real3* v, const orc_real3* f, orc_real3* a,
-                     orc_real3* b, orc_real3* c, double dt, int32_t n, double npos,
-                     int32_t* flag, float energy_loss) {
+/* hits (may be NULL): per wall branch, the atoms it acted on, in the order
+ * x+ reflect, x+ teleport, x-, y+, y-, z+, z- */
+static void correct_counted(orc_real4* r, orc_real3* v, const orc_real3* f, orc_real3* a,
+                            orc_real3* b, orc_real3* c, double dt, int32_t n, double npos,
+                            int32_t* flag, float energy_loss, int64_t* hits) {
     double k[4];
     gear_coeffs(dt, k);
     double cr = P_GEAR1 * k[1], cv = P_GEAR2 * k[1] / k[0];
@@ -274,13 +276,24 @@ void orc_ref_correct(orc_real4* r, orc_real3* v, const orc_real3* f, orc_real3* 
             B[d] = B[d] + cb * corr;
             Cc[d] = Cc[d] + cc * corr;
         }
-        if (R[0] > npos && R[1] > 0) { hit = 1; V[0] = -V[0] * energy_loss; R[0] = npos; }
-        if (R[0] > npos && R[1] < 0) { hit = 1; R[0] = -npos; }
-        if (R[0] < -npos) { hit = 1; V[0] = -V[0]; R[0] = -npos; }
+        if (R[0] > npos && R[1] > 0) { hit = 1; V[0] = -V[0] * energy_loss; R[0] = npos; if (hits) hits[0]++; }
+        if (R[0] > npos && R[1] < 0) { hit = 1; R[0] = -npos; if (hits) hits[1]++; }
+        if (R[0] < -npos) { hit = 1; V[0] = -V[0]; R[0] = -npos; if (hits) hits[2]++; }
         for (int d = 1; d < 3; d++)
-            if (fabs(R[d]) > npos) { hit = 1; V[d] = -V[d]; R[d] = npos * (R[d] / fabs(R[d])); }
+            if (fabs(R[d]) > npos) {
+                hit = 1;
+                if (hits) hits[2 * d + (R[d] > 0 ? 1 : 2)]++;
+                V[d] = -V[d];
+                R[d] = npos * (R[d] / fabs(R[d]));
+            }
     }
     *flag = hit;
+}
+
+void orc_ref_correct(orc_real4* r, orc_real3* v, const orc_real3* f, orc_real3* a,
+                     orc_real3* b, orc_real3* c, double dt, int32_t n, double npos,
+                     int32_t* flag, float energy_loss) {
+    correct_counted(r, v, f, a, b, c, dt, n, npos, flag, energy_loss, NULL);
 }
 
 void orc_ref_accelerations(const orc_real4* r, orc_real3* acc, const orc_real3* f, int32_t n) {
@@ -643,7 +656,11 @@ int32_t orc_ref_run(const orc_ref_run_cfg* cfg, orc_real4* r, orc_real3* v,
     int32_t n = cfg->n_lj + cfg->n_mb;
     double dt = (double)cfg->dt;
     double box = pow((double)n, 1.0 / 3.0) * 52.8;
+    const double npos = cfg->wall > 0 ? cfg->wall : box;   /* Host.cpp:166: the walls sit at +-boxSize */
     float eloss = sqrtf(cfg->energy_loss);
+    int64_t* st = cfg->stats;
+    if (st) memset(st, 0, sizeof(int64_t) * ORC_RUN_STATS);
+    int32_t teleported = 0;   /* a teleport happened since the last LJ list build */
     orc_real3* f = (orc_real3*)calloc((size_t)n, sizeof(orc_real3));
     orc_real3* a = (orc_real3*)calloc((size_t)n, sizeof(orc_real3));
     orc_real3* b = (orc_real3*)calloc((size_t)n, sizeof(orc_real3));
@@ -680,14 +697,24 @@ int32_t orc_ref_run(const orc_ref_run_cfg* cfg, orc_real4* r, orc_real3* v,
     while (rc == 0 && t < cfg->timesteps) {
         if (lj.next == t || mb.next == t) {
             orc_ref_ready_list(r, v, n, &vmax, &bsl);
-            if (lj.on && lj.next == t && rebuild(&lj, r, bsl, buckets, vmax, dt, t)) rc = -1;
+            if (lj.on && lj.next == t) {
+                if (rebuild(&lj, r, bsl, buckets, vmax, dt, t)) rc = -1;
+                if (st) { st[8]++; st[9] += teleported; }
+                teleported = 0;
+            }
             if (mb.on && mb.next == t && rebuild(&mb, r, bsl, buckets, vmax, dt, t)) rc = -1;
         }
         orc_ref_predict(r, v, a, b, c, dt, n);
         FORCES();
         int32_t flag = 0;
-        orc_ref_correct(r, v, f, a, b, c, dt, n, box, &flag, eloss);
-        if (flag) { FORCES(); orc_ref_accelerations(r, a, f, n); }
+        const int64_t tele0 = st ? st[1] : 0;
+        correct_counted(r, v, f, a, b, c, dt, n, npos, &flag, eloss, st);
+        if (st && st[1] > tele0) teleported = 1;
+        if (flag) {
+            if (st) st[7]++;
+            FORCES();
+            orc_ref_accelerations(r, a, f, n);
+        }
         if (printout == t) {
             if (nrec < max_records) record(records + 11 * nrec++, r, v, n, t, dt);
             printout += cfg->output_timesteps;

@@ -26,7 +26,7 @@ class RunCfg(C.Structure):
                 ("output_timesteps", C.c_int32), ("dt", C.c_float), ("lj_rs", C.c_float),
                 ("lj_rcut", C.c_float), ("mb_rs", C.c_float), ("mb_rcut", C.c_float),
                 ("energy_loss", C.c_float), ("vxcm", C.c_float), ("use_lj", C.c_int32),
-                ("use_mb", C.c_int32)]
+                ("use_mb", C.c_int32), ("wall", C.c_double), ("stats", C.c_void_p)]
 
 
 class TersoffParams(C.Structure):
@@ -65,6 +65,8 @@ def lib() -> C.CDLL:
         L.orc_ref_run.argtypes = [C.POINTER(RunCfg), P, P, P, I, C.POINTER(D)]
         L.orc_pbc_build_list.restype = C.POINTER(C.c_int32)
         L.orc_pbc_build_list.argtypes = [P, I, D, D, C.POINTER(I), C.POINTER(I)]
+        L.orc_pbc_pair_list_check.argtypes = [P, I, D, D, D, D, P, P, I, P]
+        L.orc_pbc_pair_list_check.restype = I
         L.orc_pbc_lj_forces.restype = D
         L.orc_pbc_lj_forces.argtypes = [P, P, I, D, D, D, D, I, P, C.POINTER(D)]
         L.orc_gear_predict.argtypes = [P, P, P, P, P, D, I]
@@ -184,15 +186,24 @@ def param(which, ti, tj):
     return lib().orc_ref_param(which, ti, tj)
 
 
+RUN_STATS = ("x+ reflect", "x+ teleport", "x-", "y+", "y-", "z+", "z-", "hit steps", "lj builds",
+             "builds after teleport")
+
+
 def ref_run(pos4, vel3, n_lj, n_mb, timesteps, output_timesteps, dt, lj_rs, lj_rcut, mb_rs, mb_rcut,
-            energy_loss=1.0, vxcm=0.0, use_lj=True, use_mb=False, max_records=64):
+            energy_loss=1.0, vxcm=0.0, use_lj=True, use_mb=False, max_records=64, wall=0.0, stats=False):
+    """The reference loop; wall > 0 moves the walls to +-wall.  With
+    stats=True a fifth value holds the loop's counts keyed by RUN_STATS."""
+    st = np.zeros(len(RUN_STATS), dtype=np.int64)
     cfg = RunCfg(n_lj, n_mb, timesteps, output_timesteps, dt, lj_rs, lj_rcut, mb_rs, mb_rcut,
-                 energy_loss, vxcm, 1 if use_lj else 0, 1 if use_mb else 0)
+                 energy_loss, vxcm, 1 if use_lj else 0, 1 if use_mb else 0, wall, st.ctypes.data)
     r, v = f64(pos4).copy(), f64(vel3).copy()
     rec = np.zeros((max_records, 11))
     el = C.c_double()
     k = lib().orc_ref_run(C.byref(cfg), ptr(r), ptr(v), ptr(rec), max_records, C.byref(el))
     assert k >= 0
+    if stats:
+        return r, v, rec[:k], el.value, dict(zip(RUN_STATS, st.tolist()))
     return r, v, rec[:k], el.value
 
 
@@ -206,6 +217,22 @@ def pbc_list(pos3, L, rs):
     arr = np.ctypeslib.as_array(p, shape=(max(n * nls.value, 1),))[: n * nls.value].copy().reshape(n, nls.value)
     lib().orc_free(C.cast(p, C.c_void_p))
     return arr, nlg.value
+
+
+PAIR_CHECK = ("entries", "missing within rs", "beyond rs + slack", "in slack band", "bad entries",
+              "pairs within rc", "rc pairs listed", "pairs within rs")
+
+
+def pair_list_check(pos3, L, rs, rc, rows, counts, slack2):
+    """Counts (keyed by PAIR_CHECK) of an exported pair list against the
+    exact FP64 sets within rs and rc at the build positions pos3."""
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    counts = np.ascontiguousarray(counts, dtype=np.int32)
+    st = np.zeros(8, dtype=np.int64)
+    rc_ = lib().orc_pbc_pair_list_check(ptr(f64(pos3)), len(pos3), L, rs, rc, slack2, ptr(rows), ptr(counts),
+                                        rows.shape[1], ptr(st))
+    assert rc_ == 0
+    return dict(zip(PAIR_CHECK, st.tolist()))
 
 
 def pbc_lj_forces(pos3, L, rc, nlist, sigma=1.0, eps=1.0):

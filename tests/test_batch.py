@@ -45,9 +45,10 @@ def oracle_list(p, L):
     return full_list(len(p))
 
 
-def oracle_run(p, v, L, steps):
+def oracle_run(p, v, L, steps, with_force=False):
     """Gear PC with the oracle's forces over a list rebuilt from the current
-    positions at every step (the exact pair set within rc)."""
+    positions at every step (the exact pair set within rc); with_force also
+    returns the last force (of the last step's predicted positions)."""
     r, vel = p.copy(), v.copy()
     a, b, c = np.zeros_like(r), np.zeros_like(r), np.zeros_like(r)
     f, _, _ = O.pbc_lj_forces(r, L, RC, oracle_list(np.mod(r, L), L))
@@ -56,7 +57,7 @@ def oracle_run(p, v, L, steps):
         r, vel, a, b, c = O.gear_predict(r, vel, a, b, c, DT)
         f, _, _ = O.pbc_lj_forces(r, L, RC, oracle_list(np.mod(r, L), L))
         r, vel, a, b, c = O.gear_correct(r, vel, f, a, b, c, DT)
-    return r, vel
+    return (r, vel, f) if with_force else (r, vel)
 
 
 @pytest.mark.gpu
@@ -104,6 +105,41 @@ def test_batch_run_matches_oracle():
 
 
 @pytest.mark.gpu
+def test_batch_mixed_steps_match_oracle():
+    """The mixed-precision step kernels of every path (the staged replicas'
+    k_step_lds, the small replicas' k_rep_small; a step and a final step)
+    against the FP64 oracle: after 2 steps the force of each replica within
+    1e-5 relative RMS of the oracle's at the same step and the positions within
+    1e-9; after 30 steps the mixed trajectory within 1e-7 sigma of the FP64
+    batch's."""
+    ns, Ls, ps, vs = make_batch()
+    pos, vel = np.concatenate(ps), np.concatenate(vs)
+    with batch(ns, Ls, PREC_MIXED) as b:
+        b.set_state(pos, vel)
+        b.run(2)
+        st2 = b.get_state()
+        b.run(28)
+        mixed30 = b.get_state()
+    with batch(ns, Ls, PREC_FP64) as b:
+        b.set_state(pos, vel)
+        b.run(30)
+        fp64_30 = b.get_state()
+    off = 0
+    for n, L, p, v in zip(ns, Ls, ps, vs):
+        ro, vo, fo = oracle_run(p, v, L, 2, with_force=True)
+        sl = slice(off, off + n)
+        off += n
+        assert rel_rms(st2["force"][sl], fo) < 1e-5
+        d = st2["pos"][sl] - ro
+        d -= L * np.round(d / L)
+        assert np.abs(d).max() < 1e-9
+        d = mixed30["pos"][sl] - fp64_30["pos"][sl]
+        d -= L * np.round(d / L)
+        assert np.abs(d).max() < 1e-7
+        assert rel_rms(mixed30["vel"][sl], fp64_30["vel"][sl]) < 1e-6
+
+
+@pytest.mark.gpu
 def test_batch_graph_equals_eager_and_split_runs():
     ns, Ls, ps, vs = make_batch()
     pos, vel = np.concatenate(ps), np.concatenate(vs)
@@ -137,11 +173,13 @@ def test_batch_energy_conservation_and_rebuilds():
 
 
 @pytest.mark.gpu
-def test_batch_gather_path_matches_oracle():
+@pytest.mark.parametrize("precision", [PREC_FP64, PREC_MIXED])
+def test_batch_gather_path_matches_oracle(precision):
     """A long cutoff (rc 4.5, rs 5.0) puts ~150 atoms in a cell of the 3-cell box,
     more than the LDS staging takes, so that box runs on the gather kernels
-    (k_rep_list, k_rep_step); the 2-cell box takes the all-pairs kernel.  20 FP64
-    steps of each against the oracle's Gear run with the exact pair set."""
+    (k_rep_list, k_rep_step); the 2-cell box takes the all-pairs kernel.  20
+    steps of each against the oracle's Gear run with the exact pair set: FP64
+    within 1e-10, mixed within 1e-7 (positions) and 1e-6 (velocities)."""
     rc, rs, steps = 4.5, 5.0, 20
     rng = np.random.default_rng(21)
     ns, Ls, ps, vs = [], [], [], []
@@ -150,7 +188,7 @@ def test_batch_gather_path_matches_oracle():
         p = np.mod(p + rng.normal(0, 0.03, p.shape), L)
         ns.append(len(p)); Ls.append(L); ps.append(p)
         vs.append(initial.maxwell_boltzmann(len(p), T, int(rng.integers(1 << 30))))
-    with Batch(np.array(ns, np.int32), np.array(Ls), precision=PREC_FP64, dt=0.005, lj_rs=rs, lj_rcut=rc) as b:
+    with Batch(np.array(ns, np.int32), np.array(Ls), precision=precision, dt=0.005, lj_rs=rs, lj_rcut=rc) as b:
         assert b.paths().tolist() == [0, 2]
         b.set_state(np.concatenate(ps), np.concatenate(vs))
         b.run(steps)
@@ -168,6 +206,7 @@ def test_batch_gather_path_matches_oracle():
             r, vel, a, b3, c = O.gear_correct(r, vel, f, a, b3, c, DT)
         d = st["pos"][off: off + n] - r
         d -= L * np.round(d / L)
-        assert np.abs(d).max() < 1e-10
-        assert rel_rms(st["vel"][off: off + n], vel) < 1e-10
+        tol = (1e-10, 1e-10) if precision == PREC_FP64 else (1e-7, 1e-6)
+        assert np.abs(d).max() < tol[0]
+        assert rel_rms(st["vel"][off: off + n], vel) < tol[1]
         off += n

@@ -27,7 +27,7 @@ LJ, MB = (7.6, 6.8), (3.6, 3.0)
 ELOSS = 0.81
 
 
-def config_text(n_lj, n_mb, streams):
+def config_text(n_lj, n_mb, streams, **over):
     keys = {
         "Lennard Jones Particles": n_lj, "Many Body Particles": n_mb, "Timesteps": STEPS, "DT": 1,
         "Temperature": 300, "LennardJonesRS": LJ[0], "LennardJonesRCUT": LJ[1], "ManyBodyRS": MB[0],
@@ -37,6 +37,7 @@ def config_text(n_lj, n_mb, streams):
         "VXCMSTEP": 0.0015, "EnergyLoss": ELOSS, "AnimationTimesteps": ANIM, "AnimationData": "anim",
         "SweepStreams": streams,
     }
+    keys.update(over)
     return "".join(f"{k}={v}\n" for k, v in keys.items())
 
 
@@ -51,12 +52,12 @@ def read_dat(path):
     return nop, steps, vxcm, np.frombuffer(b[off + 4:], dtype="<f8").reshape(-1, 11)
 
 
-def run_sweep(tmp, pos4, vel, n_lj, n_mb, streams):
+def run_sweep(tmp, pos4, vel, n_lj, n_mb, streams, **over):
     tmp.mkdir()
     with open(tmp / "in.txt", "w") as f:
         for p, v in zip(pos4, vel):
             f.write(" ".join(f"{x:.17g}" for x in (*p, *v)) + "\n")
-    (tmp / "sweep.cfg").write_text(config_text(n_lj, n_mb, streams))
+    (tmp / "sweep.cfg").write_text(config_text(n_lj, n_mb, streams, **over))
     print(f"mdx_cpumd: {n_lj} LJ + {n_mb} bond atoms, SweepStreams={streams}", flush=True)
     r = subprocess.run([str(CPUMD), "sweep.cfg"], cwd=tmp, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
@@ -119,3 +120,44 @@ def test_missing_input_is_reported(tmp_path):
     assert r.returncode != 0
     assert "Error: Error opening input file." in r.stderr
     assert "Before Simulation" not in r.stdout
+
+
+def test_sweep_into_the_x_wall_matches_oracle(tmp_path):
+    """A Lennard-Jones sweep whose second member's centre-of-mass velocity
+    carries the gas into the reference's own +x wall (at nPos = N^(1/3) * 52.8 A,
+    cpu/cpumd/Host.cpp:31, 166) within the run: atoms with y > 0 are reflected
+    with sqrt(EnergyLoss), atoms with y < 0 teleported to -nPos, the forces and
+    a = F/m recomputed (Host.cpp:168-178), and lists rebuilt after the
+    teleports.  The oracle's run of each member counts the hits.  FP64
+    Lennard-Jones, Gear and walls perform the oracle's operations in its order,
+    so the trajectories are the same bits; the records (device reductions in
+    another order than the oracle's sums) must agree to 1e-12 of each column's
+    largest value, where a trajectory that parted from the oracle's (the wall
+    pile-up amplifies any difference) would be off by far more.  The run
+    stops before the gas piled against the wall blows up (the reference's walls
+    keep the Gear history of a reflected atom), which it does by 1,400 steps."""
+    if not CPUMD.exists():
+        pytest.fail("mdx_cpumd is not built")
+    pos4, vel = gas_mixture(10, 300, seed=5)
+    n = len(pos4)
+    steps = 1300
+    over = {"Lennard Jones Particles": n, "Many Body Particles": 0, "UseManyBody": 0, "Timesteps": steps,
+            "OutputTimesteps": 100, "AnimationTimesteps": 5000, "LennardJonesRS": 9.0,
+            "VXCMFROM": 0, "VXCMTO": 0.31, "VXCMSTEP": 0.3}
+    run_sweep(tmp_path / "wall", pos4, vel, n, 0, 2, **over)
+    dats = sorted((tmp_path / "wall").glob("res(vxcm-*).dat"))
+    assert len(dats) == 2
+    hit = 0
+    for d in dats:
+        nop, nsteps, vxcm, rec = read_dat(d)
+        assert (nop, nsteps) == (n, steps)
+        _, _, ro, _, st = O.ref_run(pos4, vel, n, 0, steps, 100, 1.0, 9.0, LJ[1], *MB, energy_loss=ELOSS,
+                                    vxcm=vxcm, use_lj=True, use_mb=False, stats=True)
+        print(f"{d.name}: oracle wall counts {st}", flush=True)
+        if st["x+ reflect"] > 0 and st["x+ teleport"] > 0 and st["builds after teleport"] > 0:
+            hit += 1
+        assert rec.shape == ro.shape == (steps // 100 + 1, 11)
+        np.testing.assert_array_equal(rec[:, 0], ro[:, 0])
+        scale = np.abs(ro).max(0) + 1e-30
+        assert (np.abs(rec - ro) / scale).max() < 1e-12, d.name
+    assert hit == 1

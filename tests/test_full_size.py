@@ -19,7 +19,7 @@ import pytest
 
 import _oracle as O
 from moleculardynamics_amd import BOND_SURROGATE, LIST_BOND, LIST_LJ, PREC_FP64, PREC_MIXED, Batch, initial
-from test_gpu_parity import periodic, rel_rms, tersoff_system
+from test_gpu_parity import LJ_LIST_SLACK2, TS_LIST_SLACK2, assert_pair_sets, periodic, rel_rms, tersoff_system
 
 pytestmark = pytest.mark.gpu
 
@@ -60,43 +60,65 @@ def test_lj_1m_fp64_forces_equal_oracle(lj1m):
     assert e == pytest.approx(eo, rel=1e-13)
 
 
+def test_lj_1m_pair_lists_are_the_exact_sets(lj1m):
+    """The step kernel's lists of all 1,048,576 atoms as sets (exact within
+    rc, every pair within rs), at the first build and after 60 steps."""
+    pos, vel, L = lj1m
+    n = len(pos)
+    with periodic(n, L, PREC_MIXED) as s:
+        s.set_state(np.c_[pos, np.zeros(n)], vel)
+        s.run(0)
+        st = assert_pair_sets(s, L, 2.8, 2.5, LJ_LIST_SLACK2, pos)
+        s.run(60)
+        assert s.build_count() >= 4
+        assert_pair_sets(s, L, 2.8, 2.5, LJ_LIST_SLACK2)
+    print(f"1M pair lists: {st}", flush=True)
+
+
 def test_lj_1m_fused_step_conserves_and_tracks_fp64(lj1m):
-    """200 fused steps (with list rebuilds) in mixed precision and in FP64
+    """1000 fused steps (with list rebuilds) in mixed precision and in FP64
     from the same state: zero net force and momentum kept to rounding in
     both; after 20 steps the mixed positions are within 1e-6 sigma of the
     FP64 ones (force differences of ~1e-7 relative over 0.1 tau); the total
-    (shifted-LJ) energy of the mixed run drifts like the FP64 run's (the
+    (shifted-LJ) energy of the mixed run drifts no faster than the FP64 run's,
+    to 2% of that rate (slopes of the energy sampled every 100 steps; the
     reference's force is truncated at rc without a shift, so neither is
     exactly conserved)."""
+    from test_gpu_parity import drift_rate
+
     pos, vel, L = lj1m
     n = len(pos)
     p_scale = n * np.sqrt(3 * 0.728)
-    drift, at20 = {}, {}
+    rate, at20 = {}, {}
+    ts = [0] + list(range(100, 1001, 100))
     for precision in (PREC_FP64, PREC_MIXED):
         with periodic(n, L, precision) as s:
             s.set_state(np.c_[pos, np.zeros(n)], vel)
+            e = [s.potential()[0] + s.kinetics(1.0)[0]]
             s.run(1)
             assert net_force_ok(s.get_state()["force"])
             k0 = s.kinetics(1.0)
-            e0 = s.potential()[0] + k0[0]
             s.run(19)
             at20[precision] = s.get_state()
-            s.run(181)
+            s.run(80)
+            for _ in range(10):
+                e.append(s.potential()[0] + s.kinetics(1.0)[0])
+                if len(e) < len(ts):
+                    s.run(100)
             k1 = s.kinetics(1.0)
-            e1 = s.potential()[0] + k1[0]
-            assert s.build_count() >= 10
+            assert s.build_count() >= 50
             assert net_force_ok(s.get_state()["force"])
         assert np.abs(k0[5:8]).max() < 1e-9 * p_scale
         assert np.abs(k1[5:8]).max() < 1e-9 * p_scale
-        drift[precision] = (e1 - e0) / n
-    print(f"energy drift per atom over 200 steps: FP64 {drift[PREC_FP64]:.3e}, mixed {drift[PREC_MIXED]:.3e}",
+        rate[precision] = drift_rate(ts, e, n)
+    print(f"energy drift per atom-step over 1000 steps: FP64 {rate[PREC_FP64]:.3e}, mixed {rate[PREC_MIXED]:.3e}",
           flush=True)
     d = at20[PREC_MIXED]["pos"][:, :3] - at20[PREC_FP64]["pos"][:, :3]
     d -= L * np.round(d / L)
     assert np.abs(d).max() < 1e-6
     assert rel_rms(at20[PREC_MIXED]["force"], at20[PREC_FP64]["force"]) < 1e-5
-    assert abs(drift[PREC_MIXED] - drift[PREC_FP64]) < 1e-5
-    assert abs(drift[PREC_FP64]) < 1e-3
+    assert abs(rate[PREC_MIXED]) <= 1.02 * abs(rate[PREC_FP64]) + 1e-10
+    assert abs(rate[PREC_FP64]) < 1e-5
 
 
 @pytest.mark.parametrize("precision", [PREC_FP64, PREC_MIXED])
@@ -118,6 +140,18 @@ def test_tersoff_262k_forces_match_oracle(si262k, precision):
         # FP32 terms; the float32-rounded parameters alone move the energy by ~1e-6
         assert rel_rms(f, fo) < 1e-5
         assert e == pytest.approx(eo, rel=2e-6)
+
+
+def test_tersoff_262k_pair_lists_are_the_exact_sets(si262k):
+    pos, vel, L = si262k
+    n = len(pos)
+    with tersoff_system(n, L, PREC_MIXED) as s:
+        s.set_state(np.c_[pos, np.zeros(n)], vel)
+        s.build_list(LIST_BOND)
+        assert_pair_sets(s, L, 3.3, 3.0, TS_LIST_SLACK2, pos)
+        s.run(150)
+        assert s.build_count() >= 2
+        assert_pair_sets(s, L, 3.3, 3.0, TS_LIST_SLACK2)
 
 
 def test_tersoff_262k_run_conserves(si262k):

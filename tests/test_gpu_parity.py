@@ -223,6 +223,45 @@ def test_walled_run_matches_reference_loop(mixture):
     assert rel_rms(st["vel"], vo) < 1e-9
 
 
+WALL_ELOSS = 0.81   # the EnergyLoss key; the engine takes its square root (cpu/cpumd/Host.cpp:73)
+
+
+@pytest.mark.parametrize("use_mb", [False, True], ids=["lj", "lj+bond"])
+def test_walled_run_wall_hits_match_reference_loop(use_mb):
+    """150 steps of a hot mixture in walls at +-15.5 A: atoms reach every wall
+    branch of cpu/common/Device.cpp:86-118 (x+ reflected with sqrt(EnergyLoss)
+    when y > 0, x+ teleported to -nPos when y < 0, x-, +-y, +-z) on many steps,
+    so the loop's recompute of the forces and a = F/m after a hit
+    (cpu/cpumd/Host.cpp:166-178) runs, and lists are rebuilt after teleports.
+    The oracle's run counts each branch; the engine must follow its trajectory
+    bit for bit with Lennard-Jones alone and within 1e-9 with the bond order
+    (device exp/tanh differ from glibc in the last bits), and must rebuild its
+    list as often."""
+    pos4, vel = gas_mixture(30, 500, seed=10)
+    vel = vel * 12
+    n, steps, wall = len(pos4), 150, 15.5
+    nb = 60 if use_mb else 0
+    ro, vo, _, _, st = O.ref_run(pos4, vel, n - nb, nb, steps, 1000, 1.0, 7.6, 6.8, 3.6, 3.0,
+                                 energy_loss=WALL_ELOSS, use_lj=True, use_mb=use_mb, wall=wall, stats=True)
+    branches = [st[k] for k in O.RUN_STATS[:7]]
+    assert min(branches) >= 3, st
+    assert st["hit steps"] >= 50 and st["builds after teleport"] >= 2, st
+    eloss = float(np.sqrt(np.float32(WALL_ELOSS)))
+    print(f"walled run with wall hits: {n} atoms, bond={use_mb}, {steps} steps, oracle counts {st}", flush=True)
+    with walled(pos4, nb, PREC_FP64, use_bond=use_mb, wall=wall, eloss=eloss) as s:
+        s.set_state(pos4, vel)
+        s.run(steps)
+        got = s.get_state()
+        builds = s.build_count()
+    assert builds == 1 + st["lj builds"]
+    if use_mb:
+        assert np.abs(got["pos"][:, :3] - ro[:, :3]).max() < 1e-9
+        assert rel_rms(got["vel"], vo) < 1e-9
+    else:
+        np.testing.assert_array_equal(got["pos"], ro)
+        np.testing.assert_array_equal(got["vel"], vo)
+
+
 def test_kinetics_and_allpairs(mixture):
     pos4, vel = mixture
     nb = 60
@@ -306,6 +345,126 @@ def test_periodic_run_matches_oracle(fluid):
     assert rel_rms(st["vel"], vo) < 1e-8
 
 
+# The band above rs^2 an exported list may reach: the Lennard-Jones list build
+# tests FP32 positions relative to the segment corner against rs^2 + 1e-4
+# (sigma^2); the Tersoff build tests the FP64 minimum image rounded to FP32
+# against rs^2 (1 + 1e-6).
+LJ_LIST_SLACK2 = 2e-4
+TS_LIST_SLACK2 = 1e-5 * 3.3 ** 2
+
+
+def assert_pair_sets(s, L, rs, rc, slack2, pos=None):
+    """The list the periodic force kernels read (caller indices, at its build
+    positions) against the exact FP64 sets of the oracle's checker: it holds
+    every pair within rs, nothing at rs^2 + slack2 or beyond, no repeats; so
+    its pairs within rc are exactly the oracle's (common/NeighborList.cpp:88-145
+    with the distance filter of cpu/common/LennardJones.h:51-56)."""
+    rows, counts, bpos = s.pair_list()
+    if pos is not None:   # built from these positions (wrapped into the box)
+        d = bpos - pos
+        d -= L * np.round(d / L)
+        assert np.abs(d).max() < 1e-12
+    st = O.pair_list_check(bpos, L, rs, rc, rows, counts, slack2)
+    assert st["missing within rs"] == 0 and st["beyond rs + slack"] == 0 and st["bad entries"] == 0, st
+    assert st["rc pairs listed"] == st["pairs within rc"] > 0, st
+    assert st["entries"] == st["pairs within rs"] + st["in slack band"], st
+    assert st["in slack band"] <= 1e-4 * st["entries"], st
+    return st
+
+
+@pytest.mark.parametrize("precision", [PREC_FP64, PREC_MIXED])
+def test_periodic_pair_lists_are_the_exact_sets(fluid, precision):
+    """k_nlist_lds's lists (16-bit LDS indices, decoded through each segment's
+    staging) as sets: at the first build and at the last one of a 200-step
+    run with rebuilds."""
+    pos, vel, L = fluid
+    n = len(pos)
+    with periodic(n, L, precision) as s:
+        s.set_state(np.c_[pos, np.zeros(n)], vel)
+        s.run(0)
+        st = assert_pair_sets(s, L, 2.8, 2.5, LJ_LIST_SLACK2, pos)
+        s.run(200)
+        assert s.build_count() >= 3
+        assert_pair_sets(s, L, 2.8, 2.5, LJ_LIST_SLACK2)
+    print(f"pair list of {n} atoms: {st}", flush=True)
+
+
+def fast_atom_lattice(v0=40.0):
+    """A quiet FCC lattice (864 atoms) with one atom moving at v0 along x
+    (0.2 sigma per step at dt = 0.005, past the list margin (rs - rc)/2 = 0.15
+    sigma).  A steadily fast atom is rebuilt for every step; the stale list
+    comes when it knocks a lattice atom out of place within one step, which
+    no decision from earlier positions can foresee."""
+    pos, _, L = initial.lj_fluid(6, 0.8442, 0.728, seed=1)
+    vel = np.zeros_like(pos)
+    vel[0, 0] = v0
+    return pos, vel, L
+
+
+def stale_step(err):
+    import re
+    m = re.search(r"step (\d+) used a stale list", str(err))
+    assert m, str(err)
+    return int(m.group(1))
+
+
+def test_periodic_stale_list_is_reported():
+    """Each periodic step kernel checks its own positions against the full list
+    margin and counts the atoms beyond it; the count reaches the host through
+    the step's flag hand-over.  A step that used a stale list must fail the
+    run with MDX_ERR_LIST_STALE naming that step, whether the host finds it
+    while later steps run, in the check at the end of the call (a call that
+    ends with the stale step), or in a second call.  (A call boundary can move
+    a rebuild: each call's first decision is exact, so the second call may
+    rebuild before the first call's stale step and go stale at a later one.)"""
+    from moleculardynamics_amd._lib import MdxError
+    pos, vel, L = fast_atom_lattice()
+    n = len(pos)
+
+    def system():
+        s = periodic(n, L, PREC_MIXED)
+        s.set_state(np.c_[pos, np.zeros(n)], vel)
+        return s
+
+    with system() as s, pytest.raises(MdxError) as e:
+        s.run(30)
+    assert e.value.code == -7, str(e.value)
+    k = stale_step(e.value)
+    print(f"stale list at step {k}", flush=True)
+    assert k >= 2
+    with system() as s, pytest.raises(MdxError) as e:
+        s.run(k + 1)
+    assert e.value.code == -7 and stale_step(e.value) == k, str(e.value)
+    with system() as s:
+        s.run(k - 1)
+        with pytest.raises(MdxError) as e:
+            s.run(30)
+    assert e.value.code == -7 and stale_step(e.value) >= k - 1, str(e.value)
+    # a single step per call is always exact: each call's first decision reads
+    # the flags of the positions it steps from, so no step goes stale
+    with system() as s:
+        for _ in range(k + 3):
+            s.run(1)
+
+
+def test_periodic_rs_equal_rc_rebuilds_every_step(fluid):
+    """rs = rc leaves no margin: the reference's schedule then rebuilds before
+    every step (cpu/cpumd/Host.cpp:288-290 with maxnlmove = 0), and so must the
+    periodic loop, matching the oracle that rebuilds every step."""
+    pos, vel, L = fluid
+    n, steps = len(pos), 20
+    ro, vo, _, builds, _ = O.pbc_lj_run(pos, vel, L, 2.5, 2.5, DT, steps, rebuild_every_step=True)
+    with periodic(n, L, PREC_FP64, rc=2.5, rs=2.5) as s:
+        s.set_state(np.c_[pos, np.zeros(n)], vel)
+        s.run(steps)
+        st = s.get_state()
+        assert s.build_count() == steps + 1
+    d = st["pos"][:, :3] - ro
+    d -= L * np.round(d / L)
+    assert np.abs(d).max() < 1e-10
+    assert rel_rms(st["vel"], vo) < 1e-10
+
+
 def test_periodic_split_runs_match_oracle(fluid):
     """The same 200 steps as above in calls of 1, 2, 3, 47 and 147 steps: each
     call starts with the predictor's flags (copied to the host) and hands the
@@ -326,21 +485,32 @@ def test_periodic_split_runs_match_oracle(fluid):
     assert rel_rms(st["vel"], vo) < 1e-8
 
 
+def drift_rate(steps, energies, n):
+    """Least-squares slope of the total energy per atom per step."""
+    return float(np.polyfit(np.asarray(steps, float), np.asarray(energies) / n, 1)[0])
+
+
 def test_periodic_mixed_energy_drift(fluid):
-    """1000 steps of the 4000-atom fluid: total energy (shifted LJ) drift of the
-    mixed-precision run stays within that of the FP64 oracle run plus 1e-4."""
+    """1000 steps of the 4000-atom fluid: the total energy (shifted LJ; the
+    force is cut at rc without a shift, so it is not exactly conserved) drifts
+    in the mixed-precision run no faster than in the FP64 oracle's run, to 5%
+    of that rate (slopes of the energy sampled every 50 steps)."""
     pos, vel, L = fluid
-    n, steps = len(pos), 1000
+    n, steps, every = len(pos), 1000, 50
     _, _, en, _, _ = O.pbc_lj_run(pos, vel, L, 2.5, 2.8, DT, steps)
-    drift_o = abs(en[-1, 2] - en[0, 2]) / n
+    ts = np.arange(0, steps + 1, every)
+    rate_o = drift_rate(ts, en[ts, 2], n)
+    e = []
     with periodic(n, L, PREC_MIXED) as s:
         s.set_state(np.c_[pos, np.zeros(n)], vel)
-        e0 = s.potential()[0] + s.kinetics(1.0)[0]
-        s.run(steps)
-        e1 = s.potential()[0] + s.kinetics(1.0)[0]
-    drift = abs(e1 - e0) / n
-    assert e0 == pytest.approx(en[0, 2], rel=1e-6)
-    assert drift <= drift_o + 1e-4
+        for k in range(len(ts)):
+            if k:
+                s.run(every)
+            e.append(s.potential()[0] + s.kinetics(1.0)[0])
+    rate = drift_rate(ts, e, n)
+    print(f"energy drift per atom-step over {steps} steps: oracle FP64 {rate_o:.3e}, mixed {rate:.3e}", flush=True)
+    assert e[0] == pytest.approx(en[0, 2], rel=1e-6)
+    assert abs(rate) <= 1.05 * abs(rate_o) + 1e-9
 
 
 # ---------------- periodic Tersoff silicon ----------------
@@ -388,6 +558,20 @@ def test_tersoff_run_matches_oracle(silicon):
     d -= L * np.round(d / L)
     assert np.abs(d).max() < 1e-9
     assert rel_rms(st["vel"], vo) < 1e-9
+
+
+def test_tersoff_pair_lists_are_the_exact_sets(silicon):
+    """k_ts_list's compact lists as sets within R + D = 3.0 A and the list
+    radius 3.3 A, at a build and after 200 steps with rebuilds."""
+    pos, vel, L = silicon
+    n = len(pos)
+    with tersoff_system(n, L, PREC_MIXED) as s:
+        s.set_state(np.c_[pos, np.zeros(n)], vel)
+        s.build_list(LIST_BOND)
+        assert_pair_sets(s, L, 3.3, 3.0, TS_LIST_SLACK2, pos)
+        s.run(200)
+        assert s.build_count() >= 2
+        assert_pair_sets(s, L, 3.3, 3.0, TS_LIST_SLACK2)
 
 
 def test_tersoff_mixed_energy_drift(silicon):

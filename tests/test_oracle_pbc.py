@@ -189,3 +189,78 @@ def test_tersoff_cell_search_matches_brute_force():
         mm = x.copy(); mm[i, d] -= h
         g = (O.tersoff(pp, L, p, False)[0] - O.tersoff(mm, L, p, False)[0]) / (2 * h)
         assert f1[i, d] == pytest.approx(-g, rel=1e-5, abs=1e-6)
+
+
+def brute_pairs(pos, L, rs):
+    d = pos[:, None, :] - pos[None, :, :]
+    d -= L * np.round(d / L)
+    r2 = (d ** 2).sum(-1)
+    np.fill_diagonal(r2, np.inf)
+    return r2
+
+
+def test_pair_list_check_pins():
+    """The checker of exported pair lists against brute-force all-pairs sets:
+    the exact list passes with every count where it belongs, and each planted
+    mistake (a pair dropped, a far pair added, a repeat, the atom itself, a
+    pair just past rs) moves exactly its own count."""
+    rng = np.random.default_rng(5)
+    L, rs, rc = 12.0, 2.8, 2.5
+    pos = rng.uniform(0, L, (500, 3))
+    r2 = brute_pairs(pos, L, rs)
+    width = int((r2 < rs * rs).sum(1).max()) + 3
+    rows = -np.ones((500, width), np.int32)
+    counts = np.zeros(500, np.int32)
+    for i in range(500):
+        js = rng.permutation(np.nonzero(r2[i] < rs * rs)[0])   # any order
+        rows[i, :len(js)] = js
+        counts[i] = len(js)
+    want = {"entries": int(counts.sum()), "missing within rs": 0, "beyond rs + slack": 0, "in slack band": 0,
+            "bad entries": 0, "pairs within rc": int((r2 < rc * rc).sum()),
+            "rc pairs listed": int((r2 < rc * rc).sum()), "pairs within rs": int(counts.sum())}
+    assert O.pair_list_check(pos, L, rs, rc, rows, counts, 1e-4) == want
+
+    def planted(edit):
+        r, c = rows.copy(), counts.copy()
+        edit(r, c)
+        got = O.pair_list_check(pos, L, rs, rc, r, c, 1e-4)
+        return {k: got[k] - want[k] for k in want if got[k] != want[k]}
+
+    i = int(np.argmax(counts))
+    near = int(np.nonzero(r2[i] < rc * rc)[0][0])
+    far = int(np.argmax(np.where(np.isfinite(r2[i]), r2[i], -1.0)))
+    k_near = int(np.nonzero(rows[i] == near)[0][0])
+
+    def drop(r, c):                  # a pair within rc dropped from row i
+        r[i, k_near] = r[i, c[i] - 1]
+        r[i, c[i] - 1] = -1
+        c[i] -= 1
+    assert planted(drop) == {"entries": -1, "missing within rs": 1, "rc pairs listed": -1}
+
+    def add_far(r, c):
+        r[i, c[i]] = far
+        c[i] += 1
+    assert planted(add_far) == {"entries": 1, "beyond rs + slack": 1}
+
+    def repeat(r, c):
+        r[i, c[i]] = r[i, 0]
+        c[i] += 1
+    assert planted(repeat) == {"entries": 1, "bad entries": 1}
+
+    def self_entry(r, c):
+        r[i, c[i]] = i
+        c[i] += 1
+    assert planted(self_entry) == {"entries": 1, "bad entries": 1}
+
+    # a pair just past rs is in the slack band, not beyond it
+    d2 = r2[i].copy()
+    d2[d2 < rs * rs] = np.inf
+    j = int(np.argmin(d2))
+    assert rs * rs <= r2[i, j] < rs * rs + 1.0
+
+    def band(r, c):
+        r[i, c[i]] = j
+        c[i] += 1
+    got = O.pair_list_check(pos, L, rs, rc, *((lambda r, c: (band(r, c), r, c)[1:])(rows.copy(), counts.copy())),
+                            float(r2[i, j] - rs * rs) + 1e-9)
+    assert got["in slack band"] == 1 and got["beyond rs + slack"] == 0
