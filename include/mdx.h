@@ -197,6 +197,22 @@ int mdx_build_count(mdx_system* sys, int64_t* builds);
 /* common/DebugCalculations.cpp:46-124 as FP64 device reductions.
  * out8 = {EK, T, COMx, COMy, COMz, Px, Py, Pz}; T uses kB = kboltz. */
 int mdx_kinetics(mdx_system* sys, double kboltz, double* out8);
+/* mdx_kinetics over caller-owned device arrays, allocating nothing.  No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).
+ * pos3 and vel3 hold component d of atom i at [d * stride + i] (stride >= n),
+ * type[i] is the species (0..3), mass4 (host) the mass of each species; out8
+ * (device) receives {EK, T, COM, P} as in mdx_kinetics.  stream is a
+ * hipStream_t (NULL: the null stream); the call only enqueues work on it.
+ * The partial sums live in a static device array of 8 * 1024 + 8 doubles
+ * (66 KB on each GPU the library runs on, held until the process ends);
+ * n is limited to 2^28 atoms (MDX_ERR_CAPACITY above).  Every call on a GPU
+ * shares that GPU's array, so calls that overlap on one GPU race.  The
+ * message of a failed call is in mdx_kinetics_noalloc_last_error(). */
+int mdx_kinetics_noalloc(const double* pos3, const double* vel3, const int32_t* type, int32_t n, int64_t stride,
+                         const double* mass4, double kboltz, double* out8, void* stream);
+const char* mdx_kinetics_noalloc_last_error(void);   /* the calling thread's */
+
 /* Potential energy over the current lists with the force cutoffs (LJ shifted
  * to zero at rc in the periodic box; unshifted in the walled box).
  * out2 = {LJ part, bond-order part}. */
