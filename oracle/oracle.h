@@ -114,6 +114,9 @@ typedef struct {
 int32_t orc_ref_run(const orc_ref_run_cfg* cfg, orc_real4* r, orc_real3* v,
                     double* records, int32_t max_records, double* elapsed_s);
 
+/* the compiler and flags the oracle was built with */
+const char* orc_build_info(void);
+
 /* ---- periodic reduced-unit Lennard-Jones (benchmark configurations) ---- */
 
 /* Candidate list in the reference's padded format built on a periodic cell
@@ -152,6 +155,11 @@ void orc_gear_correct(double* r3, double* v3, const double* f3, double* a3, doub
 int32_t orc_pbc_lj_run(double* r3, double* v3, int32_t n, double L, double rc, double rs,
                        double dt, int32_t nsteps, int32_t rebuild_every_step,
                        double* energies, double* elapsed_s);
+/* the same loop with its force, Gear and displacement loops split over
+ * `threads` OpenMP threads (1: the serial loop above, bit for bit) */
+int32_t orc_pbc_lj_run_threads(double* r, double* v, int32_t n, double L, double rc, double rs,
+                               double dt, int32_t nsteps, int32_t rebuild_every_step,
+                               double* energies, double* elapsed_s, int32_t threads);
 
 /* ---- Tersoff silicon (Tersoff 1988) ---- */
 typedef struct {

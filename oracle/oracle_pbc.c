@@ -23,6 +23,11 @@ static double wrap01(double x, double L) {
     return y >= L ? y - L : y;
 }
 
+#ifndef ORC_CFLAGS
+#define ORC_CFLAGS "?"
+#endif
+const char* orc_build_info(void) { return "gcc " __VERSION__ " " ORC_CFLAGS; }
+
 static double min_image(double d, double L) { return d - L * nearbyint(d / L); }
 static double sq3(double x, double y, double z) { return x * x + y * y + z * z; }
 
@@ -33,8 +38,8 @@ static double sq3(double x, double y, double z) { return x * x + y * y + z * z; 
  * nlistsize = min(27 * max occupancy, n - 1)) on a periodic grid of
  * floor(L/rs) cells per axis; neighbour cells wrap around.
  */
-int32_t* orc_pbc_build_list(const double* pos3, int32_t n, double L, double rs,
-                            int32_t* nlistsize, int32_t* nlargestsize) {
+static int32_t* build_list_threads(const double* pos3, int32_t n, double L, double rs,
+                                   int32_t* nlistsize, int32_t* nlargestsize, int32_t threads) {
     int32_t nc = (int32_t)floor(L / rs);
     if (nc < 3) return NULL;
     double h = L / nc;
@@ -66,6 +71,7 @@ int32_t* orc_pbc_build_list(const double* pos3, int32_t n, double L, double rs,
     int32_t* out = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nls * (size_t)n + 1));
     if (!out) return NULL;
     int32_t longest = 0;
+#pragma omp parallel for num_threads(threads) if (threads > 1) reduction(max : longest) schedule(static)
     for (int32_t i = 0; i < n; i++) {
         int32_t cz = cell[i] % nc, cy = (cell[i] / nc) % nc, cx = cell[i] / (nc * nc);
         int32_t* row = out + (size_t)nls * (size_t)i;
@@ -85,6 +91,11 @@ int32_t* orc_pbc_build_list(const double* pos3, int32_t n, double L, double rs,
     *nlargestsize = longest;
     free(cell); free(start); free(members); free(fill);
     return out;
+}
+
+int32_t* orc_pbc_build_list(const double* pos3, int32_t n, double L, double rs,
+                            int32_t* nlistsize, int32_t* nlargestsize) {
+    return build_list_threads(pos3, n, L, rs, nlistsize, nlargestsize, 1);
 }
 
 /* Check of an exported pair list (rows of width ints, counts[i] partners of
@@ -171,14 +182,17 @@ int32_t orc_pbc_pair_list_check(const double* pos3, int32_t n, double L, double 
 /* 12-6 LJ over candidate rows, minimum image, r < rc; energy shifted so the
  * pair energy is zero at rc.  Each unordered pair is visited from both ends,
  * so energy and virial take half of each visit. */
-double orc_pbc_lj_forces(const double* pos3, double* f3, int32_t n, double L,
-                         double sigma, double eps, double rc, int32_t nlistsize,
-                         const int32_t* list, double* virial) {
+/* threads > 1 splits the atoms over OpenMP threads (each atom's force is
+ * still summed in row order; only the energy and virial sums change order) */
+static double lj_forces_threads(const double* pos3, double* f3, int32_t n, double L,
+                                double sigma, double eps, double rc, int32_t nlistsize,
+                                const int32_t* list, double* virial, int32_t threads) {
     double rc2 = rc * rc, s2 = sigma * sigma;
     double coeff = 24.0 * eps / s2, e4 = 4 * eps;
     double qc = rc2 / s2, iqc6 = 1.0 / (qc * qc * qc);
     double shift = e4 * iqc6 * (iqc6 - 1);
     double e = 0, w = 0;
+#pragma omp parallel for num_threads(threads) if (threads > 1) reduction(+ : e, w) schedule(static)
     for (int32_t i = 0; i < n; i++) {
         double fx = 0, fy = 0, fz = 0;
         const int32_t* row = list + (size_t)nlistsize * (size_t)i;
@@ -205,6 +219,12 @@ double orc_pbc_lj_forces(const double* pos3, double* f3, int32_t n, double L,
     return e;
 }
 
+double orc_pbc_lj_forces(const double* pos3, double* f3, int32_t n, double L,
+                         double sigma, double eps, double rc, int32_t nlistsize,
+                         const int32_t* list, double* virial) {
+    return lj_forces_threads(pos3, f3, n, L, sigma, eps, rc, nlistsize, list, virial, 1);
+}
+
 /* ---------------- Gear 5th-order PC without walls ---------------- */
 /* coefficients as cpu/common/Device.cpp:19-22 and :48-56 */
 static void coeffs(double dt, double k[4], double kc[4]) {
@@ -218,10 +238,11 @@ static void coeffs(double dt, double k[4], double kc[4]) {
     kc[3] = 0.08333333333333333333333333333333f * k[1] / k[3];
 }
 
-void orc_gear_predict(double* r, double* v, double* a, double* b, const double* c,
-                      double dt, int32_t n) {
+static void gear_predict_threads(double* r, double* v, double* a, double* b, const double* c,
+                                 double dt, int32_t n, int32_t threads) {
     double k[4], kc[4];
     coeffs(dt, k, kc);
+#pragma omp parallel for num_threads(threads) if (threads > 1) schedule(static)
     for (size_t q = 0; q < 3 * (size_t)n; q++) {
         r[q] = r[q] + k[0] * v[q] + k[1] * a[q] + k[2] * b[q] + k[3] * c[q];
         v[q] = v[q] + k[0] * a[q] + k[1] * b[q] + k[2] * c[q];
@@ -230,10 +251,16 @@ void orc_gear_predict(double* r, double* v, double* a, double* b, const double* 
     }
 }
 
-void orc_gear_correct(double* r, double* v, const double* f, double* a, double* b,
-                      double* c, double dt, int32_t n, double mass, double scale) {
+void orc_gear_predict(double* r, double* v, double* a, double* b, const double* c,
+                      double dt, int32_t n) {
+    gear_predict_threads(r, v, a, b, c, dt, n, 1);
+}
+
+static void gear_correct_threads(double* r, double* v, const double* f, double* a, double* b,
+                                 double* c, double dt, int32_t n, double mass, double scale, int32_t threads) {
     double k[4], kc[4];
     coeffs(dt, k, kc);
+#pragma omp parallel for num_threads(threads) if (threads > 1) schedule(static)
     for (size_t q = 0; q < 3 * (size_t)n; q++) {
         double an = f[q] / mass * scale;
         double corr = an - a[q];
@@ -245,6 +272,11 @@ void orc_gear_correct(double* r, double* v, const double* f, double* a, double* 
     }
 }
 
+void orc_gear_correct(double* r, double* v, const double* f, double* a, double* b,
+                      double* c, double dt, int32_t n, double mass, double scale) {
+    gear_correct_threads(r, v, f, a, b, c, dt, n, mass, scale, 1);
+}
+
 static double kinetic(const double* v, int32_t n) {
     double s = 0;
     for (size_t q = 0; q < 3 * (size_t)n; q++) s += v[q] * v[q];
@@ -254,6 +286,12 @@ static double kinetic(const double* v, int32_t n) {
 int32_t orc_pbc_lj_run(double* r, double* v, int32_t n, double L, double rc, double rs,
                        double dt, int32_t nsteps, int32_t rebuild_every_step,
                        double* energies, double* elapsed_s) {
+    return orc_pbc_lj_run_threads(r, v, n, L, rc, rs, dt, nsteps, rebuild_every_step, energies, elapsed_s, 1);
+}
+
+int32_t orc_pbc_lj_run_threads(double* r, double* v, int32_t n, double L, double rc, double rs,
+                               double dt, int32_t nsteps, int32_t rebuild_every_step,
+                               double* energies, double* elapsed_s, int32_t threads) {
     size_t m = 3 * (size_t)n;
     double* f = (double*)calloc(m, sizeof(double));
     double* a = (double*)calloc(m, sizeof(double));
@@ -263,31 +301,34 @@ int32_t orc_pbc_lj_run(double* r, double* v, int32_t n, double L, double rc, dou
     if (!f || !a || !b || !c || !r0) return -1;
     double half_skin2 = 0.25 * (rs - rc) * (rs - rc);
     int32_t nls, nlg, builds = 1;
-    int32_t* list = orc_pbc_build_list(r, n, L, rs, &nls, &nlg);
+    int32_t* list = build_list_threads(r, n, L, rs, &nls, &nlg, threads);
     if (!list) return -1;
     memcpy(r0, r, m * sizeof(double));
     double t0 = now_s();
-    double u = orc_pbc_lj_forces(r, f, n, L, 1.0, 1.0, rc, nls, list, NULL);
+    double u = lj_forces_threads(r, f, n, L, 1.0, 1.0, rc, nls, list, NULL, threads);
     for (size_t q = 0; q < m; q++) a[q] = f[q];
     if (energies) { energies[0] = u; energies[1] = kinetic(v, n); energies[2] = u + energies[1]; }
     for (int32_t t = 1; t <= nsteps; t++) {
-        orc_gear_predict(r, v, a, b, c, dt, n);
+        gear_predict_threads(r, v, a, b, c, dt, n, threads);
         /* the list must hold every pair within rc at the positions the force
          * uses: rebuild when any atom moved skin/2 or more since the build */
         int need = rebuild_every_step;
-        for (int32_t i = 0; i < n && !need; i++) {
-            double dx = r[3 * i] - r0[3 * i], dy = r[3 * i + 1] - r0[3 * i + 1], dz = r[3 * i + 2] - r0[3 * i + 2];
-            if (dx * dx + dy * dy + dz * dz >= half_skin2) need = 1;
+        if (!need) {
+#pragma omp parallel for num_threads(threads) if (threads > 1) reduction(| : need) schedule(static)
+            for (int32_t i = 0; i < n; i++) {
+                double dx = r[3 * i] - r0[3 * i], dy = r[3 * i + 1] - r0[3 * i + 1], dz = r[3 * i + 2] - r0[3 * i + 2];
+                if (dx * dx + dy * dy + dz * dz >= half_skin2) need |= 1;
+            }
         }
         if (need) {
             free(list);
-            list = orc_pbc_build_list(r, n, L, rs, &nls, &nlg);
+            list = build_list_threads(r, n, L, rs, &nls, &nlg, threads);
             if (!list) return -1;
             memcpy(r0, r, m * sizeof(double));
             builds++;
         }
-        u = orc_pbc_lj_forces(r, f, n, L, 1.0, 1.0, rc, nls, list, NULL);
-        orc_gear_correct(r, v, f, a, b, c, dt, n, 1.0, 1.0);
+        u = lj_forces_threads(r, f, n, L, 1.0, 1.0, rc, nls, list, NULL, threads);
+        gear_correct_threads(r, v, f, a, b, c, dt, n, 1.0, 1.0, threads);
         if (energies) {
             double k = kinetic(v, n);
             energies[3 * t] = u; energies[3 * t + 1] = k; energies[3 * t + 2] = u + k;

@@ -73,6 +73,10 @@ def lib() -> C.CDLL:
         L.orc_gear_correct.argtypes = [P, P, P, P, P, P, D, I, D, D]
         L.orc_pbc_lj_run.argtypes = [P, P, I, D, D, D, D, I, I, P, C.POINTER(D)]
         L.orc_pbc_lj_run.restype = I
+        L.orc_pbc_lj_run_threads.argtypes = [P, P, I, D, D, D, D, I, I, P, C.POINTER(D), I]
+        L.orc_pbc_lj_run_threads.restype = I
+        L.orc_build_info.argtypes = []
+        L.orc_build_info.restype = C.c_char_p
         L.orc_tersoff_si_params.argtypes = [C.POINTER(TersoffParams)]
         L.orc_tersoff_energy_forces.restype = D
         L.orc_tersoff_energy_forces.argtypes = [C.POINTER(TersoffParams), P, I, D, P]
@@ -182,6 +186,10 @@ def scalar2(fn, *args):
     return a.value, b.value
 
 
+def build_info() -> str:
+    return lib().orc_build_info().decode()
+
+
 def param(which, ti, tj):
     return lib().orc_ref_param(which, ti, tj)
 
@@ -244,12 +252,13 @@ def pbc_lj_forces(pos3, L, rc, nlist, sigma=1.0, eps=1.0):
     return f, e, w.value
 
 
-def pbc_lj_run(pos3, vel3, L, rc, rs, dt, nsteps, rebuild_every_step=False):
+def pbc_lj_run(pos3, vel3, L, rc, rs, dt, nsteps, rebuild_every_step=False, threads=1):
+    """The oracle's periodic LJ loop; threads > 1 runs its OpenMP variant."""
     r, v = f64(pos3).copy(), f64(vel3).copy()
     en = np.zeros((nsteps + 1, 3))
     el = C.c_double()
-    builds = lib().orc_pbc_lj_run(ptr(r), ptr(v), len(r), L, rc, rs, dt, nsteps,
-                                  1 if rebuild_every_step else 0, ptr(en), C.byref(el))
+    builds = lib().orc_pbc_lj_run_threads(ptr(r), ptr(v), len(r), L, rc, rs, dt, nsteps,
+                                          1 if rebuild_every_step else 0, ptr(en), C.byref(el), threads)
     assert builds >= 0
     return r, v, en, builds, el.value
 
