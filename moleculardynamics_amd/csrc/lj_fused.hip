@@ -42,7 +42,7 @@ namespace {
 
 constexpr int kBlock = 512;
 constexpr int kLanes = 4;                  // lanes per atom
-constexpr int kApp = kBlock / kLanes;      // atoms per pass
+constexpr int kApp = kBlock / kLanes;      // atoms per pass (the replicas' workgroups)
 constexpr int kChunk = 8;                  // 16-bit entries per 16-byte chunk
 constexpr int kScan = 8;                   // candidates per lane per scan step
 constexpr int kScanSlack = kLanes * kScan; // staged slots past the last atom the scan may read
@@ -52,7 +52,6 @@ constexpr int kXcd = 8;
 constexpr int kLdsAtoms = 3072;            // staged atoms per segment at most (3 x 24 KiB of FP64)
 constexpr int kMaxK = 14;                  // cells per segment
 constexpr int kSpread = 64;                // flag atomics are spread over this many addresses
-constexpr int kColThreads = kBlock / 9;    // staging: threads per column
 constexpr int kCopyBatch = 4;              // staging: independent loads in flight per thread
 
 struct SegGrid {
@@ -247,7 +246,7 @@ __device__ inline int32_t staged_slot(const Stage& S, int32_t w) {
 // Stage the neighbourhood of segment (cx, cy, z0..z1), positions minus the
 // origin o (zero for the FP64 staging: the values are then exactly the
 // images).  Returns false (for the whole block) when it does not fit in LDS.
-template <typename PosT>
+template <int kB, typename PosT>
 __device__ inline bool stage(Stage& S, const PosT& P, const SegGrid& g, int32_t cx, int32_t cy, int32_t z0,
                              int32_t z1, const double* __restrict__ rx, const double* __restrict__ ry,
                              const double* __restrict__ rz, double ox = 0.0, double oy = 0.0, double oz = 0.0) {
@@ -255,6 +254,7 @@ __device__ inline bool stage(Stage& S, const PosT& P, const SegGrid& g, int32_t 
     stage_header(S, g, cx, cy, z0, z1);
     if (S.total + 1 > P.cap) return false;   // + the dummy atom
     // copy: kColThreads threads per column, kCopyBatch independent loads each
+    constexpr int32_t kColThreads = kB / 9;
     if (tid < 9 * kColThreads) {
         const int32_t col = tid / kColThreads, l = tid % kColThreads;
         int32_t x = cx + col / 3 - 1, y = cy + col % 3 - 1;
@@ -286,7 +286,7 @@ __device__ inline bool stage(Stage& S, const PosT& P, const SegGrid& g, int32_t 
             }
         }
     }
-    if (tid == kBlock - 1) P.set(S.total, kFar, kFar, kFar);
+    if (tid == kB - 1) P.set(S.total, kFar, kFar, kFar);
     __syncthreads();
     return true;
 }
@@ -298,7 +298,8 @@ __device__ inline int32_t home_cell(const Stage& S, int32_t qc) {
     return o;
 }
 
-// the lists of one segment's home atoms (the whole block)
+// the lists of one segment's home atoms (the whole block of kB threads)
+template <int kB>
 __device__ __forceinline__ void nlist_segment(Stage& S, double* dyn, const SegGrid& g, int32_t cx, int32_t cy,
                                               int32_t z0, int32_t z1, const double* __restrict__ rx,
                                               const double* __restrict__ ry, const double* __restrict__ rz,
@@ -307,7 +308,7 @@ __device__ __forceinline__ void nlist_segment(Stage& S, double* dyn, const SegGr
                                               int32_t lds_cap) {
     const int32_t nchunk = cap / (kLanes * kChunk) + 1;   // chunks per lane with the spare
     const PosF P{reinterpret_cast<float*>(dyn), lds_cap};
-    if (!stage(S, P, g, cx, cy, z0, z1, rx, ry, rz, cx * (g.L / g.n[0]), cy * (g.L / g.n[1]), z0 * (g.L / g.n[2]))) {
+    if (!stage<kB>(S, P, g, cx, cy, z0, z1, rx, ry, rz, cx * (g.L / g.n[0]), cy * (g.L / g.n[1]), z0 * (g.L / g.n[2]))) {
         if (threadIdx.x == 0) atomicMax(&err[0], S.total + 1);
         return;
     }
@@ -317,7 +318,7 @@ __device__ __forceinline__ void nlist_segment(Stage& S, double* dyn, const SegGr
     const int32_t cbase = (cx * g.n[1] + cy) * g.n[2];
     const int32_t hbeg = g.start[cbase + z0], hend = g.start[cbase + z1];
     const int32_t q = threadIdx.x & (kLanes - 1);
-    for (int32_t base = hbeg; base < hend; base += kApp) {
+    for (int32_t base = hbeg; base < hend; base += kB / kLanes) {
         const int32_t s = base + threadIdx.x / kLanes;
         if (s >= hend) continue;             // whole quads drop out together
         const int32_t qc = s - hbeg;
@@ -385,8 +386,8 @@ __device__ __forceinline__ void nlist_segment(Stage& S, double* dyn, const SegGr
 
 // The single box: one segment per block.  Replicas: a fixed grid over the work
 // list, each XCD taking a contiguous eighth of it.
-template <bool kRep>
-__global__ void __launch_bounds__(kBlock, 8)
+template <bool kRep, int kB>
+__global__ void __launch_bounds__(kB, 8)
 k_nlist_lds(SegGrid g, RepGeo rg, const double* __restrict__ rx, const double* __restrict__ ry,
             const double* __restrict__ rz, double rl2, int32_t cap, uint16_t* __restrict__ list,
             int32_t* __restrict__ count, int32_t* __restrict__ err, int32_t lds_cap) {
@@ -399,12 +400,12 @@ k_nlist_lds(SegGrid g, RepGeo rg, const double* __restrict__ rx, const double* _
         const int32_t w1 = (int32_t)((int64_t)nw * (x + 1) / kXcd);
         for (int32_t w = (int32_t)((int64_t)nw * x / kXcd) + (int32_t)(blockIdx.x / kXcd); w < w1; w += per_x) {
             rep_geometry(rg, rg.wl[w], g, cx, cy, z0, z1);
-            nlist_segment(S, dyn, g, cx, cy, z0, z1, rx, ry, rz, rl2, cap, list, count, err, lds_cap);
+            nlist_segment<kB>(S, dyn, g, cx, cy, z0, z1, rx, ry, rz, rl2, cap, list, count, err, lds_cap);
             __syncthreads();   // the staging is reused by the next segment
         }
     } else {
         if (seg_geometry(g, cx, cy, z0, z1) < 0) return;
-        nlist_segment(S, dyn, g, cx, cy, z0, z1, rx, ry, rz, rl2, cap, list, count, err, lds_cap);
+        nlist_segment<kB>(S, dyn, g, cx, cy, z0, z1, rx, ry, rz, rl2, cap, list, count, err, lds_cap);
     }
 }
 
@@ -472,14 +473,14 @@ __device__ inline double pick3(int32_t q, double a, double b, double c) { return
 
 // 8 waves per SIMD (64 VGPRs) for the mixed body; the FP64 body's divisions
 // need more registers, so it runs at 4
-template <bool kRep, bool kFp64, bool kFinal, bool kForceOut>
-__global__ void __launch_bounds__(kBlock, kFp64 ? 4 : 8)
+template <bool kRep, bool kFp64, bool kFinal, bool kForceOut, int kB>
+__global__ void __launch_bounds__(kB, kFp64 ? 4 : 8)
 k_step_lds(StepArgs a) {
     __shared__ Stage S;
     extern __shared__ double dyn[];
     SegGrid g = a.g;
     if (blockIdx.x == 0 && a.clear)
-        for (int32_t t = threadIdx.x; t < 3 * kSpread; t += kBlock) a.clear[t] = 0;
+        for (int32_t t = threadIdx.x; t < 3 * kSpread; t += kB) a.clear[t] = 0;
     int32_t cx, cy, z0, z1, rep = 0, seg = 0;
     if constexpr (kRep) {
         const int32_t b = rep_block(a.rg);
@@ -507,7 +508,7 @@ k_step_lds(StepArgs a) {
         count_first = a.count[i_first];
         chunk_first = list_of(i_first)[0];
     }
-    if (!stage(S, P, g, cx, cy, z0, z1, a.st.r[0], a.st.r[1], a.st.r[2])) {
+    if (!stage<kB>(S, P, g, cx, cy, z0, z1, a.st.r[0], a.st.r[1], a.st.r[2])) {
         // cannot happen after a successful list build on the same cells: the
         // single box reports it as stale, the replicas through their error word
         if (threadIdx.x == 0) {
@@ -518,7 +519,7 @@ k_step_lds(StepArgs a) {
         return;
     }
     double d2max = 0.0, mv2max = 0.0, stale = 0.0;
-    for (int32_t base = hbeg; base < hend; base += kApp) {
+    for (int32_t base = hbeg; base < hend; base += kB / kLanes) {
         const int32_t i = base + threadIdx.x / kLanes;
         if (i >= hend) continue;              // whole quads drop out together
         // r(t) of the home atom: its staged copy (centre column, never shifted)
@@ -667,13 +668,13 @@ void allow_lds(K kernel) {
 
 }  // namespace
 
-int32_t lds_segments_per_column(int32_t nz, double atoms_per_cell, int32_t kmax) {
-    // about 0.85 x 128 home atoms per segment (one pass of the workgroup's
-    // quads), at most kmax cells per segment
+int32_t lds_segments_per_column(int32_t nz, double atoms_per_cell, int32_t kmax, int32_t block) {
+    // about 0.85 x block / 4 home atoms per segment (one pass of the
+    // workgroup's quads), at most kmax cells per segment
     kmax = kmax < kMaxK ? kmax : kMaxK;
     kmax = kmax < 1 ? 1 : kmax;
     const double apc = atoms_per_cell > 0 ? atoms_per_cell : 1.0;
-    int32_t nseg = (int32_t)std::ceil(nz * apc / (0.85 * kApp));
+    int32_t nseg = (int32_t)std::ceil(nz * apc / (0.85 * (block / kLanes)));
     const int32_t lo = (nz + kmax - 1) / kmax;   // segments no longer than kmax cells
     nseg = nseg < lo ? lo : nseg;
     return nseg > nz ? nz : nseg;
@@ -689,16 +690,25 @@ int32_t flag_spread() { return kSpread; }
 
 hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg, double rl2,
                             int32_t n, int32_t cap, uint16_t* list, int32_t* count, int32_t* err,
-                            int32_t lds_atoms) {
+                            int32_t lds_atoms, int32_t block) {
     if (n <= 0) return hipSuccess;
     if (cap % (kLanes * kChunk) != 0) return hipErrorInvalidValue;
     if (lds_atoms < 1 || lds_atoms > kLdsAtoms) return hipErrorInvalidValue;
-    static const bool attr_set = (allow_lds(k_nlist_lds<false>), true);  // once per process, thread-safe
+    if (block != 512 && block != 1024) return hipErrorInvalidValue;
+    static const bool attr_set = [] {  // once per process, thread-safe
+        allow_lds(k_nlist_lds<false, 512>);
+        allow_lds(k_nlist_lds<false, 1024>);
+        return true;
+    }();
     (void)attr_set;
     SegGrid g = make_grid(v, nseg);
     const size_t dyn = 3 * ((size_t)lds_atoms + kScanSlack) * sizeof(float);
-    k_nlist_lds<false><<<g.per_xcd * kXcd, kBlock, dyn, s>>>(g, RepGeo{}, st.r[0], st.r[1], st.r[2], rl2, cap,
-                                                             list, count, err, lds_atoms);
+    if (block == 1024)
+        k_nlist_lds<false, 1024><<<g.per_xcd * kXcd, 1024, dyn, s>>>(g, RepGeo{}, st.r[0], st.r[1], st.r[2], rl2,
+                                                                     cap, list, count, err, lds_atoms);
+    else
+        k_nlist_lds<false, 512><<<g.per_xcd * kXcd, 512, dyn, s>>>(g, RepGeo{}, st.r[0], st.r[1], st.r[2], rl2,
+                                                                   cap, list, count, err, lds_atoms);
     return hipGetLastError();
 }
 
@@ -706,21 +716,23 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
                            double* const rn[3], double* const r0[3], const uint16_t* list, const int32_t* count,
                            int32_t cap, const LjTable& t, const MassTable& m, const GearCoeffs& gc, double hard2, int32_t n,
                            int64_t cs, int32_t precision, bool final_step, bool force_out, uint64_t* flags,
-                           int32_t lds_atoms, uint64_t* clear_next, FlagPublish pub) {
+                           int32_t lds_atoms, uint64_t* clear_next, FlagPublish pub, int32_t block) {
     if (n <= 0) return hipSuccess;
     static const bool attr_set = [] {  // once per process, thread-safe
-        allow_lds(k_step_lds<false, true, true, true>);
-        allow_lds(k_step_lds<false, true, true, false>);
-        allow_lds(k_step_lds<false, true, false, true>);
-        allow_lds(k_step_lds<false, true, false, false>);
-        allow_lds(k_step_lds<false, false, true, true>);
-        allow_lds(k_step_lds<false, false, true, false>);
-        allow_lds(k_step_lds<false, false, false, true>);
-        allow_lds(k_step_lds<false, false, false, false>);
+        allow_lds(k_step_lds<false, true, true, true, 512>);
+        allow_lds(k_step_lds<false, true, false, false, 512>);
+        allow_lds(k_step_lds<false, false, true, true, 512>);
+        allow_lds(k_step_lds<false, false, false, false, 512>);
+        allow_lds(k_step_lds<false, true, true, true, 1024>);
+        allow_lds(k_step_lds<false, true, false, false, 1024>);
+        allow_lds(k_step_lds<false, false, true, true, 1024>);
+        allow_lds(k_step_lds<false, false, false, false, 1024>);
         return true;
     }();
     (void)attr_set;
     if (lds_atoms < 1 || lds_atoms > kLdsAtoms) return hipErrorInvalidValue;
+    if (block != 512 && block != 1024) return hipErrorInvalidValue;
+    if (force_out != final_step) return hipErrorInvalidValue;   // the force is written on the last step
     StepArgs a{};
     a.g = make_grid(v, nseg);
     a.st = st;
@@ -754,13 +766,15 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
     const size_t dyn = 3 * (size_t)lds_atoms * sizeof(double);
     dim3 grid(a.g.per_xcd * kXcd);
     const bool fp64 = precision == MDX_PREC_FP64;
-#define MDX_STEP(F64, FIN, FO) k_step_lds<false, F64, FIN, FO><<<grid, kBlock, dyn, s>>>(a)
+#define MDX_STEP(F64, FIN)                                                               \
+    do {                                                                                 \
+        if (block == 1024) k_step_lds<false, F64, FIN, FIN, 1024><<<grid, 1024, dyn, s>>>(a); \
+        else k_step_lds<false, F64, FIN, FIN, 512><<<grid, 512, dyn, s>>>(a);              \
+    } while (0)
     if (fp64) {
-        if (final_step) { if (force_out) MDX_STEP(true, true, true); else MDX_STEP(true, true, false); }
-        else { if (force_out) MDX_STEP(true, false, true); else MDX_STEP(true, false, false); }
+        if (final_step) MDX_STEP(true, true); else MDX_STEP(true, false);
     } else {
-        if (final_step) { if (force_out) MDX_STEP(false, true, true); else MDX_STEP(false, true, false); }
-        else { if (force_out) MDX_STEP(false, false, true); else MDX_STEP(false, false, false); }
+        if (final_step) MDX_STEP(false, true); else MDX_STEP(false, false);
     }
 #undef MDX_STEP
     return hipGetLastError();
@@ -790,13 +804,13 @@ hipError_t launch_rep_nlist(hipStream_t s, const RepSegs& rs, const int32_t* wl,
     if (rs.nblocks <= 0) return hipSuccess;
     if (cap % (kLanes * kChunk) != 0) return hipErrorInvalidValue;
     if (lds_atoms < 1 || lds_atoms > kLdsAtoms) return hipErrorInvalidValue;
-    static const bool attr_set = (allow_lds(k_nlist_lds<true>), true);  // once per process, thread-safe
+    static const bool attr_set = (allow_lds(k_nlist_lds<true, kBlock>), true);  // once per process, thread-safe
     (void)attr_set;
     const size_t dyn = 3 * ((size_t)lds_atoms + kScanSlack) * sizeof(float);
     // a grid of resident blocks (4 a CU at the full staging), a multiple of 8
     const int32_t all = (rs.nblocks + kXcd - 1) / kXcd * kXcd;
     const int32_t grid = all < 1024 ? all : 1024;
-    k_nlist_lds<true><<<grid, kBlock, dyn, s>>>(SegGrid{}, rep_geo(rs, wl, wcount), r[0], r[1], r[2], rl2, cap, list,
+    k_nlist_lds<true, kBlock><<<grid, kBlock, dyn, s>>>(SegGrid{}, rep_geo(rs, wl, wcount), r[0], r[1], r[2], rl2, cap, list,
                                                 count, err, lds_atoms);
     return hipGetLastError();
 }
@@ -808,10 +822,10 @@ hipError_t launch_rep_step(hipStream_t s, const RepSegs& rs, const DevState& st,
                            int32_t lds_atoms) {
     if (rs.nblocks <= 0) return hipSuccess;
     static const bool attr_set = [] {  // once per process, thread-safe
-        allow_lds(k_step_lds<true, true, true, true>);
-        allow_lds(k_step_lds<true, true, false, false>);
-        allow_lds(k_step_lds<true, false, true, true>);
-        allow_lds(k_step_lds<true, false, false, false>);
+        allow_lds(k_step_lds<true, true, true, true, kBlock>);
+        allow_lds(k_step_lds<true, true, false, false, kBlock>);
+        allow_lds(k_step_lds<true, false, true, true, kBlock>);
+        allow_lds(k_step_lds<true, false, false, false, kBlock>);
         return true;
     }();
     (void)attr_set;
@@ -850,11 +864,11 @@ hipError_t launch_rep_step(hipStream_t s, const RepSegs& rs, const DevState& st,
     // a step: corrector and the next predictor; the last step of a run:
     // corrector only, with the force written for read-back
     if (fp64) {
-        if (final_step) k_step_lds<true, true, true, true><<<grid, kBlock, dyn, s>>>(a);
-        else k_step_lds<true, true, false, false><<<grid, kBlock, dyn, s>>>(a);
+        if (final_step) k_step_lds<true, true, true, true, kBlock><<<grid, kBlock, dyn, s>>>(a);
+        else k_step_lds<true, true, false, false, kBlock><<<grid, kBlock, dyn, s>>>(a);
     } else {
-        if (final_step) k_step_lds<true, false, true, true><<<grid, kBlock, dyn, s>>>(a);
-        else k_step_lds<true, false, false, false><<<grid, kBlock, dyn, s>>>(a);
+        if (final_step) k_step_lds<true, false, true, true, kBlock><<<grid, kBlock, dyn, s>>>(a);
+        else k_step_lds<true, false, false, false, kBlock><<<grid, kBlock, dyn, s>>>(a);
     }
     return hipGetLastError();
 }

@@ -251,7 +251,8 @@ hipError_t launch_gather_correct_predict_track(hipStream_t s, const DevState& st
                                                uint64_t* clear_next);
 
 // lj_fused.hip: LDS-staged segment kernels of the periodic box
-int32_t lds_segments_per_column(int32_t nz, double atoms_per_cell, int32_t kmax);
+// segments per cell column for workgroups of `block` threads (4 lanes an atom)
+int32_t lds_segments_per_column(int32_t nz, double atoms_per_cell, int32_t kmax, int32_t block);
 int32_t lds_capacity();
 int32_t flag_spread();
 int64_t list_entries(int32_t n, int32_t cap);   // 16-bit entries to allocate for the list
@@ -260,15 +261,17 @@ int32_t list_cap_multiple(); // the list capacity (entries per atom) must be a m
 // cap; err[2]: the largest staging (atoms), which sizes the step kernel's LDS
 hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg, double rl2,
                             int32_t n, int32_t cap, uint16_t* list, int32_t* count, int32_t* err,
-                            int32_t lds_atoms);
+                            int32_t lds_atoms, int32_t block);
 // st, rn and r0 hold each vector's three components in one block, component d
 // at [0] + d * cs; clear_next (may be null): the next step's flag words, zeroed;
-// pub: the flag hand-over (blocks is set here)
+// pub: the flag hand-over (blocks is set here); block: 512 or 1024 threads a
+// workgroup (the segments must have been cut for it); the force is written on
+// the final step only (force_out == final_step)
 hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg,
                            double* const rn[3], double* const r0[3], const uint16_t* list, const int32_t* count,
                            int32_t cap, const LjTable& t, const MassTable& m, const GearCoeffs& gc, double hard2,
                            int32_t n, int64_t cs, int32_t precision, bool final_step, bool force_out, uint64_t* flags,
-                           int32_t lds_atoms, uint64_t* clear_next, FlagPublish pub);
+                           int32_t lds_atoms, uint64_t* clear_next, FlagPublish pub, int32_t block);
 
 // the lists of launch_nlist_lds as caller indices: row id[i] of rows (width
 // entries) holds atom i's partners in list order, counts[id[i]] their number;

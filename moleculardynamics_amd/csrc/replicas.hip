@@ -1023,7 +1023,7 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
             continue;
         }
         const double per_cell = (double)n_atoms[r] / ((double)nc * nc * nc);
-        int32_t nseg = lds_segments_per_column(nc, per_cell, nc);
+        int32_t nseg = lds_segments_per_column(nc, per_cell, nc, 512);
         while (nseg < nc && 9.0 * ((nc + nseg - 1) / nseg + 2) * per_cell > kStageAim) nseg++;
         const int32_t K = (nc + nseg - 1) / nseg;
         if (1.2 * 9.0 * (K + 2) * per_cell + 64 > lds_capacity()) continue;
