@@ -217,6 +217,18 @@ const char* mdx_kinetics_noalloc_last_error(void);   /* the calling thread's */
  * to zero at rc in the periodic box; unshifted in the walled box).
  * out2 = {LJ part, bond-order part}. */
 int mdx_potential(mdx_system* sys, double* out2);
+/* mdx_potential with the virial: out4 = {LJ energy, bond-order energy, LJ
+ * virial, bond-order virial}.  The virial is W = sum over pairs i < j of
+ * r_ij . F_ij (r_ij = r_i - r_j, minimum image in the periodic box; F_ij the
+ * force of j on i) over the current lists with the force cutoffs, summed in
+ * FP64 (per atom, then wave-shuffle block sums); for the many-body forms
+ * W = sum_a x_a . F_a: over the per-bond gradients of the atom energies in
+ * the periodic box (-sum_i sum_j (x_j - x_i) . de_i/d(x_j - x_i)), over the
+ * bond-order atoms' bond forces in the isolated walled box.  The pressure is
+ * (2 EK + W) / (3 V).  The reference reports no virial (its records hold EK,
+ * T, EU, the centre of mass and the momentum,
+ * common/DebugCalculations.cpp:46-124); this extends its diagnostics. */
+int mdx_potential_virial(mdx_system* sys, double* out4);
 /* common/DebugCalculations.cpp:129-270: LJ over all pairs without cutoff and
  * the reference bond-order energy with xi over all atoms (O(N^2) / O(N^3)). */
 int mdx_potential_allpairs(mdx_system* sys, double* out2);
@@ -311,6 +323,10 @@ int mdx_batch_build_counts(mdx_batch* batch, int64_t* builds);
 int mdx_batch_staged(mdx_batch* batch, int32_t* path);
 /* per replica {kinetic, potential (LJ shifted to zero at rc)}: 2 * n_replicas */
 int mdx_batch_energies(mdx_batch* batch, double* out2);
+/* per replica the virial W = sum over pairs i < j of r_ij . F_ij (minimum
+ * image), n_replicas doubles; per-atom FP64 terms summed by wave shuffles.
+ * No reference counterpart (mdx_potential_virial is the single box's). */
+int mdx_batch_virials(mdx_batch* batch, double* out);
 /* per-kernel timing with events; while enabled, mdx_batch_run launches the
  * kernels one by one instead of through the graph */
 int mdx_batch_set_timing(mdx_batch* batch, int32_t enabled);

@@ -77,12 +77,13 @@ def load() -> C.CDLL:
         "mdx_correct": [P, C.POINTER(I32)], "mdx_accelerations": [P],
         "mdx_run": [P, I32], "mdx_build_count": [P, C.POINTER(I64)],
         "mdx_kinetics": [P, D, P], "mdx_potential": [P, P], "mdx_potential_allpairs": [P, P],
+        "mdx_potential_virial": [P, P],
         "mdx_set_timing": [P, I32], "mdx_kernel_times": [P, P, P, P, I32, C.POINTER(I32)],
         "mdx_synchronize": [P], "mdx_set_forces": [P, P],
         "mdx_batch_create": [C.POINTER(MdxConfig), I32, P, P, C.POINTER(P)],
         "mdx_batch_destroy": [P], "mdx_batch_set_state": [P, P, P], "mdx_batch_get_state": [P, P, P, P],
         "mdx_batch_run": [P, I32], "mdx_batch_build_counts": [P, P], "mdx_batch_staged": [P, P],
-        "mdx_batch_energies": [P, P],
+        "mdx_batch_energies": [P, P], "mdx_batch_virials": [P, P],
         "mdx_batch_set_timing": [P, I32], "mdx_batch_kernel_times": [P, P, P, P, I32, C.POINTER(I32)],
         "mdx_batch_synchronize": [P],
         "mdx_set_surrogate": [P, P, I64], "mdx_surrogate_counts": [P, P],
@@ -227,6 +228,12 @@ class System:
         self._chk("mdx_potential", self.lib.mdx_potential(self.h, _ptr(out)))
         return out
 
+    def potential_virial(self):
+        """{LJ energy, bond-order energy, LJ virial, bond-order virial}."""
+        out = np.zeros(4)
+        self._chk("mdx_potential_virial", self.lib.mdx_potential_virial(self.h, _ptr(out)))
+        return out
+
     def potential_allpairs(self):
         out = np.zeros(2)
         self._chk("mdx_potential_allpairs", self.lib.mdx_potential_allpairs(self.h, _ptr(out)))
@@ -349,6 +356,12 @@ class Batch:
         e = np.zeros((self.R, 2))
         self._chk("mdx_batch_energies", self.lib.mdx_batch_energies(self.h, _ptr(e)))
         return e
+
+    def virials(self):
+        """Per replica the virial sum over pairs of r_ij . F_ij."""
+        out = np.zeros(self.R)
+        self._chk("mdx_batch_virials", self.lib.mdx_batch_virials(self.h, _ptr(out)))
+        return out
 
     def synchronize(self):
         self._chk("mdx_batch_synchronize", self.lib.mdx_batch_synchronize(self.h))

@@ -19,7 +19,7 @@ constexpr int kBlock = 256;
 template <bool kFp64, bool kEnergy>
 __global__ void __launch_bounds__(kBlock)
 k_lj(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable t, int32_t n,
-     double* __restrict__ e_atom, const int32_t* __restrict__ gate) {
+     double* __restrict__ e_atom, double* __restrict__ w_atom, const int32_t* __restrict__ gate) {
     if (gate && *gate == 0) return;
     int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
@@ -27,7 +27,7 @@ k_lj(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable t, in
     const int32_t ti = st.type[s] & (kTypes - 1);
     const int32_t own = cell_of[s];
     const int32_t cz = own % g.g[2], cy = (own / g.g[2]) % g.g[1], cx = own / (g.g[2] * g.g[1]);
-    double fx = 0, fy = 0, fz = 0, e = 0;
+    double fx = 0, fy = 0, fz = 0, e = 0, w = 0;
     for (int dx = -1; dx <= 1; dx++) {
         int32_t x = cx + dx;
         double sx = 0.0;
@@ -71,6 +71,7 @@ k_lj(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable t, in
                         if constexpr (kEnergy) {
                             const double iq6 = 1.0 / (q * q * q);
                             e += 0.5 * (t.e4[k] * iq6 * (iq6 - 1) - t.shift[k]);
+                            w += 0.5 * fr * r2;   // r_ij . F_ij, half on each side of the pair
                         }
                     } else {
                         // s2 = (sig/r)^2, s6 = (sig/r)^6, s8 = (sig/r)^8
@@ -86,6 +87,7 @@ k_lj(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable t, in
                         fz = __builtin_fma(frd, ddz, fz);
                         if constexpr (kEnergy) {
                             e += 0.5 * ((double)(t.e4_f[k] * s6 * (s6 - 1.0f)) - t.shift[k]);
+                            w += 0.5 * frd * r2;
                         }
                     }
                 }
@@ -95,7 +97,10 @@ k_lj(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable t, in
     st.f[0][s] = fx;
     st.f[1][s] = fy;
     st.f[2][s] = fz;
-    if constexpr (kEnergy) e_atom[s] = e;
+    if constexpr (kEnergy) {
+        e_atom[s] = e;
+        w_atom[s] = w;
+    }
 }
 
 // Walled box, few atoms: one wavefront per atom.  The 27 candidate cells of
@@ -109,7 +114,7 @@ constexpr int kWaveBlock = 256;
 template <bool kFp64, bool kEnergy>
 __global__ void __launch_bounds__(kWaveBlock)
 k_lj_wave(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable t, int32_t n,
-          double* __restrict__ e_atom, const int32_t* __restrict__ gate) {
+          double* __restrict__ e_atom, double* __restrict__ w_atom, const int32_t* __restrict__ gate) {
     if (gate && *gate == 0) return;
     const int32_t s = (int32_t)((blockIdx.x * kWaveBlock + threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
@@ -138,7 +143,7 @@ k_lj_wave(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable 
         off[r] = len;
         len += __builtin_amdgcn_readlane(rn, r);
     }
-    double fx = 0, fy = 0, fz = 0, e = 0;
+    double fx = 0, fy = 0, fz = 0, e = 0, w = 0;
     for (int32_t q0 = 0; q0 < len; q0 += 64) {
         const int32_t q = q0 + lane;
         int32_t p = -1;
@@ -150,7 +155,7 @@ k_lj_wave(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable 
                 if (q >= off[r]) p = base[r] + (q - off[r]);
         }
         bool in = false;
-        double ddx = 0, ddy = 0, ddz = 0, term_e = 0, c0 = 0, c1 = 0, c2 = 0;
+        double ddx = 0, ddy = 0, ddz = 0, term_e = 0, term_w = 0, c0 = 0, c1 = 0, c2 = 0;
         [[maybe_unused]] double frd = 0;
         if (p >= 0 && p != s) {
             ddx = xi - st.r[0][p];
@@ -171,6 +176,7 @@ k_lj_wave(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable 
                     if constexpr (kEnergy) {
                         const double iq6 = 1.0 / (qq * qq * qq);
                         term_e = 0.5 * (t.e4[k] * iq6 * (iq6 - 1) - t.shift[k]);
+                        term_w = 0.5 * fr * r2;
                     }
                 } else {
                     const float s2 = __builtin_amdgcn_rcpf((float)r2 * t.inv_sig2f[k]);
@@ -178,7 +184,10 @@ k_lj_wave(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable 
                     const float s6 = s4 * s2;
                     const float s8 = s4 * s4;
                     frd = (double)(s8 * __builtin_fmaf(2.0f, s6, -1.0f) * t.coeff_f[k]);
-                    if constexpr (kEnergy) term_e = 0.5 * ((double)(t.e4_f[k] * s6 * (s6 - 1.0f)) - t.shift[k]);
+                    if constexpr (kEnergy) {
+                        term_e = 0.5 * ((double)(t.e4_f[k] * s6 * (s6 - 1.0f)) - t.shift[k]);
+                        term_w = 0.5 * frd * r2;
+                    }
                 }
             }
         }
@@ -196,14 +205,20 @@ k_lj_wave(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable 
                 fy = __builtin_fma(f, lane_double(ddy, l), fy);
                 fz = __builtin_fma(f, lane_double(ddz, l), fz);
             }
-            if constexpr (kEnergy) e += lane_double(term_e, l);
+            if constexpr (kEnergy) {
+                e += lane_double(term_e, l);
+                w += lane_double(term_w, l);
+            }
         }
     }
     if (lane == 0) {
         st.f[0][s] = fx;
         st.f[1][s] = fy;
         st.f[2][s] = fz;
-        if constexpr (kEnergy) e_atom[s] = e;
+        if constexpr (kEnergy) {
+            e_atom[s] = e;
+            w_atom[s] = w;
+        }
     }
 }
 
@@ -214,30 +229,31 @@ constexpr int32_t kWaveAtoms = 32768;
 
 hipError_t launch_lj(hipStream_t s, const DevState& st, const int32_t* cell_of, const GridView& g,
                      const LjTable& t, int32_t n, int32_t precision, double* energy_per_atom,
-                     const int32_t* gate) {
+                     const int32_t* gate, double* virial_per_atom) {
+    if (energy_per_atom && !virial_per_atom) return hipErrorInvalidValue;
     if (n <= 0) return hipSuccess;
     const bool fp64 = precision == MDX_PREC_FP64;
     if (!g.periodic && n <= kWaveAtoms) {
         dim3 wgrid((n + kWaveBlock / 64 - 1) / (kWaveBlock / 64));
         if (fp64 && energy_per_atom)
-            k_lj_wave<true, true><<<wgrid, kWaveBlock, 0, s>>>(st, cell_of, g, t, n, energy_per_atom, gate);
+            k_lj_wave<true, true><<<wgrid, kWaveBlock, 0, s>>>(st, cell_of, g, t, n, energy_per_atom, virial_per_atom, gate);
         else if (fp64)
-            k_lj_wave<true, false><<<wgrid, kWaveBlock, 0, s>>>(st, cell_of, g, t, n, nullptr, gate);
+            k_lj_wave<true, false><<<wgrid, kWaveBlock, 0, s>>>(st, cell_of, g, t, n, nullptr, nullptr, gate);
         else if (energy_per_atom)
-            k_lj_wave<false, true><<<wgrid, kWaveBlock, 0, s>>>(st, cell_of, g, t, n, energy_per_atom, gate);
+            k_lj_wave<false, true><<<wgrid, kWaveBlock, 0, s>>>(st, cell_of, g, t, n, energy_per_atom, virial_per_atom, gate);
         else
-            k_lj_wave<false, false><<<wgrid, kWaveBlock, 0, s>>>(st, cell_of, g, t, n, nullptr, gate);
+            k_lj_wave<false, false><<<wgrid, kWaveBlock, 0, s>>>(st, cell_of, g, t, n, nullptr, nullptr, gate);
         return hipGetLastError();
     }
     dim3 grid((n + kBlock - 1) / kBlock);
     if (fp64 && energy_per_atom)
-        k_lj<true, true><<<grid, kBlock, 0, s>>>(st, cell_of, g, t, n, energy_per_atom, gate);
+        k_lj<true, true><<<grid, kBlock, 0, s>>>(st, cell_of, g, t, n, energy_per_atom, virial_per_atom, gate);
     else if (fp64)
-        k_lj<true, false><<<grid, kBlock, 0, s>>>(st, cell_of, g, t, n, nullptr, gate);
+        k_lj<true, false><<<grid, kBlock, 0, s>>>(st, cell_of, g, t, n, nullptr, nullptr, gate);
     else if (energy_per_atom)
-        k_lj<false, true><<<grid, kBlock, 0, s>>>(st, cell_of, g, t, n, energy_per_atom, gate);
+        k_lj<false, true><<<grid, kBlock, 0, s>>>(st, cell_of, g, t, n, energy_per_atom, virial_per_atom, gate);
     else
-        k_lj<false, false><<<grid, kBlock, 0, s>>>(st, cell_of, g, t, n, nullptr, gate);
+        k_lj<false, false><<<grid, kBlock, 0, s>>>(st, cell_of, g, t, n, nullptr, nullptr, gate);
     return hipGetLastError();
 }
 

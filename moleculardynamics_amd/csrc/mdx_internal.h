@@ -205,9 +205,11 @@ hipError_t launch_reorder(hipStream_t s, const DevState& src, const DevState& ds
 // lj.hip
 // gate: when non-null the kernel does nothing unless *gate != 0 (device-side
 // predicate for the force recompute after a wall hit)
+// with energy_per_atom the per-atom virial (half of r_ij . F_ij over the
+// atom's pairs) goes to virial_per_atom, which must then be non-null
 hipError_t launch_lj(hipStream_t s, const DevState& st, const int32_t* cell_of, const GridView& g,
                      const LjTable& t, int32_t n, int32_t precision, double* energy_per_atom,
-                     const int32_t* gate);
+                     const int32_t* gate, double* virial_per_atom = nullptr);
 
 // gear.hip
 struct GearCoeffs {
@@ -356,6 +358,13 @@ int32_t kinetics_blocks(int32_t n);
 hipError_t launch_sum(hipStream_t s, const double* x, int32_t n, double* partials, int32_t nblocks,
                       double* out);
 hipError_t launch_ready_list(hipStream_t s, const DevState& st, int32_t n, uint64_t* out2_bits);
+// w[i] = pos.r_s . frc.f_s for slot s = map[i] (map null: s = i), i < n
+hipError_t launch_rf_virial(hipStream_t s, const DevState& pos, const DevState& frc, const int32_t* map, int32_t n,
+                            double* w);
+// per-atom virial from per-slot gradients w_is = de_i/d(x_j - x_i):
+// w_atom[i] = -sum_s (x_j - x_i) . w_is (FP32 w, or FP64 wd when non-null)
+hipError_t launch_slot_virial(hipStream_t s, const DevState& st, double L, const int32_t* list, const int32_t* count,
+                              int32_t n, const float4* w, const double4* wd, double* w_atom);
 hipError_t launch_allpairs(hipStream_t s, const DevState& st, int32_t n, int32_t n_bond,
                            const LjTable& lj, const BondRefParams& p, double* per_atom2);
 

@@ -141,6 +141,16 @@ k_ready(DevState st, int32_t n, unsigned long long* __restrict__ out) {
     }
 }
 
+// w[i] = r_s . F_s of slot s = map[i] (or i), the virial of an isolated
+// system atom by atom
+__global__ void __launch_bounds__(kBlock)
+k_rf(DevState pos, DevState frc, const int32_t* __restrict__ map, int32_t n, double* __restrict__ w) {
+    const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int32_t s = map ? map[i] : i;
+    w[i] = pos.r[0][s] * frc.f[0][s] + pos.r[1][s] * frc.f[1][s] + pos.r[2][s] * frc.f[2][s];
+}
+
 inline int32_t blocks_for(int32_t n) {
     int32_t b = (n + kBlock - 1) / kBlock;
     return b < 1 ? 1 : (b > kMaxBlocks ? kMaxBlocks : b);
@@ -165,6 +175,13 @@ hipError_t launch_sum(hipStream_t s, const double* x, int32_t n, double* partial
                       double* out) {
     k_sum1<<<nb, kBlock, 0, s>>>(x, n, partials);
     k_sum2<<<1, kBlock, 0, s>>>(partials, nb, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_rf_virial(hipStream_t s, const DevState& pos, const DevState& frc, const int32_t* map, int32_t n,
+                            double* w) {
+    if (n <= 0) return hipSuccess;
+    k_rf<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(pos, frc, map, n, w);
     return hipGetLastError();
 }
 

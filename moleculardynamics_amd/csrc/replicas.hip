@@ -669,29 +669,45 @@ __global__ void k_rep_set(int32_t R, int32_t* __restrict__ rebuild, int32_t v) {
     if (r < R) rebuild[r] = v;
 }
 
-// per-replica kinetic and shifted-LJ potential energy (validation; FP64 atomics)
+// per-replica kinetic energy, shifted-LJ potential energy and virial
+// (sum over pairs of r_ij . F_ij) into out[3 r + {0, 1, 2}], FP64: every wave
+// lies inside one replica (offsets are multiples of 64), so each wave sums its
+// lanes by shuffles and adds the three sums once
 __global__ void __launch_bounds__(kBlock)
 k_rep_energy(RepStep a, double shift, double e4, double* __restrict__ out) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.ntot) return;
-    const int32_t r = a.v.rep_of[i];
-    if (r < 0) return;
-    const double L = a.v.L[r], invL = 1.0 / L;
-    const double xi = a.r[0][i], yi = a.r[1][i], zi = a.r[2][i];
-    double u = 0;
-    const int32_t cnt = a.count[i];
-    for (int32_t k = 0; k < cnt; k++) {
-        const int32_t j = a.list[(int64_t)k * a.ntot + i];
-        const double dx = mi(xi - a.r[0][j], L, invL), dy = mi(yi - a.r[1][j], L, invL), dz = mi(zi - a.r[2][j], L, invL);
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 < a.rc2) {
-            const double q = a.sig2 / r2, q3 = q * q * q;
-            u += 0.5 * (e4 * q3 * (q3 - 1) - shift);
+    const int32_t r = i < a.ntot ? a.v.rep_of[i] : -1;
+    double ek = 0, u = 0, w = 0;
+    if (r >= 0) {
+        const double L = a.v.L[r], invL = 1.0 / L;
+        const double xi = a.r[0][i], yi = a.r[1][i], zi = a.r[2][i];
+        const int32_t cnt = a.count[i];
+        for (int32_t k = 0; k < cnt; k++) {
+            const int32_t j = a.list[(int64_t)k * a.ntot + i];
+            const double dx = mi(xi - a.r[0][j], L, invL), dy = mi(yi - a.r[1][j], L, invL),
+                         dz = mi(zi - a.r[2][j], L, invL);
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < a.rc2) {
+                const double q = a.sig2 / r2, q3 = q * q * q;
+                u += 0.5 * (e4 * q3 * (q3 - 1) - shift);
+                // F/r = 24 eps/sig^2 (2 q^7 - q^4), half the pair's r . F on each side
+                w += 0.5 * a.coeff * q3 * q * (2 * q3 - 1) * r2;
+            }
         }
+        ek = 0.5 * a.mass * (a.vel[0][i] * a.vel[0][i] + a.vel[1][i] * a.vel[1][i] + a.vel[2][i] * a.vel[2][i]);
     }
-    const double ek = 0.5 * a.mass * (a.vel[0][i] * a.vel[0][i] + a.vel[1][i] * a.vel[1][i] + a.vel[2][i] * a.vel[2][i]);
-    atomicAdd(&out[2 * r], ek);
-    atomicAdd(&out[2 * r + 1], u);
+    int32_t rw = r;
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        ek += __shfl_xor(ek, o, kWave);
+        u += __shfl_xor(u, o, kWave);
+        w += __shfl_xor(w, o, kWave);
+        rw = max(rw, __shfl_xor(rw, o, kWave));
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0 && rw >= 0) {
+        atomicAdd(&out[3 * rw], ek);
+        atomicAdd(&out[3 * rw + 1], u);
+        atomicAdd(&out[3 * rw + 2], w);
+    }
 }
 
 }  // namespace
@@ -1092,7 +1108,7 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
         (rc = balloc(b, &b->d_rq[0], n)) || (rc = balloc(b, &b->d_rq[1], n)) ||
         (rc = balloc(b, &b->d_list, (size_t)b->cap * n)) || (rc = balloc(b, &b->d_count, n)) ||
         (rc = balloc(b, &b->d_err, 2)) ||
-        (rc = balloc(b, &b->d_energy, 2 * (size_t)n_replicas)) ||
+        (rc = balloc(b, &b->d_energy, 3 * (size_t)n_replicas)) ||
         (rc = balloc3(b, b->r, n)) || (rc = balloc3(b, b->rn, n)) || (rc = balloc3(b, b->v, n)) ||
         (rc = balloc3(b, b->a, n)) || (rc = balloc3(b, b->b, n)) || (rc = balloc3(b, b->c, n)) ||
         (rc = balloc3(b, b->f, n)) || (rc = balloc3(b, b->r0, n))) {
@@ -1307,10 +1323,10 @@ int mdx_batch_staged(mdx_batch* b, int32_t* path) {
     return MDX_OK;
 }
 
-int mdx_batch_energies(mdx_batch* b, double* out2) {
-    if (!b || !out2) return bfail(b, MDX_ERR_ARG, "null argument");
+// per replica {kinetic, potential, virial} into out3 (3 R doubles)
+static int batch_energies(mdx_batch* b, double* out3) {
     hipStream_t s = b->stream;
-    BCHK(b, hipMemsetAsync(b->d_energy, 0, sizeof(double) * 2 * b->R, s));
+    BCHK(b, hipMemsetAsync(b->d_energy, 0, sizeof(double) * 3 * b->R, s));
     if (b->nseg_blocks > 0 || b->nsmall_blocks > 0) {
         // int32 lists of the staged and small replicas from their last build (the
         // list-build positions stay as they are, so the rebuild schedule is unchanged)
@@ -1323,8 +1339,29 @@ int mdx_batch_energies(mdx_batch* b, double* out2) {
     const double qc = a.rc2 / (sg * sg), iq6 = 1.0 / (qc * qc * qc);
     BLAUNCH(b, "rep_energy", (k_rep_energy<<<grid_for(b->ntot), kBlock, 0, s>>>(a, 4 * ep * iq6 * (iq6 - 1), 4 * ep,
                                                                                  b->d_energy)));
-    BCHK(b, hipMemcpyAsync(out2, b->d_energy, sizeof(double) * 2 * b->R, hipMemcpyDeviceToHost, s));
+    BCHK(b, hipMemcpyAsync(out3, b->d_energy, sizeof(double) * 3 * b->R, hipMemcpyDeviceToHost, s));
     BCHK(b, hipStreamSynchronize(s));
+    return MDX_OK;
+}
+
+int mdx_batch_energies(mdx_batch* b, double* out2) {
+    if (!b || !out2) return bfail(b, MDX_ERR_ARG, "null argument");
+    std::vector<double> e3(3 * (size_t)b->R);
+    int rc = batch_energies(b, e3.data());
+    if (rc) return rc;
+    for (int32_t r = 0; r < b->R; r++) {
+        out2[2 * r] = e3[3 * r];
+        out2[2 * r + 1] = e3[3 * r + 1];
+    }
+    return MDX_OK;
+}
+
+int mdx_batch_virials(mdx_batch* b, double* out) {
+    if (!b || !out) return bfail(b, MDX_ERR_ARG, "null argument");
+    std::vector<double> e3(3 * (size_t)b->R);
+    int rc = batch_energies(b, e3.data());
+    if (rc) return rc;
+    for (int32_t r = 0; r < b->R; r++) out[r] = e3[3 * r + 2];
     return MDX_OK;
 }
 

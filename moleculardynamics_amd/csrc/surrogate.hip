@@ -293,9 +293,40 @@ k_sg_gather(Pos3 P, const int32_t* __restrict__ list, const int32_t* __restrict_
     }
 }
 
+// per-atom virial of a sum of atom energies e_i(x_j - x_i) held as per-slot
+// gradients w_is = de_i/d(x_j - x_i): W_i = -sum_s (x_j - x_i) . w_is (FP64)
+__global__ void __launch_bounds__(kBlock)
+k_slot_virial(Pos3 P, const int32_t* __restrict__ list, const int32_t* __restrict__ count, int32_t n,
+              const float4* __restrict__ w, const double4* __restrict__ wd, double* __restrict__ w_atom) {
+    const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int32_t cnt = count[i];
+    double v = 0.0;
+    for (int32_t a = 0; a < cnt; a++) {
+        const int32_t j = list[(int64_t)a * n + i];
+        double d[3];
+        P.d(i, j, d);
+        if (wd) {
+            const double4 u = wd[(int64_t)a * n + i];
+            v -= d[0] * u.x + d[1] * u.y + d[2] * u.z;
+        } else {
+            const float4 u = w[(int64_t)a * n + i];
+            v -= d[0] * (double)u.x + d[1] * (double)u.y + d[2] * (double)u.z;
+        }
+    }
+    w_atom[i] = v;
+}
+
 inline int grid_for(int32_t n) { return (n + kBlock - 1) / kBlock; }
 
 }  // namespace
+
+hipError_t launch_slot_virial(hipStream_t s, const DevState& st, double L, const int32_t* list, const int32_t* count,
+                              int32_t n, const float4* w, const double4* wd, double* w_atom) {
+    if (n <= 0) return hipSuccess;
+    k_slot_virial<<<grid_for(n), kBlock, 0, s>>>(pos3(st, L), list, count, n, w, wd, w_atom);
+    return hipGetLastError();
+}
 
 // Host-side packing of the FP32 parameter blob into the kernel's BF16 MFMA
 // fragments (fragment f, lane l: 8 values) and the small FP32 vectors.

@@ -65,7 +65,7 @@ struct mdx_system {
     int32_t* dint = nullptr;         // [0] wall hit, [1] bin error, [2] max count, [3] max row,
                                      // [4..6] list-build error words, [8..9] pair-list export
     uint64_t* dbits = nullptr;       // [0] max disp^2, [1] max v^2, [2] max |coord|
-    double* e_atom = nullptr;        // 2n scratch
+    double* e_atom = nullptr;        // 4n scratch: energies (LJ, bond order), then virials (LJ, bond order)
     void* pinned = nullptr;          // 256 bytes
     LjTable ljt{};
     MassTable mt{};
@@ -523,15 +523,20 @@ int build_list(mdx_system* s, int32_t which, double box, bool sizes) {
     return MDX_OK;
 }
 
-int lj_forces(mdx_system* s, double* e_atom, const int32_t* gate, const DevState* target = nullptr) {
+int lj_forces(mdx_system* s, double* e_atom, const int32_t* gate, const DevState* target = nullptr,
+              double* w_atom = nullptr) {
     if (!s->lj.built) return fail(s, MDX_ERR_STATE, "Lennard-Jones list not built");
     const DevState& d = target ? *target : s->st;
+    // per-atom energies come with per-atom virials (scratch after the energies when not asked for)
+    if (e_atom && !w_atom) w_atom = s->e_atom + 2 * (size_t)s->n;
     LAUNCH(s, "lj_forces", launch_lj(s->stream, d, s->cell_of, s->lj.view, s->ljt, s->n,
-                                     s->cfg.precision, e_atom, gate));
+                                     s->cfg.precision, e_atom, gate, w_atom));
     return MDX_OK;
 }
 
-int bond_forces(mdx_system* s, double* e_atom, const int32_t* gate, const DevState* target = nullptr) {
+// assign: -1 the reference's rule (add to the LJ force when LJ is on), 1 assign
+int bond_forces(mdx_system* s, double* e_atom, const int32_t* gate, const DevState* target = nullptr,
+                int assign_rule = -1) {
     const mdx_config& c = s->cfg;
     if (is_surrogate(s)) {
         if (!s->ts_built) return fail(s, MDX_ERR_STATE, "Tersoff list not built");
@@ -556,7 +561,7 @@ int bond_forces(mdx_system* s, double* e_atom, const int32_t* gate, const DevSta
         return fail(s, MDX_ERR_ARG, "this bond-order form is not available in this box");
     if (!s->mb.built) return fail(s, MDX_ERR_STATE, "bond-order list not built");
     const DevState& d = target ? *target : s->st;
-    const int assign = c.use_lj ? 0 : 1;  // BondOrderFuncs.h:325-336
+    const int assign = assign_rule >= 0 ? assign_rule : (c.use_lj ? 0 : 1);  // BondOrderFuncs.h:325-336
     LAUNCH(s, "bond_forces",
            launch_bond_ref(s->stream, d, s->slot_of_id, s->mb_cell_of, s->mb_ids, s->mb.view, s->bp,
                            c.n_bond, s->xi_cap, c.precision, assign, s->xi_j, s->xi_v, e_atom, gate,
@@ -895,7 +900,7 @@ int mdx_create(const mdx_config* cfg, mdx_system** out) {
         (rc = dalloc(s, &s->cell_of_alt, n)) || (rc = dalloc(s, &s->sorted, n)) ||
         (rc = dalloc(s, &s->cell_by_id, n)) || (rc = dalloc(s, &s->mb_cell_of, std::max(cfg->n_bond, 1))) ||
         (rc = dalloc(s, &s->mb_ids, std::max(cfg->n_bond, 1))) || (rc = dalloc3(s, s->rref, n)) ||
-        (rc = dalloc(s, &s->e_atom, 2 * (size_t)n)) || (rc = dalloc(s, &s->dscal, 16)) ||
+        (rc = dalloc(s, &s->e_atom, 4 * (size_t)n)) || (rc = dalloc(s, &s->dscal, 16)) ||
         (rc = dalloc(s, &s->dint, 16)) || (rc = dalloc(s, &s->dbits, 4))) {
         g_err = s->err;
         return bail(rc);
@@ -1266,6 +1271,53 @@ int mdx_potential(mdx_system* s, double* out2) {
     std::memcpy(v, s->pinned, sizeof v);
     if (s->cfg.use_lj) out2[0] = v[0];
     if (s->cfg.use_bond) out2[1] = v[1];
+    return MDX_OK;
+}
+
+int mdx_potential_virial(mdx_system* s, double* out4) {
+    if (!s || !out4) return fail(s, MDX_ERR_ARG, "null argument");
+    for (int k = 0; k < 4; k++) out4[k] = 0;
+    int rc;
+    hipStream_t st = s->stream;
+    const int32_t n = s->n;
+    // (a build permutes the state, so it must precede taking the state pointers)
+    if (s->cfg.box == MDX_BOX_PERIODIC && !s->lj.built && (rc = build_list(s, MDX_LIST_LJ, 0.0, false)))
+        return rc;
+    double* e_lj = s->e_atom;
+    double* e_b = s->e_atom + n;
+    double* w_lj = s->e_atom + 2 * (size_t)n;
+    double* w_b = s->e_atom + 3 * (size_t)n;
+    DevState scratch = s->st;
+    for (int d = 0; d < 3; d++) scratch.f[d] = s->alt.f[d];
+    if (s->cfg.use_lj) {
+        if ((rc = lj_forces(s, e_lj, nullptr, &scratch, w_lj))) return rc;
+        LAUNCH(s, "energy_sum", launch_sum(st, e_lj, n, s->partials, s->nb, s->dscal + 8));
+        LAUNCH(s, "energy_sum", launch_sum(st, w_lj, n, s->partials, s->nb, s->dscal + 10));
+    }
+    if (s->cfg.use_bond) {
+        if (is_tersoff(s)) {
+            // the force evaluation leaves its per-slot gradients behind
+            if ((rc = bond_forces(s, e_b, nullptr, &scratch))) return rc;
+            LAUNCH(s, "slot_virial", launch_slot_virial(st, s->st, s->cfg.box_length, s->ts_list, s->ts_count, n,
+                                                        s->sg_w, ts_wide(s) ? s->ts_wd : nullptr, w_b));
+            LAUNCH(s, "energy_sum", launch_sum(st, w_b, n, s->partials, s->nb, s->dscal + 11));
+        } else {
+            // the walled box is isolated: the bond-order virial is sum r_i . F_i
+            // over the bond-order atoms, from their bond forces alone
+            DevState bonly = s->st;
+            for (int d = 0; d < 3; d++) bonly.f[d] = s->alt.a[d];
+            if ((rc = bond_forces(s, e_b, nullptr, &bonly, 1))) return rc;
+            LAUNCH(s, "rf_virial", launch_rf_virial(st, s->st, bonly, s->slot_of_id, s->cfg.n_bond, w_b));
+            LAUNCH(s, "energy_sum", launch_sum(st, w_b, s->cfg.n_bond, s->partials, s->nb, s->dscal + 11));
+        }
+        LAUNCH(s, "energy_sum", launch_sum(st, e_b, s->cfg.n_bond, s->partials, s->nb, s->dscal + 9));
+    }
+    HIPCHK(s, hipMemcpyAsync(s->pinned, s->dscal + 8, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(s, hipStreamSynchronize(st));
+    double v[4];
+    std::memcpy(v, s->pinned, sizeof v);
+    if (s->cfg.use_lj) { out4[0] = v[0]; out4[2] = v[2]; }
+    if (s->cfg.use_bond) { out4[1] = v[1]; out4[3] = v[3]; }
     return MDX_OK;
 }
 

@@ -87,6 +87,24 @@ def test_batch_forces_match_oracle(precision):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", [PREC_FP64, PREC_MIXED])
+def test_batch_virials_match_oracle(precision):
+    """mdx_batch_virials: per replica the sum over pairs of r_ij . F_ij (FP64
+    terms, wave-shuffle sums) against the oracle's pair virial."""
+    ns, Ls, ps, vs = make_batch()
+    with batch(ns, Ls, precision) as b:
+        b.set_state(np.concatenate(ps), np.concatenate(vs))
+        b.run(0)
+        w = b.virials()
+        e = b.energies()
+    for r, (n, L, p, v) in enumerate(zip(ns, Ls, ps, vs)):
+        _, eo, wo = O.pbc_lj_forces(p, L, RC, oracle_list(p, L))
+        assert w[r] == pytest.approx(wo, rel=1e-10)
+        assert e[r, 1] == pytest.approx(eo, rel=1e-10)
+        assert e[r, 0] == pytest.approx(0.5 * np.sum(v ** 2), rel=1e-12)
+
+
+@pytest.mark.gpu
 def test_batch_run_matches_oracle():
     ns, Ls, ps, vs = make_batch()
     steps = 30

@@ -262,6 +262,26 @@ def test_walled_run_wall_hits_match_reference_loop(use_mb):
         np.testing.assert_array_equal(got["vel"], vo)
 
 
+def test_walled_virials_match_oracle(mixture):
+    """The walled box is isolated: its virial is sum_i r_i . F_i.  The LJ part
+    (summed pair by pair on the device) and the bond-order part (from the
+    bond-order atoms' bond forces) against that sum over the oracle's forces."""
+    pos4, vel = mixture
+    nb = 60
+    with walled(pos4, nb, PREC_FP64) as s:
+        s.set_state(pos4, vel)
+        _, box = s.ready_list()
+        s.build_list(LIST_LJ, box)
+        s.build_list(LIST_BOND, box)
+        e_lj, e_b, w_lj, w_b = s.potential_virial()
+    ref, ls = O.build_neighbor_list(pos4, 7.6, 6.8, box)
+    f_lj = O.lj_forces(pos4, ref, ls.nlargestsize, float(ls.rcutsq))
+    mb, lsb = O.build_neighbor_list(pos4[:nb], 3.6, 3.0, box)
+    f_b = O.mb_forces(pos4, mb, lsb.nlargestsize, float(lsb.rcutsq), f_in=np.zeros((len(pos4), 3)), ljexist=False)
+    assert w_lj == pytest.approx(float(np.sum(pos4[:, :3] * f_lj)), rel=1e-9)
+    assert w_b == pytest.approx(float(np.sum(pos4[:nb, :3] * f_b[:nb])), rel=1e-10)
+
+
 def test_kinetics_and_allpairs(mixture):
     pos4, vel = mixture
     nb = 60
@@ -387,6 +407,36 @@ def test_periodic_pair_lists_are_the_exact_sets(fluid, precision):
         assert s.build_count() >= 3
         assert_pair_sets(s, L, 2.8, 2.5, LJ_LIST_SLACK2)
     print(f"pair list of {n} atoms: {st}", flush=True)
+
+
+@pytest.mark.parametrize("precision", [PREC_FP64, PREC_MIXED])
+def test_periodic_virial_matches_oracle(fluid, precision):
+    """mdx_potential_virial: the LJ virial (sum over pairs of r_ij . F_ij)
+    against the oracle's, which is pinned by the scaling derivative of its
+    energy; FP64 to summation order, mixed to 1e-6."""
+    pos, vel, L = fluid
+    n = len(pos)
+    with periodic(n, L, precision) as s:
+        s.set_state(np.c_[pos, np.zeros(n)], vel)
+        s.build_list(LIST_LJ)
+        e_lj, e_b, w_lj, w_b = s.potential_virial()
+    ref, _ = O.pbc_list(pos, L, 2.8)
+    _, eo, wo = O.pbc_lj_forces(pos, L, 2.5, ref)
+    tol = 1e-12 if precision == PREC_FP64 else 1e-6
+    assert w_lj == pytest.approx(wo, rel=tol)
+    assert e_lj == pytest.approx(eo, rel=tol)
+    assert e_b == 0 and w_b == 0
+
+
+def test_periodic_virial_of_perfect_fcc():
+    """Perfect FCC: the virial per atom is the closed-form lattice sum."""
+    from test_oracle_pbc import fcc_lattice_virial
+    pos, vel, L = initial.lj_fluid(10, 0.8442, 0.728, seed=1)
+    n = len(pos)
+    with periodic(n, L, PREC_FP64) as s:
+        s.set_state(np.c_[pos, np.zeros(n)], vel)
+        w = s.potential_virial()[2]
+    assert w / n == pytest.approx(fcc_lattice_virial(L / 10, 2.5), rel=1e-12)
 
 
 def fast_atom_lattice(v0=40.0):
@@ -572,6 +622,28 @@ def test_tersoff_pair_lists_are_the_exact_sets(silicon):
         s.run(200)
         assert s.build_count() >= 2
         assert_pair_sets(s, L, 3.3, 3.0, TS_LIST_SLACK2)
+
+
+@pytest.mark.parametrize("precision", [PREC_FP64, PREC_MIXED])
+def test_tersoff_virial_is_the_scaling_derivative(silicon, precision):
+    """The many-body virial -sum_i sum_j (x_j - x_i) . de_i/d(x_j - x_i) from the
+    per-bond gradients equals -dE/dlambda of the oracle's Tersoff energy with
+    positions and box scaled by lambda (central differences): FP64 to 1e-7;
+    mixed (FP32 bond terms, whose float32-rounded parameters alone move the
+    energy by ~1e-6; the virial is a sum with strong cancellation, measured
+    1.3e-5 off) to 5e-5."""
+    pos, vel, L = silicon
+    n = len(pos)
+    with tersoff_system(n, L, precision) as s:
+        s.set_state(np.c_[pos, np.zeros(n)], vel)
+        s.build_list(LIST_BOND)
+        e_lj, e_b, w_lj, w_b = s.potential_virial()
+    h = 1e-6
+    fd = -(O.tersoff(pos * (1 + h), L * (1 + h), forces=False)[0] -
+           O.tersoff(pos * (1 - h), L * (1 - h), forces=False)[0]) / (2 * h)
+    assert w_b == pytest.approx(fd, rel=1e-7 if precision == PREC_FP64 else 5e-5)
+    assert e_b == pytest.approx(O.tersoff(pos, L, forces=False)[0], rel=1e-12 if precision == PREC_FP64 else 2e-6)
+    assert e_lj == 0 and w_lj == 0
 
 
 def test_tersoff_mixed_energy_drift(silicon):

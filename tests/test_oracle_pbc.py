@@ -264,3 +264,44 @@ def test_pair_list_check_pins():
     got = O.pair_list_check(pos, L, rs, rc, *((lambda r, c: (band(r, c), r, c)[1:])(rows.copy(), counts.copy())),
                             float(r2[i, j] - rs * rs) + 1e-9)
     assert got["in slack band"] == 1 and got["beyond rs + slack"] == 0
+
+
+def scaled_lj_energy(pos, L, lam, rc=2.5, rs=2.8):
+    p, Ls = pos * lam, L * lam
+    lst, _ = O.pbc_list(np.mod(p, Ls), Ls, rs)
+    return O.pbc_lj_forces(p, Ls, rc, lst)[1]
+
+
+def test_lj_virial_is_the_scaling_derivative():
+    """W = sum over pairs of r_ij . F_ij is -dE/dlambda for positions and box
+    scaled together by lambda (the shifted pair energy is continuous at rc),
+    taken by central differences; a virial with a wrong factor or sign fails."""
+    pos, _, L = initial.lj_fluid(6, 0.8442, 0.728, seed=3)
+    pos = np.mod(pos + np.random.default_rng(1).normal(0, 0.05, pos.shape), L)
+    lst, _ = O.pbc_list(pos, L, 2.8)
+    _, _, w = O.pbc_lj_forces(pos, L, 2.5, lst)
+    h = 1e-6
+    fd = -(scaled_lj_energy(pos, L, 1 + h) - scaled_lj_energy(pos, L, 1 - h)) / (2 * h)
+    assert w == pytest.approx(fd, rel=1e-6)
+
+
+def fcc_lattice_virial(a, rc):
+    """Per-atom virial of perfect FCC (lattice constant a) in closed form:
+    half the sum over lattice vectors within rc of r F(r), F(r) = 24 (2 r^-13
+    - r^-7), the vectors enumerated shell by shell."""
+    m = int(np.ceil(rc / a)) + 1
+    g = np.arange(-m, m + 1)
+    cube = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).astype(float)
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    v = (cube[:, None, :] + basis[None]).reshape(-1, 3) * a
+    r = np.sqrt((v ** 2).sum(1))
+    r = r[(r > 0) & (r < rc)]
+    return 0.5 * np.sum(r * 24 * (2 * r ** -13 - r ** -7))
+
+
+def test_lj_virial_of_perfect_fcc():
+    pos, _, L = initial.lj_fluid(6, 0.8442, 0.728, seed=1)
+    a = L / 6
+    lst, _ = O.pbc_list(pos, L, 2.8)
+    _, _, w = O.pbc_lj_forces(pos, L, 2.5, lst)
+    assert w / len(pos) == pytest.approx(fcc_lattice_virial(a, 2.5), rel=1e-12)
