@@ -62,18 +62,18 @@ void calculateAccelerations(real4* pos, real3* acc, real3* force, int n);
 int* buildNeighborList(real4* pos, listSettings* settings, real boxSize, int buckets, int n);
 
 /*
- * No counterpart in the reference.  The reference functions do not say which
- * list a buildNeighborList call builds or how many atoms the arrays hold;
- * hostMain knows, and passes it once before the first call:
- * lennardJonesParticles + manyBodyParticles atoms, the first
- * manyBodyParticles of them with the bond-order potential, and the two
- * listSettings objects it passes to buildNeighborList, already filled with
- * the radii (cpu/cpumd/Host.cpp:50-61).  The adapter reads the radii from those
- * objects whenever it (re)creates its device system.
+ * Nothing besides these calls is needed: which listSettings object is the
+ * Lennard-Jones one and which the bond-order one, the atom counts, and whether
+ * Lennard-Jones is on are read from the calls themselves (a list handed to
+ * lennardJonesForces or manyBodyForces names its settings' role, manyBodyForces
+ * passes ManyBodyParticles and useLennardJones, predict and correct the atom
+ * count, dt, the wall and the restitution).  A list built before its role is
+ * known is built by a list-only device system of the same atoms and radii,
+ * which gives the same rows.
+ *
+ * No counterpart in the reference: releases the device state (optional; also
+ * done at exit).
  */
-void mdxRefSetup(int lennardJonesParticles, int manyBodyParticles, bool useLennardJones, bool useManyBody,
-                 const listSettings* lennardJonesList, const listSettings* manyBodyList);
-/* releases the device state (optional; also done at exit) */
 void mdxRefRelease();
 
 #endif

@@ -2,7 +2,8 @@
 // value (cpu/cpumd/Host.cpp:78-198) written against the reference's function
 // signatures, linked with libmdx_ref.so (the engine behind those signatures).
 // The parts of hostMain that stay in the caller (readyList and the rebuild
-// schedule, Host.cpp:245-291) come from the oracle.  The GPU tests run it and
+// schedule, Host.cpp:245-291) come from the oracle.  Nothing but the
+// reference's own calls reaches the library.  The GPU tests run it and
 // compare its final state with the oracle's own loop.
 //
 // usage: ref_adapter_harness IN OUT
@@ -58,7 +59,6 @@ int main(int argc, char** argv) {
     listSettings lj{}, mb{};
     settings(lj, fl[1], fl[2]);
     settings(mb, fl[3], fl[4]);
-    mdxRefSetup(n_lj, n_mb, use_lj, use_mb, &lj, &mb);
 
     const double half = box / 2;                                  // Host.cpp:94-106 (vxcm = 0)
     for (int i = 0; i < n; i++) {

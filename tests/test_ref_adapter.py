@@ -79,3 +79,18 @@ def test_adapter_lj_bond_loop():
     assert np.abs(r[:, :3] - ro[:, :3]).max() < 1e-9
     np.testing.assert_array_equal(r[:, 3], ro[:, 3])
     assert np.sqrt(((v - vo) ** 2).sum() / (vo ** 2).sum()) < 1e-9
+
+
+@pytest.mark.gpu
+def test_adapter_bond_only_loop():
+    """Bond order only, with Lennard-Jones atoms present: the first force call
+    passes ManyBodyParticles, the atom count grows to all atoms only at
+    calculateAccelerations, and the Lennard-Jones atoms keep the forces the
+    caller's array holds (the reference assigns the first ManyBodyParticles)."""
+    pos4, vel = gas_mixture(30, 300, seed=5)
+    n, nb, steps = len(pos4), 60, 30
+    ro, vo, _, _ = O.ref_run(pos4, vel, n - nb, nb, steps, 1000, 1.0, 7.6, 6.8, 3.6, 3.0, use_lj=False, use_mb=True)
+    r, v, _ = run_harness(pos4, vel, n - nb, nb, steps, False, True)
+    assert np.abs(r[:, :3] - ro[:, :3]).max() < 1e-9
+    assert np.abs(v - vo).max() < 1e-9
+    np.testing.assert_array_equal(r[:, 3], ro[:, 3])
