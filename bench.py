@@ -8,9 +8,10 @@ Default run (what `python bench.py` measures, one JSON line):
     FP32 pair arithmetic with FP64 accumulation and FP64 Gear 5th-order
     predictor-corrector, dt = 0.005.  One step = [cell-list rebuild when
     due], LJ forces, corrector fused with the next predictor.  --steps K
-    steps are timed by the host clock around the run with no per-launch
-    events; per-kernel times come from a second pass of K steps with HIP
-    events around every launch.
+    steps (default 1000, the configuration's run length, so the list
+    rebuilds are sampled over the whole run) are timed by the host clock
+    around the run with no per-launch events; per-kernel times come from a
+    second pass of K steps with HIP events around every launch.
   * subrecords (the rest of the metric, each with its own roofline and CPU
     baseline): Tersoff silicon 262,144 atoms (32^3 diamond cells x8, 0.05 A
     displacements, 1000 K, R 2.7 A, D 0.3 A, list radius 3.3 A, dt 1 fs,
@@ -67,7 +68,7 @@ def measured_traffic(name: str) -> int | None:
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--steps", type=int, default=1000)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--ncell", type=int, default=64, help="FCC cells per edge (4 ncell^3 atoms)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")

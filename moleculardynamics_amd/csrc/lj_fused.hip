@@ -17,14 +17,15 @@
 // enough waves to cover the HBM and LDS latencies of the gathers.
 //
 //  k_nlist_lds (at each rebuild): the quad scans the atom's candidates, 32 at
-//    a time, and keeps those within the list radius rs.  Hit number k (in the
-//    quad's scan order) goes to lane k % 4 as that lane's entry k / 4, so the
-//    four lanes' lists differ in length by at most one.  Entries are 16-bit
-//    LDS indices; a lane's entries are packed 8 to a 16-byte chunk.  Atoms
-//    are grouped 16 to a block; block B holds its chunks c = 0 .. NC-1 one
-//    after the other, each as [16 atoms][4 lanes] 16-byte chunks (1 KiB), so a
-//    wave's chunk loads cover one or two contiguous runs and an atom's chunks
-//    are 1 KiB apart.
+//    a time, and keeps those within the list radius rs.  Entries are 16-bit
+//    LDS indices, numbered in the quad's scan order and grouped 32 to a row:
+//    entry k is entry k % 8 of lane (k / 8) % 4's 16-byte chunk in row k / 32.
+//    The hits go to a two-row ring in LDS first, and each lane writes its
+//    16-byte chunk of a row once the row is complete (the last row padded
+//    with the dummy atom).  Atoms are grouped 16 to a block; block B holds its
+//    rows c = 0 .. NC-1 one after the other, each as [16 atoms][4 lanes]
+//    16-byte chunks (1 KiB), so a wave's chunk loads cover one or two
+//    contiguous runs and an atom's rows are 1 KiB apart.
 //  k_step_lds (every step): LJ force over the list (FP64 displacement, FP32
 //    12-6 force law in the mixed mode, FP64 accumulation), summed over the
 //    quad, then the Gear corrector of step t and the predictor of step t+1,
@@ -48,6 +49,8 @@ constexpr int kScan = 8;                   // candidates per lane per scan step
 constexpr int kScanSlack = kLanes * kScan; // staged slots past the last atom the scan may read
 constexpr int kListBlock = 16;             // atoms per list block
 constexpr int kChunkRow = kListBlock * kLanes * kChunk;   // 16-bit entries in one chunk of a block (512)
+constexpr int kRowEntries = kLanes * kChunk;               // entries in one row of an atom (32)
+constexpr int kRing = 2 * kRowEntries;                     // the list build's LDS ring per quad (two rows)
 constexpr int kXcd = 8;
 constexpr int kLdsAtoms = 3072;            // staged atoms per segment at most (3 x 24 KiB of FP64)
 constexpr int kMaxK = 14;                  // cells per segment
@@ -298,6 +301,12 @@ __device__ inline int32_t home_cell(const Stage& S, int32_t qc) {
     return o;
 }
 
+// LDS bytes ahead of the list build's row rings: the FP32 staging of lds_cap
+// atoms and the scan's slack, rounded up to 16 bytes
+__host__ __device__ constexpr int32_t ring_offset(int32_t lds_cap) {
+    return (3 * (lds_cap + kScanSlack) * (int32_t)sizeof(float) + 15) / 16 * 16;
+}
+
 // the lists of one segment's home atoms (the whole block of kB threads)
 template <int kB>
 __device__ __forceinline__ void nlist_segment(Stage& S, double* dyn, const SegGrid& g, int32_t cx, int32_t cy,
@@ -306,7 +315,8 @@ __device__ __forceinline__ void nlist_segment(Stage& S, double* dyn, const SegGr
                                               double rl2, int32_t cap, uint16_t* __restrict__ list,
                                               int32_t* __restrict__ count, int32_t* __restrict__ err,
                                               int32_t lds_cap) {
-    const int32_t nchunk = cap / (kLanes * kChunk) + 1;   // chunks per lane with the spare
+    const int32_t nchunk = cap / kRowEntries + 1;   // rows per atom with the spare
+    const int32_t rows = cap / kRowEntries;         // rows that hold entries
     const PosF P{reinterpret_cast<float*>(dyn), lds_cap};
     if (!stage<kB>(S, P, g, cx, cy, z0, z1, rx, ry, rz, cx * (g.L / g.n[0]), cy * (g.L / g.n[1]), z0 * (g.L / g.n[2]))) {
         if (threadIdx.x == 0) atomicMax(&err[0], S.total + 1);
@@ -318,6 +328,10 @@ __device__ __forceinline__ void nlist_segment(Stage& S, double* dyn, const SegGr
     const int32_t cbase = (cx * g.n[1] + cy) * g.n[2];
     const int32_t hbeg = g.start[cbase + z0], hend = g.start[cbase + z1];
     const int32_t q = threadIdx.x & (kLanes - 1);
+    // this quad's two-row ring; lane q's chunk of a row is 16 bytes at entry 8q
+    uint16_t* const ring = reinterpret_cast<uint16_t*>(reinterpret_cast<char*>(dyn) + ring_offset(lds_cap)) +
+                           (threadIdx.x / kLanes) * kRing;
+    const uint16_t dummy = (uint16_t)S.total;
     for (int32_t base = hbeg; base < hend; base += kB / kLanes) {
         const int32_t s = base + threadIdx.x / kLanes;
         if (s >= hend) continue;             // whole quads drop out together
@@ -325,9 +339,19 @@ __device__ __forceinline__ void nlist_segment(Stage& S, double* dyn, const SegGr
         const int32_t o = home_cell(S, qc);
         const int32_t self = S.colbase[4] + S.cellpre[4][1] + qc;
         const float xi = P.x(self), yi = P.y(self), zi = P.z(self);
-        // entry kk of atom s: lane kk % 4, that lane's entry jj = kk / 4, i.e.
-        // chunk jj / 8 (= kk >> 5), place jj % 8 of the lane's 8
-        uint16_t* const la = list + ((int64_t)(s / kListBlock) * nchunk * kListBlock + s % kListBlock) * (kLanes * kChunk);
+        // lane q's chunk of row r of atom s
+        uint4* const la = reinterpret_cast<uint4*>(list + ((int64_t)(s / kListBlock) * nchunk * kListBlock + s % kListBlock) *
+                                                              kRowEntries) + q;
+        // row r of the ring to lane q's chunk in the list (rows past the
+        // capacity are dropped; the count reports the overflow).  The ring was
+        // written by the quad's lanes: LDS operations of one wavefront take
+        // effect in program order, and the asm keeps the compiler from moving
+        // the read above those writes.
+        auto flush = [&](int32_t r) {
+            asm volatile("" ::: "memory");
+            if (r < rows) la[(int64_t)r * (kChunkRow / kChunk)] = *reinterpret_cast<const uint4*>(ring + (r & 1) * kRowEntries + q * kChunk);
+            asm volatile("" ::: "memory");
+        };
         int32_t k = 0;                        // hits so far, the same in all 4 lanes
         for (int32_t c = 0; c < 9; c++) {
             const int32_t b = S.colbase[c] + S.cellpre[c][o - 1], e = S.colbase[c] + S.cellpre[c][o + 2];
@@ -354,21 +378,24 @@ __device__ __forceinline__ void nlist_segment(Stage& S, double* dyn, const SegGr
                     mask |= (uint32_t)(r2 < rlf) << u;
                 }
                 // candidate u is p0 + kLanes u + q: keep u below the range end,
-                // drop the atom itself
-                const int32_t left = e - p0 - q, ds = self - p0 - q;
-                mask &= (1u << (left <= 0 ? 0 : min((left + kLanes - 1) / kLanes, kScan))) - 1u;
-                if (ds >= 0 && ds < kLanes * kScan && ds % kLanes == 0) mask &= ~(1u << (ds / kLanes));
-                // hits are numbered lane-major within the quad
+                // and in the home column (c = 4, the same for the whole
+                // wavefront) drop the atom itself
+                const int32_t nv = min(max((e - p0 - q + kLanes - 1) >> 2, 0), kScan);
+                mask &= (1u << nv) - 1u;
+                if (c == 4) {
+                    const int32_t ds = self - p0 - q;
+                    if (ds >= 0 && ds < kLanes * kScan && ds % kLanes == 0) mask &= ~(1u << (ds / kLanes));
+                }
+                // hits are numbered lane-major within the quad; at most 32 of
+                // them, so they complete at most one row
                 const int32_t h = __popc(mask);
                 int32_t total;
                 int32_t kk = k + quad_excl(h, q, total);
-                // store the set bits only (about 1.5 of 8 per lane)
-                for (uint32_t mb = mask; mb; mb &= mb - 1, kk++) {
-                    const int32_t u = __builtin_ctz(mb);
-                    if (kk < cap)
-                        la[(kk >> 5) * kChunkRow + (((kk & 3) << 3) | ((kk >> 2) & 7))] = (uint16_t)(p0 + kLanes * u + q);
-                }
-                k += total;
+                for (uint32_t mb = mask; mb; mb &= mb - 1, kk++)
+                    ring[kk & (kRing - 1)] = (uint16_t)(p0 + kLanes * __builtin_ctz(mb) + q);
+                const int32_t kn = k + total;
+                if ((kn ^ k) >= kRowEntries) flush(k / kRowEntries);   // row k / 32 is complete
+                k = kn;
             }
         }
         const int32_t kc = k < cap ? k : cap;
@@ -376,11 +403,16 @@ __device__ __forceinline__ void nlist_segment(Stage& S, double* dyn, const SegGr
             count[s] = kc;
             if (k > cap) atomicMax(&err[1], k);
         }
-        // pad the rest of this lane's last chunk with the dummy atom, so the
-        // step kernel evaluates whole chunks without a validity test
-        const int32_t mq = (kc - q + kLanes - 1) / kLanes;
-        const int32_t mend = (mq + kChunk - 1) / kChunk * kChunk;
-        for (int32_t jj = mq; jj < mend; jj++) la[(jj / kChunk) * kChunkRow + q * kChunk + jj % kChunk] = (uint16_t)S.total;
+        // the last row: the dummy atom after the entries, so the step kernel
+        // evaluates whole chunks without a validity test
+        if (k % kRowEntries != 0) {
+            const int32_t r = k / kRowEntries, used = k % kRowEntries;
+            uint16_t* const row = ring + (r & 1) * kRowEntries + q * kChunk;
+#pragma unroll
+            for (int u = 0; u < kChunk; u++)
+                if (q * kChunk + u >= used) row[u] = dummy;
+            flush(r);
+        }
     }
 }
 
@@ -437,7 +469,7 @@ k_nlist_decode(SegGrid g, RepGeo rg, const uint16_t* __restrict__ list, const in
             continue;
         }
         for (int32_t kk = 0; kk < c; kk++) {
-            const int32_t w = la[(kk >> 5) * kChunkRow + (((kk & 3) << 3) | ((kk >> 2) & 7))];
+            const int32_t w = la[(kk / kRowEntries) * kChunkRow + kk % kRowEntries];
             // an index past the staged atoms (the padding's dummy) inside the count would be a bug: -2
             rows[(int64_t)me * width + kk] = w < S.total ? id[staged_slot(S, w)] : -2;
         }
@@ -527,14 +559,14 @@ k_step_lds(StepArgs a) {
         const double xi = P.x(self), yi = P.y(self), zi = P.z(self);
         double fx = 0, fy = 0, fz = 0;
         const bool first = base == hbeg;
-        const int32_t m = ((first ? count_first : a.count[i]) - q + kLanes - 1) / kLanes;   // this lane's entries
-        // chunks of this lane are 16 x 4 uint4 apart; the next chunk is loaded
-        // while the current one is evaluated (the list has one spare chunk per
-        // lane, so the look-ahead stays in bounds); the last chunk is padded
-        // with the dummy atom
+        const int32_t m = first ? count_first : a.count[i];   // the atom's entries, 32 to a row
+        // lane q takes its 16-byte chunk of every row; rows are 16 x 4 uint4
+        // apart; the next row's chunk is loaded while the current one is
+        // evaluated (the list has one spare row, so the look-ahead stays in
+        // bounds); the last row is padded with the dummy atom
         const uint4* lp = list_of(i);
         uint4 nx = first ? chunk_first : lp[0];
-        for (int32_t k0 = 0; k0 < m; k0 += kChunk) {
+        for (int32_t k0 = 0; k0 < m; k0 += kRowEntries) {
             const uint4 cu = nx;
             lp += cstride;
             // the empty asm hides that this address is the next iteration's
@@ -661,7 +693,10 @@ SegGrid make_grid(const GridView& v, int32_t nseg) {
 
 template <typename K>
 void allow_lds(K kernel) {
-    // dynamic LDS above the 64 KiB default, up to the largest staging
+    // dynamic LDS above the 64 KiB default, up to the largest staging (the
+    // step kernel's FP64 one; the list build's FP32 staging and row rings
+    // stay below it)
+    static_assert(ring_offset(kLdsAtoms) + 1024 / kLanes * kRing * 2 <= 3 * kLdsAtoms * 8, "list build LDS");
     (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                               3 * kLdsAtoms * (int)sizeof(double));
 }
@@ -702,7 +737,7 @@ hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v
     }();
     (void)attr_set;
     SegGrid g = make_grid(v, nseg);
-    const size_t dyn = 3 * ((size_t)lds_atoms + kScanSlack) * sizeof(float);
+    const size_t dyn = ring_offset(lds_atoms) + (size_t)(block / kLanes) * kRing * sizeof(uint16_t);
     if (block == 1024)
         k_nlist_lds<false, 1024><<<g.per_xcd * kXcd, 1024, dyn, s>>>(g, RepGeo{}, st.r[0], st.r[1], st.r[2], rl2,
                                                                      cap, list, count, err, lds_atoms);
@@ -806,7 +841,7 @@ hipError_t launch_rep_nlist(hipStream_t s, const RepSegs& rs, const int32_t* wl,
     if (lds_atoms < 1 || lds_atoms > kLdsAtoms) return hipErrorInvalidValue;
     static const bool attr_set = (allow_lds(k_nlist_lds<true, kBlock>), true);  // once per process, thread-safe
     (void)attr_set;
-    const size_t dyn = 3 * ((size_t)lds_atoms + kScanSlack) * sizeof(float);
+    const size_t dyn = ring_offset(lds_atoms) + (size_t)(kBlock / kLanes) * kRing * sizeof(uint16_t);
     // a grid of resident blocks (4 a CU at the full staging), a multiple of 8
     const int32_t all = (rs.nblocks + kXcd - 1) / kXcd * kXcd;
     const int32_t grid = all < 1024 ? all : 1024;
