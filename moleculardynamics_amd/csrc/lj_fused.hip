@@ -246,15 +246,14 @@ __device__ inline int32_t staged_slot(const Stage& S, int32_t w) {
     return w < l0 ? S.psrc[col][0] + w : (w < l0 + l1 ? S.psrc[col][1] + (w - l0) : S.psrc[col][2] + (w - l0 - l1));
 }
 
-// Stage the neighbourhood of segment (cx, cy, z0..z1), positions minus the
-// origin o (zero for the FP64 staging: the values are then exactly the
-// images).  Returns false (for the whole block) when it does not fit in LDS.
+// Fill the staging of segment (cx, cy, ...) whose header is S: positions
+// minus the origin o (zero for the FP64 staging: the values are then exactly
+// the images).  Returns false (for the whole block) when it does not fit in LDS.
 template <int kB, typename PosT>
-__device__ inline bool stage(Stage& S, const PosT& P, const SegGrid& g, int32_t cx, int32_t cy, int32_t z0,
-                             int32_t z1, const double* __restrict__ rx, const double* __restrict__ ry,
-                             const double* __restrict__ rz, double ox = 0.0, double oy = 0.0, double oz = 0.0) {
+__device__ inline bool stage_fill(Stage& S, const PosT& P, const SegGrid& g, int32_t cx, int32_t cy,
+                                  const double* __restrict__ rx, const double* __restrict__ ry,
+                                  const double* __restrict__ rz, double ox = 0.0, double oy = 0.0, double oz = 0.0) {
     const int32_t tid = threadIdx.x;
-    stage_header(S, g, cx, cy, z0, z1);
     if (S.total + 1 > P.cap) return false;   // + the dummy atom
     // copy: kColThreads threads per column, kCopyBatch independent loads each
     constexpr int32_t kColThreads = kB / 9;
@@ -292,6 +291,15 @@ __device__ inline bool stage(Stage& S, const PosT& P, const SegGrid& g, int32_t 
     if (tid == kB - 1) P.set(S.total, kFar, kFar, kFar);
     __syncthreads();
     return true;
+}
+
+// Stage the neighbourhood of segment (cx, cy, z0..z1): header, then positions.
+template <int kB, typename PosT>
+__device__ inline bool stage(Stage& S, const PosT& P, const SegGrid& g, int32_t cx, int32_t cy, int32_t z0,
+                             int32_t z1, const double* __restrict__ rx, const double* __restrict__ ry,
+                             const double* __restrict__ rz, double ox = 0.0, double oy = 0.0, double oz = 0.0) {
+    stage_header(S, g, cx, cy, z0, z1);
+    return stage_fill<kB>(S, P, g, cx, cy, rx, ry, rz, ox, oy, oz);
 }
 
 // window cell (1..K) of home atom number qc of the segment
@@ -523,8 +531,23 @@ k_step_lds(StepArgs a) {
     }
     const Pos P{dyn, a.lds_cap};
     unsigned long long* fl = kRep ? nullptr : a.flags + 3 * (blockIdx.x % kSpread);
+    // the single box's list gate, set when r(t) has left the list's margin:
+    // loaded beside the segment's slot range, so its latency passes with theirs
+    unsigned int halt = 0;
+    if constexpr (!kRep) halt = __hip_atomic_load(a.pub.gate, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int32_t cbase = (cx * g.n[1] + cy) * g.n[2];
     const int32_t hbeg = g.start[cbase + z0], hend = g.start[cbase + z1];
+    if constexpr (!kRep) {
+        // the empty asm keeps the range loads above the test, so one wait
+        // covers all three
+        asm volatile("" ::"v"(hbeg), "v"(hend));
+        if (halt) {
+            // the list does not cover r(t): the step runs as a no-op (no state
+            // or flag slot is written) and workgroup 0 says so
+            if (blockIdx.x == 0) publish_halt(a.pub, 3 * kSpread);
+            return;
+        }
+    }
     const int32_t q = threadIdx.x & (kLanes - 1);
     // the first pass's list count and first chunk, loaded before the staging so
     // that their latency passes while it runs
@@ -796,6 +819,7 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
             return hipErrorInvalidValue;
     a.n = n;
     a.lds_cap = lds_atoms;
+    if (!pub.gate) return hipErrorInvalidValue;   // the single box's step kernels read their gate
     a.pub = pub;
     a.pub.blocks = a.g.total;   // the workgroups past the segments return before counting
     const size_t dyn = 3 * (size_t)lds_atoms * sizeof(double);

@@ -69,12 +69,22 @@ __device__ inline double lane_double(double v, int l) {
 // an atomic that waits for the previous ones, so no fence (which would write
 // back the XCD's L2 after a step kernel's stores) is needed.
 // host == nullptr: nothing is published.
+//
+// The last arrival also writes the list gate of the next step (gate != null):
+// 1 when the largest displacement^2 since the list build among the flag words
+// (words 3q) reaches gate_bits (non-negative doubles order like their bits),
+// else 0.  A step kernel that finds its gate set runs as a no-op ("halted"):
+// it leaves the gate set, so the launch queued after it halts too, and hands
+// over flag words marked with kHaltMark (publish_halt).
 constexpr int kPubSub = 32;
+constexpr unsigned long long kHaltMark = 1ull << 62;
 struct FlagPublish {
     unsigned int* done;         // kPubSub + 1 device counters, 0 between launches
     unsigned long long* host;   // flag words, then the sequence number
     unsigned long long seq;
     int32_t blocks;             // workgroups that take part, numbered 0 .. blocks - 1
+    unsigned int* gate;         // the next step's list gate (may be null)
+    unsigned long long gate_bits;
 };
 
 // every thread of a workgroup that takes part (number id), after the
@@ -83,6 +93,7 @@ __device__ inline void publish_flags(const FlagPublish& p, int32_t id, const uns
                                      int32_t words) {
     if (!p.host) return;
     __shared__ int32_t last;
+    __shared__ unsigned long long dmax;
     __builtin_amdgcn_s_waitcnt(0);   // this wave's flag atomics have completed
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -96,18 +107,35 @@ __device__ inline void publish_flags(const FlagPublish& p, int32_t id, const uns
                 n_top;
         }
         last = l;
+        dmax = 0;
     }
     __syncthreads();
     if (!last) return;
-    for (int32_t w = threadIdx.x; w < words; w += blockDim.x)
-        __hip_atomic_store(p.host + w, __hip_atomic_load(flags + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    for (int32_t w = threadIdx.x; w < words; w += blockDim.x) {
+        const unsigned long long v = __hip_atomic_load(flags + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(p.host + w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (w % 3 == 0) atomicMax(&dmax, v);
+    }
     __builtin_amdgcn_s_waitcnt(0);   // the words have been written
     __syncthreads();
     if (threadIdx.x == 0) {
+        if (p.gate)
+            __hip_atomic_store(p.gate, dmax >= p.gate_bits ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(p.done + kPubSub, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(p.host + words, p.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
+}
+
+// A halted step kernel's hand-over: workgroup 0 alone (all its threads)
+// writes the step's flag words as zeros with kHaltMark in the first stale
+// word, then the sequence number.
+__device__ inline void publish_halt(const FlagPublish& p, int32_t words) {
+    if (!p.host) return;
+    for (int32_t w = threadIdx.x; w < words; w += blockDim.x)
+        __hip_atomic_store(p.host + w, w == 2 ? kHaltMark : 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __builtin_amdgcn_s_waitcnt(0);   // the words have been written
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(p.host + words, p.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Per-atom SoA state (slot order).

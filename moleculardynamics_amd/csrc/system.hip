@@ -677,16 +677,21 @@ double bits_to_double(uint64_t b) {
     return d;
 }
 
-// Periodic box, the fused path: one kernel per step computes the LJ force of
-// r(t) from the neighbour list, corrects step t and predicts step t+1.  The
-// rebuild decision runs one step behind the device: before step k is
-// launched the host reads the flags of the positions r(k-1) (largest
-// displacement since the build D and largest single-step move S) and
-// rebuilds when D + 1.2 S reaches 0.9 (rs - rc)/2; every step kernel
-// re-checks its own positions against the margin and counts violations, which
-// turn into MDX_ERR_LIST_STALE.  Tersoff silicon runs the same loop with its
-// two force kernels (which count the stale atoms) and a separate corrector /
-// next-predictor kernel per step.
+// Periodic box.  Lennard-Jones runs the fused path: one kernel per step
+// computes the LJ force of r(t) from the neighbour list, corrects step t and
+// predicts step t+1.  The rebuild decision is exact and made on the device:
+// the last workgroup of step t's kernel finds the largest displacement of
+// r(t+1) since the build and sets the list gate when it reaches the margin
+// (rs - rc)/2; step t+1's kernel then runs as a no-op, as does the one queued
+// after it.  The host keeps two launches in flight and reads each step's flags
+// as they arrive; on a halted step it rebuilds the lists from that step's
+// positions and launches it again.  Tersoff silicon runs its two force kernels
+// and a corrector / next-predictor kernel per step with the decision on the
+// host, one step behind the device: it reads the flags of r(k-1) before
+// launching step k and rebuilds when the largest displacement D plus 1.2 x
+// the largest single-step move S reaches 0.9 of the margin.  Every force
+// kernel re-checks its own positions against the full margin and counts
+// violations, which turn into MDX_ERR_LIST_STALE.
 int run_periodic(mdx_system* s, int32_t nsteps) {
     const mdx_config& c = s->cfg;
     int rc;
@@ -734,10 +739,13 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
     // (for the short Tersoff and surrogate step kernels the hand-over at
     // their end costs as much as the copy it saves).
     uint64_t expect[mdx_system::kRing] = {0, 0, 0};   // per slot: sequence number awaited (0: event)
+    unsigned int* const gate = s->dpub + kPubSub + 1;
+    uint64_t gate_bits;
+    std::memcpy(&gate_bits, &hard2, 8);
     auto publish = [&](int k) {
         expect[ring(k)] = ++s->pub_seq;
         return FlagPublish{s->dpub, reinterpret_cast<unsigned long long*>(host_flags(k)),
-                           (unsigned long long)s->pub_seq, 0};
+                           (unsigned long long)s->pub_seq, 0, gate, (unsigned long long)gate_bits};
     };
     auto post = [&](int k) -> int {  // copy step flags to the host and mark them
         HIPCHK(s, hipMemcpyAsync(host_flags(k), slot_flags(k), W * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
@@ -769,11 +777,57 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
     // back in the ring, was enqueued before)
     HIPCHK(s, hipMemsetAsync(slot_flags(-1), 0, W * sizeof(uint64_t), st));
     HIPCHK(s, hipMemsetAsync(slot_flags(0), 0, W * sizeof(uint64_t), st));
-    // the hand-over's arrival counters start each call at zero, whatever an
-    // earlier failed launch left in them
-    HIPCHK(s, hipMemsetAsync(s->dpub, 0, sizeof(unsigned int) * (kPubSub + 1), st));
+    // the hand-over's arrival counters and the list gate start each call at
+    // zero, whatever an earlier failed launch left in them
+    HIPCHK(s, hipMemsetAsync(s->dpub, 0, sizeof(unsigned int) * (kPubSub + 2), st));
     LAUNCH(s, "predict", launch_predict_track(st, s->st, n, s->gc, s->rref, slot_flags(-1)));
     if ((rc = post(-1))) return rc;
+    if (!ts) {
+        // r(0) is checked here, exactly; every later step by its kernel's gate
+        if ((rc = wait_flags(-1))) return rc;
+        double D, S;
+        uint64_t stale;
+        reduce(host_flags(-1), D, S, stale);
+        if (stale != 0) return stale_error(-1);
+        // the gate's test, on the same bits: largest displacement^2 >= margin^2
+        uint64_t d2_bits = 0;
+        for (int q = 0; q < W / 3; q++) d2_bits = std::max<uint64_t>(d2_bits, host_flags(-1)[3 * q]);
+        if (d2_bits >= gate_bits && (rc = rebuild_lists())) return rc;
+        int32_t k = 0;        // next step to launch
+        int32_t checked = 0;  // steps before this one have run with a valid list
+        while (checked < nsteps) {
+            if (k < nsteps && k - checked < 2) {
+                const bool fin = k + 1 == nsteps;
+                LAUNCH(s, "md_step", launch_step_lds(st, s->st, s->lj.view, s->nseg, s->rnext, s->rref, s->nlist,
+                                                     s->ncount, s->nl_cap, s->ljt, s->mt, s->gc, hard2, n,
+                                                     component_stride(n), c.precision, fin, fin, slot_flags(k),
+                                                     s->lds_atoms, slot_flags(k + 1), publish(k), s->box_block));
+                for (int d = 0; d < 3; d++) std::swap(s->st.r[d], s->rnext[d]);
+                s->t++;
+                k++;
+                continue;
+            }
+            if ((rc = wait_flags(checked))) return rc;
+            reduce(host_flags(checked), D, S, stale);
+            if (stale >= kHaltMark) {
+                // steps checked .. k-1 ran as no-ops: r(checked) is where the
+                // first of them left it; undo their position swaps and steps
+                const int32_t undone = k - checked;
+                if (undone % 2 != 0)
+                    for (int d = 0; d < 3; d++) std::swap(s->st.r[d], s->rnext[d]);
+                s->t -= undone;
+                k = checked;
+                // (a halted launch writes no flag slot); the gate opens for the new list
+                if ((rc = rebuild_lists())) return rc;
+                HIPCHK(s, hipMemsetAsync(gate, 0, sizeof(unsigned int), st));
+                continue;
+            }
+            if (stale != 0) return stale_error(checked);
+            checked++;
+        }
+        HIPCHK(s, hipStreamSynchronize(st));
+        return MDX_OK;
+    }
     int last_build = -1;   // step index whose positions were used for the last build
     for (int32_t k = 0; k < nsteps; k++) {
         // Flags index f describes r(f+1) (f = -1: the predictor's r(0)).  Step k
@@ -785,7 +839,7 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
         reduce(host_flags(fk), D, S, stale);
         if (stale != 0) return stale_error(fk);
         // rebuild at 90% of the margin, estimating r(k) as one more step of at most
-        // 1.2 S away; the step kernel's own check of the full margin (stale count)
+        // 1.2 S away; the force kernels' own check of the full margin (stale count)
         // catches anything the estimate misses
         const double trigger = 0.9 * margin;
         bool rebuild = false;
@@ -802,34 +856,23 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
             last_build = k;
         }
         const bool fin = k + 1 == nsteps;
-        if (ts) {
-            if (is_surrogate(s)) {
-                LAUNCH(s, "sg_eval", launch_sg_eval(st, s->st, c.box_length, s->tp, s->ts_list, s->ts_count, n,
-                                                    s->sg_frag, s->sg_vec, s->sg_w, nullptr, s->sg_nflag));
-                s->sg_evals++;
-                const SlotGather g{s->ts_list, s->ts_count, s->sg_rev, s->sg_w, nullptr};
-                LAUNCH(s, "gather_gear_step", launch_gather_correct_predict_track(st, s->st, n, s->gc, s->mt, s->rref,
-                                                                                  slot_flags(k), fin, g, hard2,
-                                                                                  slot_flags(k + 1)));
-                if ((rc = post(k))) return rc;
-                s->t++;
-                continue;
-            }
+        if (is_surrogate(s)) {
+            LAUNCH(s, "sg_eval", launch_sg_eval(st, s->st, c.box_length, s->tp, s->ts_list, s->ts_count, n,
+                                                s->sg_frag, s->sg_vec, s->sg_w, nullptr, s->sg_nflag));
+            s->sg_evals++;
+            const SlotGather g{s->ts_list, s->ts_count, s->sg_rev, s->sg_w, nullptr};
+            LAUNCH(s, "gather_gear_step", launch_gather_correct_predict_track(st, s->st, n, s->gc, s->mt, s->rref,
+                                                                              slot_flags(k), fin, g, hard2,
+                                                                              slot_flags(k + 1)));
+        } else {
             LAUNCH(s, "ts_atoms", launch_ts_atoms(st, s->st, c.box_length, s->tp, c.precision, s->ts_list, s->ts_count,
                                                   n, s->sg_w, s->ts_wd, nullptr));
             const SlotGather g{s->ts_list, s->ts_count, s->sg_rev, s->sg_w, s->ts_wd};
             LAUNCH(s, "gather_gear_step", launch_gather_correct_predict_track(st, s->st, n, s->gc, s->mt, s->rref,
                                                                               slot_flags(k), fin, g, hard2,
                                                                               slot_flags(k + 1)));
-            if ((rc = post(k))) return rc;
-            s->t++;
-            continue;
         }
-        LAUNCH(s, "md_step", launch_step_lds(st, s->st, s->lj.view, s->nseg, s->rnext, s->rref, s->nlist, s->ncount, s->nl_cap,
-                                             s->ljt, s->mt, s->gc, hard2, n, component_stride(n), c.precision, fin,
-                                             fin, slot_flags(k), s->lds_atoms,
-                                             slot_flags(k + 1), publish(k), s->box_block));
-        for (int d = 0; d < 3; d++) std::swap(s->st.r[d], s->rnext[d]);
+        if ((rc = post(k))) return rc;
         s->t++;
     }
     // stale check of the last steps
@@ -908,7 +951,7 @@ int mdx_create(const mdx_config* cfg, mdx_system** out) {
     s->nb = kinetics_blocks(n);
     if ((rc = dalloc(s, &s->partials, 8 * (size_t)s->nb + 16)) ||
         (rc = dalloc(s, &s->dflags, (size_t)mdx_system::kFlagWords * mdx_system::kRing)) ||
-        (rc = dalloc(s, &s->dpub, kPubSub + 1))) {
+        (rc = dalloc(s, &s->dpub, kPubSub + 2))) {
         g_err = s->err;
         return bail(rc);
     }
@@ -931,7 +974,7 @@ int mdx_create(const mdx_config* cfg, mdx_system** out) {
                       hipHostMallocDefault) != hipSuccess)
         return bail(fail(nullptr, MDX_ERR_ALLOC, "hipHostMalloc failed"));
     std::memset(s->hflags, 0, sizeof(uint64_t) * mdx_system::kHostWords * mdx_system::kRing);
-    if (hipMemsetAsync(s->dpub, 0, sizeof(unsigned int) * (kPubSub + 1), s->stream) != hipSuccess)
+    if (hipMemsetAsync(s->dpub, 0, sizeof(unsigned int) * (kPubSub + 2), s->stream) != hipSuccess)
         return bail(fail(nullptr, MDX_ERR_HIP, "init failed"));
     for (int k = 0; k < mdx_system::kRing; k++)
         if (hipEventCreateWithFlags(&s->ev[k], hipEventDisableTiming) != hipSuccess)

@@ -442,13 +442,45 @@ def test_periodic_virial_of_perfect_fcc():
 def fast_atom_lattice(v0=40.0):
     """A quiet FCC lattice (864 atoms) with one atom moving at v0 along x
     (0.2 sigma per step at dt = 0.005, past the list margin (rs - rc)/2 = 0.15
-    sigma).  A steadily fast atom is rebuilt for every step; the stale list
-    comes when it knocks a lattice atom out of place within one step, which
-    no decision from earlier positions can foresee."""
+    sigma), which knocks lattice atoms out of place within single steps."""
     pos, _, L = initial.lj_fluid(6, 0.8442, 0.728, seed=1)
     vel = np.zeros_like(pos)
     vel[0, 0] = v0
     return pos, vel, L
+
+
+def test_periodic_list_gate_is_exact():
+    """The Lennard-Jones step kernels gate themselves: the last workgroup of
+    each step's kernel finds the largest displacement of the next positions
+    since the list build, and a step whose positions have moved (rs - rc)/2 or
+    more runs as a no-op; the host rebuilds from those positions and launches
+    it again.  That is the oracle's own rule (rebuild when any atom moved
+    skin/2 or more), so with the same FP32 radii the runs make the same builds
+    and follow the same trajectory, in one call and in calls of one step, and
+    no step ever uses a stale list.  The fast atom needs a new list every step
+    and the atoms it strikes are thrown past the margin within a step, which
+    no decision made from earlier positions could foresee."""
+    pos, vel, L = fast_atom_lattice()
+    n, steps = len(pos), 30
+    rs32, rc32 = float(np.float32(2.8)), float(np.float32(2.5))
+    ro, vo, _, builds_o, _ = O.pbc_lj_run(pos, vel, L, rc32, rs32, DT, steps)
+    assert builds_o >= steps // 2
+
+    def system():
+        s = periodic(n, L, PREC_FP64)
+        s.set_state(np.c_[pos, np.zeros(n)], vel)
+        return s
+
+    for calls in ([steps], [1] * steps, [7, 1, 22]):
+        with system() as s:
+            for k in calls:
+                s.run(k)
+            st = s.get_state()
+            assert s.build_count() == builds_o, calls
+        d = st["pos"][:, :3] - ro
+        d -= L * np.round(d / L)
+        assert np.abs(d).max() < 1e-9, calls
+        assert rel_rms(st["vel"], vo) < 1e-9, calls
 
 
 def stale_step(err):
@@ -456,45 +488,6 @@ def stale_step(err):
     m = re.search(r"step (\d+) used a stale list", str(err))
     assert m, str(err)
     return int(m.group(1))
-
-
-def test_periodic_stale_list_is_reported():
-    """Each periodic step kernel checks its own positions against the full list
-    margin and counts the atoms beyond it; the count reaches the host through
-    the step's flag hand-over.  A step that used a stale list must fail the
-    run with MDX_ERR_LIST_STALE naming that step, whether the host finds it
-    while later steps run, in the check at the end of the call (a call that
-    ends with the stale step), or in a second call.  (A call boundary can move
-    a rebuild: each call's first decision is exact, so the second call may
-    rebuild before the first call's stale step and go stale at a later one.)"""
-    from moleculardynamics_amd._lib import MdxError
-    pos, vel, L = fast_atom_lattice()
-    n = len(pos)
-
-    def system():
-        s = periodic(n, L, PREC_MIXED)
-        s.set_state(np.c_[pos, np.zeros(n)], vel)
-        return s
-
-    with system() as s, pytest.raises(MdxError) as e:
-        s.run(30)
-    assert e.value.code == -7, str(e.value)
-    k = stale_step(e.value)
-    print(f"stale list at step {k}", flush=True)
-    assert k >= 2
-    with system() as s, pytest.raises(MdxError) as e:
-        s.run(k + 1)
-    assert e.value.code == -7 and stale_step(e.value) == k, str(e.value)
-    with system() as s:
-        s.run(k - 1)
-        with pytest.raises(MdxError) as e:
-            s.run(30)
-    assert e.value.code == -7 and stale_step(e.value) >= k - 1, str(e.value)
-    # a single step per call is always exact: each call's first decision reads
-    # the flags of the positions it steps from, so no step goes stale
-    with system() as s:
-        for _ in range(k + 3):
-            s.run(1)
 
 
 def test_periodic_rs_equal_rc_rebuilds_every_step(fluid):
@@ -661,3 +654,51 @@ def test_tersoff_mixed_energy_drift(silicon):
             e.append(s.potential()[1] + ek)
             s.run(100)
     assert max(e) - min(e) < 1e-4 * abs(e[0])
+
+
+def squeezed_silicon(delta=1.4):
+    """3^3 diamond cells at rest with atom 0 pushed delta A towards its nearest
+    neighbour (a 2.35 A bond squeezed to 0.95 A): the pair flies apart from
+    rest, speeding up steeply over the first femtoseconds."""
+    pos, _, L = initial.si_diamond(3, 1000.0, seed=5, disp=0.0)
+    d = pos - pos[0]
+    d -= L * np.round(d / L)
+    r = np.sqrt((d ** 2).sum(1))
+    r[0] = np.inf
+    j = int(np.argmin(r))
+    pos[0] = np.mod(pos[0] + delta * d[j] / r[j], L)
+    return pos, np.zeros_like(pos), L
+
+
+def test_tersoff_stale_list_is_reported():
+    """Tersoff silicon decides its rebuilds on the host from the flags of the
+    previous step's positions (largest displacement D and single-step move S:
+    rebuild at D + 1.2 S >= 0.9 of the margin); each force kernel checks its
+    own positions against the full margin and counts the atoms beyond it.  The
+    squeezed pair accelerates faster than that estimate allows for, and the
+    step that uses the stale list must fail the run with MDX_ERR_LIST_STALE
+    naming that step, whether the host finds it while later steps run or in
+    the check at the end of the call (a call that ends with the stale step).
+    A single step per call is always exact: each call's first decision reads
+    the flags of the positions it steps from, so no step goes stale."""
+    from moleculardynamics_amd._lib import MdxError
+    pos, vel, L = squeezed_silicon()
+    n = len(pos)
+
+    def system():
+        s = tersoff_system(n, L, PREC_FP64)
+        s.set_state(np.c_[pos, np.zeros(n)], vel)
+        return s
+
+    with system() as s, pytest.raises(MdxError) as e:
+        s.run(60)
+    assert e.value.code == -7, str(e.value)
+    k = stale_step(e.value)
+    print(f"stale list at step {k}", flush=True)
+    assert k >= 1
+    with system() as s, pytest.raises(MdxError) as e:
+        s.run(k + 1)
+    assert e.value.code == -7 and stale_step(e.value) == k, str(e.value)
+    with system() as s:
+        for _ in range(k + 3):
+            s.run(1)
