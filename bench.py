@@ -204,6 +204,7 @@ def bench_lj(a) -> dict:
     log(f"LJ {n} atoms, L={L:.4f}; warmup {a.warmup}, steps {a.steps}")
     s = System(n_atoms=n, box=BOX_PERIODIC, precision=PREC_MIXED, use_lj=1, lj_table=LJ_UNIFORM,
                dt=0.005, lj_rs=2.8, lj_rcut=2.5, box_length=L, lj_sigma=1.0, lj_eps=1.0, mass=1.0)
+    shape = s.launch_shape()
     s.set_state(np.c_[pos, np.zeros(n)], vel)
     s.run(a.warmup)
     b0 = s.build_count()
@@ -216,7 +217,8 @@ def bench_lj(a) -> dict:
     out = record(value, a.steps, a.warmup, elapsed, "fp32-pair/fp64-accum",
                  "synthetic (seeded FCC lattice + Maxwell-Boltzmann velocities)",
                  {"workload": f"periodic LJ {n} atoms (FCC {a.ncell}^3 x4), rho 0.8442, T* 0.728, "
-                              f"rc 2.5 skin 0.3, dt 0.005", "atoms": n, "list_builds": builds})
+                              f"rc 2.5 skin 0.3, dt 0.005", "atoms": n, "list_builds": builds,
+                  "launch_shape": shape})
     out["roofline"] = roofline(["md_step"], times, n, measured_traffic("md_step_lj1m.md"),
                                "k_step_lds (md_step: LJ force, corrector, next predictor)")
     out["roofline"]["traffic_source"] = "profiles/md_step_lj1m.md"
@@ -411,6 +413,7 @@ def bench_replicas(a, steps: int) -> dict:
     n = int(ns.sum())
     log(f"{len(ns)} LJ replicas, {n} atoms; warmup {a.warmup}, steps {steps}")
     b = Batch(ns, Ls, precision=PREC_MIXED, dt=0.005, lj_rs=2.8, lj_rcut=2.5)
+    shape = b.launch_shape()
     staged = b.staged()
     n_staged = int(ns[staged].sum())
     n_small = int(ns[b.paths() == 2].sum())
@@ -427,7 +430,7 @@ def bench_replicas(a, steps: int) -> dict:
                  {"workload": f"{len(ns)} periodic LJ replicas, {n} atoms ({int(ns.min())}-{int(ns.max())} per "
                               f"replica), rho U[0.70,0.95], T* U[0.5,2.0], rc 2.5 skin 0.3, dt 0.005", "atoms": n,
                   "replica_list_builds": builds, "staged_replicas": int(staged.sum()), "staged_atoms": n_staged,
-                  "all_pairs_atoms": n_small})
+                  "all_pairs_atoms": n_small, "launch_shape": shape})
     out["metric"] = "atom-steps/s per GPU (512 LJ replicas, mixed precision)"
     out["roofline"] = roofline(["rep_step_lds"], times, n_staged, None,
                                "k_step_lds over the staged replicas' segments (their whole step)")

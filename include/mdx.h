@@ -334,6 +334,34 @@ int mdx_batch_kernel_times(mdx_batch* batch, char* names, int64_t* counts, doubl
                            int32_t cap, int32_t* n);
 int mdx_batch_synchronize(mdx_batch* batch);
 
+/*
+ * Launch shapes of the LDS-staged kernels.  No counterpart in the reference,
+ * whose CUDA path hard-wires its block shapes (cuda/cudamd/Host.cpp:86-88,373);
+ * here they come from a tuning table written by the project's tuner
+ * (python -m moleculardynamics_amd.tune) and read by mdx_create (the periodic
+ * Lennard-Jones box, kind "lj", by its atom count) and mdx_batch_create (kind
+ * "replicas", by the batch's total atom count).  The table is the file named
+ * by MDX_TUNING_FILE when that is set (empty: none), else data/tuning.json
+ * beside libmdx.so; without a table or a matching entry the built-in shape
+ * applies.  The shape changes only how the work is cut into workgroups: every
+ * atom's candidates, and so its force, come out the same.
+ */
+typedef struct mdx_shape {
+    int32_t block;     /* threads a workgroup of the single box's LDS kernels: 512 or 1024 */
+    float fill;        /* home atoms a segment aims at, as a fraction of one pass (block / 4) */
+    float stage_aim;   /* replicas: staged atoms a segment's neighbourhood aims at */
+    int32_t source;    /* MDX_SHAPE_DEFAULT or MDX_SHAPE_TUNED */
+} mdx_shape;
+#define MDX_SHAPE_DEFAULT 0   /* built in: block 512, fill 0.85, stage_aim 1250 */
+#define MDX_SHAPE_TUNED 1     /* from the tuning table */
+/* the shape a table at path gives a system of kind on a device of
+ * architecture arch ("gfx950"); host code only, no device needed */
+int mdx_tuning_lookup(const char* path, const char* arch, const char* kind, int64_t atoms, mdx_shape* out);
+const char* mdx_tuning_last_error(void);   /* the calling thread's */
+/* the shape a system or batch runs with */
+int mdx_launch_shape(mdx_system* sys, mdx_shape* out);
+int mdx_batch_launch_shape(mdx_batch* batch, mdx_shape* out);
+
 #ifdef __cplusplus
 }
 #endif

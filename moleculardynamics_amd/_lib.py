@@ -41,6 +41,17 @@ class MdxConfig(C.Structure):
     ]
 
 
+SHAPE_DEFAULT, SHAPE_TUNED = 0, 1
+
+
+class MdxShape(C.Structure):
+    _fields_ = [("block", C.c_int32), ("fill", C.c_float), ("stage_aim", C.c_float), ("source", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {"block": self.block, "fill": round(float(self.fill), 6), "stage_aim": round(float(self.stage_aim), 3),
+                "source": "tuned" if self.source == SHAPE_TUNED else "default"}
+
+
 class MdxError(RuntimeError):
     def __init__(self, fn: str, code: int, msg: str):
         super().__init__(f"{fn} -> {ERRORS.get(code, code)}: {msg}")
@@ -86,6 +97,8 @@ def load() -> C.CDLL:
         "mdx_batch_energies": [P, P], "mdx_batch_virials": [P, P],
         "mdx_batch_set_timing": [P, I32], "mdx_batch_kernel_times": [P, P, P, P, I32, C.POINTER(I32)],
         "mdx_batch_synchronize": [P],
+        "mdx_launch_shape": [P, C.POINTER(MdxShape)], "mdx_batch_launch_shape": [P, C.POINTER(MdxShape)],
+        "mdx_tuning_lookup": [C.c_char_p, C.c_char_p, C.c_char_p, I64, C.POINTER(MdxShape)],
         "mdx_set_surrogate": [P, P, I64], "mdx_surrogate_counts": [P, P],
         "mdx_sg_trainer_create": [I32, I32, D, P, P, P, C.POINTER(P)], "mdx_sg_trainer_destroy": [P],
         "mdx_sg_trainer_feature_stats": [P, P], "mdx_sg_trainer_descriptors": [P, I32, P],
@@ -102,8 +115,22 @@ def load() -> C.CDLL:
     lib.mdx_batch_last_error.restype = C.c_char_p
     lib.mdx_sg_trainer_last_error.argtypes = [P]
     lib.mdx_sg_trainer_last_error.restype = C.c_char_p
+    lib.mdx_tuning_last_error.argtypes = []
+    lib.mdx_tuning_last_error.restype = C.c_char_p
     _lib = lib
     return lib
+
+
+def tuning_lookup(path, arch: str, kind: str, atoms: int) -> dict:
+    """The launch shape a tuning table gives a system of `kind` ("lj" or
+    "replicas") of `atoms` atoms on a device of architecture `arch` (host code,
+    no device needed)."""
+    lib = load()
+    sh = MdxShape()
+    rc = lib.mdx_tuning_lookup(str(path).encode(), arch.encode(), kind.encode(), int(atoms), C.byref(sh))
+    if rc != 0:
+        raise MdxError("mdx_tuning_lookup", rc, lib.mdx_tuning_last_error().decode())
+    return sh.as_dict()
 
 
 def _ptr(a: np.ndarray | None):
@@ -217,6 +244,12 @@ class System:
         b = C.c_int64()
         self._chk("mdx_build_count", self.lib.mdx_build_count(self.h, C.byref(b)))
         return b.value
+
+    def launch_shape(self) -> dict:
+        """The LDS kernels' launch shape and where it came from."""
+        sh = MdxShape()
+        self._chk("mdx_launch_shape", self.lib.mdx_launch_shape(self.h, C.byref(sh)))
+        return sh.as_dict()
 
     def kinetics(self, kboltz):
         out = np.zeros(8)
@@ -351,6 +384,12 @@ class Batch:
     def staged(self):
         """True for the replicas on the LDS-staged path."""
         return self.paths() == 1
+
+    def launch_shape(self) -> dict:
+        """The staged replicas' launch shape and where it came from."""
+        sh = MdxShape()
+        self._chk("mdx_batch_launch_shape", self.lib.mdx_batch_launch_shape(self.h, C.byref(sh)))
+        return sh.as_dict()
 
     def energies(self):
         e = np.zeros((self.R, 2))

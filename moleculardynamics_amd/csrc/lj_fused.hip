@@ -726,13 +726,13 @@ void allow_lds(K kernel) {
 
 }  // namespace
 
-int32_t lds_segments_per_column(int32_t nz, double atoms_per_cell, int32_t kmax, int32_t block) {
-    // about 0.85 x block / 4 home atoms per segment (one pass of the
-    // workgroup's quads), at most kmax cells per segment
+int32_t lds_segments_per_column(int32_t nz, double atoms_per_cell, int32_t kmax, int32_t block, double fill) {
+    // about fill x block / 4 home atoms per segment (a fraction of one pass of
+    // the workgroup's quads), at most kmax cells per segment
     kmax = kmax < kMaxK ? kmax : kMaxK;
     kmax = kmax < 1 ? 1 : kmax;
     const double apc = atoms_per_cell > 0 ? atoms_per_cell : 1.0;
-    int32_t nseg = (int32_t)std::ceil(nz * apc / (0.85 * (block / kLanes)));
+    int32_t nseg = (int32_t)std::ceil(nz * apc / (fill * (block / kLanes)));
     const int32_t lo = (nz + kmax - 1) / kmax;   // segments no longer than kmax cells
     nseg = nseg < lo ? lo : nseg;
     return nseg > nz ? nz : nseg;

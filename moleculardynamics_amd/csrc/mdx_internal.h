@@ -282,7 +282,11 @@ hipError_t launch_gather_correct_predict_track(hipStream_t s, const DevState& st
 
 // lj_fused.hip: LDS-staged segment kernels of the periodic box
 // segments per cell column for workgroups of `block` threads (4 lanes an atom)
-int32_t lds_segments_per_column(int32_t nz, double atoms_per_cell, int32_t kmax, int32_t block);
+int32_t lds_segments_per_column(int32_t nz, double atoms_per_cell, int32_t kmax, int32_t block, double fill);
+// launch shapes (tuning.hip): the built-in one, and the tuning table's for a
+// system of kind ("lj" or "replicas") of this many atoms on the current device
+mdx_shape default_shape();
+int tuned_shape(const char* kind, int64_t atoms, mdx_shape* out, std::string& err);
 int32_t lds_capacity();
 int32_t flag_spread();
 int64_t list_entries(int32_t n, int32_t cap);   // 16-bit entries to allocate for the list

@@ -54,9 +54,6 @@ constexpr int kSortBlock = 1024;
 constexpr int kMaxNc = 10;                               // cells per axis at most (larger cells stay correct)
 constexpr int kMaxCells = kMaxNc * kMaxNc * kMaxNc;
 constexpr int kMaxReplica = 16384;                       // atoms per replica (the LDS sort holds them all)
-// staged atoms a segment of a staged replica aims at: the step kernel's LDS,
-// 1.2 x the largest staging, then stays within a quarter of a CU's 160 KiB
-constexpr double kStageAim = 1250.0;
 // the paths of a replica (mdx_batch_staged reports them)
 constexpr uint8_t kGather = 0, kStaged = 1, kSmall = 2;
 constexpr int kSmallMax = 2048;   // atoms of a small replica (its FP64 positions, 48 KiB of LDS)
@@ -728,6 +725,7 @@ struct mdx_batch {
     double4* d_rq[2] = {nullptr, nullptr};   // position records of buffers A (r) and B (rn)
     int32_t *d_list = nullptr, *d_count = nullptr, *d_err = nullptr;
     int32_t cap = 0;
+    mdx_shape shape{};                    // launch shape (segments' fill, staging aim)
     // staged replicas (the single box's LDS path, see the file header)
     uint8_t* d_staged = nullptr;          // R flags
     int32_t* d_staged_i = nullptr;        // staged and small replicas (int32 lists only for the energies)
@@ -1004,6 +1002,18 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
     mdx_batch* b = new mdx_batch;
     b->cfg = c;
     b->R = n_replicas;
+    {
+        // the launch shape for the batch's total atom count from the tuning table
+        int64_t total = 0;
+        for (int32_t r = 0; r < n_replicas; r++) total += n_atoms[r] > 0 ? n_atoms[r] : 0;
+        std::string why;
+        const int rc = tuned_shape("replicas", total, &b->shape, why);
+        if (rc) {
+            delete b;
+            return bfail(nullptr, rc, why);
+        }
+    }
+    const double stage_aim = b->shape.stage_aim;
     int64_t coff = 0;
     for (int32_t r = 0; r < n_replicas; r++) {
         const double L = box_length[r];
@@ -1021,9 +1031,9 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
         coff += (nc ? (int64_t)nc * nc * nc : 1) + 1;
     }
     // staged replicas: at least 3 cells per axis, the z-columns cut into
-    // segments of about 0.85 x 128 home atoms (as the single box cuts them),
+    // segments of about fill x 128 home atoms (as the single box cuts them),
     // shorter where their neighbourhood, 9 columns of K + 2 cells, would stage
-    // more than kStageAim atoms; a replica whose segments cannot fit the LDS
+    // more than the shape's stage_aim atoms; a replica whose segments cannot fit the LDS
     // staging with a fifth to spare stays on the gather path.  Boxes narrower
     // than 3 cells of up to kSmallMax atoms take k_rep_small.
     std::vector<uint8_t> staged(n_replicas, 0), path(n_replicas, kGather);
@@ -1039,8 +1049,8 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
             continue;
         }
         const double per_cell = (double)n_atoms[r] / ((double)nc * nc * nc);
-        int32_t nseg = lds_segments_per_column(nc, per_cell, nc, 512);
-        while (nseg < nc && 9.0 * ((nc + nseg - 1) / nseg + 2) * per_cell > kStageAim) nseg++;
+        int32_t nseg = lds_segments_per_column(nc, per_cell, nc, 512, b->shape.fill);
+        while (nseg < nc && 9.0 * ((nc + nseg - 1) / nseg + 2) * per_cell > stage_aim) nseg++;
         const int32_t K = (nc + nseg - 1) / nseg;
         if (1.2 * 9.0 * (K + 2) * per_cell + 64 > lds_capacity()) continue;
         staged[r] = 1;
@@ -1402,6 +1412,12 @@ int mdx_batch_kernel_times(mdx_batch* b, char* names, int64_t* counts, double* t
 int mdx_batch_synchronize(mdx_batch* b) {
     if (!b) return bfail(b, MDX_ERR_ARG, "null batch");
     BCHK(b, hipStreamSynchronize(b->stream));
+    return MDX_OK;
+}
+
+int mdx_batch_launch_shape(mdx_batch* b, mdx_shape* out) {
+    if (!b || !out) return bfail(b, MDX_ERR_ARG, "null argument");
+    *out = b->shape;
     return MDX_OK;
 }
 

@@ -88,6 +88,7 @@ struct mdx_system {
     int32_t lds_atoms = 0;           // largest segment staging of the last build
     int32_t build_lds = 0;           // staging capacity the list build runs with
     int32_t box_block = 512;         // threads a workgroup of the LDS kernels (512 or 1024)
+    mdx_shape shape{};               // launch shape (box_block and the segments' fill come from it)
     // periodic Tersoff silicon: compact lists [k][slot] and per-bond derivatives
     TersoffParams tp{};
     int32_t* ts_list = nullptr;
@@ -637,7 +638,7 @@ int rebuild_periodic(mdx_system* s) {
     }
     auto round64 = [](double a) { return ((int32_t)std::ceil(a) + 63) / 64 * 64; };
     for (;;) {
-        s->nseg = lds_segments_per_column(v.g[2], per_cell, s->seg_kmax, s->box_block);
+        s->nseg = lds_segments_per_column(v.g[2], per_cell, s->seg_kmax, s->box_block, s->shape.fill);
         if (s->build_lds == 0) {
             // staging estimate: 9 columns of K + 2 cells at 1.3 x the mean occupancy
             const int32_t kcells = (v.g[2] + s->nseg - 1) / s->nseg;
@@ -947,6 +948,16 @@ int mdx_create(const mdx_config* cfg, mdx_system** out) {
         (rc = dalloc(s, &s->dint, 16)) || (rc = dalloc(s, &s->dbits, 4))) {
         g_err = s->err;
         return bail(rc);
+    }
+    // launch shape of the periodic LJ box's LDS kernels: the tuning table's
+    s->shape = default_shape();
+    if (cfg->box == MDX_BOX_PERIODIC && cfg->use_lj) {
+        std::string why;
+        if ((rc = tuned_shape("lj", n, &s->shape, why))) {
+            g_err = why;
+            return bail(rc);
+        }
+        s->box_block = s->shape.block;
     }
     s->nb = kinetics_blocks(n);
     if ((rc = dalloc(s, &s->partials, 8 * (size_t)s->nb + 16)) ||
@@ -1409,6 +1420,12 @@ int mdx_kernel_times(mdx_system* s, char* names, int64_t* counts, double* total_
 int mdx_synchronize(mdx_system* s) {
     if (!s) return fail(s, MDX_ERR_ARG, "null system");
     HIPCHK(s, hipStreamSynchronize(s->stream));
+    return MDX_OK;
+}
+
+int mdx_launch_shape(mdx_system* s, mdx_shape* out) {
+    if (!s || !out) return fail(s, MDX_ERR_ARG, "null argument");
+    *out = s->shape;
     return MDX_OK;
 }
 
