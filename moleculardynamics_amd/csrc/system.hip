@@ -796,6 +796,9 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
         if (d2_bits >= gate_bits && (rc = rebuild_lists())) return rc;
         int32_t k = 0;        // next step to launch
         int32_t checked = 0;  // steps before this one have run with a valid list
+        // with timing on: the record of each launch in flight, so that a
+        // halted one is counted as "md_step_halted" instead of "md_step"
+        int64_t rec[4] = {-1, -1, -1, -1};
         while (checked < nsteps) {
             if (k < nsteps && k - checked < 2) {
                 const bool fin = k + 1 == nsteps;
@@ -803,6 +806,7 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
                                                      s->ncount, s->nl_cap, s->ljt, s->mt, s->gc, hard2, n,
                                                      component_stride(n), c.precision, fin, fin, slot_flags(k),
                                                      s->lds_atoms, slot_flags(k + 1), publish(k), s->box_block));
+                rec[k % 4] = s->timing ? (int64_t)s->pending.size() - 1 : -1;
                 for (int d = 0; d < 3; d++) std::swap(s->st.r[d], s->rnext[d]);
                 s->t++;
                 k++;
@@ -814,6 +818,8 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
                 // steps checked .. k-1 ran as no-ops: r(checked) is where the
                 // first of them left it; undo their position swaps and steps
                 const int32_t undone = k - checked;
+                for (int32_t j = checked; j < k; j++)
+                    if (rec[j % 4] >= 0) s->pending[rec[j % 4]].name = "md_step_halted";
                 if (undone % 2 != 0)
                     for (int d = 0; d < 3; d++) std::swap(s->st.r[d], s->rnext[d]);
                 s->t -= undone;

@@ -3,7 +3,8 @@ shipped network.
 
 Everything that computes runs on the GPU through the C-ABI:
   * frames come from plain-Tersoff trajectories of the engine (FP64 Tersoff,
-    periodic 512-atom diamond boxes, 300-2000 K);
+    periodic 512-atom diamond boxes, 48 temperatures over 300-2000 K, three
+    of them held out);
   * training is minibatch Adam on the surrogate's own FP32 forward/backward
     kernels (mdx_sg_trainer_*), loss = force MSE + w_E * (energy/atom)^2;
   * inference (bond_form BOND_SURROGATE) runs the network on BF16 MFMA.
@@ -49,7 +50,7 @@ def tersoff_forces(pos, L, precision=PREC_FP64):
     return e, f
 
 
-def tersoff_frames(n_traj=24, frames_per_traj=84, t_min=300.0, t_max=2000.0, ncell=4, stride=20, equil=200,
+def tersoff_frames(n_traj=48, frames_per_traj=84, t_min=300.0, t_max=2000.0, ncell=4, stride=20, equil=200,
                    seed=1000, log=None):
     """Frames of plain-Tersoff MD (Gear PC, dt = 1 fs): trajectory k starts
     from diamond at twice its target temperature T_k (equipartition halves
@@ -146,9 +147,9 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--out", default=str(WEIGHTS))
     ap.add_argument("--report", default=None, help="JSON summary path")
-    ap.add_argument("--trajectories", type=int, default=24)
+    ap.add_argument("--trajectories", type=int, default=48)
     ap.add_argument("--frames", type=int, default=84, help="frames per trajectory")
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=120)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--seed", type=int, default=0)

@@ -230,3 +230,35 @@ def test_shipped_network_against_plain_tersoff():
     assert rmse_untrained > 20 * rmse
     assert counts[0] == 0
     assert abs(e1 - e0) < 1e-4 * abs(e0)        # NVE drift over 400 fs, as plain Tersoff's test
+
+
+@pytest.mark.gpu
+def test_shipped_network_at_1850K():
+    """The hot end of the training range on a trajectory the network never
+    saw: plain-Tersoff FP64 MD of a 512-atom box at 1850 K from a seed outside
+    the training set's (1000-1047), 40 frames 20 fs apart.  The network does
+    not reach the 0.05 eV/A of the 1000 K configuration here: its error grows
+    with temperature (held out in training: 0.014 eV/A at 517 K, 0.026 at
+    1132 K, 0.044 at 1819 K) and along this trajectory (0.071 over its first
+    10 frames, 0.096 over all 40), about 4% of the 2.3 eV/A force RMS.
+    Training five times longer left the held-out error at the hot end
+    unchanged, so the limit is the 48-64-64-1 network and its descriptor, not
+    the training.  The bound here, 5% of the force RMS, guards that level."""
+    from moleculardynamics_amd import surrogate as SG
+
+    blob = SG.load_weights()
+    P, F, _, _, L = SG.tersoff_frames(n_traj=1, frames_per_traj=40, t_min=1850.0, t_max=1850.0, seed=4242)
+    n = P.shape[1]
+    err2 = 0.0
+    with SG.silicon_system(n, L, bond_form=BOND_SURROGATE) as s:
+        s.set_surrogate(blob)
+        for p, f in zip(P, F):
+            s.set_state(np.c_[p, np.zeros(n)], np.zeros((n, 3)))
+            s.build_list(LIST_BOND)
+            s.bond_forces()
+            err2 += float(np.sum((s.get_state()["force"] - f) ** 2))
+        fallbacks = int(s.surrogate_counts()[0])
+    rmse = np.sqrt(err2 / (3 * n * len(P)))
+    print(f"1850 K, {len(P)} frames of {n} atoms: force RMSE {rmse:.4f} eV/A, force RMS "
+          f"{np.sqrt(np.mean(F ** 2)):.3f}, fallback atom-evaluations {fallbacks}", flush=True)
+    assert rmse < 0.05 * np.sqrt(np.mean(F ** 2))
