@@ -241,9 +241,10 @@ def test_shipped_network_at_1850K():
     with temperature (held out in training: 0.014 eV/A at 517 K, 0.026 at
     1132 K, 0.044 at 1819 K) and along this trajectory (0.071 over its first
     10 frames, 0.096 over all 40), about 4% of the 2.3 eV/A force RMS.
-    Training five times longer left the held-out error at the hot end
-    unchanged, so the limit is the 48-64-64-1 network and its descriptor, not
-    the training.  The bound here, 5% of the force RMS, guards that level."""
+    Training five times longer (on the earlier 24-temperature set) left the
+    hot end's held-out error unchanged, which points at the 48-64-64-1
+    network and its descriptor rather than the training.  The bound here, 5%
+    of the force RMS, guards that level."""
     from moleculardynamics_amd import surrogate as SG
 
     blob = SG.load_weights()
