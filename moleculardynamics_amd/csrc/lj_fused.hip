@@ -267,19 +267,34 @@ __device__ inline bool stage_fill(Stage& S, const PosT& P, const SegGrid& g, int
         const int32_t len = l0 + l1 + l2, dst = S.colbase[col];
         for (int32_t w0 = l; w0 < len; w0 += kCopyBatch * kColThreads) {
             double bx[kCopyBatch], by[kCopyBatch], bz[kCopyBatch];
+            // every load goes out, the ones past the column's end clamped to
+            // its last atom (read, never stored); the empty asm takes all of
+            // them as inputs, so they are issued together and one wait covers
+            // the batch (the loads are of read-only memory, which a memory
+            // clobber alone would not hold back)
+            int32_t src[kCopyBatch];
+            double sz[kCopyBatch];
 #pragma unroll
             for (int u = 0; u < kCopyBatch; u++) {
-                const int32_t w = w0 + u * kColThreads;
-                if (w < len) {
-                    int32_t src;
-                    double sz;
-                    if (w < l0) { src = s0 + w; sz = -g.L; }
-                    else if (w < l0 + l1) { src = s1 + (w - l0); sz = 0.0; }
-                    else { src = s2 + (w - l0 - l1); sz = g.L; }
-                    bx[u] = rx[src] + sx - ox;
-                    by[u] = ry[src] + sy - oy;
-                    bz[u] = rz[src] + sz - oz;
-                }
+                const int32_t w = min(w0 + u * kColThreads, len - 1);
+                const bool p0 = w < l0, p1 = w < l0 + l1;
+                src[u] = p0 ? s0 + w : (p1 ? s1 + (w - l0) : s2 + (w - l0 - l1));
+                sz[u] = p0 ? -g.L : (p1 ? 0.0 : g.L);
+            }
+#pragma unroll
+            for (int u = 0; u < kCopyBatch; u++) {
+                bx[u] = rx[src[u]];
+                by[u] = ry[src[u]];
+                bz[u] = rz[src[u]];
+            }
+            static_assert(kCopyBatch == 4, "the asm below names four atoms");
+            asm volatile("" ::"v"(bx[0]), "v"(bx[1]), "v"(bx[2]), "v"(bx[3]), "v"(by[0]), "v"(by[1]), "v"(by[2]),
+                         "v"(by[3]), "v"(bz[0]), "v"(bz[1]), "v"(bz[2]), "v"(bz[3]));
+#pragma unroll
+            for (int u = 0; u < kCopyBatch; u++) {
+                bx[u] = bx[u] + sx - ox;
+                by[u] = by[u] + sy - oy;
+                bz[u] = bz[u] + sz[u] - oz;
             }
 #pragma unroll
             for (int u = 0; u < kCopyBatch; u++) {
