@@ -31,7 +31,10 @@
 //    quad, then the Gear corrector of step t and the predictor of step t+1,
 //    lane q handling component q.  Positions are double-buffered: r(t) is read
 //    (and staged) from A and r(t+1) written to B, so no workgroup reads a
-//    position another one has already advanced.
+//    position another one has already advanced.  The staging layout of a
+//    segment (column starts, copy pieces, home range) changes only with the
+//    cells, so the list build writes it to a per-segment header table and the
+//    step kernel loads it instead of deriving it from the cell starts.
 // Workgroup-to-segment mapping is XCD-aware: workgroups are dealt to the 8
 // XCDs round robin, so block b takes segment (b % 8) * per + b / 8 and each
 // XCD's L2 serves one contiguous slab of the box.
@@ -181,14 +184,25 @@ struct PosF {
         p[3 * i + 2] = (float)z;
     }
 };
-// LDS header shared by both kernels
-struct Stage {
+// The part of a segment's staging layout the step kernel needs.  The list
+// build writes it to a table (one per segment, kHdrWords words) and the step
+// kernels load it from there: the cells do not change between builds.
+struct SegHdr {
     int32_t colbase[9];                 // LDS start of each column
-    int32_t cellpre[9][kMaxK + 3];      // per column, prefix counts over the K+2 cells
-    int32_t cstart[9][kMaxK + 2];       // per column and window cell, first slot
     int32_t plen[9][3];                 // per column, lengths of the 3 copy pieces
     int32_t psrc[9][3];                 // per column, first slot of each piece
-    int32_t total;
+    int32_t total;                      // staged atoms
+    int32_t self0;                      // LDS index of the first home atom
+    int32_t hbeg, hend;                 // slot range of the home atoms
+    int32_t pad;
+};
+constexpr int kHdrWords = 68;
+static_assert(sizeof(SegHdr) == kHdrWords * sizeof(int32_t), "segment header layout");
+// LDS header shared by both kernels
+struct Stage {
+    SegHdr h;
+    int32_t cellpre[9][kMaxK + 3];      // per column, prefix counts over the K+2 cells (list build)
+    int32_t cstart[9][kMaxK + 2];       // per column and window cell, first slot (list build)
     unsigned long long red[3];          // step kernel: max disp^2 bits, max step^2 bits, stale count
 };
 
@@ -218,21 +232,26 @@ __device__ inline void stage_header(Stage& S, const SegGrid& g, int32_t cx, int3
         for (int32_t o = 1; o <= K2; o++) S.cellpre[tid][o] += S.cellpre[tid][o - 1];
         const bool wlo = z0 == 0, whi = z1 == g.n[2];
         const int32_t olo = wlo ? 1 : 0, ohi = whi ? K2 - 2 : K2 - 1;
-        S.plen[tid][0] = wlo ? S.cellpre[tid][1] : 0;
-        S.psrc[tid][0] = S.cstart[tid][0];
-        S.plen[tid][1] = S.cellpre[tid][ohi + 1] - S.cellpre[tid][olo];
-        S.psrc[tid][1] = S.cstart[tid][olo];
-        S.plen[tid][2] = whi ? S.cellpre[tid][K2] - S.cellpre[tid][K2 - 1] : 0;
-        S.psrc[tid][2] = S.cstart[tid][K2 - 1];
+        S.h.plen[tid][0] = wlo ? S.cellpre[tid][1] : 0;
+        S.h.psrc[tid][0] = S.cstart[tid][0];
+        S.h.plen[tid][1] = S.cellpre[tid][ohi + 1] - S.cellpre[tid][olo];
+        S.h.psrc[tid][1] = S.cstart[tid][olo];
+        S.h.plen[tid][2] = whi ? S.cellpre[tid][K2] - S.cellpre[tid][K2 - 1] : 0;
+        S.h.psrc[tid][2] = S.cstart[tid][K2 - 1];
     }
     __syncthreads();
     if (tid == 0) {
         int32_t acc = 0;
         for (int32_t col = 0; col < 9; col++) {
-            S.colbase[col] = acc;
+            S.h.colbase[col] = acc;
             acc += S.cellpre[col][K2];
         }
-        S.total = acc;
+        S.h.total = acc;
+        // the home atoms: window cells 1 .. K2 - 2 of the centre column (z0 .. z1 - 1)
+        S.h.self0 = S.h.colbase[4] + S.cellpre[4][1];
+        S.h.hbeg = S.cstart[4][1];
+        S.h.hend = S.cstart[4][1] + (S.cellpre[4][K2 - 1] - S.cellpre[4][1]);
+        S.h.pad = 0;
     }
     __syncthreads();
 }
@@ -240,10 +259,10 @@ __device__ inline void stage_header(Stage& S, const SegGrid& g, int32_t cx, int3
 // the slot of staged atom w (LDS index) of a segment whose header is S
 __device__ inline int32_t staged_slot(const Stage& S, int32_t w) {
     int32_t col = 0;
-    while (col < 8 && S.colbase[col + 1] <= w) col++;
-    w -= S.colbase[col];
-    const int32_t l0 = S.plen[col][0], l1 = S.plen[col][1];
-    return w < l0 ? S.psrc[col][0] + w : (w < l0 + l1 ? S.psrc[col][1] + (w - l0) : S.psrc[col][2] + (w - l0 - l1));
+    while (col < 8 && S.h.colbase[col + 1] <= w) col++;
+    w -= S.h.colbase[col];
+    const int32_t l0 = S.h.plen[col][0], l1 = S.h.plen[col][1];
+    return w < l0 ? S.h.psrc[col][0] + w : (w < l0 + l1 ? S.h.psrc[col][1] + (w - l0) : S.h.psrc[col][2] + (w - l0 - l1));
 }
 
 // Fill the staging of segment (cx, cy, ...) whose header is S: positions
@@ -254,7 +273,7 @@ __device__ inline bool stage_fill(Stage& S, const PosT& P, const SegGrid& g, int
                                   const double* __restrict__ rx, const double* __restrict__ ry,
                                   const double* __restrict__ rz, double ox = 0.0, double oy = 0.0, double oz = 0.0) {
     const int32_t tid = threadIdx.x;
-    if (S.total + 1 > P.cap) return false;   // + the dummy atom
+    if (S.h.total + 1 > P.cap) return false;   // + the dummy atom
     // copy: kColThreads threads per column, kCopyBatch independent loads each
     constexpr int32_t kColThreads = kB / 9;
     if (tid < 9 * kColThreads) {
@@ -262,9 +281,9 @@ __device__ inline bool stage_fill(Stage& S, const PosT& P, const SegGrid& g, int
         int32_t x = cx + col / 3 - 1, y = cy + col % 3 - 1;
         const double sx = x < 0 ? -g.L : (x >= g.n[0] ? g.L : 0.0);
         const double sy = y < 0 ? -g.L : (y >= g.n[1] ? g.L : 0.0);
-        const int32_t l0 = S.plen[col][0], l1 = S.plen[col][1], l2 = S.plen[col][2];
-        const int32_t s0 = S.psrc[col][0], s1 = S.psrc[col][1], s2 = S.psrc[col][2];
-        const int32_t len = l0 + l1 + l2, dst = S.colbase[col];
+        const int32_t l0 = S.h.plen[col][0], l1 = S.h.plen[col][1], l2 = S.h.plen[col][2];
+        const int32_t s0 = S.h.psrc[col][0], s1 = S.h.psrc[col][1], s2 = S.h.psrc[col][2];
+        const int32_t len = l0 + l1 + l2, dst = S.h.colbase[col];
         for (int32_t w0 = l; w0 < len; w0 += kCopyBatch * kColThreads) {
             double bx[kCopyBatch], by[kCopyBatch], bz[kCopyBatch];
             // every load goes out, the ones past the column's end clamped to
@@ -303,19 +322,11 @@ __device__ inline bool stage_fill(Stage& S, const PosT& P, const SegGrid& g, int
             }
         }
     }
-    if (tid == kB - 1) P.set(S.total, kFar, kFar, kFar);
+    if (tid == kB - 1) P.set(S.h.total, kFar, kFar, kFar);
     __syncthreads();
     return true;
 }
 
-// Stage the neighbourhood of segment (cx, cy, z0..z1): header, then positions.
-template <int kB, typename PosT>
-__device__ inline bool stage(Stage& S, const PosT& P, const SegGrid& g, int32_t cx, int32_t cy, int32_t z0,
-                             int32_t z1, const double* __restrict__ rx, const double* __restrict__ ry,
-                             const double* __restrict__ rz, double ox = 0.0, double oy = 0.0, double oz = 0.0) {
-    stage_header(S, g, cx, cy, z0, z1);
-    return stage_fill<kB>(S, P, g, cx, cy, rx, ry, rz, ox, oy, oz);
-}
 
 // window cell (1..K) of home atom number qc of the segment
 __device__ inline int32_t home_cell(const Stage& S, int32_t qc) {
@@ -337,30 +348,32 @@ __device__ __forceinline__ void nlist_segment(Stage& S, double* dyn, const SegGr
                                               const double* __restrict__ ry, const double* __restrict__ rz,
                                               double rl2, int32_t cap, uint16_t* __restrict__ list,
                                               int32_t* __restrict__ count, int32_t* __restrict__ err,
-                                              int32_t lds_cap) {
+                                              int32_t lds_cap, int32_t* __restrict__ hdr_out) {
     const int32_t nchunk = cap / kRowEntries + 1;   // rows per atom with the spare
     const int32_t rows = cap / kRowEntries;         // rows that hold entries
     const PosF P{reinterpret_cast<float*>(dyn), lds_cap};
-    if (!stage<kB>(S, P, g, cx, cy, z0, z1, rx, ry, rz, cx * (g.L / g.n[0]), cy * (g.L / g.n[1]), z0 * (g.L / g.n[2]))) {
-        if (threadIdx.x == 0) atomicMax(&err[0], S.total + 1);
+    stage_header(S, g, cx, cy, z0, z1);
+    // this segment's header for the step kernels
+    if (threadIdx.x < kHdrWords) hdr_out[threadIdx.x] = reinterpret_cast<const int32_t*>(&S.h)[threadIdx.x];
+    if (!stage_fill<kB>(S, P, g, cx, cy, rx, ry, rz, cx * (g.L / g.n[0]), cy * (g.L / g.n[1]), z0 * (g.L / g.n[2]))) {
+        if (threadIdx.x == 0) atomicMax(&err[0], S.h.total + 1);
         return;
     }
     const float rlf = (float)(rl2 + 1e-4);   // superset of the FP64 test (see PosF)
     // largest staging (with the dummy atom), sizes the step kernel's LDS
-    if (threadIdx.x == 0) atomicMax(&err[2], S.total + 1);
-    const int32_t cbase = (cx * g.n[1] + cy) * g.n[2];
-    const int32_t hbeg = g.start[cbase + z0], hend = g.start[cbase + z1];
+    if (threadIdx.x == 0) atomicMax(&err[2], S.h.total + 1);
+    const int32_t hbeg = S.h.hbeg, hend = S.h.hend;
     const int32_t q = threadIdx.x & (kLanes - 1);
     // this quad's two-row ring; lane q's chunk of a row is 16 bytes at entry 8q
     uint16_t* const ring = reinterpret_cast<uint16_t*>(reinterpret_cast<char*>(dyn) + ring_offset(lds_cap)) +
                            (threadIdx.x / kLanes) * kRing;
-    const uint16_t dummy = (uint16_t)S.total;
+    const uint16_t dummy = (uint16_t)S.h.total;
     for (int32_t base = hbeg; base < hend; base += kB / kLanes) {
         const int32_t s = base + threadIdx.x / kLanes;
         if (s >= hend) continue;             // whole quads drop out together
         const int32_t qc = s - hbeg;
         const int32_t o = home_cell(S, qc);
-        const int32_t self = S.colbase[4] + S.cellpre[4][1] + qc;
+        const int32_t self = S.h.colbase[4] + S.cellpre[4][1] + qc;
         const float xi = P.x(self), yi = P.y(self), zi = P.z(self);
         // lane q's chunk of row r of atom s
         uint4* const la = reinterpret_cast<uint4*>(list + ((int64_t)(s / kListBlock) * nchunk * kListBlock + s % kListBlock) *
@@ -377,7 +390,7 @@ __device__ __forceinline__ void nlist_segment(Stage& S, double* dyn, const SegGr
         };
         int32_t k = 0;                        // hits so far, the same in all 4 lanes
         for (int32_t c = 0; c < 9; c++) {
-            const int32_t b = S.colbase[c] + S.cellpre[c][o - 1], e = S.colbase[c] + S.cellpre[c][o + 2];
+            const int32_t b = S.h.colbase[c] + S.cellpre[c][o - 1], e = S.h.colbase[c] + S.cellpre[c][o + 2];
             for (int32_t p0 = b; p0 < e; p0 += kLanes * kScan) {
                 // lane q tests candidates p0 + kLanes u + q (the list radius test
                 // may round differently from the reference's; the list only has to
@@ -445,7 +458,7 @@ template <bool kRep, int kB>
 __global__ void __launch_bounds__(kB, 8)
 k_nlist_lds(SegGrid g, RepGeo rg, const double* __restrict__ rx, const double* __restrict__ ry,
             const double* __restrict__ rz, double rl2, int32_t cap, uint16_t* __restrict__ list,
-            int32_t* __restrict__ count, int32_t* __restrict__ err, int32_t lds_cap) {
+            int32_t* __restrict__ count, int32_t* __restrict__ err, int32_t lds_cap, int32_t* __restrict__ hdr) {
     __shared__ Stage S;
     extern __shared__ double dyn[];
     int32_t cx, cy, z0, z1;
@@ -454,13 +467,17 @@ k_nlist_lds(SegGrid g, RepGeo rg, const double* __restrict__ rx, const double* _
         const int32_t per_x = (int32_t)gridDim.x / kXcd, x = (int32_t)(blockIdx.x % kXcd);
         const int32_t w1 = (int32_t)((int64_t)nw * (x + 1) / kXcd);
         for (int32_t w = (int32_t)((int64_t)nw * x / kXcd) + (int32_t)(blockIdx.x / kXcd); w < w1; w += per_x) {
-            rep_geometry(rg, rg.wl[w], g, cx, cy, z0, z1);
-            nlist_segment<kB>(S, dyn, g, cx, cy, z0, z1, rx, ry, rz, rl2, cap, list, count, err, lds_cap);
+            const int32_t b = rg.wl[w];
+            rep_geometry(rg, b, g, cx, cy, z0, z1);
+            nlist_segment<kB>(S, dyn, g, cx, cy, z0, z1, rx, ry, rz, rl2, cap, list, count, err, lds_cap,
+                              hdr + (int64_t)b * kHdrWords);
             __syncthreads();   // the staging is reused by the next segment
         }
     } else {
-        if (seg_geometry(g, cx, cy, z0, z1) < 0) return;
-        nlist_segment<kB>(S, dyn, g, cx, cy, z0, z1, rx, ry, rz, rl2, cap, list, count, err, lds_cap);
+        const int32_t seg = seg_geometry(g, cx, cy, z0, z1);
+        if (seg < 0) return;
+        nlist_segment<kB>(S, dyn, g, cx, cy, z0, z1, rx, ry, rz, rl2, cap, list, count, err, lds_cap,
+                          hdr + (int64_t)seg * kHdrWords);
     }
 }
 
@@ -494,7 +511,7 @@ k_nlist_decode(SegGrid g, RepGeo rg, const uint16_t* __restrict__ list, const in
         for (int32_t kk = 0; kk < c; kk++) {
             const int32_t w = la[(kk / kRowEntries) * kChunkRow + kk % kRowEntries];
             // an index past the staged atoms (the padding's dummy) inside the count would be a bug: -2
-            rows[(int64_t)me * width + kk] = w < S.total ? id[staged_slot(S, w)] : -2;
+            rows[(int64_t)me * width + kk] = w < S.h.total ? id[staged_slot(S, w)] : -2;
         }
     }
 }
@@ -506,6 +523,7 @@ struct StepArgs {
     const double* r0[3];    // positions at the last list build
     const uint16_t* list;
     const int32_t* count;
+    const int32_t* hdr;     // segment headers (kHdrWords each) written by the list build
     int32_t nchunk;         // list chunks per lane (capacity / 32 + the spare)
     double rc2;
     float f12, f6;          // F/r = x^4 (f12 x^3 + f6), x = 1/r^2: 48 eps sig^12, -24 eps sig^6
@@ -538,24 +556,24 @@ k_step_lds(StepArgs a) {
         for (int32_t t = threadIdx.x; t < 3 * kSpread; t += kB) a.clear[t] = 0;
     int32_t cx, cy, z0, z1, rep = 0, seg = 0;
     if constexpr (kRep) {
-        const int32_t b = rep_block(a.rg);
-        if (b < 0) return;
-        rep = rep_geometry(a.rg, b, g, cx, cy, z0, z1);
+        seg = rep_block(a.rg);   // the segment table's entry
+        if (seg < 0) return;
+        rep = rep_geometry(a.rg, seg, g, cx, cy, z0, z1);
     } else if ((seg = seg_geometry(g, cx, cy, z0, z1)) < 0) {
         return;
     }
     const Pos P{dyn, a.lds_cap};
     unsigned long long* fl = kRep ? nullptr : a.flags + 3 * (blockIdx.x % kSpread);
-    // the single box's list gate, set when r(t) has left the list's margin:
-    // loaded beside the segment's slot range, so its latency passes with theirs
+    // the single box's list gate (set when r(t) has left the list's margin)
+    // and this segment's header from the last list build, loaded together
     unsigned int halt = 0;
     if constexpr (!kRep) halt = __hip_atomic_load(a.pub.gate, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int32_t cbase = (cx * g.n[1] + cy) * g.n[2];
-    const int32_t hbeg = g.start[cbase + z0], hend = g.start[cbase + z1];
+    int32_t hw = 0;
+    if (threadIdx.x < kHdrWords) hw = a.hdr[(int64_t)seg * kHdrWords + threadIdx.x];
     if constexpr (!kRep) {
-        // the empty asm keeps the range loads above the test, so one wait
-        // covers all three
-        asm volatile("" ::"v"(hbeg), "v"(hend));
+        // the empty asm keeps the header load above the test, so one wait
+        // covers both
+        asm volatile("" ::"v"(hw));
         if (halt) {
             // the list does not cover r(t): the step runs as a no-op (no state
             // or flag slot is written) and workgroup 0 says so
@@ -563,6 +581,10 @@ k_step_lds(StepArgs a) {
             return;
         }
     }
+    if (threadIdx.x < kHdrWords) reinterpret_cast<int32_t*>(&S.h)[threadIdx.x] = hw;
+    if (threadIdx.x < 3) S.red[threadIdx.x] = 0;
+    __syncthreads();
+    const int32_t hbeg = S.h.hbeg, hend = S.h.hend;
     const int32_t q = threadIdx.x & (kLanes - 1);
     // the first pass's list count and first chunk, loaded before the staging so
     // that their latency passes while it runs
@@ -578,11 +600,11 @@ k_step_lds(StepArgs a) {
         count_first = a.count[i_first];
         chunk_first = list_of(i_first)[0];
     }
-    if (!stage<kB>(S, P, g, cx, cy, z0, z1, a.st.r[0], a.st.r[1], a.st.r[2])) {
+    if (!stage_fill<kB>(S, P, g, cx, cy, a.st.r[0], a.st.r[1], a.st.r[2])) {
         // cannot happen after a successful list build on the same cells: the
         // single box reports it as stale, the replicas through their error word
         if (threadIdx.x == 0) {
-            if constexpr (kRep) atomicMax(&a.err[0], S.total + 1);
+            if constexpr (kRep) atomicMax(&a.err[0], S.h.total + 1);
             else atomicAdd(&fl[2], 1ull << 32);
         }
         if constexpr (!kRep) publish_flags(a.pub, seg, a.flags, 3 * kSpread);
@@ -593,7 +615,7 @@ k_step_lds(StepArgs a) {
         const int32_t i = base + threadIdx.x / kLanes;
         if (i >= hend) continue;              // whole quads drop out together
         // r(t) of the home atom: its staged copy (centre column, never shifted)
-        const int32_t self = S.colbase[4] + S.cellpre[4][1] + (i - hbeg);
+        const int32_t self = S.h.self0 + (i - hbeg);
         const double xi = P.x(self), yi = P.y(self), zi = P.z(self);
         double fx = 0, fy = 0, fz = 0;
         const bool first = base == hbeg;
@@ -761,9 +783,12 @@ int64_t list_entries(int32_t n, int32_t cap) {
 int32_t list_cap_multiple() { return kLanes * kChunk; }
 int32_t flag_spread() { return kSpread; }
 
+int32_t seg_header_words() { return kHdrWords; }
+int64_t seg_count(const GridView& v, int32_t nseg) { return (int64_t)v.g[0] * v.g[1] * nseg; }
+
 hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg, double rl2,
                             int32_t n, int32_t cap, uint16_t* list, int32_t* count, int32_t* err,
-                            int32_t lds_atoms, int32_t block) {
+                            int32_t lds_atoms, int32_t block, int32_t* hdr) {
     if (n <= 0) return hipSuccess;
     if (cap % (kLanes * kChunk) != 0) return hipErrorInvalidValue;
     if (lds_atoms < 1 || lds_atoms > kLdsAtoms) return hipErrorInvalidValue;
@@ -778,10 +803,10 @@ hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v
     const size_t dyn = ring_offset(lds_atoms) + (size_t)(block / kLanes) * kRing * sizeof(uint16_t);
     if (block == 1024)
         k_nlist_lds<false, 1024><<<g.per_xcd * kXcd, 1024, dyn, s>>>(g, RepGeo{}, st.r[0], st.r[1], st.r[2], rl2,
-                                                                     cap, list, count, err, lds_atoms);
+                                                                     cap, list, count, err, lds_atoms, hdr);
     else
         k_nlist_lds<false, 512><<<g.per_xcd * kXcd, 512, dyn, s>>>(g, RepGeo{}, st.r[0], st.r[1], st.r[2], rl2,
-                                                                   cap, list, count, err, lds_atoms);
+                                                                   cap, list, count, err, lds_atoms, hdr);
     return hipGetLastError();
 }
 
@@ -789,7 +814,8 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
                            double* const rn[3], double* const r0[3], const uint16_t* list, const int32_t* count,
                            int32_t cap, const LjTable& t, const MassTable& m, const GearCoeffs& gc, double hard2, int32_t n,
                            int64_t cs, int32_t precision, bool final_step, bool force_out, uint64_t* flags,
-                           int32_t lds_atoms, uint64_t* clear_next, FlagPublish pub, int32_t block) {
+                           int32_t lds_atoms, uint64_t* clear_next, FlagPublish pub, int32_t block,
+                           const int32_t* hdr) {
     if (n <= 0) return hipSuccess;
     static const bool attr_set = [] {  // once per process, thread-safe
         allow_lds(k_step_lds<false, true, true, true, 512>);
@@ -812,6 +838,7 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
     for (int d = 0; d < 3; d++) { a.rn[d] = rn[d]; a.r0[d] = r0[d]; }
     a.list = list;
     a.count = count;
+    a.hdr = hdr;
     if (cap % (kLanes * kChunk) != 0) return hipErrorInvalidValue;
     a.nchunk = cap / (kLanes * kChunk) + 1;
     a.rc2 = t.rc2;
@@ -874,7 +901,7 @@ RepGeo rep_geo(const RepSegs& rs, const int32_t* wl, const int32_t* wcount) {
 
 hipError_t launch_rep_nlist(hipStream_t s, const RepSegs& rs, const int32_t* wl, const int32_t* wcount,
                             double* const r[3], double rl2, int32_t cap, uint16_t* list, int32_t* count,
-                            int32_t* err, int32_t lds_atoms) {
+                            int32_t* err, int32_t lds_atoms, int32_t* hdr) {
     if (rs.nblocks <= 0) return hipSuccess;
     if (cap % (kLanes * kChunk) != 0) return hipErrorInvalidValue;
     if (lds_atoms < 1 || lds_atoms > kLdsAtoms) return hipErrorInvalidValue;
@@ -885,7 +912,7 @@ hipError_t launch_rep_nlist(hipStream_t s, const RepSegs& rs, const int32_t* wl,
     const int32_t all = (rs.nblocks + kXcd - 1) / kXcd * kXcd;
     const int32_t grid = all < 1024 ? all : 1024;
     k_nlist_lds<true, kBlock><<<grid, kBlock, dyn, s>>>(SegGrid{}, rep_geo(rs, wl, wcount), r[0], r[1], r[2], rl2, cap, list,
-                                                count, err, lds_atoms);
+                                                count, err, lds_atoms, hdr);
     return hipGetLastError();
 }
 
@@ -893,7 +920,7 @@ hipError_t launch_rep_step(hipStream_t s, const RepSegs& rs, const DevState& st,
                            double* const r0[3], const uint16_t* list, const int32_t* count, int32_t cap, double rc2,
                            double sig2, double coeff, double mass, double scale, const GearCoeffs& gc, int64_t cs,
                            int32_t precision, bool final_step, unsigned long long* dmax, int32_t* err,
-                           int32_t lds_atoms) {
+                           int32_t lds_atoms, const int32_t* hdr) {
     if (rs.nblocks <= 0) return hipSuccess;
     static const bool attr_set = [] {  // once per process, thread-safe
         allow_lds(k_step_lds<true, true, true, true, kBlock>);
@@ -915,6 +942,7 @@ hipError_t launch_rep_step(hipStream_t s, const RepSegs& rs, const DevState& st,
     for (int d = 0; d < 3; d++) { a.rn[d] = rn[d]; a.r0[d] = r0[d]; }
     a.list = list;
     a.count = count;
+    a.hdr = hdr;
     a.nchunk = cap / (kLanes * kChunk) + 1;
     a.rc2 = rc2;
     // coeff = 24 eps / sig^2: 48 eps sig^12 = 2 coeff sig^14, -24 eps sig^6 = -coeff sig^8

@@ -295,7 +295,11 @@ int32_t list_cap_multiple(); // the list capacity (entries per atom) must be a m
 // cap; err[2]: the largest staging (atoms), which sizes the step kernel's LDS
 hipError_t launch_nlist_lds(hipStream_t s, const DevState& st, const GridView& v, int32_t nseg, double rl2,
                             int32_t n, int32_t cap, uint16_t* list, int32_t* count, int32_t* err,
-                            int32_t lds_atoms, int32_t block);
+                            int32_t lds_atoms, int32_t block, int32_t* hdr);
+// the list build's per-segment headers, read by the step kernels: words per
+// segment, and the segments of a grid cut nseg times per column
+int32_t seg_header_words();
+int64_t seg_count(const GridView& v, int32_t nseg);
 // st, rn and r0 hold each vector's three components in one block, component d
 // at [0] + d * cs; clear_next (may be null): the next step's flag words, zeroed;
 // pub: the flag hand-over (blocks is set here); block: 512 or 1024 threads a
@@ -305,7 +309,8 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
                            double* const rn[3], double* const r0[3], const uint16_t* list, const int32_t* count,
                            int32_t cap, const LjTable& t, const MassTable& m, const GearCoeffs& gc, double hard2,
                            int32_t n, int64_t cs, int32_t precision, bool final_step, bool force_out, uint64_t* flags,
-                           int32_t lds_atoms, uint64_t* clear_next, FlagPublish pub, int32_t block);
+                           int32_t lds_atoms, uint64_t* clear_next, FlagPublish pub, int32_t block,
+                           const int32_t* hdr);
 
 // the lists of launch_nlist_lds as caller indices: row id[i] of rows (width
 // entries) holds atom i's partners in list order, counts[id[i]] their number;
@@ -369,12 +374,12 @@ struct RepSegs {
 // the lists of the segments wl[0 .. *wcount) (a fixed grid loops over them)
 hipError_t launch_rep_nlist(hipStream_t s, const RepSegs& rs, const int32_t* wl, const int32_t* wcount,
                             double* const r[3], double rl2, int32_t cap, uint16_t* list, int32_t* count,
-                            int32_t* err, int32_t lds_atoms);
+                            int32_t* err, int32_t lds_atoms, int32_t* hdr);
 hipError_t launch_rep_step(hipStream_t s, const RepSegs& rs, const DevState& st, double* const rn[3],
                            double* const r0[3], const uint16_t* list, const int32_t* count, int32_t cap, double rc2,
                            double sig2, double coeff, double mass, double scale, const GearCoeffs& gc, int64_t cs,
                            int32_t precision, bool final_step, unsigned long long* dmax, int32_t* err,
-                           int32_t lds_atoms);
+                           int32_t lds_atoms, const int32_t* hdr);
 
 // bond_ref.hip
 hipError_t launch_bond_ref(hipStream_t s, const DevState& st, const int32_t* slot_of_id,

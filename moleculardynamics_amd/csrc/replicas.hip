@@ -739,6 +739,7 @@ struct mdx_batch {
     double* d_ptmp = nullptr;             // permutation scratch, 15 x ntot
     int32_t* d_patom = nullptr;
     uint16_t* d_rs_list = nullptr;        // LDS-index lists of the staged replicas
+    int32_t* d_hdr = nullptr;             // the list build's segment headers, read by the step kernel
     int32_t* d_rs_count = nullptr;
     int32_t* d_rs_err = nullptr;          // staging overflow, list overflow, largest staging
     int32_t rs_cap = 0, rs_lds = 0;
@@ -906,7 +907,8 @@ int enqueue_rebuild(mdx_batch* b, bool flip, bool start = false) {
             BLAUNCH(b, "rep_worklist", (k_rep_worklist<<<1, kSortBlock, 0, s>>>(b->R, b->d_rebuild, b->d_sbeg,
                                                                                  b->d_wl, b->d_wcount)));
         BLAUNCH(b, "rep_nlist", launch_rep_nlist(s, rep_segs(b), b->d_wl, b->d_wcount, hp, rl2, b->rs_cap,
-                                                 b->d_rs_list, b->d_rs_count, b->d_rs_err, lds_capacity()));
+                                                 b->d_rs_list, b->d_rs_count, b->d_rs_err, lds_capacity(),
+                                                 b->d_hdr));
     }
     return MDX_OK;
 }
@@ -936,7 +938,8 @@ int enqueue_step(mdx_batch* b, bool flip, bool fin) {
         double* const* nxt = flip ? b->r : b->rn;
         BLAUNCH(b, "rep_step_lds", launch_rep_step(s, rep_segs(b), st, nxt, b->r0, b->d_rs_list, b->d_rs_count,
                                                    b->rs_cap, a.rc2, a.sig2, a.coeff, a.mass, a.scale, b->gc, b->ntot,
-                                                   b->cfg.precision, fin, b->d_dmax, b->d_rs_err, b->rs_lds));
+                                                   b->cfg.precision, fin, b->d_dmax, b->d_rs_err, b->rs_lds,
+                                                   b->d_hdr));
     }
     if (b->nsmall_blocks > 0) {
         const size_t dyn = (size_t)b->small_lds;
@@ -1144,6 +1147,7 @@ int mdx_batch_create(const mdx_config* common, int32_t n_replicas, const int32_t
             (rc = balloc(b, &b->d_wcount, 1)) ||
             (rc = balloc(b, &b->d_ptmp, 15 * (size_t)n)) || (rc = balloc(b, &b->d_patom, n)) ||
             (rc = balloc(b, &b->d_rs_list, (size_t)list_entries(n, b->rs_cap))) ||
+            (rc = balloc(b, &b->d_hdr, (size_t)b->nseg_blocks * seg_header_words())) ||
             (rc = balloc(b, &b->d_rs_count, n)) || (rc = balloc(b, &b->d_rs_err, 4))) {
             g_batch_err = b->err;
             return bail(rc);

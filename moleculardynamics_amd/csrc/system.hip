@@ -88,6 +88,8 @@ struct mdx_system {
     int32_t lds_atoms = 0;           // largest segment staging of the last build
     int32_t build_lds = 0;           // staging capacity the list build runs with
     int32_t box_block = 512;         // threads a workgroup of the LDS kernels (512 or 1024)
+    int32_t* seg_hdr = nullptr;      // the list build's per-segment headers for the step kernels
+    int64_t seg_hdr_cap = 0;         // words allocated
     mdx_shape shape{};               // launch shape (box_block and the segments' fill come from it)
     // periodic Tersoff silicon: compact lists [k][slot] and per-bond derivatives
     TersoffParams tp{};
@@ -639,6 +641,13 @@ int rebuild_periodic(mdx_system* s) {
     auto round64 = [](double a) { return ((int32_t)std::ceil(a) + 63) / 64 * 64; };
     for (;;) {
         s->nseg = lds_segments_per_column(v.g[2], per_cell, s->seg_kmax, s->box_block, s->shape.fill);
+        const int64_t hw = seg_count(v, s->nseg) * seg_header_words();
+        if (hw > s->seg_hdr_cap) {
+            dfree(s, s->seg_hdr);
+            s->seg_hdr = nullptr;
+            if ((rc = dalloc(s, &s->seg_hdr, (size_t)hw))) return rc;
+            s->seg_hdr_cap = hw;
+        }
         if (s->build_lds == 0) {
             // staging estimate: 9 columns of K + 2 cells at 1.3 x the mean occupancy
             const int32_t kcells = (v.g[2] + s->nseg - 1) / s->nseg;
@@ -646,7 +655,7 @@ int rebuild_periodic(mdx_system* s) {
         }
         HIPCHK(s, hipMemsetAsync(s->dint + 4, 0, 3 * sizeof(int32_t), s->stream));
         LAUNCH(s, "nlist_build", launch_nlist_lds(s->stream, s->st, v, s->nseg, rl2, n, s->nl_cap, s->nlist,
-                                                  s->ncount, s->dint + 4, s->build_lds, s->box_block));
+                                                  s->ncount, s->dint + 4, s->build_lds, s->box_block, s->seg_hdr));
         int32_t e[3];
         if ((rc = read_ints(s, e, s->dint + 4, 3))) return rc;
         s->lds_atoms = std::min(lds_capacity(), round64(e[2]));
@@ -805,7 +814,8 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
                 LAUNCH(s, "md_step", launch_step_lds(st, s->st, s->lj.view, s->nseg, s->rnext, s->rref, s->nlist,
                                                      s->ncount, s->nl_cap, s->ljt, s->mt, s->gc, hard2, n,
                                                      component_stride(n), c.precision, fin, fin, slot_flags(k),
-                                                     s->lds_atoms, slot_flags(k + 1), publish(k), s->box_block));
+                                                     s->lds_atoms, slot_flags(k + 1), publish(k), s->box_block,
+                                                     s->seg_hdr));
                 rec[k % 4] = s->timing ? (int64_t)s->pending.size() - 1 : -1;
                 for (int d = 0; d < 3; d++) std::swap(s->st.r[d], s->rnext[d]);
                 s->t++;
