@@ -7,8 +7,8 @@ libmdx.so, which mdx_create and mdx_batch_create read).  For each size bucket
 of each kind it builds one probe system and times every candidate shape on it:
 
   * kind "lj" (the periodic Lennard-Jones box, FCC at density 0.8442, T* 0.728,
-    rc 2.5, rs 2.8): workgroups of 512 or 1024 threads x segment fill 0.7,
-    0.85 or 1.0 of one pass.  Every candidate starts from the same state and
+    rc 2.5, rs 2.8): workgroups of 512 threads with segment fill 0.7, 0.75,
+    0.8, 0.85, 0.9 or 1.0 of one pass, or 1024 threads with 0.7, 0.85 or 1.0.  Every candidate starts from the same state and
     the candidates advance in lockstep; the shape does not change any atom's
     force, so they follow the same trajectory with the same list rebuilds.
   * kind "replicas" (512 LJ replicas of 256-8192 atoms, the batch benchmark's
@@ -42,7 +42,8 @@ DT = 0.005
 
 # (atoms_min, atoms_max, FCC cells per edge of the probe)
 LJ_BUCKETS = [(0, 131072, 20), (131072, 2097152, 64), (2097152, 2 ** 31 - 1, 100)]
-LJ_CANDIDATES = [{"block": b, "fill": f} for b in (512, 1024) for f in (0.7, 0.85, 1.0)]
+LJ_CANDIDATES = ([{"block": 512, "fill": f} for f in (0.7, 0.75, 0.8, 0.85, 0.9, 1.0)] +
+                 [{"block": 1024, "fill": f} for f in (0.7, 0.85, 1.0)])
 REPLICA_CANDIDATES = [{"stage_aim": a, "fill": f} for a in (900.0, 1250.0, 1600.0) for f in (0.7, 0.85)]
 DEFAULT = {"block": 512, "fill": 0.85, "stage_aim": 1250.0}
 
