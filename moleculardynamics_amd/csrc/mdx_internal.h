@@ -64,7 +64,11 @@ __device__ inline double lane_double(double v, int l) {
 // a slot of pinned host memory and then writes the step's sequence number
 // after them, so a host that sees the number sees the words.  Each counter is
 // reset by its last arrival for the next launch.  Two levels keep a kernel's
-// workgroups, which may all finish at once, from queueing on one address.
+// workgroups, which may all finish at once, from queueing on one address, and
+// each counter has a 256-byte line of its own (kPubStride words apart):
+// device-scope atomics on one line are served one after another, so counters
+// packed into one line queue as if they were one (at 1M LJ atoms, 10,112
+// arrivals per step on one line made the step kernel 206 us instead of 175).
 // The flags are only touched by device-scope atomics and every access here is
 // an atomic that waits for the previous ones, so no fence (which would write
 // back the XCD's L2 after a step kernel's stores) is needed.
@@ -77,9 +81,10 @@ __device__ inline double lane_double(double v, int l) {
 // it leaves the gate set, so the launch queued after it halts too, and hands
 // over flag words marked with kHaltMark (publish_halt).
 constexpr int kPubSub = 32;
+constexpr int kPubStride = 64;
 constexpr unsigned long long kHaltMark = 1ull << 62;
 struct FlagPublish {
-    unsigned int* done;         // kPubSub + 1 device counters, 0 between launches
+    unsigned int* done;         // kPubSub + 1 device counters kPubStride apart, 0 between launches
     unsigned long long* host;   // flag words, then the sequence number
     unsigned long long seq;
     int32_t blocks;             // workgroups that take part, numbered 0 .. blocks - 1
@@ -101,9 +106,9 @@ __device__ inline void publish_flags(const FlagPublish& p, int32_t id, const uns
         const unsigned int n_sub = (unsigned int)(p.blocks / kPubSub + (sub < p.blocks % kPubSub ? 1 : 0));
         const unsigned int n_top = (unsigned int)(p.blocks < kPubSub ? p.blocks : kPubSub);
         int32_t l = 0;
-        if (__hip_atomic_fetch_add(p.done + sub, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u == n_sub) {
-            __hip_atomic_store(p.done + sub, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            l = __hip_atomic_fetch_add(p.done + kPubSub, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u ==
+        if (__hip_atomic_fetch_add(p.done + sub * kPubStride, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u == n_sub) {
+            __hip_atomic_store(p.done + sub * kPubStride, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            l = __hip_atomic_fetch_add(p.done + kPubSub * kPubStride, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u ==
                 n_top;
         }
         last = l;
@@ -121,7 +126,7 @@ __device__ inline void publish_flags(const FlagPublish& p, int32_t id, const uns
     if (threadIdx.x == 0) {
         if (p.gate)
             __hip_atomic_store(p.gate, dmax >= p.gate_bits ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(p.done + kPubSub, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(p.done + kPubSub * kPubStride, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(p.host + words, p.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }

@@ -113,7 +113,7 @@ struct mdx_system {
     // number of the step whose words they are (written by the step kernels)
     static constexpr int kHostWords = kFlagWords + 1;
     uint64_t* hflags = nullptr;
-    unsigned int* dpub = nullptr;    // the step kernels' arrival counters (FlagPublish)
+    unsigned int* dpub = nullptr;    // the step kernels' arrival counters, then the list gate (FlagPublish)
     uint64_t pub_seq = 0;            // last sequence number handed out
     hipEvent_t ev[kRing] = {nullptr, nullptr, nullptr};
     bool timing = false;
@@ -749,7 +749,7 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
     // (for the short Tersoff and surrogate step kernels the hand-over at
     // their end costs as much as the copy it saves).
     uint64_t expect[mdx_system::kRing] = {0, 0, 0};   // per slot: sequence number awaited (0: event)
-    unsigned int* const gate = s->dpub + kPubSub + 1;
+    unsigned int* const gate = s->dpub + (kPubSub + 1) * kPubStride;
     uint64_t gate_bits;
     std::memcpy(&gate_bits, &hard2, 8);
     auto publish = [&](int k) {
@@ -789,7 +789,7 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
     HIPCHK(s, hipMemsetAsync(slot_flags(0), 0, W * sizeof(uint64_t), st));
     // the hand-over's arrival counters and the list gate start each call at
     // zero, whatever an earlier failed launch left in them
-    HIPCHK(s, hipMemsetAsync(s->dpub, 0, sizeof(unsigned int) * (kPubSub + 2), st));
+    HIPCHK(s, hipMemsetAsync(s->dpub, 0, sizeof(unsigned int) * (kPubSub + 2) * kPubStride, st));
     LAUNCH(s, "predict", launch_predict_track(st, s->st, n, s->gc, s->rref, slot_flags(-1)));
     if ((rc = post(-1))) return rc;
     if (!ts) {
@@ -978,7 +978,7 @@ int mdx_create(const mdx_config* cfg, mdx_system** out) {
     s->nb = kinetics_blocks(n);
     if ((rc = dalloc(s, &s->partials, 8 * (size_t)s->nb + 16)) ||
         (rc = dalloc(s, &s->dflags, (size_t)mdx_system::kFlagWords * mdx_system::kRing)) ||
-        (rc = dalloc(s, &s->dpub, kPubSub + 2))) {
+        (rc = dalloc(s, &s->dpub, (kPubSub + 2) * kPubStride))) {
         g_err = s->err;
         return bail(rc);
     }
@@ -1001,7 +1001,7 @@ int mdx_create(const mdx_config* cfg, mdx_system** out) {
                       hipHostMallocDefault) != hipSuccess)
         return bail(fail(nullptr, MDX_ERR_ALLOC, "hipHostMalloc failed"));
     std::memset(s->hflags, 0, sizeof(uint64_t) * mdx_system::kHostWords * mdx_system::kRing);
-    if (hipMemsetAsync(s->dpub, 0, sizeof(unsigned int) * (kPubSub + 2), s->stream) != hipSuccess)
+    if (hipMemsetAsync(s->dpub, 0, sizeof(unsigned int) * (kPubSub + 2) * kPubStride, s->stream) != hipSuccess)
         return bail(fail(nullptr, MDX_ERR_HIP, "init failed"));
     for (int k = 0; k < mdx_system::kRing; k++)
         if (hipEventCreateWithFlags(&s->ev[k], hipEventDisableTiming) != hipSuccess)
