@@ -47,13 +47,17 @@ namespace {
 constexpr int kBlock = 512;
 constexpr int kLanes = 4;                  // lanes per atom
 constexpr int kApp = kBlock / kLanes;      // atoms per pass (the replicas' workgroups)
-constexpr int kChunk = 8;                  // 16-bit entries per 16-byte chunk
+constexpr int kChunk = 4;                  // 16-bit entries per 8-byte chunk
 constexpr int kScan = 8;                   // candidates per lane per scan step
 constexpr int kScanSlack = kLanes * kScan; // staged slots past the last atom the scan may read
 constexpr int kListBlock = 16;             // atoms per list block
-constexpr int kChunkRow = kListBlock * kLanes * kChunk;   // 16-bit entries in one chunk of a block (512)
-constexpr int kRowEntries = kLanes * kChunk;               // entries in one row of an atom (32)
-constexpr int kRing = 2 * kRowEntries;                     // the list build's LDS ring per quad (two rows)
+constexpr int kChunkRow = kListBlock * kLanes * kChunk;   // 16-bit entries in one chunk of a block (256)
+constexpr int kRowEntries = kLanes * kChunk;               // entries in one row of an atom (16)
+// The list build's LDS ring per quad, four rows: a scan step adds at most
+// kLanes x kScan = 32 hits to a partly filled row, so it completes at most two
+// rows and starts a third.
+constexpr int kRing = 4 * kRowEntries;
+static_assert(kLanes * kScan <= 2 * kRowEntries, "a scan step completes at most two rows");
 constexpr int kXcd = 8;
 constexpr int kLdsAtoms = 3072;            // staged atoms per segment at most (3 x 24 KiB of FP64)
 constexpr int kMaxK = 14;                  // cells per segment
@@ -364,7 +368,7 @@ __device__ __forceinline__ void nlist_segment(Stage& S, double* dyn, const SegGr
     if (threadIdx.x == 0) atomicMax(&err[2], S.h.total + 1);
     const int32_t hbeg = S.h.hbeg, hend = S.h.hend;
     const int32_t q = threadIdx.x & (kLanes - 1);
-    // this quad's two-row ring; lane q's chunk of a row is 16 bytes at entry 8q
+    // this quad's four-row ring; lane q's chunk of a row is 8 bytes at entry 4q
     uint16_t* const ring = reinterpret_cast<uint16_t*>(reinterpret_cast<char*>(dyn) + ring_offset(lds_cap)) +
                            (threadIdx.x / kLanes) * kRing;
     const uint16_t dummy = (uint16_t)S.h.total;
@@ -376,7 +380,7 @@ __device__ __forceinline__ void nlist_segment(Stage& S, double* dyn, const SegGr
         const int32_t self = S.h.colbase[4] + S.cellpre[4][1] + qc;
         const float xi = P.x(self), yi = P.y(self), zi = P.z(self);
         // lane q's chunk of row r of atom s
-        uint4* const la = reinterpret_cast<uint4*>(list + ((int64_t)(s / kListBlock) * nchunk * kListBlock + s % kListBlock) *
+        uint2* const la = reinterpret_cast<uint2*>(list + ((int64_t)(s / kListBlock) * nchunk * kListBlock + s % kListBlock) *
                                                               kRowEntries) + q;
         // row r of the ring to lane q's chunk in the list (rows past the
         // capacity are dropped; the count reports the overflow).  The ring was
@@ -385,7 +389,9 @@ __device__ __forceinline__ void nlist_segment(Stage& S, double* dyn, const SegGr
         // the read above those writes.
         auto flush = [&](int32_t r) {
             asm volatile("" ::: "memory");
-            if (r < rows) la[(int64_t)r * (kChunkRow / kChunk)] = *reinterpret_cast<const uint4*>(ring + (r & 1) * kRowEntries + q * kChunk);
+            if (r < rows)
+                la[(int64_t)r * (kChunkRow / kChunk)] =
+                    *reinterpret_cast<const uint2*>(ring + (r & (kRing / kRowEntries - 1)) * kRowEntries + q * kChunk);
             asm volatile("" ::: "memory");
         };
         int32_t k = 0;                        // hits so far, the same in all 4 lanes
@@ -423,14 +429,18 @@ __device__ __forceinline__ void nlist_segment(Stage& S, double* dyn, const SegGr
                     if (ds >= 0 && ds < kLanes * kScan && ds % kLanes == 0) mask &= ~(1u << (ds / kLanes));
                 }
                 // hits are numbered lane-major within the quad; at most 32 of
-                // them, so they complete at most one row
+                // them, so they complete at most two rows
                 const int32_t h = __popc(mask);
                 int32_t total;
                 int32_t kk = k + quad_excl(h, q, total);
                 for (uint32_t mb = mask; mb; mb &= mb - 1, kk++)
                     ring[kk & (kRing - 1)] = (uint16_t)(p0 + kLanes * __builtin_ctz(mb) + q);
                 const int32_t kn = k + total;
-                if ((kn ^ k) >= kRowEntries) flush(k / kRowEntries);   // row k / 32 is complete
+                const int32_t r0 = k / kRowEntries, r1 = kn / kRowEntries;   // rows r0 .. r1 - 1 are complete
+                if (r1 != r0) {
+                    flush(r0);
+                    if (r1 != r0 + 1) flush(r0 + 1);
+                }
                 k = kn;
             }
         }
@@ -443,7 +453,7 @@ __device__ __forceinline__ void nlist_segment(Stage& S, double* dyn, const SegGr
         // evaluates whole chunks without a validity test
         if (k % kRowEntries != 0) {
             const int32_t r = k / kRowEntries, used = k % kRowEntries;
-            uint16_t* const row = ring + (r & 1) * kRowEntries + q * kChunk;
+            uint16_t* const row = ring + (r & (kRing / kRowEntries - 1)) * kRowEntries + q * kChunk;
 #pragma unroll
             for (int u = 0; u < kChunk; u++)
                 if (q * kChunk + u >= used) row[u] = dummy;
@@ -546,6 +556,16 @@ __device__ inline double pick3(int32_t q, double a, double b, double c) { return
 
 // 8 waves per SIMD (64 VGPRs) for the mixed body; the FP64 body's divisions
 // need more registers, so it runs at 4
+// An 8-byte chunk of the pair list, loaded with the non-temporal hint: the list
+// streams through once per step (about 170 MB at 1M atoms), and plain loads
+// would push out of the XCD's L2 the positions that the segments of the
+// neighbouring columns stage again soon after (md_step 175 -> 168 us at 1M atoms).
+__device__ inline uint2 load_list_chunk(const uint2* p) {
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    const u32x2 t = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p));
+    return make_uint2(t.x, t.y);
+}
+
 template <bool kRep, bool kFp64, bool kFinal, bool kForceOut, int kB>
 __global__ void __launch_bounds__(kB, kFp64 ? 4 : 8)
 k_step_lds(StepArgs a) {
@@ -588,17 +608,17 @@ k_step_lds(StepArgs a) {
     const int32_t q = threadIdx.x & (kLanes - 1);
     // the first pass's list count and first chunk, loaded before the staging so
     // that their latency passes while it runs
-    constexpr int32_t cstride = kListBlock * kLanes;   // uint4 between a lane's chunks
+    constexpr int32_t cstride = kListBlock * kLanes;   // chunks between a lane's rows
     auto list_of = [&](int32_t i) {
-        return reinterpret_cast<const uint4*>(a.list) +
+        return reinterpret_cast<const uint2*>(a.list) +
                ((int64_t)(i / kListBlock) * a.nchunk * kListBlock + i % kListBlock) * kLanes + q;
     };
     const int32_t i_first = hbeg + (int32_t)threadIdx.x / kLanes;
     int32_t count_first = 0;
-    uint4 chunk_first = make_uint4(0u, 0u, 0u, 0u);
+    uint2 chunk_first = make_uint2(0u, 0u);
     if (i_first < hend) {
         count_first = a.count[i_first];
-        chunk_first = list_of(i_first)[0];
+        chunk_first = load_list_chunk(list_of(i_first));
     }
     if (!stage_fill<kB>(S, P, g, cx, cy, a.st.r[0], a.st.r[1], a.st.r[2])) {
         // cannot happen after a successful list build on the same cells: the
@@ -619,23 +639,23 @@ k_step_lds(StepArgs a) {
         const double xi = P.x(self), yi = P.y(self), zi = P.z(self);
         double fx = 0, fy = 0, fz = 0;
         const bool first = base == hbeg;
-        const int32_t m = first ? count_first : a.count[i];   // the atom's entries, 32 to a row
-        // lane q takes its 16-byte chunk of every row; rows are 16 x 4 uint4
+        const int32_t m = first ? count_first : a.count[i];   // the atom's entries, 16 to a row
+        // lane q takes its 8-byte chunk of every row; rows are 16 x 4 chunks
         // apart; the next row's chunk is loaded while the current one is
         // evaluated (the list has one spare row, so the look-ahead stays in
         // bounds); the last row is padded with the dummy atom
-        const uint4* lp = list_of(i);
-        uint4 nx = first ? chunk_first : lp[0];
+        const uint2* lp = list_of(i);
+        uint2 nx = first ? chunk_first : load_list_chunk(lp);
         for (int32_t k0 = 0; k0 < m; k0 += kRowEntries) {
-            const uint4 cu = nx;
+            const uint2 cu = nx;
             lp += cstride;
             // the empty asm hides that this address is the next iteration's
             // chunk, so the compiler keeps the load here (one chunk ahead)
             // instead of merging it into a load at the top of the next iteration
             uint64_t pa = reinterpret_cast<uint64_t>(lp);
             asm volatile("" : "+v"(pa));
-            nx = *reinterpret_cast<const uint4*>(pa);
-            const uint32_t wv[4] = {cu.x, cu.y, cu.z, cu.w};
+            nx = load_list_chunk(reinterpret_cast<const uint2*>(pa));
+            const uint32_t wv[2] = {cu.x, cu.y};
             // straight-line body over the whole chunk: entries past the lane's
             // count are the dummy atom, which fails the cutoff test (a select,
             // so the 1/0 of its force law never reaches the sum)
