@@ -18,14 +18,16 @@
 //
 //  k_nlist_lds (at each rebuild): the quad scans the atom's candidates, 32 at
 //    a time, and keeps those within the list radius rs.  Entries are 16-bit
-//    LDS indices, numbered in the quad's scan order and grouped 32 to a row:
-//    entry k is entry k % 8 of lane (k / 8) % 4's 16-byte chunk in row k / 32.
-//    The hits go to a two-row ring in LDS first, and each lane writes its
-//    16-byte chunk of a row once the row is complete (the last row padded
-//    with the dummy atom).  Atoms are grouped 16 to a block; block B holds its
-//    rows c = 0 .. NC-1 one after the other, each as [16 atoms][4 lanes]
-//    16-byte chunks (1 KiB), so a wave's chunk loads cover one or two
-//    contiguous runs and an atom's rows are 1 KiB apart.
+//    LDS indices, numbered in the quad's scan order and grouped 16 to a row:
+//    entry k is entry k % 4 of lane (k / 4) % 4's 8-byte chunk in row k / 16
+//    (rows of 16 waste less on the padded last row than rows of 32: an LJ
+//    liquid atom has 70-85 entries, 5 or 6 rows).  The hits go to a four-row
+//    ring in LDS first, and each lane writes its 8-byte chunk of a row once
+//    the row is complete (the last row padded with the dummy atom).  Atoms are
+//    grouped 16 to a block; block B holds its rows c = 0 .. NC-1 one after the
+//    other, each as [16 atoms][4 lanes] 8-byte chunks (512 B), so a wave's
+//    chunk loads cover one or two contiguous runs and an atom's rows are
+//    512 B apart.
 //  k_step_lds (every step): LJ force over the list (FP64 displacement, FP32
 //    12-6 force law in the mixed mode, FP64 accumulation), summed over the
 //    quad, then the Gear corrector of step t and the predictor of step t+1,
