@@ -211,6 +211,12 @@ struct Stage {
     int32_t cstart[9][kMaxK + 2];       // per column and window cell, first slot (list build)
     unsigned long long red[3];          // step kernel: max disp^2 bits, max step^2 bits, stale count
 };
+// The step kernel's LDS header: the segment header and the block's flag
+// reduction (without the list build's cell tables, 1.2 KiB less per workgroup)
+struct StepStage {
+    SegHdr h;
+    unsigned long long red[3];          // max disp^2 bits, max step^2 bits, stale count
+};
 
 // The staging layout of segment (cx, cy, z0..z1): per column its LDS start,
 // the prefix counts of its window cells and the slot ranges of its copy
@@ -274,8 +280,8 @@ __device__ inline int32_t staged_slot(const Stage& S, int32_t w) {
 // Fill the staging of segment (cx, cy, ...) whose header is S: positions
 // minus the origin o (zero for the FP64 staging: the values are then exactly
 // the images).  Returns false (for the whole block) when it does not fit in LDS.
-template <int kB, typename PosT>
-__device__ inline bool stage_fill(Stage& S, const PosT& P, const SegGrid& g, int32_t cx, int32_t cy,
+template <int kB, typename StageT, typename PosT>
+__device__ inline bool stage_fill(StageT& S, const PosT& P, const SegGrid& g, int32_t cx, int32_t cy,
                                   const double* __restrict__ rx, const double* __restrict__ ry,
                                   const double* __restrict__ rz, double ox = 0.0, double oy = 0.0, double oz = 0.0) {
     const int32_t tid = threadIdx.x;
@@ -438,7 +444,9 @@ __device__ __forceinline__ void nlist_segment(Stage& S, double* dyn, const SegGr
                 for (uint32_t mb = mask; mb; mb &= mb - 1, kk++)
                     ring[kk & (kRing - 1)] = (uint16_t)(p0 + kLanes * __builtin_ctz(mb) + q);
                 const int32_t kn = k + total;
-                const int32_t r0 = k / kRowEntries, r1 = kn / kRowEntries;   // rows r0 .. r1 - 1 are complete
+                // rows r0 .. r1 - 1 are complete (unsigned: the counts are never
+                // negative, and the division becomes a shift)
+                const int32_t r0 = (int32_t)((uint32_t)k / kRowEntries), r1 = (int32_t)((uint32_t)kn / kRowEntries);
                 if (r1 != r0) {
                     flush(r0);
                     if (r1 != r0 + 1) flush(r0 + 1);
@@ -571,7 +579,7 @@ __device__ inline uint2 load_list_chunk(const uint2* p) {
 template <bool kRep, bool kFp64, bool kFinal, bool kForceOut, int kB>
 __global__ void __launch_bounds__(kB, kFp64 ? 4 : 8)
 k_step_lds(StepArgs a) {
-    __shared__ Stage S;
+    __shared__ StepStage S;
     extern __shared__ double dyn[];
     SegGrid g = a.g;
     if (blockIdx.x == 0 && a.clear)
