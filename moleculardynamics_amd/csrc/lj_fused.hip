@@ -544,7 +544,7 @@ struct StepArgs {
     const uint16_t* list;
     const int32_t* count;
     const int32_t* hdr;     // segment headers (kHdrWords each) written by the list build
-    int32_t nchunk;         // list chunks per lane (capacity / 32 + the spare)
+    int32_t nchunk;         // list chunks per lane (capacity / 16 + the spare)
     double rc2;
     float f12, f6;          // F/r = x^4 (f12 x^3 + f6), x = 1/r^2: 48 eps sig^12, -24 eps sig^6
     double sig2, coeff;
@@ -564,8 +564,6 @@ struct StepArgs {
 
 __device__ inline double pick3(int32_t q, double a, double b, double c) { return q == 0 ? a : (q == 1 ? b : c); }
 
-// 8 waves per SIMD (64 VGPRs) for the mixed body; the FP64 body's divisions
-// need more registers, so it runs at 4
 // An 8-byte chunk of the pair list, loaded with the non-temporal hint: the list
 // streams through once per step (about 170 MB at 1M atoms), and plain loads
 // would push out of the XCD's L2 the positions that the segments of the
@@ -576,6 +574,8 @@ __device__ inline uint2 load_list_chunk(const uint2* p) {
     return make_uint2(t.x, t.y);
 }
 
+// 8 waves per SIMD (64 VGPRs) for the mixed body; the FP64 body's divisions
+// need more registers, so it runs at 4
 template <bool kRep, bool kFp64, bool kFinal, bool kForceOut, int kB>
 __global__ void __launch_bounds__(kB, kFp64 ? 4 : 8)
 k_step_lds(StepArgs a) {
