@@ -73,7 +73,7 @@ def parse():
     p.add_argument("--ncell", type=int, default=64, help="FCC cells per edge (4 ncell^3 atoms)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     p.add_argument("--no-sub", action="store_true", help="headline only")
-    p.add_argument("--workload", choices=["lj", "tersoff", "replicas", "surrogate"], default="lj")
+    p.add_argument("--workload", choices=["lj", "tersoff", "replicas", "surrogate", "sweep"], default="lj")
     p.add_argument("--sub-steps", type=int, default=0,
                    help="steps of the subrecords (0: their configurations' 500 / 500 / 2000)")
     p.add_argument("--weights", default=None, help="surrogate network (default: the shipped one)")
@@ -441,6 +441,87 @@ def bench_replicas(a, steps: int) -> dict:
     return out
 
 
+# ---------------- the vxcm sweep of a walled N/O/Ne gas ----------------
+SWEEP_VXCM = [-0.003 + 0.001 * k for k in range(8)]   # VXCMFROM -0.003, VXCMTO 0.004, VXCMSTEP 0.001
+
+
+def sweep_config(n: int, n_bond: int) -> dict:
+    from moleculardynamics_amd import BOND_REFERENCE, BOX_WALLED, LJ_REFERENCE, PREC_FP64
+    return dict(n_atoms=n, n_bond=n_bond, box=BOX_WALLED, precision=PREC_FP64, use_lj=1, use_bond=1,
+                lj_table=LJ_REFERENCE, bond_form=BOND_REFERENCE, dt=1.0, lj_rs=7.6, lj_rcut=6.8, mb_rs=3.6,
+                mb_rcut=3.0, energy_loss=1.0, wall=n ** (1 / 3) * 52.8)
+
+
+def cpu_baseline_sweep(pos4, vel, n_bond: int, steps: int) -> dict:
+    """The FP64 oracle's run of the reference loop (cpu/cpumd/Host.cpp:141-195)
+    for the sweep's first member."""
+    O = oracle()
+    n = len(pos4)
+    _, _, _, el = O.ref_run(pos4, vel, n - n_bond, n_bond, steps, 10 ** 9, 1.0, 7.6, 6.8, 3.6, 3.0,
+                            vxcm=float(np.float32(SWEEP_VXCM[0])), use_lj=True, use_mb=True)
+    return {"value": n * steps / el, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+            "sample": f"the first member ({n} atoms, vxcm {SWEEP_VXCM[0]}), {steps} steps of the reference loop, "
+                      f"{el:.1f} s on one thread", **host_facts(), "compiler": O.build_info()}
+
+
+def bench_sweep(a, steps: int) -> dict:
+    from moleculardynamics_amd import Sweep, System, initial
+
+    n_mol, n_ne = 30, 4000
+    pos4, vel = initial.walled_mixture(n_mol, n_ne, 300.0, seed=7)
+    n, nb, R = len(pos4), 2 * n_mol, len(SWEEP_VXCM)
+    cfg = sweep_config(n, nb)
+    starts = []
+    for vx in SWEEP_VXCM:
+        v = vel.copy()
+        v[:, 0] += float(np.float32(vx))
+        starts.append(v)
+    log(f"vxcm sweep: {R} members of {n} atoms ({nb} bond order), warmup {a.warmup}, steps {steps}")
+    # the members one after another, as the reference runs them (one system each)
+    seq = 0.0
+    seq_builds = 0
+    for v in starts:
+        with System(**cfg) as s:
+            s.set_state(pos4, v)
+            s.run(a.warmup)
+            seq += timed_run(s, steps)
+            seq_builds += s.build_count()
+    w = Sweep(R, **cfg)
+    for m, v in enumerate(starts):
+        w.member(m).set_state(pos4, v)
+    w.run(a.warmup)
+    b0 = sum(w.member(m).build_count() for m in range(R))
+    w.member(0).synchronize()
+    t0 = time.perf_counter()
+    w.run(steps)
+    w.member(0).synchronize()
+    elapsed = time.perf_counter() - t0
+    builds = sum(w.member(m).build_count() for m in range(R)) - b0
+    w.set_timing(True)
+    w.run(min(steps, 100))
+    times = w.member(0).kernel_times()
+    w.set_timing(False)
+    w.close()
+    times = {k: (v[0], v[1] / max(v[0], 1)) for k, v in times.items()}
+    log(f"sweep {elapsed:.4f} s ({builds} member list builds), one after another {seq:.4f} s; kernels {times}")
+    out = record(R * n * steps / elapsed, steps, a.warmup, elapsed, "fp64",
+                 "synthetic (seeded N2/O2/NO + Ne gas on a jittered grid, Maxwell-Boltzmann velocities at 300 K)",
+                 {"workload": f"vxcm sweep of {R} members (vxcm -0.003 .. 0.004 step 0.001), walled N/O/Ne gas of "
+                              f"{n} atoms ({nb} bond order), LJ rs 7.6 rc 6.8, bond order rs 3.6 rc 3.0, dt 1, walls "
+                              f"at N^(1/3) 52.8", "atoms": n, "members": R, "member_list_builds": builds})
+    out["metric"] = "atom-steps/s per GPU (vxcm sweep of a walled N/O/Ne gas, FP64)"
+    out["one_after_another"] = {"value": R * n * steps / seq, "ms_per_step_all_members": 1e3 * seq / steps,
+                                "member_list_builds": seq_builds, "speedup_of_batch": seq / elapsed}
+    step_kernels = ["sweep_predict", "sweep_lj", "sweep_bond", "sweep_correct", "sweep_accelerations"]
+    out["roofline"] = roofline(step_kernels, times, R * n, None,
+                               "the batched step's kernels (predict, LJ, bond order, corrector, gated a = F/m)")
+    out["kernel_ms"] = {k: v[1] for k, v in times.items()}
+    if not a.no_cpu:
+        log("cpu baseline (sweep) ...")
+        out["cpu_baseline"] = cpu_baseline_sweep(pos4, vel, nb, 2000)
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -449,7 +530,8 @@ def main():
     if a.copy_bytes:
         copy_calibration(a.copy_bytes)
         return
-    sub = {"tersoff": a.sub_steps or 500, "surrogate": a.sub_steps or 500, "replicas": a.sub_steps or 2000}
+    sub = {"tersoff": a.sub_steps or 500, "surrogate": a.sub_steps or 500, "replicas": a.sub_steps or 2000,
+           "sweep": a.sub_steps or 1000}
     if a.workload == "tersoff":
         print(json.dumps(bench_tersoff(a, a.steps)))
         return
@@ -459,11 +541,15 @@ def main():
     if a.workload == "replicas":
         print(json.dumps(bench_replicas(a, a.steps)))
         return
+    if a.workload == "sweep":
+        print(json.dumps(bench_sweep(a, a.steps)))
+        return
     out = bench_lj(a)
     if not a.no_sub:
         out["subrecords"] = {"tersoff_262k": bench_tersoff(a, sub["tersoff"]),
                              "surrogate_vs_plain_262k": bench_surrogate(a, sub["surrogate"]),
-                             "replicas_512": bench_replicas(a, sub["replicas"])}
+                             "replicas_512": bench_replicas(a, sub["replicas"]),
+                             "vxcm_sweep_8": bench_sweep(a, sub["sweep"])}
     print(json.dumps(out))
 
 

@@ -28,6 +28,7 @@
  *                                                            mdx_kinetics
  *   calculatePotentional          common/DebugCalculations.cpp:129-270
  *                                                            mdx_potential_allpairs
+ *   the vxcm sweep                cpu/cpumd/Host.cpp:75-216  mdx_sweep_*
  *
  * Arrays crossing the boundary use the reference's host layout: positions as
  * real4 (x, y, z, species code in w) and v/a/b/c/f as real3, all double
@@ -36,8 +37,9 @@
  * in place of the reference's print-and-exit (common/Global.h:116-117);
  * mdx_last_error() gives the message.  A system is single-threaded: calls on
  * one system must not overlap.  Different systems are independent, each with
- * its own HIP stream, and may be driven from different host threads at once
- * (the cpumd driver runs vxcm sweep members that way).
+ * its own HIP stream, and may be driven from different host threads at once.
+ * The members of a sweep (mdx_sweep_*) share the sweep's stream: calls on one
+ * sweep and on its members must not overlap.
  */
 #ifndef MDX_H
 #define MDX_H
@@ -333,6 +335,34 @@ int mdx_batch_set_timing(mdx_batch* batch, int32_t enabled);
 int mdx_batch_kernel_times(mdx_batch* batch, char* names, int64_t* counts, double* total_ms,
                            int32_t cap, int32_t* n);
 int mdx_batch_synchronize(mdx_batch* batch);
+
+/*
+ * The vxcm sweep as one batch.  The reference runs its sweep's members one
+ * after another (cpu/cpumd/Host.cpp:75-216): the same walled system, re-read
+ * for each centre-of-mass velocity offset, through the whole step loop.  A
+ * sweep holds `members` walled systems of one configuration (cfg as for
+ * mdx_create, box MDX_BOX_WALLED) on one stream; mdx_sweep_run advances all
+ * of them with one launch of each step kernel for every member (predict,
+ * Lennard-Jones and bond-order forces, corrector with walls, and the forces
+ * and a = F/m again for the members whose corrector hit a wall), each member
+ * keeping its own list schedule (Host.cpp:143-152, 276-291) and wall flag.
+ * The rebuilds of all members due at a step share two read-backs.  Each
+ * member's trajectory is the one mdx_run gives the same system, bit for bit.
+ */
+typedef struct mdx_sweep mdx_sweep;
+int mdx_sweep_create(const mdx_config* cfg, int32_t members, mdx_sweep** out);
+int mdx_sweep_destroy(mdx_sweep* sweep);
+const char* mdx_sweep_last_error(const mdx_sweep* sweep);  /* sweep NULL: this thread's */
+/* The system of one member (0 <= member < members), for every per-system
+ * call: state, readyList and list builds, diagnostics, mdx_run of that member
+ * alone.  It belongs to the sweep: mdx_destroy refuses it. */
+int mdx_sweep_member(mdx_sweep* sweep, int32_t member, mdx_system** out);
+/* nsteps steps of every member (mdx_run's loop); a member's first step
+ * computes its missing lists, its initial forces and a = F/m (Host.cpp:117-138) */
+int mdx_sweep_run(mdx_sweep* sweep, int32_t nsteps);
+/* per-kernel timing of the members' launches (mdx_set_timing on every member);
+ * the sweep's step kernels are counted in member 0's mdx_kernel_times */
+int mdx_sweep_set_timing(mdx_sweep* sweep, int32_t enabled);
 
 /*
  * Launch shapes of the LDS-staged kernels.  No counterpart in the reference,

@@ -104,6 +104,8 @@ def load() -> C.CDLL:
         "mdx_sg_trainer_feature_stats": [P, P], "mdx_sg_trainer_descriptors": [P, I32, P],
         "mdx_sg_trainer_set_params": [P, P, I64], "mdx_sg_trainer_get_params": [P, P, I64],
         "mdx_sg_trainer_gradient": [P, P, I32, D, P, P], "mdx_sg_trainer_step": [P, P, I32, D, D, P],
+        "mdx_sweep_create": [C.POINTER(MdxConfig), I32, C.POINTER(P)], "mdx_sweep_destroy": [P],
+        "mdx_sweep_member": [P, I32, C.POINTER(P)], "mdx_sweep_run": [P, I32], "mdx_sweep_set_timing": [P, I32],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -113,6 +115,8 @@ def load() -> C.CDLL:
     lib.mdx_last_error.restype = C.c_char_p
     lib.mdx_batch_last_error.argtypes = [P]
     lib.mdx_batch_last_error.restype = C.c_char_p
+    lib.mdx_sweep_last_error.argtypes = [P]
+    lib.mdx_sweep_last_error.restype = C.c_char_p
     lib.mdx_sg_trainer_last_error.argtypes = [P]
     lib.mdx_sg_trainer_last_error.restype = C.c_char_p
     lib.mdx_tuning_last_error.argtypes = []
@@ -137,19 +141,29 @@ def _ptr(a: np.ndarray | None):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
-class System:
-    """Device-resident system; a thin owner of an mdx_system handle."""
+def make_config(**cfg) -> MdxConfig:
+    c = MdxConfig()
+    c.accel_scale = 1.0
+    c.lj_sigma = c.lj_eps = c.mass = 1.0
+    c.energy_loss = 1.0
+    for k, v in cfg.items():
+        setattr(c, k, v)
+    return c
 
-    def __init__(self, **cfg):
+
+class System:
+    """Device-resident system; a thin owner of an mdx_system handle (or a
+    borrowed handle of a sweep member, which the sweep destroys)."""
+
+    def __init__(self, _handle=None, _cfg=None, **cfg):
         self.lib = load()
-        c = MdxConfig()
-        c.accel_scale = 1.0
-        c.lj_sigma = c.lj_eps = c.mass = 1.0
-        c.energy_loss = 1.0
-        for k, v in cfg.items():
-            setattr(c, k, v)
+        if _handle is not None:
+            self.cfg, self.n, self.h, self.owned = _cfg, _cfg.n_atoms, _handle, False
+            return
+        c = make_config(**cfg)
         self.cfg = c
         self.n = c.n_atoms
+        self.owned = True
         h = C.c_void_p()
         rc = self.lib.mdx_create(C.byref(c), C.byref(h))
         if rc != MDX_OK:
@@ -162,7 +176,8 @@ class System:
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.mdx_destroy(self.h)
+            if self.owned:
+                self.lib.mdx_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -417,6 +432,59 @@ class Batch:
         self._chk("mdx_batch_kernel_times",
                   self.lib.mdx_batch_kernel_times(self.h, names, _ptr(counts), _ptr(tot), cap, C.byref(n)))
         return _kernel_table(names, counts, tot, n.value)
+
+
+class Sweep:
+    """Walled systems of one configuration advanced together (mdx_sweep_*):
+    the reference's vxcm sweep as one batch.  member(m) is the m-th member's
+    System (state, lists, diagnostics), owned by the sweep."""
+
+    def __init__(self, members, **cfg):
+        self.lib = load()
+        self.cfg = make_config(**cfg)
+        self.R = int(members)
+        h = C.c_void_p()
+        rc = self.lib.mdx_sweep_create(C.byref(self.cfg), self.R, C.byref(h))
+        if rc != MDX_OK:
+            raise MdxError("mdx_sweep_create", rc, self.lib.mdx_sweep_last_error(None).decode())
+        self.h = h
+        self.members = []
+        for m in range(self.R):
+            mh = C.c_void_p()
+            self._chk("mdx_sweep_member", self.lib.mdx_sweep_member(self.h, m, C.byref(mh)))
+            self.members.append(System(_handle=mh, _cfg=self.cfg))
+
+    def _chk(self, fn: str, rc: int):
+        if rc != MDX_OK:
+            raise MdxError(fn, rc, self.lib.mdx_sweep_last_error(self.h).decode())
+
+    def member(self, m) -> System:
+        return self.members[m]
+
+    def run(self, nsteps):
+        self._chk("mdx_sweep_run", self.lib.mdx_sweep_run(self.h, int(nsteps)))
+
+    def set_timing(self, on: bool):
+        self._chk("mdx_sweep_set_timing", self.lib.mdx_sweep_set_timing(self.h, 1 if on else 0))
+
+    def close(self):
+        if getattr(self, "h", None):
+            for m in self.members:
+                m.h = None
+            self.lib.mdx_sweep_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class SgTrainer:

@@ -129,14 +129,11 @@ template <class F> __device__ inline void for_row(const Ctx& c, int32_t self, F&
 
 __device__ inline int32_t type_of(const Ctx& c, int32_t id) { return c.st.type[c.slot_of_id[id]]; }
 
+// pass 1 for bond-order atom i (thread per atom)
 template <class T>
-__global__ void __launch_bounds__(kBlock)
-k_bond_pass1(Ctx c, BondRefParams p, int32_t n_bond, int32_t cap, int32_t assign,
-             int32_t* __restrict__ xi_j, double* __restrict__ xi_v, double* __restrict__ e_atom,
-             const int32_t* __restrict__ gate) {
-    if (gate && *gate == 0) return;
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_bond) return;
+__device__ inline void bond_pass1_atom(const Ctx& c, const BondRefParams& p, int32_t i, int32_t cap, int32_t assign,
+                                       int32_t* __restrict__ xi_j, double* __restrict__ xi_v,
+                                       double* __restrict__ e_atom) {
     const int32_t ti = type_of(c, i);
     double Fx = 0, Fy = 0, Fz = 0, e = 0;
     int32_t nx = 0;
@@ -221,11 +218,19 @@ k_bond_pass1(Ctx c, BondRefParams p, int32_t n_bond, int32_t cap, int32_t assign
 
 template <class T>
 __global__ void __launch_bounds__(kBlock)
-k_bond_pass2(Ctx c, BondRefParams p, int32_t n_bond, int32_t cap, const int32_t* __restrict__ xi_j,
-             const double* __restrict__ xi_v, const int32_t* __restrict__ gate) {
+k_bond_pass1(Ctx c, BondRefParams p, int32_t n_bond, int32_t cap, int32_t assign,
+             int32_t* __restrict__ xi_j, double* __restrict__ xi_v, double* __restrict__ e_atom,
+             const int32_t* __restrict__ gate) {
     if (gate && *gate == 0) return;
-    int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_bond) return;
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_bond) return;
+    bond_pass1_atom<T>(c, p, i, cap, assign, xi_j, xi_v, e_atom);
+}
+
+// pass 2 for bond-order atom k (thread per atom)
+template <class T>
+__device__ inline void bond_pass2_atom(const Ctx& c, const BondRefParams& p, int32_t k, int32_t cap,
+                                       const int32_t* __restrict__ xi_j, const double* __restrict__ xi_v) {
     double Fx = 0, Fy = 0, Fz = 0;
     for_row(c, k, [&](int32_t i) {
         double rik2d;
@@ -291,6 +296,16 @@ k_bond_pass2(Ctx c, BondRefParams p, int32_t n_bond, int32_t cap, const int32_t*
     c.st.f[2][sk] += Fz;
 }
 
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_bond_pass2(Ctx c, BondRefParams p, int32_t n_bond, int32_t cap, const int32_t* __restrict__ xi_j,
+             const double* __restrict__ xi_v, const int32_t* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_bond) return;
+    bond_pass2_atom<T>(c, p, k, cap, xi_j, xi_v);
+}
+
 // ---- one wavefront per bond-order atom (few atoms, short rows) ----
 //
 // The atom's candidate row (9 contiguous runs of the cell-sorted member
@@ -352,19 +367,12 @@ __device__ inline int32_t stage_row(const Ctx& c, int32_t self, RowEntry* row, i
     return len;
 }
 
+// pass 1 of bond-order atom i by the wavefront (lane = its lane) whose
+// staged row is row[0..len)
 template <class T>
-__global__ void __launch_bounds__(kWaveBlock)
-k_bond_wave1(Ctx c, BondRefParams p, int32_t n_bond, int32_t cap, int32_t assign,
-             int32_t* __restrict__ xi_j, double* __restrict__ xi_v, double* __restrict__ e_atom,
-             const int32_t* __restrict__ gate) {
-    __shared__ RowEntry rows[kWaveBlock / 64][kRowMax];
-    if (gate && *gate == 0) return;   // the same for every workgroup
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int32_t i = (int32_t)blockIdx.x * (kWaveBlock / 64) + w;
-    RowEntry* row = rows[w];
-    const int32_t len = i < n_bond ? stage_row(c, i, row, lane) : 0;
-    __syncthreads();
-    if (i >= n_bond) return;
+__device__ inline void bond_wave1_atom(const Ctx& c, const BondRefParams& p, int32_t i, int32_t cap, int32_t assign,
+                                       int32_t* __restrict__ xi_j, double* __restrict__ xi_v,
+                                       double* __restrict__ e_atom, const RowEntry* row, int32_t len, int lane) {
     const int32_t si = c.slot_of_id[i];
     const double px = c.st.r[0][si], py = c.st.r[1][si], pz = c.st.r[2][si];
     const int32_t ti = c.st.type[si];
@@ -467,20 +475,28 @@ k_bond_wave1(Ctx c, BondRefParams p, int32_t n_bond, int32_t cap, int32_t assign
     if (e_atom) e_atom[i] = e;
 }
 
+template <class T>
+__global__ void __launch_bounds__(kWaveBlock)
+k_bond_wave1(Ctx c, BondRefParams p, int32_t n_bond, int32_t cap, int32_t assign,
+             int32_t* __restrict__ xi_j, double* __restrict__ xi_v, double* __restrict__ e_atom,
+             const int32_t* __restrict__ gate) {
+    __shared__ RowEntry rows[kWaveBlock / 64][kRowMax];
+    if (gate && *gate == 0) return;   // the same for every workgroup
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int32_t i = (int32_t)blockIdx.x * (kWaveBlock / 64) + w;
+    RowEntry* row = rows[w];
+    const int32_t len = i < n_bond ? stage_row(c, i, row, lane) : 0;
+    __syncthreads();
+    if (i >= n_bond) return;
+    bond_wave1_atom<T>(c, p, i, cap, assign, xi_j, xi_v, e_atom, row, len, lane);
+}
+
 // pass 2 per wavefront: atom k; i runs over k's row in order, j over k's row
 // on the lanes; the xi table of i (at most 64 entries) is held one entry per lane
 template <class T>
-__global__ void __launch_bounds__(kWaveBlock)
-k_bond_wave2(Ctx c, BondRefParams p, int32_t n_bond, int32_t cap, const int32_t* __restrict__ xi_j,
-             const double* __restrict__ xi_v, const int32_t* __restrict__ gate) {
-    __shared__ RowEntry rows[kWaveBlock / 64][kRowMax];
-    if (gate && *gate == 0) return;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int32_t k = (int32_t)blockIdx.x * (kWaveBlock / 64) + w;
-    RowEntry* row = rows[w];
-    const int32_t len = k < n_bond ? stage_row(c, k, row, lane) : 0;
-    __syncthreads();
-    if (k >= n_bond) return;
+__device__ inline void bond_wave2_atom(const Ctx& c, const BondRefParams& p, int32_t k, int32_t cap,
+                                       const int32_t* __restrict__ xi_j, const double* __restrict__ xi_v,
+                                       const RowEntry* row, int32_t len, int lane) {
     const int32_t sk = c.slot_of_id[k];
     const double px = c.st.r[0][sk], py = c.st.r[1][sk], pz = c.st.r[2][sk];
     double Fx = 0, Fy = 0, Fz = 0;
@@ -576,9 +592,83 @@ k_bond_wave2(Ctx c, BondRefParams p, int32_t n_bond, int32_t cap, const int32_t*
     }
 }
 
+template <class T>
+__global__ void __launch_bounds__(kWaveBlock)
+k_bond_wave2(Ctx c, BondRefParams p, int32_t n_bond, int32_t cap, const int32_t* __restrict__ xi_j,
+             const double* __restrict__ xi_v, const int32_t* __restrict__ gate) {
+    __shared__ RowEntry rows[kWaveBlock / 64][kRowMax];
+    if (gate && *gate == 0) return;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int32_t k = (int32_t)blockIdx.x * (kWaveBlock / 64) + w;
+    RowEntry* row = rows[w];
+    const int32_t len = k < n_bond ? stage_row(c, k, row, lane) : 0;
+    __syncthreads();
+    if (k >= n_bond) return;
+    bond_wave2_atom<T>(c, p, k, cap, xi_j, xi_v, row, len, lane);
+}
+
 // below this many bond-order atoms (with rows and xi tables that fit) the
 // wavefront-per-atom kernels run
 constexpr int32_t kWaveAtoms = 32768;
+
+// ---- sweep members in one launch (blockIdx.y = member) ----
+// Each member runs on the kind of kernel launch_bond_ref picks for it
+// (v.bond_wave); the blocks of the other kind return at once.  gated: only
+// the members whose wall-hit word is set.
+__device__ inline Ctx member_ctx(const MemberView& v) { return Ctx{v.st, v.slot_of_id, v.mb_cell_of, v.mb_ids, v.mb}; }
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_sweep_bond1(const MemberView* __restrict__ views, BondRefParams p, int32_t n_bond, int32_t assign, int32_t gated) {
+    const MemberView& v = views[blockIdx.y];
+    if (v.bond_wave || (gated && *v.hit == 0)) return;
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_bond) return;
+    bond_pass1_atom<T>(member_ctx(v), p, i, v.xi_cap, assign, v.xi_j, v.xi_v, nullptr);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_sweep_bond2(const MemberView* __restrict__ views, BondRefParams p, int32_t n_bond, int32_t gated) {
+    const MemberView& v = views[blockIdx.y];
+    if (v.bond_wave || (gated && *v.hit == 0)) return;
+    int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_bond) return;
+    bond_pass2_atom<T>(member_ctx(v), p, k, v.xi_cap, v.xi_j, v.xi_v);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kWaveBlock)
+k_sweep_bond_wave1(const MemberView* __restrict__ views, BondRefParams p, int32_t n_bond, int32_t assign,
+                   int32_t gated) {
+    __shared__ RowEntry rows[kWaveBlock / 64][kRowMax];
+    const MemberView& v = views[blockIdx.y];
+    if (!v.bond_wave || (gated && *v.hit == 0)) return;   // the same for the whole workgroup
+    const Ctx c = member_ctx(v);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int32_t i = (int32_t)blockIdx.x * (kWaveBlock / 64) + w;
+    RowEntry* row = rows[w];
+    const int32_t len = i < n_bond ? stage_row(c, i, row, lane) : 0;
+    __syncthreads();
+    if (i >= n_bond) return;
+    bond_wave1_atom<T>(c, p, i, v.xi_cap, assign, v.xi_j, v.xi_v, nullptr, row, len, lane);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kWaveBlock)
+k_sweep_bond_wave2(const MemberView* __restrict__ views, BondRefParams p, int32_t n_bond, int32_t gated) {
+    __shared__ RowEntry rows[kWaveBlock / 64][kRowMax];
+    const MemberView& v = views[blockIdx.y];
+    if (!v.bond_wave || (gated && *v.hit == 0)) return;
+    const Ctx c = member_ctx(v);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int32_t k = (int32_t)blockIdx.x * (kWaveBlock / 64) + w;
+    RowEntry* row = rows[w];
+    const int32_t len = k < n_bond ? stage_row(c, k, row, lane) : 0;
+    __syncthreads();
+    if (k >= n_bond) return;
+    bond_wave2_atom<T>(c, p, k, v.xi_cap, v.xi_j, v.xi_v, row, len, lane);
+}
 
 // common/DebugCalculations.cpp:129-270: per-atom share of the brute-force
 // potential.  Ordered pairs with a Ne atom add 2 eps [(s/r)^12 - (s/r)^6]
@@ -647,6 +737,41 @@ hipError_t launch_allpairs(hipStream_t s, const DevState& st, int32_t n, int32_t
     return hipGetLastError();
 }
 
+bool bond_ref_wave(int32_t n_bond, int64_t row_bound, int32_t cap) {
+    return n_bond <= kWaveAtoms && row_bound <= kRowMax && cap <= 64;
+}
+
+hipError_t launch_sweep_bond(hipStream_t s, const MemberView* views, int32_t members, const BondRefParams& p,
+                             int32_t n_bond, int32_t precision, int32_t assign, bool gated, bool any_wave,
+                             bool any_thread) {
+    if (n_bond <= 0 || members <= 0) return hipSuccess;
+    const int32_t gt = gated ? 1 : 0;
+    const bool fp64 = precision == MDX_PREC_FP64;
+    if (any_wave) {
+        dim3 wgrid((n_bond + kWaveBlock / 64 - 1) / (kWaveBlock / 64), members);
+        if (fp64) {
+            k_sweep_bond_wave1<double><<<wgrid, kWaveBlock, 0, s>>>(views, p, n_bond, assign, gt);
+            k_sweep_bond_wave2<double><<<wgrid, kWaveBlock, 0, s>>>(views, p, n_bond, gt);
+        } else {
+            k_sweep_bond_wave1<float><<<wgrid, kWaveBlock, 0, s>>>(views, p, n_bond, assign, gt);
+            k_sweep_bond_wave2<float><<<wgrid, kWaveBlock, 0, s>>>(views, p, n_bond, gt);
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (any_thread) {
+        dim3 grid((n_bond + kBlock - 1) / kBlock, members);
+        if (fp64) {
+            k_sweep_bond1<double><<<grid, kBlock, 0, s>>>(views, p, n_bond, assign, gt);
+            k_sweep_bond2<double><<<grid, kBlock, 0, s>>>(views, p, n_bond, gt);
+        } else {
+            k_sweep_bond1<float><<<grid, kBlock, 0, s>>>(views, p, n_bond, assign, gt);
+            k_sweep_bond2<float><<<grid, kBlock, 0, s>>>(views, p, n_bond, gt);
+        }
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_bond_ref(hipStream_t s, const DevState& st, const int32_t* slot_of_id,
                            const int32_t* mb_cell_of, const int32_t* mb_ids, const GridView& g,
                            const BondRefParams& p, int32_t n_bond, int32_t cap,
@@ -654,7 +779,7 @@ hipError_t launch_bond_ref(hipStream_t s, const DevState& st, const int32_t* slo
                            double* energy_per_atom, const int32_t* gate, int64_t row_bound) {
     if (n_bond <= 0) return hipSuccess;
     Ctx c{st, slot_of_id, mb_cell_of, mb_ids, g};
-    if (n_bond <= kWaveAtoms && row_bound <= kRowMax && cap <= 64) {
+    if (bond_ref_wave(n_bond, row_bound, cap)) {
         dim3 wgrid((n_bond + kWaveBlock / 64 - 1) / (kWaveBlock / 64));
         if (precision == MDX_PREC_FP64) {
             k_bond_wave1<double><<<wgrid, kWaveBlock, 0, s>>>(c, p, n_bond, cap, assign, xi_j, xi_v,

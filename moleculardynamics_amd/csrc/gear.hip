@@ -15,10 +15,7 @@ namespace {
 
 constexpr int kBlock = 256;
 
-__global__ void __launch_bounds__(kBlock)
-k_predict(DevState st, int32_t n, GearCoeffs gc) {
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ inline void predict_atom(const DevState& st, int32_t i, const GearCoeffs& gc) {
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         double r = st.r[d][i], v = st.v[d][i], a = st.a[d][i], b = st.b[d][i], c = st.c[d][i];
@@ -27,6 +24,13 @@ k_predict(DevState st, int32_t n, GearCoeffs gc) {
         st.a[d][i] = a + gc.k[0] * b + gc.k[1] * c;
         st.b[d][i] = b + gc.k[0] * c;
     }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_predict(DevState st, int32_t n, GearCoeffs gc) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    predict_atom(st, i, gc);
 }
 
 // corrector of one component; returns the corrected position
@@ -41,32 +45,43 @@ __device__ inline void correct1(double f, double mass, double scale, const GearC
     c = c + gc.kc[3] * corr;
 }
 
-// Device.cpp:44-120: corrector, then the walls at +-npos.
+// Device.cpp:44-120: corrector, then the walls at +-npos; true when atom i
+// hit a wall
+__device__ inline bool correct_walled_atom(const DevState& st, int32_t i, const GearCoeffs& gc, const MassTable& m,
+                                           double npos, float eloss) {
+    bool h = false;
+    const double mass = m.mass[st.type[i] & (kTypes - 1)];
+    double R[3], V[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double r = st.r[d][i], v = st.v[d][i], a = st.a[d][i], b = st.b[d][i], c = st.c[d][i];
+        correct1(st.f[d][i], mass, m.scale, gc, r, v, a, b, c);
+        R[d] = r; V[d] = v;
+        st.a[d][i] = a; st.b[d][i] = b; st.c[d][i] = c;
+    }
+    if (R[0] > npos && R[1] > 0) { h = true; V[0] = -V[0] * (double)eloss; R[0] = npos; }
+    if (R[0] > npos && R[1] < 0) { h = true; R[0] = -npos; }
+    if (R[0] < -npos) { h = true; V[0] = -V[0]; R[0] = -npos; }
+#pragma unroll
+    for (int d = 1; d < 3; d++)
+        if (fabs(R[d]) > npos) { h = true; V[d] = -V[d]; R[d] = npos * (R[d] / fabs(R[d])); }
+#pragma unroll
+    for (int d = 0; d < 3; d++) { st.r[d][i] = R[d]; st.v[d][i] = V[d]; }
+    return h;
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_correct_walled(DevState st, int32_t n, GearCoeffs gc, MassTable m, double npos, float eloss,
                  int32_t* __restrict__ hit) {
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool h = false;
-    if (i < n) {
-        const double mass = m.mass[st.type[i] & (kTypes - 1)];
-        double R[3], V[3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            double r = st.r[d][i], v = st.v[d][i], a = st.a[d][i], b = st.b[d][i], c = st.c[d][i];
-            correct1(st.f[d][i], mass, m.scale, gc, r, v, a, b, c);
-            R[d] = r; V[d] = v;
-            st.a[d][i] = a; st.b[d][i] = b; st.c[d][i] = c;
-        }
-        if (R[0] > npos && R[1] > 0) { h = true; V[0] = -V[0] * (double)eloss; R[0] = npos; }
-        if (R[0] > npos && R[1] < 0) { h = true; R[0] = -npos; }
-        if (R[0] < -npos) { h = true; V[0] = -V[0]; R[0] = -npos; }
-#pragma unroll
-        for (int d = 1; d < 3; d++)
-            if (fabs(R[d]) > npos) { h = true; V[d] = -V[d]; R[d] = npos * (R[d] / fabs(R[d])); }
-#pragma unroll
-        for (int d = 0; d < 3; d++) { st.r[d][i] = R[d]; st.v[d][i] = V[d]; }
-    }
+    const bool h = i < n && correct_walled_atom(st, i, gc, m, npos, eloss);
     if (__any(h) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(hit, 1);
+}
+
+__device__ inline void accelerations_atom(const DevState& st, int32_t i, const MassTable& m) {
+    const double mass = m.mass[st.type[i] & (kTypes - 1)];
+#pragma unroll
+    for (int d = 0; d < 3; d++) st.a[d][i] = st.f[d][i] / mass * m.scale;
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -74,9 +89,39 @@ k_accelerations(DevState st, int32_t n, MassTable m, const int32_t* __restrict__
     if (gate && *gate == 0) return;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double mass = m.mass[st.type[i] & (kTypes - 1)];
-#pragma unroll
-    for (int d = 0; d < 3; d++) st.a[d][i] = st.f[d][i] / mass * m.scale;
+    accelerations_atom(st, i, m);
+}
+
+// ---- sweep members, all in one launch (blockIdx.y = member) ----
+
+// the predictor; block 0 of each member clears the member's wall-hit word for
+// the corrector that follows
+__global__ void __launch_bounds__(kBlock)
+k_sweep_predict(const MemberView* __restrict__ views, int32_t n, GearCoeffs gc) {
+    const MemberView& v = views[blockIdx.y];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *v.hit = 0;
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    predict_atom(v.st, i, gc);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_sweep_correct(const MemberView* __restrict__ views, int32_t n, GearCoeffs gc, MassTable m, double npos,
+                float eloss) {
+    const MemberView& v = views[blockIdx.y];
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool h = i < n && correct_walled_atom(v.st, i, gc, m, npos, eloss);
+    if (__any(h) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(v.hit, 1);
+}
+
+// a = F / m for the members whose corrector hit a wall
+__global__ void __launch_bounds__(kBlock)
+k_sweep_accelerations(const MemberView* __restrict__ views, int32_t n, MassTable m) {
+    const MemberView& v = views[blockIdx.y];
+    if (*v.hit == 0) return;
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    accelerations_atom(v.st, i, m);
 }
 
 // Corrector of step t then predictor of step t+1 on each component, and the
@@ -328,6 +373,27 @@ hipError_t launch_accelerations(hipStream_t s, const DevState& st, int32_t n, co
                                 const int32_t* gate) {
     if (n <= 0) return hipSuccess;
     k_accelerations<<<grid_of(n), kBlock, 0, s>>>(st, n, m, gate);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_predict(hipStream_t s, const MemberView* views, int32_t members, int32_t n,
+                                const GearCoeffs& gc) {
+    if (n <= 0 || members <= 0) return hipSuccess;
+    k_sweep_predict<<<dim3((n + kBlock - 1) / kBlock, members), kBlock, 0, s>>>(views, n, gc);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_correct(hipStream_t s, const MemberView* views, int32_t members, int32_t n,
+                                const GearCoeffs& gc, const MassTable& m, double npos, float eloss) {
+    if (n <= 0 || members <= 0) return hipSuccess;
+    k_sweep_correct<<<dim3((n + kBlock - 1) / kBlock, members), kBlock, 0, s>>>(views, n, gc, m, npos, eloss);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_accelerations(hipStream_t s, const MemberView* views, int32_t members, int32_t n,
+                                      const MassTable& m) {
+    if (n <= 0 || members <= 0) return hipSuccess;
+    k_sweep_accelerations<<<dim3((n + kBlock - 1) / kBlock, members), kBlock, 0, s>>>(views, n, m);
     return hipGetLastError();
 }
 

@@ -16,13 +16,11 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// the force on slot s (thread per atom)
 template <bool kFp64, bool kEnergy>
-__global__ void __launch_bounds__(kBlock)
-k_lj(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable t, int32_t n,
-     double* __restrict__ e_atom, double* __restrict__ w_atom, const int32_t* __restrict__ gate) {
-    if (gate && *gate == 0) return;
-    int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
+__device__ inline void lj_atom(const DevState& st, const int32_t* __restrict__ cell_of, const GridView& g,
+                               const LjTable& t, int32_t s, double* __restrict__ e_atom,
+                               double* __restrict__ w_atom) {
     const double xi = st.r[0][s], yi = st.r[1][s], zi = st.r[2][s];
     const int32_t ti = st.type[s] & (kTypes - 1);
     const int32_t own = cell_of[s];
@@ -103,6 +101,16 @@ k_lj(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable t, in
     }
 }
 
+template <bool kFp64, bool kEnergy>
+__global__ void __launch_bounds__(kBlock)
+k_lj(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable t, int32_t n,
+     double* __restrict__ e_atom, double* __restrict__ w_atom, const int32_t* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    lj_atom<kFp64, kEnergy>(st, cell_of, g, t, s, e_atom, w_atom);
+}
+
 // Walled box, few atoms: one wavefront per atom.  The 27 candidate cells of
 // the reference's row are 9 z-runs of up to 3 consecutive cells, i.e. 9
 // contiguous slot ranges in row order; the 64 lanes take 64 consecutive row
@@ -111,14 +119,11 @@ k_lj(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable t, in
 // thread-per-atom kernel's, operation for operation.
 constexpr int kWaveBlock = 256;
 
+// the force on slot s, the whole wavefront (lane = its lane)
 template <bool kFp64, bool kEnergy>
-__global__ void __launch_bounds__(kWaveBlock)
-k_lj_wave(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable t, int32_t n,
-          double* __restrict__ e_atom, double* __restrict__ w_atom, const int32_t* __restrict__ gate) {
-    if (gate && *gate == 0) return;
-    const int32_t s = (int32_t)((blockIdx.x * kWaveBlock + threadIdx.x) >> 6);
-    const int lane = threadIdx.x & 63;
-    if (s >= n) return;
+__device__ inline void lj_atom_wave(const DevState& st, const int32_t* __restrict__ cell_of, const GridView& g,
+                                    const LjTable& t, int32_t s, int lane, double* __restrict__ e_atom,
+                                    double* __restrict__ w_atom) {
     const double xi = st.r[0][s], yi = st.r[1][s], zi = st.r[2][s];
     const int32_t ti = st.type[s] & (kTypes - 1);
     const int32_t own = cell_of[s];
@@ -222,8 +227,40 @@ k_lj_wave(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable 
     }
 }
 
+template <bool kFp64, bool kEnergy>
+__global__ void __launch_bounds__(kWaveBlock)
+k_lj_wave(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable t, int32_t n,
+          double* __restrict__ e_atom, double* __restrict__ w_atom, const int32_t* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    const int32_t s = (int32_t)((blockIdx.x * kWaveBlock + threadIdx.x) >> 6);
+    if (s >= n) return;
+    lj_atom_wave<kFp64, kEnergy>(st, cell_of, g, t, s, threadIdx.x & 63, e_atom, w_atom);
+}
+
 // below this many atoms a walled box takes the wavefront-per-atom kernel
 constexpr int32_t kWaveAtoms = 32768;
+
+// sweep members in one launch (blockIdx.y = member); gated: only the members
+// whose wall-hit word is set
+template <bool kFp64>
+__global__ void __launch_bounds__(kBlock)
+k_sweep_lj(const MemberView* __restrict__ views, LjTable t, int32_t n, int32_t gated) {
+    const MemberView& v = views[blockIdx.y];
+    if (gated && *v.hit == 0) return;
+    int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    lj_atom<kFp64, false>(v.st, v.cell_of, v.lj, t, s, nullptr, nullptr);
+}
+
+template <bool kFp64>
+__global__ void __launch_bounds__(kWaveBlock)
+k_sweep_lj_wave(const MemberView* __restrict__ views, LjTable t, int32_t n, int32_t gated) {
+    const MemberView& v = views[blockIdx.y];
+    if (gated && *v.hit == 0) return;
+    const int32_t s = (int32_t)((blockIdx.x * kWaveBlock + threadIdx.x) >> 6);
+    if (s >= n) return;
+    lj_atom_wave<kFp64, false>(v.st, v.cell_of, v.lj, t, s, threadIdx.x & 63, nullptr, nullptr);
+}
 
 }  // namespace
 
@@ -254,6 +291,23 @@ hipError_t launch_lj(hipStream_t s, const DevState& st, const int32_t* cell_of, 
         k_lj<false, true><<<grid, kBlock, 0, s>>>(st, cell_of, g, t, n, energy_per_atom, virial_per_atom, gate);
     else
         k_lj<false, false><<<grid, kBlock, 0, s>>>(st, cell_of, g, t, n, nullptr, nullptr, gate);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_lj(hipStream_t s, const MemberView* views, int32_t members, int32_t n, const LjTable& t,
+                           int32_t precision, bool gated) {
+    if (n <= 0 || members <= 0) return hipSuccess;
+    const bool fp64 = precision == MDX_PREC_FP64;
+    const int32_t gt = gated ? 1 : 0;
+    if (n <= kWaveAtoms) {
+        dim3 wgrid((n + kWaveBlock / 64 - 1) / (kWaveBlock / 64), members);
+        if (fp64) k_sweep_lj_wave<true><<<wgrid, kWaveBlock, 0, s>>>(views, t, n, gt);
+        else k_sweep_lj_wave<false><<<wgrid, kWaveBlock, 0, s>>>(views, t, n, gt);
+        return hipGetLastError();
+    }
+    dim3 grid((n + kBlock - 1) / kBlock, members);
+    if (fp64) k_sweep_lj<true><<<grid, kBlock, 0, s>>>(views, t, n, gt);
+    else k_sweep_lj<false><<<grid, kBlock, 0, s>>>(views, t, n, gt);
     return hipGetLastError();
 }
 

@@ -202,6 +202,25 @@ struct TersoffParams {
     int32_t m;
 };
 
+// One member of a sweep (mdx_sweep_*): a walled system whose step kernels run
+// in one launch with the other members' (blockIdx.y = member).  The host
+// rewrites a member's view after each of its list builds.
+struct MemberView {
+    DevState st;
+    const int32_t* cell_of;      // LJ grid cell per slot
+    GridView lj;
+    const int32_t* slot_of_id;
+    const int32_t* mb_cell_of;   // bond-order grid cell per id
+    const int32_t* mb_ids;       // bond-order grid members (ids), cell-sorted
+    GridView mb;
+    int32_t* xi_j;
+    double* xi_v;
+    int32_t xi_cap;              // xi table entries per bond-order atom
+    int32_t bond_wave;           // 1: the bond-order force of this member runs one wavefront per atom
+    int32_t* hit;                // wall-hit word of the member's corrector
+    int32_t pad;
+};
+
 // ---- host-side launch wrappers (one translation unit each) ----
 // cells.hip
 hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v);
@@ -243,6 +262,10 @@ hipError_t launch_reorder(hipStream_t s, const DevState& src, const DevState& ds
 hipError_t launch_lj(hipStream_t s, const DevState& st, const int32_t* cell_of, const GridView& g,
                      const LjTable& t, int32_t n, int32_t precision, double* energy_per_atom,
                      const int32_t* gate, double* virial_per_atom = nullptr);
+// the LJ force of every sweep member (the kernel launch_lj picks for a walled
+// box of n atoms); gated: only the members whose wall-hit word is set
+hipError_t launch_sweep_lj(hipStream_t s, const MemberView* views, int32_t members, int32_t n, const LjTable& t,
+                           int32_t precision, bool gated);
 
 // gear.hip
 struct GearCoeffs {
@@ -257,6 +280,16 @@ hipError_t launch_accelerations(hipStream_t s, const DevState& st, int32_t n, co
                                 const int32_t* gate);
 hipError_t launch_correct_predict(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
                                   const MassTable& m, double* const rref[3], uint32_t* maxdisp2_bits);
+// the same kernels over every member of a sweep (views: device array of
+// `members` MemberViews, n atoms each).  The predictor clears each member's
+// wall-hit word; the corrector sets it; the accelerations (a = F/m) run only
+// for the members whose word is set.
+hipError_t launch_sweep_predict(hipStream_t s, const MemberView* views, int32_t members, int32_t n,
+                                const GearCoeffs& gc);
+hipError_t launch_sweep_correct(hipStream_t s, const MemberView* views, int32_t members, int32_t n,
+                                const GearCoeffs& gc, const MassTable& m, double npos, float eloss);
+hipError_t launch_sweep_accelerations(hipStream_t s, const MemberView* views, int32_t members, int32_t n,
+                                      const MassTable& m);
 
 // predictor that also max-reduces |r - rref|^2 and the squared step
 // |r_new - r_old|^2 (bits of non-negative doubles) into the spread flag
@@ -392,6 +425,14 @@ hipError_t launch_bond_ref(hipStream_t s, const DevState& st, const int32_t* slo
                            const BondRefParams& p, int32_t n_bond, int32_t nlistsize,
                            int32_t precision, int32_t assign, int32_t* xi_j, double* xi_v,
                            double* energy_per_atom, const int32_t* gate, int64_t row_bound);
+// whether launch_bond_ref runs the wavefront-per-atom kernels for these sizes
+bool bond_ref_wave(int32_t n_bond, int64_t row_bound, int32_t cap);
+// the bond-order force of every sweep member, each on the kernels
+// launch_bond_ref would pick for it (MemberView::bond_wave); launches only
+// the kinds named by any_wave / any_thread; gated as launch_sweep_lj
+hipError_t launch_sweep_bond(hipStream_t s, const MemberView* views, int32_t members, const BondRefParams& p,
+                             int32_t n_bond, int32_t precision, int32_t assign, bool gated, bool any_wave,
+                             bool any_thread);
 
 // reduce.hip
 hipError_t launch_kinetics(hipStream_t s, const DevState& st, int32_t n, const MassTable& m,

@@ -116,6 +116,8 @@ struct mdx_system {
     unsigned int* dpub = nullptr;    // the step kernels' arrival counters, then the list gate (FlagPublish)
     uint64_t pub_seq = 0;            // last sequence number handed out
     hipEvent_t ev[kRing] = {nullptr, nullptr, nullptr};
+    bool owns_stream = true;         // false for the members of a sweep (the sweep's stream)
+    bool sweep_member = false;       // destroyed with its sweep only
     bool timing = false;
     std::vector<EventPair> pending;
     std::vector<hipEvent_t> free_events;
@@ -342,17 +344,30 @@ int read_ints(mdx_system* s, int32_t* host, const int32_t* dev, int count) {
     return MDX_OK;
 }
 
+// Byte offsets in the pinned buffer of read-backs that a sweep's batched
+// rebuild collects from several members before one synchronisation.
+constexpr int kPinReady = 128;   // readyList: max v^2 bits, max |coord| bits
+constexpr int kPinLj = 160;      // LJ list sizes (3 words)
+constexpr int kPinMb = 176;      // bond-order list sizes (3 words)
+
 // reference list sizes: nlistsize = min(27 * largest, n - 1) (NeighborList.cpp:88-90).
 // One read-back brings the binning's out-of-box flag (dint[1]; an atom outside
 // the grid is counted in cell 0, so the kernels before stay in bounds) with
-// the two sizes.
-int finish_sizes(mdx_system* s, Grid& g, const int32_t* cell_by_item, int32_t items) {
+// the two sizes.  sizes_launch queues the kernels and the copy into the
+// pinned buffer at byte offset po; sizes_finish reads it after a
+// synchronisation.
+int sizes_launch(mdx_system* s, Grid& g, const int32_t* cell_by_item, int32_t items, int po) {
     HIPCHK(s, hipMemsetAsync(s->dint + 2, 0, 2 * sizeof(int32_t), s->stream));
     LAUNCH(s, "list_max_count", launch_max_count(s->stream, g.count, g.ncell, s->dint + 2));
     LAUNCH(s, "list_row_lengths", launch_row_lengths(s->stream, g.view, cell_by_item, items, s->dint + 3));
+    HIPCHK(s, hipMemcpyAsync((char*)s->pinned + po, s->dint + 1, 3 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                             s->stream));
+    return MDX_OK;
+}
+
+int sizes_finish(mdx_system* s, Grid& g, int32_t items, int po) {
     int32_t h[3];
-    int rc = read_ints(s, h, s->dint + 1, 3);
-    if (rc) return rc;
+    std::memcpy(h, (char*)s->pinned + po, sizeof h);
     if (h[0]) return fail(s, MDX_ERR_ARG, "an atom lies outside the list box (box_size_list too small)");
     g.largest = h[1];
     int64_t nls = 27 * (int64_t)h[1];
@@ -363,9 +378,17 @@ int finish_sizes(mdx_system* s, Grid& g, const int32_t* cell_by_item, int32_t it
     return MDX_OK;
 }
 
+int finish_sizes(mdx_system* s, Grid& g, const int32_t* cell_by_item, int32_t items) {
+    int rc = sizes_launch(s, g, cell_by_item, items, 0);
+    if (rc) return rc;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return sizes_finish(s, g, items, 0);
+}
+
 // Primary grid: bins every atom, counting-sorts slots into cell order (ascending
-// id within a cell) and permutes the whole state into that order.
-int build_primary(mdx_system* s, Grid& g, float rs, double box_size_list, bool sizes) {
+// id within a cell) and permutes the whole state into that order.  With
+// sizes_po >= 0 the sizes are only queued (sizes_launch at that offset).
+int build_primary(mdx_system* s, Grid& g, float rs, double box_size_list, bool sizes, int sizes_po = -1) {
     int rc = set_geometry(s, g, rs, box_size_list);
     if (rc) return rc;
     hipStream_t st = s->stream;
@@ -393,14 +416,34 @@ int build_primary(mdx_system* s, Grid& g, float rs, double box_size_list, bool s
         // counts = start[c+1] - start[c]; recompute by binning again
         LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
         LAUNCH(s, "cells_bin", launch_bin(st, s->st, nullptr, n, g.view, s->cell_by_id, g.count, s->dint + 1));
-        return finish_sizes(s, g, s->cell_of, n);
+        return sizes_po >= 0 ? sizes_launch(s, g, s->cell_of, n, sizes_po) : finish_sizes(s, g, s->cell_of, n);
     }
     return MDX_OK;
 }
 
+// the xi table of the bond-order force: one entry per candidate within the
+// cutoff, at most the row length (after the bond-order list's sizes)
+int bond_walled_finish(mdx_system* s) {
+    Grid& g = s->mb;
+    int32_t cap = std::max(g.nlargestsize, 1);
+    int64_t need = (int64_t)s->cfg.n_bond * cap;
+    if (need > s->xi_alloc) {
+        dfree(s, s->xi_j);
+        dfree(s, s->xi_v);
+        int rc;
+        if ((rc = dalloc(s, &s->xi_j, need)) || (rc = dalloc(s, &s->xi_v, 2 * need))) return rc;
+        s->xi_alloc = need;
+    }
+    s->xi_cap = cap;
+    g.built = true;
+    return MDX_OK;
+}
+
 // Bond-order grid of the walled box: the first n_bond atoms (by id) binned with
-// the bond list radius; members are ids, the state is not permuted.
-int build_bond_walled(mdx_system* s, double box_size_list) {
+// the bond list radius; members are ids, the state is not permuted.  With
+// sizes_po >= 0 the sizes are only queued; sizes_finish and bond_walled_finish
+// complete the build after a synchronisation.
+int build_bond_walled(mdx_system* s, double box_size_list, int sizes_po = -1) {
     Grid& g = s->mb;
     int rc = set_geometry(s, g, s->cfg.mb_rs, box_size_list);
     if (rc) return rc;
@@ -416,32 +459,36 @@ int build_bond_walled(mdx_system* s, double box_size_list) {
     // rebuild the histogram for the sizes (the out-of-box flag is read with them)
     LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
     LAUNCH(s, "cells_bin", launch_bin(st, s->st, s->slot_of_id, nb, g.view, s->cell_by_id, g.count, s->dint + 1));
+    if (sizes_po >= 0) return sizes_launch(s, g, s->mb_cell_of, nb, sizes_po);
     if ((rc = finish_sizes(s, g, s->mb_cell_of, nb))) return rc;
-    // xi table: one entry per candidate within the cutoff, at most the row length
-    int32_t cap = std::max(g.nlargestsize, 1);
-    int64_t need = (int64_t)nb * cap;
-    if (need > s->xi_alloc) {
-        dfree(s, s->xi_j);
-        dfree(s, s->xi_v);
-        if ((rc = dalloc(s, &s->xi_j, need)) || (rc = dalloc(s, &s->xi_v, 2 * need))) return rc;
-        s->xi_alloc = need;
-    }
-    s->xi_cap = cap;
-    g.built = true;
+    return bond_walled_finish(s);
+}
+
+// readyList (cpu/cpumd/Host.cpp:245-271) on the device: the kernel and the
+// copy of its two words into the pinned buffer at byte offset po, then (after
+// a synchronisation) vmax and boxSizeList from them
+int ready_launch(mdx_system* s, int po) {
+    HIPCHK(s, hipMemsetAsync(s->dbits + 1, 0, 2 * sizeof(uint64_t), s->stream));
+    LAUNCH(s, "ready_list", launch_ready_list(s->stream, s->st, s->n, s->dbits + 1));
+    HIPCHK(s, hipMemcpyAsync((char*)s->pinned + po, s->dbits + 1, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                             s->stream));
     return MDX_OK;
 }
 
-int ready_list(mdx_system* s) {
-    HIPCHK(s, hipMemsetAsync(s->dbits + 1, 0, 2 * sizeof(uint64_t), s->stream));
-    LAUNCH(s, "ready_list", launch_ready_list(s->stream, s->st, s->n, s->dbits + 1));
-    HIPCHK(s, hipMemcpyAsync(s->pinned, s->dbits + 1, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(s, hipStreamSynchronize(s->stream));
+void ready_finish(mdx_system* s, int po) {
     double v2, c;
-    std::memcpy(&v2, (char*)s->pinned, 8);
-    std::memcpy(&c, (char*)s->pinned + 8, 8);
+    std::memcpy(&v2, (char*)s->pinned + po, 8);
+    std::memcpy(&c, (char*)s->pinned + po + 8, 8);
     s->vmax = std::sqrt(v2);
     s->box_size_list = 2 * c;
     s->have_ready = true;
+}
+
+int ready_list(mdx_system* s) {
+    int rc = ready_launch(s, 0);
+    if (rc) return rc;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    ready_finish(s, 0);
     return MDX_OK;
 }
 
@@ -931,11 +978,158 @@ bool valid_config(const mdx_config* c, std::string& why) {
 
 }  // namespace
 
+// A vxcm sweep run as one batch: members walled systems of one configuration,
+// all on the sweep's stream; each step kernel is launched once for every
+// member (blockIdx.y = member) over a device table of their views.
+struct mdx_sweep {
+    mdx_config cfg{};
+    hipStream_t stream = nullptr;
+    std::vector<mdx_system*> sys;
+    MemberView* dviews = nullptr;    // device table
+    MemberView* hviews = nullptr;    // pinned staging of the table
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_sweep_err;   // errors before a sweep exists
+
+int sweep_fail(mdx_sweep* w, int code, const std::string& msg) {
+    if (w) w->err = msg; else g_sweep_err = msg;
+    return code;
+}
+
+// a member's error, reported by the sweep
+int member_fail(mdx_sweep* w, int32_t m, int code) {
+    return sweep_fail(w, code, "member " + std::to_string(m) + ": " + w->sys[m]->err);
+}
+
+MemberView member_view(const mdx_system* s) {
+    MemberView v{};
+    v.st = s->st;
+    v.cell_of = s->cell_of;
+    v.lj = s->lj.view;
+    v.slot_of_id = s->slot_of_id;
+    v.mb_cell_of = s->mb_cell_of;
+    v.mb_ids = s->mb_ids;
+    v.mb = s->mb.view;
+    v.xi_j = s->xi_j;
+    v.xi_v = s->xi_v;
+    v.xi_cap = s->xi_cap;
+    const int32_t nb = s->cfg.n_bond;
+    v.bond_wave = bond_ref_wave(nb, std::min<int64_t>(27 * (int64_t)s->mb.largest, nb), s->xi_cap) ? 1 : 0;
+    v.hit = s->dint;
+    return v;
+}
+
+// Write the views of members ms to the device table.  The pinned staging is
+// rewritten only after a synchronisation of the stream (a rebuild's
+// read-backs or the end of a run), so the previous copy has completed.
+int upload_views(mdx_sweep* w, const std::vector<int32_t>& ms) {
+    if (ms.empty()) return MDX_OK;
+    for (int32_t m : ms) w->hviews[m] = member_view(w->sys[m]);
+    const size_t bytes = sizeof(MemberView) * w->sys.size();
+    if (hipMemcpyAsync(w->dviews, w->hviews, bytes, hipMemcpyHostToDevice, w->stream) != hipSuccess)
+        return sweep_fail(w, MDX_ERR_HIP, "view table upload failed");
+    return MDX_OK;
+}
+
+// The rebuilds of every member due at its current step (cpu/cpumd/Host.cpp:143-152),
+// with two synchronisations in all instead of three per member: every
+// member's readyList, one wait; every list build up to its sizes, one wait;
+// then the sizes, the next build steps and the xi tables on the host.
+int sweep_rebuild(mdx_sweep* w, const std::vector<int32_t>& due, const std::vector<uint8_t>& lj_due,
+                  const std::vector<uint8_t>& mb_due) {
+    const mdx_config& c = w->cfg;
+    int rc;
+    for (int32_t m : due)
+        if ((rc = ready_launch(w->sys[m], kPinReady))) return member_fail(w, m, rc);
+    if (hipStreamSynchronize(w->stream) != hipSuccess) return sweep_fail(w, MDX_ERR_HIP, "readyList failed");
+    for (int32_t m : due) {
+        mdx_system* s = w->sys[m];
+        ready_finish(s, kPinReady);
+        if (lj_due[m] && (rc = build_primary(s, s->lj, c.lj_rs, s->box_size_list, true, kPinLj)))
+            return member_fail(w, m, rc);
+        if (mb_due[m] && (rc = build_bond_walled(s, s->box_size_list, kPinMb))) return member_fail(w, m, rc);
+    }
+    if (hipStreamSynchronize(w->stream) != hipSuccess) return sweep_fail(w, MDX_ERR_HIP, "list build failed");
+    for (int32_t m : due) {
+        mdx_system* s = w->sys[m];
+        if (lj_due[m]) {
+            if ((rc = sizes_finish(s, s->lj, s->n, kPinLj))) return member_fail(w, m, rc);
+            s->lj.next_build = next_build(c.lj_rs, c.lj_rcut, s->vmax, c.dt, s->t);
+        }
+        if (mb_due[m]) {
+            if ((rc = sizes_finish(s, s->mb, c.n_bond, kPinMb)) || (rc = bond_walled_finish(s)))
+                return member_fail(w, m, rc);
+            s->mb.next_build = next_build(c.mb_rs, c.mb_rcut, s->vmax, c.dt, s->t);
+        }
+    }
+    return upload_views(w, due);
+}
+
+// nsteps of the reference loop body (cpu/cpumd/Host.cpp:141-195) for every
+// member: rebuilds when due, predict, forces, corrector with walls, and the
+// forces and a = F/m again for the members whose corrector hit a wall
+int sweep_run(mdx_sweep* w, int32_t nsteps) {
+    const mdx_config& c = w->cfg;
+    const int32_t R = (int32_t)w->sys.size();
+    mdx_system* s0 = w->sys[0];
+    int rc;
+    std::vector<int32_t> all(R);
+    for (int32_t m = 0; m < R; m++) {
+        all[m] = m;
+        // first call: the member's missing lists, its forces and a = F/m (Host.cpp:132-138)
+        if (!w->sys[m]->initialized && (rc = run_walled(w->sys[m], 0))) return member_fail(w, m, rc);
+    }
+    if ((rc = upload_views(w, all))) return rc;
+    const int32_t assign = c.use_lj ? 0 : 1;   // BondOrderFuncs.h:325-336
+    std::vector<int32_t> due;
+    std::vector<uint8_t> lj_due(R), mb_due(R);
+    for (int32_t k = 0; k < nsteps; k++) {
+        due.clear();
+        for (int32_t m = 0; m < R; m++) {
+            const mdx_system* s = w->sys[m];
+            lj_due[m] = c.use_lj && s->lj.next_build == s->t;
+            mb_due[m] = c.use_bond && s->mb.next_build == s->t;
+            if (lj_due[m] || mb_due[m]) due.push_back(m);
+        }
+        if (!due.empty() && (rc = sweep_rebuild(w, due, lj_due, mb_due))) return rc;
+        bool any_wave = false, any_thread = false;
+        for (int32_t m = 0; m < R; m++) (w->hviews[m].bond_wave ? any_wave : any_thread) = true;
+        for (int pass = 0; pass < 2; pass++) {
+            const bool gated = pass == 1;   // the recompute after a wall hit (Host.cpp:168-178)
+            if (!gated) {
+                LAUNCH(s0, "sweep_predict", launch_sweep_predict(w->stream, w->dviews, R, s0->n, s0->gc));
+            }
+            if (c.use_lj) {
+                LAUNCH(s0, "sweep_lj", launch_sweep_lj(w->stream, w->dviews, R, s0->n, s0->ljt, c.precision, gated));
+            }
+            if (c.use_bond) {
+                LAUNCH(s0, "sweep_bond", launch_sweep_bond(w->stream, w->dviews, R, s0->bp, c.n_bond, c.precision,
+                                                           assign, gated, any_wave, any_thread));
+            }
+            if (!gated) {
+                LAUNCH(s0, "sweep_correct", launch_sweep_correct(w->stream, w->dviews, R, s0->n, s0->gc, s0->mt,
+                                                                 c.wall, c.energy_loss));
+            } else {
+                LAUNCH(s0, "sweep_accelerations", launch_sweep_accelerations(w->stream, w->dviews, R, s0->n, s0->mt));
+            }
+        }
+        for (mdx_system* s : w->sys) s->t++;
+    }
+    if (hipStreamSynchronize(w->stream) != hipSuccess) return sweep_fail(w, MDX_ERR_HIP, "sweep step failed");
+    return MDX_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* mdx_last_error(const mdx_system* s) { return s ? s->err.c_str() : g_err.c_str(); }
 
-int mdx_create(const mdx_config* cfg, mdx_system** out) {
+// mdx_create on a stream of its own (shared == nullptr) or on the caller's
+static int create_system(const mdx_config* cfg, hipStream_t shared, mdx_system** out) {
     if (!out) return fail(nullptr, MDX_ERR_ARG, "out is null");
     *out = nullptr;
     std::string why;
@@ -952,8 +1146,12 @@ int mdx_create(const mdx_config* cfg, mdx_system** out) {
         mdx_destroy(s);
         return code;
     };
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+    if (shared) {
+        s->stream = shared;
+        s->owns_stream = false;
+    } else if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
         return bail(fail(nullptr, MDX_ERR_HIP, "hipStreamCreate failed"));
+    }
     const int n = s->n;
     if ((rc = alloc_state(s, s->st)) || (rc = alloc_state(s, s->alt))) { g_err = s->err; return bail(rc); }
     if ((rc = dalloc(s, &s->slot_of_id, n)) || (rc = dalloc(s, &s->cell_of, n)) ||
@@ -1014,8 +1212,9 @@ int mdx_create(const mdx_config* cfg, mdx_system** out) {
     return MDX_OK;
 }
 
-int mdx_destroy(mdx_system* s) {
-    if (!s) return MDX_OK;
+int mdx_create(const mdx_config* cfg, mdx_system** out) { return create_system(cfg, nullptr, out); }
+
+static void destroy_system(mdx_system* s) {
     if (s->stream) hipStreamSynchronize(s->stream);
     for (auto& p : s->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
     for (auto e : s->free_events) hipEventDestroy(e);
@@ -1024,8 +1223,14 @@ int mdx_destroy(mdx_system* s) {
         if (e) hipEventDestroy(e);
     if (s->pinned) hipHostFree(s->pinned);
     if (s->hflags) hipHostFree(s->hflags);
-    if (s->stream) hipStreamDestroy(s->stream);
+    if (s->stream && s->owns_stream) hipStreamDestroy(s->stream);
     delete s;
+}
+
+int mdx_destroy(mdx_system* s) {
+    if (!s) return MDX_OK;
+    if (s->sweep_member) return fail(s, MDX_ERR_STATE, "a sweep member is destroyed with its sweep (mdx_sweep_destroy)");
+    destroy_system(s);
     return MDX_OK;
 }
 
@@ -1442,6 +1647,73 @@ int mdx_synchronize(mdx_system* s) {
 int mdx_launch_shape(mdx_system* s, mdx_shape* out) {
     if (!s || !out) return fail(s, MDX_ERR_ARG, "null argument");
     *out = s->shape;
+    return MDX_OK;
+}
+
+const char* mdx_sweep_last_error(const mdx_sweep* w) { return w ? w->err.c_str() : g_sweep_err.c_str(); }
+
+int mdx_sweep_create(const mdx_config* cfg, int32_t members, mdx_sweep** out) {
+    if (!out) return sweep_fail(nullptr, MDX_ERR_ARG, "out is null");
+    *out = nullptr;
+    if (members < 1) return sweep_fail(nullptr, MDX_ERR_ARG, "a sweep needs at least one member");
+    if (members > 65535) return sweep_fail(nullptr, MDX_ERR_ARG, "a sweep holds at most 65535 members");
+    std::string why;
+    if (!valid_config(cfg, why)) return sweep_fail(nullptr, MDX_ERR_ARG, why);
+    if (cfg->box != MDX_BOX_WALLED) return sweep_fail(nullptr, MDX_ERR_ARG, "sweep members are walled systems");
+    mdx_sweep* w = new mdx_sweep();
+    w->cfg = *cfg;
+    auto bail = [&](int code, const std::string& msg) {
+        mdx_sweep_destroy(w);
+        return sweep_fail(nullptr, code, msg);
+    };
+    if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(MDX_ERR_HIP, "hipStreamCreate failed");
+    for (int32_t m = 0; m < members; m++) {
+        mdx_system* s = nullptr;
+        int rc = create_system(cfg, w->stream, &s);
+        if (rc) return bail(rc, "member " + std::to_string(m) + ": " + g_err);
+        s->sweep_member = true;
+        w->sys.push_back(s);
+    }
+    const size_t bytes = sizeof(MemberView) * members;
+    if (hipMalloc((void**)&w->dviews, bytes) != hipSuccess) return bail(MDX_ERR_ALLOC, "hipMalloc failed");
+    if (hipHostMalloc((void**)&w->hviews, bytes, hipHostMallocDefault) != hipSuccess)
+        return bail(MDX_ERR_ALLOC, "hipHostMalloc failed");
+    std::memset(w->hviews, 0, bytes);
+    *out = w;
+    return MDX_OK;
+}
+
+int mdx_sweep_destroy(mdx_sweep* w) {
+    if (!w) return MDX_OK;
+    if (w->stream) hipStreamSynchronize(w->stream);
+    for (mdx_system* s : w->sys) destroy_system(s);
+    if (w->dviews) hipFree(w->dviews);
+    if (w->hviews) hipHostFree(w->hviews);
+    if (w->stream) hipStreamDestroy(w->stream);
+    delete w;
+    return MDX_OK;
+}
+
+int mdx_sweep_member(mdx_sweep* w, int32_t member, mdx_system** out) {
+    if (!w || !out) return sweep_fail(w, MDX_ERR_ARG, "null argument");
+    if (member < 0 || member >= (int32_t)w->sys.size()) return sweep_fail(w, MDX_ERR_ARG, "member out of range");
+    *out = w->sys[member];
+    return MDX_OK;
+}
+
+int mdx_sweep_run(mdx_sweep* w, int32_t nsteps) {
+    if (!w) return sweep_fail(w, MDX_ERR_ARG, "null sweep");
+    if (nsteps < 0) return sweep_fail(w, MDX_ERR_ARG, "nsteps must be >= 0");
+    return sweep_run(w, nsteps);
+}
+
+int mdx_sweep_set_timing(mdx_sweep* w, int32_t enabled) {
+    if (!w) return sweep_fail(w, MDX_ERR_ARG, "null sweep");
+    for (mdx_system* s : w->sys) {
+        int rc = mdx_set_timing(s, enabled);
+        if (rc) return sweep_fail(w, rc, s->err);
+    }
     return MDX_OK;
 }
 

@@ -163,7 +163,7 @@ bool readConfiguration(const std::string& filename, Configuration* c, std::strin
     cp.GetDouble("LJSigma", c->LJSigma);
     cp.GetDouble("LJEpsilon", c->LJEpsilon);
     cp.GetDouble("Mass", c->Mass);
-    cp.GetInt("SweepStreams", c->SweepStreams);
+    cp.GetInt("SweepBatch", c->SweepBatch);
     return true;
 }
 

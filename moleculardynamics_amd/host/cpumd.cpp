@@ -2,28 +2,25 @@
 // engine through include/mdx.h.  Each vxcm member keeps the reference's order
 // of events (initial lists, initial output, timer, initial forces, steps,
 // in-loop outputs at printout == t, animation frames, final output, files);
-// steps between two events run device-resident in one mdx_run call.
+// steps between two events run device-resident.
 //
 // The reference runs the sweep's members one after another
-// (Host.cpp:75-216).  They are independent systems, and one small system
-// leaves most of the GPU idle, so here up to SweepStreams members run at
-// once, each on a host thread with its own mdx_system (and so its own HIP
-// stream).  Every member's text is buffered and printed, and its files are
-// written, in sweep order by the main thread: stdout and the files are those
-// of the sequential sweep, except that "Time took" is each member's own wall
-// time while it shared the GPU.
+// (Host.cpp:75-216).  They are independent systems of one configuration, and
+// one small system leaves most of the GPU idle, so here up to SweepBatch
+// members form one mdx_sweep: every step kernel is launched once for all of
+// them, each member keeping its own list schedule and wall flag.  Members of
+// a batch share their events, so they advance in lockstep between them.
+// Every member's text is printed, and its files are written, in sweep order:
+// stdout and the files are those of the sequential sweep, except that "Time
+// took" is the wall time of the member's batch.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
-#include <condition_variable>
 #include <cstdio>
-#include <future>
 #include <iomanip>
 #include <iostream>
 #include <memory>
-#include <mutex>
 #include <sstream>
-#include <thread>
 
 #include "mdx.h"
 #include "mdx_host.h"
@@ -93,8 +90,10 @@ mdx_config make_config(const Configuration& c, int n, double boxSize) {
     return mc;
 }
 
-// The body of the sweep loop (Host.cpp:84-197) for one vxcm on system s.
-bool run_member(mdx_system* s, const Configuration& c, int n, double boxSize, Member& m) {
+// The sweep loop's set-up for one vxcm on system s (Host.cpp:84-128): read
+// the input, clamp it into +-boxSize/2, add vxcm, build the lists and write
+// the initial output.
+bool set_up_member(mdx_system* s, const Configuration& c, int n, double boxSize, Member& m) {
     const bool periodic = c.BoxLength > 0;
     m.pos.assign(4 * (size_t)n, 0.0);
     m.vel.assign(3 * (size_t)n, 0.0);
@@ -121,37 +120,61 @@ bool run_member(mdx_system* s, const Configuration& c, int n, double boxSize, Me
         return report(s, m, "buildList (many body)");
     if (!perform_output(s, m, 0, c.DT, true, periodic)) return report(s, m, "performOutput");
     out << std::endl;
-    if (c.animts > -1) pushAnim(m.posVec, m.velVec, pos, vel, n);
+    if (c.animts > -1) pushAnim(m.posVec, m.velVec, m.pos.data(), m.vel.data(), n);
+    return true;
+}
 
+// The members of one batch through the sweep loop's body (Host.cpp:84-197)
+// in lockstep: every member has the same events (in-loop outputs at
+// printout == t, animation frames), so the steps between two events advance
+// all of them at once (advance(k): k steps of every member).  Each member's
+// "Time took" is the batch's wall time.  Returns false when a member failed
+// (its err says why) or advance did (it keeps its own message).
+template <class Advance>
+bool run_batch(const std::vector<mdx_system*>& sys, const std::vector<Member*>& ms, const Configuration& c, int n,
+               double boxSize, Advance&& advance) {
+    const bool periodic = c.BoxLength > 0;
+    const size_t R = sys.size();
+    for (size_t k = 0; k < R; k++)
+        if (!set_up_member(sys[k], c, n, boxSize, *ms[k])) return false;
     auto start = std::chrono::steady_clock::now();
-    if (mdx_run(s, 0) != MDX_OK) return report(s, m, "initial forces");  // Host.cpp:132-138
+    if (!advance(0, "initial forces")) return false;   // Host.cpp:132-138
     int t = 0, printout = c.OutputTimesteps, animt = c.animts;
     while (t < c.Timesteps) {
         int e = c.Timesteps - 1;
         if (printout >= t && printout < e) e = printout;
         if (animt >= t && animt < e) e = animt;
-        if (mdx_run(s, e - t + 1) != MDX_OK) return report(s, m, "step loop");
+        if (!advance(e - t + 1, "step loop")) return false;
         t = e + 1;
         if (printout == e) {
-            if (!perform_output(s, m, e, c.DT, c.Debug, periodic)) return report(s, m, "performOutput");
+            for (size_t k = 0; k < R; k++)
+                if (!perform_output(sys[k], *ms[k], e, c.DT, c.Debug, periodic))
+                    return report(sys[k], *ms[k], "performOutput");
             printout += c.OutputTimesteps;
         }
         if (animt == e) {
-            if (mdx_get_state(s, pos, vel, nullptr, nullptr, nullptr, nullptr) != MDX_OK)
-                return report(s, m, "mdx_get_state");
-            pushAnim(m.posVec, m.velVec, pos, vel, n);
+            for (size_t k = 0; k < R; k++) {
+                Member& m = *ms[k];
+                if (mdx_get_state(sys[k], m.pos.data(), m.vel.data(), nullptr, nullptr, nullptr, nullptr) != MDX_OK)
+                    return report(sys[k], m, "mdx_get_state");
+                pushAnim(m.posVec, m.velVec, m.pos.data(), m.vel.data(), n);
+            }
             animt += c.animts;
         }
     }
-    mdx_synchronize(s);
+    for (mdx_system* s : sys) mdx_synchronize(s);
     auto stop = std::chrono::steady_clock::now();
-    m.elapsed = std::chrono::duration<double, std::milli>(stop - start).count();
-
-    out << "\nAfter Simulation [VXCM = " << m.vxcm << "]:" << std::endl;
-    if (!perform_output(s, m, t, c.DT, true, periodic)) return report(s, m, "performOutput");
-    out << "\nTime took: " << (long long)m.elapsed << "ms" << std::endl << std::endl;
-    if (mdx_get_state(s, pos, vel, nullptr, nullptr, nullptr, nullptr) != MDX_OK)
-        return report(s, m, "mdx_get_state");
+    const double elapsed = std::chrono::duration<double, std::milli>(stop - start).count();
+    for (size_t k = 0; k < R; k++) {
+        Member& m = *ms[k];
+        m.elapsed = elapsed;
+        m.out << "\nAfter Simulation [VXCM = " << m.vxcm << "]:" << std::endl;
+        if (!perform_output(sys[k], m, t, c.DT, true, periodic)) return report(sys[k], m, "performOutput");
+        m.out << "\nTime took: " << (long long)m.elapsed << "ms" << std::endl << std::endl;
+        if (mdx_get_state(sys[k], m.pos.data(), m.vel.data(), nullptr, nullptr, nullptr, nullptr) != MDX_OK)
+            return report(sys[k], m, "mdx_get_state");
+        m.ok = true;
+    }
     return true;
 }
 
@@ -175,77 +198,76 @@ int hostMain(Configuration* c) {
     const mdx_config mc = make_config(*c, n, boxSize);
     const std::vector<float> values = sweepValues(*c);
     const int count = (int)values.size();
-    const int workers = std::max(1, std::min(c->SweepStreams, count));
+    const bool periodic = c->BoxLength > 0;
+    // walled members run as one mdx_sweep per batch; the periodic box (not a
+    // configuration of the reference) one member at a time
+    const int batch = periodic ? 1 : std::max(1, std::min(c->SweepBatch, count));
     const Configuration& cfg = *c;
 
-    std::vector<std::unique_ptr<Member>> members(count);
-    std::vector<std::promise<void>> done(count);
-    std::vector<std::future<void>> ready;
-    for (int i = 0; i < count; i++) {
-        members[i].reset(new Member);
-        members[i]->vxcm = values[i];
-        ready.push_back(done[i].get_future());
-    }
-    // Workers take members in sweep order, at most 2 x workers ahead of the
-    // last member written out, so finished members do not pile up in memory.
-    std::mutex mu;
-    std::condition_variable cv;
-    int next = 0, written = 0;
-    bool stop = false;
-    auto worker = [&]() {
-        mdx_system* s = nullptr;
-        std::string create_err;
-        if (mdx_create(&mc, &s) != MDX_OK) create_err = std::string("mdx_create: ") + mdx_last_error(nullptr);
-        for (;;) {
-            int i;
-            {
-                std::unique_lock<std::mutex> lk(mu);
-                cv.wait(lk, [&] { return stop || next >= count || next < written + 2 * workers; });
-                if (stop || next >= count) break;
-                i = next++;
+    for (int b0 = 0; b0 < count; b0 += batch) {
+        const int nb = std::min(batch, count - b0);
+        std::vector<std::unique_ptr<Member>> members(nb);
+        std::vector<Member*> ms(nb);
+        for (int k = 0; k < nb; k++) {
+            members[k].reset(new Member);
+            members[k]->vxcm = values[b0 + k];
+            ms[k] = members[k].get();
+        }
+        std::vector<mdx_system*> sys(nb, nullptr);
+        mdx_sweep* w = nullptr;
+        mdx_system* single = nullptr;
+        std::string batch_err;
+        bool ok = true;
+        if (periodic) {
+            if (mdx_create(&mc, &single) != MDX_OK) {
+                batch_err = std::string("mdx_create: ") + mdx_last_error(nullptr);
+                ok = false;
             }
-            Member& m = *members[i];
-            if (!create_err.empty()) m.err = create_err;
-            else m.ok = run_member(s, cfg, n, boxSize, m);
-            done[i].set_value();
-        }
-        if (s) mdx_destroy(s);
-    };
-    std::vector<std::thread> pool;
-    for (int w = 0; w < workers; w++) pool.emplace_back(worker);
-
-    int status = EXIT_SUCCESS;
-    for (int i = 0; i < count; i++) {
-        ready[i].wait();
-        Member& m = *members[i];
-        std::cout << m.out.str() << std::flush;
-        std::string err = m.err;
-        bool ok = m.ok;
-        if (ok && (!writeOutput(cfg, m.pos.data(), m.vel.data(), &err) ||
-                   !writeResults(m.rv, cfg, (float)m.elapsed, tailResults(m.pos.data(), cfg.ManyBodyParticles),
-                                 m.vxcm, &err) ||
-                   (cfg.animts > -1 && !writeAnimationBinaryData(m.posVec, m.velVec, cfg, m.vxcm, &err))))
+            sys[0] = single;
+        } else if (mdx_sweep_create(&mc, nb, &w) != MDX_OK) {
+            batch_err = std::string("mdx_sweep_create: ") + mdx_sweep_last_error(nullptr);
             ok = false;
-        members[i].reset();
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            written = i + 1;
-            if (!ok) stop = true;
+        } else {
+            for (int k = 0; k < nb; k++) mdx_sweep_member(w, k, &sys[k]);
         }
-        cv.notify_all();
-        if (!ok) {
-            std::cerr << "Error: " << err << std::endl;
-            status = EXIT_FAILURE;
-            break;
+        if (ok) {
+            auto advance = [&](int steps, const char* what) {
+                int rc = periodic ? mdx_run(single, steps) : mdx_sweep_run(w, steps);
+                if (rc != MDX_OK) {
+                    batch_err = std::string(what) + ": " + (periodic ? mdx_last_error(single) : mdx_sweep_last_error(w));
+                    return false;
+                }
+                return true;
+            };
+            ok = run_batch(sys, ms, cfg, n, boxSize, advance);
+        }
+        if (w) mdx_sweep_destroy(w);
+        if (single) mdx_destroy(single);
+        // output in sweep order; the first failure ends the sweep, and a
+        // member stopped by another's failure reports that one's error
+        std::string first_err = batch_err;
+        for (Member* m : ms)
+            if (!m->err.empty()) {
+                first_err = m->err;
+                break;
+            }
+        for (int k = 0; k < nb; k++) {
+            Member& m = *ms[k];
+            std::cout << m.out.str() << std::flush;
+            std::string err = m.err.empty() ? first_err : m.err;
+            bool mok = m.ok;
+            if (mok && (!writeOutput(cfg, m.pos.data(), m.vel.data(), &err) ||
+                        !writeResults(m.rv, cfg, (float)m.elapsed, tailResults(m.pos.data(), cfg.ManyBodyParticles),
+                                      m.vxcm, &err) ||
+                        (cfg.animts > -1 && !writeAnimationBinaryData(m.posVec, m.velVec, cfg, m.vxcm, &err))))
+                mok = false;
+            if (!mok) {
+                std::cerr << "Error: " << err << std::endl;
+                return EXIT_FAILURE;
+            }
         }
     }
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        stop = true;
-    }
-    cv.notify_all();
-    for (auto& th : pool) th.join();
-    return status;
+    return EXIT_SUCCESS;
 }
 
 }  // namespace mdxh

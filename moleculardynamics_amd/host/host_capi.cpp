@@ -29,7 +29,7 @@ std::string to_json(const mdxh::Configuration& c) {
       << ",\"energyLoss\":" << c.energyLoss << ",\"animts\":" << c.animts << ",\"Input\":\"" << c.Input
       << "\",\"Output\":\"" << c.Output << "\",\"Filename\":\"" << c.Filename << "\",\"AnimFile\":\"" << c.AnimFile
       << "\",\"BoxLength\":" << c.BoxLength << ",\"Precision\":" << c.Precision
-      << ",\"SweepStreams\":" << c.SweepStreams << "}";
+      << ",\"SweepBatch\":" << c.SweepBatch << "}";
     return j.str();
 }
 

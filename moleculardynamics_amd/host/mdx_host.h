@@ -39,7 +39,7 @@ struct Configuration {
     double BoxLength = 0;       // BOXLENGTH: > 0 selects the periodic box
     int Precision = 1;          // PRECISION: 0 mixed (FP32 pair), 1 FP64 (the reference's real = double)
     double LJSigma = 1, LJEpsilon = 1, Mass = 1;  // LJSIGMA / LJEPSILON / MASS for the periodic box
-    int SweepStreams = 4;       // SWEEPSTREAMS: vxcm sweep members in flight at once, one stream each
+    int SweepBatch = 8;         // SWEEPBATCH: vxcm sweep members advanced together (one launch per step kernel)
 };
 
 // common/ConfigParser.cpp:22-98 semantics: "//" comment lines and trailing

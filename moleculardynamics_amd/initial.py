@@ -96,3 +96,39 @@ def replica_batch(n_replicas: int, seed: int, n_min: int = 256, n_max: int = 819
         pos.append(p)
         vel.append(v)
     return np.array(ns, dtype=np.int32), np.array(boxes), np.concatenate(pos), np.concatenate(vel)
+
+
+# the reference's walled N/O/Ne gas: species codes (common/Constants.h:13-15),
+# masses in amu (common/Constants.h:36-38) and kB in its energy units
+# (common/DebugCalculations.cpp:122)
+SPECIES_N, SPECIES_O, SPECIES_NE = 0, 1, 2
+REF_MASS = {SPECIES_N: 14.00674, SPECIES_O: 15.9994, SPECIES_NE: 20.1797}
+REF_KB = 8.314e-7
+
+
+def walled_mixture(n_mol: int, n_ne: int, temperature_k: float, seed: int, spacing: float = 3.3,
+                   bond: float = 1.14):
+    """n_mol diatomic molecules (N2, O2 or NO, bond length `bond` A) and n_ne Ne
+    atoms on a jittered cubic grid of `spacing` A around the origin, with
+    Maxwell-Boltzmann velocities at temperature_k (reference units), all from
+    one seed.  The molecules' atoms come first, as the reference's bond-order
+    atoms must (cpu/cpumd/Host.cpp:120).  Returns (pos4 with the species in
+    column 3, vel3)."""
+    rng = np.random.default_rng(seed)
+    sites = n_mol + n_ne
+    k = int(np.ceil(sites ** (1 / 3)))
+    g = np.stack(np.meshgrid(*[np.arange(k)] * 3, indexing="ij"), -1).reshape(-1, 3)[:sites].astype(float)
+    g = (g - (k - 1) / 2) * spacing + rng.uniform(-0.1, 0.1, g.shape) * spacing
+    rng.shuffle(g)
+    kinds = np.array([(SPECIES_N, SPECIES_N), (SPECIES_O, SPECIES_O), (SPECIES_N, SPECIES_O)])
+    u = rng.normal(size=(n_mol, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    mk = kinds[rng.integers(0, 3, n_mol)]
+    a = np.c_[g[:n_mol] + 0.5 * bond * u, mk[:, 0]]
+    b = np.c_[g[:n_mol] - 0.5 * bond * u, mk[:, 1]]
+    mol = np.stack([a, b], 1).reshape(-1, 4)
+    ne = np.c_[g[n_mol:], np.full(n_ne, SPECIES_NE)]
+    pos4 = np.concatenate([mol, ne]).astype(np.float64)
+    m = np.array([REF_MASS[int(t)] for t in pos4[:, 3]])
+    vel = rng.normal(size=(len(pos4), 3)) * np.sqrt(REF_KB * temperature_k / m)[:, None]
+    return pos4, vel

@@ -60,3 +60,29 @@ def test_create_without_device_fails_cleanly():
     cfg.lj_rcut = 3.0
     assert lib.mdx_create(C.byref(cfg), C.byref(h)) == -1
     assert b"RCUT" in lib.mdx_last_error(None)
+
+
+def test_sweep_create_checks_arguments_before_the_device():
+    # a sweep holds walled members only, at least one; both are checked before
+    # any device call, and a valid sweep on a machine without a GPU reports
+    # MDX_ERR_HIP instead of crashing
+    lib = _lib.load()
+    cfg = _lib.MdxConfig()
+    cfg.n_atoms, cfg.n_bond = 8, 2
+    cfg.box = _lib.BOX_WALLED
+    cfg.use_lj, cfg.use_bond = 1, 1
+    cfg.lj_rs, cfg.lj_rcut, cfg.mb_rs, cfg.mb_rcut, cfg.dt = 7.6, 6.8, 3.6, 3.0, 1.0
+    cfg.wall, cfg.energy_loss = 100.0, 1.0
+    h = C.c_void_p()
+    assert lib.mdx_sweep_create(C.byref(cfg), 0, C.byref(h)) == -1
+    assert b"member" in lib.mdx_sweep_last_error(None)
+    rc = lib.mdx_sweep_create(C.byref(cfg), 3, C.byref(h))
+    if rc == 0:
+        lib.mdx_sweep_destroy(h)
+    else:
+        assert rc == -2, lib.mdx_sweep_last_error(None)
+    cfg.box = _lib.BOX_PERIODIC
+    cfg.box_length = 20.0
+    cfg.use_bond = 0
+    assert lib.mdx_sweep_create(C.byref(cfg), 3, C.byref(h)) == -1
+    assert b"walled" in lib.mdx_sweep_last_error(None)

@@ -4,9 +4,10 @@ input and the .dat/.txt/.anm/Output files of the reference
 (cpu/cpumd/Host.cpp:14-240, common/Loader.cpp, common/Results.cpp).
 
 Each member's energy records must equal the oracle's run of the reference
-loop for that vxcm (FP64, 1e-9 relative); running the members concurrently
-(SweepStreams=3) must give the same files and stdout, byte for byte, as
-running them one after another (SweepStreams=1), apart from the wall times.
+loop for that vxcm (FP64, 1e-9 relative); running the members as one batch
+(SweepBatch=3: every step kernel launched once for all members) must give the
+same files and stdout, byte for byte, as running them one after another
+(SweepBatch=1), apart from the wall times.
 """
 import re
 import struct
@@ -27,7 +28,7 @@ LJ, MB = (7.6, 6.8), (3.6, 3.0)
 ELOSS = 0.81
 
 
-def config_text(n_lj, n_mb, streams, **over):
+def config_text(n_lj, n_mb, batch, **over):
     keys = {
         "Lennard Jones Particles": n_lj, "Many Body Particles": n_mb, "Timesteps": STEPS, "DT": 1,
         "Temperature": 300, "LennardJonesRS": LJ[0], "LennardJonesRCUT": LJ[1], "ManyBodyRS": MB[0],
@@ -35,7 +36,7 @@ def config_text(n_lj, n_mb, streams, **over):
         "Use Cuda": 1, "UseLennardJones": 1, "UseManyBody": 1, "LennardJonesBPP": 0, "ManyBodyBPP": 0,
         "OutputTimesteps": EVERY, "Debug": 1, "Fallback": 1, "VXCMFROM": 0, "VXCMTO": 0.003,
         "VXCMSTEP": 0.0015, "EnergyLoss": ELOSS, "AnimationTimesteps": ANIM, "AnimationData": "anim",
-        "SweepStreams": streams,
+        "SweepBatch": batch,
     }
     keys.update(over)
     return "".join(f"{k}={v}\n" for k, v in keys.items())
@@ -52,13 +53,13 @@ def read_dat(path):
     return nop, steps, vxcm, np.frombuffer(b[off + 4:], dtype="<f8").reshape(-1, 11)
 
 
-def run_sweep(tmp, pos4, vel, n_lj, n_mb, streams, **over):
+def run_sweep(tmp, pos4, vel, n_lj, n_mb, batch, **over):
     tmp.mkdir()
     with open(tmp / "in.txt", "w") as f:
         for p, v in zip(pos4, vel):
             f.write(" ".join(f"{x:.17g}" for x in (*p, *v)) + "\n")
-    (tmp / "sweep.cfg").write_text(config_text(n_lj, n_mb, streams, **over))
-    print(f"mdx_cpumd: {n_lj} LJ + {n_mb} bond atoms, SweepStreams={streams}", flush=True)
+    (tmp / "sweep.cfg").write_text(config_text(n_lj, n_mb, batch, **over))
+    print(f"mdx_cpumd: {n_lj} LJ + {n_mb} bond atoms, SweepBatch={batch}", flush=True)
     r = subprocess.run([str(CPUMD), "sweep.cfg"], cwd=tmp, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     return r.stdout
@@ -99,7 +100,7 @@ def test_sweep_records_match_oracle(sweeps):
     assert sorted(seen) == pytest.approx([0.0, 0.0015, 0.003])
 
 
-def test_concurrent_sweep_equals_sequential(sweeps):
+def test_batched_sweep_equals_sequential(sweeps):
     _, _, _, _, one, three, out1, out3 = sweeps
     assert out1.count("Before Simulation") == 3
     assert without_times(out1) == without_times(out3)

@@ -77,15 +77,15 @@ def test_read_configuration_all_keys(host, tmp_path):
     assert c["useLennardJones"] == 1 and c["Fallback"] == 1 and c["animts"] == -1
     assert c["vxcmTo"] == pytest.approx(0.02) and c["Input"] == "in.txt" and c["AnimFile"] == "anim"
     assert c["BoxLength"] == pytest.approx(107.49) and c["Precision"] == 0
-    assert c["SweepStreams"] == 4                  # the default: all of HIP's 4 hardware queues
+    assert c["SweepBatch"] == 8                    # the default: members advanced together
 
 
-def test_sweep_streams_key(host, tmp_path):
+def test_sweep_batch_key(host, tmp_path):
     p = tmp_path / "default.cfg"
-    write_cfg(p, {**KEYS, "SweepStreams": "2"})
+    write_cfg(p, {**KEYS, "SweepBatch": "2"})
     buf = C.create_string_buffer(8192)
     assert host.mdxh_read_configuration(str(p).encode(), buf, 8192) == 0
-    assert json.loads(buf.value.decode())["SweepStreams"] == 2
+    assert json.loads(buf.value.decode())["SweepBatch"] == 2
 
 
 def float32_sweep(a, b, step):
