@@ -568,9 +568,14 @@ __device__ inline double pick3(int32_t q, double a, double b, double c) { return
 // streams through once per step (about 170 MB at 1M atoms), and plain loads
 // would push out of the XCD's L2 the positions that the segments of the
 // neighbouring columns stage again soon after (md_step 175 -> 168 us at 1M atoms).
+// The pointer is cast to the global address space: the step kernel hides
+// the look-ahead address from the compiler, which then no longer knows it is
+// global and would emit a flat load, whose completion the waits can only
+// track together with the LDS reads (so every pair would wait for the list).
 __device__ inline uint2 load_list_chunk(const uint2* p) {
     typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-    const u32x2 t = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p));
+    typedef __attribute__((address_space(1))) const u32x2 global_u32x2;
+    const u32x2 t = __builtin_nontemporal_load((global_u32x2*)reinterpret_cast<uintptr_t>(p));
     return make_uint2(t.x, t.y);
 }
 
@@ -741,17 +746,27 @@ k_step_lds(StepArgs a) {
         d2max = fmax(d2max, d2);
         mv2max = fmax(mv2max, mv2);
     }
-    // block reduction through LDS atomics (non-negative doubles order like
-    // their bits), then one spread global atomic per block (the single box)
-    // or one into the replica's displacement accumulator (the exact rebuild
-    // decision of replicas.hip; stale atoms cannot occur there)
-    if (q == 0) {
-        if (!kFinal) {
-            atomicMax(&S.red[0], (unsigned long long)__double_as_longlong(d2max));
-            atomicMax(&S.red[1], (unsigned long long)__double_as_longlong(mv2max));
+    // block reduction (non-negative doubles order like their bits): the
+    // wave's largest values by shuffles across its quads (the lanes of a quad
+    // hold equal values), one LDS atomic per wave, then one spread global
+    // atomic per block (the single box) or one into the replica's
+    // displacement accumulator (the exact rebuild decision of replicas.hip;
+    // stale atoms cannot occur there).  An atomic issued by the 16 quad
+    // leaders of a wave would run as a loop over them, one lane at a time.
+    if constexpr (!kFinal) {
+        unsigned long long b0 = (unsigned long long)__double_as_longlong(d2max);
+        unsigned long long b1 = (unsigned long long)__double_as_longlong(mv2max);
+        for (int o = kLanes; o < kWave; o <<= 1) {
+            const unsigned long long c0 = __shfl_xor(b0, o, kWave), c1 = __shfl_xor(b1, o, kWave);
+            b0 = c0 > b0 ? c0 : b0;
+            b1 = c1 > b1 ? c1 : b1;
         }
-        if (stale > 0) atomicAdd(&S.red[2], (unsigned long long)stale);
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+            atomicMax(&S.red[0], b0);
+            atomicMax(&S.red[1], b1);
+        }
     }
+    if (q == 0 && stale > 0) atomicAdd(&S.red[2], (unsigned long long)stale);
     __syncthreads();
     if (threadIdx.x == 0) {
         if constexpr (kRep) {

@@ -50,6 +50,15 @@ __device__ inline int32_t quad_excl(int32_t h, int32_t q, int32_t& total) {
     return incl - h;
 }
 
+// A 32-bit load from global memory at an address the compiler cannot see
+// through (the look-ahead loads hide their address behind an empty asm so
+// they stay one iteration ahead).  The cast keeps the load a global one: a
+// flat load's completion is tracked together with the LDS reads', so every
+// LDS wait after it would wait for this load too.
+__device__ inline int32_t load_global_i32(uint64_t addr) {
+    return *(__attribute__((address_space(1))) const int32_t*)(uintptr_t)addr;
+}
+
 // lane l's value of v in every lane (l uniform across the wavefront)
 __device__ inline double lane_double(double v, int l) {
     return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),

@@ -398,7 +398,7 @@ k_rep_step(RepStep a) {
             // the compiler from moving the load to the top of the next iteration
             uint64_t pa = reinterpret_cast<uint64_t>(lp + (int64_t)(k + 4 < cnt ? k + 4 : k) * a.ntot);
             asm volatile("" : "+v"(pa));
-            jn = *reinterpret_cast<const int32_t*>(pa);
+            jn = load_global_i32(pa);
             const double4 pj = a.rq[j];
             const double dx = mi(xi - pj.x, L, invL), dy = mi(yi - pj.y, L, invL), dz = mi(zi - pj.z, L, invL);
             if constexpr (kFp64) {
