@@ -365,7 +365,180 @@ __global__ void k_scatter3(const double* __restrict__ s0, const double* __restri
     d2[k] = s2[i];
 }
 
+// ---- the whole build of a small walled grid in one workgroup ----
+// The kernels above run a grid build as ~15 launches, which for the walled
+// box's few thousand atoms and few thousand cells costs launch latency, not
+// work.  Here one workgroup of kOneBlock threads does every phase in LDS,
+// separated by barriers, and writes the same outputs: the same cells, prefix
+// sums, ascending keys inside each cell, permutation and sizes.  The
+// histogram lives in LDS (global atomics from one workgroup would queue on a
+// few hundred addresses); after the scan it holds each cell's start and,
+// after the scatter, each cell's end, which the later phases read as the
+// cells' ranges.
+constexpr int kOneBlock = 1024;
+
+__device__ inline int32_t one_block_excl_scan(int32_t x, int32_t* wsum, int32_t* total) {
+    const int32_t incl = wave_incl_scan(x);
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    if (lane == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    int32_t off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kOneBlock / kWave; i++) {
+        tot += wsum[i];
+        if (i < w) off += wsum[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + incl - x;
+}
+
+__global__ void __launch_bounds__(kOneBlock)
+k_small_build(SmallBuild b) {
+    __shared__ int32_t cell_l[kSmallBuildItems];   // cell of item q
+    __shared__ int32_t item_l[kSmallBuildItems];   // scattered items, cell-ordered ranges
+    __shared__ int32_t key_l[kSmallBuildItems];    // their keys
+    __shared__ int32_t sorted_l[kSmallBuildItems]; // items in cell order, ascending key inside a cell
+    __shared__ int32_t cnt_l[kSmallBuildCells + 1];  // histogram -> cell starts -> cell ends
+    __shared__ int32_t wsum[kOneBlock / kWave];
+    __shared__ int32_t wmax[2][kOneBlock / kWave];
+    __shared__ int32_t bad_any;
+    const int32_t tid = threadIdx.x;
+    const GridView& g = b.g;
+    const int32_t ncell = (int32_t)b.ncell;
+    if (tid == 0) bad_any = 0;
+    for (int32_t c = tid; c <= ncell; c += kOneBlock) cnt_l[c] = 0;
+    __syncthreads();
+    // bin (k_bin): an atom outside the grid is reported and counted in cell 0
+    int32_t bad = 0;
+    for (int32_t q = tid; q < b.items; q += kOneBlock) {
+        const int32_t sl = b.item_slot ? b.item_slot[q] : q;
+        int32_t c[3];
+        bool ok = true;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double y = (b.src.r[d][sl] + g.npos) / g.h;
+            ok = ok && (y > -1.0 && y < (double)g.g[d]);
+            c[d] = ok ? (int32_t)y : 0;
+            ok = ok && (c[d] >= 0 && c[d] < g.g[d]);
+        }
+        const int32_t cell = ok ? (c[0] * g.g[1] + c[1]) * g.g[2] + c[2] : 0;
+        bad |= ok ? 0 : 1;
+        cell_l[q] = cell;
+        b.cell_out[q] = cell;
+        atomicAdd(&cnt_l[cell], 1);
+    }
+    if (bad) atomicOr(&bad_any, 1);
+    __syncthreads();
+    // exclusive scan (each thread a contiguous run of cells) into start and
+    // into cnt_l, which then holds each cell's start
+    int32_t* start = const_cast<int32_t*>(g.start);
+    {
+        const int32_t per = (ncell + kOneBlock - 1) / kOneBlock;
+        const int32_t c0 = tid * per, c1 = c0 + per < ncell ? c0 + per : ncell;
+        int32_t sum = 0;
+        for (int32_t c = c0; c < c1; c++) sum += cnt_l[c];
+        int32_t total;
+        int32_t run = one_block_excl_scan(sum, wsum, &total);
+        for (int32_t c = c0; c < c1; c++) {
+            const int32_t k = cnt_l[c];
+            start[c] = run;
+            cnt_l[c] = run;
+            run += k;
+        }
+        if (tid == 0) start[ncell] = total;
+    }
+    __syncthreads();
+    // scatter (any order inside a cell) with each item's key; cnt_l[c]
+    // becomes the end of cell c, i.e. the start of cell c + 1
+    for (int32_t q = tid; q < b.items; q += kOneBlock) {
+        const int32_t pos = atomicAdd(&cnt_l[cell_l[q]], 1);
+        item_l[pos] = q;
+        key_l[pos] = b.key_of ? b.key_of[q] : q;
+    }
+    __syncthreads();
+    // each item's rank among its cell's keys (keys are distinct) is its place
+    for (int32_t p = tid; p < b.items; p += kOneBlock) {
+        const int32_t q = item_l[p], key = key_l[p];
+        const int32_t c = cell_l[q];
+        const int32_t lo = c > 0 ? cnt_l[c - 1] : 0, hi = cnt_l[c];
+        int32_t rank = 0;
+        for (int32_t o = lo; o < hi; o++) rank += key_l[o] < key ? 1 : 0;
+        b.sorted[lo + rank] = q;
+        sorted_l[lo + rank] = q;
+    }
+    __syncthreads();
+    // the state in the new slot order (k_permute), when asked for
+    if (b.dst.r[0]) {
+        for (int32_t k = tid; k < b.items; k += kOneBlock) {
+            const int32_t sl = sorted_l[k];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                b.dst.r[d][k] = b.src.r[d][sl];
+                b.dst.v[d][k] = b.src.v[d][sl];
+                b.dst.a[d][k] = b.src.a[d][sl];
+                b.dst.b[d][k] = b.src.b[d][sl];
+                b.dst.c[d][k] = b.src.c[d][sl];
+                b.dst.f[d][k] = b.src.f[d][sl];
+            }
+            b.dst.type[k] = b.src.type[sl];
+            const int32_t id = b.src.id[sl];
+            b.dst.id[k] = id;
+            b.cell_of_dst[k] = cell_l[sl];
+            b.slot_of_id[id] = k;
+        }
+    }
+    // sizes: the largest cell and the longest row (27 cells, self excluded,
+    // cells past `cells` skipped as in k_row_lengths)
+    auto count_of = [&](int32_t c) { return cnt_l[c] - (c > 0 ? cnt_l[c - 1] : 0); };
+    int32_t mc = 0, ml = 0;
+    for (int32_t c = tid; c < ncell; c += kOneBlock) mc = max(mc, count_of(c));
+    for (int32_t q = tid; q < b.items; q += kOneBlock) {
+        int32_t cx, cy, cz;
+        const int32_t own = cell_l[q];
+        decode_cell(own, g, cx, cy, cz);
+        int32_t len = 0;
+        for (int dx = -1; dx <= 1; dx++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dz = -1; dz <= 1; dz++) {
+                    const int32_t c = neighbour_cell(g, cx + dx, cy + dy, cz + dz);
+                    if (c < 0) continue;
+                    len += count_of(c);
+                    if (c == own) len -= 1;
+                }
+        ml = max(ml, len);
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        mc = max(mc, __shfl_xor(mc, o, kWave));
+        ml = max(ml, __shfl_xor(ml, o, kWave));
+    }
+    if ((tid & (kWave - 1)) == 0) {
+        wmax[0][tid / kWave] = mc;
+        wmax[1][tid / kWave] = ml;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int w = 1; w < kOneBlock / kWave; w++) {
+            mc = max(mc, wmax[0][w]);
+            ml = max(ml, wmax[1][w]);
+        }
+        b.words[0] |= bad_any;
+        b.words[1] = mc;
+        b.words[2] = ml;
+    }
+}
+
 }  // namespace
+
+bool small_build_fits(int32_t items, int64_t ncell) {
+    return items <= kSmallBuildItems && ncell <= kSmallBuildCells;
+}
+
+hipError_t launch_small_build(hipStream_t s, const SmallBuild& b) {
+    if (!small_build_fits(b.items, b.ncell)) return hipErrorInvalidValue;
+    k_small_build<<<1, kOneBlock, 0, s>>>(b);
+    return hipGetLastError();
+}
 
 hipError_t launch_scatter3(hipStream_t s, double* const src[3], const int32_t* idx, double* const dst[3], int32_t n) {
     if (n <= 0) return hipSuccess;

@@ -250,6 +250,27 @@ hipError_t launch_permute(hipStream_t s, const DevState& src, const DevState& ds
                           int32_t* cell_of_dst, int32_t* slot_of_id, double* const rref[3],
                           int32_t n, int32_t wrap, double L);
 hipError_t launch_max_count(hipStream_t s, const int32_t* counts, int64_t ncell, int32_t* out);
+// A walled grid build of few items and cells in one workgroup (one launch):
+// the bin, scan, scatter, in-cell sort, state permutation and sizes of the
+// kernels above, with the same results.
+struct SmallBuild {
+    DevState src;              // positions to bin (and the state to permute)
+    DevState dst;              // the permuted state (dst.r[0] null: no permutation)
+    const int32_t* item_slot;  // slot of item q (null: item q is slot q)
+    int32_t items;
+    GridView g;                // the grid; g.start (ncell + 1) is written
+    int64_t ncell;
+    int32_t* cell_out;         // the cell of each item (item order)
+    int32_t* sorted;           // items in cell order, ascending key inside a cell
+    const int32_t* key_of;     // key of item q (null: q itself)
+    int32_t* cell_of_dst;      // with dst: the cells in the new slot order
+    int32_t* slot_of_id;       // with dst: the slot of each atom id
+    int32_t* words;            // [0] |= out-of-box flag, [1] largest cell, [2] longest row
+};
+constexpr int32_t kSmallBuildItems = 6144;
+constexpr int64_t kSmallBuildCells = 12287;
+bool small_build_fits(int32_t items, int64_t ncell);
+hipError_t launch_small_build(hipStream_t s, const SmallBuild& b);
 hipError_t launch_row_lengths(hipStream_t s, const GridView& g, const int32_t* cell_by_item,
                               int32_t items, int32_t* max_len);
 hipError_t launch_emit_list(hipStream_t s, const GridView& g, const int32_t* cell_by_item,

@@ -385,6 +385,17 @@ int finish_sizes(mdx_system* s, Grid& g, const int32_t* cell_by_item, int32_t it
     return sizes_finish(s, g, items, 0);
 }
 
+// The sizes words a one-workgroup build wrote (dint[1..3]) into the pinned
+// buffer at po, then (po < 0) read at once.
+int small_sizes(mdx_system* s, Grid& g, int32_t items, int po) {
+    const int at = po < 0 ? 0 : po;
+    HIPCHK(s, hipMemcpyAsync((char*)s->pinned + at, s->dint + 1, 3 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                             s->stream));
+    if (po >= 0) return MDX_OK;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return sizes_finish(s, g, items, 0);
+}
+
 // Primary grid: bins every atom, counting-sorts slots into cell order (ascending
 // id within a cell) and permutes the whole state into that order.  With
 // sizes_po >= 0 the sizes are only queued (sizes_launch at that offset).
@@ -394,6 +405,27 @@ int build_primary(mdx_system* s, Grid& g, float rs, double box_size_list, bool s
     hipStream_t st = s->stream;
     const int32_t n = s->n;
     HIPCHK(s, hipMemsetAsync(s->dint + 1, 0, sizeof(int32_t), st));
+    if (s->cfg.box == MDX_BOX_WALLED && small_build_fits(n, g.ncell)) {
+        // few atoms and cells: the whole build in one workgroup
+        SmallBuild b{};
+        b.src = s->st;
+        b.dst = s->alt;
+        b.items = n;
+        b.g = g.view;
+        b.ncell = g.ncell;
+        b.cell_out = s->cell_of;
+        b.sorted = s->sorted;
+        b.key_of = s->st.id;
+        b.cell_of_dst = s->cell_of_alt;
+        b.slot_of_id = s->slot_of_id;
+        b.words = s->dint + 1;
+        LAUNCH(s, "cells_small_build", launch_small_build(st, b));
+        std::swap(s->st, s->alt);
+        std::swap(s->cell_of, s->cell_of_alt);
+        g.built = true;
+        s->builds++;
+        return sizes ? small_sizes(s, g, n, sizes_po) : MDX_OK;
+    }
     LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
     LAUNCH(s, "cells_bin", launch_bin(st, s->st, nullptr, n, g.view, s->cell_of, g.count, s->dint + 1));
     LAUNCH(s, "cells_scan", launch_exclusive_scan(st, g.count, g.start, g.ncell, s->scan_sums, s->scan_cap));
@@ -450,6 +482,22 @@ int build_bond_walled(mdx_system* s, double box_size_list, int sizes_po = -1) {
     hipStream_t st = s->stream;
     const int32_t nb = s->cfg.n_bond;
     HIPCHK(s, hipMemsetAsync(s->dint + 1, 0, sizeof(int32_t), st));
+    if (small_build_fits(nb, g.ncell)) {
+        // few atoms and cells: the whole build in one workgroup
+        SmallBuild b{};
+        b.src = s->st;
+        b.item_slot = s->slot_of_id;
+        b.items = nb;
+        b.g = g.view;
+        b.ncell = g.ncell;
+        b.cell_out = s->mb_cell_of;
+        b.sorted = s->mb_ids;
+        b.words = s->dint + 1;
+        LAUNCH(s, "cells_small_build", launch_small_build(st, b));
+        if (sizes_po >= 0) return small_sizes(s, g, nb, sizes_po);
+        if ((rc = small_sizes(s, g, nb, -1))) return rc;
+        return bond_walled_finish(s);
+    }
     LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
     LAUNCH(s, "cells_bin", launch_bin(st, s->st, s->slot_of_id, nb, g.view, s->mb_cell_of, g.count, s->dint + 1));
     LAUNCH(s, "cells_scan", launch_exclusive_scan(st, g.count, g.start, g.ncell, s->scan_sums, s->scan_cap));

@@ -80,6 +80,41 @@ def test_lists_bit_exact_walled(mixture):
         np.testing.assert_array_equal(s.neighbor_list(LIST_BOND, nls, items=nb), ref)
 
 
+def test_lists_bit_exact_walled_both_builds():
+    """Walled grids of few atoms and cells are built by one workgroup in LDS
+    (at most 6,144 atoms and 12,287 cells), larger ones by the multi-kernel
+    build.  Two Ne atoms moved out to +-75 A widen the grids: the bond-order
+    grid (cells of 3.6 A) past the one-workgroup limit, the Lennard-Jones grid
+    (7.6 A) still under it.  Both lists and both sets of forces of the
+    mixture must equal the oracle's."""
+    pos4, vel = gas_mixture(30, 500, seed=1)
+    pos4 = pos4.copy()
+    pos4[-1, :3] = (75.0, 0.5, -0.5)
+    pos4[-2, :3] = (-75.0, -0.5, 0.5)
+    nb = 60
+    _, box = O.ready_list(pos4, vel)
+    assert (int(box / 3.6 + 0.99) + 1) ** 3 > 12287 >= (int(box / 7.6 + 0.99) + 1) ** 3
+    with walled(pos4, nb, PREC_FP64) as s:
+        s.set_state(pos4, vel)
+        _, box_d = s.ready_list()
+        assert box_d == box
+        nls, nlg = s.build_list(LIST_LJ, box)
+        ref, ls = O.build_neighbor_list(pos4, 7.6, 6.8, box)
+        assert (nls, nlg) == (ls.nlistsize, ls.nlargestsize)
+        np.testing.assert_array_equal(s.neighbor_list(LIST_LJ, nls), ref)
+        nls, nlg = s.build_list(LIST_BOND, box)
+        ref_mb, lsb = O.build_neighbor_list(pos4[:nb], 3.6, 3.0, box)
+        assert (nls, nlg) == (lsb.nlistsize, lsb.nlargestsize)
+        np.testing.assert_array_equal(s.neighbor_list(LIST_BOND, nls, items=nb), ref_mb)
+        s.lj_forces()
+        s.bond_forces()
+        f = s.get_state()["force"]
+    fo = O.lj_forces(pos4, ref, ls.nlargestsize, float(ls.rcutsq))
+    np.testing.assert_array_equal(f[nb:], fo[nb:])
+    fo = O.mb_forces(pos4, ref_mb, lsb.nlargestsize, float(lsb.rcutsq), f_in=fo, ljexist=True)
+    assert rel_rms(f[:nb], fo[:nb]) < 1e-12
+
+
 def test_list_edge_cell_walled():
     pos4 = np.array([[3.005, 0, 0, 2], [-3.005, 0, 0, 2], [2.2, 0.1, 0.1, 2], [0, 0, 0, 2], [1.0, -1, 0.5, 2]])
     with walled(pos4, 0, PREC_FP64, use_bond=False, lj=(2.0, 1.5)) as s:
