@@ -522,7 +522,7 @@ k_small_build(SmallBuild b) {
             mc = max(mc, wmax[0][w]);
             ml = max(ml, wmax[1][w]);
         }
-        b.words[0] |= bad_any;
+        b.words[0] = bad_any;
         b.words[1] = mc;
         b.words[2] = ml;
     }

@@ -26,9 +26,12 @@ __device__ inline void predict_atom(const DevState& st, int32_t i, const GearCoe
     }
 }
 
+// clear (may be null): a word zeroed by the first thread (the corrector's
+// wall-hit word, so no separate fill runs before the corrector)
 __global__ void __launch_bounds__(kBlock)
-k_predict(DevState st, int32_t n, GearCoeffs gc) {
+k_predict(DevState st, int32_t n, GearCoeffs gc, int32_t* __restrict__ clear) {
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (clear && i == 0) *clear = 0;
     if (i >= n) return;
     predict_atom(st, i, gc);
 }
@@ -356,9 +359,9 @@ GearCoeffs gear_coeffs(float dtf) {
 
 static inline dim3 grid_of(int32_t n) { return dim3((n + kBlock - 1) / kBlock); }
 
-hipError_t launch_predict(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc) {
+hipError_t launch_predict(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc, int32_t* clear) {
     if (n <= 0) return hipSuccess;
-    k_predict<<<grid_of(n), kBlock, 0, s>>>(st, n, gc);
+    k_predict<<<grid_of(n), kBlock, 0, s>>>(st, n, gc, clear);
     return hipGetLastError();
 }
 

@@ -265,7 +265,7 @@ struct SmallBuild {
     const int32_t* key_of;     // key of item q (null: q itself)
     int32_t* cell_of_dst;      // with dst: the cells in the new slot order
     int32_t* slot_of_id;       // with dst: the slot of each atom id
-    int32_t* words;            // [0] |= out-of-box flag, [1] largest cell, [2] longest row
+    int32_t* words;            // [0] out-of-box flag, [1] largest cell, [2] longest row
 };
 constexpr int32_t kSmallBuildItems = 6144;
 constexpr int64_t kSmallBuildCells = 12287;
@@ -303,7 +303,8 @@ struct GearCoeffs {
     double kc[4];   // corrector: GEAR1 k1, GEAR2 k1/k0, GEAR3 k1/k2, GEAR4 k1/k3
 };
 GearCoeffs gear_coeffs(float dt);
-hipError_t launch_predict(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc);
+// clear (may be null): a word the kernel zeroes (the corrector's wall-hit word)
+hipError_t launch_predict(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc, int32_t* clear = nullptr);
 hipError_t launch_correct_walled(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
                                  const MassTable& m, double npos, float eloss, int32_t* hit);
 hipError_t launch_accelerations(hipStream_t s, const DevState& st, int32_t n, const MassTable& m,
@@ -470,6 +471,7 @@ hipError_t launch_kinetics(hipStream_t s, const DevState& st, int32_t n, const M
 int32_t kinetics_blocks(int32_t n);
 hipError_t launch_sum(hipStream_t s, const double* x, int32_t n, double* partials, int32_t nblocks,
                       double* out);
+// out2_bits: max |v|^2 and max |coordinate| (bits of non-negative doubles); clears them itself
 hipError_t launch_ready_list(hipStream_t s, const DevState& st, int32_t n, uint64_t* out2_bits);
 // w[i] = pos.r_s . frc.f_s for slot s = map[i] (map null: s = i), i < n
 hipError_t launch_rf_virial(hipStream_t s, const DevState& pos, const DevState& frc, const int32_t* map, int32_t n,

@@ -120,7 +120,9 @@ k_sum2(const double* __restrict__ part, int32_t nb, double* __restrict__ out) {
     if (threadIdx.x == 0) *out = v[0];
 }
 
-// max |v|^2 and max |coordinate| as bit patterns (both non-negative)
+// max |v|^2 and max |coordinate| as bit patterns (both non-negative); one
+// workgroup (kOne) writes them, several max-reduce into words cleared before
+template <bool kOne>
 __global__ void __launch_bounds__(kBlock)
 k_ready(DevState st, int32_t n, unsigned long long* __restrict__ out) {
     double v2 = 0, c = 0;
@@ -135,7 +137,22 @@ k_ready(DevState st, int32_t n, unsigned long long* __restrict__ out) {
         a = x > a ? x : a;
         b = y > b ? y : b;
     }
-    if ((threadIdx.x & (kWave - 1)) == 0) {
+    if constexpr (kOne) {
+        __shared__ unsigned long long wm[kBlock / kWave][2];
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+            wm[threadIdx.x / kWave][0] = a;
+            wm[threadIdx.x / kWave][1] = b;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < kBlock / kWave; w++) {
+                a = wm[w][0] > a ? wm[w][0] : a;
+                b = wm[w][1] > b ? wm[w][1] : b;
+            }
+            out[0] = a;
+            out[1] = b;
+        }
+    } else if ((threadIdx.x & (kWave - 1)) == 0) {
         atomicMax(&out[0], a);
         atomicMax(&out[1], b);
     }
@@ -185,8 +202,18 @@ hipError_t launch_rf_virial(hipStream_t s, const DevState& pos, const DevState& 
     return hipGetLastError();
 }
 
+// up to this many atoms one workgroup reduces them (no clearing, no atomics)
+constexpr int32_t kReadyOneBlock = 65536;
+
 hipError_t launch_ready_list(hipStream_t s, const DevState& st, int32_t n, uint64_t* out2_bits) {
-    k_ready<<<blocks_for(n), kBlock, 0, s>>>(st, n, reinterpret_cast<unsigned long long*>(out2_bits));
+    auto* out = reinterpret_cast<unsigned long long*>(out2_bits);
+    if (n <= kReadyOneBlock) {
+        k_ready<true><<<1, kBlock, 0, s>>>(st, n, out);
+    } else {
+        hipError_t e = hipMemsetAsync(out2_bits, 0, 2 * sizeof(uint64_t), s);
+        if (e != hipSuccess) return e;
+        k_ready<false><<<blocks_for(n), kBlock, 0, s>>>(st, n, out);
+    }
     return hipGetLastError();
 }
 

@@ -404,7 +404,6 @@ int build_primary(mdx_system* s, Grid& g, float rs, double box_size_list, bool s
     if (rc) return rc;
     hipStream_t st = s->stream;
     const int32_t n = s->n;
-    HIPCHK(s, hipMemsetAsync(s->dint + 1, 0, sizeof(int32_t), st));
     if (s->cfg.box == MDX_BOX_WALLED && small_build_fits(n, g.ncell)) {
         // few atoms and cells: the whole build in one workgroup
         SmallBuild b{};
@@ -426,6 +425,7 @@ int build_primary(mdx_system* s, Grid& g, float rs, double box_size_list, bool s
         s->builds++;
         return sizes ? small_sizes(s, g, n, sizes_po) : MDX_OK;
     }
+    HIPCHK(s, hipMemsetAsync(s->dint + 1, 0, sizeof(int32_t), st));
     LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
     LAUNCH(s, "cells_bin", launch_bin(st, s->st, nullptr, n, g.view, s->cell_of, g.count, s->dint + 1));
     LAUNCH(s, "cells_scan", launch_exclusive_scan(st, g.count, g.start, g.ncell, s->scan_sums, s->scan_cap));
@@ -481,7 +481,6 @@ int build_bond_walled(mdx_system* s, double box_size_list, int sizes_po = -1) {
     if (rc) return rc;
     hipStream_t st = s->stream;
     const int32_t nb = s->cfg.n_bond;
-    HIPCHK(s, hipMemsetAsync(s->dint + 1, 0, sizeof(int32_t), st));
     if (small_build_fits(nb, g.ncell)) {
         // few atoms and cells: the whole build in one workgroup
         SmallBuild b{};
@@ -498,6 +497,7 @@ int build_bond_walled(mdx_system* s, double box_size_list, int sizes_po = -1) {
         if ((rc = small_sizes(s, g, nb, -1))) return rc;
         return bond_walled_finish(s);
     }
+    HIPCHK(s, hipMemsetAsync(s->dint + 1, 0, sizeof(int32_t), st));
     LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
     LAUNCH(s, "cells_bin", launch_bin(st, s->st, s->slot_of_id, nb, g.view, s->mb_cell_of, g.count, s->dint + 1));
     LAUNCH(s, "cells_scan", launch_exclusive_scan(st, g.count, g.start, g.ncell, s->scan_sums, s->scan_cap));
@@ -516,7 +516,6 @@ int build_bond_walled(mdx_system* s, double box_size_list, int sizes_po = -1) {
 // copy of its two words into the pinned buffer at byte offset po, then (after
 // a synchronisation) vmax and boxSizeList from them
 int ready_launch(mdx_system* s, int po) {
-    HIPCHK(s, hipMemsetAsync(s->dbits + 1, 0, 2 * sizeof(uint64_t), s->stream));
     LAUNCH(s, "ready_list", launch_ready_list(s->stream, s->st, s->n, s->dbits + 1));
     HIPCHK(s, hipMemcpyAsync((char*)s->pinned + po, s->dbits + 1, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost,
                              s->stream));
@@ -700,9 +699,9 @@ int run_walled(mdx_system* s, int32_t nsteps) {
             if (c.use_lj && lj_due && (rc = build_list(s, MDX_LIST_LJ, s->box_size_list, true))) return rc;
             if (c.use_bond && mb_due && (rc = build_list(s, MDX_LIST_BOND, s->box_size_list, true))) return rc;
         }
-        LAUNCH(s, "predict", launch_predict(s->stream, s->st, s->n, s->gc));
+        // the predictor clears the wall-hit word the corrector sets
+        LAUNCH(s, "predict", launch_predict(s->stream, s->st, s->n, s->gc, s->dint));
         if ((rc = forces(s, nullptr))) return rc;
-        HIPCHK(s, hipMemsetAsync(s->dint, 0, sizeof(int32_t), s->stream));
         LAUNCH(s, "correct", launch_correct_walled(s->stream, s->st, s->n, s->gc, s->mt, c.wall,
                                                    c.energy_loss, s->dint));
         // wall hit: forces again and a = F/m (Host.cpp:168-178), predicated on the device flag
