@@ -587,8 +587,24 @@ k_step_lds(StepArgs a) {
     __shared__ StepStage S;
     extern __shared__ double dyn[];
     SegGrid g = a.g;
-    if (blockIdx.x == 0 && a.clear)
-        for (int32_t t = threadIdx.x; t < 3 * kSpread; t += kB) a.clear[t] = 0;
+    // the single box's list gate: set when the previous launch's flags say
+    // that r(t) has left the list's margin (every wavefront loads the
+    // kSpread displacement words, one a lane, and tests them by a ballot)
+    unsigned long long gw = 0;
+    if constexpr (!kRep) gw = a.pub.prev[3 * (threadIdx.x & (kWave - 1))];
+    if (blockIdx.x == 0) {
+        if (a.clear)
+            for (int32_t t = threadIdx.x; t < 3 * kSpread; t += kB) a.clear[t] = 0;
+        if constexpr (!kRep) {
+            hand_over(a.pub, 3 * kSpread);
+            const bool halt = !a.pub.open && __ballot(gw >= a.pub.gate_bits) != 0;   // every lane votes
+            if (threadIdx.x == 0 && halt) {
+                // a halted step's own words: the next launch halts, the host sees why
+                a.flags[0] = kHaltBits;
+                a.flags[2] = kHaltMark;
+            }
+        }
+    }
     int32_t cx, cy, z0, z1, rep = 0, seg = 0;
     if constexpr (kRep) {
         seg = rep_block(a.rg);   // the segment table's entry
@@ -599,22 +615,16 @@ k_step_lds(StepArgs a) {
     }
     const Pos P{dyn, a.lds_cap};
     unsigned long long* fl = kRep ? nullptr : a.flags + 3 * (blockIdx.x % kSpread);
-    // the single box's list gate (set when r(t) has left the list's margin)
-    // and this segment's header from the last list build, loaded together
-    unsigned int halt = 0;
-    if constexpr (!kRep) halt = __hip_atomic_load(a.pub.gate, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // this segment's header from the last list build, loaded with the gate's words
     int32_t hw = 0;
     if (threadIdx.x < kHdrWords) hw = a.hdr[(int64_t)seg * kHdrWords + threadIdx.x];
     if constexpr (!kRep) {
         // the empty asm keeps the header load above the test, so one wait
         // covers both
         asm volatile("" ::"v"(hw));
-        if (halt) {
-            // the list does not cover r(t): the step runs as a no-op (no state
-            // or flag slot is written) and workgroup 0 says so
-            if (blockIdx.x == 0) publish_halt(a.pub, 3 * kSpread);
-            return;
-        }
+        // the list does not cover r(t): the step runs as a no-op (workgroup 0
+        // has marked its flag words)
+        if (!a.pub.open && __ballot(gw >= a.pub.gate_bits) != 0) return;
     }
     if (threadIdx.x < kHdrWords) reinterpret_cast<int32_t*>(&S.h)[threadIdx.x] = hw;
     if (threadIdx.x < 3) S.red[threadIdx.x] = 0;
@@ -642,7 +652,6 @@ k_step_lds(StepArgs a) {
             if constexpr (kRep) atomicMax(&a.err[0], S.h.total + 1);
             else atomicAdd(&fl[2], 1ull << 32);
         }
-        if constexpr (!kRep) publish_flags(a.pub, seg, a.flags, 3 * kSpread);
         return;
     }
     double d2max = 0.0, mv2max = 0.0, stale = 0.0;
@@ -779,7 +788,6 @@ k_step_lds(StepArgs a) {
             if (S.red[2] > 0) atomicAdd(&fl[2], S.red[2]);
         }
     }
-    if constexpr (!kRep) publish_flags(a.pub, seg, a.flags, 3 * kSpread);
 }
 
 SegGrid make_grid(const GridView& v, int32_t nseg) {
@@ -906,9 +914,8 @@ hipError_t launch_step_lds(hipStream_t s, const DevState& st, const GridView& v,
             return hipErrorInvalidValue;
     a.n = n;
     a.lds_cap = lds_atoms;
-    if (!pub.gate) return hipErrorInvalidValue;   // the single box's step kernels read their gate
+    if (!pub.prev) return hipErrorInvalidValue;   // the single box's step kernels read their gate
     a.pub = pub;
-    a.pub.blocks = a.g.total;   // the workgroups past the segments return before counting
     const size_t dyn = 3 * (size_t)lds_atoms * sizeof(double);
     dim3 grid(a.g.per_xcd * kXcd);
     const bool fp64 = precision == MDX_PREC_FP64;

@@ -66,87 +66,39 @@ __device__ inline double lane_double(double v, int l) {
 }
 
 // The per-step flag hand-over of the periodic LJ step kernel (k_step_lds), in
-// place of a device-to-host copy after each launch.  Every workgroup that takes part
-// arrives, once its flag atomics have completed, at one of kPubSub counters
-// (its number modulo kPubSub); the last arrival at each of those arrives at
-// the top counter, and the last arrival there copies the step's flag words to
-// a slot of pinned host memory and then writes the step's sequence number
-// after them, so a host that sees the number sees the words.  Each counter is
-// reset by its last arrival for the next launch.  Two levels keep a kernel's
-// workgroups, which may all finish at once, from queueing on one address, and
-// each counter has a 256-byte line of its own (kPubStride words apart):
-// device-scope atomics on one line are served one after another, so counters
-// packed into one line queue as if they were one (at 1M LJ atoms, 10,112
-// arrivals per step on one line made the step kernel 206 us instead of 175).
-// The flags are only touched by device-scope atomics and every access here is
-// an atomic that waits for the previous ones, so no fence (which would write
-// back the XCD's L2 after a step kernel's stores) is needed.
-// host == nullptr: nothing is published.
+// place of a device-to-host copy after each launch.  A launch reads the flag
+// words of the launch before it, which are complete since that launch has
+// ended: every wavefront takes its list gate from them, and workgroup 0
+// copies them to a slot of pinned host memory and then writes the sequence
+// number after them, so a host that sees the number sees the words.  The
+// step kernel's workgroups thus end right after their flag atomics, with no
+// wait for their stores and no arrival count (the last arrival of a count
+// at the end of each workgroup handing the words over kept every
+// workgroup's LDS the time of a store drain and a returning atomic longer).
+// The last launch's words are copied by the host.
 //
-// The last arrival also writes the list gate of the next step (gate != null):
-// 1 when the largest displacement^2 since the list build among the flag words
-// (words 3q) reaches gate_bits (non-negative doubles order like their bits),
-// else 0.  A step kernel that finds its gate set runs as a no-op ("halted"):
-// it leaves the gate set, so the launch queued after it halts too, and hands
-// over flag words marked with kHaltMark (publish_halt).
-constexpr int kPubSub = 32;
-constexpr int kPubStride = 64;
+// The list gate: a step kernel whose previous flag words hold a largest
+// displacement^2 since the list build (words 3q) at or above gate_bits
+// (non-negative doubles order like their bits), unless it is the first
+// launch on a new list (open), runs as a no-op ("halted"):
+// no state is written, and workgroup 0 marks the step's own flag words with
+// kHaltBits (so the launch queued after it halts too) and kHaltMark in the
+// first stale word (so the host sees it).
 constexpr unsigned long long kHaltMark = 1ull << 62;
+constexpr unsigned long long kHaltBits = 0x7ff0000000000000ull;   // +inf
 struct FlagPublish {
-    unsigned int* done;         // kPubSub + 1 device counters kPubStride apart, 0 between launches
-    unsigned long long* host;   // flag words, then the sequence number
+    const unsigned long long* prev;   // the previous launch's flag words
+    unsigned long long* host;         // their host slot, then the sequence number (null: none)
     unsigned long long seq;
-    int32_t blocks;             // workgroups that take part, numbered 0 .. blocks - 1
-    unsigned int* gate;         // the next step's list gate (may be null)
     unsigned long long gate_bits;
+    int32_t open;                     // 1: the list was built from r(t), no gate
 };
 
-// every thread of a workgroup that takes part (number id), after the
-// workgroup's flag atomics
-__device__ inline void publish_flags(const FlagPublish& p, int32_t id, const unsigned long long* flags,
-                                     int32_t words) {
-    if (!p.host) return;
-    __shared__ int32_t last;
-    __shared__ unsigned long long dmax;
-    __builtin_amdgcn_s_waitcnt(0);   // this wave's flag atomics have completed
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int32_t sub = id % kPubSub;
-        const unsigned int n_sub = (unsigned int)(p.blocks / kPubSub + (sub < p.blocks % kPubSub ? 1 : 0));
-        const unsigned int n_top = (unsigned int)(p.blocks < kPubSub ? p.blocks : kPubSub);
-        int32_t l = 0;
-        if (__hip_atomic_fetch_add(p.done + sub * kPubStride, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u == n_sub) {
-            __hip_atomic_store(p.done + sub * kPubStride, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            l = __hip_atomic_fetch_add(p.done + kPubSub * kPubStride, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u ==
-                n_top;
-        }
-        last = l;
-        dmax = 0;
-    }
-    __syncthreads();
-    if (!last) return;
-    for (int32_t w = threadIdx.x; w < words; w += blockDim.x) {
-        const unsigned long long v = __hip_atomic_load(flags + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(p.host + w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        if (w % 3 == 0) atomicMax(&dmax, v);
-    }
-    __builtin_amdgcn_s_waitcnt(0);   // the words have been written
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (p.gate)
-            __hip_atomic_store(p.gate, dmax >= p.gate_bits ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(p.done + kPubSub * kPubStride, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(p.host + words, p.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
-}
-
-// A halted step kernel's hand-over: workgroup 0 alone (all its threads)
-// writes the step's flag words as zeros with kHaltMark in the first stale
-// word, then the sequence number.
-__device__ inline void publish_halt(const FlagPublish& p, int32_t words) {
+// workgroup 0, every thread: prev's words to the host slot, then the number
+__device__ inline void hand_over(const FlagPublish& p, int32_t words) {
     if (!p.host) return;
     for (int32_t w = threadIdx.x; w < words; w += blockDim.x)
-        __hip_atomic_store(p.host + w, w == 2 ? kHaltMark : 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(p.host + w, p.prev[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __builtin_amdgcn_s_waitcnt(0);   // the words have been written
     __syncthreads();
     if (threadIdx.x == 0) __hip_atomic_store(p.host + words, p.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);

@@ -113,7 +113,6 @@ struct mdx_system {
     // number of the step whose words they are (written by the step kernels)
     static constexpr int kHostWords = kFlagWords + 1;
     uint64_t* hflags = nullptr;
-    unsigned int* dpub = nullptr;    // the step kernels' arrival counters, then the list gate (FlagPublish)
     uint64_t pub_seq = 0;            // last sequence number handed out
     hipEvent_t ev[kRing] = {nullptr, nullptr, nullptr};
     bool owns_stream = true;         // false for the members of a sweep (the sweep's stream)
@@ -837,19 +836,30 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
         D = std::sqrt(d2);
         S = std::sqrt(s2);
     };
-    // The LJ step kernel hands its flag words to the host slot itself and
-    // writes the step's sequence number after them (FlagPublish); the other
-    // steps' and the predictor's flags are copied and marked with an event
-    // (for the short Tersoff and surrogate step kernels the hand-over at
-    // their end costs as much as the copy it saves).
+    // Each LJ step kernel hands the previous step's flag words to their host
+    // slot and writes a sequence number after them (FlagPublish); the last
+    // step's, the other steps' and the predictor's flags are copied and
+    // marked with an event (for the short Tersoff and surrogate step kernels
+    // the hand-over costs as much as the copy it saves).
     uint64_t expect[mdx_system::kRing] = {0, 0, 0};   // per slot: sequence number awaited (0: event)
-    unsigned int* const gate = s->dpub + (kPubSub + 1) * kPubStride;
     uint64_t gate_bits;
     std::memcpy(&gate_bits, &hard2, 8);
-    auto publish = [&](int k) {
-        expect[ring(k)] = ++s->pub_seq;
-        return FlagPublish{s->dpub, reinterpret_cast<unsigned long long*>(host_flags(k)),
-                           (unsigned long long)s->pub_seq, 0, gate, (unsigned long long)gate_bits};
+    // step k's hand-over: the words of step k - 1 when the host has yet to read them
+    // (open: step k is the first on its list, which needs no gate)
+    auto publish = [&](int k, bool to_host, bool open) {
+        FlagPublish p{reinterpret_cast<const unsigned long long*>(slot_flags(k - 1)), nullptr, 0,
+                      (unsigned long long)gate_bits, open ? 1 : 0};
+        if (to_host) {
+            expect[ring(k - 1)] = ++s->pub_seq;
+            p.host = reinterpret_cast<unsigned long long*>(host_flags(k - 1));
+            p.seq = (unsigned long long)s->pub_seq;
+        }
+        return p;
+    };
+    // the device slots cleared of the halted launches' marks
+    auto clear_slots = [&]() -> int {
+        HIPCHK(s, hipMemsetAsync(s->dflags, 0, (size_t)R * W * sizeof(uint64_t), st));
+        return MDX_OK;
     };
     auto post = [&](int k) -> int {  // copy step flags to the host and mark them
         HIPCHK(s, hipMemcpyAsync(host_flags(k), slot_flags(k), W * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
@@ -877,13 +887,10 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
     };
     // flags of r(0) come from the first predictor (index -1)
     // the predictor's slot and step 0's slot start cleared; from then on each
-    // step's last kernel clears the next step's slot (its host copy, two steps
-    // back in the ring, was enqueued before)
+    // step's last kernel clears the next step's slot (the launch before it
+    // has read that slot, two steps back in the ring, and copied it)
     HIPCHK(s, hipMemsetAsync(slot_flags(-1), 0, W * sizeof(uint64_t), st));
     HIPCHK(s, hipMemsetAsync(slot_flags(0), 0, W * sizeof(uint64_t), st));
-    // the hand-over's arrival counters and the list gate start each call at
-    // zero, whatever an earlier failed launch left in them
-    HIPCHK(s, hipMemsetAsync(s->dpub, 0, sizeof(unsigned int) * (kPubSub + 2) * kPubStride, st));
     LAUNCH(s, "predict", launch_predict_track(st, s->st, n, s->gc, s->rref, slot_flags(-1)));
     if ((rc = post(-1))) return rc;
     if (!ts) {
@@ -896,9 +903,10 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
         // the gate's test, on the same bits: largest displacement^2 >= margin^2
         uint64_t d2_bits = 0;
         for (int q = 0; q < W / 3; q++) d2_bits = std::max<uint64_t>(d2_bits, host_flags(-1)[3 * q]);
-        if (d2_bits >= gate_bits && (rc = rebuild_lists())) return rc;
+        if (d2_bits >= gate_bits && ((rc = rebuild_lists()) || (rc = clear_slots()))) return rc;
         int32_t k = 0;        // next step to launch
         int32_t checked = 0;  // steps before this one have run with a valid list
+        int32_t fresh = 0;    // the first step on the current list (r(0) was checked above)
         // with timing on: the record of each launch in flight, so that a
         // halted one is counted as "md_step_halted" instead of "md_step"
         int64_t rec[4] = {-1, -1, -1, -1};
@@ -908,9 +916,10 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
                 LAUNCH(s, "md_step", launch_step_lds(st, s->st, s->lj.view, s->nseg, s->rnext, s->rref, s->nlist,
                                                      s->ncount, s->nl_cap, s->ljt, s->mt, s->gc, hard2, n,
                                                      component_stride(n), c.precision, fin, fin, slot_flags(k),
-                                                     s->lds_atoms, slot_flags(k + 1), publish(k), s->box_block,
-                                                     s->seg_hdr));
+                                                     s->lds_atoms, slot_flags(k + 1), publish(k, k - 1 >= checked, k == fresh),
+                                                     s->box_block, s->seg_hdr));
                 rec[k % 4] = s->timing ? (int64_t)s->pending.size() - 1 : -1;
+                if (fin && (rc = post(k))) return rc;   // no launch follows to hand these over
                 for (int d = 0; d < 3; d++) std::swap(s->st.r[d], s->rnext[d]);
                 s->t++;
                 k++;
@@ -928,9 +937,9 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
                     for (int d = 0; d < 3; d++) std::swap(s->st.r[d], s->rnext[d]);
                 s->t -= undone;
                 k = checked;
-                // (a halted launch writes no flag slot); the gate opens for the new list
-                if ((rc = rebuild_lists())) return rc;
-                HIPCHK(s, hipMemsetAsync(gate, 0, sizeof(unsigned int), st));
+                // the halted launches marked their flag words; the gate opens for the new list
+                if ((rc = rebuild_lists()) || (rc = clear_slots())) return rc;
+                fresh = checked;
                 continue;
             }
             if (stale != 0) return stale_error(checked);
@@ -1222,8 +1231,7 @@ static int create_system(const mdx_config* cfg, hipStream_t shared, mdx_system**
     }
     s->nb = kinetics_blocks(n);
     if ((rc = dalloc(s, &s->partials, 8 * (size_t)s->nb + 16)) ||
-        (rc = dalloc(s, &s->dflags, (size_t)mdx_system::kFlagWords * mdx_system::kRing)) ||
-        (rc = dalloc(s, &s->dpub, (kPubSub + 2) * kPubStride))) {
+        (rc = dalloc(s, &s->dflags, (size_t)mdx_system::kFlagWords * mdx_system::kRing))) {
         g_err = s->err;
         return bail(rc);
     }
@@ -1246,8 +1254,6 @@ static int create_system(const mdx_config* cfg, hipStream_t shared, mdx_system**
                       hipHostMallocDefault) != hipSuccess)
         return bail(fail(nullptr, MDX_ERR_ALLOC, "hipHostMalloc failed"));
     std::memset(s->hflags, 0, sizeof(uint64_t) * mdx_system::kHostWords * mdx_system::kRing);
-    if (hipMemsetAsync(s->dpub, 0, sizeof(unsigned int) * (kPubSub + 2) * kPubStride, s->stream) != hipSuccess)
-        return bail(fail(nullptr, MDX_ERR_HIP, "init failed"));
     for (int k = 0; k < mdx_system::kRing; k++)
         if (hipEventCreateWithFlags(&s->ev[k], hipEventDisableTiming) != hipSuccess)
             return bail(fail(nullptr, MDX_ERR_HIP, "hipEventCreate failed"));
