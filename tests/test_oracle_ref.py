@@ -202,6 +202,30 @@ def test_bond_order_two_atoms_closed_form():
         assert f[1, 0] == pytest.approx(dvr - dva + dvdw, rel=1e-12)
 
 
+def test_bond_order_third_atom_filter():
+    """bondOrder1 counts a third atom m only while r_im^2 < rcutsq
+    (cpu/common/BondOrderFuncs.h:51), though the cutoff function is still
+    ~1e-3 there (f_c(3.0) = (1 - tanh(3.4))/2 with ALFA 2, R0 1.3).  An m at
+    exactly rc from i and beyond rc from j, in every row, must then leave the
+    forces of the bonded pair bit for bit those of the pair alone (and feel
+    none itself): no filter, or '>' for '>=', would count it.  Moved inside
+    rc it changes them at the 1e-3 level, which shows the test can fail.
+    That the filter tests i's distance rather than j's is fixed by the source
+    alone: where the two differ, b_ij != b_ji and no energy has these forces."""
+    rc2 = 9.0
+    for (ti, tj, tm, r) in [(N, N, N, 1.2), (OX, N, OX, 1.3), (OX, OX, N, 1.25)]:
+        pair = np.array([[0, 0, 0, ti], [r, 0, 0, tj]], float)
+        f2 = O.mb_forces(pair, np.array([[1], [0]], dtype=np.int32), 1, rc2)
+        lst = np.array([[1, 2], [0, 2], [0, 1]], dtype=np.int32)
+        at_rc = np.vstack([pair, [-3.0, 0, 0, tm]])          # r_im^2 = 9 exactly, r_jm > 3
+        f3 = O.mb_forces(at_rc, lst, 2, rc2)
+        np.testing.assert_array_equal(f3[:2], f2)
+        np.testing.assert_array_equal(f3[2], 0.0)
+        inside = np.vstack([pair, [-2.99, 0.05, 0, tm]])     # r_im < 3 <= r_jm
+        fin = O.mb_forces(inside, lst, 2, rc2)
+        assert np.abs(fin[0] - f2[0]).max() > 1e-5 * np.abs(f2[0]).max()
+
+
 def quartic_state(coef, t):
     """x(t) = sum coef[k] t^k and its first four derivatives."""
     p = np.polynomial.Polynomial(coef)
