@@ -114,9 +114,14 @@ k_lj(DevState st, const int32_t* __restrict__ cell_of, GridView g, LjTable t, in
 // Walled box, few atoms: one wavefront per atom.  The 27 candidate cells of
 // the reference's row are 9 z-runs of up to 3 consecutive cells, i.e. 9
 // contiguous slot ranges in row order; the 64 lanes take 64 consecutive row
-// entries at a time and evaluate their pairs, then every lane adds the
-// in-cutoff terms in row order (read back lane by lane), so the sums are the
-// thread-per-atom kernel's, operation for operation.
+// entries at a time and test them against rc.  The pairs within rc are
+// appended, in row order, to a 64-entry buffer of the wavefront in LDS
+// (displacement, r^2, pair type): about one candidate in nine of a walled gas
+// lies within rc, so the force law, with its three FP64 divisions, then runs
+// over full wavefronts of pairs instead of over every batch of candidates.  A
+// full (or the last) buffer is evaluated one pair a lane, and lanes 0-2 (3
+// and 4: energy and virial) add their component of the terms in buffer
+// order: the thread-per-atom kernel's additions, operation for operation.
 constexpr int kWaveBlock = 256;
 
 // the force on slot s, the whole wavefront (lane = its lane)
@@ -124,6 +129,14 @@ template <bool kFp64, bool kEnergy>
 __device__ inline void lj_atom_wave(const DevState& st, const int32_t* __restrict__ cell_of, const GridView& g,
                                     const LjTable& t, int32_t s, int lane, double* __restrict__ e_atom,
                                     double* __restrict__ w_atom) {
+    // per wavefront: dx, dy, dz, r^2 of the pending pairs; after their
+    // evaluation rows 0-2 hold the force terms (FP64) or stay the
+    // displacement with the FP32 law's F/r in row 3 (mixed); rows 4-5 the
+    // energy and virial terms
+    __shared__ double pend[kWaveBlock / 64][6][64];
+    __shared__ int32_t pend_k[kWaveBlock / 64][64];
+    double (*const B)[64] = pend[threadIdx.x >> 6];
+    int32_t* const K = pend_k[threadIdx.x >> 6];
     const double xi = st.r[0][s], yi = st.r[1][s], zi = st.r[2][s];
     const int32_t ti = st.type[s] & (kTypes - 1);
     const int32_t own = cell_of[s];
@@ -148,7 +161,62 @@ __device__ inline void lj_atom_wave(const DevState& st, const int32_t* __restric
         off[r] = len;
         len += __builtin_amdgcn_readlane(rn, r);
     }
-    double fx = 0, fy = 0, fz = 0, e = 0, w = 0;
+    // lane c < 3: force component c; 3: energy; 4: virial
+    constexpr int kSums = kEnergy ? 5 : 3;
+    const int row = lane < 3 ? lane : lane + 1;
+    double acc = 0.0;
+    // evaluate the first c pending pairs and add their terms in order
+    auto flush = [&](int32_t c) {
+        // LDS operations of one wavefront take effect in program order; the
+        // empty asm keeps the compiler from moving reads of other lanes'
+        // entries above their writes
+        asm volatile("" ::: "memory");
+        if (lane < c) {
+            const double ddx = B[0][lane], ddy = B[1][lane], ddz = B[2][lane], r2 = B[3][lane];
+            const int32_t k = K[lane];
+            double te = 0.0, tw = 0.0;
+            if constexpr (kFp64) {
+                const double qq = r2 / t.sig2[k];
+                const double iq4 = 1.0 / (qq * qq * qq * qq);
+                const double iq7 = iq4 / (qq * qq * qq);
+                const double fr = (2 * iq7 - iq4) * t.coeff[k];
+                B[0][lane] = ddx * fr;
+                B[1][lane] = ddy * fr;
+                B[2][lane] = ddz * fr;
+                if constexpr (kEnergy) {
+                    const double iq6 = 1.0 / (qq * qq * qq);
+                    te = 0.5 * (t.e4[k] * iq6 * (iq6 - 1) - t.shift[k]);
+                    tw = 0.5 * fr * r2;
+                }
+            } else {
+                const float s2 = __builtin_amdgcn_rcpf((float)r2 * t.inv_sig2f[k]);
+                const float s4 = s2 * s2;
+                const float s6 = s4 * s2;
+                const float s8 = s4 * s4;
+                const double frd = (double)(s8 * __builtin_fmaf(2.0f, s6, -1.0f) * t.coeff_f[k]);
+                B[3][lane] = frd;
+                if constexpr (kEnergy) {
+                    te = 0.5 * ((double)(t.e4_f[k] * s6 * (s6 - 1.0f)) - t.shift[k]);
+                    tw = 0.5 * frd * r2;
+                }
+            }
+            if constexpr (kEnergy) {
+                B[4][lane] = te;
+                B[5][lane] = tw;
+            }
+        }
+        asm volatile("" ::: "memory");
+        if (lane < kSums) {
+            // FP64: acc + term, written as fma(1, term, acc), which rounds the
+            // same; mixed: fma(F/r, d, acc) for the force components
+            for (int32_t l = 0; l < c; l++) {
+                const double m = (!kFp64 && lane < 3) ? B[3][l] : 1.0;
+                acc = __builtin_fma(m, B[row][l], acc);
+            }
+        }
+        asm volatile("" ::: "memory");
+    };
+    int32_t pending = 0;
     for (int32_t q0 = 0; q0 < len; q0 += 64) {
         const int32_t q = q0 + lane;
         int32_t p = -1;
@@ -160,70 +228,39 @@ __device__ inline void lj_atom_wave(const DevState& st, const int32_t* __restric
                 if (q >= off[r]) p = base[r] + (q - off[r]);
         }
         bool in = false;
-        double ddx = 0, ddy = 0, ddz = 0, term_e = 0, term_w = 0, c0 = 0, c1 = 0, c2 = 0;
-        [[maybe_unused]] double frd = 0;
+        double ddx = 0, ddy = 0, ddz = 0, r2 = 0;
+        int32_t k = 0;
         if (p >= 0 && p != s) {
             ddx = xi - st.r[0][p];
             ddy = yi - st.r[1][p];
             ddz = zi - st.r[2][p];
-            const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            const int32_t k = ti * kTypes + (st.type[p] & (kTypes - 1));
+            r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            k = ti * kTypes + (st.type[p] & (kTypes - 1));
             in = r2 < t.rc2 && ((t.active >> k) & 1u);
-            if (in) {
-                if constexpr (kFp64) {
-                    const double qq = r2 / t.sig2[k];
-                    const double iq4 = 1.0 / (qq * qq * qq * qq);
-                    const double iq7 = iq4 / (qq * qq * qq);
-                    const double fr = (2 * iq7 - iq4) * t.coeff[k];
-                    c0 = ddx * fr;
-                    c1 = ddy * fr;
-                    c2 = ddz * fr;
-                    if constexpr (kEnergy) {
-                        const double iq6 = 1.0 / (qq * qq * qq);
-                        term_e = 0.5 * (t.e4[k] * iq6 * (iq6 - 1) - t.shift[k]);
-                        term_w = 0.5 * fr * r2;
-                    }
-                } else {
-                    const float s2 = __builtin_amdgcn_rcpf((float)r2 * t.inv_sig2f[k]);
-                    const float s4 = s2 * s2;
-                    const float s6 = s4 * s2;
-                    const float s8 = s4 * s4;
-                    frd = (double)(s8 * __builtin_fmaf(2.0f, s6, -1.0f) * t.coeff_f[k]);
-                    if constexpr (kEnergy) {
-                        term_e = 0.5 * ((double)(t.e4_f[k] * s6 * (s6 - 1.0f)) - t.shift[k]);
-                        term_w = 0.5 * frd * r2;
-                    }
-                }
-            }
         }
-        uint64_t m = __ballot(in);
-        while (m) {
-            const int l = __builtin_ctzll(m);
-            m &= m - 1;
-            if constexpr (kFp64) {
-                fx += lane_double(c0, l);
-                fy += lane_double(c1, l);
-                fz += lane_double(c2, l);
-            } else {
-                const double f = lane_double(frd, l);
-                fx = __builtin_fma(f, lane_double(ddx, l), fx);
-                fy = __builtin_fma(f, lane_double(ddy, l), fy);
-                fz = __builtin_fma(f, lane_double(ddz, l), fz);
-            }
-            if constexpr (kEnergy) {
-                e += lane_double(term_e, l);
-                w += lane_double(term_w, l);
-            }
+        const uint64_t m = __ballot(in);
+        const int32_t nin = (int32_t)__popcll(m);
+        if (pending + nin > 64) {
+            flush(pending);
+            pending = 0;
         }
+        if (in) {
+            // this lane's place among the batch's pairs: the set lanes below it
+            const int32_t at = pending + (int32_t)__builtin_amdgcn_mbcnt_hi(
+                                             (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            B[0][at] = ddx;
+            B[1][at] = ddy;
+            B[2][at] = ddz;
+            B[3][at] = r2;
+            K[at] = k;
+        }
+        pending += nin;
     }
-    if (lane == 0) {
-        st.f[0][s] = fx;
-        st.f[1][s] = fy;
-        st.f[2][s] = fz;
-        if constexpr (kEnergy) {
-            e_atom[s] = e;
-            w_atom[s] = w;
-        }
+    if (pending > 0) flush(pending);
+    if (lane < 3) st.f[lane][s] = acc;
+    if constexpr (kEnergy) {
+        if (lane == 3) e_atom[s] = acc;
+        if (lane == 4) w_atom[s] = acc;
     }
 }
 
