@@ -217,9 +217,12 @@ __global__ void __launch_bounds__(kBlock)
 k_correct_predict_track(DevState st, int32_t n, GearCoeffs gc, MassTable m, const double* __restrict__ r0x,
                         const double* __restrict__ r0y, const double* __restrict__ r0z,
                         unsigned long long* __restrict__ flags, int32_t spread, SlotGather g, double hard2,
-                        unsigned long long* __restrict__ clear) {
-    if (blockIdx.x == 0 && clear)   // the next step's flag words
-        for (int32_t t = threadIdx.x; t < 3 * spread; t += blockDim.x) clear[t] = 0;
+                        unsigned long long* __restrict__ clear, FlagPublish pub) {
+    if (blockIdx.x == 0) {
+        if (clear)   // the next step's flag words
+            for (int32_t t = threadIdx.x; t < 3 * spread; t += blockDim.x) clear[t] = 0;
+        hand_over(pub, 3 * spread);   // the previous step's words to the host (pub.host non-null)
+    }
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     double d2 = 0.0, s2 = 0.0;
     const double* r0[3] = {r0x, r0y, r0z};
@@ -309,26 +312,27 @@ hipError_t launch_correct_predict_track(hipStream_t s, const DevState& st, int32
     const SlotGather none{nullptr, nullptr, nullptr, nullptr, nullptr};
     if (final_step)
         k_correct_predict_track<true, false><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(
-            st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), none, 0.0, c);
+            st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), none, 0.0, c, FlagPublish{});
     else
         k_correct_predict_track<false, false><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(
-            st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), none, 0.0, c);
+            st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), none, 0.0, c, FlagPublish{});
     return hipGetLastError();
 }
 
 hipError_t launch_gather_correct_predict_track(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
                                                const MassTable& m, double* const rref[3], uint64_t* flags,
                                                bool final_step, const SlotGather& g, double hard2,
-                                               uint64_t* clear_next) {
+                                               uint64_t* clear_next, const FlagPublish& pub) {
     if (n <= 0) return hipSuccess;
+    if (pub.host && !pub.prev) return hipErrorInvalidValue;
     auto* f = reinterpret_cast<unsigned long long*>(flags);
     auto* c = reinterpret_cast<unsigned long long*>(clear_next);
     if (final_step)
         k_correct_predict_track<true, true><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(
-            st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), g, hard2, c);
+            st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), g, hard2, c, pub);
     else
         k_correct_predict_track<false, true><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(
-            st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), g, hard2, c);
+            st, n, gc, m, rref[0], rref[1], rref[2], f, flag_spread(), g, hard2, c, pub);
     return hipGetLastError();
 }
 

@@ -295,11 +295,12 @@ struct SlotGather {
     const double4* wd;  // the same in FP64 (used instead of w when non-null)
 };
 // the gather, the stale count of st's positions against rref (sqrt(hard2)),
-// then launch_correct_predict_track's corrector and predictor, in one kernel
+// then launch_correct_predict_track's corrector and predictor, in one kernel;
+// workgroup 0 hands pub.prev's flag words over to pub.host first (no gate)
 hipError_t launch_gather_correct_predict_track(hipStream_t s, const DevState& st, int32_t n, const GearCoeffs& gc,
                                                const MassTable& m, double* const rref[3], uint64_t* flags,
                                                bool final_step, const SlotGather& g, double hard2,
-                                               uint64_t* clear_next);
+                                               uint64_t* clear_next, const FlagPublish& pub);
 
 // lj_fused.hip: LDS-staged segment kernels of the periodic box
 // segments per cell column for workgroups of `block` threads (4 lanes an atom)

@@ -836,11 +836,10 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
         D = std::sqrt(d2);
         S = std::sqrt(s2);
     };
-    // Each LJ step kernel hands the previous step's flag words to their host
+    // Each step's last kernel (the LJ step kernel; the Tersoff and surrogate
+    // gather + Gear kernel) hands the previous step's flag words to their host
     // slot and writes a sequence number after them (FlagPublish); the last
-    // step's, the other steps' and the predictor's flags are copied and
-    // marked with an event (for the short Tersoff and surrogate step kernels
-    // the hand-over costs as much as the copy it saves).
+    // step's and the predictor's flags are copied and marked with an event.
     uint64_t expect[mdx_system::kRing] = {0, 0, 0};   // per slot: sequence number awaited (0: event)
     uint64_t gate_bits;
     std::memcpy(&gate_bits, &hard2, 8);
@@ -949,6 +948,8 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
         return MDX_OK;
     }
     int last_build = -1;   // step index whose positions were used for the last build
+    // step k hands over step k - 1's words (the predictor's, k = 0, were copied)
+    auto handover = [&](int32_t k) { return publish(k, k >= 1, false); };
     for (int32_t k = 0; k < nsteps; k++) {
         // Flags index f describes r(f+1) (f = -1: the predictor's r(0)).  Step k
         // uses r(k); the host reads index k-2, i.e. r(k-1), while launch k-1 runs.
@@ -983,16 +984,16 @@ int run_periodic(mdx_system* s, int32_t nsteps) {
             const SlotGather g{s->ts_list, s->ts_count, s->sg_rev, s->sg_w, nullptr};
             LAUNCH(s, "gather_gear_step", launch_gather_correct_predict_track(st, s->st, n, s->gc, s->mt, s->rref,
                                                                               slot_flags(k), fin, g, hard2,
-                                                                              slot_flags(k + 1)));
+                                                                              slot_flags(k + 1), handover(k)));
         } else {
             LAUNCH(s, "ts_atoms", launch_ts_atoms(st, s->st, c.box_length, s->tp, c.precision, s->ts_list, s->ts_count,
                                                   n, s->sg_w, s->ts_wd, nullptr));
             const SlotGather g{s->ts_list, s->ts_count, s->sg_rev, s->sg_w, s->ts_wd};
             LAUNCH(s, "gather_gear_step", launch_gather_correct_predict_track(st, s->st, n, s->gc, s->mt, s->rref,
                                                                               slot_flags(k), fin, g, hard2,
-                                                                              slot_flags(k + 1)));
+                                                                              slot_flags(k + 1), handover(k)));
         }
-        if ((rc = post(k))) return rc;
+        if (fin && (rc = post(k))) return rc;   // no launch follows to hand these over
         s->t++;
     }
     // stale check of the last steps
