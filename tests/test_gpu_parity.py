@@ -638,6 +638,27 @@ def test_tersoff_run_matches_oracle(silicon):
     assert rel_rms(st["vel"], vo) < 1e-9
 
 
+def test_tersoff_split_runs_match_oracle(silicon):
+    """The same 40 steps in calls of 1, 2, 3, 6 and 28 steps: each call starts
+    from the predictor's flags (copied to the host), each step's gather + Gear
+    kernel hands the previous step's flags over, and the last step's are
+    copied, so the rebuild decisions made across calls must keep the
+    trajectory on the oracle's."""
+    pos, vel, L = silicon
+    n, chunks = len(pos), [1, 2, 3, 6, 28]
+    ro, vo = O.tersoff_run(pos, vel, L, 1.0, sum(chunks), initial.SI_MASS, initial.ACCEL_EV_A_AMU)
+    with tersoff_system(n, L, PREC_FP64) as s:
+        s.set_state(np.c_[pos, np.zeros(n)], vel)
+        for k in chunks:
+            s.run(k)
+        st = s.get_state()
+        assert s.build_count() >= 1
+    d = st["pos"][:, :3] - ro
+    d -= L * np.round(d / L)
+    assert np.abs(d).max() < 1e-9
+    assert rel_rms(st["vel"], vo) < 1e-9
+
+
 def test_tersoff_pair_lists_are_the_exact_sets(silicon):
     """k_ts_list's compact lists as sets within R + D = 3.0 A and the list
     radius 3.3 A, at a build and after 200 steps with rebuilds."""
