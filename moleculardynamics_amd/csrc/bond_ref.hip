@@ -331,8 +331,10 @@ template <class T> __device__ inline T wave_sum(T v) {
     return v;
 }
 
-// stage the row of `self` into row[0..len); the host guarantees len <= kRowMax
-__device__ inline int32_t stage_row(const Ctx& c, int32_t self, RowEntry* row, int lane) {
+// stage the row of `self` into row[0..len), entries first, first + stride, ...
+// (lane: the caller's lane in its wavefront); the host guarantees len <= kRowMax
+__device__ inline int32_t stage_row(const Ctx& c, int32_t self, RowEntry* row, int lane, int32_t first,
+                                    int32_t stride) {
     const int32_t own = c.cell_of[self];
     const int32_t cz = own % c.g.g[2], cy = (own / c.g.g[2]) % c.g.g[1], cx = own / (c.g.g[2] * c.g.g[1]);
     int32_t rb = 0, rn = 0;
@@ -355,7 +357,7 @@ __device__ inline int32_t stage_row(const Ctx& c, int32_t self, RowEntry* row, i
         len += __builtin_amdgcn_readlane(rn, r);
     }
     len = min(len, kRowMax);
-    for (int32_t q = lane; q < len; q += 64) {
+    for (int32_t q = first; q < len; q += stride) {
         int32_t p = 0;
 #pragma unroll
         for (int r = 0; r < 9; r++)
@@ -367,29 +369,50 @@ __device__ inline int32_t stage_row(const Ctx& c, int32_t self, RowEntry* row, i
     return len;
 }
 
-// pass 1 of bond-order atom i by the wavefront (lane = its lane) whose
-// staged row is row[0..len)
+// One workgroup of kAtomWaves wavefronts per bond-order atom: the row is
+// staged once, and the atom's outer loop (over j in pass 1, over i in pass 2)
+// is dealt to the wavefronts round robin in row order, so each wavefront's
+// chain of dependent FP64 work is a quarter as long (with one wavefront per
+// atom a sweep's few hundred bond-order atoms left the GPU waiting on them).
+constexpr int kAtomWaves = kWaveBlock / 64;
+
+// pass 1 of bond-order atom i by its workgroup, whose staged row is
+// row[0..len).  Every wavefront walks the same candidates j in row order;
+// candidate o is evaluated by wavefront o % kAtomWaves, which parks its force
+// and energy terms in part[o] (and writes its xi table entry, numbered among
+// the j's with fcut > 0 as the reference's loop numbers them).  Thread 0 then
+// adds the terms in row order: the additions of the one-wavefront loop.
 template <class T>
-__device__ inline void bond_wave1_atom(const Ctx& c, const BondRefParams& p, int32_t i, int32_t cap, int32_t assign,
-                                       int32_t* __restrict__ xi_j, double* __restrict__ xi_v,
-                                       double* __restrict__ e_atom, const RowEntry* row, int32_t len, int lane) {
+__device__ inline void bond_wave1_block(const Ctx& c, const BondRefParams& p, int32_t i, int32_t cap, int32_t assign,
+                                        int32_t* __restrict__ xi_j, double* __restrict__ xi_v,
+                                        double* __restrict__ e_atom, const RowEntry* row, int32_t len,
+                                        double (*part)[4]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int32_t si = c.slot_of_id[i];
     const double px = c.st.r[0][si], py = c.st.r[1][si], pz = c.st.r[2][si];
     const int32_t ti = c.st.type[si];
-    double Fx = 0, Fy = 0, Fz = 0, e = 0;
-    int32_t nx = 0;
+    int32_t ord = 0, nx = 0;   // candidates so far; those with fcut > 0
     for (int32_t q0 = 0; q0 < len; q0 += 64) {
-        bool cand = false;
+        bool cand = false, live = false;
         if (q0 + lane < len) {
             const RowEntry a = row[q0 + lane];
             const double dx = px - a.x, dy = py - a.y, dz = pz - a.z;
             const double r2d = dx * dx + dy * dy + dz * dz;
             cand = a.id >= 0 && !(r2d >= p.rcut2 || r2d > p.pair_cut2) && a.type != kNe;
+            if (cand) {
+                T fc, dfc;
+                cutoff<T>(d_sqrt((T)r2d) / (T)10, fc, dfc);
+                live = fc > 0;
+            }
         }
         uint64_t mj = __ballot(cand);
+        const uint64_t mf = __ballot(live);
         while (mj) {
             const int l = __builtin_ctzll(mj);
             mj &= mj - 1;
+            const int32_t o = ord++;
+            if (o % kAtomWaves != w) continue;
+            const int32_t nxj = nx + __popcll(mf & ((1ull << l) - 1));   // this j's xi table entry
             const RowEntry bj = row[q0 + l];
             const int32_t j = bj.id;
             const double djx = px - bj.x, djy = py - bj.y, djz = pz - bj.z;
@@ -448,24 +471,34 @@ __device__ inline void bond_wave1_atom(const Ctx& c, const BondRefParams& p, int
                 T qi = (T)1 / (((T)1 + xi) * ((T)1 + xi)), qj = (T)1 / (((T)1 + xj) * ((T)1 + xj));
                 bo = (T)0.5 * ((T)1 / ((T)1 + xi) + (T)1 / ((T)1 + xj));
                 dbo = (dxi * qi + dxj * qj) * (T)0.5;
-                if (nx < cap && lane == 0) {
-                    int64_t q = (int64_t)i * cap + nx;
+                if (nxj < cap && lane == 0) {
+                    int64_t q = (int64_t)i * cap + nxj;
                     xi_j[q] = j;
                     xi_v[2 * q] = (double)xi;
                     xi_v[2 * q + 1] = (double)xj;
                 }
-                nx++;
             }
             vanderwaals<T>(rij, (T)p.r0v[kd], vdw, dvdw);
             T fij = dvr - bo * dva + dvdw;
-            T s = fij / rij;
-            Fx += (double)(ij.x * s - dbo.x * va);
-            Fy += (double)(ij.y * s - dbo.y * va);
-            Fz += (double)(ij.z * s - dbo.z * va);
-            e += 0.5 * (double)((vr - bo * va) + vdw);
+            T sf = fij / rij;
+            if (lane == 0) {
+                part[o][0] = (double)(ij.x * sf - dbo.x * va);
+                part[o][1] = (double)(ij.y * sf - dbo.y * va);
+                part[o][2] = (double)(ij.z * sf - dbo.z * va);
+                part[o][3] = 0.5 * (double)((vr - bo * va) + vdw);
+            }
         }
+        nx += __popcll(mf);
     }
-    if (lane != 0) return;
+    __syncthreads();   // every candidate's terms are parked
+    if (threadIdx.x != 0) return;
+    double Fx = 0, Fy = 0, Fz = 0, e = 0;
+    for (int32_t o = 0; o < ord; o++) {
+        Fx += part[o][0];
+        Fy += part[o][1];
+        Fz += part[o][2];
+        e += part[o][3];
+    }
     if (nx < cap) xi_j[(int64_t)i * cap + nx] = -1;
     if (assign) {
         c.st.f[0][si] = Fx; c.st.f[1][si] = Fy; c.st.f[2][si] = Fz;
@@ -480,26 +513,29 @@ __global__ void __launch_bounds__(kWaveBlock)
 k_bond_wave1(Ctx c, BondRefParams p, int32_t n_bond, int32_t cap, int32_t assign,
              int32_t* __restrict__ xi_j, double* __restrict__ xi_v, double* __restrict__ e_atom,
              const int32_t* __restrict__ gate) {
-    __shared__ RowEntry rows[kWaveBlock / 64][kRowMax];
+    __shared__ RowEntry row[kRowMax];
+    __shared__ double part[kRowMax][4];
     if (gate && *gate == 0) return;   // the same for every workgroup
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int32_t i = (int32_t)blockIdx.x * (kWaveBlock / 64) + w;
-    RowEntry* row = rows[w];
-    const int32_t len = i < n_bond ? stage_row(c, i, row, lane) : 0;
-    __syncthreads();
+    const int32_t i = (int32_t)blockIdx.x;   // one workgroup per bond-order atom
     if (i >= n_bond) return;
-    bond_wave1_atom<T>(c, p, i, cap, assign, xi_j, xi_v, e_atom, row, len, lane);
+    const int32_t len = stage_row(c, i, row, threadIdx.x & 63, (int32_t)threadIdx.x, kWaveBlock);
+    __syncthreads();
+    bond_wave1_block<T>(c, p, i, cap, assign, xi_j, xi_v, e_atom, row, len, part);
 }
 
-// pass 2 per wavefront: atom k; i runs over k's row in order, j over k's row
-// on the lanes; the xi table of i (at most 64 entries) is held one entry per lane
+// pass 2 of atom k by its workgroup: i runs over k's row in order, candidate
+// o on wavefront o % kAtomWaves, j over k's row on the lanes; the xi table of
+// i (at most 64 entries) is held one entry per lane.  Each wavefront sums its
+// lanes' terms, and thread 0 adds the wavefronts' sums in wavefront order.
 template <class T>
-__device__ inline void bond_wave2_atom(const Ctx& c, const BondRefParams& p, int32_t k, int32_t cap,
-                                       const int32_t* __restrict__ xi_j, const double* __restrict__ xi_v,
-                                       const RowEntry* row, int32_t len, int lane) {
+__device__ inline void bond_wave2_block(const Ctx& c, const BondRefParams& p, int32_t k, int32_t cap,
+                                        const int32_t* __restrict__ xi_j, const double* __restrict__ xi_v,
+                                        const RowEntry* row, int32_t len, double (*part)[4]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int32_t sk = c.slot_of_id[k];
     const double px = c.st.r[0][sk], py = c.st.r[1][sk], pz = c.st.r[2][sk];
     double Fx = 0, Fy = 0, Fz = 0;
+    int32_t ord = 0;
     for (int32_t q0 = 0; q0 < len; q0 += 64) {
         bool cand = false;
         if (q0 + lane < len) {
@@ -511,6 +547,7 @@ __device__ inline void bond_wave2_atom(const Ctx& c, const BondRefParams& p, int
         while (mi) {
             const int l = __builtin_ctzll(mi);
             mi &= mi - 1;
+            if (ord++ % kAtomWaves != w) continue;
             const RowEntry bi = row[q0 + l];
             const int32_t i = bi.id;
             const double kx = px - bi.x, ky = py - bi.y, kz = pz - bi.z;
@@ -586,25 +623,35 @@ __device__ inline void bond_wave2_atom(const Ctx& c, const BondRefParams& p, int
     Fy = wave_sum(Fy);
     Fz = wave_sum(Fz);
     if (lane == 0) {
-        c.st.f[0][sk] += Fx;
-        c.st.f[1][sk] += Fy;
-        c.st.f[2][sk] += Fz;
+        part[w][0] = Fx;
+        part[w][1] = Fy;
+        part[w][2] = Fz;
     }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double Gx = 0, Gy = 0, Gz = 0;
+    for (int q = 0; q < kAtomWaves; q++) {
+        Gx += part[q][0];
+        Gy += part[q][1];
+        Gz += part[q][2];
+    }
+    c.st.f[0][sk] += Gx;
+    c.st.f[1][sk] += Gy;
+    c.st.f[2][sk] += Gz;
 }
 
 template <class T>
 __global__ void __launch_bounds__(kWaveBlock)
 k_bond_wave2(Ctx c, BondRefParams p, int32_t n_bond, int32_t cap, const int32_t* __restrict__ xi_j,
              const double* __restrict__ xi_v, const int32_t* __restrict__ gate) {
-    __shared__ RowEntry rows[kWaveBlock / 64][kRowMax];
+    __shared__ RowEntry row[kRowMax];
+    __shared__ double part[kAtomWaves][4];
     if (gate && *gate == 0) return;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int32_t k = (int32_t)blockIdx.x * (kWaveBlock / 64) + w;
-    RowEntry* row = rows[w];
-    const int32_t len = k < n_bond ? stage_row(c, k, row, lane) : 0;
-    __syncthreads();
+    const int32_t k = (int32_t)blockIdx.x;   // one workgroup per bond-order atom
     if (k >= n_bond) return;
-    bond_wave2_atom<T>(c, p, k, cap, xi_j, xi_v, row, len, lane);
+    const int32_t len = stage_row(c, k, row, threadIdx.x & 63, (int32_t)threadIdx.x, kWaveBlock);
+    __syncthreads();
+    bond_wave2_block<T>(c, p, k, cap, xi_j, xi_v, row, len, part);
 }
 
 // below this many bond-order atoms (with rows and xi tables that fit) the
@@ -641,33 +688,31 @@ template <class T>
 __global__ void __launch_bounds__(kWaveBlock)
 k_sweep_bond_wave1(const MemberView* __restrict__ views, BondRefParams p, int32_t n_bond, int32_t assign,
                    int32_t gated) {
-    __shared__ RowEntry rows[kWaveBlock / 64][kRowMax];
+    __shared__ RowEntry row[kRowMax];
+    __shared__ double part[kRowMax][4];
     const MemberView& v = views[blockIdx.y];
     if (!v.bond_wave || (gated && *v.hit == 0)) return;   // the same for the whole workgroup
-    const Ctx c = member_ctx(v);
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int32_t i = (int32_t)blockIdx.x * (kWaveBlock / 64) + w;
-    RowEntry* row = rows[w];
-    const int32_t len = i < n_bond ? stage_row(c, i, row, lane) : 0;
-    __syncthreads();
+    const int32_t i = (int32_t)blockIdx.x;   // one workgroup per bond-order atom
     if (i >= n_bond) return;
-    bond_wave1_atom<T>(c, p, i, v.xi_cap, assign, v.xi_j, v.xi_v, nullptr, row, len, lane);
+    const Ctx c = member_ctx(v);
+    const int32_t len = stage_row(c, i, row, threadIdx.x & 63, (int32_t)threadIdx.x, kWaveBlock);
+    __syncthreads();
+    bond_wave1_block<T>(c, p, i, v.xi_cap, assign, v.xi_j, v.xi_v, nullptr, row, len, part);
 }
 
 template <class T>
 __global__ void __launch_bounds__(kWaveBlock)
 k_sweep_bond_wave2(const MemberView* __restrict__ views, BondRefParams p, int32_t n_bond, int32_t gated) {
-    __shared__ RowEntry rows[kWaveBlock / 64][kRowMax];
+    __shared__ RowEntry row[kRowMax];
+    __shared__ double part[kAtomWaves][4];
     const MemberView& v = views[blockIdx.y];
     if (!v.bond_wave || (gated && *v.hit == 0)) return;
-    const Ctx c = member_ctx(v);
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int32_t k = (int32_t)blockIdx.x * (kWaveBlock / 64) + w;
-    RowEntry* row = rows[w];
-    const int32_t len = k < n_bond ? stage_row(c, k, row, lane) : 0;
-    __syncthreads();
+    const int32_t k = (int32_t)blockIdx.x;   // one workgroup per bond-order atom
     if (k >= n_bond) return;
-    bond_wave2_atom<T>(c, p, k, v.xi_cap, v.xi_j, v.xi_v, row, len, lane);
+    const Ctx c = member_ctx(v);
+    const int32_t len = stage_row(c, k, row, threadIdx.x & 63, (int32_t)threadIdx.x, kWaveBlock);
+    __syncthreads();
+    bond_wave2_block<T>(c, p, k, v.xi_cap, v.xi_j, v.xi_v, row, len, part);
 }
 
 // common/DebugCalculations.cpp:129-270: per-atom share of the brute-force
@@ -748,7 +793,7 @@ hipError_t launch_sweep_bond(hipStream_t s, const MemberView* views, int32_t mem
     const int32_t gt = gated ? 1 : 0;
     const bool fp64 = precision == MDX_PREC_FP64;
     if (any_wave) {
-        dim3 wgrid((n_bond + kWaveBlock / 64 - 1) / (kWaveBlock / 64), members);
+        dim3 wgrid(n_bond, members);   // one workgroup per bond-order atom
         if (fp64) {
             k_sweep_bond_wave1<double><<<wgrid, kWaveBlock, 0, s>>>(views, p, n_bond, assign, gt);
             k_sweep_bond_wave2<double><<<wgrid, kWaveBlock, 0, s>>>(views, p, n_bond, gt);
@@ -780,7 +825,7 @@ hipError_t launch_bond_ref(hipStream_t s, const DevState& st, const int32_t* slo
     if (n_bond <= 0) return hipSuccess;
     Ctx c{st, slot_of_id, mb_cell_of, mb_ids, g};
     if (bond_ref_wave(n_bond, row_bound, cap)) {
-        dim3 wgrid((n_bond + kWaveBlock / 64 - 1) / (kWaveBlock / 64));
+        dim3 wgrid(n_bond);   // one workgroup per bond-order atom
         if (precision == MDX_PREC_FP64) {
             k_bond_wave1<double><<<wgrid, kWaveBlock, 0, s>>>(c, p, n_bond, cap, assign, xi_j, xi_v,
                                                              energy_per_atom, gate);
