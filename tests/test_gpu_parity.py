@@ -163,6 +163,33 @@ def test_bond_forces_walled(mixture, precision):
     assert rel_rms(f, fo) < tol
 
 
+@pytest.mark.parametrize("precision", [PREC_FP64, PREC_MIXED])
+def test_lj_forces_walled_dense_rows(precision):
+    """Ne atoms 2.0 A apart: about 110 pairs within rc per atom (up to 171),
+    so the wavefront-per-atom kernel fills its 64-entry buffer of in-cutoff
+    pairs and evaluates it mid-row, for some atoms twice.  Forces against the oracle
+    (FP64 bit for bit, mixed 1e-5) and, in FP64, the virial (the energy
+    variant of the kernel) against sum_i r_i . F_i."""
+    pos4, vel = gas_mixture(0, 1000, seed=3, spacing=2.0)
+    with walled(pos4, 0, precision, use_bond=False) as s:
+        s.set_state(pos4, vel)
+        _, box = s.ready_list()
+        s.build_list(LIST_LJ, box)
+        s.lj_forces()
+        f = s.get_state()["force"]
+        w_lj = s.potential_virial()[2] if precision == PREC_FP64 else None
+    ref, ls = O.build_neighbor_list(pos4, 7.6, 6.8, box)
+    fo = O.lj_forces(pos4, ref, ls.nlargestsize, float(ls.rcutsq))
+    d = pos4[:, None, :3] - pos4[None, :, :3]
+    pairs = ((d ** 2).sum(-1) < 6.8 ** 2).sum(1) - 1
+    assert pairs.max() > 128, pairs.max()   # at least two mid-row evaluations for some atoms
+    if precision == PREC_FP64:
+        np.testing.assert_array_equal(f, fo)
+        assert w_lj == pytest.approx(float(np.sum(pos4[:, :3] * fo)), rel=1e-9)
+    else:
+        assert rel_rms(f, fo) < 1e-5
+
+
 def test_lj_forces_walled_thread_per_atom():
     """33,000 Ne atoms: above the size where the walled LJ force runs one
     wavefront per atom, so this covers the thread-per-atom kernel (same bar:
