@@ -216,3 +216,33 @@ def test_replica_batch_512_conserves():
         builds = b.build_counts()
     assert np.all(builds >= 1)
     assert np.all(np.abs(e1 - e0) < 2e-3 * np.abs(e0) + 1e-3 * ns)
+
+
+def test_vxcm_sweep_benchmark_members_follow_oracle():
+    """The benchmark's sweep itself: 8 members of the 4,060-atom walled N/O/Ne
+    gas (60 bond-order atoms, 300 K, FP64) with centre-of-mass x velocities
+    -0.003 .. 0.004, 40 batched steps, each member against the oracle's
+    reference loop (cpu/cpumd/Host.cpp:141-195) from the same start, to 1e-9
+    (the bond-order bar: device exp and tanh differ from glibc in the last
+    bits)."""
+    from moleculardynamics_amd import BOND_REFERENCE, BOX_WALLED, LJ_REFERENCE, Sweep
+    pos4, vel = initial.walled_mixture(30, 4000, 300.0, seed=7)
+    n, nb, steps = len(pos4), 60, 40
+    wall = n ** (1 / 3) * 52.8
+    vxs = [float(np.float32(-0.003 + 0.001 * k)) for k in range(8)]
+    cfg = dict(n_atoms=n, n_bond=nb, box=BOX_WALLED, precision=PREC_FP64, use_lj=1, use_bond=1,
+               lj_table=LJ_REFERENCE, bond_form=BOND_REFERENCE, dt=1.0, lj_rs=7.6, lj_rcut=6.8, mb_rs=3.6,
+               mb_rcut=3.0, energy_loss=1.0, wall=wall)
+    print(f"benchmark sweep vs oracle: {len(vxs)} members of {n} atoms, {steps} steps", flush=True)
+    with Sweep(len(vxs), **cfg) as w:
+        for m, vx in enumerate(vxs):
+            v = vel.copy()
+            v[:, 0] += vx
+            w.member(m).set_state(pos4, v)
+        w.run(steps)
+        got = [w.member(m).get_state() for m in range(len(vxs))]
+    for m, vx in enumerate(vxs):
+        ro, vo = O.ref_run(pos4, vel, n - nb, nb, steps, 10 ** 9, 1.0, 7.6, 6.8, 3.6, 3.0, vxcm=vx,
+                           use_lj=True, use_mb=True, wall=wall)[:2]
+        assert np.abs(got[m]["pos"][:, :3] - ro[:, :3]).max() < 1e-9, m
+        assert rel_rms(got[m]["vel"], vo) < 1e-9, m
