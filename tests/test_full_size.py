@@ -218,6 +218,27 @@ def test_replica_batch_512_conserves():
     assert np.all(np.abs(e1 - e0) < 2e-3 * np.abs(e0) + 1e-3 * ns)
 
 
+def test_replica_batch_512_forces_match_oracle():
+    """Every 16th of the 512 benchmark replicas (32 replicas, on the staged and
+    the all-pairs paths) after 2 mixed-precision steps of the whole batch:
+    each one's force within 1e-5 relative RMS of the FP64 oracle's at the same
+    step and its positions within 1e-9."""
+    from test_batch import RC, RS, oracle_run
+    ns, Ls, pos, vel = initial.replica_batch(512, seed=2024)
+    offs = np.concatenate([[0], np.cumsum(ns)])
+    with Batch(ns, Ls, precision=PREC_MIXED, dt=0.005, lj_rs=RS, lj_rcut=RC) as b:
+        b.set_state(pos, vel)
+        b.run(2)
+        st = b.get_state()
+    for r in range(0, len(ns), 16):
+        sl = slice(offs[r], offs[r + 1])
+        ro, _, fo = oracle_run(pos[sl], vel[sl], Ls[r], 2, with_force=True)
+        assert rel_rms(st["force"][sl], fo) < 1e-5, r
+        d = st["pos"][sl] - ro
+        d -= Ls[r] * np.round(d / Ls[r])
+        assert np.abs(d).max() < 1e-9, r
+
+
 def test_vxcm_sweep_benchmark_members_follow_oracle():
     """The benchmark's sweep itself: 8 members of the 4,060-atom walled N/O/Ne
     gas (60 bond-order atoms, 300 K, FP64) with centre-of-mass x velocities
