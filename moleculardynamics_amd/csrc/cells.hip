@@ -30,7 +30,9 @@ __device__ inline double wrap_periodic(double x, double L) {
     return y >= L ? y - L : y;
 }
 
-__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v, int32_t* __restrict__ zero,
+                           int32_t nzero) {
+    if (blockIdx.x == 0 && (int32_t)threadIdx.x < nzero) zero[threadIdx.x] = 0;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = v;
@@ -546,11 +548,12 @@ hipError_t launch_scatter3(hipStream_t s, double* const src[3], const int32_t* i
     return hipGetLastError();
 }
 
-hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v) {
-    if (n <= 0) return hipSuccess;
-    int g = grid_for(n);
+hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v, int32_t* zero, int32_t nzero) {
+    if (nzero < 0 || nzero > kBlock || (nzero > 0 && !zero)) return hipErrorInvalidValue;
+    if (n <= 0 && nzero == 0) return hipSuccess;
+    int g = n > 0 ? grid_for(n) : 1;
     if (g > 65536) g = 65536;
-    k_fill_i32<<<g, kBlock, 0, s>>>(p, n, v);
+    k_fill_i32<<<g, kBlock, 0, s>>>(p, n, v, zero, nzero);
     return hipGetLastError();
 }
 

@@ -184,7 +184,10 @@ struct MemberView {
 
 // ---- host-side launch wrappers (one translation unit each) ----
 // cells.hip
-hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v);
+// p[0..n) = v; block 0 also zeroes zero[0..nzero) (nzero <= 256), which saves
+// a memset's launch for a few flag words
+hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v, int32_t* zero = nullptr,
+                           int32_t nzero = 0);
 hipError_t launch_bin(hipStream_t s, const DevState& st, const int32_t* item_slot, int32_t items,
                       const GridView& g, int32_t* cell_out, int32_t* counts, int32_t* err);
 hipError_t launch_exclusive_scan(hipStream_t s, const int32_t* in, int32_t* out, int64_t n,

@@ -64,6 +64,7 @@ struct mdx_system {
     double* dscal = nullptr;         // [0..7] kinetics, [8..9] sums
     int32_t* dint = nullptr;         // [0] wall hit, [1] bin error, [2] max count, [3] max row,
                                      // [4..6] list-build error words, [8..9] pair-list export
+    bool words_zeroed = false;       // dint[1..6] zeroed by this rebuild's cell clear, not yet used
     uint64_t* dbits = nullptr;       // [0] max disp^2, [1] max v^2, [2] max |coord|
     double* e_atom = nullptr;        // 4n scratch: energies (LJ, bond order), then virials (LJ, bond order)
     void* pinned = nullptr;          // 256 bytes
@@ -424,8 +425,10 @@ int build_primary(mdx_system* s, Grid& g, float rs, double box_size_list, bool s
         s->builds++;
         return sizes ? small_sizes(s, g, n, sizes_po) : MDX_OK;
     }
-    HIPCHK(s, hipMemsetAsync(s->dint + 1, 0, sizeof(int32_t), st));
-    LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
+    // the first clear also zeroes the binning's out-of-grid word (dint[1]) and
+    // the list builds' error words (dint[2..6]), in place of their memsets
+    LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0, s->dint + 1, 6));
+    s->words_zeroed = true;
     LAUNCH(s, "cells_bin", launch_bin(st, s->st, nullptr, n, g.view, s->cell_of, g.count, s->dint + 1));
     LAUNCH(s, "cells_scan", launch_exclusive_scan(st, g.count, g.start, g.ncell, s->scan_sums, s->scan_cap));
     LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
@@ -496,8 +499,8 @@ int build_bond_walled(mdx_system* s, double box_size_list, int sizes_po = -1) {
         if ((rc = small_sizes(s, g, nb, -1))) return rc;
         return bond_walled_finish(s);
     }
-    HIPCHK(s, hipMemsetAsync(s->dint + 1, 0, sizeof(int32_t), st));
-    LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
+    // the first clear also zeroes the binning's out-of-grid word (dint[1])
+    LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0, s->dint + 1, 1));
     LAUNCH(s, "cells_bin", launch_bin(st, s->st, s->slot_of_id, nb, g.view, s->mb_cell_of, g.count, s->dint + 1));
     LAUNCH(s, "cells_scan", launch_exclusive_scan(st, g.count, g.start, g.ncell, s->scan_sums, s->scan_cap));
     LAUNCH(s, "cells_clear", launch_fill_i32(st, g.count, g.ncell, 0));
@@ -570,7 +573,9 @@ int ts_build_list(mdx_system* s) {
         // the list, then the slot of each atom in its partners' lists (for the
         // force gather); one read-back covers both: dint[4] the longest list
         // when one overflowed, dint[5] a partner list without the atom
-        HIPCHK(s, hipMemsetAsync(s->dint + 4, 0, 2 * sizeof(int32_t), s->stream));
+        // (on the first attempt the cell clear of this rebuild zeroed them)
+        if (!s->words_zeroed) HIPCHK(s, hipMemsetAsync(s->dint + 4, 0, 2 * sizeof(int32_t), s->stream));
+        s->words_zeroed = false;
         LAUNCH(s, "ts_list", launch_ts_list(s->stream, s->st, s->cell_of, s->lj.view, (double)s->cfg.mb_rs, n,
                                             s->ts_cap, s->ts_list, s->ts_count, s->dint + 4));
         LAUNCH(s, "sg_rev", launch_sg_rev(s->stream, s->ts_list, s->ts_count, n, s->sg_rev, s->dint + 5));
@@ -746,7 +751,9 @@ int rebuild_periodic(mdx_system* s) {
             const int32_t kcells = (v.g[2] + s->nseg - 1) / s->nseg;
             s->build_lds = std::min(lds_capacity(), round64(9.0 * (kcells + 2) * per_cell * 1.3 + 64));
         }
-        HIPCHK(s, hipMemsetAsync(s->dint + 4, 0, 3 * sizeof(int32_t), s->stream));
+        // (on the first attempt the cell clear of this rebuild zeroed them)
+        if (!s->words_zeroed) HIPCHK(s, hipMemsetAsync(s->dint + 4, 0, 3 * sizeof(int32_t), s->stream));
+        s->words_zeroed = false;
         LAUNCH(s, "nlist_build", launch_nlist_lds(s->stream, s->st, v, s->nseg, rl2, n, s->nl_cap, s->nlist,
                                                   s->ncount, s->dint + 4, s->build_lds, s->box_block, s->seg_hdr));
         int32_t e[3];
